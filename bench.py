@@ -1,0 +1,182 @@
+"""Newton-GMRES throughput of the implicit-DG step on one MI355X.
+
+Workload: 2D compressible Navier-Stokes, DG p = 3 (10 basis functions, 4 equations,
+40 x 40 Jacobian blocks), Re = 5000, M = 0.5, Pr = 0.72 on a seeded perturbed
+structured triangulation of [-5,5]^2 around a unit cylinder, 200,000 triangles
+(8.0 M DOFs).  One step = one BDF1 pseudo-time step with one Newton iteration:
+residual + exact element Jacobian assembly, block-ILU0 factorisation (RCM order),
+left-preconditioned GMRES(30) to the reference's default tolerance 1e-2, update
+with the positivity limiter.  Synthetic data: the freestream state of the model.
+
+The same steps run twice from the same state: with FP64 preconditioner factors
+and sweeps, and mixed (FP32 factors/sweeps under the FP64 residual, Jacobian and
+GMRES).  `value` is the mixed-precision DOF-updates/s; the FP64 rate, their ratio
+and both normalised residual histories are reported beside it.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_case(n_theta, n_r):
+    from flowpro_v2_amd import cases
+    return cases.cylinder_mesh(n_theta, n_r)
+
+
+def run_solver(case, props, bits, warmup, steps):
+    from flowpro_v2_amd import solver
+    t0 = time.perf_counter()
+    dev = solver.DeviceCase(case, props, precond_bits=bits)
+    dev.set_state(dev.initial_state())
+    log(f"fp{bits}: context ready in {time.perf_counter() - t0:.1f} s, dofs {dev.dofs}, "
+        f"ILU levels {dev.ilu_levels}, block nonzeros {dev.nnzb}")
+    s = solver.GpuImplicitSolver(dev, props)
+    s.solve(warmup)
+    log(f"fp{bits}: warmup {warmup} steps done")
+    t0 = time.perf_counter()
+    recs = s.solve(steps)[warmup:]
+    elapsed = time.perf_counter() - t0
+    updates = sum(r.newton for r in recs)
+    iters = [it for r in recs for it in r.gmres_iters]
+    log(f"fp{bits}: {steps} steps in {elapsed:.3f} s, GMRES iterations {iters}, residuals "
+        + " ".join(f"{r.residuum:.4e}" for r in recs))
+    return dev, dict(elapsed=elapsed, updates=updates, iters=iters, residuals=[r.residuum for r in recs],
+                     failures=s.linear_failures)
+
+
+def kernel_table(dev, reps=5):
+    names = {0: "residual", 1: "jacobian", 2: "spmv", 3: "ilu_factor", 4: "ilu_apply", 5: "jtv"}
+    return {names[k]: dev.time_kernel(k, reps) for k in (0, 1, 2, 3, 4)}
+
+
+def roofline_jacobian(dev, ms):
+    """Algorithmic bytes of one Jacobian-assembly launch: per element row 8 B^2 (1 + n_nbr)
+    block bytes written + residual-input bytes 8 B (2 + 1) + 112 B geometry + 12 B indices."""
+    B = dev.B
+    blocks_bytes = 8.0 * B * B * dev.nnzb
+    elem_bytes = dev.n_elem * (8.0 * B * 3 + 112 + 12)
+    traffic = blocks_bytes + elem_bytes
+    achieved = traffic / (ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_jacobian",
+                algorithmic_bytes=traffic, avg_launch_ms=ms)
+
+
+def roofline_spmv(dev, ms):
+    B = dev.B
+    traffic = 8.0 * B * B * dev.nnzb + 8.0 * B * 2 * dev.n_elem + 4.0 * dev.nnzb
+    achieved = traffic / (ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_bsr_spmv",
+                algorithmic_bytes=traffic, avg_launch_ms=ms)
+
+
+def cpu_baseline(threads, n_theta=64, n_r=24):
+    """The oracle (C++ restatement of the reference's Java path: FD Jacobian, CRS, sequential
+    block ILU0, MGS GMRES, OpenMP element loops) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from flowpro_v2_amd import cases
+    case = cases.cylinder_mesh(n_theta, n_r)
+    props = cases.ns_props(order=4, CFL=10.0)
+    o = O.Oracle(case, props, threads=threads)
+    o.run(1)     # first step (setup of CRS / ILU structure happens here as in LinearSolver.factory)
+    t0 = time.perf_counter()
+    nsteps = 2
+    o.run(nsteps)
+    el = time.perf_counter() - t0
+    rate = o.dofs * nsteps / el
+    return dict(value=round(rate, 1), unit="DOF-updates/s", cores=threads, kind="port",
+                sample=f"{nsteps} Newton steps (FD Jacobian h=1e-8 as the reference, blockilu0 GMRES) of the same "
+                       f"NS p=3 Re=5000 M=0.5 case on a {case.n_elem}-triangle cylinder mesh, {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-theta", type=int, default=400)
+    ap.add_argument("--n-r", type=int, default=250)
+    ap.add_argument("--cfl", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is "
+              "element partitioning with a halo layer, not built)", file=sys.stderr)
+        sys.exit(2)
+    from flowpro_v2_amd import cases
+    case = make_case(args.n_theta, args.n_r)
+    props = cases.ns_props(order=4, CFL=args.cfl)
+    log(f"mesh {case.name}: {case.n_elem} triangles")
+    out = {}
+    devs = {}
+    for bits in ([64, 32] if args.only is None else ([64] if args.only == "fp64" else [32])):
+        dev, res = run_solver(case, props, bits, args.warmup, args.steps)
+        res["rate"] = dev.dofs * res["updates"] / res["elapsed"]
+        out[bits] = res
+        if bits == 32 or args.only is not None:
+            kt = kernel_table(dev)
+            log("kernel ms per launch: " + json.dumps({k: round(v, 4) for k, v in kt.items()}))
+            devs["kt"] = kt
+            devs["dev"] = dev
+        else:
+            dev.close()
+    dev = devs["dev"]
+    kt = devs["kt"]
+    main_bits = 32 if 32 in out else 64
+    r = out[main_bits]
+    jac_ms = kt["jacobian"]
+    roof = roofline_jacobian(dev, jac_ms)
+    result = {
+        "metric": "Newton-GMRES DOF-updates/s, mixed vs FP64 GPU at equal residual; % HBM peak",
+        "value": round(r["rate"], 1),
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * r["elapsed"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 residual/Jacobian/GMRES, f32 preconditioner" if main_bits == 32 else "f64",
+        "data": "synthetic (freestream state, seeded mesh)",
+        "config": {"workload": f"NS2D DG p=3 Re=5000 M=0.5 cylinder, {case.n_elem} triangles, "
+                               f"{dev.dofs} DOFs, BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) GMRES(30) tol 1e-2",
+                   "elements": case.n_elem, "dofs": dev.dofs, "order": 4, "precond_bits": main_bits},
+        "fp64_value": round(out[64]["rate"], 1) if 64 in out else None,
+        "mixed_over_fp64": round(out[32]["rate"] / out[64]["rate"], 3) if (32 in out and 64 in out) else None,
+        "residuals": {f"fp{b}": [float(f"{v:.6e}") for v in out[b]["residuals"]] for b in out},
+        "gmres_iterations": {f"fp{b}": out[b]["iters"] for b in out},
+        "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+        "roofline": roof,
+    }
+    dev.close()
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads ...")
+        result["cpu_baseline"] = cpu_baseline(threads)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,206 @@
+"""Synthetic cases in FlowPro's mesh format and the parameters of each workload.
+
+A case is what ``FlowProMain.solverFactory`` reads from a simulation directory
+(FlowPro/src/main/java/flowpro/core/FlowProMain.java:486-730): vertices
+``PXY[nPts][2]``, triangles ``TP[nElem][3]`` (counter-clockwise), boundary faces
+``boundaryType.txt`` rows ``(type, v0, v1)`` and the parameters.txt keys of the
+model and solver.  The generators are seeded and deterministic.
+
+Boundary codes follow Aerodynamics.BoundaryType
+(ModuleProjects/Equations/Aerodynamics/src/main/java/flowpro/user/equation/Aerodynamics.java:11-19):
+-1 WALL, -2 INLET, -3 OUTLET, -4 INVISCID_WALL.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+WALL, INLET, OUTLET, INVISCID_WALL = -1, -2, -3, -4
+
+
+@dataclass
+class Case:
+    name: str
+    PXY: np.ndarray            # (nPts, 2) float64
+    TP: np.ndarray             # (nElem, 3) int32, counter-clockwise
+    bnd: np.ndarray            # (nBnd, 3) int32: type, v0, v1
+    wall_distance: np.ndarray | None = None
+    props: dict = field(default_factory=dict)
+
+    @property
+    def n_elem(self) -> int:
+        return int(self.TP.shape[0])
+
+
+class JavaRandom:
+    """java.util.Random restated from its published LCG specification.
+
+    seed: (s ^ 0x5DEECE66D) & (2**48 - 1); next(b): s = s*0x5DEECE66D + 0xB mod 2**48,
+    returns s >>> (48 - b); nextDouble = ((next(26) << 27) + next(27)) * 2**-53.
+    The reference's tests draw their states with it (FlowPro/src/test/java/flowpro/core/
+    JacobianTest.java:36-41 with seed 1, LinearSolverTest.java:32-38 with seed 2).
+    """
+
+    MULT = 0x5DEECE66D
+    MASK = (1 << 48) - 1
+
+    def __init__(self, seed: int):
+        self.s = (seed ^ self.MULT) & self.MASK
+
+    def next(self, bits: int) -> int:
+        self.s = (self.s * self.MULT + 0xB) & self.MASK
+        return self.s >> (48 - bits)
+
+    def next_double(self) -> float:
+        return ((self.next(26) << 27) + self.next(27)) * (1.0 / (1 << 53))
+
+
+def _orient_ccw(PXY: np.ndarray, TP: np.ndarray) -> np.ndarray:
+    a, b, c = PXY[TP[:, 0]], PXY[TP[:, 1]], PXY[TP[:, 2]]
+    area2 = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0])
+    TP = TP.copy()
+    neg = area2 < 0
+    TP[neg, 1], TP[neg, 2] = TP[neg, 2].copy(), TP[neg, 1].copy()
+    return TP
+
+
+def _split_quads(vid: np.ndarray) -> np.ndarray:
+    """Structured (ni+1) x (nj+1) vertex grid -> 2*ni*nj triangles with the
+    alternating diagonals of the reference's test meshes
+    (FlowPro/src/test/java/flowpro/core/TestSimulation.java:41-53)."""
+    ni, nj = vid.shape[0] - 1, vid.shape[1] - 1
+    I, J = np.meshgrid(np.arange(ni), np.arange(nj), indexing="ij")
+    I, J = I.T.ravel(), J.T.ravel()          # j outer, i inner as in the reference loop
+    a, b = vid[I, J], vid[I + 1, J]
+    c, d = vid[I + 1, J + 1], vid[I, J + 1]
+    even = (I + J) % 2 == 0
+    t1 = np.where(even[:, None], np.stack([a, b, c], 1), np.stack([a, b, d], 1))
+    t2 = np.where(even[:, None], np.stack([a, c, d], 1), np.stack([b, c, d], 1))
+    return np.stack([t1, t2], 1).reshape(-1, 3).astype(np.int32)
+
+
+def square_mesh(n: int, size: float = 1.0, boundary_type: int = INLET) -> Case:
+    """TestSimulation.writeSquareMesh (TestSimulation.java:28-80): [0,size]^2, n x n
+    cells split with alternating diagonals, all boundary faces of one type."""
+    h = size / n
+    j, i = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    PXY = np.stack([i.ravel() * h, j.ravel() * h], 1).astype(np.float64)
+
+    def v(i, j):
+        return j * (n + 1) + i
+
+    vid = np.array([[v(ii, jj) for jj in range(n + 1)] for ii in range(n + 1)])
+    TP = _split_quads(vid)
+    rows = []
+    for k in range(n):
+        rows += [(boundary_type, v(k, 0), v(k + 1, 0)), (boundary_type, v(n, k), v(n, k + 1)),
+                 (boundary_type, v(k + 1, n), v(k, n)), (boundary_type, v(0, k + 1), v(0, k))]
+    wd = np.array([min(min(ii, jj), min(n - ii, n - jj)) * h for jj in range(n + 1) for ii in range(n + 1)])
+    return Case(f"square{n}", PXY, TP, np.array(rows, dtype=np.int32), wd)
+
+
+def channel_bump(nx: int = 64, ny: int = 16) -> Case:
+    """Channel [0,3]x[0,1] with a 10%-height circular-arc bump on x in [1,2]
+    (y_b = sqrt(1.69 - (x-1.5)^2) - 1.2, the GAMM geometry), (nx+1)x(ny+1) vertices,
+    y = y_b(x) + eta (1 - y_b(x)); walls -1 bottom/top, inlet -2 left, outlet -3 right."""
+    x = np.linspace(0.0, 3.0, nx + 1)
+    yb = np.where((x >= 1.0) & (x <= 2.0), np.sqrt(np.maximum(1.69 - (x - 1.5) ** 2, 0.0)) - 1.2, 0.0)
+    yb = np.maximum(yb, 0.0)
+    eta = np.linspace(0.0, 1.0, ny + 1)
+    X = np.repeat(x[:, None], ny + 1, 1)
+    Y = yb[:, None] + eta[None, :] * (1.0 - yb[:, None])
+    vid = np.arange((nx + 1) * (ny + 1)).reshape(nx + 1, ny + 1)
+    PXY = np.stack([X.ravel(), Y.ravel()], 1)
+    TP = _orient_ccw(PXY, _split_quads(vid))
+    rows = []
+    for i in range(nx):
+        rows.append((WALL, vid[i, 0], vid[i + 1, 0]))
+        rows.append((WALL, vid[i + 1, ny], vid[i, ny]))
+    for j in range(ny):
+        rows.append((INLET, vid[0, j + 1], vid[0, j]))
+        rows.append((OUTLET, vid[nx, j], vid[nx, j + 1]))
+    return Case(f"channel_bump_{nx}x{ny}", PXY, TP, np.array(rows, dtype=np.int32))
+
+
+def cylinder_mesh(n_theta: int = 400, n_r: int = 250, growth: float = 1.01, jitter: float = 0.15,
+                  seed: int = 2) -> Case:
+    """[-5,5]^2 around a unit-diameter cylinder: an O-grid from the circle r = 0.5 to
+    the square boundary, geometric radial spacing, each cell split into two triangles
+    (alternating diagonals), interior vertices jittered by a seeded fraction of the
+    local spacing.  Cylinder -1 (no-slip), left -2, right -3, top/bottom -4."""
+    assert n_theta % 8 == 0, "the square's corners must be vertices"
+    th = 2 * np.pi * np.arange(n_theta) / n_theta
+    circ = 0.5 * np.stack([np.cos(th), np.sin(th)], 1)
+    sq = 5.0 * np.stack([np.cos(th), np.sin(th)], 1) / np.maximum(np.abs(np.cos(th)), np.abs(np.sin(th)))[:, None]
+    s = (growth ** np.arange(n_r + 1) - 1.0) / (growth ** n_r - 1.0)
+    P = circ[:, None, :] + s[None, :, None] * (sq - circ)[:, None, :]
+    rng = np.random.default_rng(seed)
+    if jitter > 0:
+        ds = np.diff(s)
+        h_r = np.minimum(ds[:-1], ds[1:])                     # interior radial spacing
+        r_mid = np.linalg.norm(P[:, 1:-1, :], axis=2)
+        h_t = 2 * np.pi * r_mid / n_theta
+        h = np.minimum(h_r[None, :] * np.linalg.norm(sq - circ, axis=1)[:, None], h_t)
+        P[:, 1:-1, :] += jitter * h[:, :, None] * rng.uniform(-1, 1, size=P[:, 1:-1, :].shape)
+    PXY = P.reshape(-1, 2)
+    vid = np.arange(n_theta * (n_r + 1)).reshape(n_theta, n_r + 1)
+    vid = np.concatenate([vid, vid[:1]], 0)                # periodic in theta
+    TP = _orient_ccw(PXY, _split_quads(vid))
+    rows = []
+    for i in range(n_theta):
+        a, b = vid[i, 0], vid[i + 1, 0]
+        rows.append((WALL, a, b))
+        a, b = vid[i, n_r], vid[i + 1, n_r]
+        mx, my = 0.5 * (PXY[a] + PXY[b])
+        if abs(mx + 5.0) < 1e-9:
+            t = INLET
+        elif abs(mx - 5.0) < 1e-9:
+            t = OUTLET
+        else:
+            t = INVISCID_WALL
+        rows.append((t, a, b))
+    return Case(f"cylinder_{n_theta}x{n_r}", PXY, TP, np.array(rows, dtype=np.int32))
+
+
+def min_signed_area(case: Case) -> float:
+    P, T = case.PXY, case.TP
+    a, b, c = P[T[:, 0]], P[T[:, 1]], P[T[:, 2]]
+    return float((((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0])) / 2).min())
+
+
+def pout_for_mach(mach: float, kappa: float = 1.4) -> float:
+    """Static/stagnation pressure ratio of the isentropic outlet for a Mach number."""
+    return (1 + 0.5 * (kappa - 1) * mach * mach) ** (-kappa / (kappa - 1))
+
+
+def euler_props(mach: float = 0.67, order: int = 2, **kw) -> dict:
+    p = dict(model="NavierStokes", isFlowViscous=False, kappa=1.4, cv=1.0, pIn0=1.0, rhoIn0=1.0,
+             pOut=pout_for_mach(mach), order=order, timeOrder=1, preconditioner="blockilu0",
+             elementOrdering="rcm", newtonIters=1, CFL=1.0, penalty=0.0, beta0=2.0,
+             iterativeSolverTol=1e-2, newtonIterTol=1e-4)
+    p.update(kw)
+    return p
+
+
+def ns_props(mach: float = 0.5, reynolds: float = 5000.0, order: int = 4, **kw) -> dict:
+    p = euler_props(mach=mach, order=order, isFlowViscous=True, reynolds=reynolds, prandtl=0.72, penalty=1.0)
+    p.update(kw)
+    return p
+
+
+def perturbed_state(w0: np.ndarray, n_elem: int, n_basis: int, rel: float = 0.01, seed: int = 1) -> np.ndarray:
+    """Element-major Java layout W[e][m*Nb + j]: constant state w0 times (1 + rel*N(0,1))
+    per coefficient (numpy default_rng(seed))."""
+    n_eqs = len(w0)
+    rng = np.random.default_rng(seed)
+    base = np.repeat(np.asarray(w0, dtype=np.float64), n_basis)
+    W = np.tile(base, (n_elem, 1))
+    return W * (1.0 + rel * rng.standard_normal(W.shape))
+
+
+def java_random_state(n_elem: int, b: int, seed: int) -> np.ndarray:
+    """States as the reference's tests draw them: elements in mesh order, each W[i] in order."""
+    r = JavaRandom(seed)
+    return np.array([[r.next_double() for _ in range(b)] for _ in range(n_elem)], dtype=np.float64)

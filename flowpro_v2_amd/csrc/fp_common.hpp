@@ -1,0 +1,58 @@
+// Shared device-side data structures of the implicit-DG path.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fp_model.hpp"
+
+namespace fp {
+
+// Per-case constants handed to every element kernel by value.
+//
+// Element data live in HBM in the element order of the degrees of freedom
+// (reverse Cuthill-McKee by default), element e owning DOFs [e*B, (e+1)*B) with
+// the reference's local numbering m*Nb + j (equation-major inside an element):
+//   geo  [e*GEO + ...]   inverse Jacobian iT00 iT01 iT10 iT11, |det J|/2, affine A0 A1 A2 B0 B1 B2
+//   fgeo [(e*3+k)*4 + .] face normal nx ny, face Jacobian (edge length / 2), dL
+//   nbr  [e*3+k]         neighbour element or boundary code (< 0)
+//   nbrf [e*3+k]         neighbour's local face index of the shared face
+//   blkn [e*3+k]         BSR block index of (row nbr, column e), -1 on boundary faces
+//   blkd [e]             BSR block index of the diagonal block
+// Reference-element tables (tab): basis and reference gradients at the volume
+// points, per local face at the face points, weights, point coordinates and the
+// reference mass matrix and its inverse.
+constexpr int GEO = 12;
+
+struct Tables {
+    const double* vb;    // [NQV][NB]
+    const double* vd;    // [NQV][NB][2]
+    const double* vw;    // [NQV]
+    const double* vx;    // [NQV][2]
+    const double* fb;    // [3][NQF][NB]
+    const double* fd;    // [3][NQF][NB][2]
+    const double* fw;    // [NQF]
+    const double* fx;    // [3][NQF][2]
+    const double* mref;  // [NB][NB]
+    const double* mrefinv;
+};
+
+struct Mesh {
+    int nElem;
+    const double* geo;
+    const double* fgeo;
+    const int* nbr;
+    const int* nbrf;
+    const int* blkn;
+    const int* blkd;
+};
+
+// compile-time shape of the element kernels selected at run time
+struct ElemLaunch {
+    int model, nb, neq, nqv, nqf, visc;
+};
+
+struct DGParams {
+    double beta0;   // DDG constant (0 when the global order is 1)
+    double cIP;     // interior penalty (penalty when diffusive and order > 1, else 0)
+};
+
+}  // namespace fp

@@ -1,0 +1,1051 @@
+// Host side of libflowpro_hip.so: case setup (neighbour matrix, RCM numbering,
+// Lagrange basis tables, element geometry, BSR pattern, ILU level sets), device
+// memory, GMRES orchestration and the extern "C" entry points of flowpro_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/flowpro_hip.h"
+#include "fp_common.hpp"
+
+namespace fp {
+// launchers (fp_element.hip, fp_linalg.hip)
+bool element_supported(const ElemLaunch& L);
+hipError_t launch_residual(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
+                           const double*, const double*, const double*, double, double, double, double*, double*, double*);
+template <typename BT>
+hipError_t launch_jacobian(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
+                           const double*, double, BT*, const double*, double*, int);
+hipError_t launch_jvp(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
+                      const double*, const double*, double, double*);
+hipError_t launch_size_over_lambda(const ElemLaunch&, hipStream_t, const ModelParams&, const Tables&, const Mesh&,
+                                   const double*, const double*, const double*, double*);
+hipError_t launch_positivity(const ElemLaunch&, hipStream_t, const Tables&, const Mesh&, double*, const double*, int*);
+hipError_t launch_spmv(hipStream_t, int, int, const int*, const int*, const double*, const double*, const double*, double*, int);
+template <typename TF> hipError_t launch_ilu_copy(hipStream_t, size_t, const double*, TF*);
+template <typename TF>
+hipError_t launch_ilu_factor_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, int*);
+template <typename TF>
+hipError_t launch_ilu_fwd_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, const TF*, const double*,
+                                TF*);
+template <typename TF>
+hipError_t launch_ilu_bwd_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, const TF*, const TF*, TF*,
+                                double*);
+hipError_t launch_block_jacobi(hipStream_t, int, int, double*, const double*, double*);
+hipError_t launch_gather_diag(hipStream_t, int, int, const int*, const double*, double*);
+hipError_t launch_min(hipStream_t, size_t, const double*, double*, double*);
+hipError_t launch_colsum(hipStream_t, int, int, const double*, double*);
+hipError_t launch_dot(hipStream_t, size_t, const double*, const double*, double*, double*, int);
+hipError_t launch_axpy_dot(hipStream_t, size_t, double*, const double*, const double*, const double*, double*, double*, int);
+hipError_t launch_scale_div(hipStream_t, size_t, double*, const double*, const double*);
+hipError_t launch_update_solution(hipStream_t, size_t, int, const double*, size_t, const double*, double*);
+hipError_t launch_axpy(hipStream_t, size_t, double, const double*, double*);
+hipError_t launch_fill(hipStream_t, size_t, double, double*);
+hipError_t launch_copy(hipStream_t, size_t, const double*, double*);
+hipError_t launch_nonfinite(hipStream_t, size_t, const double*, int*);
+}  // namespace fp
+
+using namespace fp;
+
+struct fp_ctx {
+    std::string err;
+    // sizes
+    int nElem = 0, order = 0, NB = 0, NEQ = 0, B = 0, NQV = 0, NQF = 0;
+    long long dofs = 0;
+    ElemLaunch L{};
+    ModelParams mp{};
+    DGParams dg{};
+    int precond = 2, bits = 64, m = 30, cycles = 5, positivity = 1;
+    double tol = 1e-2;
+    // host copies
+    std::vector<int> order_;        // order_[r] = mesh element at position r
+    std::vector<int> rank;          // rank[mesh element] = r
+    std::vector<int> blk_own_nb;    // block (row r, column of face k neighbour), -1 on boundary
+    std::vector<int> h_nbr;
+    int nnzb = 0;
+    std::vector<int> lvl_f_ptr, lvl_b_ptr;
+    // device
+    hipStream_t s = nullptr;
+    double *geo = nullptr, *fgeo = nullptr, *tab = nullptr, *esize = nullptr, *isref = nullptr;
+    int *nbr = nullptr, *nbrf = nullptr, *blkn = nullptr, *blkd = nullptr;
+    int *rowptr = nullptr, *colidx = nullptr, *diagpos = nullptr, *lvl_f = nullptr, *lvl_b = nullptr;
+    double* val = nullptr;        // FP64 matrix (nnzb * B * B)
+    void* F = nullptr;            // preconditioner factors (FP64 or FP32)
+    void *iy = nullptr, *ixs = nullptr;
+    double* bjw = nullptr;        // block Jacobi working copies
+    double *W = nullptr, *Wo = nullptr, *Wo2 = nullptr, *R = nullptr, *RHS = nullptr, *enorm = nullptr, *tmp = nullptr;
+    double *x = nullptr, *V = nullptr, *w = nullptr, *r = nullptr, *aux = nullptr, *partial = nullptr, *scal = nullptr;
+    double* y = nullptr;
+    int* istat = nullptr;
+    Tables tb{};
+    Mesh ms{};
+    bool jac_valid = false;
+    double last_a1 = 0;
+};
+
+#define HIPCHK(ctx, expr)                                                                                   \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) {                                                                             \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                                \
+            return FP_EHIP;                                                                                 \
+        }                                                                                                   \
+    } while (0)
+
+// ============================================================== integer setup
+extern "C" int fp_neighbor_matrix(int n, const int* TP, int nb, const int* bnd, int* TT) {
+    if (n < 0 || !TP || !TT || (nb > 0 && !bnd)) return FP_EINVAL;
+    auto key = [&](int i, int k) {
+        long long a = TP[3 * i + k], b = TP[3 * i + (k + 1) % 3];
+        if (a > b) std::swap(a, b);
+        return (a << 32) | b;
+    };
+    std::unordered_map<long long, std::pair<int, int>> faces;
+    faces.reserve((size_t)n * 2);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            auto it = faces.find(key(i, k));
+            if (it == faces.end()) faces.emplace(key(i, k), std::make_pair(i, -1000));
+            else it->second.second = i;
+        }
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            const auto& f = faces[key(i, k)];
+            int t = -1000;
+            if (f.first == i) {
+                if (f.second != -1000) t = f.second;
+            } else {
+                t = f.first;
+            }
+            TT[3 * i + k] = t;
+        }
+    std::unordered_map<long long, int> bmap;
+    for (int r = 0; r < nb; r++) {
+        long long a = bnd[3 * r + 1], b = bnd[3 * r + 2];
+        if (a > b) std::swap(a, b);
+        bmap[(a << 32) | b] = bnd[3 * r];
+    }
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++)
+            if (TT[3 * i + k] == -1000) {
+                auto it = bmap.find(key(i, k));
+                TT[3 * i + k] = it != bmap.end() ? it->second : -1;
+            }
+    return FP_OK;
+}
+
+// breadth-first search from `node` over unvisited elements; returns the last element of the
+// last level (smallest degree among ties), the pseudo-peripheral search of ElementOrdering.
+static int peripheral(const std::vector<int>& adjp, const std::vector<int>& adj, const std::vector<char>& excl, int start,
+                      std::vector<int>& level) {
+    int node = start, ecc = -1;
+    std::vector<int> q;
+    for (int iter = 0; iter < 10; iter++) {
+        q.clear();
+        q.push_back(node);
+        level[node] = 0;
+        std::vector<int> touched = {node};
+        int last = node;
+        for (size_t h = 0; h < q.size(); h++) {
+            int e = q[h];
+            for (int p = adjp[e]; p < adjp[e + 1]; p++) {
+                int t = adj[p];
+                if (!excl[t] && level[t] == -1) {
+                    level[t] = level[e] + 1;
+                    touched.push_back(t);
+                    q.push_back(t);
+                    int dt = adjp[t + 1] - adjp[t], dl = adjp[last + 1] - adjp[last];
+                    if (level[t] > level[last] || (level[t] == level[last] && dt < dl)) last = t;
+                }
+            }
+        }
+        int lv = level[last];
+        for (int t : touched) level[t] = -1;
+        if (lv <= ecc) break;
+        ecc = lv;
+        node = last;
+    }
+    return node;
+}
+
+extern "C" int fp_rcm_order(int n, const int* TT, int* order) {
+    if (n < 0 || !TT || !order) return FP_EINVAL;
+    std::vector<int> deg(n, 0), adjp(n + 1, 0), adj;
+    adj.reserve((size_t)n * 3);
+    for (int i = 0; i < n; i++) {
+        int cnt = 0;
+        int nb[3];
+        for (int k = 0; k < 3; k++) {
+            int t = TT[3 * i + k];
+            bool dup = false;
+            for (int c = 0; c < cnt; c++) dup |= nb[c] == t;
+            if (t > -1 && t != i && !dup) nb[cnt++] = t;
+        }
+        for (int c = 0; c < cnt; c++) adj.push_back(nb[c]);
+        adjp[i + 1] = (int)adj.size();
+        deg[i] = cnt;
+    }
+    for (int i = 0; i < n; i++)   // neighbours by increasing degree, stable
+        std::stable_sort(adj.begin() + adjp[i], adj.begin() + adjp[i + 1], [&](int a, int b) { return deg[a] < deg[b]; });
+    std::vector<char> visited(n, 0);
+    std::vector<int> level(n, -1), q;
+    int count = 0;
+    // the first unvisited element of minimal degree is found by scanning; keep a cursor per degree
+    while (count < n) {
+        int start = -1;
+        for (int i = 0; i < n; i++)
+            if (!visited[i] && (start == -1 || deg[i] < deg[start])) {
+                start = i;
+                if (deg[i] == 0) break;
+            }
+        start = peripheral(adjp, adj, visited, start, level);
+        q.clear();
+        q.push_back(start);
+        visited[start] = 1;
+        for (size_t h = 0; h < q.size(); h++) {
+            int e = q[h];
+            order[count++] = e;
+            for (int p = adjp[e]; p < adjp[e + 1]; p++)
+                if (!visited[adj[p]]) { visited[adj[p]] = 1; q.push_back(adj[p]); }
+        }
+    }
+    std::reverse(order, order + n);
+    return FP_OK;
+}
+
+// ============================================================== basis (basis2Dtriangle)
+namespace {
+struct Basis {
+    int order, nb;
+    std::vector<int> xe, ye;
+    std::vector<double> coef;   // coef[i*nb + m]
+};
+
+bool invert(std::vector<double>& A, int n, std::vector<double>& inv) {
+    // Gauss-Jordan with partial pivoting in long double
+    std::vector<long double> a(A.begin(), A.end()), b((size_t)n * n, 0.0L);
+    for (int i = 0; i < n; i++) b[(size_t)i * n + i] = 1;
+    for (int p = 0; p < n; p++) {
+        int best = p;
+        for (int r = p + 1; r < n; r++)
+            if (std::fabs((double)a[(size_t)r * n + p]) > std::fabs((double)a[(size_t)best * n + p])) best = r;
+        if (a[(size_t)best * n + p] == 0) return false;
+        for (int c = 0; c < n; c++) {
+            std::swap(a[(size_t)p * n + c], a[(size_t)best * n + c]);
+            std::swap(b[(size_t)p * n + c], b[(size_t)best * n + c]);
+        }
+        long double d = a[(size_t)p * n + p];
+        for (int c = 0; c < n; c++) { a[(size_t)p * n + c] /= d; b[(size_t)p * n + c] /= d; }
+        for (int r = 0; r < n; r++)
+            if (r != p) {
+                long double f = a[(size_t)r * n + p];
+                if (f != 0)
+                    for (int c = 0; c < n; c++) {
+                        a[(size_t)r * n + c] -= f * a[(size_t)p * n + c];
+                        b[(size_t)r * n + c] -= f * b[(size_t)p * n + c];
+                    }
+            }
+    }
+    inv.assign((size_t)n * n, 0.0);
+    for (size_t i = 0; i < inv.size(); i++) inv[i] = (double)b[i];
+    return true;
+}
+
+// Lagrange basis on the equispaced barycentric nodes, monomial coefficients from the
+// inverse Vandermonde matrix (FlowPro/src/main/java/flowpro/core/basis/basis2Dtriangle.java:22-62)
+Basis make_basis(int order) {
+    Basis b;
+    b.order = order;
+    b.nb = order * (order + 1) / 2;
+    b.xe.assign(order * order, 0);
+    b.ye.assign(order * order, 0);
+    int s = 0;
+    for (int i = 0; i < order; i++)
+        for (int j = 0; j <= i; j++) { b.xe[s] = i - j; b.ye[s] = j; s++; }
+    for (int i = order - 1; i > 0; i--)
+        for (int j = 0; j < i; j++) { b.xe[s] = order - 1 - j; b.ye[s] = order - i + j; s++; }
+    if (order == 1) {
+        b.coef = {1.0};
+        return b;
+    }
+    std::vector<double> nodes;   // (xi, eta) of the nodes in the reference's order
+    for (int i = 0; i < order; i++) {
+        double xi_ = (double)i / (order - 1);
+        for (int j = 0; j <= i; j++) {
+            double y = i == 0 ? 0.0 : (double)j / i * xi_;
+            nodes.push_back(y);          // b[1]
+            nodes.push_back(xi_ - y);    // b[2]
+        }
+    }
+    std::vector<double> V((size_t)b.nb * b.nb), inv;
+    for (int i = 0; i < b.nb; i++)
+        for (int j = 0; j < b.nb; j++) V[(size_t)i * b.nb + j] = std::pow(nodes[2 * i], b.xe[j]) * std::pow(nodes[2 * i + 1], b.ye[j]);
+    invert(V, b.nb, inv);
+    b.coef = inv;
+    return b;
+}
+double basis_fun(const Basis& b, int m, double x, double y) {
+    double v = 0;
+    for (int i = 0; i < b.nb; i++) v += b.coef[(size_t)i * b.nb + m] * std::pow(x, b.xe[i]) * std::pow(y, b.ye[i]);
+    return v;
+}
+double basis_der(const Basis& b, int m, double x, double y, int d) {
+    double v = 0;
+    for (int i = 0; i < b.nb; i++) {
+        int ex = b.xe[i], ey = b.ye[i];
+        if (d == 0 && ex > 0) v += b.coef[(size_t)i * b.nb + m] * ex * std::pow(x, ex - 1) * std::pow(y, ey);
+        if (d == 1 && ey > 0) v += b.coef[(size_t)i * b.nb + m] * ey * std::pow(x, ex) * std::pow(y, ey - 1);
+    }
+    return v;
+}
+
+template <typename T> hipError_t dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void**)p, n * sizeof(T));
+}
+template <typename T> hipError_t upload(T** p, const std::vector<T>& h) {
+    hipError_t e = dalloc(p, h.size());
+    if (e != hipSuccess) return e;
+    if (h.empty()) return hipSuccess;
+    return hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+}  // namespace
+
+// ============================================================== create / destroy
+static void free_all(fp_ctx* c) {
+    void* ptrs[] = {c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
+                    c->diagpos, c->lvl_f, c->lvl_b, c->val, c->F, c->iy, c->ixs, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
+                    c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->istat};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->s) (void)hipStreamDestroy(c->s);
+}
+
+static int create_impl(const fp_case* cs, fp_ctx* c) {
+    const int n = cs->n_elems;
+    c->nElem = n;
+    c->order = cs->order;
+    Basis bs = make_basis(cs->order);
+    c->NB = bs.nb;
+    c->NEQ = cs->model == MODEL_SCALAR ? 1 : 4;
+    c->B = c->NB * c->NEQ;
+    c->NQV = cs->n_vol;
+    c->NQF = cs->n_face;
+    c->dofs = (long long)n * c->B;
+    const bool diffusive = cs->model == MODEL_SCALAR ? cs->scalar_k > 0 : cs->viscous != 0;
+    c->L = ElemLaunch{cs->model, c->NB, c->NEQ, c->NQV, c->NQF, diffusive ? 1 : 0};
+    if (!element_supported(c->L)) {
+        c->err = "no element kernel for this model / order / quadrature";
+        return FP_UNSUPPORTED;
+    }
+    if (cs->preconditioner < 0 || cs->preconditioner > 2 || (cs->precond_bits != 64 && cs->precond_bits != 32)) {
+        c->err = "bad preconditioner settings";
+        return FP_EINVAL;
+    }
+    c->precond = cs->preconditioner;
+    c->bits = cs->precond_bits;
+    c->m = cs->gmres_restart > 0 ? cs->gmres_restart : 30;
+    c->cycles = cs->gmres_cycles > 0 ? cs->gmres_cycles : 5;
+    c->tol = cs->gmres_tol;
+    c->positivity = cs->positivity_limiter;
+    ModelParams& P = c->mp;
+    P.kind = cs->model;
+    P.nEqs = c->NEQ;
+    P.diffusive = diffusive;
+    P.has_attack = cs->has_attack;
+    P.kapa = cs->kappa;
+    P.pOut = cs->p_out;
+    P.Re = cs->reynolds;
+    P.Pr = cs->prandtl;
+    P.attack = cs->attack;
+    P.a0 = cs->scalar_a0;
+    P.a1 = cs->scalar_a1;
+    P.k = cs->scalar_k;
+    P.lambda = std::sqrt(cs->scalar_a0 * cs->scalar_a0 + cs->scalar_a1 * cs->scalar_a1);
+    c->dg.beta0 = cs->order == 1 ? 0.0 : cs->beta0;                  // DGpure.java:287-289
+    c->dg.cIP = (cs->order > 1 && diffusive) ? cs->penalty : 0.0;   // DGpure.limiter :396-398
+
+    // --- topology and numbering
+    std::vector<int> TT((size_t)3 * n);
+    int rc = fp_neighbor_matrix(n, cs->tp, cs->n_bnd, cs->bnd, TT.data());
+    if (rc) return rc;
+    c->order_.resize(n);
+    if (cs->ordering == 1) fp_rcm_order(n, TT.data(), c->order_.data());
+    else
+        for (int i = 0; i < n; i++) c->order_[i] = i;
+    c->rank.resize(n);
+    for (int r = 0; r < n; r++) c->rank[c->order_[r]] = r;
+
+    // --- reference tables
+    const int NB = c->NB, NQV = c->NQV, NQF = c->NQF;
+    std::vector<double> vb(NQV * NB), vd(NQV * NB * 2), vw(cs->vol_w, cs->vol_w + NQV), vx(cs->vol_xi, cs->vol_xi + 2 * NQV);
+    for (int q = 0; q < NQV; q++)
+        for (int j = 0; j < NB; j++) {
+            double x = cs->vol_xi[2 * q], y = cs->vol_xi[2 * q + 1];
+            vb[q * NB + j] = basis_fun(bs, j, x, y);
+            vd[(q * NB + j) * 2] = basis_der(bs, j, x, y, 0);
+            vd[(q * NB + j) * 2 + 1] = basis_der(bs, j, x, y, 1);
+        }
+    static const double CX[3][2] = {{0, 0}, {1, 0}, {0, 1}};
+    std::vector<double> fb(3 * NQF * NB), fd(3 * NQF * NB * 2), fw(cs->face_w, cs->face_w + NQF), fx(3 * NQF * 2);
+    for (int k = 0; k < 3; k++)
+        for (int p = 0; p < NQF; p++) {
+            double t = cs->face_xi[p];
+            int a = k, b = (k + 1) % 3;
+            double x = (1 + t) / 2 * CX[b][0] + (1 - t) / 2 * CX[a][0];
+            double y = (1 + t) / 2 * CX[b][1] + (1 - t) / 2 * CX[a][1];
+            fx[(k * NQF + p) * 2] = x;
+            fx[(k * NQF + p) * 2 + 1] = y;
+            for (int j = 0; j < NB; j++) {
+                fb[(k * NQF + p) * NB + j] = basis_fun(bs, j, x, y);
+                fd[((k * NQF + p) * NB + j) * 2] = basis_der(bs, j, x, y, 0);
+                fd[((k * NQF + p) * NB + j) * 2 + 1] = basis_der(bs, j, x, y, 1);
+            }
+        }
+    std::vector<double> mref(NB * NB, 0.0), mrefinv, isref(NB, 0.0);
+    double wsum = 0;
+    for (int q = 0; q < NQV; q++) wsum += vw[q];
+    for (int i = 0; i < NB; i++) {
+        for (int j = 0; j < NB; j++)
+            for (int q = 0; q < NQV; q++) mref[i * NB + j] += vw[q] * vb[q * NB + i] * vb[q * NB + j];
+        for (int q = 0; q < NQV; q++) isref[i] += vw[q] * vb[q * NB + i];
+        isref[i] /= wsum;   // Is_i = int phi_i / area (DGpure.java:561-563)
+    }
+    invert(mref, NB, mrefinv);
+    std::vector<double> tab;
+    auto put = [&](const std::vector<double>& v) {
+        size_t off = tab.size();
+        tab.insert(tab.end(), v.begin(), v.end());
+        return off;
+    };
+    size_t o_vb = put(vb), o_vd = put(vd), o_vw = put(vw), o_vx = put(vx), o_fb = put(fb), o_fd = put(fd), o_fw = put(fw),
+           o_fx = put(fx), o_m = put(mref), o_mi = put(mrefinv);
+    HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    HIPCHK(c, upload(&c->tab, tab));
+    HIPCHK(c, upload(&c->isref, isref));
+    c->tb = Tables{c->tab + o_vb, c->tab + o_vd, c->tab + o_vw, c->tab + o_vx, c->tab + o_fb,
+                   c->tab + o_fd, c->tab + o_fw, c->tab + o_fx, c->tab + o_m,  c->tab + o_mi};
+
+    // --- geometry in the DOF order (transformation2Dtriangle, faceTransformation2Dline,
+    //     Element.computeGeometry)
+    std::vector<double> geo((size_t)n * GEO), fgeo((size_t)n * 12), esize(n), Xs(2 * (size_t)n);
+    std::vector<int> nbr(3 * (size_t)n), nbrf(3 * (size_t)n);
+    for (int r = 0; r < n; r++) {
+        int e = c->order_[r];
+        const double* p0 = cs->pxy + 2 * cs->tp[3 * e];
+        const double* p1 = cs->pxy + 2 * cs->tp[3 * e + 1];
+        const double* p2 = cs->pxy + 2 * cs->tp[3 * e + 2];
+        double A0 = p1[0] - p0[0], A1 = p2[0] - p0[0], B0 = p1[1] - p0[1], B1 = p2[1] - p0[1];
+        double det = A0 * B1 - A1 * B0;
+        double* g = &geo[(size_t)r * GEO];
+        g[0] = B1 / det;    // iT00 = d xi / d x
+        g[1] = -A1 / det;   // iT01 = d xi / d y
+        g[2] = -B0 / det;   // iT10 = d eta / d x
+        g[3] = A0 / det;    // iT11 = d eta / d y
+        g[4] = std::fabs(det) / 2;
+        g[5] = A0; g[6] = A1; g[7] = p0[0];
+        g[8] = B0; g[9] = B1; g[10] = p0[1];
+        g[11] = 0;
+        Xs[2 * r] = A0 / 3 + A1 / 3 + p0[0];
+        Xs[2 * r + 1] = B0 / 3 + B1 / 3 + p0[1];
+    }
+    for (int r = 0; r < n; r++) {
+        int e = c->order_[r];
+        const double* g = &geo[(size_t)r * GEO];
+        double perim = 0;
+        for (int k = 0; k < 3; k++) {
+            int a = k, b = (k + 1) % 3;
+            double s0 = CX[b][0] - CX[a][0], s1 = CX[b][1] - CX[a][1];
+            double Js0 = g[5] * s0 + g[6] * s1, Js1 = g[8] * s0 + g[9] * s1;
+            double len = std::sqrt(Js0 * Js0 + Js1 * Js1);
+            double nx = Js1 / len, ny = -Js0 / len;
+            int t = TT[3 * (size_t)e + k];
+            double dL;
+            if (t > -1) {
+                int rt = c->rank[t];
+                dL = (Xs[2 * r] - Xs[2 * rt]) * nx + (Xs[2 * r + 1] - Xs[2 * rt + 1]) * ny;
+                nbr[3 * r + k] = rt;
+                int kr = 0;
+                for (int s = 0; s < 3; s++)
+                    if (TT[3 * (size_t)t + s] == e) { kr = s; break; }
+                nbrf[3 * r + k] = kr;
+            } else {
+                double mx = g[5] * ((CX[a][0] + CX[b][0]) / 2) + g[6] * ((CX[a][1] + CX[b][1]) / 2) + g[7];
+                double my = g[8] * ((CX[a][0] + CX[b][0]) / 2) + g[9] * ((CX[a][1] + CX[b][1]) / 2) + g[10];
+                dL = (Xs[2 * r] - mx) * nx + (Xs[2 * r + 1] - my) * ny;
+                nbr[3 * r + k] = t;
+                nbrf[3 * r + k] = 0;
+            }
+            double* f = &fgeo[((size_t)r * 3 + k) * 4];
+            f[0] = nx; f[1] = ny; f[2] = len / 2; f[3] = std::fabs(dL);
+            perim += len;
+        }
+        esize[r] = 3 * g[4] / perim;   // elemSize = nFaces * area / sum S (Element.java:549-554)
+    }
+    c->h_nbr = nbr;
+
+    // --- BSR pattern: row r = own block + face neighbours, sorted by column
+    std::vector<int> rowptr(n + 1, 0), colidx, diagpos(n);
+    for (int r = 0; r < n; r++) {
+        std::vector<int> cols = {r};
+        for (int k = 0; k < 3; k++)
+            if (nbr[3 * r + k] > -1 && std::find(cols.begin(), cols.end(), nbr[3 * r + k]) == cols.end())
+                cols.push_back(nbr[3 * r + k]);
+        std::sort(cols.begin(), cols.end());
+        for (size_t i = 0; i < cols.size(); i++) {
+            if (cols[i] == r) diagpos[r] = (int)colidx.size();
+            colidx.push_back(cols[i]);
+        }
+        rowptr[r + 1] = (int)colidx.size();
+    }
+    c->nnzb = (int)colidx.size();
+    auto find_blk = [&](int row, int col) {
+        for (int s = rowptr[row]; s < rowptr[row + 1]; s++)
+            if (colidx[s] == col) return s;
+        return -1;
+    };
+    std::vector<int> blkn(3 * (size_t)n, -1), blkd(n);
+    c->blk_own_nb.assign(3 * (size_t)n, -1);
+    for (int r = 0; r < n; r++) {
+        blkd[r] = diagpos[r];
+        for (int k = 0; k < 3; k++)
+            if (nbr[3 * r + k] > -1) {
+                blkn[3 * r + k] = find_blk(nbr[3 * r + k], r);
+                c->blk_own_nb[3 * r + k] = find_blk(r, nbr[3 * r + k]);
+            }
+    }
+    // --- ILU level sets (forward: lower blocks, backward: upper blocks)
+    std::vector<int> lf(n, 0), lb(n, 0);
+    int nlf = 0, nlb = 0;
+    for (int r = 0; r < n; r++) {
+        int lv = 0;
+        for (int s = rowptr[r]; s < diagpos[r]; s++) lv = std::max(lv, lf[colidx[s]] + 1);
+        lf[r] = lv;
+        nlf = std::max(nlf, lv + 1);
+    }
+    for (int r = n - 1; r >= 0; r--) {
+        int lv = 0;
+        for (int s = diagpos[r] + 1; s < rowptr[r + 1]; s++) lv = std::max(lv, lb[colidx[s]] + 1);
+        lb[r] = lv;
+        nlb = std::max(nlb, lv + 1);
+    }
+    auto bucket = [&](const std::vector<int>& lv, int nl, std::vector<int>& ptr, std::vector<int>& rows) {
+        ptr.assign(nl + 1, 0);
+        for (int r = 0; r < n; r++) ptr[lv[r] + 1]++;
+        for (int l = 0; l < nl; l++) ptr[l + 1] += ptr[l];
+        rows.assign(n, 0);
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for (int r = 0; r < n; r++) rows[fill[lv[r]]++] = r;
+    };
+    std::vector<int> rowsf, rowsb;
+    bucket(lf, nlf, c->lvl_f_ptr, rowsf);
+    bucket(lb, nlb, c->lvl_b_ptr, rowsb);
+
+    HIPCHK(c, upload(&c->geo, geo));
+    HIPCHK(c, upload(&c->fgeo, fgeo));
+    HIPCHK(c, upload(&c->esize, esize));
+    HIPCHK(c, upload(&c->nbr, nbr));
+    HIPCHK(c, upload(&c->nbrf, nbrf));
+    HIPCHK(c, upload(&c->blkn, blkn));
+    HIPCHK(c, upload(&c->blkd, blkd));
+    HIPCHK(c, upload(&c->rowptr, rowptr));
+    HIPCHK(c, upload(&c->colidx, colidx));
+    HIPCHK(c, upload(&c->diagpos, diagpos));
+    HIPCHK(c, upload(&c->lvl_f, rowsf));
+    HIPCHK(c, upload(&c->lvl_b, rowsb));
+    c->ms = Mesh{n, c->geo, c->fgeo, c->nbr, c->nbrf, c->blkn, c->blkd};
+
+    // --- state, matrix, Krylov workspace
+    const size_t N = (size_t)c->dofs, BB = (size_t)c->B * c->B;
+    HIPCHK(c, dalloc(&c->W, N));
+    HIPCHK(c, dalloc(&c->Wo, N));
+    HIPCHK(c, dalloc(&c->Wo2, N));
+    HIPCHK(c, dalloc(&c->R, N));
+    HIPCHK(c, dalloc(&c->RHS, N));
+    HIPCHK(c, dalloc(&c->tmp, std::max(N, (size_t)n)));
+    HIPCHK(c, dalloc(&c->enorm, (size_t)n * 2 * c->NEQ));
+    HIPCHK(c, dalloc(&c->val, (size_t)c->nnzb * BB));
+    if (c->precond == 2) {
+        size_t es = c->bits == 32 ? sizeof(float) : sizeof(double);
+        HIPCHK(c, hipMalloc(&c->F, (size_t)c->nnzb * BB * es));
+        HIPCHK(c, hipMalloc(&c->iy, N * es));
+        HIPCHK(c, hipMalloc(&c->ixs, N * es));
+    } else if (c->precond == 1) {
+        HIPCHK(c, dalloc(&c->bjw, (size_t)n * BB));
+    }
+    HIPCHK(c, dalloc(&c->x, N));
+    HIPCHK(c, dalloc(&c->V, N * (c->m + 1)));
+    HIPCHK(c, dalloc(&c->w, N));
+    HIPCHK(c, dalloc(&c->r, N));
+    HIPCHK(c, dalloc(&c->aux, N));
+    HIPCHK(c, dalloc(&c->partial, 4096));
+    HIPCHK(c, dalloc(&c->scal, 64 + 2 * (size_t)c->m + 8));
+    HIPCHK(c, dalloc(&c->y, (size_t)c->m + 1));
+    HIPCHK(c, dalloc(&c->istat, 4));
+    HIPCHK(c, hipMemsetAsync(c->W, 0, N * sizeof(double), c->s));
+    HIPCHK(c, hipMemsetAsync(c->Wo, 0, N * sizeof(double), c->s));
+    HIPCHK(c, hipMemsetAsync(c->Wo2, 0, N * sizeof(double), c->s));
+    HIPCHK(c, hipMemsetAsync(c->x, 0, N * sizeof(double), c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return FP_OK;
+}
+
+extern "C" int fp_create(const fp_case* cs, fp_ctx** out) {
+    if (!cs || !out || cs->n_elems <= 0 || !cs->tp || !cs->pxy || cs->order < 1 || cs->n_vol <= 0 || cs->n_face <= 0 ||
+        !cs->vol_xi || !cs->vol_w || !cs->face_xi || !cs->face_w)
+        return FP_EINVAL;
+    *out = nullptr;
+    fp_ctx* c = new fp_ctx();
+    int rc = create_impl(cs, c);
+    if (rc != FP_OK) {
+        std::fprintf(stderr, "fp_create: %s\n", c->err.c_str());
+        free_all(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return FP_OK;
+}
+
+extern "C" int fp_destroy(fp_ctx* c) {
+    if (!c) return FP_EINVAL;
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    free_all(c);
+    delete c;
+    return FP_OK;
+}
+
+extern "C" const char* fp_last_error(const fp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int fp_info(const fp_ctx* c, long long* info) {
+    if (!c || !info) return FP_EINVAL;
+    info[0] = c->NB;
+    info[1] = c->NEQ;
+    info[2] = c->dofs;
+    info[3] = (long long)c->lvl_f_ptr.size() - 1;
+    info[4] = (long long)c->lvl_b_ptr.size() - 1;
+    info[5] = c->nnzb;
+    return FP_OK;
+}
+
+extern "C" int fp_element_order(const fp_ctx* c, int* order) {
+    if (!c || !order) return FP_EINVAL;
+    std::memcpy(order, c->order_.data(), sizeof(int) * c->nElem);
+    return FP_OK;
+}
+
+extern "C" void* fp_stream(fp_ctx* c) { return c ? (void*)c->s : nullptr; }
+
+// ============================================================== state
+// element layout, mesh order <-> device (DOF order)
+static int to_device(fp_ctx* c, const double* src, double* dst) {
+    const size_t B = c->B;
+    std::vector<double> h((size_t)c->dofs);
+    for (int r = 0; r < c->nElem; r++) std::memcpy(&h[r * B], src + (size_t)c->order_[r] * B, B * sizeof(double));
+    HIPCHK(c, hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return FP_OK;
+}
+static int from_device(fp_ctx* c, const double* src, double* dst) {
+    const size_t B = c->B;
+    std::vector<double> h((size_t)c->dofs);
+    HIPCHK(c, hipMemcpyAsync(h.data(), src, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    for (int r = 0; r < c->nElem; r++) std::memcpy(dst + (size_t)c->order_[r] * B, &h[r * B], B * sizeof(double));
+    return FP_OK;
+}
+
+extern "C" int fp_set_state(fp_ctx* c, const double* W, const double* Wo, const double* Wo2) {
+    if (!c || !W) return FP_EINVAL;
+    int rc = to_device(c, W, c->W);
+    if (rc) return rc;
+    rc = to_device(c, Wo ? Wo : W, c->Wo);
+    if (rc) return rc;
+    return to_device(c, Wo2 ? Wo2 : (Wo ? Wo : W), c->Wo2);
+}
+
+extern "C" int fp_get_state(fp_ctx* c, double* W) {
+    if (!c || !W) return FP_EINVAL;
+    return from_device(c, c->W, W);
+}
+
+extern "C" int fp_next_level(fp_ctx* c) {
+    if (!c) return FP_EINVAL;
+    const size_t N = (size_t)c->dofs * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(c->Wo2, c->Wo, N, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipMemcpyAsync(c->Wo, c->W, N, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return FP_OK;
+}
+
+extern "C" int fp_restore_level(fp_ctx* c) {
+    if (!c) return FP_EINVAL;
+    HIPCHK(c, hipMemcpyAsync(c->W, c->Wo, (size_t)c->dofs * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return FP_OK;
+}
+
+// ============================================================== Newton step
+extern "C" int fp_time_step(fp_ctx* c, double* out) {
+    if (!c || !out) return FP_EINVAL;
+    HIPCHK(c, launch_size_over_lambda(c->L, c->s, c->mp, c->tb, c->ms, c->W, c->esize, c->isref, c->tmp));
+    HIPCHK(c, launch_min(c->s, (size_t)c->nElem, c->tmp, c->partial, c->scal));
+    HIPCHK(c, hipMemcpyAsync(out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return FP_OK;
+}
+
+// BDF2.coefficients (FlowPro/src/main/java/flowpro/core/element/BDF2.java:53-61)
+static void bdf_coefficients(int time_order, double dt, double dto, double* a) {
+    if (time_order == 2 && dto > 0) {
+        a[0] = (2 * dt + dto) / (dt * (dt + dto));
+        a[1] = -(dt + dto) / (dt * dto);
+        a[2] = dt / (dto * (dt + dto));
+    } else {
+        a[0] = 1 / dt;
+        a[1] = -1 / dt;
+        a[2] = 0;
+    }
+}
+
+static int assemble_impl(fp_ctx* c, double dt, double dto, int time_order, int with_jac, double* eq_norms, double* rhs_norm) {
+    double a[3];
+    bdf_coefficients(time_order, dt, dto, a);
+    HIPCHK(c, launch_residual(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R, c->RHS,
+                              c->enorm));
+    if (with_jac) {
+        HIPCHK(c, launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a[0], c->val, nullptr, nullptr, 0));
+        c->jac_valid = true;
+        c->last_a1 = a[0];
+    }
+    if (eq_norms || rhs_norm) {
+        const int ne = c->NEQ;
+        HIPCHK(c, launch_colsum(c->s, c->nElem, 2 * ne, c->enorm, c->scal));
+        double h[10];
+        HIPCHK(c, hipMemcpyAsync(h, c->scal, sizeof(double) * 2 * ne, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        if (eq_norms)
+            for (int m = 0; m < ne; m++) eq_norms[m] = std::sqrt(h[m]);
+        if (rhs_norm) {
+            double s = 0;
+            for (int m = 0; m < ne; m++) s += h[ne + m];
+            *rhs_norm = std::sqrt(s);
+        }
+    }
+    return FP_OK;
+}
+
+extern "C" int fp_assemble(fp_ctx* c, double dt, double dto, int time_order, int with_jacobian, double* eq_norms,
+                           double* rhs_norm) {
+    if (!c || !(dt > 0)) return FP_EINVAL;
+    int rc = assemble_impl(c, dt, dto, time_order, with_jacobian, eq_norms, rhs_norm);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return FP_OK;
+}
+
+// --- preconditioner
+template <typename TF> static int ilu_factor_t(fp_ctx* c) {
+    TF* F = (TF*)c->F;
+    const size_t tot = (size_t)c->nnzb * c->B * c->B;
+    HIPCHK(c, launch_ilu_copy<TF>(c->s, tot, c->val, F));
+    HIPCHK(c, hipMemsetAsync(c->istat, 0, sizeof(int), c->s));
+    for (size_t l = 0; l + 1 < c->lvl_f_ptr.size(); l++) {
+        int b = c->lvl_f_ptr[l], e = c->lvl_f_ptr[l + 1];
+        HIPCHK(c, launch_ilu_factor_level<TF>(c->s, c->B, c->lvl_f + b, e - b, c->rowptr, c->colidx, c->diagpos, F, c->istat));
+    }
+    return FP_OK;
+}
+template <typename TF> static int ilu_apply_t(fp_ctx* c, double* out, const double* in) {
+    const TF* F = (const TF*)c->F;
+    for (size_t l = 0; l + 1 < c->lvl_f_ptr.size(); l++) {
+        int b = c->lvl_f_ptr[l], e = c->lvl_f_ptr[l + 1];
+        HIPCHK(c, launch_ilu_fwd_level<TF>(c->s, c->B, c->lvl_f + b, e - b, c->rowptr, c->colidx, c->diagpos, F, in, (TF*)c->iy));
+    }
+    for (size_t l = 0; l + 1 < c->lvl_b_ptr.size(); l++) {
+        int b = c->lvl_b_ptr[l], e = c->lvl_b_ptr[l + 1];
+        HIPCHK(c, launch_ilu_bwd_level<TF>(c->s, c->B, c->lvl_b + b, e - b, c->rowptr, c->colidx, c->diagpos, F, (const TF*)c->iy,
+                                           (TF*)c->ixs, out));
+    }
+    return FP_OK;
+}
+
+static int prec_factor(fp_ctx* c) {
+    if (c->precond == 2) return c->bits == 32 ? ilu_factor_t<float>(c) : ilu_factor_t<double>(c);
+    if (c->precond == 1) {
+        HIPCHK(c, launch_gather_diag(c->s, c->nElem, c->B * c->B, c->blkd, c->val, c->bjw));
+    }
+    return FP_OK;
+}
+static int prec_apply(fp_ctx* c, double* out, const double* in) {
+    if (c->precond == 2) return c->bits == 32 ? ilu_apply_t<float>(c, out, in) : ilu_apply_t<double>(c, out, in);
+    if (c->precond == 1) {
+        HIPCHK(c, launch_block_jacobi(c->s, c->nElem, c->B, c->bjw, in, out));
+        return FP_OK;
+    }
+    HIPCHK(c, launch_copy(c->s, (size_t)c->dofs, in, out));
+    return FP_OK;
+}
+
+static int dnorm(fp_ctx* c, const double* v, double* h) {
+    HIPCHK(c, launch_dot(c->s, (size_t)c->dofs, v, v, c->partial, c->scal, 1));
+    HIPCHK(c, hipMemcpyAsync(h, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return FP_OK;
+}
+
+// Givens rotation (Gmres.rotmat, Gmres.java:132-148)
+static void rotmat(double a, double b, double& cs, double& sn) {
+    if (b == 0) { cs = 1; sn = 0; }
+    else if (std::fabs(b) > std::fabs(a)) { double t = a / b; sn = 1 / std::sqrt(1 + t * t); cs = t * sn; }
+    else { double t = b / a; cs = 1 / std::sqrt(1 + t * t); sn = t * cs; }
+}
+// elimination on the rotated (upper triangular) H (Gmres.lsolve, Gmres.java:177-199)
+static std::vector<double> lsolve(std::vector<std::vector<double>> A, std::vector<double> b, int N) {
+    for (int p = 0; p < N; p++)
+        for (int i = p + 1; i < N; i++) {
+            double alpha = A[i][p] / A[p][p];
+            b[i] -= alpha * b[p];
+            for (int j = p; j < N; j++) A[i][j] -= alpha * A[p][j];
+        }
+    std::vector<double> x(N);
+    for (int i = N - 1; i >= 0; i--) {
+        double sum = 0.0;
+        for (int j = i + 1; j < N; j++) sum += A[i][j] * x[j];
+        x[i] = (b[i] - sum) / A[i][i];
+    }
+    return x;
+}
+
+// Gmres.solve (FlowPro/src/main/java/flowpro/core/LinearSolvers/Gmres.java:42-122): left
+// preconditioned, restart m, modified Gram-Schmidt, residual relative to |M^-1 b|.
+static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_out, bool* ok) {
+    const size_t N = (size_t)c->dofs;
+    const int m = c->m;
+    int rc;
+    *iters = 0;
+    std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
+    std::vector<double> cs(m), sn(m), s(m + 1);
+    double bnrm2, nr, error;
+    if ((rc = prec_apply(c, c->r, b))) return rc;
+    if ((rc = dnorm(c, c->r, &bnrm2))) return rc;
+    if (bnrm2 == 0) bnrm2 = 1;
+    HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, x, b, c->aux, 1));
+    if ((rc = prec_apply(c, c->r, c->aux))) return rc;
+    if ((rc = dnorm(c, c->r, &nr))) return rc;
+    error = nr / bnrm2;
+    if (error < c->tol) { *ok = true; *err_out = error; return FP_OK; }
+    double* hdev = c->scal + 8;   // H column on the device
+    std::vector<double> hcol(m + 2);
+    for (int it = 0; it < c->cycles; it++) {
+        double norm_r;
+        if ((rc = dnorm(c, c->r, &norm_r))) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->scal + 1, &norm_r, sizeof(double), hipMemcpyHostToDevice, c->s));
+        HIPCHK(c, launch_scale_div(c->s, N, c->V, c->r, c->scal + 1));
+        std::fill(s.begin(), s.end(), 0.0);
+        s[0] = norm_r;
+        int nused = m;
+        bool done = false;
+        for (int i = 0; i < m; i++) {
+            double* Vi = c->V + (size_t)i * N;
+            HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, Vi, nullptr, c->aux, 0));
+            if ((rc = prec_apply(c, c->w, c->aux))) return rc;
+            (*iters)++;
+            // MGS: h_k = (w, V_k); w -= h_k V_k  (the next dot is fused into the update)
+            HIPCHK(c, launch_dot(c->s, N, c->w, c->V, c->partial, hdev, 0));
+            for (int k = 0; k <= i; k++) {
+                const double* u = k < i ? c->V + (size_t)(k + 1) * N : nullptr;
+                HIPCHK(c, launch_axpy_dot(c->s, N, c->w, c->V + (size_t)k * N, hdev + k, u, c->partial, hdev + k + 1,
+                                          k < i ? 0 : 1));
+            }
+            HIPCHK(c, launch_scale_div(c->s, N, c->V + (size_t)(i + 1) * N, c->w, hdev + i + 1));
+            HIPCHK(c, hipMemcpyAsync(hcol.data(), hdev, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, c->s));
+            HIPCHK(c, hipStreamSynchronize(c->s));
+            for (int k = 0; k <= i + 1; k++) H[k][i] = hcol[k];
+            for (int k = 0; k <= i - 1; k++) {
+                double temp = cs[k] * H[k][i] + sn[k] * H[k + 1][i];
+                H[k + 1][i] = -sn[k] * H[k][i] + cs[k] * H[k + 1][i];
+                H[k][i] = temp;
+            }
+            rotmat(H[i][i], H[i + 1][i], cs[i], sn[i]);
+            double temp = cs[i] * s[i];
+            s[i + 1] = -sn[i] * s[i];
+            s[i] = temp;
+            H[i][i] = cs[i] * H[i][i] + sn[i] * H[i + 1][i];
+            H[i + 1][i] = 0;
+            error = std::fabs(s[i + 1]) / bnrm2;
+            if (error <= c->tol) {
+                nused = i + 1;
+                done = true;
+                break;
+            }
+        }
+        std::vector<double> yv = lsolve(H, s, nused);
+        HIPCHK(c, hipMemcpyAsync(c->y, yv.data(), sizeof(double) * nused, hipMemcpyHostToDevice, c->s));
+        HIPCHK(c, launch_update_solution(c->s, N, nused, c->V, N, c->y, x));
+        if (done) break;
+        HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, x, b, c->aux, 1));
+        if ((rc = prec_apply(c, c->r, c->aux))) return rc;
+        if ((rc = dnorm(c, c->r, &nr))) return rc;
+        error = nr / bnrm2;
+        if (error <= c->tol) break;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    *ok = error <= c->tol;
+    *err_out = error;
+    return FP_OK;
+}
+
+extern "C" int fp_linear_solve(fp_ctx* c, int* iterations, double* rel_residual) {
+    if (!c) return FP_EINVAL;
+    if (!c->jac_valid) {
+        c->err = "fp_linear_solve before fp_assemble(with_jacobian=1)";
+        return FP_EINVAL;
+    }
+    int rc = prec_factor(c);
+    if (rc) return rc;
+    int st = 0;
+    if (c->precond == 2) {
+        HIPCHK(c, hipMemcpyAsync(&st, c->istat, sizeof(int), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        if (st) {
+            c->err = "singular diagonal block during block ILU";
+            return FP_SINGULAR_BLOCK;
+        }
+    }
+    HIPCHK(c, launch_fill(c->s, (size_t)c->dofs, 0.0, c->x));
+    int it = 0;
+    double err = 0;
+    bool ok = false;
+    rc = gmres(c, c->RHS, c->x, &it, &err, &ok);
+    if (rc) return rc;
+    if (iterations) *iterations = it;
+    if (rel_residual) *rel_residual = err;
+    HIPCHK(c, hipMemsetAsync(c->istat, 0, sizeof(int), c->s));
+    HIPCHK(c, launch_nonfinite(c->s, (size_t)c->dofs, c->x, c->istat));
+    HIPCHK(c, hipMemcpyAsync(&st, c->istat, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (st) return FP_NONFINITE;
+    return ok ? FP_OK : FP_NOT_CONVERGED;
+}
+
+extern "C" int fp_update(fp_ctx* c, int* n_limited) {
+    if (!c) return FP_EINVAL;
+    HIPCHK(c, launch_axpy(c->s, (size_t)c->dofs, 1.0, c->x, c->W));
+    int cnt = 0;
+    if (c->positivity) {
+        HIPCHK(c, hipMemsetAsync(c->istat + 1, 0, sizeof(int), c->s));
+        HIPCHK(c, launch_positivity(c->L, c->s, c->tb, c->ms, c->W, c->isref, c->istat + 1));
+        HIPCHK(c, hipMemcpyAsync(&cnt, c->istat + 1, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (n_limited) *n_limited = cnt;
+    return FP_OK;
+}
+
+// ============================================================== exports
+extern "C" int fp_export_residual(fp_ctx* c, double* R, double* RHS) {
+    if (!c) return FP_EINVAL;
+    int rc = 0;
+    if (R && (rc = from_device(c, c->R, R))) return rc;
+    if (RHS && (rc = from_device(c, c->RHS, RHS))) return rc;
+    return FP_OK;
+}
+
+extern "C" int fp_export_jacobian(fp_ctx* c, double* diag, double* neigh) {
+    if (!c || !c->jac_valid) return FP_EINVAL;
+    const size_t BB = (size_t)c->B * c->B;
+    std::vector<double> h((size_t)c->nnzb * BB);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->val, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    std::vector<int> diagpos(c->nElem);
+    HIPCHK(c, hipMemcpy(diagpos.data(), c->diagpos, sizeof(int) * c->nElem, hipMemcpyDeviceToHost));
+    for (int r = 0; r < c->nElem; r++) {
+        const size_t e = c->order_[r];
+        if (diag) std::memcpy(diag + e * BB, &h[(size_t)diagpos[r] * BB], BB * sizeof(double));
+        if (neigh)
+            for (int k = 0; k < 3; k++) {
+                double* dst = neigh + (e * 3 + k) * BB;
+                int blk = c->blk_own_nb[3 * r + k];
+                if (blk >= 0) std::memcpy(dst, &h[(size_t)blk * BB], BB * sizeof(double));
+                else std::memset(dst, 0, BB * sizeof(double));
+            }
+    }
+    return FP_OK;
+}
+
+extern "C" int fp_export_solution(fp_ctx* c, double* x) {
+    if (!c || !x) return FP_EINVAL;
+    HIPCHK(c, hipMemcpyAsync(x, c->x, (size_t)c->dofs * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return FP_OK;
+}
+
+extern "C" int fp_jv(fp_ctx* c, double a1, const double* v, double* out) {
+    if (!c || !v || !out) return FP_EINVAL;
+    int rc = to_device(c, v, c->aux);
+    if (rc) return rc;
+    HIPCHK(c, launch_jvp(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->aux, a1, c->w));
+    return from_device(c, c->w, out);
+}
+
+extern "C" int fp_jtv(fp_ctx* c, double a1, const double* v, double* out) {
+    if (!c || !v || !out) return FP_EINVAL;
+    int rc = to_device(c, v, c->aux);
+    if (rc) return rc;
+    HIPCHK(c, launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, nullptr, c->aux, c->w, 1));
+    return from_device(c, c->w, out);
+}
+
+// ============================================================== measurement
+extern "C" int fp_time_kernel(fp_ctx* c, int which, int reps, double* avg_ms) {
+    if (!c || !avg_ms || reps < 1) return FP_EINVAL;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    double a[3] = {c->last_a1 > 0 ? c->last_a1 : 1.0, -(c->last_a1 > 0 ? c->last_a1 : 1.0), 0.0};
+    int rc = FP_OK;
+    auto once = [&]() -> int {
+        switch (which) {
+            case 0:
+                HIPCHK(c, launch_residual(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R,
+                                          c->RHS, c->enorm));
+                return FP_OK;
+            case 1:
+                HIPCHK(c, launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a[0], c->val, nullptr, nullptr, 0));
+                c->jac_valid = true;
+                return FP_OK;
+            case 2:
+                HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, c->RHS, nullptr, c->aux, 0));
+                return FP_OK;
+            case 3:
+                return prec_factor(c);
+            case 4:
+                return prec_apply(c, c->aux, c->RHS);
+            case 5:
+                HIPCHK(c, launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a[0], nullptr, c->RHS, c->w, 1));
+                return FP_OK;
+        }
+        return FP_EINVAL;
+    };
+    if ((rc = once())) return rc;   // warm-up
+    HIPCHK(c, hipEventRecord(e0, c->s));
+    for (int i = 0; i < reps && rc == FP_OK; i++) rc = once();
+    HIPCHK(c, hipEventRecord(e1, c->s));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    return rc;
+}

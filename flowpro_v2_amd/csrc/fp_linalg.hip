@@ -1,0 +1,499 @@
+// Linear algebra of the Newton step on gfx950.
+//
+// Matrix: block-sparse rows (BSR) over elements, blocks of B x B stored
+// column-major (val[blk*B*B + c*B + r] = A[r][c]), which is the memory layout of
+// the reference's transposed element blocks ADiag[perturbed][residual]
+// (FlowPro/src/main/java/flowpro/core/element/Implicit.java:14-126), so that the
+// element Jacobian kernel writes one contiguous column per thread and the SpMV
+// reads the block coalesced, lane r <- row r.
+//
+// Block ILU(0) (BlockILU0.java:98-185): factor and sweeps in the DOF order of
+// the reference, parallelised over level sets of the block dependency graph
+// (rows of one level depend only on rows of earlier levels, so the arithmetic
+// of every row is the reference's).  Factors in FP64 or FP32 (TF).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fp {
+
+// ------------------------------------------------------------------ SpMV
+// y = A x (mode 0) or y = b - A x (mode 1); SparseMatrix.Mult / SubstrMult
+// (FlowPro/src/main/java/flowpro/core/LinearSolvers/SparseMatrix.java:177-204).
+__global__ void __launch_bounds__(256) k_bsr_spmv(int nrows, int B, const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                                  const double* __restrict__ val, const double* __restrict__ x,
+                                                  const double* __restrict__ b, double* __restrict__ y, int mode) {
+    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int r = (int)(g / B), t = (int)(g % B);
+    if (r >= nrows) return;
+    double s = mode ? b[g] : 0.0;
+    const size_t BB = (size_t)B * B;
+    for (int blk = rowptr[r]; blk < rowptr[r + 1]; blk++) {
+        const double* a = val + (size_t)blk * BB + t;
+        const double* xc = x + (size_t)colidx[blk] * B;
+        if (mode) {
+            for (int c = 0; c < B; c++) s -= a[(size_t)c * B] * xc[c];
+        } else {
+            for (int c = 0; c < B; c++) s += a[(size_t)c * B] * xc[c];
+        }
+    }
+    y[g] = s;
+}
+
+// ------------------------------------------------------------------ ILU copy
+template <typename TF>
+__global__ void k_ilu_copy(size_t n, const double* __restrict__ src, TF* __restrict__ dst) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = (TF)src[i];
+}
+
+// ------------------------------------------------------------------ ILU factor
+// One workgroup per block row of the level.  LDS: three B x B tiles.
+template <typename TF>
+__global__ void __launch_bounds__(256) k_ilu_factor_level(int B, const int* __restrict__ rows, int nrows,
+                                                          const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                                          const int* __restrict__ diagpos, TF* __restrict__ F,
+                                                          int* __restrict__ status) {
+    extern __shared__ unsigned char smem_raw[];
+    TF* As = reinterpret_cast<TF*>(smem_raw);
+    TF* Ds = As + B * B;
+    TF* Ls = Ds + B * B;
+    __shared__ int piv;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int BB = B * B;
+    if ((int)blockIdx.x >= nrows) return;
+    const int i = rows[blockIdx.x];
+    const int r0 = rowptr[i], r1 = rowptr[i + 1], di = diagpos[i];
+    for (int kk = r0; kk < di; kk++) {
+        const int k = colidx[kk];
+        const TF* Aik = F + (size_t)kk * BB;
+        const TF* Dk = F + (size_t)diagpos[k] * BB;   // holds D_k^-1 after row k was factored
+        for (int q = tid; q < BB; q += nt) { As[q] = Aik[q]; Ds[q] = Dk[q]; }
+        __syncthreads();
+        // L = A_ik * D_k^-1   (column-major: X[c*B + r])
+        for (int q = tid; q < BB; q += nt) {
+            int r = q % B, c = q / B;
+            TF s = 0;
+            for (int m = 0; m < B; m++) s += As[m * B + r] * Ds[c * B + m];
+            Ls[q] = s;
+        }
+        __syncthreads();
+        TF* Lg = F + (size_t)kk * BB;
+        for (int q = tid; q < BB; q += nt) Lg[q] = Ls[q];
+        // A_ij -= L * U_kj for the blocks j > k present in both rows
+        for (int jj = kk + 1; jj < r1; jj++) {
+            const int j = colidx[jj];
+            int kj = -1;
+            for (int s = rowptr[k]; s < rowptr[k + 1]; s++)
+                if (colidx[s] == j) kj = s;
+            if (kj < 0) continue;
+            const TF* U = F + (size_t)kj * BB;
+            for (int q = tid; q < BB; q += nt) As[q] = U[q];
+            __syncthreads();
+            TF* C = F + (size_t)jj * BB;
+            for (int q = tid; q < BB; q += nt) {
+                int r = q % B, c = q / B;
+                TF s = 0;
+                for (int m = 0; m < B; m++) s += Ls[m * B + r] * As[c * B + m];
+                C[q] -= s;
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+    }
+    // D_i^-1 by Gauss-Jordan elimination with partial pivoting (Mat.invert, API/Mat.java:605-642)
+    TF* M = As;
+    TF* Inv = Ds;
+    TF* Dg = F + (size_t)di * BB;
+    for (int q = tid; q < BB; q += nt) {
+        M[q] = Dg[q];
+        Inv[q] = (q % B == q / B) ? (TF)1 : (TF)0;
+    }
+    __syncthreads();
+    for (int p = 0; p < B; p++) {
+        if (tid == 0) {
+            int best = p;
+            TF bv = fabs(M[p * B + p]);
+            for (int r = p + 1; r < B; r++) {
+                TF v = fabs(M[p * B + r]);
+                if (v > bv) { bv = v; best = r; }
+            }
+            piv = best;
+        }
+        __syncthreads();
+        const int pr = piv;
+        if (pr != p)
+            for (int c = tid; c < B; c += nt) {
+                TF t1 = M[c * B + p]; M[c * B + p] = M[c * B + pr]; M[c * B + pr] = t1;
+                TF t2 = Inv[c * B + p]; Inv[c * B + p] = Inv[c * B + pr]; Inv[c * B + pr] = t2;
+            }
+        __syncthreads();
+        const TF dinv = (TF)1 / M[p * B + p];
+        __syncthreads();
+        for (int c = tid; c < B; c += nt) { M[c * B + p] *= dinv; Inv[c * B + p] *= dinv; }
+        __syncthreads();
+        // eliminate column p from all other rows; Ls keeps the multipliers
+        for (int r = tid; r < B; r += nt) Ls[r] = (r == p) ? (TF)0 : M[p * B + r];
+        __syncthreads();
+        for (int q = tid; q < BB; q += nt) {
+            int r = q % B, c = q / B;
+            TF f = Ls[r];
+            if (f != (TF)0) {
+                M[q] -= f * M[c * B + p];
+                Inv[q] -= f * Inv[c * B + p];
+            }
+        }
+        __syncthreads();
+    }
+    int bad = 0;
+    for (int q = tid; q < BB; q += nt) {
+        TF v = Inv[q];
+        if (!isfinite((double)v)) bad = 1;
+        Dg[q] = v;
+    }
+    if (bad) atomicOr(status, 1);
+}
+
+// ------------------------------------------------------------------ ILU sweeps
+// forward: y_i = b_i - sum_{k<i} L_ik y_k ; one 64-lane wave per row of the level
+template <typename TF>
+__global__ void __launch_bounds__(64) k_ilu_fwd_level(int B, const int* __restrict__ rows, int nrows,
+                                                      const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                                      const int* __restrict__ diagpos, const TF* __restrict__ F,
+                                                      const double* __restrict__ b, TF* __restrict__ y) {
+    if ((int)blockIdx.x >= nrows) return;
+    const int i = rows[blockIdx.x];
+    const size_t BB = (size_t)B * B;
+    for (int t = threadIdx.x; t < B; t += blockDim.x) {
+        TF s = (TF)b[(size_t)i * B + t];
+        for (int kk = rowptr[i]; kk < diagpos[i]; kk++) {
+            const TF* L = F + (size_t)kk * BB + t;
+            const TF* yk = y + (size_t)colidx[kk] * B;
+            TF sum = 0;
+            for (int c = 0; c < B; c++) sum += L[(size_t)c * B] * yk[c];
+            s -= sum;
+        }
+        y[(size_t)i * B + t] = s;
+    }
+}
+
+// backward: x_i = D_i^-1 (y_i - sum_{j>i} U_ij x_j)
+template <typename TF>
+__global__ void __launch_bounds__(64) k_ilu_bwd_level(int B, const int* __restrict__ rows, int nrows,
+                                                      const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                                      const int* __restrict__ diagpos, const TF* __restrict__ F,
+                                                      const TF* __restrict__ y, TF* __restrict__ xs, double* __restrict__ x) {
+    extern __shared__ unsigned char smem_raw[];
+    TF* rhs = reinterpret_cast<TF*>(smem_raw);
+    if ((int)blockIdx.x >= nrows) return;
+    const int i = rows[blockIdx.x];
+    const size_t BB = (size_t)B * B;
+    for (int t = threadIdx.x; t < B; t += blockDim.x) {
+        TF s = y[(size_t)i * B + t];
+        for (int jj = diagpos[i] + 1; jj < rowptr[i + 1]; jj++) {
+            const TF* U = F + (size_t)jj * BB + t;
+            const TF* xj = xs + (size_t)colidx[jj] * B;
+            TF sum = 0;
+            for (int c = 0; c < B; c++) sum += U[(size_t)c * B] * xj[c];
+            s -= sum;
+        }
+        rhs[t] = s;
+    }
+    __syncthreads();
+    const TF* D = F + (size_t)diagpos[i] * BB;
+    for (int t = threadIdx.x; t < B; t += blockDim.x) {
+        TF s = 0;
+        for (int c = 0; c < B; c++) s += D[(size_t)c * B + t] * rhs[c];
+        xs[(size_t)i * B + t] = s;
+        x[(size_t)i * B + t] = (double)s;
+    }
+}
+
+// ------------------------------------------------------------------ block Jacobi
+// blockJacobi.apply with Mat.lsolveT (FlowPro/src/main/java/flowpro/core/LinearSolvers/
+// preconditioners/blockJacobi.java:74-90, API/Mat.java:1307-1329): unpivoted elimination
+// on the transposed block, in place in the working copy `work` (kept across the applies
+// of one linear solve, as the reference overwrites ADiag), then back substitution.
+// One workgroup per element.  A[p][i] of the reference = work[blk*B*B + p*B + i].
+__global__ void __launch_bounds__(128) k_block_jacobi(int nElem, int B, double* __restrict__ work, const double* __restrict__ b,
+                                                      double* __restrict__ x) {
+    extern __shared__ unsigned char smem_raw[];
+    double* bs = reinterpret_cast<double*>(smem_raw);
+    double* xs = bs + B;
+    const int e = blockIdx.x;
+    if (e >= nElem) return;
+    double* A = work + (size_t)e * B * B;
+    for (int t = threadIdx.x; t < B; t += blockDim.x) bs[t] = b[(size_t)e * B + t];
+    __syncthreads();
+    for (int p = 0; p < B; p++) {
+        const double app = A[(size_t)p * B + p];
+        const double bp = bs[p];
+        for (int i = p + 1 + threadIdx.x; i < B; i += blockDim.x) {
+            double alpha = A[(size_t)p * B + i] / app;
+            bs[i] -= alpha * bp;
+            for (int j = p; j < B; j++) A[(size_t)j * B + i] -= alpha * A[(size_t)j * B + p];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int i = B - 1; i >= 0; i--) {
+            double sum = 0.0;
+            for (int j = i + 1; j < B; j++) sum += A[(size_t)j * B + i] * xs[j];
+            xs[i] = (bs[i] - sum) / A[(size_t)i * B + i];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < B; t += blockDim.x) x[(size_t)e * B + t] = xs[t];
+}
+
+// ------------------------------------------------------------------ vector kernels
+// Deterministic two-stage reductions: fixed grid, block tree, then one block.
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+
+__device__ inline double block_sum(double v) {
+    __shared__ double sh[RED_THREADS];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(RED_THREADS) k_dot_partial(size_t n, const double* __restrict__ a, const double* __restrict__ b,
+                                                             double* __restrict__ partial) {
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) s += a[i] * b[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// out = sum(partial) (op 0), sqrt(sum) (op 1)
+__global__ void __launch_bounds__(RED_THREADS) k_reduce_final(int np, const double* __restrict__ partial, double* __restrict__ out,
+                                                              int op) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s += partial[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) *out = op == 1 ? sqrt(s) : s;
+}
+
+// w -= h * v ; partial dot(w, u) (u may be null)
+__global__ void __launch_bounds__(RED_THREADS) k_axpy_dot(size_t n, double* __restrict__ w, const double* __restrict__ v,
+                                                          const double* __restrict__ h, const double* __restrict__ u,
+                                                          double* __restrict__ partial) {
+    const double hv = *h;
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double wi = w[i] - v[i] * hv;
+        w[i] = wi;
+        if (u) s += wi * u[i];
+        else s += wi * wi;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// dst = (nrm > 0) ? src / nrm : 0   (Gmres.java:82-85, happy breakdown)
+__global__ void k_scale_div(size_t n, double* __restrict__ dst, const double* __restrict__ src, const double* __restrict__ nrm) {
+    const double h = *nrm;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = h > 0 ? src[i] / h : 0.0;
+}
+
+// x += sum_k V_k y_k (Gmres.updateSolution)
+__global__ void k_update_solution(size_t n, int nv, const double* __restrict__ V, size_t ldv, const double* __restrict__ y,
+                                  double* __restrict__ x) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double xi = x[i];
+        for (int k = 0; k < nv; k++) xi += V[(size_t)k * ldv + i] * y[k];
+        x[i] = xi;
+    }
+}
+
+__global__ void k_axpy(size_t n, double a, const double* __restrict__ x, double* __restrict__ y) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] += a * x[i];
+}
+
+__global__ void k_fill(size_t n, double v, double* __restrict__ y) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = v;
+}
+
+__global__ void k_copy(size_t n, const double* __restrict__ x, double* __restrict__ y) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = x[i];
+}
+
+// non-finite check of a vector (LinearSolver.java:105-110)
+__global__ void k_nonfinite(size_t n, const double* __restrict__ x, int* __restrict__ flag) {
+    int bad = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        if (!isfinite(x[i])) bad = 1;
+    if (bad) atomicOr(flag, 1);
+}
+
+// ------------------------------------------------------------------ host launchers
+static inline unsigned grid_for(size_t n, int t) {
+    size_t g = (n + t - 1) / t;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+hipError_t launch_spmv(hipStream_t s, int nrows, int B, const int* rowptr, const int* colidx, const double* val,
+                       const double* x, const double* b, double* y, int mode) {
+    long long n = (long long)nrows * B;
+    hipLaunchKernelGGL(k_bsr_spmv, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nrows, B, rowptr, colidx, val, x, b, y,
+                       mode);
+    return hipGetLastError();
+}
+
+template <typename TF>
+hipError_t launch_ilu_copy(hipStream_t s, size_t n, const double* src, TF* dst) {
+    hipLaunchKernelGGL(k_ilu_copy<TF>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, src, dst);
+    return hipGetLastError();
+}
+
+template <typename TF>
+hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* colidx,
+                                   const int* diagpos, TF* F, int* status) {
+    size_t sm = 3 * (size_t)B * B * sizeof(TF);
+    hipLaunchKernelGGL(k_ilu_factor_level<TF>, dim3(nrows), dim3(256), sm, s, B, rows, nrows, rowptr, colidx, diagpos, F, status);
+    return hipGetLastError();
+}
+
+template <typename TF>
+hipError_t launch_ilu_fwd_level(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* colidx,
+                                const int* diagpos, const TF* F, const double* b, TF* y) {
+    hipLaunchKernelGGL(k_ilu_fwd_level<TF>, dim3(nrows), dim3(64), 0, s, B, rows, nrows, rowptr, colidx, diagpos, F, b, y);
+    return hipGetLastError();
+}
+
+template <typename TF>
+hipError_t launch_ilu_bwd_level(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* colidx,
+                                const int* diagpos, const TF* F, const TF* y, TF* xs, double* x) {
+    hipLaunchKernelGGL(k_ilu_bwd_level<TF>, dim3(nrows), dim3(64), B * sizeof(TF), s, B, rows, nrows, rowptr, colidx, diagpos, F,
+                       y, xs, x);
+    return hipGetLastError();
+}
+
+#define FP_ILU_INST(TF)                                                                                                       \
+    template hipError_t launch_ilu_copy<TF>(hipStream_t, size_t, const double*, TF*);                                          \
+    template hipError_t launch_ilu_factor_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, \
+                                                    int*);                                                                     \
+    template hipError_t launch_ilu_fwd_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*,        \
+                                                 const TF*, const double*, TF*);                                               \
+    template hipError_t launch_ilu_bwd_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*,        \
+                                                 const TF*, const TF*, TF*, double*);
+FP_ILU_INST(double)
+FP_ILU_INST(float)
+
+hipError_t launch_block_jacobi(hipStream_t s, int nElem, int B, double* work, const double* b, double* x) {
+    hipLaunchKernelGGL(k_block_jacobi, dim3(nElem), dim3(128), 2 * B * sizeof(double), s, nElem, B, work, b, x);
+    return hipGetLastError();
+}
+
+// work[e] = val[blkd[e]] (the diagonal blocks, copied before the in-place eliminations)
+__global__ void k_gather_diag(int nElem, int BB, const int* __restrict__ blkd, const double* __restrict__ val, double* __restrict__ work) {
+    size_t n = (size_t)nElem * BB;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        size_t e = i / BB, q = i % BB;
+        work[i] = val[(size_t)blkd[e] * BB + q];
+    }
+}
+hipError_t launch_gather_diag(hipStream_t s, int nElem, int BB, const int* blkd, const double* val, double* work) {
+    hipLaunchKernelGGL(k_gather_diag, dim3(grid_for((size_t)nElem * BB, 256)), dim3(256), 0, s, nElem, BB, blkd, val, work);
+    return hipGetLastError();
+}
+
+// deterministic minimum (State.getTimeStep min-reduction)
+__device__ inline double block_min(double v) {
+    __shared__ double sh[RED_THREADS];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+__global__ void __launch_bounds__(RED_THREADS) k_min_partial(size_t n, const double* __restrict__ a, double* __restrict__ partial) {
+    double s = __builtin_huge_val();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) s = fmin(s, a[i]);
+    s = block_min(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+__global__ void __launch_bounds__(RED_THREADS) k_min_final(int np, const double* __restrict__ partial, double* __restrict__ out) {
+    double s = __builtin_huge_val();
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s = fmin(s, partial[i]);
+    s = block_min(s);
+    if (threadIdx.x == 0) *out = s;
+}
+hipError_t launch_min(hipStream_t s, size_t n, const double* a, double* partial, double* out) {
+    hipLaunchKernelGGL(k_min_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, n, a, partial);
+    hipLaunchKernelGGL(k_min_final, dim3(1), dim3(RED_THREADS), 0, s, RED_BLOCKS, partial, out);
+    return hipGetLastError();
+}
+
+// sum of per-element norms: out[c] = sum_e en[e*nc + c] (deterministic, one block per component)
+__global__ void __launch_bounds__(RED_THREADS) k_colsum(int nElem, int nc, const double* __restrict__ en, double* __restrict__ out) {
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int e = threadIdx.x; e < nElem; e += blockDim.x) s += en[(size_t)e * nc + c];
+    s = block_sum(s);
+    if (threadIdx.x == 0) out[c] = s;
+}
+hipError_t launch_colsum(hipStream_t s, int nElem, int nc, const double* en, double* out) {
+    hipLaunchKernelGGL(k_colsum, dim3(nc), dim3(RED_THREADS), 0, s, nElem, nc, en, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_dot(hipStream_t s, size_t n, const double* a, const double* b, double* partial, double* out, int op) {
+    hipLaunchKernelGGL(k_dot_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, n, a, b, partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, s, RED_BLOCKS, partial, out, op);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy_dot(hipStream_t s, size_t n, double* w, const double* v, const double* h, const double* u,
+                           double* partial, double* out, int op) {
+    hipLaunchKernelGGL(k_axpy_dot, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, n, w, v, h, u, partial);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, s, RED_BLOCKS, partial, out, op);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_div(hipStream_t s, size_t n, double* dst, const double* src, const double* nrm) {
+    hipLaunchKernelGGL(k_scale_div, dim3(grid_for(n, 256)), dim3(256), 0, s, n, dst, src, nrm);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_solution(hipStream_t s, size_t n, int nv, const double* V, size_t ldv, const double* y, double* x) {
+    hipLaunchKernelGGL(k_update_solution, dim3(grid_for(n, 256)), dim3(256), 0, s, n, nv, V, ldv, y, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy(hipStream_t s, size_t n, double a, const double* x, double* y) {
+    hipLaunchKernelGGL(k_axpy, dim3(grid_for(n, 256)), dim3(256), 0, s, n, a, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(hipStream_t s, size_t n, double v, double* y) {
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, n, v, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(hipStream_t s, size_t n, const double* x, double* y) {
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256)), dim3(256), 0, s, n, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_nonfinite(hipStream_t s, size_t n, const double* x, int* flag) {
+    hipLaunchKernelGGL(k_nonfinite, dim3(grid_for(n, 256)), dim3(256), 0, s, n, x, flag);
+    return hipGetLastError();
+}
+
+}  // namespace fp

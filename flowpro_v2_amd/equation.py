@@ -1,0 +1,85 @@
+"""Model plugins behind the reference's `Equation` contract (FlowProAPI/src/main/java/
+flowpro/api/Equation.java:5-128), restricted to what the device path needs: the
+derived constants of `init` and the constant initial condition.
+
+NavierStokes      ModuleProjects/Equations/Aerodynamics/src/main/java/flowpro/user/equation/
+                  Aerodynamics.java:119-312 (init), NavierStokes.java:18-21, 101-138
+TestScalarEquation FlowPro/src/test/java/flowpro/user/equation/TestScalarEquation.java:28-37
+Any other model name is unsupported on the device and runs on the reference's Java path.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+MODEL_IDS = {"NavierStokes": 0, "TestScalarEquation": 2}
+
+
+@dataclass
+class ModelConstants:
+    model: str
+    model_id: int
+    n_eqs: int
+    dim: int
+    diffusive: bool
+    kappa: float = 1.4
+    cv: float = 1.0
+    p_ref: float = 1.0
+    rho_ref: float = 1.0
+    velocity_ref: float = 1.0
+    p_out: float = 1.0        # normalised by p_ref
+    reynolds: float = -1.0    # after the rescaling of Aerodynamics.init
+    prandtl: float = -1.0
+    attack: float | None = None
+    a: tuple = (1.0, 0.5)
+    k: float = 0.0
+
+
+class UnsupportedModel(ValueError):
+    pass
+
+
+def init(props: dict) -> ModelConstants:
+    """Equation.init for the supported models (parameters.txt keys)."""
+    model = props.get("model", "NavierStokes")
+    if model == "TestScalarEquation":
+        a = tuple(float(v) for v in props.get("a", (1.0, 0.5)))
+        k = float(props.get("k", 0.0))
+        return ModelConstants(model, 2, 1, 2, k > 0, a=a, k=k)
+    if model != "NavierStokes":
+        raise UnsupportedModel(model)
+    if int(props.get("dimension", 2)) != 2:
+        raise UnsupportedModel("NavierStokes in 3D")
+    kappa, cv = float(props["kappa"]), float(props["cv"])
+    # subsonic stagnation inlet: pIn0, rhoIn0, pOut (Aerodynamics.java:169-182)
+    p_ref, rho_ref = float(props["pIn0"]), float(props["rhoIn0"])
+    p_out = float(props["pOut"]) / p_ref
+    velocity_ref = math.sqrt(p_ref / rho_ref)
+    diffusive = bool(props.get("isFlowViscous", False))
+    re = pr = -1.0
+    if diffusive:
+        # far-field Reynolds number rescaled to the reference values (Aerodynamics.java:262-281)
+        pin0 = 1.0
+        mach_inf = math.sqrt(2 / (kappa - 1) * (math.pow(pin0 / p_out, (kappa - 1) / kappa) - 1))
+        base = 1 + (kappa - 1) / 2 * mach_inf * mach_inf
+        rho_inf = rho_ref / math.pow(base, 1 / (kappa - 1))
+        p_inf = p_ref / math.pow(base, kappa / (kappa - 1))
+        u_inf = mach_inf * math.sqrt(kappa * p_inf / rho_inf)
+        re = float(props["reynolds"]) * (rho_ref * velocity_ref) / (rho_inf * u_inf)
+        pr = float(props["prandtl"])
+    attack = float(props["attackAngle"]) if "attackAngle" in props else None
+    return ModelConstants(model, 0, 4, 2, diffusive, kappa=kappa, cv=cv, p_ref=p_ref, rho_ref=rho_ref,
+                          velocity_ref=velocity_ref, p_out=p_out, reynolds=re, prandtl=pr, attack=attack)
+
+
+def const_init_condition(mc: ModelConstants) -> list[float]:
+    """NavierStokes.constInitCondition, subsonic inlet branch (NavierStokes.java:105-137)."""
+    if mc.model_id == 2:
+        return [0.0]
+    kappa = mc.kappa
+    mach_in = math.sqrt(2 / (kappa - 1) * (math.pow(1 / mc.p_out, (kappa - 1) / kappa) - 1))
+    rho_in = math.pow(1 + ((kappa - 1) / 2) * mach_in * mach_in, 1 / (1 - kappa))
+    v_in = mach_in * math.sqrt((kappa * mc.p_out) / rho_in)
+    alpha = mc.attack or 0.0
+    return [rho_in, rho_in * v_in * math.cos(alpha), rho_in * v_in * math.sin(alpha),
+            mc.p_out / (kappa - 1) + 0.5 * rho_in * v_in * v_in]

@@ -1,0 +1,120 @@
+"""ctypes binding of libflowpro_hip.so (include/flowpro_hip.h).
+
+The library must be built in-tree (``python flowpro_v2_amd/build.py``); there is
+no fallback: importing the binding without the library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflowpro_hip.so")
+
+FP_OK, FP_NOT_CONVERGED, FP_SINGULAR_BLOCK, FP_NONFINITE, FP_UNSUPPORTED = 0, 1, 2, 3, 4
+FP_EINVAL, FP_EHIP, FP_ENOMEM = -1, -2, -3
+RECOVERABLE = {FP_NOT_CONVERGED: "GMRES did not converge", FP_SINGULAR_BLOCK: "singular diagonal block",
+               FP_NONFINITE: "non-finite solution"}
+
+# every symbol declared in include/flowpro_hip.h
+EXPORTS = ["fp_neighbor_matrix", "fp_rcm_order", "fp_create", "fp_destroy", "fp_last_error", "fp_info",
+           "fp_element_order", "fp_set_state", "fp_get_state", "fp_next_level", "fp_restore_level", "fp_time_step",
+           "fp_assemble", "fp_linear_solve", "fp_update", "fp_export_residual", "fp_export_jacobian",
+           "fp_export_solution", "fp_jv", "fp_jtv", "fp_time_kernel", "fp_stream"]
+
+
+class FpCase(C.Structure):
+    _fields_ = [
+        ("n_points", C.c_int), ("pxy", C.POINTER(C.c_double)),
+        ("n_elems", C.c_int), ("tp", C.POINTER(C.c_int)),
+        ("n_bnd", C.c_int), ("bnd", C.POINTER(C.c_int)),
+        ("wall_distance", C.POINTER(C.c_double)),
+        ("order", C.c_int),
+        ("n_vol", C.c_int), ("vol_xi", C.POINTER(C.c_double)), ("vol_w", C.POINTER(C.c_double)),
+        ("n_face", C.c_int), ("face_xi", C.POINTER(C.c_double)), ("face_w", C.POINTER(C.c_double)),
+        ("model", C.c_int), ("viscous", C.c_int),
+        ("kappa", C.c_double), ("p_out", C.c_double), ("reynolds", C.c_double), ("prandtl", C.c_double),
+        ("has_attack", C.c_int), ("attack", C.c_double),
+        ("scalar_a0", C.c_double), ("scalar_a1", C.c_double), ("scalar_k", C.c_double),
+        ("beta0", C.c_double), ("penalty", C.c_double),
+        ("ordering", C.c_int), ("preconditioner", C.c_int), ("precond_bits", C.c_int),
+        ("gmres_restart", C.c_int), ("gmres_cycles", C.c_int), ("gmres_tol", C.c_double),
+        ("positivity_limiter", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the in-tree library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python flowpro_v2_amd/build.py`")
+    L = C.CDLL(LIB_PATH)
+    vp, ip, dp, lp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_longlong)
+    sig = {
+        "fp_neighbor_matrix": (C.c_int, [C.c_int, ip, C.c_int, ip, ip]),
+        "fp_rcm_order": (C.c_int, [C.c_int, ip, ip]),
+        "fp_create": (C.c_int, [C.POINTER(FpCase), C.POINTER(vp)]),
+        "fp_destroy": (C.c_int, [vp]),
+        "fp_last_error": (C.c_char_p, [vp]),
+        "fp_info": (C.c_int, [vp, lp]),
+        "fp_element_order": (C.c_int, [vp, ip]),
+        "fp_set_state": (C.c_int, [vp, dp, dp, dp]),
+        "fp_get_state": (C.c_int, [vp, dp]),
+        "fp_next_level": (C.c_int, [vp]),
+        "fp_restore_level": (C.c_int, [vp]),
+        "fp_time_step": (C.c_int, [vp, dp]),
+        "fp_assemble": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp]),
+        "fp_linear_solve": (C.c_int, [vp, ip, dp]),
+        "fp_update": (C.c_int, [vp, ip]),
+        "fp_export_residual": (C.c_int, [vp, dp, dp]),
+        "fp_export_jacobian": (C.c_int, [vp, dp, dp]),
+        "fp_export_solution": (C.c_int, [vp, dp]),
+        "fp_jv": (C.c_int, [vp, C.c_double, dp, dp]),
+        "fp_jtv": (C.c_int, [vp, C.c_double, dp, dp]),
+        "fp_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, dp]),
+        "fp_stream": (vp, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class FatalError(RuntimeError):
+    """A fatal (< 0) return code: bad arguments or a HIP error."""
+
+
+def neighbor_matrix(TP: np.ndarray, bnd: np.ndarray) -> np.ndarray:
+    TP = np.ascontiguousarray(TP, dtype=np.int32)
+    bnd = np.ascontiguousarray(bnd, dtype=np.int32)
+    TT = np.zeros_like(TP)
+    rc = lib().fp_neighbor_matrix(TP.shape[0], iptr(TP), bnd.shape[0], iptr(bnd), iptr(TT))
+    if rc:
+        raise FatalError(f"fp_neighbor_matrix: {rc}")
+    return TT
+
+
+def rcm_order(TT: np.ndarray) -> np.ndarray:
+    TT = np.ascontiguousarray(TT, dtype=np.int32)
+    order = np.zeros(TT.shape[0], np.int32)
+    rc = lib().fp_rcm_order(TT.shape[0], iptr(TT), iptr(order))
+    if rc:
+        raise FatalError(f"fp_rcm_order: {rc}")
+    return order

@@ -1,0 +1,355 @@
+"""Host mirror of the reference's implicit solver around the device Newton step.
+
+`DeviceCase` owns one `fp_ctx` (include/flowpro_hip.h).  `GpuImplicitSolver.solve`
+restates the time-step loop of LocalImplicitSolver.solve
+(FlowPro/src/main/java/flowpro/core/solver/LocalImplicitSolver.java:107-285) with
+State (FlowPro/src/main/java/flowpro/core/State.java:145-240) and CFLSetup
+(FlowPro/src/main/java/flowpro/core/solver/CFLSetup.java:34-63); every Newton step
+(assembly, preconditioned GMRES, update) runs on the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import equation, native, quadrature
+from .cases import Case
+
+PRECONDITIONERS = {"none": 0, "blockjacobi": 1, "blockilu0": 2}
+MAX_VALUE = 1.7976931348623157e308
+
+
+class UnsupportedCase(RuntimeError):
+    """The device path has no kernel for this case (the reference's Java path applies)."""
+
+
+class DeviceCase:
+    """One case on the GPU: MasterSolver.factory + Mesh + LinearSolver.factory for the device."""
+
+    def __init__(self, case: Case, props: dict, precond_bits: int = 64):
+        L = native.lib()
+        if props.get("spatialMethod", "DGpure") not in ("DGpure", "DG"):
+            raise UnsupportedCase("spatial method")
+        if props.get("movingMesh", False) or props.get("curvedBoundary", False):
+            raise UnsupportedCase("moving or curved mesh")
+        if float(props.get("dampTol", 0.0)) > 0 or float(props.get("dampConst", 0.0)) != 0:
+            raise UnsupportedCase("artificial viscosity (dampTol, dampConst)")
+        prec = props.get("preconditioner", "blockilu0")
+        if prec not in PRECONDITIONERS:
+            raise UnsupportedCase(f"preconditioner {prec}")
+        self.mc = equation.init(props)
+        order = int(props["order"])
+        self.order = order
+        self.props = props
+        self.case = case
+        vxi, vw = quadrature.triangle_rule(order)
+        fxi, fw = quadrature.line_rule(order)
+        self._keep = [np.ascontiguousarray(case.PXY, np.float64), np.ascontiguousarray(case.TP, np.int32),
+                      np.ascontiguousarray(case.bnd, np.int32), np.ascontiguousarray(vxi), np.ascontiguousarray(vw),
+                      np.ascontiguousarray(fxi), np.ascontiguousarray(fw)]
+        pxy, tp, bnd, vxi, vw, fxi, fw = self._keep
+        wd = None
+        if case.wall_distance is not None:
+            wd = np.ascontiguousarray(case.wall_distance, np.float64)
+            self._keep.append(wd)
+        cs = native.FpCase()
+        cs.n_points, cs.pxy = pxy.shape[0], native.dptr(pxy)
+        cs.n_elems, cs.tp = tp.shape[0], native.iptr(tp)
+        cs.n_bnd, cs.bnd = bnd.shape[0], native.iptr(bnd)
+        cs.wall_distance = native.dptr(wd) if wd is not None else None
+        cs.order = order
+        cs.n_vol, cs.vol_xi, cs.vol_w = vw.shape[0], native.dptr(vxi), native.dptr(vw)
+        cs.n_face, cs.face_xi, cs.face_w = fw.shape[0], native.dptr(fxi), native.dptr(fw)
+        mc = self.mc
+        cs.model, cs.viscous = mc.model_id, int(mc.diffusive)
+        cs.kappa, cs.p_out, cs.reynolds, cs.prandtl = mc.kappa, mc.p_out, mc.reynolds, mc.prandtl
+        cs.has_attack, cs.attack = int(mc.attack is not None), float(mc.attack or 0.0)
+        cs.scalar_a0, cs.scalar_a1, cs.scalar_k = float(mc.a[0]), float(mc.a[1]), float(mc.k)
+        cs.beta0 = float(props.get("beta0", 2.0))
+        cs.penalty = float(props.get("penalty", 0.0))
+        cs.ordering = 1 if str(props.get("elementOrdering", "rcm")).lower() == "rcm" else 0
+        cs.preconditioner = PRECONDITIONERS[prec]
+        cs.precond_bits = int(precond_bits)
+        cs.gmres_restart, cs.gmres_cycles = 30, 5
+        cs.gmres_tol = float(props.get("iterativeSolverTol", 1e-2))
+        cs.positivity_limiter = int(bool(props.get("positivityLimiter", True)))
+        h = C.c_void_p()
+        rc = L.fp_create(C.byref(cs), C.byref(h))
+        if rc == native.FP_UNSUPPORTED:
+            raise UnsupportedCase("no device kernel for this model / order")
+        if rc != native.FP_OK:
+            raise native.FatalError(f"fp_create failed with {rc}")
+        self.h = h
+        info = np.zeros(6, np.int64)
+        L.fp_info(self.h, info.ctypes.data_as(C.POINTER(C.c_longlong)))
+        self.nb, self.neq, self.dofs = int(info[0]), int(info[1]), int(info[2])
+        self.ilu_levels = (int(info[3]), int(info[4]))
+        self.nnzb = int(info[5])
+        self.B = self.nb * self.neq
+        self.n_elem = case.n_elem
+        self.precond_bits = precond_bits
+
+    def close(self):
+        if getattr(self, "h", None):
+            native.lib().fp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            msg = native.lib().fp_last_error(self.h)
+            raise native.FatalError(f"{what}: {rc} ({msg.decode() if msg else ''})")
+        return rc
+
+    # ---- state
+    def set_state(self, W, Wo=None, Wo2=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64).reshape(self.n_elem, self.B)
+                for a in (W, Wo, Wo2)]
+        ptr = [native.dptr(a) if a is not None else None for a in arrs]
+        self._check(native.lib().fp_set_state(self.h, *ptr), "fp_set_state")
+
+    def get_state(self) -> np.ndarray:
+        out = np.zeros((self.n_elem, self.B))
+        self._check(native.lib().fp_get_state(self.h, native.dptr(out)), "fp_get_state")
+        return out
+
+    def next_level(self):
+        self._check(native.lib().fp_next_level(self.h), "fp_next_level")
+
+    def restore_level(self):
+        self._check(native.lib().fp_restore_level(self.h), "fp_restore_level")
+
+    # ---- Newton step
+    def size_over_lambda(self) -> float:
+        v = C.c_double()
+        self._check(native.lib().fp_time_step(self.h, C.byref(v)), "fp_time_step")
+        return v.value
+
+    def assemble(self, dt: float, dto: float, time_order: int = 1, with_jacobian: bool = True):
+        norms = np.zeros(self.neq)
+        rhs = C.c_double()
+        self._check(native.lib().fp_assemble(self.h, dt, dto, time_order, int(with_jacobian), native.dptr(norms),
+                                             C.byref(rhs)), "fp_assemble")
+        return norms, rhs.value
+
+    def linear_solve(self):
+        it, err = C.c_int(), C.c_double()
+        rc = self._check(native.lib().fp_linear_solve(self.h, C.byref(it), C.byref(err)), "fp_linear_solve")
+        return rc, it.value, err.value
+
+    def update(self) -> int:
+        n = C.c_int()
+        self._check(native.lib().fp_update(self.h, C.byref(n)), "fp_update")
+        return n.value
+
+    # ---- exports
+    def residual(self):
+        R = np.zeros((self.n_elem, self.B))
+        RHS = np.zeros((self.n_elem, self.B))
+        self._check(native.lib().fp_export_residual(self.h, native.dptr(R), native.dptr(RHS)), "fp_export_residual")
+        return R, RHS
+
+    def jacobian(self):
+        d = np.zeros((self.n_elem, self.B, self.B))
+        n = np.zeros((self.n_elem, 3, self.B, self.B))
+        self._check(native.lib().fp_export_jacobian(self.h, native.dptr(d), native.dptr(n)), "fp_export_jacobian")
+        return d, n
+
+    def solution(self) -> np.ndarray:
+        x = np.zeros(self.dofs)
+        self._check(native.lib().fp_export_solution(self.h, native.dptr(x)), "fp_export_solution")
+        return x
+
+    def element_order(self) -> np.ndarray:
+        o = np.zeros(self.n_elem, np.int32)
+        self._check(native.lib().fp_element_order(self.h, native.iptr(o)), "fp_element_order")
+        return o
+
+    def jv(self, a1: float, v):
+        v = np.ascontiguousarray(v, np.float64).reshape(self.n_elem, self.B)
+        out = np.zeros_like(v)
+        self._check(native.lib().fp_jv(self.h, a1, native.dptr(v), native.dptr(out)), "fp_jv")
+        return out
+
+    def jtv(self, a1: float, v):
+        v = np.ascontiguousarray(v, np.float64).reshape(self.n_elem, self.B)
+        out = np.zeros_like(v)
+        self._check(native.lib().fp_jtv(self.h, a1, native.dptr(v), native.dptr(out)), "fp_jtv")
+        return out
+
+    def time_kernel(self, which: int, reps: int = 10) -> float:
+        ms = C.c_double()
+        self._check(native.lib().fp_time_kernel(self.h, which, reps, C.byref(ms)), "fp_time_kernel")
+        return ms.value
+
+    def initial_state(self) -> np.ndarray:
+        w0 = equation.const_init_condition(self.mc)
+        return np.tile(np.repeat(np.asarray(w0), self.nb), (self.n_elem, 1))
+
+
+# ---------------------------------------------------------------- State / CFLSetup
+@dataclass
+class State:
+    """State.java: CFL control, time step and the normalised residual history."""
+    cfl: float
+    vary_cfl: bool = False
+    max_vary_cfl: float = 1e4
+    fixed_dt: float | None = None
+    steps: int = 0
+    t: float = 0.0
+    current_cfl: float = 0.0
+    residuum: float = MAX_VALUE
+    dt: float = 0.0
+    normalisation: list = field(default_factory=list)
+    n_norm_steps: int = 0
+    res_old: float = -1.0
+    equation_residuum: list = field(default_factory=list)
+
+    def __post_init__(self):
+        if self.fixed_dt is not None:
+            self.dt = self.fixed_dt
+
+    def update_cfl(self):                       # State.updateCFL + CFLSetup.getCFL
+        if self.fixed_dt is not None:
+            self.dt = min(2 * self.dt, self.fixed_dt)
+            return
+        if not self.vary_cfl:
+            self.current_cfl = min(self.current_cfl + self.cfl / 20, self.cfl)
+            return
+        known = 0 < self.residuum < MAX_VALUE
+        if self.current_cfl <= 0:
+            self.res_old = self.residuum if known else -1
+            self.current_cfl = self.cfl
+            return
+        factor = 1.0
+        if known and self.res_old > 0:
+            factor = max(1 / 2.0, min(self.res_old / self.residuum, 2.0))
+        self.res_old = self.residuum if known else -1
+        self.current_cfl = max(min(self.current_cfl, self.cfl), min(self.current_cfl * factor, self.max_vary_cfl))
+
+    def reduce_cfl(self):                       # State.reduceCFL + CFLSetup.reduceCFL
+        if self.fixed_dt is not None:
+            self.dt /= 2
+        else:
+            self.res_old = -1
+            self.current_cfl /= 1.5
+
+    def convergence_hopeless(self) -> bool:     # State.isConvergenceHopeless (maxCFL = par.cfl)
+        if self.fixed_dt is not None:
+            return self.dt < self.fixed_dt / 1024
+        return 0 < self.current_cfl < self.cfl / 20
+
+    def time_step(self, size_over_lambda: float) -> float:   # State.getTimeStep
+        if self.fixed_dt is not None:
+            self.current_cfl = self.dt / size_over_lambda
+        else:
+            self.dt = self.current_cfl * size_over_lambda
+        return self.dt
+
+    def set_residuum(self, norms):              # State.setResiduum, N_NORMALISATION_STEPS = 5
+        n = len(norms)
+        if len(self.normalisation) != n:
+            self.normalisation = [0.0] * n
+            self.n_norm_steps = 0
+        if self.n_norm_steps < 5:
+            self.normalisation = [max(a, b) for a, b in zip(self.normalisation, norms)]
+            self.n_norm_steps += 1
+        mx = max(self.normalisation)
+        self.equation_residuum = []
+        res = 0.0
+        for m in range(n):
+            nrm = self.normalisation[m] if self.normalisation[m] > 0 else mx
+            r = norms[m] / nrm if nrm > 0 else norms[m]
+            self.equation_residuum.append(r)
+            if math.isnan(r):
+                res = math.nan
+            elif not math.isnan(res):
+                res = max(res, r)
+        self.residuum = res
+
+
+@dataclass
+class StepRecord:
+    step: int
+    residuum: float
+    dt: float
+    cfl: float
+    newton: int
+    gmres_iters: list
+    limited: int
+
+
+class GpuImplicitSolver:
+    """LocalImplicitSolver.solve with the Newton step on the device."""
+
+    def __init__(self, dev: DeviceCase, props: dict):
+        self.dev = dev
+        self.props = props
+        if "dt" in props and "CFL" not in props:
+            self.state = State(cfl=0.0, fixed_dt=float(props["dt"]))
+        else:
+            cfl = float(props.get("CFL", 1.0))
+            self.state = State(cfl=1.0 if cfl <= 0 else cfl, vary_cfl=cfl <= 0,
+                               max_vary_cfl=float(props.get("maxCFL", 1e4)))
+        self.time_order = int(props.get("timeOrder", 1))
+        self.newton_iters = int(props.get("newtonIters", 1))
+        self.newton_tol = float(props.get("newtonIterTol", 1e-4))
+        self.stop_residuum = float(props.get("residuum", -1.0))
+        self.history: list[StepRecord] = []
+        self.newton_history: list[float] = []
+        self.linear_failures = 0
+
+    def solve(self, steps: int) -> list[StepRecord]:
+        st, dev = self.state, self.dev
+        dto = -1.0
+        converges = True
+        total = st.steps + steps
+        st.steps += 1
+        while st.steps <= total and st.residuum > self.stop_residuum:
+            if converges:
+                st.update_cfl()
+            if st.convergence_hopeless():
+                break
+            dt = st.time_step(dev.size_over_lambda())
+            rhs0 = 0.0
+            iters = []
+            limited = 0
+            newton_done = 0
+            converges = True
+            for s in range(self.newton_iters):
+                if s > 0:
+                    _, rhs = dev.assemble(dt, dto, self.time_order, with_jacobian=False)
+                    rel = rhs / rhs0 if rhs0 > 0 else 0.0
+                    self.newton_history.append(rel)
+                    if rel < self.newton_tol:
+                        break
+                norms, rhs = dev.assemble(dt, dto, self.time_order, with_jacobian=True)
+                if s == 0:
+                    st.set_residuum(list(norms))
+                    rhs0 = rhs
+                rc, it, _ = dev.linear_solve()
+                iters.append(it)
+                if rc != native.FP_OK:
+                    dev.restore_level()
+                    st.reduce_cfl()
+                    st.steps -= 1
+                    self.linear_failures += 1
+                    converges = False
+                    break
+                limited += dev.update()
+                newton_done += 1
+            if converges:
+                if not math.isfinite(st.residuum):
+                    break
+                st.t += dt
+                dto = dt
+                dev.next_level()
+                self.history.append(StepRecord(st.steps, st.residuum, dt, st.current_cfl, newton_done, iters, limited))
+            st.steps += 1
+        st.steps -= 1
+        return self.history

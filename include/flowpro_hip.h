@@ -1,0 +1,134 @@
+/*
+ * flowpro_hip.h -- C ABI of the MI355X implicit-DG Newton step for FlowPro.
+ *
+ * The library replaces, for one case, the body of the Newton loop of
+ * LocalImplicitSolver.solve (FlowPro/src/main/java/flowpro/core/solver/
+ * LocalImplicitSolver.java:107-285): residual and element-Jacobian assembly
+ * (JacobiAssembler.assemble -> BDF1/BDF2.assembleJacobiMatrix -> DGpure.residuum),
+ * the preconditioned GMRES (LinearSolver.solve -> Gmres.solve with BlockILU0 or
+ * blockJacobi) and the update W += x with the positivity limiter.  The caller
+ * keeps the time-step loop, CFL control and residual bookkeeping (State).
+ *
+ * Plain C types only.  Arrays passed in are copied; the library never keeps a
+ * caller pointer.  Element-wise arrays use the reference's layout: element-major
+ * in mesh order, W[e][m*Nb + j] (Element.java:119-121).  Vectors of the linear
+ * system use the reference's DOF numbering (gIndex, Element.setGlobalIndex).
+ *
+ * Return codes: 0 success; > 0 recoverable (the reference's "converges = false"
+ * branch: restore Wo, reduce the CFL, repeat the step); < 0 fatal.
+ */
+#ifndef FLOWPRO_HIP_H
+#define FLOWPRO_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FP_OK = 0,
+    FP_NOT_CONVERGED = 1,   /* Gmres.solve returned false (Gmres.java:121) */
+    FP_SINGULAR_BLOCK = 2,  /* BlockILU0.factor ArithmeticException (BlockILU0.java:119-127) */
+    FP_NONFINITE = 3,       /* non-finite x (LinearSolver.java:105-110) */
+    FP_UNSUPPORTED = 4,     /* no kernel for this model / order / feature: use the Java path */
+    FP_EINVAL = -1,
+    FP_EHIP = -2,
+    FP_ENOMEM = -3
+};
+
+typedef struct fp_ctx fp_ctx;
+
+/* Case description: the inputs of FlowProMain.solverFactory and Mesh
+ * (FlowProMain.java:486-730, Mesh.java:69-224) for straight triangles of one order. */
+typedef struct fp_case {
+    int n_points;
+    const double* pxy;            /* [n_points][2] */
+    int n_elems;
+    const int* tp;                /* [n_elems][3], counter-clockwise */
+    int n_bnd;
+    const int* bnd;               /* [n_bnd][3]: boundary type, v0, v1 (boundaryType.txt) */
+    const double* wall_distance;  /* [n_points] or NULL */
+    int order;                    /* element order k = p + 1 */
+    int n_vol;                    /* volume rule (QuadratureCentral: degree 2k-1, positive weights) */
+    const double* vol_xi;         /* [n_vol][2] reference-triangle coordinates */
+    const double* vol_w;          /* [n_vol] */
+    int n_face;                   /* Gauss-Legendre rule of order k on [-1, 1] */
+    const double* face_xi;        /* [n_face] */
+    const double* face_w;         /* [n_face] */
+    int model;                    /* 0 NavierStokes (Euler when viscous == 0), 2 TestScalarEquation */
+    int viscous;
+    double kappa;                 /* Aerodynamics.init derived constants */
+    double p_out;                 /* outlet pressure / pRef */
+    double reynolds;              /* Re after the reference's rescaling (Aerodynamics.java:262-281) */
+    double prandtl;
+    int has_attack;
+    double attack;
+    double scalar_a0, scalar_a1, scalar_k;  /* TestScalarEquation a[0], a[1], k */
+    double beta0;                 /* DDG constant (DGpure, default 2) */
+    double penalty;               /* interior penalty (DGpure, default 0) */
+    int ordering;                 /* 0 none, 1 rcm (ElementOrdering) */
+    int preconditioner;           /* 0 none, 1 blockjacobi, 2 blockilu0 */
+    int precond_bits;             /* 64 or 32: precision of the preconditioner factors and sweeps */
+    int gmres_restart;            /* 30 */
+    int gmres_cycles;             /* 5 */
+    double gmres_tol;             /* iterativeSolverTol */
+    int positivity_limiter;       /* Parameters.positivityLimiter */
+} fp_case;
+
+/* --- integer setup (bit-exact with the reference) ------------------------------ */
+/* FlowProMain.createNeighborMatrix (FlowProMain.java:772-873): tt[n_elems][3] */
+int fp_neighbor_matrix(int n_elems, const int* tp, int n_bnd, const int* bnd, int* tt);
+/* ElementOrdering.reverseCuthillMcKee (ElementOrdering.java:52-142): order[n_elems] */
+int fp_rcm_order(int n_elems, const int* tt, int* order);
+
+/* --- context --------------------------------------------------------------------- */
+/* Mesh constructor + Mesh.init + LinearSolver.factory (Mesh.java:69-224, LinearSolver.java:35-89) */
+int fp_create(const fp_case* c, fp_ctx** out);
+int fp_destroy(fp_ctx* ctx);
+const char* fp_last_error(const fp_ctx* ctx);
+/* info: [0] Nb, [1] nEqs, [2] dofs, [3] forward ILU levels, [4] backward ILU levels, [5] block nonzeros */
+int fp_info(const fp_ctx* ctx, long long* info);
+/* the element order of the DOF numbering: order[r] = mesh element at position r */
+int fp_element_order(const fp_ctx* ctx, int* order);
+
+/* --- state (Element.W, Wo, Wo2) ---------------------------------------------------- */
+int fp_set_state(fp_ctx* ctx, const double* W, const double* Wo, const double* Wo2);
+int fp_get_state(fp_ctx* ctx, double* W);
+/* copyW2Wo (Element.java:479-482) and copyWo2W (Element.java:485-487) */
+int fp_next_level(fp_ctx* ctx);
+int fp_restore_level(fp_ctx* ctx);
+
+/* --- Newton step --------------------------------------------------------------------- */
+/* State.getTimeStep without the CFL factor (State.java:177-192): min_e elemSize / lambda_max */
+int fp_time_step(fp_ctx* ctx, double* min_size_over_lambda);
+/* BDF1/BDF2.assembleJacobiMatrix for all elements (JacobiAssembler.assemble) or only the
+ * right-hand side (assembleResiduum) when with_jacobian == 0.  dto <= 0: first step (BDF1).
+ * eq_norms[nEqs] = MasterSolver.residuumNorms, rhs_norm = LocalImplicitSolver.rhsNorm. */
+int fp_assemble(fp_ctx* ctx, double dt, double dto, int time_order, int with_jacobian, double* eq_norms,
+                double* rhs_norm);
+/* LinearSolver.solve: factor the preconditioner, GMRES from x = 0 */
+int fp_linear_solve(fp_ctx* ctx, int* iterations, double* rel_residual);
+/* Element.updateW for all elements, then Element.positivityLimiter (Element.java:334-338, 385-476) */
+int fp_update(fp_ctx* ctx, int* n_limited);
+
+/* --- parity and adjoint ---------------------------------------------------------------- */
+int fp_export_residual(fp_ctx* ctx, double* R, double* RHS);
+/* blocks in the reference's transposed layout: diag[e][i][j] = ADiag, neigh[e][k][i][j] = ANeighs[k].A
+ * (zero for boundary faces) */
+int fp_export_jacobian(fp_ctx* ctx, double* diag, double* neigh);
+/* the last Newton correction x in DOF numbering */
+int fp_export_solution(fp_ctx* ctx, double* x);
+/* matrix-free (a1 M - dR/dW) v and its transpose, element layout in and out */
+int fp_jv(fp_ctx* ctx, double a1, const double* v, double* out);
+int fp_jtv(fp_ctx* ctx, double a1, const double* v, double* out);
+
+/* --- measurement ------------------------------------------------------------------------ */
+/* average device time (ms) of one launch of a kernel family, timed with HIP events on the
+ * library's stream: 0 residual, 1 Jacobian, 2 SpMV, 3 ILU factor, 4 ILU apply, 5 J^T v */
+int fp_time_kernel(fp_ctx* ctx, int which, int reps, double* avg_ms);
+/* hipStream_t of the context (for callers that record their own events) */
+void* fp_stream(fp_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

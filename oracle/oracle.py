@@ -1,0 +1,301 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so, built by oracle/Makefile).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker; the product (flowpro_v2_amd) never
+imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+RULES = os.path.join(os.path.dirname(HERE), "flowpro_v2_amd", "gauss_rules")
+
+_I = ["model", "viscous", "order", "time_order", "preconditioner", "ordering", "newton_iters", "steps",
+      "vary_cfl", "fixed_dt", "positivity_limiter", "jacobian_mode", "gmres_m", "gmres_cycles",
+      "has_attack_angle", "threads"]
+_D = ["kappa", "cv", "pIn0", "rhoIn0", "pOut", "reynolds", "prandtl", "attack_angle", "cfl", "max_cfl", "dt",
+      "newton_tol", "gmres_tol", "penalty", "beta0", "h", "residuum_stop", "scalar_a0", "scalar_a1", "scalar_k"]
+
+
+class Props(C.Structure):
+    _fields_ = [(n, C.c_int) for n in _I] + [(n, C.c_double) for n in _D]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "liboracle.so"], check=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB)
+        vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+        sig = {
+            "or_create": (vp, [C.POINTER(Props), C.c_int, dp, C.c_int, ip, C.c_int, ip, dp, C.c_char_p]),
+            "or_destroy": (None, [vp]),
+            "or_dofs": (C.c_int, [vp]), "or_nbasis": (C.c_int, [vp]), "or_neqs": (C.c_int, [vp]),
+            "or_model_const": (None, [vp, dp]),
+            "or_tt": (None, [vp, ip]), "or_order": (None, [vp, ip]), "or_gindex": (None, [vp, ip]),
+            "or_set_w": (None, [vp, dp, dp, dp]), "or_get_w": (None, [vp, dp]),
+            "or_limiter": (None, [vp]), "or_residual": (None, [vp, dp]),
+            "or_assemble": (None, [vp, C.c_double, C.c_double]),
+            "or_assemble_residuum": (None, [vp, C.c_double, C.c_double]),
+            "or_get_blocks": (None, [vp, dp, dp]), "or_get_rhs": (None, [vp, dp, dp]),
+            "or_linear_solve": (C.c_int, [vp, dp, ip, dp]),
+            "or_spmv": (None, [vp, dp, dp]),
+            "or_rhs_norm": (C.c_double, [vp]), "or_residuum_norms": (None, [vp, dp]),
+            "or_run": (C.c_int, [vp, C.c_int, dp, C.c_int]), "or_state": (None, [vp, dp]),
+            "or_time_step": (C.c_double, [vp]), "or_set_cfl": (None, [vp, C.c_double]),
+            "or_jvp": (None, [vp, C.c_double, dp, dp]), "or_vjp": (None, [vp, C.c_double, dp, dp]),
+            "or_functional": (C.c_double, [vp, C.c_int]), "or_functional_grad_fd": (None, [vp, C.c_int, dp]),
+            "or_elem_geometry": (None, [vp, C.c_int, dp, dp, dp, dp, dp, dp]),
+            "or_nqf": (C.c_int, [vp]), "or_l2_error_scalar": (C.c_double, [vp]), "or_nqv": (C.c_int, [vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+_PREC = {"none": 0, "blockjacobi": 1, "blockilu0": 2}
+_MODEL = {"NavierStokes": 0, "TestScalarEquation": 2}
+
+
+def props_from(p: dict, threads: int = 1) -> Props:
+    """parameters.txt keys -> the oracle's Props (same defaults as C/Parameters.java)."""
+    P = Props()
+    P.model = _MODEL[p.get("model", "NavierStokes")]
+    P.viscous = int(bool(p.get("isFlowViscous", False)))
+    P.order = int(p["order"])
+    P.time_order = int(p.get("timeOrder", 1))
+    P.preconditioner = _PREC[p.get("preconditioner", "blockilu0")]
+    P.ordering = 1 if p.get("elementOrdering", "rcm") == "rcm" else 0
+    P.newton_iters = int(p.get("newtonIters", 1))
+    P.steps = int(p.get("steps", 1))
+    if "dt" in p and "CFL" not in p:
+        P.fixed_dt, P.dt, P.cfl = 1, float(p["dt"]), 0.0
+    else:
+        cfl = float(p.get("CFL", 1.0))
+        P.vary_cfl = int(cfl <= 0)
+        P.cfl = 1.0 if cfl <= 0 else cfl
+    P.max_cfl = float(p.get("maxCFL", 1e4))
+    P.positivity_limiter = int(bool(p.get("positivityLimiter", True)))
+    P.jacobian_mode = {"fd": 0, "exact": 1}[p.get("jacobian", "fd")]
+    P.gmres_m, P.gmres_cycles = 30, 5
+    P.threads = threads
+    P.kappa, P.cv = float(p.get("kappa", 1.4)), float(p.get("cv", 1.0))
+    P.pIn0, P.rhoIn0, P.pOut = float(p.get("pIn0", 1.0)), float(p.get("rhoIn0", 1.0)), float(p.get("pOut", 0.7))
+    P.reynolds, P.prandtl = float(p.get("reynolds", 0.0)), float(p.get("prandtl", 0.72))
+    if "attackAngle" in p:
+        P.has_attack_angle, P.attack_angle = 1, float(p["attackAngle"])
+    P.newton_tol = float(p.get("newtonIterTol", 1e-4))
+    P.gmres_tol = float(p.get("iterativeSolverTol", 1e-2))
+    P.penalty, P.beta0 = float(p.get("penalty", 0.0)), float(p.get("beta0", 2.0))
+    P.h = 1e-8
+    P.residuum_stop = float(p.get("residuum", -1.0))
+    a = p.get("a", (1.0, 0.5))
+    P.scalar_a0, P.scalar_a1, P.scalar_k = float(a[0]), float(a[1]), float(p.get("k", 0.0))
+    return P
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class Oracle:
+    """One case in the oracle: mirrors the reference's Mesh + LocalImplicitSolver."""
+
+    def __init__(self, case, props: dict, threads: int = 1):
+        L = lib()
+        self.case = case
+        self.P = props_from(props, threads)
+        PXY = np.ascontiguousarray(case.PXY, dtype=np.float64)
+        TP = np.ascontiguousarray(case.TP, dtype=np.int32)
+        bnd = np.ascontiguousarray(case.bnd, dtype=np.int32)
+        wd = None if case.wall_distance is None else np.ascontiguousarray(case.wall_distance, dtype=np.float64)
+        self.h = L.or_create(C.byref(self.P), PXY.shape[0], _dp(PXY), TP.shape[0], _ip(TP), bnd.shape[0], _ip(bnd),
+                             _dp(wd) if wd is not None else None, RULES.encode())
+        if not self.h:
+            raise RuntimeError("oracle case construction failed")
+        self.n_elem = TP.shape[0]
+        self.nb = L.or_nbasis(self.h)
+        self.neq = L.or_neqs(self.h)
+        self.B = self.nb * self.neq
+        self.dofs = L.or_dofs(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_destroy(self.h)
+            self.h = None
+
+    def model_const(self):
+        out = np.zeros(7)
+        lib().or_model_const(self.h, _dp(out))
+        return dict(zip(["kapa", "pOut", "Re", "Pr", "pRef", "rhoRef", "velocityRef"], out))
+
+    def TT(self):
+        out = np.zeros((self.n_elem, 3), np.int32)
+        lib().or_tt(self.h, _ip(out))
+        return out
+
+    def order(self):
+        out = np.zeros(self.n_elem, np.int32)
+        lib().or_order(self.h, _ip(out))
+        return out
+
+    def gindex(self):
+        out = np.zeros(self.n_elem, np.int32)
+        lib().or_gindex(self.h, _ip(out))
+        return out
+
+    def set_w(self, W, Wo=None, Wo2=None):
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(self.n_elem, self.B)
+        Wo = None if Wo is None else np.ascontiguousarray(Wo, dtype=np.float64)
+        Wo2 = None if Wo2 is None else np.ascontiguousarray(Wo2, dtype=np.float64)
+        lib().or_set_w(self.h, _dp(W), _dp(Wo) if Wo is not None else None, _dp(Wo2) if Wo2 is not None else None)
+
+    def get_w(self):
+        out = np.zeros((self.n_elem, self.B))
+        lib().or_get_w(self.h, _dp(out))
+        return out
+
+    def limiter(self):
+        lib().or_limiter(self.h)
+
+    def residual(self):
+        out = np.zeros((self.n_elem, self.B))
+        lib().or_residual(self.h, _dp(out))
+        return out
+
+    def assemble(self, dt, dto=-1.0):
+        lib().or_assemble(self.h, dt, dto)
+
+    def assemble_residuum(self, dt, dto=-1.0):
+        lib().or_assemble_residuum(self.h, dt, dto)
+
+    def blocks(self):
+        d = np.zeros((self.n_elem, self.B, self.B))
+        n = np.zeros((self.n_elem, 3, self.B, self.B))
+        lib().or_get_blocks(self.h, _dp(d), _dp(n))
+        return d, n
+
+    def rhs(self):
+        r = np.zeros((self.n_elem, self.B))
+        s = np.zeros((self.n_elem, self.B))
+        lib().or_get_rhs(self.h, _dp(r), _dp(s))
+        return r, s
+
+    def linear_solve(self):
+        x = np.zeros(self.dofs)
+        it = C.c_int(0)
+        err = C.c_double(0)
+        ok = lib().or_linear_solve(self.h, _dp(x), C.byref(it), C.byref(err))
+        return bool(ok), x, it.value, err.value
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.dofs)
+        lib().or_spmv(self.h, _dp(x), _dp(y))
+        return y
+
+    def rhs_norm(self):
+        return lib().or_rhs_norm(self.h)
+
+    def residuum_norms(self):
+        out = np.zeros(self.neq)
+        lib().or_residuum_norms(self.h, _dp(out))
+        return out
+
+    def run(self, nsteps):
+        hist = np.zeros(nsteps + 8)
+        n = lib().or_run(self.h, nsteps, _dp(hist), len(hist))
+        return hist[:n]
+
+    def state(self):
+        out = np.zeros(6)
+        lib().or_state(self.h, _dp(out))
+        return dict(zip(["t", "CFL", "dt", "residuum", "lin_fail", "steps"], out))
+
+    def time_step(self):
+        return lib().or_time_step(self.h)
+
+    def set_cfl(self, cfl):
+        lib().or_set_cfl(self.h, cfl)
+
+    def jvp(self, a1, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(self.n_elem, self.B)
+        out = np.zeros_like(v)
+        lib().or_jvp(self.h, a1, _dp(v), _dp(out))
+        return out
+
+    def vjp(self, a1, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(self.n_elem, self.B)
+        out = np.zeros_like(v)
+        lib().or_vjp(self.h, a1, _dp(v), _dp(out))
+        return out
+
+    def functional(self, which=0):
+        return lib().or_functional(self.h, which)
+
+    def functional_grad_fd(self, which=0):
+        out = np.zeros((self.n_elem, self.B))
+        lib().or_functional_grad_fd(self.h, which, _dp(out))
+        return out
+
+    def geometry(self, i):
+        nqf = lib().or_nqf(self.h)
+        area, size = C.c_double(), C.c_double()
+        Xs = np.zeros(2)
+        nrm = np.zeros((3, nqf, 2))
+        jf = np.zeros((3, nqf))
+        M = np.zeros((self.nb, self.nb))
+        lib().or_elem_geometry(self.h, i, C.byref(area), C.byref(size), _dp(Xs), _dp(nrm), _dp(jf), _dp(M))
+        return dict(area=area.value, elemSize=size.value, Xs=Xs, normals=nrm, jacF=jf, M=M)
+
+    def l2_error_scalar(self):
+        return lib().or_l2_error_scalar(self.h)
+
+    def global_matrix(self):
+        """Dense (a1 M - dR/dW) in DOF numbering from the element blocks (small cases only)."""
+        d, nb = self.blocks()
+        g = self.gindex()
+        TT = self.TT()
+        A = np.zeros((self.dofs, self.dofs))
+        for e in range(self.n_elem):
+            ge = g[e]
+            A[ge:ge + self.B, ge:ge + self.B] += d[e].T
+            for k in range(3):
+                t = TT[e, k]
+                if t > -1:
+                    A[ge:ge + self.B, g[t]:g[t] + self.B] += nb[e, k].T
+        return A
+
+    def to_dof(self, W):
+        """Element-major Java layout -> vector in DOF numbering (gIndex)."""
+        g = self.gindex()
+        x = np.zeros(self.dofs)
+        for e in range(self.n_elem):
+            x[g[e]:g[e] + self.B] = W[e]
+        return x
+
+    def from_dof(self, x):
+        g = self.gindex()
+        return np.stack([x[g[e]:g[e] + self.B] for e in range(self.n_elem)])

@@ -1,0 +1,224 @@
+// FlowPro implicit-DG oracle: CPU restatement of the reference's Java path.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in flowpro_v2_amd/ links, imports or runs
+// this code; it is the checker used by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py.
+//
+// Every function cites the Java file:line it restates.  Abbreviations:
+//   C/    FlowPro/src/main/java/flowpro/core/
+//   API/  FlowProAPI/src/main/java/flowpro/api/
+//   AERO/ ModuleProjects/Equations/Aerodynamics/src/main/java/flowpro/user/equation/
+//   T/    FlowPro/src/test/java/flowpro/
+//
+// Arithmetic: compiled with -ffp-contract=off (Java never fuses a*b+c), the
+// expression order of the Java sources is kept so that the FD Jacobian (which
+// amplifies rounding by 1/h) follows the reference as closely as the C library
+// allows.  java.lang.Math.pow/exp are replaced by glibc pow/exp: ulp-level
+// agreement with the JDK is "parity unpinned" (no reference test pins it).
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace oracle {
+
+// ---------------------------------------------------------------- dual number
+// Forward-mode derivative carrier used for the exact Jacobian, J*v and J^T*v.
+struct Dual {
+    double v, d;
+    Dual() : v(0), d(0) {}
+    Dual(double a) : v(a), d(0) {}
+    Dual(double a, double b) : v(a), d(b) {}
+};
+inline Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+inline Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+inline Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+inline Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+inline Dual operator/(Dual a, Dual b) {
+    double q = a.v / b.v;
+    return Dual(q, (a.d - q * b.d) / b.v);
+}
+inline Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
+inline Dual& operator-=(Dual& a, Dual b) { a = a - b; return a; }
+inline Dual& operator*=(Dual& a, Dual b) { a = a * b; return a; }
+inline Dual& operator/=(Dual& a, Dual b) { a = a / b; return a; }
+inline double val(double a) { return a; }
+inline double val(const Dual& a) { return a.v; }
+inline double dsqrt(double a) { return std::sqrt(a); }
+inline Dual dsqrt(Dual a) {
+    double s = std::sqrt(a.v);
+    return Dual(s, a.d == 0 ? 0.0 : a.d / (2 * s));
+}
+inline double dexp(double a) { return std::exp(a); }
+inline Dual dexp(Dual a) { double e = std::exp(a.v); return Dual(e, e * a.d); }
+inline double dpow(double a, double b) { return std::pow(a, b); }
+inline Dual dpow(Dual a, double b) {
+    double p = std::pow(a.v, b);
+    return Dual(p, a.d == 0 ? 0.0 : b * std::pow(a.v, b - 1) * a.d);
+}
+inline double dabs(double a) { return std::fabs(a); }
+inline Dual dabs(Dual a) { return a.v < 0 ? -a : a; }
+// java.lang.Math.max: (a >= b) ? a : b for ordinary values
+inline double dmax(double a, double b) { return (a >= b) ? a : b; }
+inline Dual dmax(Dual a, Dual b) { return (a.v >= b.v) ? a : b; }
+
+// ---------------------------------------------------------------- properties
+// Raw parameters.txt values (the keys of C/Parameters.java and
+// AERO/Aerodynamics.java init); the model's derived constants are computed by
+// Model::init below.
+struct Props {
+    int model;            // 0 NavierStokes (Euler when viscous == 0), 2 TestScalarEquation
+    int viscous;          // isFlowViscous
+    int order;            // element order k = p + 1 (global "order")
+    int time_order;       // BDF 1 or 2
+    int preconditioner;   // 0 none, 1 blockjacobi, 2 blockilu0
+    int ordering;         // 0 none, 1 rcm
+    int newton_iters;
+    int steps;
+    int vary_cfl;         // CFL <= 0 in parameters.txt
+    int fixed_dt;         // "dt" given instead of "CFL"
+    int positivity_limiter;
+    int jacobian_mode;    // 0 FD (reference, h = 1e-8), 1 exact (dual numbers)
+    int gmres_m;          // 30
+    int gmres_cycles;     // 5
+    int has_attack_angle;
+    int threads;
+    double kappa, cv;
+    double pIn0, rhoIn0, pOut;   // stagnation inlet / outlet (raw)
+    double reynolds, prandtl;
+    double attack_angle;
+    double cfl, max_cfl, dt;
+    double newton_tol, gmres_tol;
+    double penalty, beta0;
+    double h;
+    double residuum_stop;
+    double scalar_a0, scalar_a1, scalar_k;   // TestScalarEquation a, k
+};
+
+struct ModelConst {
+    int kind;        // 0 NS/Euler, 2 scalar
+    int dim, nEqs;
+    bool diffusive;
+    double kapa, cv, cp;
+    double pRef, rhoRef, velocityRef, lRef, tRef;
+    double pOut;     // normalised by pRef
+    double Re, Pr;
+    bool hasAttack;
+    double attack;
+    // scalar model
+    double a[2], k, lambda;
+};
+
+// C/Aerodynamics init restated: AERO/Aerodynamics.java:119-312, AERO/NavierStokes.java:18-21
+ModelConst model_init(const Props& P);
+// AERO/NavierStokes.java:101-138 constInitCondition
+std::vector<double> const_init_condition(const ModelConst& M);
+
+// ---------------------------------------------------------------- quadrature
+struct Rule {
+    int n = 0;
+    std::vector<double> x;   // n * dim
+    std::vector<double> w;
+};
+// C/quadrature/QuadratureCentral.java:46-60 (degree 2k-1, negative weights -> next degree)
+Rule load_line_rule(const std::string& dir, int order);
+Rule load_triangle_rule(const std::string& dir, int order);
+
+// ---------------------------------------------------------------- mesh / elements
+struct Elem {
+    int index;
+    int order, nBasis;
+    int TT[3];
+    int TP[3];
+    double vert[3][2];
+    double wallDist[3];
+    // transformation2Dtriangle coefficients (C/transformation/transformation2Dtriangle.java:62-83)
+    double A[3], B[3], iA[3], iB[3];
+    // integration tables (C/Integration.java:40-60, 108-153)
+    int nq;
+    std::vector<double> JacV, wV, baseV, dbaseV, interpV;   // baseV[q*Nb+j], dbaseV[(q*Nb+j)*2+d], interpV[q*3+v]
+    int nqf[3];
+    std::vector<double> JacF[3], wF[3], baseL[3], dbaseL[3], baseR[3], dbaseR[3], interpF[3];
+    // geometry (C/element/Element.java:516-567)
+    double n[3][16][2];
+    double S[3], Xs[2], Xes[3][2], area, elemSize;
+    int faceIndexReverse[3];
+    // mass (C/element/DGpure.java:547-573)
+    std::vector<double> M, Is;
+    // state
+    std::vector<double> W, Wo, Wo2;
+    // limiter outputs (C/element/DGpure.java:358-400)
+    double eps = 0, c_IP = 0;
+    std::vector<double> dampInner;
+    int gIndex0 = 0;
+    // implicit storage (C/element/Implicit.java:14-126)
+    std::vector<double> ADiag;          // [B][B], ADiag[i*B + j] = A[perturbed i][residual j]
+    std::vector<double> ANeigh[3];      // [B_nbr][B]
+    std::vector<double> RHS, spatialR;
+};
+
+struct Case {
+    Props P;
+    ModelConst Mc;
+    int nPts = 0, nElem = 0;
+    std::vector<double> PXY;
+    std::vector<int> TP;
+    std::vector<int> TT;    // nElem * 3
+    std::vector<Elem> E;
+    std::vector<int> order; // element visiting order for DOF numbering
+    int dofs = 0;
+    Rule vol, face;
+    // linear system (C/LinearSolvers/SparseMatrix.java)
+    std::vector<int> Icrs, Jcrs;
+    std::vector<double> H;
+    std::vector<int> blkRowStart;   // per element: offsets into H, see build_crs
+    // block ILU0 (C/LinearSolvers/preconditioners/BlockILU0.java)
+    std::vector<int> ilu_elems, ilu_rank, ilu_diag;
+    std::vector<std::vector<int>> ilu_cols;
+    std::vector<std::vector<std::vector<double>>> ilu_blocks;   // [i][kk] -> n x m row-major
+    std::vector<std::vector<double>> ilu_dinv;
+    std::vector<std::vector<double>> ilu_y;
+    // blockJacobi scratch: copies of ADiag eliminated in place by lsolveT
+    std::vector<std::vector<double>> bj_work;
+    // GMRES stats
+    int last_iters = 0;
+    double last_error = 0;
+    // driver state (C/State.java)
+    int steps_done = 0;
+    double t = 0, currentCFL = 0, residuum = 1.7976931348623157e308;
+    std::vector<double> resNorm;   // residuumNormalisation
+    int nNormSteps = 0;
+    double cflResOld = -1;
+    double state_dt = 0;
+    std::vector<double> eqResiduum;
+    std::vector<double> history;   // residuum per completed step
+    std::vector<double> newton_history;   // relative Newton residual per inner iteration
+    int lin_fail = 0;
+};
+
+// FlowProMain.createNeighborMatrix (C/FlowProMain.java:772-873)
+std::vector<int> neighbor_matrix(int nElem, const int* TP, int nBnd, const int* bnd);
+// ElementOrdering.reverseCuthillMcKee (C/ElementOrdering.java:52-142)
+std::vector<int> rcm_order(int nElem, const int* TT);
+
+Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const int* TP,
+                 int nBnd, const int* bnd, const double* wallDist, const std::string& rulesDir);
+
+void limiter_all(Case& c);
+void residual_all(Case& c, double* R);          // spatial residual R(W), Java layout
+void assemble(Case& c, double dt, double dto);  // BDF1/2 assembleJacobiMatrix for every element
+void assemble_residuum(Case& c, double dt, double dto);
+bool linear_solve(Case& c, double* x);          // LinearSolver.solve
+double rhs_norm(const Case& c);
+std::vector<double> residuum_norms(const Case& c);
+int run(Case& c, int nsteps);                   // LocalImplicitSolver.solve main loop
+double time_step(Case& c);
+// exact derivative products (dual numbers): out = (a1 M - dR/dW) v  and its transpose
+void jvp(Case& c, double a1, const double* v, double* out);
+void vjp(Case& c, double a1, const double* v, double* out);
+// Functionals (FOI/DragCoefficient.java, FOI/LiftCoefficient.java)
+double functional(const Case& c, int which, const double* W /*nullable*/);
+void functional_grad_fd(Case& c, int which, double* dIdW);
+
+}  // namespace oracle

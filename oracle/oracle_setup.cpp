@@ -1,0 +1,606 @@
+// Oracle (test infrastructure only): setup restatement -- model constants,
+// quadrature rule files, Lagrange bases, affine maps, integration tables,
+// geometry, mass matrices, neighbour matrix and RCM numbering.
+#include "oracle_core.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+
+namespace oracle {
+
+// ------------------------------------------------------------------ Mat
+// API/Mat.java:502-563 gaussian (scaled partial pivoting), :605-642 invert
+static std::vector<int> gaussian(std::vector<std::vector<double>>& A) {
+    int n = (int)A.size();
+    std::vector<int> index(n);
+    std::vector<double> c(n);
+    for (int i = 0; i < n; ++i) index[i] = i;
+    for (int i = 0; i < n; ++i) {
+        double c1 = 0;
+        for (int j = 0; j < n; ++j) {
+            double c0 = std::fabs(A[i][j]);
+            if (c0 > c1) c1 = c0;
+        }
+        c[i] = c1;
+    }
+    for (int j = 0; j < n - 1; ++j) {
+        int k = j;
+        double pi1 = 0;
+        for (int i = j; i < n; ++i) {
+            double pi0 = std::fabs(A[index[i]][j]);
+            pi0 /= c[index[i]];
+            if (pi0 > pi1) { pi1 = pi0; k = i; }
+        }
+        int itmp = index[j]; index[j] = index[k]; index[k] = itmp;
+        for (int i = j + 1; i < n; ++i) {
+            double pj = A[index[i]][j] / A[index[j]][j];
+            A[index[i]][j] = pj;
+            for (int l = j + 1; l < n; ++l) A[index[i]][l] -= pj * A[index[j]][l];
+        }
+    }
+    return index;
+}
+
+std::vector<std::vector<double>> mat_invert(const std::vector<std::vector<double>>& A) {
+    int n = (int)A.size();
+    std::vector<std::vector<double>> a = A, x(n, std::vector<double>(n, 0.0)), b(n, std::vector<double>(n, 0.0));
+    for (int i = 0; i < n; ++i) b[i][i] = 1;
+    std::vector<int> index = gaussian(a);
+    for (int i = 0; i < n - 1; ++i)
+        for (int j = i + 1; j < n; ++j)
+            for (int k = 0; k < n; ++k) b[index[j]][k] -= a[index[j]][i] * b[index[i]][k];
+    for (int i = 0; i < n; ++i) {
+        x[n - 1][i] = b[index[n - 1]][i] / a[index[n - 1]][n - 1];
+        for (int j = n - 2; j >= 0; --j) {
+            x[j][i] = b[index[j]][i];
+            for (int k = j + 1; k < n; ++k) x[j][i] -= a[index[j]][k] * x[k][i];
+            x[j][i] /= a[index[j]][j];
+        }
+    }
+    return x;
+}
+
+// flat row-major variant used by BlockILU0 (same arithmetic)
+void mat_invert_flat(const double* A, int n, double* out) {
+    std::vector<std::vector<double>> a(n, std::vector<double>(n));
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) a[i][j] = A[i * n + j];
+    auto x = mat_invert(a);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) out[i * n + j] = x[i][j];
+}
+
+// ------------------------------------------------------------------ model init
+static double stagP(const ModelConst& M, double mach) {   // AERO/Aerodynamics.java:314-316
+    return std::pow(1 + (M.kapa - 1) / 2 * mach * mach, M.kapa / (M.kapa - 1));
+}
+static double stagRho(const ModelConst& M, double mach) {  // AERO/Aerodynamics.java:318-320
+    return std::pow(1 + (M.kapa - 1) / 2 * mach * mach, 1 / (M.kapa - 1));
+}
+
+ModelConst model_init(const Props& P) {
+    ModelConst M;
+    std::memset(&M, 0, sizeof(M));
+    M.dim = 2;
+    if (P.model == 2) {   // T/user/equation/TestScalarEquation.java:28-37
+        M.kind = 2;
+        M.nEqs = 1;
+        M.a[0] = P.scalar_a0;
+        M.a[1] = P.scalar_a1;
+        M.k = P.scalar_k;
+        double norm = 0;
+        for (int d = 0; d < 2; d++) norm += M.a[d] * M.a[d];
+        M.lambda = std::sqrt(norm);
+        M.diffusive = M.k > 0;
+        return M;
+    }
+    M.kind = 0;
+    M.nEqs = M.dim + 2;
+    M.diffusive = P.viscous != 0;
+    // kappa + cv given (AERO/Aerodynamics.java:125-128)
+    M.kapa = P.kappa;
+    M.cv = P.cv;
+    M.cp = M.cv * M.kapa;
+    // subsonic stagnation inlet (AERO/Aerodynamics.java:169-182)
+    M.pRef = P.pIn0;
+    M.rhoRef = P.rhoIn0;
+    M.pOut = P.pOut / M.pRef;
+    M.lRef = 1;
+    M.velocityRef = std::sqrt(M.pRef / M.rhoRef);
+    M.tRef = M.lRef / M.velocityRef;
+    M.hasAttack = P.has_attack_angle != 0;
+    M.attack = P.attack_angle;
+    if (M.diffusive) {   // AERO/Aerodynamics.java:262-281
+        const double pIn0 = 1;
+        double machInf = std::sqrt(2 / (M.kapa - 1) * (std::pow((pIn0 / M.pOut), (M.kapa - 1) / M.kapa) - 1));
+        double rhoInf = M.rhoRef / stagRho(M, machInf);
+        double pInf = M.pRef / stagP(M, machInf);
+        double aInf = std::sqrt(M.kapa * pInf / rhoInf);
+        double uInf = machInf * aInf;
+        M.Re = P.reynolds * (M.rhoRef * M.velocityRef) / (rhoInf * uInf);
+        M.Pr = P.prandtl;
+    } else {
+        M.Re = -1;
+        M.Pr = -1;
+    }
+    return M;
+}
+
+std::vector<double> const_init_condition(const ModelConst& M) {
+    if (M.kind == 2) return {0.0};
+    // AERO/NavierStokes.java:105-137 (subsonic inlet branch)
+    double kapa = M.kapa;
+    double machIn = std::sqrt(2 / (kapa - 1) * (std::pow((1 / M.pOut), (kapa - 1) / kapa) - 1));
+    double rhoIn = std::pow(1 + ((kapa - 1) / 2) * machIn * machIn, 1 / (1 - kapa));
+    double VIn = machIn * std::sqrt((kapa * M.pOut) / rhoIn);
+    double alpha = M.hasAttack ? M.attack : 0;
+    double vIn[2] = {VIn * std::cos(alpha), VIn * std::sin(alpha)};
+    std::vector<double> W(M.nEqs);
+    W[0] = rhoIn;
+    for (int d = 0; d < M.dim; ++d) W[d + 1] = rhoIn * vIn[d];
+    W[M.dim + 1] = M.pOut / (kapa - 1) + 0.5 * rhoIn * VIn * VIn;
+    return W;
+}
+
+// ------------------------------------------------------------------ quadrature
+static Rule read_rule(const std::string& path, int dimCols) {
+    std::ifstream f(path);
+    if (!f) throw std::runtime_error("cannot open quadrature file " + path);
+    Rule r;
+    f >> r.n;
+    if (dimCols == 1) {
+        r.x.resize(r.n);
+        for (int i = 0; i < r.n; i++) f >> r.x[i];
+    } else {
+        // barycentric -> Cartesian on the reference triangle (0,0),(1,0),(0,1)
+        // (C/quadrature/QuadratureTriangle.java:26-62)
+        r.x.resize(2 * r.n);
+        const double X[3] = {0, 1, 0}, Y[3] = {0, 0, 1};
+        for (int i = 0; i < r.n; i++) {
+            double b[3];
+            f >> b[0] >> b[1] >> b[2];
+            double cx = 0, cy = 0;
+            for (int j = 0; j < 3; j++) { cx += X[j] * b[j]; cy += Y[j] * b[j]; }
+            r.x[2 * i] = cx;
+            r.x[2 * i + 1] = cy;
+        }
+    }
+    r.w.resize(r.n);
+    for (int i = 0; i < r.n; i++) f >> r.w[i];
+    if (!f) throw std::runtime_error("bad quadrature file " + path);
+    return r;
+}
+
+Rule load_line_rule(const std::string& dir, int order) {
+    return read_rule(dir + "/gauss/" + std::to_string(order) + ".txt", 1);
+}
+
+Rule load_triangle_rule(const std::string& dir, int order) {
+    int deg = 2 * order - 1;   // C/quadrature/QuadratureCentral.java:46-60
+    Rule r = read_rule(dir + "/gaussTri/" + std::to_string(deg) + ".txt", 2);
+    bool neg = false;
+    for (double w : r.w) if (w < 0) neg = true;
+    if (neg) r = read_rule(dir + "/gaussTri/" + std::to_string(deg + 1) + ".txt", 2);
+    return r;
+}
+
+// ------------------------------------------------------------------ basis
+// C/basis/basis2Dtriangle.java:22-118
+struct BasisTri {
+    int order, nBasis;
+    std::vector<int> xExp, yExp;
+    std::vector<std::vector<double>> coeffs;
+    explicit BasisTri(int ord) : order(ord) {
+        nBasis = order * (1 + order) / 2;
+        xExp.assign(order * order, 0);
+        yExp.assign(order * order, 0);
+        int s = 0;
+        for (int i = 0; i < order; i++)
+            for (int j = 0; j <= i; j++) { xExp[s] = i - j; yExp[s] = j; ++s; }
+        for (int i = order - 1; i > 0; i--)
+            for (int j = 0; j < i; j++) { xExp[s] = order - 1 - j; yExp[s] = order - i + j; ++s; }
+        if (order == 1) {
+            coeffs = {{1.0}};
+        } else {
+            auto b = barycentr(order, nBasis);
+            std::vector<std::vector<double>> V(nBasis, std::vector<double>(nBasis));
+            for (int i = 0; i < nBasis; i++) {
+                double X0 = b[i][1], X1 = b[i][2];
+                for (int j = 0; j < nBasis; j++) V[i][j] = std::pow(X0, xExp[j]) * std::pow(X1, yExp[j]);
+            }
+            coeffs = mat_invert(V);
+        }
+    }
+    static std::vector<double> linspace(double a, double b, int n) {   // API/Mat.java:42-53
+        std::vector<double> x(n);
+        if (n == 1) x[0] = (a + b) / 2;
+        else
+            for (int i = 0; i < n; i++) {
+                double xi = (double)i / (n - 1);
+                x[i] = (1 - xi) * a + xi * b;
+            }
+        return x;
+    }
+    static std::vector<std::array<double, 3>> barycentr(int rad, int ne) {
+        std::vector<std::array<double, 3>> b(ne);
+        auto x = linspace(0, 1, rad);
+        int s = 0;
+        for (int i = 0; i < rad; i++) {
+            auto y = linspace(0, x[i], i + 1);
+            for (int j = 0; j < i + 1; j++) {
+                b[s][0] = 1 - x[i];
+                b[s][1] = y[j];
+                b[s][2] = x[i] - y[j];
+                s++;
+            }
+        }
+        return b;
+    }
+    double fun(int m, const double* Xi) const {
+        double v = 0;
+        for (int i = 0; i < nBasis; i++) v = v + coeffs[i][m] * std::pow(Xi[0], xExp[i]) * std::pow(Xi[1], yExp[i]);
+        return v;
+    }
+    double der(int m, const double* Xi, int dim) const {
+        double der = 0;
+        if (dim == 0) {
+            for (int i = 0; i < nBasis; i++)
+                if (xExp[i] > 0)
+                    der = der + coeffs[i][m] * xExp[i] * std::pow(Xi[0], xExp[i] - 1) * std::pow(Xi[1], yExp[i]);
+        } else {
+            for (int i = 0; i < nBasis; i++)
+                if (yExp[i] > 0)
+                    der = der + coeffs[i][m] * yExp[i] * std::pow(Xi[0], xExp[i]) * std::pow(Xi[1], yExp[i] - 1);
+        }
+        return der;
+    }
+};
+
+// ------------------------------------------------------------------ transformation
+// C/transformation/transformation2Dtriangle.java:40-119
+static void transform_init(Elem& e) {
+    static const double CX[3][2] = {{0, 0}, {1, 0}, {0, 1}};
+    std::vector<std::vector<double>> V(3, std::vector<double>(3)), Vi(3, std::vector<double>(3));
+    for (int i = 0; i < 3; i++) {
+        V[i][0] = CX[i][0]; V[i][1] = CX[i][1]; V[i][2] = 1;
+        Vi[i][0] = e.vert[i][0]; Vi[i][1] = e.vert[i][1]; Vi[i][2] = 1;
+    }
+    auto invV = mat_invert(V);
+    auto invVi = mat_invert(Vi);
+    auto times = [](const std::vector<std::vector<double>>& A, const double* b, double* out) {
+        for (int i = 0; i < 3; i++) {
+            double s = 0;
+            for (int j = 0; j < 3; j++) s += A[i][j] * b[j];
+            out[i] = s;
+        }
+    };
+    double aux[3];
+    for (int i = 0; i < 3; i++) aux[i] = e.vert[i][0];
+    times(invV, aux, e.A);
+    for (int i = 0; i < 3; i++) aux[i] = e.vert[i][1];
+    times(invV, aux, e.B);
+    for (int i = 0; i < 3; i++) aux[i] = CX[i][0];
+    times(invVi, aux, e.iA);
+    for (int i = 0; i < 3; i++) aux[i] = CX[i][1];
+    times(invVi, aux, e.iB);
+}
+static void getX(const Elem& e, const double* Xi, double* X) {
+    X[0] = e.A[0] * Xi[0] + e.A[1] * Xi[1] + e.A[2];
+    X[1] = e.B[0] * Xi[0] + e.B[1] * Xi[1] + e.B[2];
+}
+static void getXi(const Elem& e, const double* X, double* Xi) {
+    Xi[0] = e.iA[0] * X[0] + e.iA[1] * X[1] + e.iA[2];
+    Xi[1] = e.iB[0] * X[0] + e.iB[1] * X[1] + e.iB[2];
+}
+static double jacobianVol(const Elem& e) {
+    return std::fabs(e.A[0] * e.B[1] - e.A[1] * e.B[0]) / 2;
+}
+// Transformation.transformBasis: iT = invert(jacobiMatrix), d_x = sum_j iT[j][i] d_xi_j
+// (C/transformation/Transformation.java:146-163)
+static void transform_basis(const Elem& e, const double* dXi, double* dX, int nb) {
+    std::vector<std::vector<double>> J = {{e.A[0], e.A[1]}, {e.B[0], e.B[1]}};
+    auto iT = mat_invert(J);
+    for (int m = 0; m < nb; m++)
+        for (int i = 0; i < 2; i++) {
+            double s = 0;
+            for (int j = 0; j < 2; j++) s += iT[j][i] * dXi[m * 2 + j];
+            dX[m * 2 + i] = s;
+        }
+}
+
+// ------------------------------------------------------------------ neighbour matrix
+std::vector<int> neighbor_matrix(int nElem, const int* TP, int nBnd, const int* bnd) {
+    // key = sorted vertex pair (C/FlowProMain.java:786-806 uses the string "a$b$")
+    std::map<std::pair<int, int>, std::pair<int, int>> faces;
+    auto key = [&](int i, int k) {
+        int a = TP[3 * i + k], b = TP[3 * i + (k + 1) % 3];
+        return a < b ? std::make_pair(a, b) : std::make_pair(b, a);
+    };
+    for (int i = 0; i < nElem; i++)
+        for (int k = 0; k < 3; k++) {
+            auto kk = key(i, k);
+            auto it = faces.find(kk);
+            if (it == faces.end()) faces[kk] = {i, -1000};
+            else it->second.second = i;
+        }
+    std::vector<int> TT(3 * nElem, -1000);
+    for (int i = 0; i < nElem; i++)
+        for (int k = 0; k < 3; k++) {
+            auto& f = faces[key(i, k)];
+            if (f.first == i) {
+                if (f.second != -1000) TT[3 * i + k] = f.second;
+            } else {
+                TT[3 * i + k] = f.first;
+            }
+        }
+    std::map<std::pair<int, int>, int> bmap;   // :834-848
+    for (int r = 0; r < nBnd; r++) {
+        int a = bnd[3 * r + 1], b = bnd[3 * r + 2];
+        bmap[a < b ? std::make_pair(a, b) : std::make_pair(b, a)] = bnd[3 * r];
+    }
+    for (int i = 0; i < nElem; i++)
+        for (int k = 0; k < 3; k++)
+            if (TT[3 * i + k] == -1000) {
+                auto it = bmap.find(key(i, k));
+                TT[3 * i + k] = it != bmap.end() ? it->second : -1;
+            }
+    return TT;
+}
+
+// ------------------------------------------------------------------ RCM
+static int pseudo_peripheral(const std::vector<std::vector<int>>& adj, const std::vector<char>& excluded, int start) {
+    int node = start;
+    int ecc = -1;
+    int n = (int)adj.size();
+    std::vector<int> level(n);
+    for (int iter = 0; iter < 10; iter++) {
+        std::fill(level.begin(), level.end(), -1);
+        std::deque<int> q;
+        q.push_back(node);
+        level[node] = 0;
+        int last = node;
+        while (!q.empty()) {
+            int e = q.front(); q.pop_front();
+            for (int t : adj[e]) {
+                if (!excluded[t] && level[t] == -1) {
+                    level[t] = level[e] + 1;
+                    q.push_back(t);
+                    if (level[t] > level[last] || (level[t] == level[last] && adj[t].size() < adj[last].size())) last = t;
+                }
+            }
+        }
+        if (level[last] <= ecc) break;
+        ecc = level[last];
+        node = last;
+    }
+    return node;
+}
+
+std::vector<int> rcm_order(int n, const int* TT) {
+    std::vector<std::vector<int>> adj(n);
+    std::vector<int> degree(n);
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) {
+            int t = TT[3 * i + k];
+            if (t > -1 && t != i && std::find(adj[i].begin(), adj[i].end(), t) == adj[i].end()) adj[i].push_back(t);
+        }
+        degree[i] = (int)adj[i].size();
+    }
+    for (int i = 0; i < n; i++)   // stable sort by degree (Stream.sorted is stable)
+        std::stable_sort(adj[i].begin(), adj[i].end(), [&](int a, int b) { return degree[a] < degree[b]; });
+    std::vector<int> order(n);
+    std::vector<char> visited(n, 0);
+    int count = 0;
+    while (count < n) {
+        int start = -1;
+        for (int i = 0; i < n; i++)
+            if (!visited[i] && (start == -1 || degree[i] < degree[start])) start = i;
+        start = pseudo_peripheral(adj, visited, start);
+        std::deque<int> q;
+        q.push_back(start);
+        visited[start] = 1;
+        while (!q.empty()) {
+            int e = q.front(); q.pop_front();
+            order[count++] = e;
+            for (int t : adj[e])
+                if (!visited[t]) { visited[t] = 1; q.push_back(t); }
+        }
+    }
+    for (int i = 0; i < n / 2; i++) std::swap(order[i], order[n - 1 - i]);
+    return order;
+}
+
+// ------------------------------------------------------------------ case build
+// C/Mesh.java:69-224 (constructor and init) for straight triangles, one global order.
+Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const int* TP,
+                 int nBnd, const int* bnd, const double* wallDist, const std::string& rulesDir) {
+    Case* c = new Case();
+    c->P = P;
+    c->Mc = model_init(P);
+    c->state_dt = P.fixed_dt ? P.dt : 0.0;   // State constructor (C/State.java:75-79)
+    c->nPts = nPts;
+    c->nElem = nElem;
+    c->PXY.assign(PXY, PXY + 2 * nPts);
+    c->TP.assign(TP, TP + 3 * nElem);
+    c->TT = neighbor_matrix(nElem, TP, nBnd, bnd);
+    const int nEqs = c->Mc.nEqs;
+    const int order = P.order;
+    c->vol = load_triangle_rule(rulesDir, order);
+    c->face = load_line_rule(rulesDir, order);
+    BasisTri basis(order);
+    const int Nb = basis.nBasis;
+    const int B = Nb * nEqs;
+    // reference-element tables (same for every straight triangle)
+    const Rule& qv = c->vol;
+    std::vector<double> baseRef(qv.n * Nb), dXiRef(qv.n * Nb * 2);
+    for (int p = 0; p < qv.n; p++)
+        for (int m = 0; m < Nb; m++) baseRef[p * Nb + m] = basis.fun(m, &qv.x[2 * p]);
+    for (int d = 0; d < 2; d++)
+        for (int p = 0; p < qv.n; p++)
+            for (int m = 0; m < Nb; m++) dXiRef[(p * Nb + m) * 2 + d] = basis.der(m, &qv.x[2 * p], d);
+    c->E.resize(nElem);
+    for (int i = 0; i < nElem; i++) {
+        Elem& e = c->E[i];
+        e.index = i;
+        e.order = order;
+        e.nBasis = Nb;
+        for (int k = 0; k < 3; k++) {
+            e.TT[k] = c->TT[3 * i + k];
+            e.TP[k] = TP[3 * i + k];
+            e.vert[k][0] = PXY[2 * TP[3 * i + k]];
+            e.vert[k][1] = PXY[2 * TP[3 * i + k] + 1];
+            e.wallDist[k] = wallDist ? wallDist[TP[3 * i + k]] : 0.0;
+        }
+        transform_init(e);
+        e.dampInner.assign(nEqs, 0.0);
+    }
+    // DOF numbering (C/Mesh.java:181-187, C/element/Element.java:241-248)
+    if (P.ordering == 1) c->order = rcm_order(nElem, c->TT.data());
+    else {
+        c->order.resize(nElem);
+        for (int i = 0; i < nElem; i++) c->order[i] = i;
+    }
+    int dofs0 = 0;
+    for (int idx : c->order) { c->E[idx].gIndex0 = dofs0; dofs0 += B; }
+    c->dofs = dofs0;
+    // integration tables, geometry, mass (C/Integration.java, C/element/Element.java:516-567)
+    for (int i = 0; i < nElem; i++) {
+        Elem& e = c->E[i];
+        const double jac = jacobianVol(e);
+        e.nq = qv.n;
+        e.JacV.assign(qv.n, jac);
+        e.wV = qv.w;
+        e.baseV = baseRef;
+        e.dbaseV.resize(qv.n * Nb * 2);
+        for (int p = 0; p < qv.n; p++) transform_basis(e, &dXiRef[p * Nb * 2], &e.dbaseV[p * Nb * 2], Nb);
+        e.interpV.resize(qv.n * 3);
+        for (int p = 0; p < qv.n; p++) {
+            double xi = qv.x[2 * p], eta = qv.x[2 * p + 1];
+            e.interpV[3 * p] = 1 - xi - eta;
+            e.interpV[3 * p + 1] = xi;
+            e.interpV[3 * p + 2] = eta;
+        }
+    }
+    static const double CX[3][2] = {{0, 0}, {1, 0}, {0, 1}};
+    const Rule& qf = c->face;
+    for (int i = 0; i < nElem; i++) {
+        Elem& e = c->E[i];
+        for (int k = 0; k < 3; k++) {
+            // faceTransformation2Dline (C/transformation/faceTransformation2Dline.java:23-60)
+            int f0 = k, f1 = (k + 1) % 3;
+            double ref0[2] = {CX[f0][0], CX[f0][1]}, ref1[2] = {CX[f1][0], CX[f1][1]};
+            int nf = qf.n;
+            e.nqf[k] = nf;
+            e.wF[k] = qf.w;
+            e.JacF[k].resize(nf);
+            e.baseL[k].resize(nf * Nb);
+            e.dbaseL[k].resize(nf * Nb * 2);
+            e.interpF[k].resize(nf * 2);
+            double s[2] = {ref1[0] - ref0[0], ref1[1] - ref0[1]};
+            // Js = J * s with J = [[A0,A1],[B0,B1]]
+            double Js[2] = {e.A[0] * s[0] + e.A[1] * s[1], e.B[0] * s[0] + e.B[1] * s[1]};
+            double Jsn = std::sqrt(Js[0] * Js[0] + Js[1] * Js[1]);
+            for (int p = 0; p < nf; p++) {
+                double xi = qf.x[p];
+                double XiRef[2];
+                XiRef[0] = (1 + xi) / 2 * ref1[0] + (1 - xi) / 2 * ref0[0];
+                XiRef[1] = (1 + xi) / 2 * ref1[1] + (1 - xi) / 2 * ref0[1];
+                e.JacF[k][p] = Jsn / 2;
+                double nrm[2] = {Js[1], -Js[0]};
+                double nn = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1]);
+                e.n[k][p][0] = nrm[0] / nn;
+                e.n[k][p][1] = nrm[1] / nn;
+                e.interpF[k][2 * p] = (1 - xi) / 2;
+                e.interpF[k][2 * p + 1] = (1 + xi) / 2;
+                for (int m = 0; m < Nb; m++) e.baseL[k][p * Nb + m] = basis.fun(m, XiRef);
+                double dXi[64];
+                for (int d = 0; d < 2; d++)
+                    for (int m = 0; m < Nb; m++) dXi[m * 2 + d] = basis.der(m, XiRef, d);
+                transform_basis(e, dXi, &e.dbaseL[k][p * Nb * 2], Nb);
+            }
+            int t = e.TT[k];
+            if (t > -1) {
+                Elem& er = c->E[t];
+                e.baseR[k].resize(nf * Nb);
+                e.dbaseR[k].resize(nf * Nb * 2);
+                for (int p = 0; p < nf; p++) {
+                    double xi = qf.x[p];
+                    double XiRef[2];
+                    XiRef[0] = (1 + xi) / 2 * ref1[0] + (1 - xi) / 2 * ref0[0];
+                    XiRef[1] = (1 + xi) / 2 * ref1[1] + (1 - xi) / 2 * ref0[1];
+                    double X[2], XiR[2];
+                    getX(e, XiRef, X);
+                    getXi(er, X, XiR);
+                    for (int m = 0; m < Nb; m++) e.baseR[k][p * Nb + m] = basis.fun(m, XiR);
+                    double dXi[64];
+                    for (int d = 0; d < 2; d++)
+                        for (int m = 0; m < Nb; m++) dXi[m * 2 + d] = basis.der(m, XiR, d);
+                    transform_basis(er, dXi, &e.dbaseR[k][p * Nb * 2], Nb);
+                }
+            }
+        }
+        // computeGeometry
+        for (int k = 0; k < 3; k++) {
+            double mid[2] = {0.5 * (0), 0};
+            int f0 = k, f1 = (k + 1) % 3;
+            double XiRef[2];
+            // faceTransform.getXs = getX(Xi = 0): (1+0)/2*ref1 + (1-0)/2*ref0
+            XiRef[0] = (1 + 0.0) / 2 * CX[f1][0] + (1 - 0.0) / 2 * CX[f0][0];
+            XiRef[1] = (1 + 0.0) / 2 * CX[f1][1] + (1 - 0.0) / 2 * CX[f0][1];
+            getX(e, XiRef, mid);
+            e.Xes[k][0] = mid[0];
+            e.Xes[k][1] = mid[1];
+            e.S[k] = 0;
+            for (int p = 0; p < e.nqf[k]; p++) e.S[k] = e.S[k] + e.JacF[k][p] * e.wF[k][p];
+        }
+        double third[2] = {1.0 / 3, 1.0 / 3};
+        getX(e, third, e.Xs);
+        e.area = 0;
+        for (int p = 0; p < e.nq; p++) e.area = e.area + e.JacV[p] * e.wV[p];
+        e.elemSize = 0;
+        for (int k = 0; k < 3; k++) e.elemSize += e.S[k];
+        e.elemSize = 3 * e.area / e.elemSize;
+        // mass matrix and basis weights (C/element/DGpure.java:547-573)
+        e.M.assign(Nb * Nb, 0.0);
+        e.Is.assign(Nb, 0.0);
+        for (int a = 0; a < Nb; a++) {
+            for (int b = 0; b < Nb; b++)
+                for (int p = 0; p < e.nq; p++)
+                    e.M[a * Nb + b] = e.M[a * Nb + b] + e.JacV[p] * e.wV[p] * e.baseV[p * Nb + a] * e.baseV[p * Nb + b];
+            for (int p = 0; p < e.nq; p++) e.Is[a] = e.Is[a] + e.JacV[p] * e.wV[p] * e.baseV[p * Nb + a] / e.area;
+        }
+        e.ADiag.assign(B * B, 0.0);
+        for (int k = 0; k < 3; k++)
+            if (e.TT[k] > -1) e.ANeigh[k].assign(B * B, 0.0);
+        e.RHS.assign(B, 0.0);
+        e.spatialR.assign(B, 0.0);
+    }
+    for (int i = 0; i < nElem; i++) {
+        Elem& e = c->E[i];
+        for (int k = 0; k < 3; k++) {
+            e.faceIndexReverse[k] = 0;
+            if (e.TT[k] > -1)
+                for (int s = 0; s < 3; s++)
+                    if (c->E[e.TT[k]].TT[s] == i) { e.faceIndexReverse[k] = s; break; }
+        }
+    }
+    // initial condition (C/element/DGpure.java:74-107, lagrange branch)
+    auto W0 = const_init_condition(c->Mc);
+    for (auto& e : c->E) {
+        e.W.assign(B, 0.0);
+        for (int j = 0; j < Nb; j++)
+            for (int m = 0; m < nEqs; m++) e.W[m * Nb + j] = W0[m];
+        e.Wo = e.W;
+        e.Wo2 = e.W;
+    }
+    return c;
+}
+
+}  // namespace oracle

@@ -1,0 +1,146 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on the same inputs.
+
+Tolerances (stated per quantity, equation-norm relative unless said otherwise):
+  residual / BDF right-hand side      1e-12
+  exact Jacobian blocks               1e-10 (max |diff| / max |entry|)
+  J v, J^T v (matrix free)            1e-10
+  GMRES correction (tight tol 1e-13)  1e-8  (max |diff| / max |x|)
+  Newton residual history              1e-8  (relative, per step)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from flowpro_v2_amd import cases, solver
+import oracle as O
+from helpers import block_rel_err, eq_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(case, props, seed=3, rel=0.01, precond_bits=64):
+    props = dict(props)
+    props["jacobian"] = "exact"
+    o = O.Oracle(case, props)
+    dev = solver.DeviceCase(case, props, precond_bits=precond_bits)
+    W0 = dev.initial_state()
+    W = cases.perturbed_state(W0[0][:: dev.nb], case.n_elem, dev.nb, rel, seed) if dev.neq > 1 else \
+        cases.java_random_state(case.n_elem, dev.B, seed)
+    o.set_w(W)
+    o.limiter()
+    dev.set_state(W)
+    return o, dev, W
+
+
+CONFIGS = [
+    ("euler_p1", lambda: cases.channel_bump(16, 4), lambda: cases.euler_props(order=2)),
+    ("euler_p2", lambda: cases.channel_bump(12, 4), lambda: cases.euler_props(order=3)),
+    ("ns_p1", lambda: cases.cylinder_mesh(16, 6), lambda: cases.ns_props(order=2, reynolds=100.0)),
+    ("ns_p3", lambda: cases.cylinder_mesh(16, 5), lambda: cases.ns_props(order=4)),
+    ("scalar_p2_diff", lambda: cases.square_mesh(4, 1.0, -2),
+     lambda: dict(model="TestScalarEquation", a=(1.0, 0.5), k=0.1, order=3, dt=1e8, penalty=0.0)),
+]
+
+
+@pytest.mark.parametrize("name,mk_case,mk_props", CONFIGS, ids=[c[0] for c in CONFIGS])
+def test_residual_and_jacobian(name, mk_case, mk_props):
+    case, props = mk_case(), mk_props()
+    o, dev, W = _setup(case, props)
+    dt, dto = 0.05, -1.0
+    o.assemble(dt, dto)
+    norms, rhs = dev.assemble(dt, dto, 1, True)
+    R, RHS = dev.residual()
+    Rref = o.residual()
+    RHSref, _ = o.rhs()
+    assert eq_rel_err(R, Rref, dev.nb, dev.neq) < 1e-12
+    assert eq_rel_err(RHS, RHSref, dev.nb, dev.neq) < 1e-12
+    np.testing.assert_allclose(norms, o.residuum_norms(), rtol=1e-11)
+    assert math.isclose(rhs, o.rhs_norm(), rel_tol=1e-11)
+    d, n = dev.jacobian()
+    dref, nref = o.blocks()
+    assert block_rel_err(d, dref) < 1e-10
+    assert block_rel_err(n, nref) < 1e-10
+    dev.close()
+
+
+@pytest.mark.parametrize("name,mk_case,mk_props", CONFIGS[:4], ids=[c[0] for c in CONFIGS[:4]])
+def test_jv_jtv(name, mk_case, mk_props):
+    case, props = mk_case(), mk_props()
+    o, dev, W = _setup(case, props)
+    rng = np.random.default_rng(7)
+    v = rng.standard_normal((case.n_elem, dev.B))
+    a1 = 20.0
+    jv, jtv = dev.jv(a1, v), dev.jtv(a1, v)
+    assert block_rel_err(jv, o.jvp(a1, v)) < 1e-10
+    assert block_rel_err(jtv, o.vjp(a1, v)) < 1e-10
+    # transposition consistency <u, J v> = <J^T u, v>
+    u = rng.standard_normal(v.shape)
+    lhs, rhs = float((u * dev.jv(a1, v)).sum()), float((dev.jtv(a1, u) * v).sum())
+    assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1.0)
+    dev.close()
+
+
+@pytest.mark.parametrize("prec", ["blockilu0", "blockjacobi", "none"])
+def test_linear_solve_matches_oracle(prec):
+    case = cases.channel_bump(16, 4)
+    props = cases.euler_props(order=2, preconditioner=prec, iterativeSolverTol=1e-13)
+    o, dev, W = _setup(case, props)
+    o.assemble(0.05, -1.0)
+    dev.assemble(0.05, -1.0, 1, True)
+    ok, xref, it_ref, _ = o.linear_solve()
+    rc, it, err = dev.linear_solve()
+    x = dev.solution()
+    assert ok and rc == 0
+    assert np.abs(x - xref).max() / np.abs(xref).max() < 1e-8
+    # the reference's LinearSolverTest criterion against a dense direct solution
+    A = o.global_matrix()
+    r, _ = o.rhs()
+    xd = np.linalg.solve(A, o.to_dof(r))
+    assert np.abs(x - xd).max() / np.abs(xd).max() < 1e-8
+    dev.close()
+
+
+def test_fp32_preconditioner_same_solution():
+    case = cases.channel_bump(16, 4)
+    props = cases.euler_props(order=2, iterativeSolverTol=1e-12)
+    o, dev, W = _setup(case, props, precond_bits=32)
+    o.assemble(0.05, -1.0)
+    dev.assemble(0.05, -1.0, 1, True)
+    ok, xref, _, _ = o.linear_solve()
+    rc, it, err = dev.linear_solve()
+    assert rc == 0
+    x = dev.solution()
+    assert np.abs(x - xref).max() / np.abs(xref).max() < 1e-8
+    dev.close()
+
+
+@pytest.mark.parametrize("name,mk_case,mk_props,steps", [
+    ("euler_p1", lambda: cases.channel_bump(16, 4),
+     lambda: cases.euler_props(order=2, CFL=2.0, iterativeSolverTol=1e-12, newtonIters=2), 8),
+    ("ns_p2", lambda: cases.cylinder_mesh(16, 6),
+     lambda: cases.ns_props(order=3, reynolds=100.0, CFL=2.0, iterativeSolverTol=1e-12, timeOrder=2), 5),
+])
+def test_newton_history(name, mk_case, mk_props, steps):
+    case, props = mk_case(), dict(mk_props())
+    props["jacobian"] = "exact"
+    o = O.Oracle(case, props)
+    hist_ref = o.run(steps)
+    dev = solver.DeviceCase(case, props)
+    dev.set_state(dev.initial_state())
+    s = solver.GpuImplicitSolver(dev, props)
+    hist = [r.residuum for r in s.solve(steps)]
+    assert len(hist) == len(hist_ref) == steps
+    np.testing.assert_allclose(hist, hist_ref, rtol=1e-8)
+    Wg, Wr = dev.get_state(), o.get_w()
+    assert np.abs(Wg - Wr).max() / np.abs(Wr).max() < 1e-8
+    dev.close()
+
+
+def test_time_step():
+    case = cases.channel_bump(16, 4)
+    props = cases.euler_props(order=2)
+    o, dev, W = _setup(case, props)
+    o.set_cfl(1.0)
+    assert math.isclose(dev.size_over_lambda(), o.time_step(), rel_tol=1e-13)
+    dev.close()
