@@ -50,6 +50,13 @@ struct ElemLaunch {
     int model, nb, neq, nqv, nqf, visc;
 };
 
+// vectors of one preconditioner application, read by the sweep kernels through a device
+// pointer so that the launch sequence can be replayed from a graph for any vectors
+struct IluIO {
+    const double* b;
+    double* x;
+};
+
 struct DGParams {
     double beta0;   // DDG constant (0 when the global order is 1)
     double cIP;     // interior penalty (penalty when diffusive and order > 1, else 0)

@@ -39,6 +39,7 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
     const double iT[4] = {ge[0], ge[1], ge[2], ge[3]};
     const double jac = ge[4];
     // ---------------- faces (DGpure.residuumWall) ----------------
+#pragma unroll 1
     for (int k = 0; k < 3; k++) {
         const int TT = ms.nbr[e * 3 + k];
         const double* fg = ms.fgeo + ((size_t)e * 3 + k) * 4;
@@ -52,6 +53,7 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
             iTR[0] = gn[0]; iTR[1] = gn[1]; iTR[2] = gn[2]; iTR[3] = gn[3];
         }
         kr.face_begin(k);
+#pragma unroll 1
         for (int p = 0; p < NQF; p++) {
             const double* phiL = tb.fb + (k * NQF + p) * NB;
             const double* dphiL = tb.fd + (k * NQF + p) * NB * 2;
@@ -134,6 +136,7 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
     }
     // ---------------- volume (DGpure.residuum) ----------------
     if constexpr (NB > 1 || MK == MODEL_SCALAR) {
+#pragma unroll 1
         for (int q = 0; q < NQV; q++) {
             const double* phi = tb.vb + q * NB;
             const double* dphi = tb.vd + q * NB * 2;

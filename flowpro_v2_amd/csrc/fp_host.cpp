@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -24,6 +25,10 @@ hipError_t launch_residual(const ElemLaunch&, hipStream_t, const ModelParams&, c
 template <typename BT>
 hipError_t launch_jacobian(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
                            const double*, double, BT*, const double*, double*, int);
+hipError_t launch_residual_team(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
+                                const double*, const double*, const double*, double, double, double, double*, double*, double*);
+hipError_t launch_jacobian_team(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
+                                const double*, double, double*);
 hipError_t launch_jvp(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
                       const double*, const double*, double, double*);
 hipError_t launch_size_over_lambda(const ElemLaunch&, hipStream_t, const ModelParams&, const Tables&, const Mesh&,
@@ -34,11 +39,12 @@ template <typename TF> hipError_t launch_ilu_copy(hipStream_t, size_t, const dou
 template <typename TF>
 hipError_t launch_ilu_factor_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, int*);
 template <typename TF>
-hipError_t launch_ilu_fwd_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, const TF*, const double*,
+hipError_t launch_ilu_fwd_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, const TF*, const IluIO*,
                                 TF*);
 template <typename TF>
 hipError_t launch_ilu_bwd_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, const TF*, const TF*, TF*,
-                                double*);
+                                const IluIO*);
+hipError_t launch_set_io(hipStream_t, IluIO*, const double*, double*);
 hipError_t launch_block_jacobi(hipStream_t, int, int, double*, const double*, double*);
 hipError_t launch_gather_diag(hipStream_t, int, int, const int*, const double*, double*);
 hipError_t launch_min(hipStream_t, size_t, const double*, double*, double*);
@@ -85,6 +91,9 @@ struct fp_ctx {
     double *x = nullptr, *V = nullptr, *w = nullptr, *r = nullptr, *aux = nullptr, *partial = nullptr, *scal = nullptr;
     double* y = nullptr;
     int* istat = nullptr;
+    IluIO* io = nullptr;
+    bool use_graph = true;
+    hipGraphExec_t g_factor = nullptr, g_apply = nullptr;
     Tables tb{};
     Mesh ms{};
     bool jac_valid = false;
@@ -322,6 +331,9 @@ template <typename T> hipError_t upload(T** p, const std::vector<T>& h) {
 
 // ============================================================== create / destroy
 static void free_all(fp_ctx* c) {
+    if (c->g_factor) (void)hipGraphExecDestroy(c->g_factor);
+    if (c->g_apply) (void)hipGraphExecDestroy(c->g_apply);
+    if (c->io) (void)hipFree(c->io);
     void* ptrs[] = {c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
                     c->diagpos, c->lvl_f, c->lvl_b, c->val, c->F, c->iy, c->ixs, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
                     c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->istat};
@@ -591,6 +603,11 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     HIPCHK(c, dalloc(&c->scal, 64 + 2 * (size_t)c->m + 8));
     HIPCHK(c, dalloc(&c->y, (size_t)c->m + 1));
     HIPCHK(c, dalloc(&c->istat, 4));
+    HIPCHK(c, hipMalloc((void**)&c->io, sizeof(IluIO)));
+    {
+        const char* g = std::getenv("FLOWPRO_GRAPH");
+        c->use_graph = !(g && g[0] == '0');
+    }
     HIPCHK(c, hipMemsetAsync(c->W, 0, N * sizeof(double), c->s));
     HIPCHK(c, hipMemsetAsync(c->Wo, 0, N * sizeof(double), c->s));
     HIPCHK(c, hipMemsetAsync(c->Wo2, 0, N * sizeof(double), c->s));
@@ -720,10 +737,10 @@ static void bdf_coefficients(int time_order, double dt, double dto, double* a) {
 static int assemble_impl(fp_ctx* c, double dt, double dto, int time_order, int with_jac, double* eq_norms, double* rhs_norm) {
     double a[3];
     bdf_coefficients(time_order, dt, dto, a);
-    HIPCHK(c, launch_residual(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R, c->RHS,
-                              c->enorm));
+    HIPCHK(c, launch_residual_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R, c->RHS,
+                                   c->enorm));
     if (with_jac) {
-        HIPCHK(c, launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a[0], c->val, nullptr, nullptr, 0));
+        HIPCHK(c, launch_jacobian_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a[0], c->val));
         c->jac_valid = true;
         c->last_a1 = a[0];
     }
@@ -754,29 +771,54 @@ extern "C" int fp_assemble(fp_ctx* c, double dt, double dto, int time_order, int
 }
 
 // --- preconditioner
-template <typename TF> static int ilu_factor_t(fp_ctx* c) {
+// The level sequences of the block-ILU0 factorisation and of the two sweeps are captured
+// once per context into HIP graphs and replayed (the per-level launches dominate otherwise).
+template <typename TF> static int ilu_factor_launches(fp_ctx* c) {
     TF* F = (TF*)c->F;
     const size_t tot = (size_t)c->nnzb * c->B * c->B;
     HIPCHK(c, launch_ilu_copy<TF>(c->s, tot, c->val, F));
-    HIPCHK(c, hipMemsetAsync(c->istat, 0, sizeof(int), c->s));
     for (size_t l = 0; l + 1 < c->lvl_f_ptr.size(); l++) {
         int b = c->lvl_f_ptr[l], e = c->lvl_f_ptr[l + 1];
         HIPCHK(c, launch_ilu_factor_level<TF>(c->s, c->B, c->lvl_f + b, e - b, c->rowptr, c->colidx, c->diagpos, F, c->istat));
     }
     return FP_OK;
 }
-template <typename TF> static int ilu_apply_t(fp_ctx* c, double* out, const double* in) {
+template <typename TF> static int ilu_apply_launches(fp_ctx* c) {
     const TF* F = (const TF*)c->F;
     for (size_t l = 0; l + 1 < c->lvl_f_ptr.size(); l++) {
         int b = c->lvl_f_ptr[l], e = c->lvl_f_ptr[l + 1];
-        HIPCHK(c, launch_ilu_fwd_level<TF>(c->s, c->B, c->lvl_f + b, e - b, c->rowptr, c->colidx, c->diagpos, F, in, (TF*)c->iy));
+        HIPCHK(c, launch_ilu_fwd_level<TF>(c->s, c->B, c->lvl_f + b, e - b, c->rowptr, c->colidx, c->diagpos, F, c->io, (TF*)c->iy));
     }
     for (size_t l = 0; l + 1 < c->lvl_b_ptr.size(); l++) {
         int b = c->lvl_b_ptr[l], e = c->lvl_b_ptr[l + 1];
         HIPCHK(c, launch_ilu_bwd_level<TF>(c->s, c->B, c->lvl_b + b, e - b, c->rowptr, c->colidx, c->diagpos, F, (const TF*)c->iy,
-                                           (TF*)c->ixs, out));
+                                           (TF*)c->ixs, c->io));
     }
     return FP_OK;
+}
+template <typename TF> static int run_captured(fp_ctx* c, hipGraphExec_t* exec, int (*body)(fp_ctx*)) {
+    if (!c->use_graph) return body(c);
+    if (!*exec) {
+        hipGraph_t g;
+        HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        int rc = body(c);
+        hipError_t e = hipStreamEndCapture(c->s, &g);
+        if (rc) return rc;
+        HIPCHK(c, e);
+        e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(c, e);
+    }
+    HIPCHK(c, hipGraphLaunch(*exec, c->s));
+    return FP_OK;
+}
+template <typename TF> static int ilu_factor_t(fp_ctx* c) {
+    HIPCHK(c, hipMemsetAsync(c->istat, 0, sizeof(int), c->s));
+    return run_captured<TF>(c, &c->g_factor, ilu_factor_launches<TF>);
+}
+template <typename TF> static int ilu_apply_t(fp_ctx* c, double* out, const double* in) {
+    HIPCHK(c, launch_set_io(c->s, c->io, in, out));
+    return run_captured<TF>(c, &c->g_apply, ilu_apply_launches<TF>);
 }
 
 static int prec_factor(fp_ctx* c) {
@@ -1017,11 +1059,11 @@ extern "C" int fp_time_kernel(fp_ctx* c, int which, int reps, double* avg_ms) {
     auto once = [&]() -> int {
         switch (which) {
             case 0:
-                HIPCHK(c, launch_residual(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R,
-                                          c->RHS, c->enorm));
+                HIPCHK(c, launch_residual_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R,
+                                               c->RHS, c->enorm));
                 return FP_OK;
             case 1:
-                HIPCHK(c, launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a[0], c->val, nullptr, nullptr, 0));
+                HIPCHK(c, launch_jacobian_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a[0], c->val));
                 c->jac_valid = true;
                 return FP_OK;
             case 2:

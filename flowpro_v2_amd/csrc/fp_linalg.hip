@@ -15,6 +15,8 @@
 
 #include <cstdint>
 
+#include "fp_common.hpp"
+
 namespace fp {
 
 // ------------------------------------------------------------------ SpMV
@@ -49,19 +51,27 @@ __global__ void k_ilu_copy(size_t n, const double* __restrict__ src, TF* __restr
 }
 
 // ------------------------------------------------------------------ ILU factor
-// One workgroup per block row of the level.  LDS: three B x B tiles.
+// One workgroup (256 threads) per block row of the level; B is a compile-time constant
+// so that the block products unroll.  LDS: three B x B tiles.
 template <typename TF>
-__global__ void __launch_bounds__(256) k_ilu_factor_level(int B, const int* __restrict__ rows, int nrows,
-                                                          const int* __restrict__ rowptr, const int* __restrict__ colidx,
-                                                          const int* __restrict__ diagpos, TF* __restrict__ F,
-                                                          int* __restrict__ status) {
-    extern __shared__ unsigned char smem_raw[];
-    TF* As = reinterpret_cast<TF*>(smem_raw);
-    TF* Ds = As + B * B;
-    TF* Ls = Ds + B * B;
+__device__ inline void wave_argmax(TF& v, int& idx) {
+    for (int off = 32; off > 0; off >>= 1) {
+        TF v2 = __shfl_xor(v, off);
+        int i2 = __shfl_xor(idx, off);
+        if (v2 > v || (v2 == v && i2 < idx)) { v = v2; idx = i2; }
+    }
+}
+
+template <typename TF, int B>
+__global__ void __launch_bounds__(256) k_ilu_factor_level(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
+                                                          const int* __restrict__ colidx, const int* __restrict__ diagpos,
+                                                          TF* __restrict__ F, int* __restrict__ status) {
+    constexpr int BB = B * B;
+    __shared__ TF As[BB];
+    __shared__ TF Ds[BB];
+    __shared__ TF Ls[BB > 3 * B ? BB : 3 * B];
     __shared__ int piv;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int BB = B * B;
     if ((int)blockIdx.x >= nrows) return;
     const int i = rows[blockIdx.x];
     const int r0 = rowptr[i], r1 = rowptr[i + 1], di = diagpos[i];
@@ -72,15 +82,20 @@ __global__ void __launch_bounds__(256) k_ilu_factor_level(int B, const int* __re
         for (int q = tid; q < BB; q += nt) { As[q] = Aik[q]; Ds[q] = Dk[q]; }
         __syncthreads();
         // L = A_ik * D_k^-1   (column-major: X[c*B + r])
+        TF* Lg = F + (size_t)kk * BB;
         for (int q = tid; q < BB; q += nt) {
-            int r = q % B, c = q / B;
-            TF s = 0;
-            for (int m = 0; m < B; m++) s += As[m * B + r] * Ds[c * B + m];
-            Ls[q] = s;
+            const int r = q % B, c = q / B;
+            TF s0 = 0, s1 = 0;
+#pragma unroll
+            for (int m = 0; m + 1 < B; m += 2) {
+                s0 += As[m * B + r] * Ds[c * B + m];
+                s1 += As[(m + 1) * B + r] * Ds[c * B + m + 1];
+            }
+            if (B % 2) s0 += As[(B - 1) * B + r] * Ds[c * B + B - 1];
+            Ls[q] = s0 + s1;
+            Lg[q] = s0 + s1;
         }
         __syncthreads();
-        TF* Lg = F + (size_t)kk * BB;
-        for (int q = tid; q < BB; q += nt) Lg[q] = Ls[q];
         // A_ij -= L * U_kj for the blocks j > k present in both rows
         for (int jj = kk + 1; jj < r1; jj++) {
             const int j = colidx[jj];
@@ -93,16 +108,21 @@ __global__ void __launch_bounds__(256) k_ilu_factor_level(int B, const int* __re
             __syncthreads();
             TF* C = F + (size_t)jj * BB;
             for (int q = tid; q < BB; q += nt) {
-                int r = q % B, c = q / B;
-                TF s = 0;
-                for (int m = 0; m < B; m++) s += Ls[m * B + r] * As[c * B + m];
-                C[q] -= s;
+                const int r = q % B, c = q / B;
+                TF s0 = 0, s1 = 0;
+#pragma unroll
+                for (int m = 0; m + 1 < B; m += 2) {
+                    s0 += Ls[m * B + r] * As[c * B + m];
+                    s1 += Ls[(m + 1) * B + r] * As[c * B + m + 1];
+                }
+                if (B % 2) s0 += Ls[(B - 1) * B + r] * As[c * B + B - 1];
+                C[q] -= s0 + s1;
             }
             __syncthreads();
         }
-        __syncthreads();
     }
-    // D_i^-1 by Gauss-Jordan elimination with partial pivoting (Mat.invert, API/Mat.java:605-642)
+    // D_i^-1 by Gauss-Jordan elimination with partial pivoting (the reference inverts with
+    // Mat.invert, API/Mat.java:605-642)
     TF* M = As;
     TF* Inv = Ds;
     TF* Dg = F + (size_t)di * BB;
@@ -112,14 +132,15 @@ __global__ void __launch_bounds__(256) k_ilu_factor_level(int B, const int* __re
     }
     __syncthreads();
     for (int p = 0; p < B; p++) {
-        if (tid == 0) {
-            int best = p;
-            TF bv = fabs(M[p * B + p]);
-            for (int r = p + 1; r < B; r++) {
-                TF v = fabs(M[p * B + r]);
-                if (v > bv) { bv = v; best = r; }
+        if (tid < 64) {
+            TF v = -1;
+            int idx = p;
+            for (int r = p + tid; r < B; r += 64) {
+                TF a = fabs(M[p * B + r]);
+                if (a > v) { v = a; idx = r; }
             }
-            piv = best;
+            wave_argmax(v, idx);
+            if (tid == 0) piv = idx;
         }
         __syncthreads();
         const int pr = piv;
@@ -130,18 +151,24 @@ __global__ void __launch_bounds__(256) k_ilu_factor_level(int B, const int* __re
             }
         __syncthreads();
         const TF dinv = (TF)1 / M[p * B + p];
-        __syncthreads();
-        for (int c = tid; c < B; c += nt) { M[c * B + p] *= dinv; Inv[c * B + p] *= dinv; }
-        __syncthreads();
-        // eliminate column p from all other rows; Ls keeps the multipliers
+        // multipliers of column p (row p scaled below); Ls[B + c] keeps row p of M and Inv
         for (int r = tid; r < B; r += nt) Ls[r] = (r == p) ? (TF)0 : M[p * B + r];
+        for (int c = tid; c < B; c += nt) {
+            Ls[B + c] = M[c * B + p] * dinv;
+            Ls[2 * B + c] = Inv[c * B + p] * dinv;
+        }
         __syncthreads();
         for (int q = tid; q < BB; q += nt) {
-            int r = q % B, c = q / B;
-            TF f = Ls[r];
-            if (f != (TF)0) {
-                M[q] -= f * M[c * B + p];
-                Inv[q] -= f * Inv[c * B + p];
+            const int r = q % B, c = q / B;
+            if (r == p) {
+                M[q] = Ls[B + c];
+                Inv[q] = Ls[2 * B + c];
+            } else {
+                const TF f = Ls[r];
+                if (f != (TF)0) {
+                    M[q] -= f * Ls[B + c];
+                    Inv[q] -= f * Ls[2 * B + c];
+                }
             }
         }
         __syncthreads();
@@ -155,59 +182,256 @@ __global__ void __launch_bounds__(256) k_ilu_factor_level(int B, const int* __re
     if (bad) atomicOr(status, 1);
 }
 
-// ------------------------------------------------------------------ ILU sweeps
-// forward: y_i = b_i - sum_{k<i} L_ik y_k ; one 64-lane wave per row of the level
-template <typename TF>
-__global__ void __launch_bounds__(64) k_ilu_fwd_level(int B, const int* __restrict__ rows, int nrows,
-                                                      const int* __restrict__ rowptr, const int* __restrict__ colidx,
-                                                      const int* __restrict__ diagpos, const TF* __restrict__ F,
-                                                      const double* __restrict__ b, TF* __restrict__ y) {
+// One 64-lane wavefront per block row (B <= 64): lane r keeps row r of the blocks in
+// registers; the B x B operands it multiplies with are staged in LDS and read as
+// broadcasts.  Pivoting and elimination need wave-level barriers only.
+// sum_m x[m] * s[m] with s in LDS, read 16 bytes at a time when the length allows it
+template <typename TF, int B>
+__device__ __attribute__((always_inline)) inline TF row_dot(const TF (&x)[B], const TF* s) {
+    constexpr int V = 16 / sizeof(TF);
+    TF s0 = 0, s1 = 0;
+    if constexpr (B % V == 0 && V == 4) {
+#pragma unroll
+        for (int m = 0; m < B; m += 4) {
+            const float4 v = *reinterpret_cast<const float4*>(s + m);
+            s0 += x[m] * v.x + x[m + 2] * v.z;
+            s1 += x[m + 1] * v.y + x[m + 3] * v.w;
+        }
+    } else if constexpr (B % V == 0 && V == 2) {
+#pragma unroll
+        for (int m = 0; m < B; m += 2) {
+            const double2 v = *reinterpret_cast<const double2*>(s + m);
+            s0 += x[m] * v.x;
+            s1 += x[m + 1] * v.y;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < B; m++) s0 += x[m] * s[m];
+    }
+    return s0 + s1;
+}
+
+// value of v in lane `lane` (uniform) without LDS
+__device__ inline float lane_read(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ inline double lane_read(double v, int lane) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(x & 0xffffffff), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(x >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <typename TF, int B>
+__global__ void __launch_bounds__(64) k_ilu_factor_level_w(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
+                                                           const int* __restrict__ colidx, const int* __restrict__ diagpos,
+                                                           TF* __restrict__ F, int* __restrict__ status) {
+    constexpr int BB = B * B;
+    constexpr int PER = (BB + 63) / 64;   // staging loads per lane, all issued before the LDS stores
+    __shared__ __attribute__((aligned(16))) TF S[BB];
+    const int r = threadIdx.x;
+    const bool act = r < B;
     if ((int)blockIdx.x >= nrows) return;
     const int i = rows[blockIdx.x];
-    const size_t BB = (size_t)B * B;
-    for (int t = threadIdx.x; t < B; t += blockDim.x) {
-        TF s = (TF)b[(size_t)i * B + t];
-        for (int kk = rowptr[i]; kk < diagpos[i]; kk++) {
-            const TF* L = F + (size_t)kk * BB + t;
-            const TF* yk = y + (size_t)colidx[kk] * B;
-            TF sum = 0;
-            for (int c = 0; c < B; c++) sum += L[(size_t)c * B] * yk[c];
-            s -= sum;
+    const int r1 = rowptr[i + 1], di = diagpos[i];
+    TF a[B], l[B];
+    for (int kk = rowptr[i]; kk < di; kk++) {
+        const int k = colidx[kk];
+        TF* Aik = F + (size_t)kk * BB;
+        const TF* Dk = F + (size_t)diagpos[k] * BB;   // D_k^-1 (row k is factored)
+        if (act) {
+#pragma unroll
+            for (int m = 0; m < B; m++) a[m] = Aik[m * B + r];
         }
-        y[(size_t)i * B + t] = s;
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int q = r + 64 * u;
+            if (q < BB) S[q] = Dk[q];
+        }
+        __syncthreads();
+        // L = A_ik D_k^-1, row r: l[c] = sum_m a[m] Dinv[m][c]
+        if (act) {
+#pragma unroll
+            for (int c = 0; c < B; c++) l[c] = row_dot<TF, B>(a, S + c * B);
+#pragma unroll
+            for (int c = 0; c < B; c++) Aik[c * B + r] = l[c];
+        }
+        __syncthreads();
+        for (int jj = kk + 1; jj < r1; jj++) {
+            const int j = colidx[jj];
+            int kj = -1;
+            for (int s2 = rowptr[k]; s2 < rowptr[k + 1]; s2++)
+                if (colidx[s2] == j) kj = s2;
+            if (kj < 0) continue;
+            const TF* U = F + (size_t)kj * BB;
+            TF* C = F + (size_t)jj * BB;
+            if (act) {
+#pragma unroll
+                for (int c = 0; c < B; c++) a[c] = C[c * B + r];
+            }
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                const int q = r + 64 * u;
+                if (q < BB) S[q] = U[q];
+            }
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (int c = 0; c < B; c++) a[c] -= row_dot<TF, B>(l, S + c * B);
+#pragma unroll
+                for (int c = 0; c < B; c++) C[c * B + r] = a[c];
+            }
+            __syncthreads();
+        }
+    }
+    // D_i^-1: Gauss-Jordan with partial pivoting done implicitly (no row exchanges): at step p
+    // the unused row with the largest |M[.][p]| is normalised and eliminated from all other
+    // rows; the rows of the result are then the rows of the inverse in pivot order.  Lane r
+    // holds row r of M (in a) and of the identity (in l); the pivot row reaches the other
+    // lanes through readlane (uniform lane index), so the loop needs no LDS and no barrier.
+    TF* Dg = F + (size_t)di * BB;
+    if (act) {
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+            a[c] = Dg[c * B + r];
+            l[c] = (c == r) ? (TF)1 : (TF)0;
+        }
+    }
+    bool used = !act;
+    int dest = 0;
+    bool singular = false;
+#pragma unroll
+    for (int p = 0; p < B; p++) {
+        const float key = used ? -1.0f : (float)fabs(a[p]);
+        float kmax = key;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) kmax = fmaxf(kmax, __shfl_xor(kmax, off));
+        const unsigned long long cand = __ballot(!used && key == kmax);
+        int pr = cand ? (int)__ffsll((long long)cand) - 1 : 0;
+        pr = __builtin_amdgcn_readfirstlane(pr);
+        if (!(kmax > 0.0f)) singular = true;
+        const TF d = (TF)1 / lane_read(a[p], pr);
+        TF pa[B], pl[B];
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+            pa[c] = lane_read(a[c], pr) * d;
+            pl[c] = lane_read(l[c], pr) * d;
+        }
+        if (r == pr) {
+#pragma unroll
+            for (int c = 0; c < B; c++) { a[c] = pa[c]; l[c] = pl[c]; }
+            used = true;
+            dest = p;
+        } else if (act) {
+            const TF f = a[p];
+#pragma unroll
+            for (int c = 0; c < B; c++) { a[c] -= f * pa[c]; l[c] -= f * pl[c]; }
+        }
+    }
+    if (act) {
+        int bad = singular ? 1 : 0;
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+            if (!isfinite((double)l[c])) bad = 1;
+            Dg[c * B + dest] = l[c];
+        }
+        if (bad) atomicOr(status, 1);
     }
 }
 
+// ------------------------------------------------------------------ ILU sweeps
+// Several block rows per 64-lane workgroup when B < 64 (lane = row of a block).
+// The right-hand side and the result are passed through `io` so that the level
+// sequence can be captured once in a graph and replayed for any vectors.
+
+template <int B> struct RowsPerWave { static constexpr int value = B <= 32 ? 64 / B : 1; };
+
+// forward: y_i = b_i - sum_{k<i} L_ik y_k
+template <typename TF, int B>
+__global__ void __launch_bounds__(64) k_ilu_fwd_level(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
+                                                      const int* __restrict__ colidx, const int* __restrict__ diagpos,
+                                                      const TF* __restrict__ F, const IluIO* __restrict__ io,
+                                                      TF* __restrict__ y) {
+    constexpr int RPW = RowsPerWave<B>::value;
+    constexpr size_t BB = (size_t)B * B;
+    const int slot = threadIdx.x / B, t = threadIdx.x % B;
+    const int li = blockIdx.x * RPW + slot;
+    if (slot >= RPW || li >= nrows) return;
+    const int i = rows[li];
+    const double* b = io->b;
+    TF s = (TF)b[(size_t)i * B + t];
+    for (int kk = rowptr[i]; kk < diagpos[i]; kk++) {
+        const TF* L = F + (size_t)kk * BB + t;
+        const TF* yk = y + (size_t)colidx[kk] * B;
+        TF a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+        for (int c = 0; c + 3 < B; c += 4) {
+            a0 += L[(size_t)c * B] * yk[c];
+            a1 += L[(size_t)(c + 1) * B] * yk[c + 1];
+            a2 += L[(size_t)(c + 2) * B] * yk[c + 2];
+            a3 += L[(size_t)(c + 3) * B] * yk[c + 3];
+        }
+#pragma unroll
+        for (int c = B - B % 4; c < B; c++) a0 += L[(size_t)c * B] * yk[c];
+        s -= (a0 + a1) + (a2 + a3);
+    }
+    y[(size_t)i * B + t] = s;
+}
+
 // backward: x_i = D_i^-1 (y_i - sum_{j>i} U_ij x_j)
-template <typename TF>
-__global__ void __launch_bounds__(64) k_ilu_bwd_level(int B, const int* __restrict__ rows, int nrows,
-                                                      const int* __restrict__ rowptr, const int* __restrict__ colidx,
-                                                      const int* __restrict__ diagpos, const TF* __restrict__ F,
-                                                      const TF* __restrict__ y, TF* __restrict__ xs, double* __restrict__ x) {
-    extern __shared__ unsigned char smem_raw[];
-    TF* rhs = reinterpret_cast<TF*>(smem_raw);
-    if ((int)blockIdx.x >= nrows) return;
-    const int i = rows[blockIdx.x];
-    const size_t BB = (size_t)B * B;
-    for (int t = threadIdx.x; t < B; t += blockDim.x) {
+template <typename TF, int B>
+__global__ void __launch_bounds__(64) k_ilu_bwd_level(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
+                                                      const int* __restrict__ colidx, const int* __restrict__ diagpos,
+                                                      const TF* __restrict__ F, const TF* __restrict__ y, TF* __restrict__ xs,
+                                                      const IluIO* __restrict__ io) {
+    constexpr int RPW = RowsPerWave<B>::value;
+    constexpr size_t BB = (size_t)B * B;
+    __shared__ TF rhs[RPW * B];
+    const int slot = threadIdx.x / B, t = threadIdx.x % B;
+    const int li = blockIdx.x * RPW + slot;
+    const bool active = slot < RPW && li < nrows;
+    const int i = active ? rows[li] : 0;
+    if (active) {
         TF s = y[(size_t)i * B + t];
         for (int jj = diagpos[i] + 1; jj < rowptr[i + 1]; jj++) {
             const TF* U = F + (size_t)jj * BB + t;
             const TF* xj = xs + (size_t)colidx[jj] * B;
-            TF sum = 0;
-            for (int c = 0; c < B; c++) sum += U[(size_t)c * B] * xj[c];
-            s -= sum;
+            TF a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+            for (int c = 0; c + 3 < B; c += 4) {
+                a0 += U[(size_t)c * B] * xj[c];
+                a1 += U[(size_t)(c + 1) * B] * xj[c + 1];
+                a2 += U[(size_t)(c + 2) * B] * xj[c + 2];
+                a3 += U[(size_t)(c + 3) * B] * xj[c + 3];
+            }
+#pragma unroll
+            for (int c = B - B % 4; c < B; c++) a0 += U[(size_t)c * B] * xj[c];
+            s -= (a0 + a1) + (a2 + a3);
         }
-        rhs[t] = s;
+        rhs[slot * B + t] = s;
     }
     __syncthreads();
-    const TF* D = F + (size_t)diagpos[i] * BB;
-    for (int t = threadIdx.x; t < B; t += blockDim.x) {
-        TF s = 0;
-        for (int c = 0; c < B; c++) s += D[(size_t)c * B + t] * rhs[c];
-        xs[(size_t)i * B + t] = s;
-        x[(size_t)i * B + t] = (double)s;
+    if (!active) return;
+    const TF* D = F + (size_t)diagpos[i] * BB + t;
+    const TF* rr = rhs + slot * B;
+    TF a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+    for (int c = 0; c + 3 < B; c += 4) {
+        a0 += D[(size_t)c * B] * rr[c];
+        a1 += D[(size_t)(c + 1) * B] * rr[c + 1];
+        a2 += D[(size_t)(c + 2) * B] * rr[c + 2];
+        a3 += D[(size_t)(c + 3) * B] * rr[c + 3];
     }
+#pragma unroll
+    for (int c = B - B % 4; c < B; c++) a0 += D[(size_t)c * B] * rr[c];
+    const TF v = (a0 + a1) + (a2 + a3);
+    xs[(size_t)i * B + t] = v;
+    io->x[(size_t)i * B + t] = (double)v;
+}
+
+__global__ void k_set_io(IluIO* io, const double* b, double* x) {
+    io->b = b;
+    io->x = x;
 }
 
 // ------------------------------------------------------------------ block Jacobi
@@ -357,26 +581,54 @@ hipError_t launch_ilu_copy(hipStream_t s, size_t n, const double* src, TF* dst) 
     return hipGetLastError();
 }
 
+#define FP_BLOCK_SIZES(X) X(1) X(3) X(6) X(10) X(12) X(24) X(40)
+
 template <typename TF>
 hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* colidx,
                                    const int* diagpos, TF* F, int* status) {
-    size_t sm = 3 * (size_t)B * B * sizeof(TF);
-    hipLaunchKernelGGL(k_ilu_factor_level<TF>, dim3(nrows), dim3(256), sm, s, B, rows, nrows, rowptr, colidx, diagpos, F, status);
-    return hipGetLastError();
+#define X(BS)                                                                                                          \
+    if (B == BS) {                                                                                                     \
+        hipLaunchKernelGGL((k_ilu_factor_level_w<TF, BS>), dim3(nrows), dim3(64), 0, s, rows, nrows, rowptr, colidx, diagpos, \
+                           F, status);                                                                                 \
+        return hipGetLastError();                                                                                      \
+    }
+    FP_BLOCK_SIZES(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 template <typename TF>
 hipError_t launch_ilu_fwd_level(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* colidx,
-                                const int* diagpos, const TF* F, const double* b, TF* y) {
-    hipLaunchKernelGGL(k_ilu_fwd_level<TF>, dim3(nrows), dim3(64), 0, s, B, rows, nrows, rowptr, colidx, diagpos, F, b, y);
-    return hipGetLastError();
+                                const int* diagpos, const TF* F, const IluIO* io, TF* y) {
+#define X(BS)                                                                                                      \
+    if (B == BS) {                                                                                                 \
+        constexpr int R = RowsPerWave<BS>::value;                                                                  \
+        hipLaunchKernelGGL((k_ilu_fwd_level<TF, BS>), dim3((nrows + R - 1) / R), dim3(64), 0, s, rows, nrows, rowptr, \
+                           colidx, diagpos, F, io, y);                                                             \
+        return hipGetLastError();                                                                                  \
+    }
+    FP_BLOCK_SIZES(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 template <typename TF>
 hipError_t launch_ilu_bwd_level(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* colidx,
-                                const int* diagpos, const TF* F, const TF* y, TF* xs, double* x) {
-    hipLaunchKernelGGL(k_ilu_bwd_level<TF>, dim3(nrows), dim3(64), B * sizeof(TF), s, B, rows, nrows, rowptr, colidx, diagpos, F,
-                       y, xs, x);
+                                const int* diagpos, const TF* F, const TF* y, TF* xs, const IluIO* io) {
+#define X(BS)                                                                                                      \
+    if (B == BS) {                                                                                                 \
+        constexpr int R = RowsPerWave<BS>::value;                                                                  \
+        hipLaunchKernelGGL((k_ilu_bwd_level<TF, BS>), dim3((nrows + R - 1) / R), dim3(64), 0, s, rows, nrows, rowptr, \
+                           colidx, diagpos, F, y, xs, io);                                                         \
+        return hipGetLastError();                                                                                  \
+    }
+    FP_BLOCK_SIZES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_set_io(hipStream_t s, IluIO* io, const double* b, double* x) {
+    hipLaunchKernelGGL(k_set_io, dim3(1), dim3(1), 0, s, io, b, x);
     return hipGetLastError();
 }
 
@@ -385,9 +637,9 @@ hipError_t launch_ilu_bwd_level(hipStream_t s, int B, const int* rows, int nrows
     template hipError_t launch_ilu_factor_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, \
                                                     int*);                                                                     \
     template hipError_t launch_ilu_fwd_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*,        \
-                                                 const TF*, const double*, TF*);                                               \
+                                                 const TF*, const IluIO*, TF*);                                                \
     template hipError_t launch_ilu_bwd_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*,        \
-                                                 const TF*, const TF*, TF*, double*);
+                                                 const TF*, const TF*, TF*, const IluIO*);
 FP_ILU_INST(double)
 FP_ILU_INST(float)
 

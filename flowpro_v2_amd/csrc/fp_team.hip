@@ -1,0 +1,536 @@
+// Element-team kernels of the implicit DG step (residual and exact Jacobian assembly).
+//
+// A workgroup owns EPB elements.  Phase 0 stages the reference tables, the elements'
+// coefficients and their physical basis gradients in LDS; phase 1a interpolates the
+// state (own and neighbour side) at every volume and face quadrature point; phase 1b
+// evaluates the pointwise integrands (residual) or their derivatives with respect to
+// the 3*nEqs local inputs (W, dW/dx, dW/dy) with forward-mode dual numbers (Jacobian);
+// phase 2 contracts them with the test functions.  The arithmetic is the one of
+// DGpure.residuum / residuumWall (FlowPro/src/main/java/flowpro/core/element/
+// DGpure.java:108-355): faces first, then the volume; interior faces evaluated from
+// both sides; the neighbour-side face integrand (KR) gives the off-diagonal blocks
+// as BDF1.assembleJacobiMatrix does (BDF1.java:89-118), here exactly.
+#include <hip/hip_runtime.h>
+
+#include "fp_common.hpp"
+
+namespace fp {
+
+constexpr int TEAM_THREADS = 256;   // workgroup size of the team kernels
+
+template <int NB, int NEQ, int NQV, int NQF>
+struct TeamLayout {
+    static constexpr int B = NB * NEQ;
+    static constexpr int NQFT = 3 * NQF;                 // face points of an element
+    static constexpr int NQ = NQV + NQFT;
+    // shared tables
+    static constexpr int T_VB = 0;                       // [NQV][NB]
+    static constexpr int T_VW = T_VB + NQV * NB;         // [NQV]
+    static constexpr int T_FB = T_VW + NQV;              // [3*NQF][NB]
+    static constexpr int T_FW = T_FB + NQFT * NB;        // [NQF]
+    static constexpr int T_VD = T_FW + NQF;              // [NQV][NB][2] reference gradients
+    static constexpr int T_FD = T_VD + NQV * NB * 2;     // [3*NQF][NB][2]
+    static constexpr int T_M = T_FD + NQFT * NB * 2;     // [NB][NB]
+    static constexpr int T_MI = T_M + NB * NB;
+    static constexpr int T_FX = T_MI + NB * NB;          // [3*NQF][2]
+    static constexpr int T_VX = T_FX + NQFT * 2;         // [NQV][2]
+    static constexpr int TAB = T_VX + NQV * 2;
+    // per element
+    static constexpr int E_W = 0;                        // [B]
+    static constexpr int E_GV = E_W + B;                 // [NQV][NB][2] physical gradients (volume)
+    static constexpr int E_GF = E_GV + NQV * NB * 2;     // [3*NQF][NB][2] physical gradients (own face points)
+    static constexpr int E_SV = E_GF + NQFT * NB * 2;    // [NQV][12]  W, dW at volume points
+    static constexpr int E_SF = E_SV + NQV * 12;         // [3*NQF][24] WL, dWL, WR, dWR at face points
+    static constexpr int ELEM0 = E_SF + NQFT * 24;
+};
+
+// ------------------------------------------------------------------ phase 0 / 1a helpers
+template <int NB, int NEQ, int NQV, int NQF>
+__device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double* __restrict__ W, int e0, int nel, double* tab,
+                                  double* eb, int elemStride) {
+    using LY = TeamLayout<NB, NEQ, NQV, NQF>;
+    constexpr int B = LY::B;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    // one flat copy of the tables; the trip count is a compile-time constant so that all loads
+    // of a thread are in flight before its LDS stores
+    constexpr int NTAB = LY::TAB;
+    constexpr int PER = (NTAB + TEAM_THREADS - 1) / TEAM_THREADS;
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int q = tid + TEAM_THREADS * u;
+        if (q < NTAB) {
+            double v;
+            if (q < LY::T_VW) v = tb.vb[q - LY::T_VB];
+            else if (q < LY::T_FB) v = tb.vw[q - LY::T_VW];
+            else if (q < LY::T_FW) v = tb.fb[q - LY::T_FB];
+            else if (q < LY::T_VD) v = tb.fw[q - LY::T_FW];
+            else if (q < LY::T_FD) v = tb.vd[q - LY::T_VD];
+            else if (q < LY::T_M) v = tb.fd[q - LY::T_FD];
+            else if (q < LY::T_MI) v = tb.mref[q - LY::T_M];
+            else if (q < LY::T_FX) v = tb.mrefinv[q - LY::T_MI];
+            else if (q < LY::T_VX) v = tb.fx[q - LY::T_FX];
+            else v = tb.vx[q - LY::T_VX];
+            tab[q] = v;
+        }
+    }
+    for (int q = tid; q < nel * B; q += nt) {
+        const int el = q / B, i = q % B;
+        eb[el * elemStride + LY::E_W + i] = W[(size_t)(e0 + el) * B + i];
+    }
+    __syncthreads();
+    // physical gradients d_x = iT^T d_xi (Transformation.transformBasis)
+    constexpr int NG = (NQV + LY::NQFT) * NB;
+    for (int q = tid; q < nel * NG; q += nt) {
+        const int el = q / NG, r = q % NG;
+        const double* g = ms.geo + (size_t)(e0 + el) * GEO;
+        const double* dxi = r < NQV * NB ? &tab[LY::T_VD + 2 * r] : &tab[LY::T_FD + 2 * (r - NQV * NB)];
+        double* out = r < NQV * NB ? &eb[el * elemStride + LY::E_GV + 2 * r] : &eb[el * elemStride + LY::E_GF + 2 * (r - NQV * NB)];
+        out[0] = g[0] * dxi[0] + g[2] * dxi[1];
+        out[1] = g[1] * dxi[0] + g[3] * dxi[1];
+    }
+    __syncthreads();
+    // states at the quadrature points (own side from LDS, neighbour side from HBM)
+    for (int q = tid; q < nel * LY::NQ; q += nt) {
+        const int el = q / LY::NQ, qp = q % LY::NQ;
+        double* E = eb + el * elemStride;
+        const double* w = E + LY::E_W;
+        if (qp < NQV) {
+            const double* phi = &tab[LY::T_VB + qp * NB];
+            const double* gr = &E[LY::E_GV + qp * NB * 2];
+            double* st = &E[LY::E_SV + qp * 12];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) {
+                double s = 0, sx = 0, sy = 0;
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const double c = w[m * NB + j];
+                    s += c * phi[j];
+                    sx += c * gr[2 * j];
+                    sy += c * gr[2 * j + 1];
+                }
+                st[m] = s;
+                st[4 + m] = sx;
+                st[8 + m] = sy;
+            }
+        } else {
+            const int f = qp - NQV, k = f / NQF, p = f % NQF;
+            const int e = e0 + el;
+            const double* phi = &tab[LY::T_FB + f * NB];
+            const double* gr = &E[LY::E_GF + f * NB * 2];
+            double* st = &E[LY::E_SF + f * 24];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) {
+                double s = 0, sx = 0, sy = 0;
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const double c = w[m * NB + j];
+                    s += c * phi[j];
+                    sx += c * gr[2 * j];
+                    sy += c * gr[2 * j + 1];
+                }
+                st[m] = s;
+                st[4 + m] = sx;
+                st[8 + m] = sy;
+            }
+            const int TT = ms.nbr[e * 3 + k];
+            if (TT > -1) {
+                const int kr = ms.nbrf[e * 3 + k];
+                const int pr = NQF - 1 - p;
+                const double* gn = ms.geo + (size_t)TT * GEO;
+                const double* phR = &tab[LY::T_FB + (kr * NQF + pr) * NB];
+                const double* dR = &tab[LY::T_FD + (kr * NQF + pr) * NB * 2];
+                const double* wn = W + (size_t)TT * B;
+                double wv[B];
+#pragma unroll
+                for (int q2 = 0; q2 < B; q2++) wv[q2] = wn[q2];
+#pragma unroll
+                for (int m = 0; m < NEQ; m++) {
+                    double s = 0, sx = 0, sy = 0;
+#pragma unroll
+                    for (int j = 0; j < NB; j++) {
+                        const double c = wv[m * NB + j];
+                        const double gx = gn[0] * dR[2 * j] + gn[2] * dR[2 * j + 1];
+                        const double gy = gn[1] * dR[2 * j] + gn[3] * dR[2 * j + 1];
+                        s += c * phR[j];
+                        sx += c * gx;
+                        sy += c * gy;
+                    }
+                    st[12 + m] = s;
+                    st[16 + m] = sx;
+                    st[20 + m] = sy;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// pointwise face integrand g (the reference's K update per face point divided by the
+// test function): -fn + fvn - cIP [W] for the given own-side state
+template <int MK, int NEQ, bool VISC, class T>
+__device__ inline void face_integrand(const ModelParams& P, const DGParams& G, int TT, const double* n, double dL, const double* X,
+                                      T* WL, T* dWL, const double* st, T* g) {
+    T WR[4], dWR[8];
+    if (TT > -1) {
+#pragma unroll
+        for (int m = 0; m < NEQ; m++) { WR[m] = st[12 + m]; dWR[m] = st[16 + m]; dWR[4 + m] = st[20 + m]; }
+    } else {
+        boundary_value<MK>(P, WL, n, TT, X, WR);
+#pragma unroll
+        for (int m = 0; m < NEQ; m++) { dWR[m] = dWL[m]; dWR[4 + m] = dWL[4 + m]; }
+    }
+    T fn[4];
+    numerical_convective_flux<MK>(P, WL, WR, n, TT, fn);
+#pragma unroll
+    for (int m = 0; m < NEQ; m++) g[m] = -fn[m];
+    if constexpr (VISC) {
+        T Wc[4], dWc[8], fvn[4];
+#pragma unroll
+        for (int m = 0; m < NEQ; m++) {
+            Wc[m] = TT > -1 ? (WL[m] + WR[m]) / 2.0 : WR[m];
+            T jump = G.beta0 * (WR[m] - WL[m]) / dL;
+            dWc[m] = (dWL[m] + dWR[m]) / 2.0 + jump * n[0];
+            dWc[4 + m] = (dWL[4 + m] + dWR[4 + m]) / 2.0 + jump * n[1];
+        }
+        numerical_diffusive_flux<MK>(P, Wc, dWc, n, TT, fvn);
+        const bool ip = TT > -1 || is_ip_face<MK>(TT);
+#pragma unroll
+        for (int m = 0; m < NEQ; m++) {
+            g[m] += fvn[m];
+            if (ip) g[m] -= G.cIP * (WL[m] - WR[m]);
+        }
+    }
+}
+
+// pointwise volume fluxes (convective minus viscous) in x and y
+template <int MK, int NB, int NEQ, bool VISC, class T>
+__device__ inline void volume_integrand(const ModelParams& P, T* Wq, T* dWq, T* fx, T* fy) {
+    if constexpr (NB > 1) {
+        const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
+        convective_flux<MK>(P, Wq, ex, fx);
+        convective_flux<MK>(P, Wq, ey, fy);
+        if constexpr (VISC) {
+            T gx[4], gy[4];
+            diffusive_flux<MK>(P, Wq, dWq, ex, gx);
+            diffusive_flux<MK>(P, Wq, dWq, ey, gy);
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) { fx[m] -= gx[m]; fy[m] -= gy[m]; }
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < NEQ; m++) { fx[m] = 0.0; fy[m] = 0.0; }
+    }
+}
+
+template <int NB, int NEQ, int NQV, int NQF, int EXTRA>
+struct TeamEPB {
+    using LY = TeamLayout<NB, NEQ, NQV, NQF>;
+    static constexpr int PER = LY::ELEM0 + EXTRA;
+    static constexpr int BUDGET = 60 * 1024 / 8;   // doubles of LDS per workgroup
+    static constexpr int E0 = (BUDGET - LY::TAB) / PER;
+    static constexpr int value = E0 < 1 ? 1 : (E0 > 16 ? 16 : E0);
+};
+
+// ------------------------------------------------------------------ residual
+template <int MK, int NB, int NEQ, int NQV, int NQF, bool VISC>
+__global__ void __launch_bounds__(256) k_residual_team(ModelParams P, DGParams G, Tables tb, Mesh ms, const double* __restrict__ W,
+                                                       const double* __restrict__ Wo, const double* __restrict__ Wo2, double a1,
+                                                       double a2, double a3, double* __restrict__ R, double* __restrict__ RHS,
+                                                       double* __restrict__ enorm) {
+    using LY = TeamLayout<NB, NEQ, NQV, NQF>;
+    constexpr int B = LY::B;
+    constexpr int EXTRA = NQV * 2 * NEQ + LY::NQFT * NEQ + 2 * B;   // integrands, K, RHS
+    constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EXTRA>::value;
+    constexpr int STRIDE = LY::ELEM0 + EXTRA;
+    __shared__ double tab[LY::TAB];
+    __shared__ double eb[EPB * STRIDE];
+    const int e0 = blockIdx.x * EPB;
+    const int nel = min(EPB, ms.nElem - e0);
+    if (nel <= 0) return;
+    team_stage<NB, NEQ, NQV, NQF>(tb, ms, W, e0, nel, tab, eb, STRIDE);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    constexpr int O_IV = LY::ELEM0, O_IF = O_IV + NQV * 2 * NEQ, O_K = O_IF + LY::NQFT * NEQ, O_H = O_K + B;
+    // phase 1b: integrands at every quadrature point, weighted by J w
+    for (int q = tid; q < nel * LY::NQ; q += nt) {
+        const int el = q / LY::NQ, qp = q % LY::NQ, e = e0 + el;
+        double* E = eb + el * STRIDE;
+        const double* ge = ms.geo + (size_t)e * GEO;
+        if (qp < NQV) {
+            const double* st = &E[LY::E_SV + qp * 12];
+            double Wq[4], dWq[8], fx[4], fy[4];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) { Wq[m] = st[m]; dWq[m] = st[4 + m]; dWq[4 + m] = st[8 + m]; }
+            volume_integrand<MK, NB, NEQ, VISC>(P, Wq, dWq, fx, fy);
+            const double jw = ge[4] * tab[LY::T_VW + qp];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) {
+                E[O_IV + (qp * 2) * NEQ + m] = jw * fx[m];
+                E[O_IV + (qp * 2 + 1) * NEQ + m] = jw * fy[m];
+            }
+        } else {
+            const int f = qp - NQV, k = f / NQF, p = f % NQF;
+            const int TT = ms.nbr[e * 3 + k];
+            const double* fg = ms.fgeo + ((size_t)e * 3 + k) * 4;
+            const double n[2] = {fg[0], fg[1]};
+            const double* st = &E[LY::E_SF + f * 24];
+            double WL[4], dWL[8], g[4];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) { WL[m] = st[m]; dWL[m] = st[4 + m]; dWL[4 + m] = st[8 + m]; }
+            const double* xi = &tab[LY::T_FX + 2 * f];
+            double X[2] = {ge[5] * xi[0] + ge[6] * xi[1] + ge[7], ge[8] * xi[0] + ge[9] * xi[1] + ge[10]};
+            face_integrand<MK, NEQ, VISC>(P, G, TT, n, fg[3], X, WL, dWL, st, g);
+            const double jw = fg[2] * tab[LY::T_FW + p];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) E[O_IF + f * NEQ + m] = jw * g[m];
+        }
+    }
+    __syncthreads();
+    // phase 2: test-function contraction, faces first (DGpure.residuum order), then RHS
+    for (int q = tid; q < nel * B; q += nt) {
+        const int el = q / B, i = q % B, m = i / NB, j = i % NB, e = e0 + el;
+        double* E = eb + el * STRIDE;
+        double K = 0.0;
+        for (int f = 0; f < LY::NQFT; f++) K += tab[LY::T_FB + f * NB + j] * E[O_IF + f * NEQ + m];
+        if constexpr (NB > 1) {
+            for (int qv = 0; qv < NQV; qv++)
+                K += E[O_IV + (qv * 2) * NEQ + m] * E[LY::E_GV + (qv * NB + j) * 2] +
+                     E[O_IV + (qv * 2 + 1) * NEQ + m] * E[LY::E_GV + (qv * NB + j) * 2 + 1];
+        }
+        if constexpr (MK == MODEL_SCALAR) {
+            const double* ge = ms.geo + (size_t)e * GEO;
+            for (int qv = 0; qv < NQV; qv++) {
+                const double* xi = &tab[LY::T_VX + 2 * qv];
+                double X[2] = {ge[5] * xi[0] + ge[6] * xi[1] + ge[7], ge[8] * xi[0] + ge[9] * xi[1] + ge[10]};
+                double src[1];
+                source_term(P, X, src);
+                K += ge[4] * tab[LY::T_VW + qv] * src[0] * tab[LY::T_VB + qv * NB + j];
+            }
+        }
+        const double jac = ms.geo[(size_t)e * GEO + 4];
+        double s = 0.0;
+        for (int jj = 0; jj < NB; jj++) {
+            const size_t o = (size_t)e * B + m * NB + jj;
+            const double mij = jac * tab[LY::T_M + j * NB + jj];
+            s += mij * a1 * E[LY::E_W + m * NB + jj] + mij * a2 * Wo[o] + mij * a3 * Wo2[o];
+        }
+        R[(size_t)e * B + i] = K;
+        RHS[(size_t)e * B + i] = K - s;
+        E[O_K + i] = K;
+        E[O_H + i] = K - s;
+    }
+    if (!enorm) return;
+    __syncthreads();
+    // per-element R_m^T M^-1 R_m and RHS_m^T M^-1 RHS_m (Element.l2NormsSquared)
+    for (int q = tid; q < nel * NEQ * 2; q += nt) {
+        const int el = q / (NEQ * 2), c = q % (NEQ * 2), m = c % NEQ, which = c / NEQ, e = e0 + el;
+        const double* v = eb + el * STRIDE + (which ? O_H : O_K) + m * NB;
+        double sum = 0.0;
+        for (int a = 0; a < NB; a++) {
+            double t = 0.0;
+            for (int b = 0; b < NB; b++) t += tab[LY::T_MI + a * NB + b] * v[b];
+            sum += v[a] * t;
+        }
+        enorm[(size_t)e * 2 * NEQ + c] = sum / ms.geo[(size_t)e * GEO + 4];
+    }
+}
+
+// ------------------------------------------------------------------ Jacobian
+// NIN local inputs per point: the value (and with viscosity the x and y derivatives) of
+// every equation.  Phase 1b stores d(J w * integrand)/d(input) for every point and input;
+// phase 2 thread (element, column i = (m', j')) builds column i of the diagonal block and
+// of the three neighbour blocks.
+template <int MK, int NB, int NEQ, int NQV, int NQF, bool VISC, typename BT>
+__global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G, Tables tb, Mesh ms, const double* __restrict__ W,
+                                                       double a1, BT* __restrict__ blocks) {
+    using LY = TeamLayout<NB, NEQ, NQV, NQF>;
+    constexpr int B = LY::B;
+    constexpr int NIN = VISC ? 3 * NEQ : NEQ;
+    constexpr int EXTRA = NQV * NIN * 2 * NEQ + LY::NQFT * NIN * NEQ;
+    constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EXTRA>::value;
+    constexpr int STRIDE = LY::ELEM0 + EXTRA;
+    constexpr int O_DV = LY::ELEM0, O_DF = O_DV + NQV * NIN * 2 * NEQ;
+    __shared__ double tab[LY::TAB];
+    __shared__ double eb[EPB * STRIDE];
+    const int e0 = blockIdx.x * EPB;
+    const int nel = min(EPB, ms.nElem - e0);
+    if (nel <= 0) return;
+    team_stage<NB, NEQ, NQV, NQF>(tb, ms, W, e0, nel, tab, eb, STRIDE);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    // phase 1b: tangents of the integrands, one (point, input) pair per task
+    for (int q = tid; q < nel * LY::NQ * NIN; q += nt) {
+        const int el = q / (LY::NQ * NIN), r = q % (LY::NQ * NIN), qp = r / NIN, c = r % NIN, e = e0 + el;
+        const int s = c / NEQ, mp = c % NEQ;
+        double* E = eb + el * STRIDE;
+        const double* ge = ms.geo + (size_t)e * GEO;
+        if (qp < NQV) {
+            const double* st = &E[LY::E_SV + qp * 12];
+            dual Wq[4], dWq[8], fx[4], fy[4];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) {
+                Wq[m] = dual(st[m], (s == 0 && m == mp) ? 1.0 : 0.0);
+                dWq[m] = dual(st[4 + m], (s == 1 && m == mp) ? 1.0 : 0.0);
+                dWq[4 + m] = dual(st[8 + m], (s == 2 && m == mp) ? 1.0 : 0.0);
+            }
+            volume_integrand<MK, NB, NEQ, VISC>(P, Wq, dWq, fx, fy);
+            const double jw = ge[4] * tab[LY::T_VW + qp];
+            double* out = &E[O_DV + ((qp * NIN + c) * 2) * NEQ];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) { out[m] = jw * fx[m].d; out[NEQ + m] = jw * fy[m].d; }
+        } else {
+            const int f = qp - NQV, k = f / NQF, p = f % NQF;
+            const int TT = ms.nbr[e * 3 + k];
+            const double* fg = ms.fgeo + ((size_t)e * 3 + k) * 4;
+            const double n[2] = {fg[0], fg[1]};
+            const double* st = &E[LY::E_SF + f * 24];
+            dual WL[4], dWL[8], g[4];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) {
+                WL[m] = dual(st[m], (s == 0 && m == mp) ? 1.0 : 0.0);
+                dWL[m] = dual(st[4 + m], (s == 1 && m == mp) ? 1.0 : 0.0);
+                dWL[4 + m] = dual(st[8 + m], (s == 2 && m == mp) ? 1.0 : 0.0);
+            }
+            const double* xi = &tab[LY::T_FX + 2 * f];
+            double X[2] = {ge[5] * xi[0] + ge[6] * xi[1] + ge[7], ge[8] * xi[0] + ge[9] * xi[1] + ge[10]};
+            face_integrand<MK, NEQ, VISC>(P, G, TT, n, fg[3], X, WL, dWL, st, g);
+            const double jw = fg[2] * tab[LY::T_FW + p];
+            double* out = &E[O_DF + (f * NIN + c) * NEQ];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) out[m] = jw * g[m].d;
+        }
+    }
+    __syncthreads();
+    // phase 2: thread (element, column i = (mp, jp))
+    const size_t BB = (size_t)B * B;
+    for (int q = tid; q < nel * B; q += nt) {
+        const int el = q / B, i = q % B, mp = i / NB, jp = i % NB, e = e0 + el;
+        const double* E = eb + el * STRIDE;
+        double acc[B];
+#pragma unroll
+        for (int o = 0; o < B; o++) acc[o] = 0.0;
+        for (int k = 0; k < 3; k++) {
+            const int blkn = ms.blkn[e * 3 + k];
+            const int kr = ms.nbrf[e * 3 + k];
+            double accR[B];
+#pragma unroll
+            for (int o = 0; o < B; o++) accR[o] = 0.0;
+            for (int p = 0; p < NQF; p++) {
+                const int f = k * NQF + p;
+                // t_m = sum_s d g_m / d u_(s, mp) * psi_s(jp)
+                const double psi0 = tab[LY::T_FB + f * NB + jp];
+                double t[4];
+#pragma unroll
+                for (int m = 0; m < NEQ; m++) t[m] = E[O_DF + (f * NIN + mp) * NEQ + m] * psi0;
+                if constexpr (VISC) {
+                    const double psx = E[LY::E_GF + (f * NB + jp) * 2], psy = E[LY::E_GF + (f * NB + jp) * 2 + 1];
+#pragma unroll
+                    for (int m = 0; m < NEQ; m++)
+                        t[m] += E[O_DF + (f * NIN + NEQ + mp) * NEQ + m] * psx + E[O_DF + (f * NIN + 2 * NEQ + mp) * NEQ + m] * psy;
+                }
+#pragma unroll
+                for (int m = 0; m < NEQ; m++)
+#pragma unroll
+                    for (int j = 0; j < NB; j++) acc[m * NB + j] += tab[LY::T_FB + f * NB + j] * t[m];
+                if (blkn >= 0) {
+                    const double* phR = &tab[LY::T_FB + (kr * NQF + (NQF - 1 - p)) * NB];
+#pragma unroll
+                    for (int m = 0; m < NEQ; m++)
+#pragma unroll
+                        for (int j = 0; j < NB; j++) accR[m * NB + j] += phR[j] * t[m];
+                }
+            }
+            if (blkn >= 0) {
+                BT* col = blocks + (size_t)blkn * BB + (size_t)i * B;
+#pragma unroll
+                for (int o = 0; o < B; o++) col[o] = (BT)accR[o];
+            }
+        }
+        if constexpr (NB > 1) {
+            for (int qv = 0; qv < NQV; qv++) {
+                const double psi0 = tab[LY::T_VB + qv * NB + jp];
+                const double psx = E[LY::E_GV + (qv * NB + jp) * 2], psy = E[LY::E_GV + (qv * NB + jp) * 2 + 1];
+                double tx[4], ty[4];
+#pragma unroll
+                for (int m = 0; m < NEQ; m++) {
+                    const double* d0 = &E[O_DV + ((qv * NIN + mp) * 2) * NEQ];
+                    tx[m] = d0[m] * psi0;
+                    ty[m] = d0[NEQ + m] * psi0;
+                    if constexpr (VISC) {
+                        const double* d1 = &E[O_DV + ((qv * NIN + NEQ + mp) * 2) * NEQ];
+                        const double* d2 = &E[O_DV + ((qv * NIN + 2 * NEQ + mp) * 2) * NEQ];
+                        tx[m] += d1[m] * psx + d2[m] * psy;
+                        ty[m] += d1[NEQ + m] * psx + d2[NEQ + m] * psy;
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const double gx = E[LY::E_GV + (qv * NB + j) * 2], gy = E[LY::E_GV + (qv * NB + j) * 2 + 1];
+#pragma unroll
+                    for (int m = 0; m < NEQ; m++) acc[m * NB + j] += tx[m] * gx + ty[m] * gy;
+                }
+            }
+        }
+        const double jac = ms.geo[(size_t)e * GEO + 4];
+        BT* col = blocks + (size_t)ms.blkd[e] * BB + (size_t)i * B;
+#pragma unroll
+        for (int o = 0; o < B; o++) {
+            double v = -acc[o];
+            if (o / NB == mp) v += a1 * (jac * tab[LY::T_M + jp * NB + (o % NB)]);
+            col[o] = (BT)v;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ dispatch
+#define FP_TEAM_CASES(X)                          \
+    X(MODEL_NAVIER_STOKES, 3, 4, 6, 2, false)     \
+    X(MODEL_NAVIER_STOKES, 6, 4, 7, 3, false)     \
+    X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, false)   \
+    X(MODEL_NAVIER_STOKES, 3, 4, 6, 2, true)      \
+    X(MODEL_NAVIER_STOKES, 6, 4, 7, 3, true)      \
+    X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, true)    \
+    X(MODEL_SCALAR, 1, 1, 1, 1, false)            \
+    X(MODEL_SCALAR, 3, 1, 6, 2, false)            \
+    X(MODEL_SCALAR, 6, 1, 7, 3, false)            \
+    X(MODEL_SCALAR, 10, 1, 16, 4, false)          \
+    X(MODEL_SCALAR, 3, 1, 6, 2, true)             \
+    X(MODEL_SCALAR, 6, 1, 7, 3, true)             \
+    X(MODEL_SCALAR, 10, 1, 16, 4, true)
+
+#define FP_TMATCH(MK, NB, NEQ, NQV, NQF, V) \
+    (L.model == MK && L.nb == NB && L.neq == NEQ && L.nqv == NQV && L.nqf == NQF && (bool)L.visc == V)
+
+hipError_t launch_residual_team(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const Tables& tb,
+                                const Mesh& ms, const double* W, const double* Wo, const double* Wo2, double a1, double a2,
+                                double a3, double* R, double* RHS, double* enorm) {
+#define X(MK, NB, NEQ, NQV, NQF, V)                                                                                   \
+    if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V)) {                                                                         \
+        using LY = TeamLayout<NB, NEQ, NQV, NQF>;                                                                      \
+        constexpr int EX = NQV * 2 * NEQ + LY::NQFT * NEQ + 2 * LY::B;                                                 \
+        constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EX>::value;                                                     \
+        hipLaunchKernelGGL((k_residual_team<MK, NB, NEQ, NQV, NQF, V>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), 0, s, P, \
+                           G, tb, ms, W, Wo, Wo2, a1, a2, a3, R, RHS, enorm);                                          \
+        return hipGetLastError();                                                                                      \
+    }
+    FP_TEAM_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_jacobian_team(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const Tables& tb,
+                                const Mesh& ms, const double* W, double a1, double* blocks) {
+#define X(MK, NB, NEQ, NQV, NQF, V)                                                                                   \
+    if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V)) {                                                                         \
+        using LY = TeamLayout<NB, NEQ, NQV, NQF>;                                                                      \
+        constexpr int NIN = V ? 3 * NEQ : NEQ;                                                                         \
+        constexpr int EX = NQV * NIN * 2 * NEQ + LY::NQFT * NIN * NEQ;                                                 \
+        constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EX>::value;                                                     \
+        hipLaunchKernelGGL((k_jacobian_team<MK, NB, NEQ, NQV, NQF, V, double>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), \
+                           0, s, P, G, tb, ms, W, a1, blocks);                                                         \
+        return hipGetLastError();                                                                                      \
+    }
+    FP_TEAM_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+}  // namespace fp
