@@ -551,4 +551,67 @@ hipError_t launch_positivity(const ElemLaunch& L, hipStream_t s, const Tables& t
     return hipErrorInvalidValue;
 }
 
+// ------------------------------------------------------------------ functionals
+// DragCoefficient / LiftCoefficient (ModuleProjects/FunctionalsOfInterest/src/main/java/flowpro/
+// user/functional/DragCoefficient.java:31-48, LiftCoefficient.java:32-34): I = sum over WALL
+// faces of int p(W_L) n_which, p = (kappa - 1)(E - |m|^2 / (2 rho)) without density clipping,
+// as Element.computeFunctional integrates it (Element.java:789-862).  Also the exact gradient
+// dI/dW in the element layout.
+template <int NB, int NQF>
+__global__ void k_functional(Tables tb, Mesh ms, const double* __restrict__ W, double kappa, int which,
+                             double* __restrict__ ival, double* __restrict__ grad) {
+    constexpr int NEQ = 4, B = NB * NEQ;
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ms.nElem) return;
+    double g[B];
+#pragma unroll
+    for (int i = 0; i < B; i++) g[i] = 0.0;
+    double I = 0.0;
+    for (int k = 0; k < 3; k++) {
+        if (ms.nbr[e * 3 + k] != BT_WALL) continue;
+        const double* fg = ms.fgeo + ((size_t)e * 3 + k) * 4;
+        const double nk = fg[which];
+        for (int p = 0; p < NQF; p++) {
+            const double* phi = tb.fb + (k * NQF + p) * NB;
+            double w[4];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) {
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; j++) s += W[(size_t)e * B + m * NB + j] * phi[j];
+                w[m] = s;
+            }
+            const double mom2 = w[1] * w[1] + w[2] * w[2];
+            const double pr = (kappa - 1) * (w[3] - mom2 / (2 * w[0]));
+            const double jw = fg[2] * tb.fw[p];
+            I += jw * pr * nk;
+            const double dp[4] = {(kappa - 1) * mom2 / (2 * w[0] * w[0]), -(kappa - 1) * w[1] / w[0], -(kappa - 1) * w[2] / w[0],
+                                  kappa - 1};
+#pragma unroll
+            for (int m = 0; m < NEQ; m++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) g[m * NB + j] += jw * nk * dp[m] * phi[j];
+        }
+    }
+    ival[e] = I;
+    if (grad) {
+#pragma unroll
+        for (int i = 0; i < B; i++) grad[(size_t)e * B + i] = g[i];
+    }
+}
+
+hipError_t launch_functional(const ElemLaunch& L, hipStream_t s, const Tables& tb, const Mesh& ms, const double* W, double kappa,
+                             int which, double* ival, double* grad) {
+    if (L.model != MODEL_NAVIER_STOKES) return hipErrorInvalidValue;
+    dim3 blk(128), grd((ms.nElem + 127) / 128);
+#define X(MK, NB, NEQ, NQV, NQF, V)                                                                          \
+    if (MK == MODEL_NAVIER_STOKES && FP_MATCH(MK, NB, NEQ, NQV, NQF, V)) {                                    \
+        hipLaunchKernelGGL((k_functional<NB, NQF>), grd, blk, 0, s, tb, ms, W, kappa, which, ival, grad);   \
+        return hipGetLastError();                                                                             \
+    }
+    FP_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 }  // namespace fp

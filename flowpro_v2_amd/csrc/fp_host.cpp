@@ -33,6 +33,7 @@ hipError_t launch_jvp(const ElemLaunch&, hipStream_t, const ModelParams&, const 
                       const double*, const double*, double, double*);
 hipError_t launch_size_over_lambda(const ElemLaunch&, hipStream_t, const ModelParams&, const Tables&, const Mesh&,
                                    const double*, const double*, const double*, double*);
+hipError_t launch_functional(const ElemLaunch&, hipStream_t, const Tables&, const Mesh&, const double*, double, int, double*, double*);
 hipError_t launch_positivity(const ElemLaunch&, hipStream_t, const Tables&, const Mesh&, double*, const double*, int*);
 hipError_t launch_spmv(hipStream_t, int, int, const int*, const int*, const double*, const double*, const double*, double*, int);
 template <typename TF> hipError_t launch_ilu_copy(hipStream_t, size_t, const double*, TF*);
@@ -98,6 +99,9 @@ struct fp_ctx {
     Mesh ms{};
     bool jac_valid = false;
     double last_a1 = 0;
+    double kappa = 1.4;
+    int op = 0;          // GMRES operator: 0 assembled matrix, 1 matrix-free J, 2 matrix-free J^T
+    double op_a1 = 0;
 };
 
 #define HIPCHK(ctx, expr)                                                                                   \
@@ -375,6 +379,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     P.diffusive = diffusive;
     P.has_attack = cs->has_attack;
     P.kapa = cs->kappa;
+    c->kappa = cs->kappa;
     P.pOut = cs->p_out;
     P.Re = cs->reynolds;
     P.Pr = cs->prandtl;
@@ -868,6 +873,37 @@ static std::vector<double> lsolve(std::vector<std::vector<double>> A, std::vecto
     return x;
 }
 
+// y = A v with the current operator
+static int op_apply(fp_ctx* c, const double* v, double* y) {
+    if (c->op == 0) {
+        HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, v, nullptr, y, 0));
+    } else if (c->op == 1) {
+        HIPCHK(c, launch_jvp(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, v, c->op_a1, y));
+    } else {
+        HIPCHK(c, launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->op_a1, nullptr, v, y, 1));
+    }
+    return FP_OK;
+}
+// y = b - A x
+static int op_residual(fp_ctx* c, const double* x, const double* b, double* y) {
+    if (c->op == 0) {
+        HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, x, b, y, 1));
+        return FP_OK;
+    }
+    int rc = op_apply(c, x, c->tmp);
+    if (rc) return rc;
+    HIPCHK(c, launch_copy(c->s, (size_t)c->dofs, b, y));
+    HIPCHK(c, launch_axpy(c->s, (size_t)c->dofs, -1.0, c->tmp, y));
+    return FP_OK;
+}
+static int op_prec(fp_ctx* c, double* out, const double* in) {
+    if (c->op == 2) {   // the adjoint system is solved without preconditioner
+        HIPCHK(c, launch_copy(c->s, (size_t)c->dofs, in, out));
+        return FP_OK;
+    }
+    return prec_apply(c, out, in);
+}
+
 // Gmres.solve (FlowPro/src/main/java/flowpro/core/LinearSolvers/Gmres.java:42-122): left
 // preconditioned, restart m, modified Gram-Schmidt, residual relative to |M^-1 b|.
 static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_out, bool* ok) {
@@ -878,11 +914,11 @@ static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_
     std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
     std::vector<double> cs(m), sn(m), s(m + 1);
     double bnrm2, nr, error;
-    if ((rc = prec_apply(c, c->r, b))) return rc;
+    if ((rc = op_prec(c, c->r, b))) return rc;
     if ((rc = dnorm(c, c->r, &bnrm2))) return rc;
     if (bnrm2 == 0) bnrm2 = 1;
-    HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, x, b, c->aux, 1));
-    if ((rc = prec_apply(c, c->r, c->aux))) return rc;
+    if ((rc = op_residual(c, x, b, c->aux))) return rc;
+    if ((rc = op_prec(c, c->r, c->aux))) return rc;
     if ((rc = dnorm(c, c->r, &nr))) return rc;
     error = nr / bnrm2;
     if (error < c->tol) { *ok = true; *err_out = error; return FP_OK; }
@@ -899,8 +935,8 @@ static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_
         bool done = false;
         for (int i = 0; i < m; i++) {
             double* Vi = c->V + (size_t)i * N;
-            HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, Vi, nullptr, c->aux, 0));
-            if ((rc = prec_apply(c, c->w, c->aux))) return rc;
+            if ((rc = op_apply(c, Vi, c->aux))) return rc;
+            if ((rc = op_prec(c, c->w, c->aux))) return rc;
             (*iters)++;
             // MGS: h_k = (w, V_k); w -= h_k V_k  (the next dot is fused into the update)
             HIPCHK(c, launch_dot(c->s, N, c->w, c->V, c->partial, hdev, 0));
@@ -935,8 +971,8 @@ static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_
         HIPCHK(c, hipMemcpyAsync(c->y, yv.data(), sizeof(double) * nused, hipMemcpyHostToDevice, c->s));
         HIPCHK(c, launch_update_solution(c->s, N, nused, c->V, N, c->y, x));
         if (done) break;
-        HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, x, b, c->aux, 1));
-        if ((rc = prec_apply(c, c->r, c->aux))) return rc;
+        if ((rc = op_residual(c, x, b, c->aux))) return rc;
+        if ((rc = op_prec(c, c->r, c->aux))) return rc;
         if ((rc = dnorm(c, c->r, &nr))) return rc;
         error = nr / bnrm2;
         if (error <= c->tol) break;
@@ -947,10 +983,9 @@ static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_
     return FP_OK;
 }
 
-extern "C" int fp_linear_solve(fp_ctx* c, int* iterations, double* rel_residual) {
-    if (!c) return FP_EINVAL;
+static int linear_solve_impl(fp_ctx* c, int op, int* iterations, double* rel_residual) {
     if (!c->jac_valid) {
-        c->err = "fp_linear_solve before fp_assemble(with_jacobian=1)";
+        c->err = "linear solve before fp_assemble(with_jacobian=1)";
         return FP_EINVAL;
     }
     int rc = prec_factor(c);
@@ -968,7 +1003,10 @@ extern "C" int fp_linear_solve(fp_ctx* c, int* iterations, double* rel_residual)
     int it = 0;
     double err = 0;
     bool ok = false;
+    c->op = op;
+    c->op_a1 = c->last_a1;
     rc = gmres(c, c->RHS, c->x, &it, &err, &ok);
+    c->op = 0;
     if (rc) return rc;
     if (iterations) *iterations = it;
     if (rel_residual) *rel_residual = err;
@@ -978,6 +1016,47 @@ extern "C" int fp_linear_solve(fp_ctx* c, int* iterations, double* rel_residual)
     HIPCHK(c, hipStreamSynchronize(c->s));
     if (st) return FP_NONFINITE;
     return ok ? FP_OK : FP_NOT_CONVERGED;
+}
+
+extern "C" int fp_linear_solve(fp_ctx* c, int* iterations, double* rel_residual) {
+    if (!c) return FP_EINVAL;
+    return linear_solve_impl(c, 0, iterations, rel_residual);
+}
+
+extern "C" int fp_linear_solve_jfnk(fp_ctx* c, int* iterations, double* rel_residual) {
+    if (!c) return FP_EINVAL;
+    return linear_solve_impl(c, 1, iterations, rel_residual);
+}
+
+extern "C" int fp_adjoint_solve(fp_ctx* c, double a1, const double* rhs, double* lambda, int* iterations, double* rel_residual) {
+    if (!c || !rhs || !lambda || c->mp.kind != MODEL_NAVIER_STOKES) return FP_EINVAL;
+    int rc = to_device(c, rhs, c->RHS);
+    if (rc) return rc;
+    HIPCHK(c, launch_fill(c->s, (size_t)c->dofs, 0.0, c->x));
+    int it = 0;
+    double err = 0;
+    bool ok = false;
+    c->op = 2;
+    c->op_a1 = a1;
+    rc = gmres(c, c->RHS, c->x, &it, &err, &ok);
+    c->op = 0;
+    c->jac_valid = false;   // RHS was overwritten
+    if (rc) return rc;
+    if (iterations) *iterations = it;
+    if (rel_residual) *rel_residual = err;
+    rc = from_device(c, c->x, lambda);
+    if (rc) return rc;
+    return ok ? FP_OK : FP_NOT_CONVERGED;
+}
+
+extern "C" int fp_functional(fp_ctx* c, int which, double* value, double* dIdW) {
+    if (!c || !value || (which != 0 && which != 1) || c->mp.kind != MODEL_NAVIER_STOKES) return FP_EINVAL;
+    HIPCHK(c, launch_functional(c->L, c->s, c->tb, c->ms, c->W, c->kappa, which, c->tmp, dIdW ? c->w : nullptr));
+    HIPCHK(c, launch_dot(c->s, (size_t)c->nElem, c->tmp, nullptr, c->partial, c->scal, 2));
+    HIPCHK(c, hipMemcpyAsync(value, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (dIdW) return from_device(c, c->w, dIdW);
+    return FP_OK;
 }
 
 extern "C" int fp_update(fp_ctx* c, int* n_limited) {

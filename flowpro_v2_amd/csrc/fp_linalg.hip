@@ -492,7 +492,8 @@ __device__ inline double block_sum(double v) {
 __global__ void __launch_bounds__(RED_THREADS) k_dot_partial(size_t n, const double* __restrict__ a, const double* __restrict__ b,
                                                              double* __restrict__ partial) {
     double s = 0.0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) s += a[i] * b[i];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s += b ? a[i] * b[i] : a[i];
     s = block_sum(s);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
@@ -503,7 +504,7 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce_final(int np, const doub
     double s = 0.0;
     for (int i = threadIdx.x; i < np; i += blockDim.x) s += partial[i];
     s = block_sum(s);
-    if (threadIdx.x == 0) *out = op == 1 ? sqrt(s) : s;
+    if (threadIdx.x == 0) *out = op == 1 ? sqrt(s) : s;   // op 0, 2: the sum
 }
 
 // w -= h * v ; partial dot(w, u) (u may be null)

@@ -22,7 +22,8 @@ RECOVERABLE = {FP_NOT_CONVERGED: "GMRES did not converge", FP_SINGULAR_BLOCK: "s
 EXPORTS = ["fp_neighbor_matrix", "fp_rcm_order", "fp_create", "fp_destroy", "fp_last_error", "fp_info",
            "fp_element_order", "fp_set_state", "fp_get_state", "fp_next_level", "fp_restore_level", "fp_time_step",
            "fp_assemble", "fp_linear_solve", "fp_update", "fp_export_residual", "fp_export_jacobian",
-           "fp_export_solution", "fp_jv", "fp_jtv", "fp_time_kernel", "fp_stream"]
+           "fp_export_solution", "fp_jv", "fp_jtv", "fp_time_kernel", "fp_stream", "fp_linear_solve_jfnk",
+           "fp_functional", "fp_adjoint_solve"]
 
 
 class FpCase(C.Structure):
@@ -80,6 +81,9 @@ def lib() -> C.CDLL:
         "fp_jtv": (C.c_int, [vp, C.c_double, dp, dp]),
         "fp_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, dp]),
         "fp_stream": (vp, [vp]),
+        "fp_linear_solve_jfnk": (C.c_int, [vp, ip, dp]),
+        "fp_functional": (C.c_int, [vp, C.c_int, dp, dp]),
+        "fp_adjoint_solve": (C.c_int, [vp, C.c_double, dp, dp, ip, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -142,6 +142,28 @@ class DeviceCase:
         rc = self._check(native.lib().fp_linear_solve(self.h, C.byref(it), C.byref(err)), "fp_linear_solve")
         return rc, it.value, err.value
 
+    def linear_solve_jfnk(self):
+        it, err = C.c_int(), C.c_double()
+        rc = self._check(native.lib().fp_linear_solve_jfnk(self.h, C.byref(it), C.byref(err)), "fp_linear_solve_jfnk")
+        return rc, it.value, err.value
+
+    def functional(self, which: int = 0, gradient: bool = True):
+        """DragCoefficient (0) / LiftCoefficient (1) value and exact dI/dW."""
+        val = C.c_double()
+        g = np.zeros((self.n_elem, self.B)) if gradient else None
+        self._check(native.lib().fp_functional(self.h, which, C.byref(val), native.dptr(g) if g is not None else None),
+                    "fp_functional")
+        return val.value, g
+
+    def adjoint_solve(self, rhs, a1: float = 0.0):
+        """(a1 M - dR/dW)^T lambda = rhs with GMRES on the matrix-free J^T v."""
+        rhs = np.ascontiguousarray(rhs, np.float64).reshape(self.n_elem, self.B)
+        lam = np.zeros_like(rhs)
+        it, err = C.c_int(), C.c_double()
+        rc = self._check(native.lib().fp_adjoint_solve(self.h, a1, native.dptr(rhs), native.dptr(lam), C.byref(it),
+                                                       C.byref(err)), "fp_adjoint_solve")
+        return rc, lam, it.value, err.value
+
     def update(self) -> int:
         n = C.c_int()
         self._check(native.lib().fp_update(self.h, C.byref(n)), "fp_update")

@@ -107,6 +107,9 @@ int fp_assemble(fp_ctx* ctx, double dt, double dto, int time_order, int with_jac
                 double* rhs_norm);
 /* LinearSolver.solve: factor the preconditioner, GMRES from x = 0 */
 int fp_linear_solve(fp_ctx* ctx, int* iterations, double* rel_residual);
+/* the same Newton system solved Jacobian-free: GMRES on the matrix-free product
+ * (a1 M - dR/dW) v (fp_jv), preconditioned by the factors of the assembled matrix */
+int fp_linear_solve_jfnk(fp_ctx* ctx, int* iterations, double* rel_residual);
 /* Element.updateW for all elements, then Element.positivityLimiter (Element.java:334-338, 385-476) */
 int fp_update(fp_ctx* ctx, int* n_limited);
 
@@ -120,6 +123,16 @@ int fp_export_solution(fp_ctx* ctx, double* x);
 /* matrix-free (a1 M - dR/dW) v and its transpose, element layout in and out */
 int fp_jv(fp_ctx* ctx, double a1, const double* v, double* out);
 int fp_jtv(fp_ctx* ctx, double a1, const double* v, double* out);
+
+/* functional of interest at the current state and its exact gradient dI/dW (element layout,
+ * NULL to skip): which 0 DragCoefficient, 1 LiftCoefficient (FunctionalsOfInterest
+ * DragCoefficient.java:31-48, LiftCoefficient.java:32-34; replaces the FD gradient of
+ * OptimisationToolExport.exportFunctionalDerivative, OptimisationToolExport.java:180-217) */
+int fp_functional(fp_ctx* ctx, int which, double* value, double* dIdW);
+/* adjoint system (a1 M - dR/dW)^T lambda = rhs by GMRES on the matrix-free fp_jtv (element
+ * layout in and out); a1 = 0 gives the steady adjoint lambda = (-dR/dW)^-T dI/dW that the
+ * reference computes in MATLAB from J.txt (toolbox/optimisation/getFunctionalGradient.m:5-12) */
+int fp_adjoint_solve(fp_ctx* ctx, double a1, const double* rhs, double* lambda, int* iterations, double* rel_residual);
 
 /* --- measurement ------------------------------------------------------------------------ */
 /* average device time (ms) of one launch of a kernel family, timed with HIP events on the

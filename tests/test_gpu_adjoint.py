@@ -1,0 +1,128 @@
+"""Adjoint side of the HIP path: drag/lift functionals with exact gradients, the
+matrix-free transposed product J^T v, the adjoint solve and Jacobian-free Newton.
+
+Checks (tolerances written per check):
+  functional value vs oracle (FOI/DragCoefficient restated)            1e-12 of the O(1) wall force scale
+  exact dI/dW vs the reference's forward FD gradient (h = 1e-8)        1e-6 (FD truncation)
+  J^T v vs central differences of the GPU residual, 64 seeded
+    directions, step 1e-7                                               1e-6 relative
+  J^T v vs the transpose of the assembled GPU Jacobian                 1e-12 relative
+  adjoint solve vs a dense solve of the oracle's transposed matrix     1e-8 relative
+  Jacobian-free Newton correction vs the assembled-matrix correction   1e-8 relative
+"""
+import numpy as np
+import pytest
+
+from flowpro_v2_amd import cases, solver
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(case, props, seed=5):
+    props = dict(props, jacobian="exact")
+    o = O.Oracle(case, props)
+    dev = solver.DeviceCase(case, props)
+    W = cases.perturbed_state(dev.initial_state()[0][:: dev.nb], case.n_elem, dev.nb, 0.01, seed)
+    o.set_w(W)
+    o.limiter()
+    dev.set_state(W)
+    return o, dev, W
+
+
+def cyl():
+    return cases.cylinder_mesh(16, 5)
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_functional_and_gradient(which):
+    o, dev, W = _pair(cyl(), cases.ns_props(order=3, reynolds=300.0))
+    val, g = dev.functional(which)
+    ref = o.functional(which)
+    # the force is a cancelling sum of O(1) pressure contributions around the cylinder (the
+    # drag of this nearly symmetric flow is ~1e-5): scale the tolerance by the contribution
+    # size, the pressure level times the wall length (~pi * 0.5 * p ~ 1)
+    assert abs(val - ref) <= 1e-12
+    gfd = o.functional_grad_fd(which)          # OptimisationToolExport: forward FD, h = 1e-8
+    assert np.abs(g - gfd).max() <= 1e-6 * np.abs(gfd).max()
+    # the gradient is only nonzero on elements with a wall face
+    TT = o.TT()
+    wall = (TT == -1).any(1)
+    assert np.abs(g[~wall]).max() == 0.0 and np.abs(g[wall]).max() > 0
+    dev.close()
+
+
+def test_jtv_against_central_differences():
+    """<u, J v> by central differences of the GPU residual along 64 seeded directions v,
+    compared with <J^T u, v> from the matrix-free transposed kernel (a1 = 0: A = -dR/dW)."""
+    case = cyl()
+    o, dev, W = _pair(case, cases.ns_props(order=4))
+    rng = np.random.default_rng(64)
+    u = rng.standard_normal(W.shape)
+    jtu = dev.jtv(0.0, u)
+    eps = 1e-7
+    worst = 0.0
+    for _ in range(64):
+        v = rng.standard_normal(W.shape) * np.abs(W)
+        dev.set_state(W + eps * v)
+        dev.assemble(1.0, -1.0, 1, False)
+        Rp, _ = dev.residual()
+        dev.set_state(W - eps * v)
+        dev.assemble(1.0, -1.0, 1, False)
+        Rm, _ = dev.residual()
+        fd = -(u * (Rp - Rm)).sum() / (2 * eps)
+        ad = (jtu * v).sum()
+        worst = max(worst, abs(fd - ad) / max(abs(ad), 1e-30))
+    dev.set_state(W)
+    assert worst < 1e-6
+    dev.close()
+
+
+def test_jtv_is_transpose_of_assembled_matrix():
+    case = cyl()
+    o, dev, W = _pair(case, cases.ns_props(order=3, reynolds=300.0))
+    dt = 0.05
+    dev.assemble(dt, -1.0, 1, True)
+    d, n = dev.jacobian()
+    TT = o.TT()
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(W.shape)
+    # (A^T u)[e] = ADiag_e u_e + sum_k ANeigh of the neighbour rows: A[row e][col t] = n[e, k].T
+    ref = np.einsum("eij,ej->ei", d, u)
+    for e in range(case.n_elem):
+        for k in range(3):
+            t = TT[e, k]
+            if t > -1:
+                ref[t] += n[e, k] @ u[e]
+    got = dev.jtv(1.0 / dt, u)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    dev.close()
+
+
+def test_adjoint_solve_matches_dense():
+    case = cases.channel_bump(16, 4)
+    props = cases.euler_props(order=2, iterativeSolverTol=1e-11)
+    o, dev, W = _pair(case, props)
+    dt = 0.05
+    o.assemble(dt, -1.0)
+    _, g = dev.functional(0)
+    rc, lam, it, err = dev.adjoint_solve(g, a1=1.0 / dt)
+    assert rc == 0
+    A = o.global_matrix()
+    lam_ref = o.from_dof(np.linalg.solve(A.T, o.to_dof(g)))
+    assert np.abs(lam - lam_ref).max() <= 1e-8 * np.abs(lam_ref).max()
+    dev.close()
+
+
+def test_jacobian_free_newton_correction():
+    case = cases.cylinder_mesh(16, 6)
+    props = cases.ns_props(order=3, reynolds=300.0, iterativeSolverTol=1e-12)
+    o, dev, W = _pair(case, props)
+    dev.assemble(0.05, -1.0, 1, True)
+    rc, it, err = dev.linear_solve()
+    x_asm = dev.solution()
+    rc2, it2, err2 = dev.linear_solve_jfnk()
+    x_mf = dev.solution()
+    assert rc == 0 and rc2 == 0
+    assert np.abs(x_mf - x_asm).max() <= 1e-8 * np.abs(x_asm).max()
+    dev.close()
