@@ -40,11 +40,12 @@ template <typename TF> hipError_t launch_ilu_copy(hipStream_t, size_t, const dou
 template <typename TF>
 hipError_t launch_ilu_factor_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, int*);
 template <typename TF>
-hipError_t launch_ilu_fwd_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, const TF*, const IluIO*,
-                                TF*);
+hipError_t launch_ilu_factor_rows(hipStream_t, int, const int*, int, const int*, const int*, const int*, const int*, TF*, int*);
+bool ilu_factor_rows_supported(int, int);
 template <typename TF>
-hipError_t launch_ilu_bwd_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, const TF*, const TF*, TF*,
-                                const IluIO*);
+hipError_t launch_ilu_fwd_level(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);
+template <typename TF>
+hipError_t launch_ilu_bwd_level(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*);
 hipError_t launch_set_io(hipStream_t, IluIO*, const double*, double*);
 hipError_t launch_block_jacobi(hipStream_t, int, int, double*, const double*, double*);
 hipError_t launch_gather_diag(hipStream_t, int, int, const int*, const double*, double*);
@@ -84,6 +85,9 @@ struct fp_ctx {
     double *geo = nullptr, *fgeo = nullptr, *tab = nullptr, *esize = nullptr, *isref = nullptr;
     int *nbr = nullptr, *nbrf = nullptr, *blkn = nullptr, *blkd = nullptr;
     int *rowptr = nullptr, *colidx = nullptr, *diagpos = nullptr, *lvl_f = nullptr, *lvl_b = nullptr;
+    int *ftptr = nullptr, *ftask = nullptr;   // per-row task lists of the ILU0 factor
+    int *meta_f = nullptr, *meta_b = nullptr; // per-row records of the two sweeps, in level order
+    bool ilu_rows = false;                    // factor with the register-resident row kernel
     double* val = nullptr;        // FP64 matrix (nnzb * B * B)
     void* F = nullptr;            // preconditioner factors (FP64 or FP32)
     void *iy = nullptr, *ixs = nullptr;
@@ -339,7 +343,7 @@ static void free_all(fp_ctx* c) {
     if (c->g_apply) (void)hipGraphExecDestroy(c->g_apply);
     if (c->io) (void)hipFree(c->io);
     void* ptrs[] = {c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
-                    c->diagpos, c->lvl_f, c->lvl_b, c->val, c->F, c->iy, c->ixs, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
+                    c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->val, c->F, c->iy, c->ixs, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
                     c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->istat};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -566,6 +570,51 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     std::vector<int> rowsf, rowsb;
     bucket(lf, nlf, c->lvl_f_ptr, rowsf);
     bucket(lb, nlb, c->lvl_b_ptr, rowsb);
+    // sweep records (layout documented at k_ilu_fwd_level): {i, n, diag, 0, blocks[6], rows[6]}
+    std::vector<int> meta_f((size_t)16 * n, 0), meta_b((size_t)16 * n, 0);
+    for (int pos = 0; pos < n; pos++) {
+        for (int side = 0; side < 2; side++) {
+            const int r = side == 0 ? rowsf[pos] : rowsb[pos];
+            int* m = (side == 0 ? meta_f.data() : meta_b.data()) + (size_t)16 * pos;
+            const int s0 = side == 0 ? rowptr[r] : diagpos[r] + 1, s1 = side == 0 ? diagpos[r] : rowptr[r + 1];
+            if (s1 - s0 > 6) {
+                c->err = "more than 6 neighbours on one side of a block row";
+                return FP_UNSUPPORTED;
+            }
+            m[0] = r;
+            m[1] = s1 - s0;
+            m[2] = diagpos[r];
+            for (int q = s0; q < s1; q++) {
+                m[4 + q - s0] = q;
+                m[10 + q - s0] = colidx[q];
+            }
+        }
+    }
+    // --- ILU0 factor tasks per row, in the order of BlockILU0.factor (BlockILU0.java:101-127):
+    // for each lower k: {slot of A_ik, block of D_k^-1}, then for each later block j of row i
+    // that row k holds: {slot of A_ij | 256, block U_kj}
+    std::vector<int> ftptr(n + 1, 0), ftask;
+    int max_row = 0;
+    for (int r = 0; r < n; r++) {
+        const int rs = rowptr[r], re = rowptr[r + 1];
+        max_row = std::max(max_row, re - rs);
+        for (int kk = rs; kk < diagpos[r]; kk++) {
+            const int k = colidx[kk];
+            ftask.push_back(kk - rs);
+            ftask.push_back(diagpos[k]);
+            for (int jj = kk + 1; jj < re; jj++) {
+                const int kj = find_blk(k, colidx[jj]);
+                if (kj < 0) continue;
+                ftask.push_back((jj - rs) | 256);
+                ftask.push_back(kj);
+            }
+        }
+        ftptr[r + 1] = (int)ftask.size() / 2;
+    }
+    {
+        const char* g = std::getenv("FLOWPRO_ILU_ROWS");   // "0" selects the per-level block kernels
+        c->ilu_rows = ilu_factor_rows_supported(c->B, max_row) && !(g && g[0] == '0');
+    }
 
     HIPCHK(c, upload(&c->geo, geo));
     HIPCHK(c, upload(&c->fgeo, fgeo));
@@ -579,6 +628,10 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     HIPCHK(c, upload(&c->diagpos, diagpos));
     HIPCHK(c, upload(&c->lvl_f, rowsf));
     HIPCHK(c, upload(&c->lvl_b, rowsb));
+    HIPCHK(c, upload(&c->ftptr, ftptr));
+    HIPCHK(c, upload(&c->meta_f, meta_f));
+    HIPCHK(c, upload(&c->meta_b, meta_b));
+    HIPCHK(c, upload(&c->ftask, ftask));
     c->ms = Mesh{n, c->geo, c->fgeo, c->nbr, c->nbrf, c->blkn, c->blkd};
 
     // --- state, matrix, Krylov workspace
@@ -778,25 +831,36 @@ extern "C" int fp_assemble(fp_ctx* c, double dt, double dto, int time_order, int
 // --- preconditioner
 // The level sequences of the block-ILU0 factorisation and of the two sweeps are captured
 // once per context into HIP graphs and replayed (the per-level launches dominate otherwise).
-template <typename TF> static int ilu_factor_launches(fp_ctx* c) {
+template <typename TF> static int ilu_copy_launch(fp_ctx* c) {
+    HIPCHK(c, launch_ilu_copy<TF>(c->s, (size_t)c->nnzb * c->B * c->B, c->val, (TF*)c->F));
+    return FP_OK;
+}
+template <typename TF> static int ilu_level_launches(fp_ctx* c) {
     TF* F = (TF*)c->F;
-    const size_t tot = (size_t)c->nnzb * c->B * c->B;
-    HIPCHK(c, launch_ilu_copy<TF>(c->s, tot, c->val, F));
     for (size_t l = 0; l + 1 < c->lvl_f_ptr.size(); l++) {
         int b = c->lvl_f_ptr[l], e = c->lvl_f_ptr[l + 1];
-        HIPCHK(c, launch_ilu_factor_level<TF>(c->s, c->B, c->lvl_f + b, e - b, c->rowptr, c->colidx, c->diagpos, F, c->istat));
+        if (c->ilu_rows)
+            HIPCHK(c, launch_ilu_factor_rows<TF>(c->s, c->B, c->lvl_f + b, e - b, c->rowptr, c->diagpos, c->ftptr, c->ftask, F,
+                                                 c->istat));
+        else
+            HIPCHK(c, launch_ilu_factor_level<TF>(c->s, c->B, c->lvl_f + b, e - b, c->rowptr, c->colidx, c->diagpos, F,
+                                                  c->istat));
     }
     return FP_OK;
+}
+template <typename TF> static int ilu_factor_launches(fp_ctx* c) {
+    int rc = ilu_copy_launch<TF>(c);
+    return rc ? rc : ilu_level_launches<TF>(c);
 }
 template <typename TF> static int ilu_apply_launches(fp_ctx* c) {
     const TF* F = (const TF*)c->F;
     for (size_t l = 0; l + 1 < c->lvl_f_ptr.size(); l++) {
         int b = c->lvl_f_ptr[l], e = c->lvl_f_ptr[l + 1];
-        HIPCHK(c, launch_ilu_fwd_level<TF>(c->s, c->B, c->lvl_f + b, e - b, c->rowptr, c->colidx, c->diagpos, F, c->io, (TF*)c->iy));
+        HIPCHK(c, launch_ilu_fwd_level<TF>(c->s, c->B, c->meta_f + 16 * (size_t)b, e - b, F, c->io, (TF*)c->iy));
     }
     for (size_t l = 0; l + 1 < c->lvl_b_ptr.size(); l++) {
         int b = c->lvl_b_ptr[l], e = c->lvl_b_ptr[l + 1];
-        HIPCHK(c, launch_ilu_bwd_level<TF>(c->s, c->B, c->lvl_b + b, e - b, c->rowptr, c->colidx, c->diagpos, F, (const TF*)c->iy,
+        HIPCHK(c, launch_ilu_bwd_level<TF>(c->s, c->B, c->meta_b + 16 * (size_t)b, e - b, F, (const TF*)c->iy,
                                            (TF*)c->ixs, c->io));
     }
     return FP_OK;
@@ -1158,6 +1222,28 @@ extern "C" int fp_time_kernel(fp_ctx* c, int which, int reps, double* avg_ms) {
         }
         return FP_EINVAL;
     };
+    if (which == 6) {
+        // the level launches of the block-ILU0 factorisation alone (the copy of the
+        // assembled values into the factor storage precedes each repetition, untimed)
+        if (c->precond != 2) return FP_EINVAL;
+        double tot = 0;
+        for (int i = 0; i <= reps; i++) {
+            rc = c->bits == 32 ? ilu_copy_launch<float>(c) : ilu_copy_launch<double>(c);
+            if (rc) return rc;
+            HIPCHK(c, hipEventRecord(e0, c->s));
+            rc = c->bits == 32 ? ilu_level_launches<float>(c) : ilu_level_launches<double>(c);
+            if (rc) return rc;
+            HIPCHK(c, hipEventRecord(e1, c->s));
+            HIPCHK(c, hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+            if (i > 0) tot += ms;   // repetition 0 warms up
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_ms = tot / reps;
+        return FP_OK;
+    }
     if ((rc = once())) return rc;   // warm-up
     HIPCHK(c, hipEventRecord(e0, c->s));
     for (int i = 0; i < reps && rc == FP_OK; i++) rc = once();

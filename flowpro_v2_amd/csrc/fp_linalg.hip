@@ -339,6 +339,409 @@ __global__ void __launch_bounds__(64) k_ilu_factor_level_w(const int* __restrict
     }
 }
 
+// Four wavefronts per block row: the B output columns are split over the waves (lane r
+// keeps row r of its columns), the operand block is staged once in LDS by all 256 threads.
+// In the Gauss-Jordan inverse the wave owning pivot column p finds the pivot row and
+// publishes the multipliers M[.][p] through a double-buffered LDS column; every wave then
+// eliminates its own columns, so each pivot costs one barrier.
+template <typename TF, int B>
+__global__ void __launch_bounds__(256) k_ilu_factor_level_4w(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
+                                                             const int* __restrict__ colidx, const int* __restrict__ diagpos,
+                                                             TF* __restrict__ F, int* __restrict__ status) {
+    constexpr int BB = B * B;
+    constexpr int NW = 4;
+    constexpr int CW = (B + NW - 1) / NW;         // columns per wave
+    constexpr int PER = (BB + 255) / 256;
+    __shared__ TF S[BB];
+    __shared__ TF Lsh[BB];
+    __shared__ TF fbuf[2][64];
+    __shared__ int pivs[2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, tid = threadIdx.x;
+    const int r = lane;
+    const bool act = r < B;
+    const int c0 = wv * CW;
+    if ((int)blockIdx.x >= nrows) return;
+    const int i = rows[blockIdx.x];
+    const int r1 = rowptr[i + 1], di = diagpos[i];
+    TF a[B], v[CW];
+    for (int kk = rowptr[i]; kk < di; kk++) {
+        const int k = colidx[kk];
+        TF* Aik = F + (size_t)kk * BB;
+        const TF* Dk = F + (size_t)diagpos[k] * BB;   // D_k^-1 (row k is factored)
+        if (act) {
+#pragma unroll
+            for (int m = 0; m < B; m++) a[m] = Aik[m * B + r];
+        }
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int q = tid + 256 * u;
+            if (q < BB) S[q] = Dk[q];
+        }
+        __syncthreads();
+        // L = A_ik D_k^-1 for this wave's columns
+        if (act) {
+#pragma unroll
+            for (int cc = 0; cc < CW; cc++) {
+                const int c = c0 + cc;
+                if (c < B) {
+                    TF s0 = 0, s1 = 0;
+#pragma unroll
+                    for (int m = 0; m + 1 < B; m += 2) {
+                        s0 += a[m] * S[c * B + m];
+                        s1 += a[m + 1] * S[c * B + m + 1];
+                    }
+                    if (B % 2) s0 += a[B - 1] * S[c * B + B - 1];
+                    v[cc] = s0 + s1;
+                    Aik[c * B + r] = v[cc];
+                    Lsh[c * B + r] = v[cc];
+                }
+            }
+        }
+        __syncthreads();
+        for (int jj = kk + 1; jj < r1; jj++) {
+            const int j = colidx[jj];
+            int kj = -1;
+            for (int s2 = rowptr[k]; s2 < rowptr[k + 1]; s2++)
+                if (colidx[s2] == j) kj = s2;
+            if (kj < 0) continue;
+            const TF* U = F + (size_t)kj * BB;
+            TF* C = F + (size_t)jj * BB;
+            if (act) {
+#pragma unroll
+                for (int cc = 0; cc < CW; cc++)
+                    if (c0 + cc < B) v[cc] = C[(c0 + cc) * B + r];
+            }
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                const int q = tid + 256 * u;
+                if (q < BB) S[q] = U[q];
+            }
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (int m = 0; m < B; m++) a[m] = Lsh[m * B + r];   // row r of L
+#pragma unroll
+                for (int cc = 0; cc < CW; cc++) {
+                    const int c = c0 + cc;
+                    if (c < B) {
+                        TF s0 = 0, s1 = 0;
+#pragma unroll
+                        for (int m = 0; m + 1 < B; m += 2) {
+                            s0 += a[m] * S[c * B + m];
+                            s1 += a[m + 1] * S[c * B + m + 1];
+                        }
+                        if (B % 2) s0 += a[B - 1] * S[c * B + B - 1];
+                        C[c * B + r] = v[cc] - (s0 + s1);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // D_i^-1, Gauss-Jordan with implicit partial pivoting (no row exchanges; the rows of the
+    // result are the rows of the inverse in pivot order).  Wave wv holds columns
+    // [c0, c0 + CW) of M (mc) and of the identity (ic) for row r.
+    TF* Dg = F + (size_t)di * BB;
+    TF mc[CW], ic[CW];
+#pragma unroll
+    for (int cc = 0; cc < CW; cc++) {
+        const int c = c0 + cc;
+        mc[cc] = (act && c < B) ? Dg[c * B + r] : (TF)0;
+        ic[cc] = (act && c == r) ? (TF)1 : (TF)0;
+    }
+    bool used = !act;
+    int dest = 0;
+    bool singular = false;
+#pragma unroll 1
+    for (int p = 0; p < B; p++) {
+        const int owner = p / CW, cp = p % CW, buf = p & 1;
+        if (wv == owner) {
+            TF x = 0;
+#pragma unroll
+            for (int cc = 0; cc < CW; cc++) x = (cc == cp) ? mc[cc] : x;
+            const float key = used ? -1.0f : (float)fabs(x);
+            float kmax = key;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) kmax = fmaxf(kmax, __shfl_xor(kmax, off));
+            const unsigned long long cand = __ballot(!used && key == kmax);
+            int pr = cand ? (int)__ffsll((long long)cand) - 1 : 0;
+            if (!(kmax > 0.0f)) pr = -1 - pr;   // encodes a singular pivot for the other waves
+            fbuf[buf][r] = x;
+            if (lane == 0) pivs[buf] = pr;
+        }
+        __syncthreads();
+        int pr = pivs[buf];
+        if (pr < 0) { singular = true; pr = -1 - pr; }
+        const TF f = fbuf[buf][r];
+        const TF d = (TF)1 / fbuf[buf][pr];
+#pragma unroll
+        for (int cc = 0; cc < CW; cc++) {
+            const TF pm = lane_read(mc[cc], pr) * d;
+            const TF pi = lane_read(ic[cc], pr) * d;
+            if (r == pr) {
+                mc[cc] = pm;
+                ic[cc] = pi;
+            } else if (act) {
+                mc[cc] -= f * pm;
+                ic[cc] -= f * pi;
+            }
+        }
+        if (r == pr) {
+            used = true;
+            dest = p;
+        }
+    }
+    if (act) {
+        int bad = singular ? 1 : 0;
+#pragma unroll
+        for (int cc = 0; cc < CW; cc++) {
+            const int c = c0 + cc;
+            if (c < B) {
+                if (!isfinite((double)ic[cc])) bad = 1;
+                Dg[c * B + dest] = ic[cc];
+            }
+        }
+        if (bad) atomicOr(status, 1);
+    }
+}
+
+// Workgroup barrier that orders LDS accesses only: outstanding global loads (prefetches)
+// stay in flight across it.  Valid where the waves of a workgroup exchange data through LDS
+// alone.
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Largest |key| over the wavefront through DPP lane moves (quad permutes, row mirrors,
+// row broadcasts 15/31); the result is read from lane 63.
+__device__ inline int wave_max_key(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));    // quad_perm [1,0,3,2]
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));    // quad_perm [2,3,0,1]
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));   // row_half_mirror
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));   // row_mirror
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));   // row_bcast:15
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));   // row_bcast:31
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// Reciprocal of a pivot: v_rcp_f32 (1 ulp) for FP32 factors, IEEE division for FP64.
+__device__ inline float pivot_reciprocal(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double pivot_reciprocal(double x) { return 1.0 / x; }
+
+// Block row of the ILU0 factor with the whole row held in registers.  The host lists, per
+// row, the tasks of BlockILU0.factor in their sequential order: for each lower neighbour k
+// (increasing) "L_ik = A_ik D_k^-1" and then, for each block j of row i after k that row k
+// also holds, "A_ij -= L_ik U_kj".  NW wavefronts split the B columns of every block of the
+// row (lane r keeps row r), so the row is read from HBM once and written once.  The operand
+// block of each task (D_k^-1 or U_kj) is staged through a double-buffered LDS tile, with the
+// next task's block already in flight in registers while the current one is used.  The rows
+// of A_ik and of L_ik are exchanged between the waves through LDS.  The diagonal block is
+// then inverted by four of the waves (Gauss-Jordan with implicit partial pivoting; pivot row
+// = largest |M[.][p]| via a DPP wave maximum on |x| with the lane index in the low bits).
+template <typename TF, int B, int MAXR, int NW>
+__global__ void __launch_bounds__(64 * NW) k_ilu_factor_rows(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
+                                                         const int* __restrict__ diagpos, const int* __restrict__ tptr,
+                                                         const int2* __restrict__ tasks, TF* __restrict__ F,
+                                                         int* __restrict__ status) {
+    constexpr int BB = B * B;
+    constexpr int NT = 64 * NW;
+    constexpr int CW = (B + NW - 1) / NW;       // columns per wave in the tasks
+    constexpr int NG = 4;                       // waves of the inversion
+    constexpr int CG = (B + NG - 1) / NG;       // identity columns per inversion wave
+    constexpr int PER = (BB + NT - 1) / NT;
+    __shared__ TF S[2][BB];
+    __shared__ TF LA[BB];
+    __shared__ TF LL[BB];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, tid = threadIdx.x;
+    const int r = lane;
+    const bool act = r < B;
+    const int c0 = wv * CW;
+    if ((int)blockIdx.x >= nrows) return;
+    const int i = rows[blockIdx.x];
+    const int rs = rowptr[i], nb = rowptr[i + 1] - rs, qd = diagpos[i] - rs;
+    const int t0 = tptr[i], nt = tptr[i + 1] - t0;
+
+    TF R[MAXR][CW];
+#pragma unroll
+    for (int q = 0; q < MAXR; q++)
+#pragma unroll
+        for (int cc = 0; cc < CW; cc++) {
+            const int c = c0 + cc;
+            R[q][cc] = (act && c < B && q < nb) ? F[(size_t)(rs + q) * BB + c * B + r] : (TF)0;
+        }
+    TF pf[PER];
+    auto load_item = [&](int t) {
+        const TF* src = F + (size_t)tasks[t0 + t].y * BB;
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int q = tid + NT * u;
+            pf[u] = q < BB ? src[q] : (TF)0;
+        }
+    };
+    auto store_item = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int q = tid + NT * u;
+            if (q < BB) S[buf][q] = pf[u];
+        }
+    };
+    if (nt > 0) {
+        load_item(0);
+        store_item(0);
+        if (nt > 1) load_item(1);
+    }
+    unsigned touched = 0;
+    TF a[B];
+    for (int t = 0; t < nt; t++) {
+        lds_barrier();   // tile t visible; the other tile, LA and LL no longer read
+        if (t + 1 < nt) {
+            store_item((t + 1) & 1);
+            if (t + 2 < nt) load_item(t + 2);
+        }
+        const int2 tk = tasks[t0 + t];
+        const int q = tk.x & 255;
+        const TF* Sb = S[t & 1];
+        touched |= 1u << q;
+        if (!(tk.x & 256)) {
+            if (act) {
+#pragma unroll
+                for (int cc = 0; cc < CW; cc++) {
+                    const int c = c0 + cc;
+                    if (c < B) {
+                        TF v = 0;
+#pragma unroll
+                        for (int qq = 0; qq < MAXR; qq++) v = (qq == q) ? R[qq][cc] : v;
+                        LA[c * B + r] = v;
+                    }
+                }
+            }
+            lds_barrier();
+            if (act) {
+#pragma unroll
+                for (int m = 0; m < B; m++) a[m] = LA[m * B + r];
+#pragma unroll
+                for (int cc = 0; cc < CW; cc++) {
+                    const int c = c0 + cc;
+                    if (c < B) {
+                        TF s0 = 0, s1 = 0;
+#pragma unroll
+                        for (int m = 0; m + 1 < B; m += 2) {
+                            s0 += a[m] * Sb[c * B + m];
+                            s1 += a[m + 1] * Sb[c * B + m + 1];
+                        }
+                        if (B % 2) s0 += a[B - 1] * Sb[c * B + B - 1];
+                        const TF L = s0 + s1;
+#pragma unroll
+                        for (int qq = 0; qq < MAXR; qq++)
+                            if (qq == q) R[qq][cc] = L;
+                        LL[c * B + r] = L;
+                    }
+                }
+            }
+        } else {
+            if (act) {
+#pragma unroll
+                for (int m = 0; m < B; m++) a[m] = LL[m * B + r];
+#pragma unroll
+                for (int cc = 0; cc < CW; cc++) {
+                    const int c = c0 + cc;
+                    if (c < B) {
+                        TF s0 = 0, s1 = 0;
+#pragma unroll
+                        for (int m = 0; m + 1 < B; m += 2) {
+                            s0 += a[m] * Sb[c * B + m];
+                            s1 += a[m + 1] * Sb[c * B + m + 1];
+                        }
+                        if (B % 2) s0 += a[B - 1] * Sb[c * B + B - 1];
+                        const TF sub = s0 + s1;
+#pragma unroll
+                        for (int qq = 0; qq < MAXR; qq++)
+                            if (qq == q) R[qq][cc] -= sub;
+                    }
+                }
+            }
+        }
+    }
+    // write back the off-diagonal blocks the tasks changed
+    if (act) {
+#pragma unroll
+        for (int q = 0; q < MAXR; q++)
+            if (q < nb && q != qd && ((touched >> q) & 1u)) {
+#pragma unroll
+                for (int cc = 0; cc < CW; cc++) {
+                    const int c = c0 + cc;
+                    if (c < B) F[(size_t)(rs + q) * BB + c * B + r] = R[q][cc];
+                }
+            }
+    }
+    // D_i^-1 of the updated diagonal block (rows of the result in pivot order).  Every wave
+    // holds the whole block (row r in lane r) and repeats the same eliminations, so the pivot
+    // search needs no exchange between the waves; the identity part, which becomes the
+    // inverse, is split by columns.  Columns left of the pivot are not needed again and are
+    // not updated.
+    lds_barrier();   // the last task's readers of LA are done
+    if (act) {
+#pragma unroll
+        for (int cc = 0; cc < CW; cc++) {
+            const int c = c0 + cc;
+            if (c < B) {
+                TF v = 0;
+#pragma unroll
+                for (int qq = 0; qq < MAXR; qq++) v = (qq == qd) ? R[qq][cc] : v;
+                LA[c * B + r] = v;
+            }
+        }
+    }
+    lds_barrier();
+    if (wv >= NG) return;
+    const int g0 = wv * CG;
+    TF mr[B], ic[CG];
+#pragma unroll
+    for (int c = 0; c < B; c++) mr[c] = act ? LA[c * B + r] : (TF)0;
+#pragma unroll
+    for (int cc = 0; cc < CG; cc++) ic[cc] = (act && g0 + cc == r) ? (TF)1 : (TF)0;
+    bool used = !act;
+    int dest = 0;
+    bool singular = false;
+    TF myd = 0;
+    // elimination without normalising the pivot row: row r of the result is scaled by the
+    // reciprocal of its own pivot at the end
+#pragma unroll
+    for (int p = 0; p < B; p++) {
+        const unsigned mag = __float_as_uint(fabsf((float)mr[p])) & ~63u;
+        const int key = used ? 0 : (int)(mag | (unsigned)(63 - lane));
+        const int kmax = wave_max_key(key);
+        const int pr = 63 - (kmax & 63);
+        singular = singular || (kmax >> 6) == 0;
+        const TF d = pivot_reciprocal(lane_read(mr[p], pr));
+        const bool isp = r == pr;
+        const TF beta = isp ? (TF)0 : -mr[p] * d;   // 0 on lanes r >= B
+#pragma unroll
+        for (int c = p + 1; c < B; c++) mr[c] = fma(beta, lane_read(mr[c], pr), mr[c]);
+#pragma unroll
+        for (int cc = 0; cc < CG; cc++) ic[cc] = fma(beta, lane_read(ic[cc], pr), ic[cc]);
+        used = used || isp;
+        dest = isp ? p : dest;
+        myd = isp ? d : myd;
+    }
+    if (act) {
+        int bad = singular ? 1 : 0;
+        TF* Dg = F + (size_t)(rs + qd) * BB;
+#pragma unroll
+        for (int cc = 0; cc < CG; cc++) {
+            const int c = g0 + cc;
+            if (c < B) {
+                const TF v = ic[cc] * myd;
+                if (!isfinite((double)v)) bad = 1;
+                Dg[c * B + dest] = v;
+            }
+        }
+        if (bad) atomicOr(status, 1);
+    }
+}
+
 // ------------------------------------------------------------------ ILU sweeps
 // Several block rows per 64-lane workgroup when B < 64 (lane = row of a block).
 // The right-hand side and the result are passed through `io` so that the level
@@ -346,84 +749,108 @@ __global__ void __launch_bounds__(64) k_ilu_factor_level_w(const int* __restrict
 
 template <int B> struct RowsPerWave { static constexpr int value = B <= 32 ? 64 / B : 1; };
 
+// Per-row records of the sweeps, one per position of the level order (16 ints): row i,
+// number n of off-diagonal blocks on the swept side (lower for the forward sweep, upper for
+// the backward one), the diagonal block, then up to ILU_MAXN block indices and the matching
+// neighbour rows.  One 64-byte load replaces the rows -> rowptr -> colidx chain.
+constexpr int ILU_MAXN = 6;
+
 // forward: y_i = b_i - sum_{k<i} L_ik y_k
 template <typename TF, int B>
-__global__ void __launch_bounds__(64) k_ilu_fwd_level(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
-                                                      const int* __restrict__ colidx, const int* __restrict__ diagpos,
-                                                      const TF* __restrict__ F, const IluIO* __restrict__ io,
-                                                      TF* __restrict__ y) {
+__global__ void __launch_bounds__(64) k_ilu_fwd_level(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
+                                                      const IluIO* __restrict__ io, TF* __restrict__ y) {
     constexpr int RPW = RowsPerWave<B>::value;
     constexpr size_t BB = (size_t)B * B;
     const int slot = threadIdx.x / B, t = threadIdx.x % B;
-    const int li = blockIdx.x * RPW + slot;
+    // with one row per wave the record index is uniform, so the record is read through the
+    // scalar cache into SGPRs and its block indices never wait behind the vector loads
+    const int li = RPW == 1 ? (int)blockIdx.x : (int)blockIdx.x * RPW + slot;
     if (slot >= RPW || li >= nrows) return;
-    const int i = rows[li];
-    const double* b = io->b;
-    TF s = (TF)b[(size_t)i * B + t];
-    for (int kk = rowptr[i]; kk < diagpos[i]; kk++) {
-        const TF* L = F + (size_t)kk * BB + t;
-        const TF* yk = y + (size_t)colidx[kk] * B;
-        TF a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    const int4* m = reinterpret_cast<const int4*>(meta) + (size_t)li * 4;
+    const int4 m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+    const int i = m0.x, cnt = m0.y;
+    const int blk[ILU_MAXN] = {m1.x, m1.y, m1.z, m1.w, m2.x, m2.y};
+    const int col[ILU_MAXN] = {m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
+    TF s = (TF)io->b[(size_t)i * B + t];
 #pragma unroll
-        for (int c = 0; c + 3 < B; c += 4) {
-            a0 += L[(size_t)c * B] * yk[c];
-            a1 += L[(size_t)(c + 1) * B] * yk[c + 1];
-            a2 += L[(size_t)(c + 2) * B] * yk[c + 2];
-            a3 += L[(size_t)(c + 3) * B] * yk[c + 3];
+    for (int n = 0; n < ILU_MAXN; n++) {
+        if (n < cnt) {
+            const TF* L = F + (size_t)blk[n] * BB + t;
+            const TF* yk = y + (size_t)col[n] * B;
+            TF a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+            for (int c = 0; c + 3 < B; c += 4) {
+                a0 += L[(size_t)c * B] * yk[c];
+                a1 += L[(size_t)(c + 1) * B] * yk[c + 1];
+                a2 += L[(size_t)(c + 2) * B] * yk[c + 2];
+                a3 += L[(size_t)(c + 3) * B] * yk[c + 3];
+            }
+#pragma unroll
+            for (int c = B - B % 4; c < B; c++) a0 += L[(size_t)c * B] * yk[c];
+            s -= (a0 + a1) + (a2 + a3);
         }
-#pragma unroll
-        for (int c = B - B % 4; c < B; c++) a0 += L[(size_t)c * B] * yk[c];
-        s -= (a0 + a1) + (a2 + a3);
     }
     y[(size_t)i * B + t] = s;
 }
 
-// backward: x_i = D_i^-1 (y_i - sum_{j>i} U_ij x_j)
+// backward: x_i = D_i^-1 (y_i - sum_{j>i} U_ij x_j); the row of D_i^-1 is loaded together
+// with the U blocks, before the exchange of the right-hand side through LDS
 template <typename TF, int B>
-__global__ void __launch_bounds__(64) k_ilu_bwd_level(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
-                                                      const int* __restrict__ colidx, const int* __restrict__ diagpos,
-                                                      const TF* __restrict__ F, const TF* __restrict__ y, TF* __restrict__ xs,
+__global__ void __launch_bounds__(64) k_ilu_bwd_level(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
+                                                      const TF* __restrict__ y, TF* __restrict__ xs,
                                                       const IluIO* __restrict__ io) {
     constexpr int RPW = RowsPerWave<B>::value;
     constexpr size_t BB = (size_t)B * B;
     __shared__ TF rhs[RPW * B];
     const int slot = threadIdx.x / B, t = threadIdx.x % B;
-    const int li = blockIdx.x * RPW + slot;
+    const int li = RPW == 1 ? (int)blockIdx.x : (int)blockIdx.x * RPW + slot;   // uniform record index as above
     const bool active = slot < RPW && li < nrows;
-    const int i = active ? rows[li] : 0;
+    int i = 0;
+    TF drow[B];
     if (active) {
+        const int4* m = reinterpret_cast<const int4*>(meta) + (size_t)li * 4;
+        const int4 m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+        i = m0.x;
+        const int cnt = m0.y;
+        const int blk[ILU_MAXN] = {m1.x, m1.y, m1.z, m1.w, m2.x, m2.y};
+        const int col[ILU_MAXN] = {m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
+        const TF* D = F + (size_t)m0.z * BB + t;
+#pragma unroll
+        for (int c = 0; c < B; c++) drow[c] = D[(size_t)c * B];
         TF s = y[(size_t)i * B + t];
-        for (int jj = diagpos[i] + 1; jj < rowptr[i + 1]; jj++) {
-            const TF* U = F + (size_t)jj * BB + t;
-            const TF* xj = xs + (size_t)colidx[jj] * B;
-            TF a0 = 0, a1 = 0, a2 = 0, a3 = 0;
 #pragma unroll
-            for (int c = 0; c + 3 < B; c += 4) {
-                a0 += U[(size_t)c * B] * xj[c];
-                a1 += U[(size_t)(c + 1) * B] * xj[c + 1];
-                a2 += U[(size_t)(c + 2) * B] * xj[c + 2];
-                a3 += U[(size_t)(c + 3) * B] * xj[c + 3];
+        for (int n = 0; n < ILU_MAXN; n++) {
+            if (n < cnt) {
+                const TF* U = F + (size_t)blk[n] * BB + t;
+                const TF* xj = xs + (size_t)col[n] * B;
+                TF a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+                for (int c = 0; c + 3 < B; c += 4) {
+                    a0 += U[(size_t)c * B] * xj[c];
+                    a1 += U[(size_t)(c + 1) * B] * xj[c + 1];
+                    a2 += U[(size_t)(c + 2) * B] * xj[c + 2];
+                    a3 += U[(size_t)(c + 3) * B] * xj[c + 3];
+                }
+#pragma unroll
+                for (int c = B - B % 4; c < B; c++) a0 += U[(size_t)c * B] * xj[c];
+                s -= (a0 + a1) + (a2 + a3);
             }
-#pragma unroll
-            for (int c = B - B % 4; c < B; c++) a0 += U[(size_t)c * B] * xj[c];
-            s -= (a0 + a1) + (a2 + a3);
         }
         rhs[slot * B + t] = s;
     }
     __syncthreads();
     if (!active) return;
-    const TF* D = F + (size_t)diagpos[i] * BB + t;
     const TF* rr = rhs + slot * B;
     TF a0 = 0, a1 = 0, a2 = 0, a3 = 0;
 #pragma unroll
     for (int c = 0; c + 3 < B; c += 4) {
-        a0 += D[(size_t)c * B] * rr[c];
-        a1 += D[(size_t)(c + 1) * B] * rr[c + 1];
-        a2 += D[(size_t)(c + 2) * B] * rr[c + 2];
-        a3 += D[(size_t)(c + 3) * B] * rr[c + 3];
+        a0 += drow[c] * rr[c];
+        a1 += drow[c + 1] * rr[c + 1];
+        a2 += drow[c + 2] * rr[c + 2];
+        a3 += drow[c + 3] * rr[c + 3];
     }
 #pragma unroll
-    for (int c = B - B % 4; c < B; c++) a0 += D[(size_t)c * B] * rr[c];
+    for (int c = B - B % 4; c < B; c++) a0 += drow[c] * rr[c];
     const TF v = (a0 + a1) + (a2 + a3);
     xs[(size_t)i * B + t] = v;
     io->x[(size_t)i * B + t] = (double)v;
@@ -589,8 +1016,12 @@ hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nr
                                    const int* diagpos, TF* F, int* status) {
 #define X(BS)                                                                                                          \
     if (B == BS) {                                                                                                     \
-        hipLaunchKernelGGL((k_ilu_factor_level_w<TF, BS>), dim3(nrows), dim3(64), 0, s, rows, nrows, rowptr, colidx, diagpos, \
-                           F, status);                                                                                 \
+        if (BS >= 24)                                                                                                  \
+            hipLaunchKernelGGL((k_ilu_factor_level_4w<TF, BS>), dim3(nrows), dim3(256), 0, s, rows, nrows, rowptr,     \
+                               colidx, diagpos, F, status);                                                            \
+        else                                                                                                           \
+            hipLaunchKernelGGL((k_ilu_factor_level_w<TF, BS>), dim3(nrows), dim3(64), 0, s, rows, nrows, rowptr,       \
+                               colidx, diagpos, F, status);                                                            \
         return hipGetLastError();                                                                                      \
     }
     FP_BLOCK_SIZES(X)
@@ -598,14 +1029,40 @@ hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nr
     return hipErrorInvalidValue;
 }
 
+#define FP_ROWREG_SIZES(X) X(24) X(40)
+constexpr int ILU_ROW_MAXR = 4;
+#ifndef ILU_ROW_WAVES
+#define ILU_ROW_WAVES 8
+#endif
+
 template <typename TF>
-hipError_t launch_ilu_fwd_level(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* colidx,
-                                const int* diagpos, const TF* F, const IluIO* io, TF* y) {
+hipError_t launch_ilu_factor_rows(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* diagpos,
+                                  const int* tptr, const int* tasks, TF* F, int* status) {
+#define X(BS)                                                                                                          \
+    if (B == BS) {                                                                                                     \
+        hipLaunchKernelGGL((k_ilu_factor_rows<TF, BS, ILU_ROW_MAXR, ILU_ROW_WAVES>), dim3(nrows), dim3(64 * ILU_ROW_WAVES), 0, \
+                           s, rows, nrows, rowptr,                                                                     \
+                           diagpos, tptr, reinterpret_cast<const int2*>(tasks), F, status);                            \
+        return hipGetLastError();                                                                                      \
+    }
+    FP_ROWREG_SIZES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+bool ilu_factor_rows_supported(int B, int max_row_blocks) {
+#define X(BS) if (B == BS) return max_row_blocks <= ILU_ROW_MAXR;
+    FP_ROWREG_SIZES(X)
+#undef X
+    return false;
+}
+
+template <typename TF>
+hipError_t launch_ilu_fwd_level(hipStream_t s, int B, const int* meta, int nrows, const TF* F, const IluIO* io, TF* y) {
 #define X(BS)                                                                                                      \
     if (B == BS) {                                                                                                 \
         constexpr int R = RowsPerWave<BS>::value;                                                                  \
-        hipLaunchKernelGGL((k_ilu_fwd_level<TF, BS>), dim3((nrows + R - 1) / R), dim3(64), 0, s, rows, nrows, rowptr, \
-                           colidx, diagpos, F, io, y);                                                             \
+        hipLaunchKernelGGL((k_ilu_fwd_level<TF, BS>), dim3((nrows + R - 1) / R), dim3(64), 0, s, meta, nrows, F, io, y); \
         return hipGetLastError();                                                                                  \
     }
     FP_BLOCK_SIZES(X)
@@ -614,13 +1071,13 @@ hipError_t launch_ilu_fwd_level(hipStream_t s, int B, const int* rows, int nrows
 }
 
 template <typename TF>
-hipError_t launch_ilu_bwd_level(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* colidx,
-                                const int* diagpos, const TF* F, const TF* y, TF* xs, const IluIO* io) {
+hipError_t launch_ilu_bwd_level(hipStream_t s, int B, const int* meta, int nrows, const TF* F, const TF* y, TF* xs,
+                                const IluIO* io) {
 #define X(BS)                                                                                                      \
     if (B == BS) {                                                                                                 \
         constexpr int R = RowsPerWave<BS>::value;                                                                  \
-        hipLaunchKernelGGL((k_ilu_bwd_level<TF, BS>), dim3((nrows + R - 1) / R), dim3(64), 0, s, rows, nrows, rowptr, \
-                           colidx, diagpos, F, y, xs, io);                                                         \
+        hipLaunchKernelGGL((k_ilu_bwd_level<TF, BS>), dim3((nrows + R - 1) / R), dim3(64), 0, s, meta, nrows, F, y, xs, \
+                           io);                                                                                    \
         return hipGetLastError();                                                                                  \
     }
     FP_BLOCK_SIZES(X)
@@ -637,10 +1094,10 @@ hipError_t launch_set_io(hipStream_t s, IluIO* io, const double* b, double* x) {
     template hipError_t launch_ilu_copy<TF>(hipStream_t, size_t, const double*, TF*);                                          \
     template hipError_t launch_ilu_factor_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, \
                                                     int*);                                                                     \
-    template hipError_t launch_ilu_fwd_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*,        \
-                                                 const TF*, const IluIO*, TF*);                                                \
-    template hipError_t launch_ilu_bwd_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*,        \
-                                                 const TF*, const TF*, TF*, const IluIO*);
+    template hipError_t launch_ilu_factor_rows<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*,      \
+                                                   const int*, TF*, int*);                                                     \
+    template hipError_t launch_ilu_fwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);             \
+    template hipError_t launch_ilu_bwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*);
 FP_ILU_INST(double)
 FP_ILU_INST(float)
 

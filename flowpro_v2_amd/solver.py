@@ -73,7 +73,9 @@ class DeviceCase:
         cs.ordering = 1 if str(props.get("elementOrdering", "rcm")).lower() == "rcm" else 0
         cs.preconditioner = PRECONDITIONERS[prec]
         cs.precond_bits = int(precond_bits)
-        cs.gmres_restart, cs.gmres_cycles = 30, 5
+        # Gmres(A, M, 30, 5, ...) in LinearSolver.java:53; the optional keys set the same arguments
+        cs.gmres_restart = int(props.get("gmresRestart", 30))
+        cs.gmres_cycles = int(props.get("gmresCycles", 5))
         cs.gmres_tol = float(props.get("iterativeSolverTol", 1e-2))
         cs.positivity_limiter = int(bool(props.get("positivityLimiter", True)))
         h = C.c_void_p()

@@ -136,7 +136,9 @@ int fp_adjoint_solve(fp_ctx* ctx, double a1, const double* rhs, double* lambda, 
 
 /* --- measurement ------------------------------------------------------------------------ */
 /* average device time (ms) of one launch of a kernel family, timed with HIP events on the
- * library's stream: 0 residual, 1 Jacobian, 2 SpMV, 3 ILU factor, 4 ILU apply, 5 J^T v */
+ * library's stream: 0 residual, 1 Jacobian, 2 SpMV, 3 ILU factor, 4 ILU apply, 5 J^T v,
+ * 6 the level launches of the ILU factor without the value copy (all levels together;
+ * divide by the forward level count of fp_info for the average level launch) */
 int fp_time_kernel(fp_ctx* ctx, int which, int reps, double* avg_ms);
 /* hipStream_t of the context (for callers that record their own events) */
 void* fp_stream(fp_ctx* ctx);

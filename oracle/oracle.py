@@ -95,7 +95,9 @@ def props_from(p: dict, threads: int = 1) -> Props:
     P.max_cfl = float(p.get("maxCFL", 1e4))
     P.positivity_limiter = int(bool(p.get("positivityLimiter", True)))
     P.jacobian_mode = {"fd": 0, "exact": 1}[p.get("jacobian", "fd")]
-    P.gmres_m, P.gmres_cycles = 30, 5
+    # Gmres(A, M, m, iterationLimit, ...) is built with m = 30, 5 cycles (LinearSolver.java:53);
+    # the optional keys reach the same constructor arguments
+    P.gmres_m, P.gmres_cycles = int(p.get("gmresRestart", 30)), int(p.get("gmresCycles", 5))
     P.threads = threads
     P.kappa, P.cv = float(p.get("kappa", 1.4)), float(p.get("cv", 1.0))
     P.pIn0, P.rhoIn0, P.pOut = float(p.get("pIn0", 1.0)), float(p.get("rhoIn0", 1.0)), float(p.get("pOut", 0.7))

@@ -101,6 +101,30 @@ def test_linear_solve_matches_oracle(prec):
     dev.close()
 
 
+@pytest.mark.parametrize("rows_kernel", ["1", "0"], ids=["row_kernel", "level_kernel"])
+@pytest.mark.parametrize("bits", [64, 32])
+@pytest.mark.parametrize("order", [3, 4], ids=["B24", "B40"])
+def test_block_ilu0_first_krylov_direction(order, bits, rows_kernel, monkeypatch):
+    """GMRES(1) with one cycle from x0 = 0 returns x = y1 v1, v1 = M^-1 b / |M^-1 b|, so x pins
+    the block-ILU0 factor and both sweeps (M^-1 b) at the NS block sizes B = 24 and 40, for
+    both factor kernels, against the oracle's sequential BlockILU0.
+    Tolerance: 1e-10 relative with FP64 factors; 1e-5 with FP32 factors (FP32 rounding of the
+    factor, ~6e-8, amplified by the conditioning of the sweeps)."""
+    monkeypatch.setenv("FLOWPRO_ILU_ROWS", rows_kernel)
+    case = cases.cylinder_mesh(16, 5)
+    props = cases.ns_props(order=order, reynolds=300.0, gmresRestart=1, gmresCycles=1, iterativeSolverTol=1e-14)
+    o, dev, W = _setup(case, props, precond_bits=bits)
+    o.assemble(0.05, -1.0)
+    dev.assemble(0.05, -1.0, 1, True)
+    _, xref, it_ref, _ = o.linear_solve()
+    rc, it, err = dev.linear_solve()
+    assert rc in (0, 1) and it == 1
+    x = dev.solution()
+    tol = 1e-10 if bits == 64 else 1e-5
+    assert np.abs(x - xref).max() <= tol * np.abs(xref).max()
+    dev.close()
+
+
 def test_fp32_preconditioner_same_solution():
     case = cases.channel_bump(16, 4)
     props = cases.euler_props(order=2, iterativeSolverTol=1e-12)
