@@ -164,6 +164,39 @@ def cylinder_mesh(n_theta: int = 400, n_r: int = 250, growth: float = 1.01, jitt
     return Case(f"cylinder_{n_theta}x{n_r}", PXY, TP, np.array(rows, dtype=np.int32))
 
 
+def plate_channel(nx: int = 64, ny: int = 24, x0: float = -0.5, x1: float = 2.0, first: float = 1e-5,
+                  growth: float = 1.15, nx_lead: int | None = None, with_wall_distance: bool = True) -> Case:
+    """Turbulent flat-plate channel of the reference's SA example (.claude/skills/flowpro-turbulence/
+    examples/plate_sa.txt: -1 plate on x in [0, x1], -2 inlet, -3 outlet, -4 symmetry for x < 0
+    and on top): [x0, x1] x [0, H], geometric wall-normal spacing with first cell `first` and
+    ratio `growth` (H = first (growth^ny - 1) / (growth - 1)), nx_lead of the nx columns ahead of
+    the plate, alternating diagonals.  The vertex wall distance is generateDistanceFunction's
+    (fp_wall_distance) when `with_wall_distance`."""
+    if nx_lead is None:
+        nx_lead = max(1, nx // 5)
+    x = np.concatenate([np.linspace(x0, 0.0, nx_lead + 1)[:-1], np.linspace(0.0, x1, nx - nx_lead + 1)])
+    y = first * (growth ** np.arange(ny + 1) - 1.0) / (growth - 1.0)
+    X = np.repeat(x[:, None], ny + 1, 1)
+    Y = np.repeat(y[None, :], nx + 1, 0)
+    vid = np.arange((nx + 1) * (ny + 1)).reshape(nx + 1, ny + 1)
+    PXY = np.stack([X.ravel(), Y.ravel()], 1)
+    TP = _orient_ccw(PXY, _split_quads(vid))
+    rows = []
+    for i in range(nx):
+        rows.append((WALL if i >= nx_lead else INVISCID_WALL, vid[i, 0], vid[i + 1, 0]))
+        rows.append((INVISCID_WALL, vid[i + 1, ny], vid[i, ny]))
+    for j in range(ny):
+        rows.append((INLET, vid[0, j + 1], vid[0, j]))
+        rows.append((OUTLET, vid[nx, j], vid[nx, j + 1]))
+    bnd = np.array(rows, dtype=np.int32)
+    wd = None
+    if with_wall_distance:
+        from . import native
+        TT = native.neighbor_matrix(TP, bnd)
+        wd = native.wall_distance(PXY, TP, TT)
+    return Case(f"plate_{nx}x{ny}", PXY, TP, bnd, wd)
+
+
 def min_signed_area(case: Case) -> float:
     P, T = case.PXY, case.TP
     a, b, c = P[T[:, 0]], P[T[:, 1]], P[T[:, 2]]
@@ -186,6 +219,15 @@ def euler_props(mach: float = 0.67, order: int = 2, **kw) -> dict:
 
 def ns_props(mach: float = 0.5, reynolds: float = 5000.0, order: int = 4, **kw) -> dict:
     p = euler_props(mach=mach, order=order, isFlowViscous=True, reynolds=reynolds, prandtl=0.72, penalty=1.0)
+    p.update(kw)
+    return p
+
+
+def sa_props(mach: float = 0.3, reynolds: float = 1e6, order: int = 3, vt_in: float = 3.0, **kw) -> dict:
+    """SpalartAllmaras parameters of the flat-plate example with the template's model constants."""
+    p = euler_props(mach=mach, order=order, model="SpalartAllmaras", isFlowViscous=True, reynolds=reynolds,
+                    prandtl=0.72, penalty=1.0, vtIn=vt_in, sigma=0.6667, cb1=0.1355, cb2=0.622, ka=0.41,
+                    cw2=0.3, cw3=2.0, cv1=7.1, Prt=0.9)
     p.update(kw)
     return p
 

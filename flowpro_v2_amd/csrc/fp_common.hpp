@@ -17,6 +17,7 @@ namespace fp {
 //   nbrf [e*3+k]         neighbour's local face index of the shared face
 //   blkn [e*3+k]         BSR block index of (row nbr, column e), -1 on boundary faces
 //   blkd [e]             BSR block index of the diagonal block
+//   wdist[e*3+v]         wall distance of vertex v (generateWallDistanceFunction), SA only
 // Reference-element tables (tab): basis and reference gradients at the volume
 // points, per local face at the face points, weights, point coordinates and the
 // reference mass matrix and its inverse.
@@ -43,6 +44,7 @@ struct Mesh {
     const int* nbrf;
     const int* blkn;
     const int* blkd;
+    const double* wdist;   // [e*3 + v] wall distance of the element's vertices (Spalart-Allmaras)
 };
 
 // compile-time shape of the element kernels selected at run time

@@ -57,7 +57,7 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
         for (int p = 0; p < NQF; p++) {
             const double* phiL = tb.fb + (k * NQF + p) * NB;
             const double* dphiL = tb.fd + (k * NQF + p) * NB * 2;
-            T WL[4], WR[4], dWL[8], dWR[8];
+            T WL[NEQ], WR[NEQ], dWL[2 * NEQ], dWR[2 * NEQ];
 #pragma unroll
             for (int m = 0; m < NEQ; m++) {
                 T s = 0.0, sx = 0.0, sy = 0.0;
@@ -72,7 +72,7 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
                 }
                 WL[m] = s;
                 dWL[m] = sx;
-                dWL[4 + m] = sy;
+                dWL[NEQ + m] = sy;
             }
             const int pr = NQF - 1 - p;   // the neighbour traverses the shared face backwards
             if (TT > -1) {
@@ -92,7 +92,7 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
                     }
                     WR[m] = s;
                     dWR[m] = sx;
-                    dWR[4 + m] = sy;
+                    dWR[NEQ + m] = sy;
                 }
             } else {
                 double X[2];
@@ -101,21 +101,21 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
                 X[1] = ge[8] * xi[0] + ge[9] * xi[1] + ge[10];
                 boundary_value<MK>(P, WL, n, TT, X, WR);
 #pragma unroll
-                for (int m = 0; m < NEQ; m++) { dWR[m] = dWL[m]; dWR[4 + m] = dWL[4 + m]; }
+                for (int m = 0; m < NEQ; m++) { dWR[m] = dWL[m]; dWR[NEQ + m] = dWL[NEQ + m]; }
             }
-            T fn[4];
+            T fn[NEQ];
             numerical_convective_flux<MK>(P, WL, WR, n, TT, fn);
-            T g[4];
+            T g[NEQ];
 #pragma unroll
             for (int m = 0; m < NEQ; m++) g[m] = -fn[m];
             if constexpr (VISC) {
-                T Wc[4], dWc[8], fvn[4];
+                T Wc[NEQ], dWc[2 * NEQ], fvn[NEQ];
 #pragma unroll
                 for (int m = 0; m < NEQ; m++) {
                     Wc[m] = TT > -1 ? (WL[m] + WR[m]) / 2.0 : WR[m];
                     T jump = G.beta0 * (WR[m] - WL[m]) / dL;
                     dWc[m] = (dWL[m] + dWR[m]) / 2.0 + jump * n[0];
-                    dWc[4 + m] = (dWL[4 + m] + dWR[4 + m]) / 2.0 + jump * n[1];
+                    dWc[NEQ + m] = (dWL[NEQ + m] + dWR[NEQ + m]) / 2.0 + jump * n[1];
                 }
                 numerical_diffusive_flux<MK>(P, Wc, dWc, n, TT, fvn);
                 const bool ip = TT > -1 || is_ip_face<MK>(TT);
@@ -135,12 +135,12 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
         kr.face_end(k);
     }
     // ---------------- volume (DGpure.residuum) ----------------
-    if constexpr (NB > 1 || MK == MODEL_SCALAR) {
+    if constexpr (NB > 1 || MK == MODEL_SCALAR || MK == MODEL_SA) {
 #pragma unroll 1
         for (int q = 0; q < NQV; q++) {
             const double* phi = tb.vb + q * NB;
             const double* dphi = tb.vd + q * NB * 2;
-            T Wq[4], dWq[8];
+            T Wq[NEQ], dWq[2 * NEQ];
 #pragma unroll
             for (int m = 0; m < NEQ; m++) {
                 T s = 0.0, sx = 0.0, sy = 0.0;
@@ -155,16 +155,16 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
                 }
                 Wq[m] = s;
                 dWq[m] = sx;
-                dWq[4 + m] = sy;
+                dWq[NEQ + m] = sy;
             }
             const double jw = jac * tb.vw[q];
-            T fx[4], fy[4];
+            T fx[NEQ], fy[NEQ];
             if constexpr (NB > 1) {
                 const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
                 convective_flux<MK>(P, Wq, ex, fx);
                 convective_flux<MK>(P, Wq, ey, fy);
                 if constexpr (VISC) {
-                    T gx[4], gy[4];
+                    T gx[NEQ], gy[NEQ];
                     diffusive_flux<MK>(P, Wq, dWq, ex, gx);
                     diffusive_flux<MK>(P, Wq, dWq, ey, gy);
 #pragma unroll
@@ -174,11 +174,17 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
 #pragma unroll
                 for (int m = 0; m < NEQ; m++) { fx[m] = 0.0; fy[m] = 0.0; }
             }
-            T src[4];
+            T src[NEQ];
             if constexpr (MK == MODEL_SCALAR) {
                 const double* xi = tb.vx + q * 2;
                 double X[2] = {ge[5] * xi[0] + ge[6] * xi[1] + ge[7], ge[8] * xi[0] + ge[9] * xi[1] + ge[10]};
                 source_term(P, X, src);
+            }
+            if constexpr (MK == MODEL_SA) {
+                const double* xi = tb.vx + q * 2;
+                const double* wd = ms.wdist + (size_t)e * 3;
+                const double D = (1.0 - xi[0] - xi[1]) * wd[0] + xi[0] * wd[1] + xi[1] * wd[2];
+                sa_source(P, Wq, dWq, D, src);
             }
 #pragma unroll
             for (int j = 0; j < NB; j++) {
@@ -187,7 +193,7 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
 #pragma unroll
                 for (int m = 0; m < NEQ; m++) {
                     T v = fx[m] * gx + fy[m] * gy;
-                    if constexpr (MK == MODEL_SCALAR) v += src[m] * phi[j];
+                    if constexpr (MK == MODEL_SCALAR || MK == MODEL_SA) v += src[m] * phi[j];
                     K[m * NB + j] += jw * v;
                 }
             }
@@ -386,6 +392,8 @@ __global__ void __launch_bounds__(128) k_jvp(ModelParams P, DGParams G, Tables t
     X(MODEL_NAVIER_STOKES, 3, 4, 6, 2, true)  \
     X(MODEL_NAVIER_STOKES, 6, 4, 7, 3, true)  \
     X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, true) \
+    X(MODEL_SA, 3, 5, 6, 2, true)             \
+    X(MODEL_SA, 6, 5, 7, 3, true)             \
     X(MODEL_SCALAR, 1, 1, 1, 1, false)        \
     X(MODEL_SCALAR, 3, 1, 6, 2, false)        \
     X(MODEL_SCALAR, 6, 1, 7, 3, false)        \
@@ -460,7 +468,9 @@ __global__ void k_size_over_lambda(ModelParams P, Tables tb, Mesh ms, const doub
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ms.nElem) return;
     constexpr int B = NB * NEQ;
-    double wa[4] = {0, 0, 0, 0};
+    double wa[NEQ];
+#pragma unroll
+    for (int m = 0; m < NEQ; m++) wa[m] = 0.0;
 #pragma unroll
     for (int m = 0; m < NEQ; m++)
 #pragma unroll
@@ -480,7 +490,7 @@ __device__ inline bool admissible(const double* W) {
 }
 __device__ inline double admissible_theta(const double* Wavg, const double* Wx) {
     if (admissible(Wx)) return 1.0;
-    double lo = 0, hi = 1, Wt[4];
+    double lo = 0, hi = 1, Wt[4];   // admissibility depends on rho, rho u, rho v, rho E only
     for (int it = 0; it < 30; it++) {
         double mid = 0.5 * (lo + hi);
         for (int m = 0; m < 4; m++) Wt[m] = Wavg[m] + mid * (Wx[m] - Wavg[m]);
@@ -489,15 +499,17 @@ __device__ inline double admissible_theta(const double* Wavg, const double* Wx) 
     }
     return lo;
 }
-template <int NB, int NQV, int NQF>
+template <int NB, int NEQ, int NQV, int NQF>
 __global__ void k_positivity(Tables tb, Mesh ms, double* __restrict__ W, const double* __restrict__ isref, int* __restrict__ count) {
-    constexpr int NEQ = 4, B = NB * NEQ;
+    constexpr int B = NB * NEQ;
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ms.nElem || NB == 1) return;
     double w[B];
 #pragma unroll
     for (int i = 0; i < B; i++) w[i] = W[(size_t)e * B + i];
-    double wa[4] = {0, 0, 0, 0};
+    double wa[NEQ];
+#pragma unroll
+    for (int m = 0; m < NEQ; m++) wa[m] = 0.0;
 #pragma unroll
     for (int m = 0; m < NEQ; m++)
 #pragma unroll
@@ -505,7 +517,8 @@ __global__ void k_positivity(Tables tb, Mesh ms, double* __restrict__ W, const d
     if (!admissible(wa)) return;
     double theta = 1.0;
     auto at = [&](const double* phi) {
-        double wx[4] = {0, 0, 0, 0};
+        double wx[NEQ];
+        for (int m = 0; m < NEQ; m++) wx[m] = 0.0;
         for (int m = 0; m < NEQ; m++)
             for (int j = 0; j < NB; j++) wx[m] += w[m * NB + j] * phi[j];
         theta = fmin(theta, admissible_theta(wa, wx));
@@ -539,11 +552,11 @@ hipError_t launch_size_over_lambda(const ElemLaunch& L, hipStream_t s, const Mod
 
 hipError_t launch_positivity(const ElemLaunch& L, hipStream_t s, const Tables& tb, const Mesh& ms, double* W,
                              const double* isref, int* count) {
-    if (L.model != MODEL_NAVIER_STOKES) return hipSuccess;
+    if (L.model == MODEL_SCALAR) return hipSuccess;
     dim3 blk(128), grd((ms.nElem + 127) / 128);
-#define X(MK, NB, NEQ, NQV, NQF, V)                                                                   \
-    if (MK == MODEL_NAVIER_STOKES && FP_MATCH(MK, NB, NEQ, NQV, NQF, V)) {                             \
-        hipLaunchKernelGGL((k_positivity<NB, NQV, NQF>), grd, blk, 0, s, tb, ms, W, isref, count);    \
+#define X(MK, NB, NEQ, NQV, NQF, V)                                                                        \
+    if (MK != MODEL_SCALAR && FP_MATCH(MK, NB, NEQ, NQV, NQF, V)) {                                         \
+        hipLaunchKernelGGL((k_positivity<NB, NEQ, NQV, NQF>), grd, blk, 0, s, tb, ms, W, isref, count);    \
         return hipGetLastError();                                                                      \
     }
     FP_CASES(X)

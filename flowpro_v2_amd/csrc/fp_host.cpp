@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -82,7 +83,7 @@ struct fp_ctx {
     std::vector<int> lvl_f_ptr, lvl_b_ptr;
     // device
     hipStream_t s = nullptr;
-    double *geo = nullptr, *fgeo = nullptr, *tab = nullptr, *esize = nullptr, *isref = nullptr;
+    double *geo = nullptr, *fgeo = nullptr, *tab = nullptr, *esize = nullptr, *isref = nullptr, *wdist = nullptr;
     int *nbr = nullptr, *nbrf = nullptr, *blkn = nullptr, *blkd = nullptr;
     int *rowptr = nullptr, *colidx = nullptr, *diagpos = nullptr, *lvl_f = nullptr, *lvl_b = nullptr;
     int *ftptr = nullptr, *ftask = nullptr;   // per-row task lists of the ILU0 factor
@@ -238,6 +239,45 @@ extern "C" int fp_rcm_order(int n, const int* TT, int* order) {
     return FP_OK;
 }
 
+// FlowProMain.generateDistanceFunction (FlowProMain.java:951-1014) with the wall faces as
+// sources: Bellman-Ford relaxation along the element edges, in the reference's sweep order
+// (elements, then vertex pairs j != k), until a sweep changes nothing.
+extern "C" int fp_wall_distance(int n_points, const double* pxy, int n_elems, const int* tp, const int* tt, double* dist) {
+    if (n_points < 0 || n_elems < 0 || !pxy || !tp || !tt || !dist) return FP_EINVAL;
+    const double inf = std::numeric_limits<double>::infinity();
+    std::fill(dist, dist + n_points, inf);
+    for (int i = 0; i < n_elems; i++)
+        for (int k = 0; k < 3; k++)
+            if (tt[3 * i + k] == -1) {   // Aerodynamics.isIPFace: the wall
+                dist[tp[3 * i + k]] = 0;
+                dist[tp[3 * i + (k + 1) % 3]] = 0;
+            }
+    bool changed = true;
+    for (int iter = 0; changed && iter <= n_points; iter++) {
+        changed = false;
+        for (int i = 0; i < n_elems; i++)
+            for (int j = 0; j < 3; j++) {
+                const int pj = tp[3 * i + j];
+                for (int k = 0; k < 3; k++) {
+                    const int pk = tp[3 * i + k];
+                    if (j == k || !(dist[pk] < dist[pj])) continue;
+                    const double dx = pxy[2 * pj] - pxy[2 * pk], dy = pxy[2 * pj + 1] - pxy[2 * pk + 1];
+                    const double y = dist[pk] + std::sqrt(dx * dx + dy * dy);
+                    if (y < dist[pj]) {
+                        dist[pj] = y;
+                        changed = true;
+                    }
+                }
+            }
+    }
+    for (int i = 0; i < n_points; i++)
+        if (std::isinf(dist[i])) {   // no wall face: the reference fills 1e30
+            std::fill(dist, dist + n_points, 1e30);
+            break;
+        }
+    return FP_OK;
+}
+
 // ============================================================== basis (basis2Dtriangle)
 namespace {
 struct Basis {
@@ -342,7 +382,7 @@ static void free_all(fp_ctx* c) {
     if (c->g_factor) (void)hipGraphExecDestroy(c->g_factor);
     if (c->g_apply) (void)hipGraphExecDestroy(c->g_apply);
     if (c->io) (void)hipFree(c->io);
-    void* ptrs[] = {c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
+    void* ptrs[] = {c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
                     c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->val, c->F, c->iy, c->ixs, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
                     c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->istat};
     for (void* p : ptrs)
@@ -356,12 +396,17 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     c->order = cs->order;
     Basis bs = make_basis(cs->order);
     c->NB = bs.nb;
-    c->NEQ = cs->model == MODEL_SCALAR ? 1 : 4;
+    c->NEQ = cs->model == MODEL_SCALAR ? 1 : (cs->model == MODEL_SA ? 5 : 4);
+    if (cs->model == MODEL_SA && !cs->wall_distance) {
+        c->err = "the Spalart-Allmaras model needs the vertex wall distance";
+        return FP_EINVAL;
+    }
     c->B = c->NB * c->NEQ;
     c->NQV = cs->n_vol;
     c->NQF = cs->n_face;
     c->dofs = (long long)n * c->B;
-    const bool diffusive = cs->model == MODEL_SCALAR ? cs->scalar_k > 0 : cs->viscous != 0;
+    // SpalartAllmaras.init passes isDiffusive = true to Aerodynamics.init (SpalartAllmaras.java:38)
+    const bool diffusive = cs->model == MODEL_SCALAR ? cs->scalar_k > 0 : (cs->model == MODEL_SA || cs->viscous != 0);
     c->L = ElemLaunch{cs->model, c->NB, c->NEQ, c->NQV, c->NQF, diffusive ? 1 : 0};
     if (!element_supported(c->L)) {
         c->err = "no element kernel for this model / order / quadrature";
@@ -392,6 +437,16 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     P.a1 = cs->scalar_a1;
     P.k = cs->scalar_k;
     P.lambda = std::sqrt(cs->scalar_a0 * cs->scalar_a0 + cs->scalar_a1 * cs->scalar_a1);
+    P.vtIn = cs->sa_vt_in;
+    P.sigma = cs->sa_sigma;
+    P.cb1 = cs->sa_cb1;
+    P.cb2 = cs->sa_cb2;
+    P.ka = cs->sa_ka;
+    P.cw1 = P.ka != 0 && P.sigma != 0 ? P.cb1 / (P.ka * P.ka) + (1 + P.cb2) / P.sigma : 0.0;   // SpalartAllmaras.java:53
+    P.cw2 = cs->sa_cw2;
+    P.cw3 = cs->sa_cw3;
+    P.cv1 = cs->sa_cv1;
+    P.Prt = cs->sa_prt;
     c->dg.beta0 = cs->order == 1 ? 0.0 : cs->beta0;                  // DGpure.java:287-289
     c->dg.cIP = (cs->order > 1 && diffusive) ? cs->penalty : 0.0;   // DGpure.limiter :396-398
 
@@ -459,6 +514,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     // --- geometry in the DOF order (transformation2Dtriangle, faceTransformation2Dline,
     //     Element.computeGeometry)
     std::vector<double> geo((size_t)n * GEO), fgeo((size_t)n * 12), esize(n), Xs(2 * (size_t)n);
+    std::vector<double> wdist(cs->wall_distance ? 3 * (size_t)n : 0);
     std::vector<int> nbr(3 * (size_t)n), nbrf(3 * (size_t)n);
     for (int r = 0; r < n; r++) {
         int e = c->order_[r];
@@ -476,6 +532,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         g[5] = A0; g[6] = A1; g[7] = p0[0];
         g[8] = B0; g[9] = B1; g[10] = p0[1];
         g[11] = 0;
+        if (cs->wall_distance)
+            for (int v = 0; v < 3; v++) wdist[3 * (size_t)r + v] = cs->wall_distance[cs->tp[3 * e + v]];
         Xs[2 * r] = A0 / 3 + A1 / 3 + p0[0];
         Xs[2 * r + 1] = B0 / 3 + B1 / 3 + p0[1];
     }
@@ -632,7 +690,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     HIPCHK(c, upload(&c->meta_f, meta_f));
     HIPCHK(c, upload(&c->meta_b, meta_b));
     HIPCHK(c, upload(&c->ftask, ftask));
-    c->ms = Mesh{n, c->geo, c->fgeo, c->nbr, c->nbrf, c->blkn, c->blkd};
+    if (!wdist.empty()) HIPCHK(c, upload(&c->wdist, wdist));
+    c->ms = Mesh{n, c->geo, c->fgeo, c->nbr, c->nbrf, c->blkn, c->blkd, c->wdist};
 
     // --- state, matrix, Krylov workspace
     const size_t N = (size_t)c->dofs, BB = (size_t)c->B * c->B;

@@ -1009,7 +1009,7 @@ hipError_t launch_ilu_copy(hipStream_t s, size_t n, const double* src, TF* dst) 
     return hipGetLastError();
 }
 
-#define FP_BLOCK_SIZES(X) X(1) X(3) X(6) X(10) X(12) X(24) X(40)
+#define FP_BLOCK_SIZES(X) X(1) X(3) X(6) X(10) X(12) X(15) X(24) X(30) X(40)
 
 template <typename TF>
 hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* colidx,
@@ -1029,7 +1029,7 @@ hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nr
     return hipErrorInvalidValue;
 }
 
-#define FP_ROWREG_SIZES(X) X(24) X(40)
+#define FP_ROWREG_SIZES(X) X(24) X(30) X(40)
 constexpr int ILU_ROW_MAXR = 4;
 #ifndef ILU_ROW_WAVES
 #define ILU_ROW_WAVES 8

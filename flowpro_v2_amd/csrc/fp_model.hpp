@@ -1,11 +1,12 @@
-// Device models of the implicit-DG path: compressible Euler / Navier-Stokes and
-// the linear scalar test equation, written once over a scalar type T that is
+// Device models of the implicit-DG path: compressible Euler / Navier-Stokes, the
+// Spalart-Allmaras RANS model and the linear scalar test equation, written once over a
+// scalar type T that is
 // either double (residual) or a forward-mode dual number (exact Jacobian
 // columns, J*v, J^T*v).  The call sequence of each function follows the
 // reference model classes so that the density clipping `limiteRho`, which
 // overwrites its argument, acts on the same values:
 //   ModuleProjects/Equations/Aerodynamics/src/main/java/flowpro/user/equation/
-//     Aerodynamics.java:323-370, NavierStokes.java:23-395
+//     Aerodynamics.java:323-370, NavierStokes.java:23-395, SpalartAllmaras.java:36-459
 //   FlowPro/src/test/java/flowpro/user/equation/TestScalarEquation.java:28-136
 #pragma once
 #include <hip/hip_runtime.h>
@@ -48,9 +49,19 @@ __device__ inline dual t_pow(dual a, double b) {
 // java.lang.Math.max for ordinary values: (a >= b) ? a : b
 __device__ inline double t_max(double a, double b) { return (a >= b) ? a : b; }
 __device__ inline dual t_max(dual a, dual b) { return (a.v >= b.v) ? a : b; }
+// SpalartAllmaras.max (SpalartAllmaras.java:453-459): (a > b) ? a : b
+__device__ inline double t_sa_max(double a, double b) { return (a > b) ? a : b; }
+__device__ inline dual t_sa_max(dual a, dual b) { return (a.v > b.v) ? a : b; }
+// java.lang.Math.min for ordinary values: (a <= b) ? a : b
+__device__ inline double t_min(double a, double b) { return (a <= b) ? a : b; }
+__device__ inline dual t_min(dual a, dual b) { return (a.v <= b.v) ? a : b; }
+__device__ inline double t_abs(double a) { return fabs(a); }
+__device__ inline dual t_abs(dual a) { return a.v < 0.0 ? -a : a; }
 
 enum BoundaryType { BT_WALL = -1, BT_INLET = -2, BT_OUTLET = -3, BT_INVISCID_WALL = -4 };
-enum ModelKind { MODEL_NAVIER_STOKES = 0, MODEL_SCALAR = 2 };
+enum ModelKind { MODEL_NAVIER_STOKES = 0, MODEL_SA = 1, MODEL_SCALAR = 2 };
+// equations per model: rho, rho u, rho v, rho E (+ rho nu~ for Spalart-Allmaras); 1 for the scalar
+template <int MK> constexpr int neq_of() { return MK == MODEL_SCALAR ? 1 : (MK == MODEL_SA ? 5 : 4); }
 
 // Derived model constants (the host computes them as Aerodynamics.init does).
 struct ModelParams {
@@ -63,24 +74,30 @@ struct ModelParams {
     double Re, Pr;
     double attack;
     double a0, a1, k, lambda;   // scalar model
+    // Spalart-Allmaras: inlet turbulence and model constants, cw1 = cb1/ka^2 + (1 + cb2)/sigma
+    double vtIn, sigma, cb1, cb2, ka, cw1, cw2, cw3, cv1, Prt;
 };
 
 constexpr double RHO_TOL = 1e-2;   // Aerodynamics.java:22
 constexpr double P_TOL = 1e-2;     // NavierStokes.java:15
+constexpr double P_TOL_SA = 1e-1;  // SpalartAllmaras.java:15
 
 template <class T> __device__ inline T limite_rho(T rho) {
     if (value(rho) < RHO_TOL) return RHO_TOL * t_exp((rho - RHO_TOL) / RHO_TOL);
     return rho;
 }
 
-template <class T> __device__ inline T pressure(const ModelParams& P, T* W) {
+// NavierStokes.pressure / SpalartAllmaras.pressure (same formula, own P_TOL)
+template <int MK, class T> __device__ inline T pressure_m(const ModelParams& P, T* W) {
+    constexpr double ptol = MK == MODEL_SA ? P_TOL_SA : P_TOL;
     W[0] = limite_rho(W[0]);
     T mom2 = W[1] * W[1];
     mom2 += W[2] * W[2];
     T p = (P.kapa - 1.0) * (W[3] - mom2 / (2.0 * W[0]));
-    if (value(p) < P_TOL) p = P_TOL * t_exp((p - P_TOL) / P_TOL);
+    if (value(p) < ptol) p = ptol * t_exp((p - ptol) / ptol);
     return p;
 }
+template <class T> __device__ inline T pressure(const ModelParams& P, T* W) { return pressure_m<MODEL_NAVIER_STOKES>(P, W); }
 
 // physical convective flux in direction n
 template <int MK, class T> __device__ inline void convective_flux(const ModelParams& P, T* W, const double* n, T* f) {
@@ -93,17 +110,18 @@ template <int MK, class T> __device__ inline void convective_flux(const ModelPar
     T V = W[1] * n[0];
     V += W[2] * n[1];
     V /= W[0];
-    T p = pressure(P, W);
+    T p = pressure_m<MK>(P, W);
     f[0] = W[0] * V;
     f[1] = W[1] * V + p * n[0];
     f[2] = W[2] * V + p * n[1];
     f[3] = (W[3] + p) * V;
+    if constexpr (MK == MODEL_SA) f[4] = W[4] * V;
 }
 
 template <int MK, class T> __device__ inline T max_eigenvalue(const ModelParams& P, T* W) {
     if constexpr (MK == MODEL_SCALAR) return T(P.lambda);
     W[0] = limite_rho(W[0]);
-    T p = pressure(P, W);
+    T p = pressure_m<MK>(P, W);
     T a = t_sqrt(P.kapa * p / W[0]);
     T v1 = W[1] / W[0], v2 = W[2] / W[0];
     T v = v1 * v1;
@@ -121,28 +139,31 @@ __device__ inline void numerical_convective_flux(const ModelParams& P, T* WL, T*
         f[0] = (fL[0] + fR[0]) / 2.0 - P.lambda * (WR[0] - WL[0]) / 2.0;
         return;
     }
+    constexpr int NE = neq_of<MK>();
     WL[0] = limite_rho(WL[0]);
     WR[0] = limite_rho(WR[0]);
     if (TT == BT_WALL || TT == BT_INVISCID_WALL) {
-        T p = pressure(P, WL);
+        // NavierStokes takes p(WL), SpalartAllmaras p(WR) (SpalartAllmaras.java:146)
+        T p = pressure_m<MK>(P, MK == MODEL_SA ? WR : WL);
         f[0] = 0.0;
         f[1] = p * n[0];
         f[2] = p * n[1];
         f[3] = 0.0;   // p * (u_mesh . n) with u_mesh = 0
+        if constexpr (MK == MODEL_SA) f[4] = 0.0;
         return;
     }
     if (TT == BT_INLET || TT == BT_OUTLET) {
         convective_flux<MK>(P, WR, n, f);
         return;
     }
-    T fL[4], fR[4];
+    T fL[NE], fR[NE];
     convective_flux<MK>(P, WL, n, fL);
     convective_flux<MK>(P, WR, n, fR);
     T lL = max_eigenvalue<MK>(P, WL);
     T lR = max_eigenvalue<MK>(P, WR);
     T lam = t_max(lL, lR);
 #pragma unroll
-    for (int j = 0; j < 4; j++) f[j] = (fL[j] + fR[j] - lam * (WR[j] - WL[j])) / 2.0;
+    for (int j = 0; j < NE; j++) f[j] = (fL[j] + fR[j] - lam * (WR[j] - WL[j])) / 2.0;
 }
 
 // boundary state from the inner state (subsonic stagnation inlet, pressure outlet, walls)
@@ -152,20 +173,28 @@ __device__ inline void boundary_value(const ModelParams& P, T* WL, const double*
         WR[0] = sin(2.0 * X[0] + X[1]);
         return;
     }
+    constexpr int NE = neq_of<MK>();
     WL[0] = limite_rho(WL[0]);
-    T p = pressure(P, WL);
+    T p = pressure_m<MK>(P, WL);
     const double kapa = P.kapa;
     if (TT == BT_WALL && P.diffusive) {
         WR[0] = WL[0];
         WR[1] = 0.0;
         WR[2] = 0.0;
         WR[3] = p / (kapa - 1.0);   // + rho |u_mesh|^2 / 2 with u_mesh = 0
+        if constexpr (MK == MODEL_SA) WR[4] = 0.0;
     } else if (TT == BT_WALL || TT == BT_INVISCID_WALL) {
         T nu = WL[1] * n[0];
         nu += WL[2] * n[1];
         WR[0] = WL[0];
-        WR[1] = WL[1] - n[0] * nu;
-        WR[2] = WL[2] - n[1] * nu;
+        if constexpr (MK == MODEL_SA) {   // SpalartAllmaras adds n nu (SpalartAllmaras.java:229-231)
+            WR[1] = WL[1] + n[0] * nu;
+            WR[2] = WL[2] + n[1] * nu;
+            WR[4] = WL[4];
+        } else {
+            WR[1] = WL[1] - n[0] * nu;
+            WR[2] = WL[2] - n[1] * nu;
+        }
         WR[3] = WL[3];
     } else if (TT == BT_INLET) {
         const double pIn0 = 1.0, rhoIn0 = 1.0;
@@ -184,31 +213,36 @@ __device__ inline void boundary_value(const ModelParams& P, T* WL, const double*
             WR[2] = rhoIn * vel * sin(P.attack);
         }
         WR[3] = E;
+        if constexpr (MK == MODEL_SA) WR[4] = rhoIn * P.vtIn;
     } else if (TT == BT_OUTLET) {
         T rhoOut = WL[0];
         T av2 = WL[1] * WL[1];
         av2 += WL[2] * WL[2];
         av2 /= rhoOut * rhoOut;
-        T a = t_sqrt(kapa * pressure(P, WL) / WL[0]);
+        T a = t_sqrt(kapa * pressure_m<MK>(P, WL) / WL[0]);
         T mach = t_sqrt(av2) / a;
         if (value(mach) < 1.0) {
             WR[0] = rhoOut;
             WR[1] = rhoOut * WL[1] / WL[0];
             WR[2] = rhoOut * WL[2] / WL[0];
             WR[3] = P.pOut / (kapa - 1.0) + rhoOut * av2 / 2.0;
+            if constexpr (MK == MODEL_SA) WR[4] = WL[4];
         } else {
-            WR[0] = WL[0]; WR[1] = WL[1]; WR[2] = WL[2]; WR[3] = WL[3];
+#pragma unroll
+            for (int m = 0; m < NE; m++) WR[m] = WL[m];
         }
     } else {
-        WR[0] = 0.0; WR[1] = 0.0; WR[2] = 0.0; WR[3] = 0.0;
+#pragma unroll
+        for (int m = 0; m < NE; m++) WR[m] = 0.0;
     }
 }
 
-// viscous flux in direction n; dW[d*4 + m] = d W_m / d x_d
+// viscous flux in direction n; dW[d*NE + m] = d W_m / d x_d
 template <int MK, class T> __device__ inline void diffusive_flux(const ModelParams& P, T* W, const T* dW, const double* n, T* flux) {
+    constexpr int NE = neq_of<MK>();
     if constexpr (MK == MODEL_SCALAR) {
-        T fl = P.k * dW[0] * n[0];   // dW[d*4 + m]: x derivative at 0, y derivative at 4
-        fl += P.k * dW[4] * n[1];
+        T fl = P.k * dW[0] * n[0];
+        fl += P.k * dW[NE] * n[1];
         flux[0] = fl;
         return;
     }
@@ -222,15 +256,15 @@ template <int MK, class T> __device__ inline void diffusive_flux(const ModelPara
 #pragma unroll
     for (int d = 0; d < 2; ++d)
 #pragma unroll
-        for (int f = 0; f < 2; ++f) vj[2 * d + f] = (dW[f * 4 + d + 1] - dW[f * 4] * vel[d]) / rho;
-    T p = pressure(P, W);
+        for (int f = 0; f < 2; ++f) vj[2 * d + f] = (dW[f * NE + d + 1] - dW[f * NE] * vel[d]) / rho;
+    T p = pressure_m<MK>(P, W);
     T prd[2];
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
         T temp = vel[0] * vj[d];
         temp += vel[1] * vj[2 + d];
-        T pDer = (P.kapa - 1.0) * (dW[d * 4 + 3] - dW[d * 4] * vel2 / 2.0 - rho * temp);
-        prd[d] = (rho * pDer - p * dW[d * 4]) / (rho * rho);
+        T pDer = (P.kapa - 1.0) * (dW[d * NE + 3] - dW[d * NE] * vel2 / 2.0 - rho * temp);
+        prd[d] = (rho * pDer - p * dW[d * NE]) / (rho * rho);
     }
     T st[4];
     T trace = vj[0];
@@ -241,20 +275,40 @@ template <int MK, class T> __device__ inline void diffusive_flux(const ModelPara
         for (int f = 0; f < 2; ++f) st[2 * d + f] = vj[2 * d + f] + vj[2 * f + d];
     st[0] += lam * trace;
     st[3] += lam * trace;
-    const double cst = P.kapa / (P.kapa - 1.0) / P.Pr;
-    flux[0] = 0.0;
-    flux[1] = 0.0;
-    flux[2] = 0.0;
-    flux[3] = 0.0;
 #pragma unroll
-    for (int d = 0; d < 2; ++d) {
-        T tmp = 0.0;
+    for (int m = 0; m < NE; m++) flux[m] = 0.0;
+    if constexpr (MK == MODEL_SA) {   // SpalartAllmaras.java:325-349
+        T vt = t_sa_max(T(0.0), W[4] / rho);
+        T vtDer[2];
 #pragma unroll
-        for (int f = 0; f < 2; ++f) {
-            flux[f + 1] += st[2 * d + f] * n[d] / P.Re;
-            tmp += vel[f] * st[2 * d + f];
+        for (int d = 0; d < 2; ++d) vtDer[d] = 1.0 / rho * (dW[d * NE + 4] - dW[d * NE] * vt);
+        T xi = rho * vt;
+        T fv1 = xi * xi * xi / (xi * xi * xi + P.cv1 * P.cv1 * P.cv1);
+        T mut = rho * vt * fv1;
+        T cst = P.kapa / (P.kapa - 1.0) * (1.0 / P.Pr + mut / P.Prt);
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            T tmp = 0.0;
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                flux[f + 1] += (1.0 + mut) * st[2 * d + f] * n[d] / P.Re;
+                tmp += vel[f] * (1.0 + mut) * st[2 * d + f];
+            }
+            flux[3] += (tmp + cst * prd[d]) * n[d] / P.Re;
+            flux[4] += 1.0 / (P.Re * P.sigma) * (1.0 + rho * vt) * vtDer[d] * n[d];
         }
-        flux[3] += (tmp + cst * prd[d]) * n[d] / P.Re;
+    } else {
+        const double cst = P.kapa / (P.kapa - 1.0) / P.Pr;
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            T tmp = 0.0;
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                flux[f + 1] += st[2 * d + f] * n[d] / P.Re;
+                tmp += vel[f] * st[2 * d + f];
+            }
+            flux[3] += (tmp + cst * prd[d]) * n[d] / P.Re;
+        }
     }
 }
 
@@ -264,13 +318,15 @@ __device__ inline void numerical_diffusive_flux(const ModelParams& P, T* Wc, con
         diffusive_flux<MK>(P, Wc, dWc, n, flux);
         return;
     }
+    constexpr int NE = neq_of<MK>();
     Wc[0] = limite_rho(Wc[0]);
     diffusive_flux<MK>(P, Wc, dWc, n, flux);
     if (TT < 0) {
         if (TT == BT_WALL) {
             flux[3] = 0.0;   // u_mesh . (tau n) with u_mesh = 0
         } else {
-            flux[0] = 0.0; flux[1] = 0.0; flux[2] = 0.0; flux[3] = 0.0;
+#pragma unroll
+            for (int m = 0; m < NE; m++) flux[m] = 0.0;
         }
     }
 }
@@ -280,6 +336,47 @@ template <int MK> __device__ inline bool is_ip_face(int TT) { return MK == MODEL
 template <class T> __device__ inline void source_term(const ModelParams& P, const double* X, T* s) {
     double phase = 2.0 * X[0] + X[1];
     s[0] = (2.0 * P.a0 + P.a1) * cos(phase) + 5.0 * P.k * sin(phase);
+}
+
+// SpalartAllmaras.sourceTerm (SpalartAllmaras.java:367-451) in 2D with ft2 = 0; D is the wall
+// distance interpolated at the point.  The stress magnitude only enters a commented-out
+// production limiter of the reference and is not evaluated.
+template <class T> __device__ inline void sa_source(const ModelParams& P, T* W, const T* dW, double D, T* s) {
+    constexpr int NE = 5;
+    W[0] = limite_rho(W[0]);
+    T rho = W[0];
+    T vel[2] = {W[1] / rho, W[2] / rho};
+    T vj[4];   // vj[2d + f] = d v_d / d x_f
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+#pragma unroll
+        for (int f = 0; f < 2; ++f) vj[2 * d + f] = (dW[f * NE + d + 1] - dW[f * NE] * vel[d]) / rho;
+    T vt = t_sa_max(T(0.0), W[4] / rho);
+    T vtDerMag = 0.0;
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        T vtDer = 1.0 / rho * (dW[d * NE + 4] - dW[d * NE] * vt);
+        vtDerMag += vtDer * vtDer;
+    }
+    if (value(vt) < 0.0) vt = 0.0;
+    const double ka = P.ka;
+    T xi = rho * vt;
+    T fv1 = xi * xi * xi / (xi * xi * xi + P.cv1 * P.cv1 * P.cv1);
+    T fv2 = 1.0 - xi / (1.0 + xi * fv1);
+    T Om = t_abs(vj[1] - vj[2]);
+    T S = Om + 1.0 / P.Re * vt / (ka * ka * D * D) * fv2;
+    T rt = t_min(vt / (P.Re * S * ka * ka * D * D), T(10.0));
+    T g = rt + P.cw2 * (t_pow(rt, 6.0) - rt);
+    const double cw36 = pow(P.cw3, 6.0);
+    T fw = g * t_pow((1.0 + cw36) / (t_pow(g, 6.0) + cw36), 1.0 / 6);
+    T d = 1.0 / P.Re * rho * P.cb2 * vtDerMag + P.cb1 * rho * S * vt -
+          1.0 / P.Re * (P.cw1 * fw) / rho * (rho * vt / D) * (rho * vt / D);
+    if (value(d) < -100.0) d = -100.0;   // limitDestruction
+    s[0] = 0.0;
+    s[1] = 0.0;
+    s[2] = 0.0;
+    s[3] = 0.0;
+    s[4] = d;
 }
 
 }  // namespace fp

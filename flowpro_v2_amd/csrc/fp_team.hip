@@ -39,9 +39,11 @@ struct TeamLayout {
     static constexpr int E_W = 0;                        // [B]
     static constexpr int E_GV = E_W + B;                 // [NQV][NB][2] physical gradients (volume)
     static constexpr int E_GF = E_GV + NQV * NB * 2;     // [3*NQF][NB][2] physical gradients (own face points)
-    static constexpr int E_SV = E_GF + NQFT * NB * 2;    // [NQV][12]  W, dW at volume points
-    static constexpr int E_SF = E_SV + NQV * 12;         // [3*NQF][24] WL, dWL, WR, dWR at face points
-    static constexpr int ELEM0 = E_SF + NQFT * 24;
+    static constexpr int SV = 3 * NEQ;                   // W, dW/dx, dW/dy at a volume point
+    static constexpr int SF = 6 * NEQ;                   // WL, dWL/dx, dWL/dy, WR, dWR/dx, dWR/dy at a face point
+    static constexpr int E_SV = E_GF + NQFT * NB * 2;    // [NQV][SV]
+    static constexpr int E_SF = E_SV + NQV * SV;         // [3*NQF][SF]
+    static constexpr int ELEM0 = E_SF + NQFT * SF;
 };
 
 // ------------------------------------------------------------------ phase 0 / 1a helpers
@@ -97,7 +99,7 @@ __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double
         if (qp < NQV) {
             const double* phi = &tab[LY::T_VB + qp * NB];
             const double* gr = &E[LY::E_GV + qp * NB * 2];
-            double* st = &E[LY::E_SV + qp * 12];
+            double* st = &E[LY::E_SV + qp * LY::SV];
 #pragma unroll
             for (int m = 0; m < NEQ; m++) {
                 double s = 0, sx = 0, sy = 0;
@@ -109,15 +111,15 @@ __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double
                     sy += c * gr[2 * j + 1];
                 }
                 st[m] = s;
-                st[4 + m] = sx;
-                st[8 + m] = sy;
+                st[NEQ + m] = sx;
+                st[2 * NEQ + m] = sy;
             }
         } else {
             const int f = qp - NQV, k = f / NQF, p = f % NQF;
             const int e = e0 + el;
             const double* phi = &tab[LY::T_FB + f * NB];
             const double* gr = &E[LY::E_GF + f * NB * 2];
-            double* st = &E[LY::E_SF + f * 24];
+            double* st = &E[LY::E_SF + f * LY::SF];
 #pragma unroll
             for (int m = 0; m < NEQ; m++) {
                 double s = 0, sx = 0, sy = 0;
@@ -129,8 +131,8 @@ __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double
                     sy += c * gr[2 * j + 1];
                 }
                 st[m] = s;
-                st[4 + m] = sx;
-                st[8 + m] = sy;
+                st[NEQ + m] = sx;
+                st[2 * NEQ + m] = sy;
             }
             const int TT = ms.nbr[e * 3 + k];
             if (TT > -1) {
@@ -155,9 +157,9 @@ __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double
                         sx += c * gx;
                         sy += c * gy;
                     }
-                    st[12 + m] = s;
-                    st[16 + m] = sx;
-                    st[20 + m] = sy;
+                    st[3 * NEQ + m] = s;
+                    st[4 * NEQ + m] = sx;
+                    st[5 * NEQ + m] = sy;
                 }
             }
         }
@@ -170,27 +172,27 @@ __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double
 template <int MK, int NEQ, bool VISC, class T>
 __device__ inline void face_integrand(const ModelParams& P, const DGParams& G, int TT, const double* n, double dL, const double* X,
                                       T* WL, T* dWL, const double* st, T* g) {
-    T WR[4], dWR[8];
+    T WR[NEQ], dWR[2 * NEQ];
     if (TT > -1) {
 #pragma unroll
-        for (int m = 0; m < NEQ; m++) { WR[m] = st[12 + m]; dWR[m] = st[16 + m]; dWR[4 + m] = st[20 + m]; }
+        for (int m = 0; m < NEQ; m++) { WR[m] = st[3 * NEQ + m]; dWR[m] = st[4 * NEQ + m]; dWR[NEQ + m] = st[5 * NEQ + m]; }
     } else {
         boundary_value<MK>(P, WL, n, TT, X, WR);
 #pragma unroll
-        for (int m = 0; m < NEQ; m++) { dWR[m] = dWL[m]; dWR[4 + m] = dWL[4 + m]; }
+        for (int m = 0; m < NEQ; m++) { dWR[m] = dWL[m]; dWR[NEQ + m] = dWL[NEQ + m]; }
     }
-    T fn[4];
+    T fn[NEQ];
     numerical_convective_flux<MK>(P, WL, WR, n, TT, fn);
 #pragma unroll
     for (int m = 0; m < NEQ; m++) g[m] = -fn[m];
     if constexpr (VISC) {
-        T Wc[4], dWc[8], fvn[4];
+        T Wc[NEQ], dWc[2 * NEQ], fvn[NEQ];
 #pragma unroll
         for (int m = 0; m < NEQ; m++) {
             Wc[m] = TT > -1 ? (WL[m] + WR[m]) / 2.0 : WR[m];
             T jump = G.beta0 * (WR[m] - WL[m]) / dL;
             dWc[m] = (dWL[m] + dWR[m]) / 2.0 + jump * n[0];
-            dWc[4 + m] = (dWL[4 + m] + dWR[4 + m]) / 2.0 + jump * n[1];
+            dWc[NEQ + m] = (dWL[NEQ + m] + dWR[NEQ + m]) / 2.0 + jump * n[1];
         }
         numerical_diffusive_flux<MK>(P, Wc, dWc, n, TT, fvn);
         const bool ip = TT > -1 || is_ip_face<MK>(TT);
@@ -202,15 +204,21 @@ __device__ inline void face_integrand(const ModelParams& P, const DGParams& G, i
     }
 }
 
-// pointwise volume fluxes (convective minus viscous) in x and y
+// number of volume integrand slots: fluxes in x and y, plus the state-dependent source of
+// the Spalart-Allmaras model (contracted with the test function itself)
+template <int MK> constexpr int volume_slots() { return MK == MODEL_SA ? 3 : 2; }
+
+// pointwise volume fluxes (convective minus viscous) in x and y and the model source, called
+// in the order of DGpure.residuum (convection x, y, diffusion x, y, source): each call clips
+// the density of the shared state as the reference does.  D: wall distance at the point.
 template <int MK, int NB, int NEQ, bool VISC, class T>
-__device__ inline void volume_integrand(const ModelParams& P, T* Wq, T* dWq, T* fx, T* fy) {
+__device__ inline void volume_integrand(const ModelParams& P, T* Wq, T* dWq, double D, T* fx, T* fy, T* sv) {
     if constexpr (NB > 1) {
         const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
         convective_flux<MK>(P, Wq, ex, fx);
         convective_flux<MK>(P, Wq, ey, fy);
         if constexpr (VISC) {
-            T gx[4], gy[4];
+            T gx[NEQ], gy[NEQ];
             diffusive_flux<MK>(P, Wq, dWq, ex, gx);
             diffusive_flux<MK>(P, Wq, dWq, ey, gy);
 #pragma unroll
@@ -220,6 +228,14 @@ __device__ inline void volume_integrand(const ModelParams& P, T* Wq, T* dWq, T* 
 #pragma unroll
         for (int m = 0; m < NEQ; m++) { fx[m] = 0.0; fy[m] = 0.0; }
     }
+    if constexpr (MK == MODEL_SA) sa_source(P, Wq, dWq, D, sv);
+}
+
+// wall distance at reference point xi: the vertex distances interpolated linearly
+// (Element.interpolateVelocityAndFillElementDataObjectOnVolume, Element.java:263-275)
+__device__ inline double wall_distance_at(const Mesh& ms, int e, const double* xi) {
+    const double* w = ms.wdist + (size_t)e * 3;
+    return (1.0 - xi[0] - xi[1]) * w[0] + xi[0] * w[1] + xi[1] * w[2];
 }
 
 template <int NB, int NEQ, int NQV, int NQF, int EXTRA>
@@ -239,7 +255,8 @@ __global__ void __launch_bounds__(256) k_residual_team(ModelParams P, DGParams G
                                                        double* __restrict__ enorm) {
     using LY = TeamLayout<NB, NEQ, NQV, NQF>;
     constexpr int B = LY::B;
-    constexpr int EXTRA = NQV * 2 * NEQ + LY::NQFT * NEQ + 2 * B;   // integrands, K, RHS
+    constexpr int NVS = volume_slots<MK>();
+    constexpr int EXTRA = NQV * NVS * NEQ + LY::NQFT * NEQ + 2 * B;   // integrands, K, RHS
     constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EXTRA>::value;
     constexpr int STRIDE = LY::ELEM0 + EXTRA;
     __shared__ double tab[LY::TAB];
@@ -249,33 +266,35 @@ __global__ void __launch_bounds__(256) k_residual_team(ModelParams P, DGParams G
     if (nel <= 0) return;
     team_stage<NB, NEQ, NQV, NQF>(tb, ms, W, e0, nel, tab, eb, STRIDE);
     const int tid = threadIdx.x, nt = blockDim.x;
-    constexpr int O_IV = LY::ELEM0, O_IF = O_IV + NQV * 2 * NEQ, O_K = O_IF + LY::NQFT * NEQ, O_H = O_K + B;
+    constexpr int O_IV = LY::ELEM0, O_IF = O_IV + NQV * NVS * NEQ, O_K = O_IF + LY::NQFT * NEQ, O_H = O_K + B;
     // phase 1b: integrands at every quadrature point, weighted by J w
     for (int q = tid; q < nel * LY::NQ; q += nt) {
         const int el = q / LY::NQ, qp = q % LY::NQ, e = e0 + el;
         double* E = eb + el * STRIDE;
         const double* ge = ms.geo + (size_t)e * GEO;
         if (qp < NQV) {
-            const double* st = &E[LY::E_SV + qp * 12];
-            double Wq[4], dWq[8], fx[4], fy[4];
+            const double* st = &E[LY::E_SV + qp * LY::SV];
+            double Wq[NEQ], dWq[2 * NEQ], fx[NEQ], fy[NEQ], sv[NEQ];
 #pragma unroll
-            for (int m = 0; m < NEQ; m++) { Wq[m] = st[m]; dWq[m] = st[4 + m]; dWq[4 + m] = st[8 + m]; }
-            volume_integrand<MK, NB, NEQ, VISC>(P, Wq, dWq, fx, fy);
+            for (int m = 0; m < NEQ; m++) { Wq[m] = st[m]; dWq[m] = st[NEQ + m]; dWq[NEQ + m] = st[2 * NEQ + m]; }
+            const double D = MK == MODEL_SA ? wall_distance_at(ms, e, &tab[LY::T_VX + 2 * qp]) : 0.0;
+            volume_integrand<MK, NB, NEQ, VISC>(P, Wq, dWq, D, fx, fy, sv);
             const double jw = ge[4] * tab[LY::T_VW + qp];
 #pragma unroll
             for (int m = 0; m < NEQ; m++) {
-                E[O_IV + (qp * 2) * NEQ + m] = jw * fx[m];
-                E[O_IV + (qp * 2 + 1) * NEQ + m] = jw * fy[m];
+                E[O_IV + (qp * NVS) * NEQ + m] = jw * fx[m];
+                E[O_IV + (qp * NVS + 1) * NEQ + m] = jw * fy[m];
+                if constexpr (NVS == 3) E[O_IV + (qp * NVS + 2) * NEQ + m] = jw * sv[m];
             }
         } else {
             const int f = qp - NQV, k = f / NQF, p = f % NQF;
             const int TT = ms.nbr[e * 3 + k];
             const double* fg = ms.fgeo + ((size_t)e * 3 + k) * 4;
             const double n[2] = {fg[0], fg[1]};
-            const double* st = &E[LY::E_SF + f * 24];
-            double WL[4], dWL[8], g[4];
+            const double* st = &E[LY::E_SF + f * LY::SF];
+            double WL[NEQ], dWL[2 * NEQ], g[NEQ];
 #pragma unroll
-            for (int m = 0; m < NEQ; m++) { WL[m] = st[m]; dWL[m] = st[4 + m]; dWL[4 + m] = st[8 + m]; }
+            for (int m = 0; m < NEQ; m++) { WL[m] = st[m]; dWL[m] = st[NEQ + m]; dWL[NEQ + m] = st[2 * NEQ + m]; }
             const double* xi = &tab[LY::T_FX + 2 * f];
             double X[2] = {ge[5] * xi[0] + ge[6] * xi[1] + ge[7], ge[8] * xi[0] + ge[9] * xi[1] + ge[10]};
             face_integrand<MK, NEQ, VISC>(P, G, TT, n, fg[3], X, WL, dWL, st, g);
@@ -293,8 +312,11 @@ __global__ void __launch_bounds__(256) k_residual_team(ModelParams P, DGParams G
         for (int f = 0; f < LY::NQFT; f++) K += tab[LY::T_FB + f * NB + j] * E[O_IF + f * NEQ + m];
         if constexpr (NB > 1) {
             for (int qv = 0; qv < NQV; qv++)
-                K += E[O_IV + (qv * 2) * NEQ + m] * E[LY::E_GV + (qv * NB + j) * 2] +
-                     E[O_IV + (qv * 2 + 1) * NEQ + m] * E[LY::E_GV + (qv * NB + j) * 2 + 1];
+                K += E[O_IV + (qv * NVS) * NEQ + m] * E[LY::E_GV + (qv * NB + j) * 2] +
+                     E[O_IV + (qv * NVS + 1) * NEQ + m] * E[LY::E_GV + (qv * NB + j) * 2 + 1];
+        }
+        if constexpr (NVS == 3) {
+            for (int qv = 0; qv < NQV; qv++) K += E[O_IV + (qv * NVS + 2) * NEQ + m] * tab[LY::T_VB + qv * NB + j];
         }
         if constexpr (MK == MODEL_SCALAR) {
             const double* ge = ms.geo + (size_t)e * GEO;
@@ -345,10 +367,11 @@ __global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G
     using LY = TeamLayout<NB, NEQ, NQV, NQF>;
     constexpr int B = LY::B;
     constexpr int NIN = VISC ? 3 * NEQ : NEQ;
-    constexpr int EXTRA = NQV * NIN * 2 * NEQ + LY::NQFT * NIN * NEQ;
+    constexpr int NVS = volume_slots<MK>();
+    constexpr int EXTRA = NQV * NIN * NVS * NEQ + LY::NQFT * NIN * NEQ;
     constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EXTRA>::value;
     constexpr int STRIDE = LY::ELEM0 + EXTRA;
-    constexpr int O_DV = LY::ELEM0, O_DF = O_DV + NQV * NIN * 2 * NEQ;
+    constexpr int O_DV = LY::ELEM0, O_DF = O_DV + NQV * NIN * NVS * NEQ;
     __shared__ double tab[LY::TAB];
     __shared__ double eb[EPB * STRIDE];
     const int e0 = blockIdx.x * EPB;
@@ -363,31 +386,36 @@ __global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G
         double* E = eb + el * STRIDE;
         const double* ge = ms.geo + (size_t)e * GEO;
         if (qp < NQV) {
-            const double* st = &E[LY::E_SV + qp * 12];
-            dual Wq[4], dWq[8], fx[4], fy[4];
+            const double* st = &E[LY::E_SV + qp * LY::SV];
+            dual Wq[NEQ], dWq[2 * NEQ], fx[NEQ], fy[NEQ], sv[NEQ];
 #pragma unroll
             for (int m = 0; m < NEQ; m++) {
                 Wq[m] = dual(st[m], (s == 0 && m == mp) ? 1.0 : 0.0);
-                dWq[m] = dual(st[4 + m], (s == 1 && m == mp) ? 1.0 : 0.0);
-                dWq[4 + m] = dual(st[8 + m], (s == 2 && m == mp) ? 1.0 : 0.0);
+                dWq[m] = dual(st[NEQ + m], (s == 1 && m == mp) ? 1.0 : 0.0);
+                dWq[NEQ + m] = dual(st[2 * NEQ + m], (s == 2 && m == mp) ? 1.0 : 0.0);
             }
-            volume_integrand<MK, NB, NEQ, VISC>(P, Wq, dWq, fx, fy);
+            const double D = MK == MODEL_SA ? wall_distance_at(ms, e, &tab[LY::T_VX + 2 * qp]) : 0.0;
+            volume_integrand<MK, NB, NEQ, VISC>(P, Wq, dWq, D, fx, fy, sv);
             const double jw = ge[4] * tab[LY::T_VW + qp];
-            double* out = &E[O_DV + ((qp * NIN + c) * 2) * NEQ];
+            double* out = &E[O_DV + ((qp * NIN + c) * NVS) * NEQ];
 #pragma unroll
-            for (int m = 0; m < NEQ; m++) { out[m] = jw * fx[m].d; out[NEQ + m] = jw * fy[m].d; }
+            for (int m = 0; m < NEQ; m++) {
+                out[m] = jw * fx[m].d;
+                out[NEQ + m] = jw * fy[m].d;
+                if constexpr (NVS == 3) out[2 * NEQ + m] = jw * sv[m].d;
+            }
         } else {
             const int f = qp - NQV, k = f / NQF, p = f % NQF;
             const int TT = ms.nbr[e * 3 + k];
             const double* fg = ms.fgeo + ((size_t)e * 3 + k) * 4;
             const double n[2] = {fg[0], fg[1]};
-            const double* st = &E[LY::E_SF + f * 24];
-            dual WL[4], dWL[8], g[4];
+            const double* st = &E[LY::E_SF + f * LY::SF];
+            dual WL[NEQ], dWL[2 * NEQ], g[NEQ];
 #pragma unroll
             for (int m = 0; m < NEQ; m++) {
                 WL[m] = dual(st[m], (s == 0 && m == mp) ? 1.0 : 0.0);
-                dWL[m] = dual(st[4 + m], (s == 1 && m == mp) ? 1.0 : 0.0);
-                dWL[4 + m] = dual(st[8 + m], (s == 2 && m == mp) ? 1.0 : 0.0);
+                dWL[m] = dual(st[NEQ + m], (s == 1 && m == mp) ? 1.0 : 0.0);
+                dWL[NEQ + m] = dual(st[2 * NEQ + m], (s == 2 && m == mp) ? 1.0 : 0.0);
             }
             const double* xi = &tab[LY::T_FX + 2 * f];
             double X[2] = {ge[5] * xi[0] + ge[6] * xi[1] + ge[7], ge[8] * xi[0] + ge[9] * xi[1] + ge[10]};
@@ -417,7 +445,7 @@ __global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G
                 const int f = k * NQF + p;
                 // t_m = sum_s d g_m / d u_(s, mp) * psi_s(jp)
                 const double psi0 = tab[LY::T_FB + f * NB + jp];
-                double t[4];
+                double t[NEQ];
 #pragma unroll
                 for (int m = 0; m < NEQ; m++) t[m] = E[O_DF + (f * NIN + mp) * NEQ + m] * psi0;
                 if constexpr (VISC) {
@@ -448,24 +476,29 @@ __global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G
             for (int qv = 0; qv < NQV; qv++) {
                 const double psi0 = tab[LY::T_VB + qv * NB + jp];
                 const double psx = E[LY::E_GV + (qv * NB + jp) * 2], psy = E[LY::E_GV + (qv * NB + jp) * 2 + 1];
-                double tx[4], ty[4];
+                double tx[NEQ], ty[NEQ], ts[NEQ];
 #pragma unroll
                 for (int m = 0; m < NEQ; m++) {
-                    const double* d0 = &E[O_DV + ((qv * NIN + mp) * 2) * NEQ];
+                    const double* d0 = &E[O_DV + ((qv * NIN + mp) * NVS) * NEQ];
                     tx[m] = d0[m] * psi0;
                     ty[m] = d0[NEQ + m] * psi0;
+                    if constexpr (NVS == 3) ts[m] = d0[2 * NEQ + m] * psi0;
                     if constexpr (VISC) {
-                        const double* d1 = &E[O_DV + ((qv * NIN + NEQ + mp) * 2) * NEQ];
-                        const double* d2 = &E[O_DV + ((qv * NIN + 2 * NEQ + mp) * 2) * NEQ];
+                        const double* d1 = &E[O_DV + ((qv * NIN + NEQ + mp) * NVS) * NEQ];
+                        const double* d2 = &E[O_DV + ((qv * NIN + 2 * NEQ + mp) * NVS) * NEQ];
                         tx[m] += d1[m] * psx + d2[m] * psy;
                         ty[m] += d1[NEQ + m] * psx + d2[NEQ + m] * psy;
+                        if constexpr (NVS == 3) ts[m] += d1[2 * NEQ + m] * psx + d2[2 * NEQ + m] * psy;
                     }
                 }
 #pragma unroll
                 for (int j = 0; j < NB; j++) {
                     const double gx = E[LY::E_GV + (qv * NB + j) * 2], gy = E[LY::E_GV + (qv * NB + j) * 2 + 1];
 #pragma unroll
-                    for (int m = 0; m < NEQ; m++) acc[m * NB + j] += tx[m] * gx + ty[m] * gy;
+                    for (int m = 0; m < NEQ; m++) {
+                        acc[m * NB + j] += tx[m] * gx + ty[m] * gy;
+                        if constexpr (NVS == 3) acc[m * NB + j] += ts[m] * tab[LY::T_VB + qv * NB + j];
+                    }
                 }
             }
         }
@@ -488,6 +521,8 @@ __global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G
     X(MODEL_NAVIER_STOKES, 3, 4, 6, 2, true)      \
     X(MODEL_NAVIER_STOKES, 6, 4, 7, 3, true)      \
     X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, true)    \
+    X(MODEL_SA, 3, 5, 6, 2, true)                 \
+    X(MODEL_SA, 6, 5, 7, 3, true)                 \
     X(MODEL_SCALAR, 1, 1, 1, 1, false)            \
     X(MODEL_SCALAR, 3, 1, 6, 2, false)            \
     X(MODEL_SCALAR, 6, 1, 7, 3, false)            \
@@ -505,7 +540,7 @@ hipError_t launch_residual_team(const ElemLaunch& L, hipStream_t s, const ModelP
 #define X(MK, NB, NEQ, NQV, NQF, V)                                                                                   \
     if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V)) {                                                                         \
         using LY = TeamLayout<NB, NEQ, NQV, NQF>;                                                                      \
-        constexpr int EX = NQV * 2 * NEQ + LY::NQFT * NEQ + 2 * LY::B;                                                 \
+        constexpr int EX = NQV * volume_slots<MK>() * NEQ + LY::NQFT * NEQ + 2 * LY::B;                                \
         constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EX>::value;                                                     \
         hipLaunchKernelGGL((k_residual_team<MK, NB, NEQ, NQV, NQF, V>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), 0, s, P, \
                            G, tb, ms, W, Wo, Wo2, a1, a2, a3, R, RHS, enorm);                                          \
@@ -522,7 +557,7 @@ hipError_t launch_jacobian_team(const ElemLaunch& L, hipStream_t s, const ModelP
     if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V)) {                                                                         \
         using LY = TeamLayout<NB, NEQ, NQV, NQF>;                                                                      \
         constexpr int NIN = V ? 3 * NEQ : NEQ;                                                                         \
-        constexpr int EX = NQV * NIN * 2 * NEQ + LY::NQFT * NIN * NEQ;                                                 \
+        constexpr int EX = NQV * NIN * volume_slots<MK>() * NEQ + LY::NQFT * NIN * NEQ;                                \
         constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EX>::value;                                                     \
         hipLaunchKernelGGL((k_jacobian_team<MK, NB, NEQ, NQV, NQF, V, double>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), \
                            0, s, P, G, tb, ms, W, a1, blocks);                                                         \

@@ -4,6 +4,7 @@ derived constants of `init` and the constant initial condition.
 
 NavierStokes      ModuleProjects/Equations/Aerodynamics/src/main/java/flowpro/user/equation/
                   Aerodynamics.java:119-312 (init), NavierStokes.java:18-21, 101-138
+SpalartAllmaras   same directory, SpalartAllmaras.java:36-61 (init), 96-137 (constInitCondition)
 TestScalarEquation FlowPro/src/test/java/flowpro/user/equation/TestScalarEquation.java:28-37
 Any other model name is unsupported on the device and runs on the reference's Java path.
 """
@@ -12,7 +13,11 @@ from __future__ import annotations
 import math
 from dataclasses import dataclass
 
-MODEL_IDS = {"NavierStokes": 0, "TestScalarEquation": 2}
+MODEL_IDS = {"NavierStokes": 0, "SpalartAllmaras": 1, "TestScalarEquation": 2}
+# model constants of the reference's SpalartAllmaras template
+# (ModuleProjects/Equations/Aerodynamics/src/main/resources/templates/SpalartAllmaras.txt)
+SA_DEFAULTS = {"sigma": 0.6667, "cb1": 0.1355, "cb2": 0.622, "ka": 0.41, "cw2": 0.3, "cw3": 2.0, "cv1": 7.1,
+               "Prt": 0.9}
 
 
 @dataclass
@@ -33,6 +38,8 @@ class ModelConstants:
     attack: float | None = None
     a: tuple = (1.0, 0.5)
     k: float = 0.0
+    vt_in: float = 0.0        # SpalartAllmaras inlet turbulence
+    sa: dict | None = None    # SpalartAllmaras constants (sigma, cb1, cb2, ka, cw2, cw3, cv1, Prt)
 
 
 class UnsupportedModel(ValueError):
@@ -46,16 +53,18 @@ def init(props: dict) -> ModelConstants:
         a = tuple(float(v) for v in props.get("a", (1.0, 0.5)))
         k = float(props.get("k", 0.0))
         return ModelConstants(model, 2, 1, 2, k > 0, a=a, k=k)
-    if model != "NavierStokes":
+    if model not in ("NavierStokes", "SpalartAllmaras"):
         raise UnsupportedModel(model)
     if int(props.get("dimension", 2)) != 2:
-        raise UnsupportedModel("NavierStokes in 3D")
+        raise UnsupportedModel(f"{model} in 3D")
+    sa = model == "SpalartAllmaras"
     kappa, cv = float(props["kappa"]), float(props["cv"])
     # subsonic stagnation inlet: pIn0, rhoIn0, pOut (Aerodynamics.java:169-182)
     p_ref, rho_ref = float(props["pIn0"]), float(props["rhoIn0"])
     p_out = float(props["pOut"]) / p_ref
     velocity_ref = math.sqrt(p_ref / rho_ref)
-    diffusive = bool(props.get("isFlowViscous", False))
+    # SpalartAllmaras.init passes isDiffusive = true (SpalartAllmaras.java:38)
+    diffusive = True if sa else bool(props.get("isFlowViscous", False))
     re = pr = -1.0
     if diffusive:
         # far-field Reynolds number rescaled to the reference values (Aerodynamics.java:262-281)
@@ -68,12 +77,17 @@ def init(props: dict) -> ModelConstants:
         re = float(props["reynolds"]) * (rho_ref * velocity_ref) / (rho_inf * u_inf)
         pr = float(props["prandtl"])
     attack = float(props["attackAngle"]) if "attackAngle" in props else None
-    return ModelConstants(model, 0, 4, 2, diffusive, kappa=kappa, cv=cv, p_ref=p_ref, rho_ref=rho_ref,
-                          velocity_ref=velocity_ref, p_out=p_out, reynolds=re, prandtl=pr, attack=attack)
+    extra = {}
+    if sa:
+        extra = dict(vt_in=float(props["vtIn"]), sa={k: float(props.get(k, v)) for k, v in SA_DEFAULTS.items()})
+    return ModelConstants(model, MODEL_IDS[model], 5 if sa else 4, 2, diffusive, kappa=kappa, cv=cv, p_ref=p_ref,
+                          rho_ref=rho_ref, velocity_ref=velocity_ref, p_out=p_out, reynolds=re, prandtl=pr,
+                          attack=attack, **extra)
 
 
 def const_init_condition(mc: ModelConstants) -> list[float]:
-    """NavierStokes.constInitCondition, subsonic inlet branch (NavierStokes.java:105-137)."""
+    """NavierStokes / SpalartAllmaras constInitCondition, subsonic inlet branch
+    (NavierStokes.java:105-137, SpalartAllmaras.java:100-137: rho nu~ = rho vtIn appended)."""
     if mc.model_id == 2:
         return [0.0]
     kappa = mc.kappa
@@ -81,5 +95,8 @@ def const_init_condition(mc: ModelConstants) -> list[float]:
     rho_in = math.pow(1 + ((kappa - 1) / 2) * mach_in * mach_in, 1 / (1 - kappa))
     v_in = mach_in * math.sqrt((kappa * mc.p_out) / rho_in)
     alpha = mc.attack or 0.0
-    return [rho_in, rho_in * v_in * math.cos(alpha), rho_in * v_in * math.sin(alpha),
-            mc.p_out / (kappa - 1) + 0.5 * rho_in * v_in * v_in]
+    w = [rho_in, rho_in * v_in * math.cos(alpha), rho_in * v_in * math.sin(alpha),
+         mc.p_out / (kappa - 1) + 0.5 * rho_in * v_in * v_in]
+    if mc.model_id == 1:
+        w.append(rho_in * mc.vt_in)
+    return w

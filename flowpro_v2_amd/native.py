@@ -23,7 +23,7 @@ EXPORTS = ["fp_neighbor_matrix", "fp_rcm_order", "fp_create", "fp_destroy", "fp_
            "fp_element_order", "fp_set_state", "fp_get_state", "fp_next_level", "fp_restore_level", "fp_time_step",
            "fp_assemble", "fp_linear_solve", "fp_update", "fp_export_residual", "fp_export_jacobian",
            "fp_export_solution", "fp_jv", "fp_jtv", "fp_time_kernel", "fp_stream", "fp_linear_solve_jfnk",
-           "fp_functional", "fp_adjoint_solve"]
+           "fp_functional", "fp_adjoint_solve", "fp_wall_distance"]
 
 
 class FpCase(C.Structure):
@@ -43,6 +43,9 @@ class FpCase(C.Structure):
         ("ordering", C.c_int), ("preconditioner", C.c_int), ("precond_bits", C.c_int),
         ("gmres_restart", C.c_int), ("gmres_cycles", C.c_int), ("gmres_tol", C.c_double),
         ("positivity_limiter", C.c_int),
+        ("sa_vt_in", C.c_double), ("sa_sigma", C.c_double), ("sa_cb1", C.c_double), ("sa_cb2", C.c_double),
+        ("sa_ka", C.c_double), ("sa_cw2", C.c_double), ("sa_cw3", C.c_double), ("sa_cv1", C.c_double),
+        ("sa_prt", C.c_double),
     ]
 
 
@@ -84,6 +87,7 @@ def lib() -> C.CDLL:
         "fp_linear_solve_jfnk": (C.c_int, [vp, ip, dp]),
         "fp_functional": (C.c_int, [vp, C.c_int, dp, dp]),
         "fp_adjoint_solve": (C.c_int, [vp, C.c_double, dp, dp, ip, dp]),
+        "fp_wall_distance": (C.c_int, [C.c_int, dp, C.c_int, ip, ip, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -113,6 +117,18 @@ def neighbor_matrix(TP: np.ndarray, bnd: np.ndarray) -> np.ndarray:
     if rc:
         raise FatalError(f"fp_neighbor_matrix: {rc}")
     return TT
+
+
+def wall_distance(PXY: np.ndarray, TP: np.ndarray, TT: np.ndarray) -> np.ndarray:
+    """Vertex distance to the wall faces along mesh edges (fp_wall_distance)."""
+    PXY = np.ascontiguousarray(PXY, dtype=np.float64)
+    TP = np.ascontiguousarray(TP, dtype=np.int32)
+    TT = np.ascontiguousarray(TT, dtype=np.int32)
+    out = np.zeros(PXY.shape[0])
+    rc = lib().fp_wall_distance(PXY.shape[0], dptr(PXY), TP.shape[0], iptr(TP), iptr(TT), dptr(out))
+    if rc:
+        raise FatalError(f"fp_wall_distance: {rc}")
+    return out
 
 
 def rcm_order(TT: np.ndarray) -> np.ndarray:

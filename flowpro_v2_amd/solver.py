@@ -78,6 +78,12 @@ class DeviceCase:
         cs.gmres_cycles = int(props.get("gmresCycles", 5))
         cs.gmres_tol = float(props.get("iterativeSolverTol", 1e-2))
         cs.positivity_limiter = int(bool(props.get("positivityLimiter", True)))
+        if mc.sa is not None:
+            if wd is None:
+                raise UnsupportedCase("SpalartAllmaras needs the vertex wall distance (Case.wall_distance)")
+            cs.sa_vt_in = mc.vt_in
+            cs.sa_sigma, cs.sa_cb1, cs.sa_cb2, cs.sa_ka = (mc.sa[k] for k in ("sigma", "cb1", "cb2", "ka"))
+            cs.sa_cw2, cs.sa_cw3, cs.sa_cv1, cs.sa_prt = (mc.sa[k] for k in ("cw2", "cw3", "cv1", "Prt"))
         h = C.c_void_p()
         rc = L.fp_create(C.byref(cs), C.byref(h))
         if rc == native.FP_UNSUPPORTED:

@@ -54,7 +54,8 @@ typedef struct fp_case {
     int n_face;                   /* Gauss-Legendre rule of order k on [-1, 1] */
     const double* face_xi;        /* [n_face] */
     const double* face_w;         /* [n_face] */
-    int model;                    /* 0 NavierStokes (Euler when viscous == 0), 2 TestScalarEquation */
+    int model;                    /* 0 NavierStokes (Euler when viscous == 0), 1 SpalartAllmaras,
+                                     2 TestScalarEquation */
     int viscous;
     double kappa;                 /* Aerodynamics.init derived constants */
     double p_out;                 /* outlet pressure / pRef */
@@ -72,6 +73,9 @@ typedef struct fp_case {
     int gmres_cycles;             /* 5 */
     double gmres_tol;             /* iterativeSolverTol */
     int positivity_limiter;       /* Parameters.positivityLimiter */
+    /* SpalartAllmaras (SpalartAllmaras.java:36-61): inlet turbulence vtIn and the model constants
+     * (cw1 is derived); wall_distance above is required for this model */
+    double sa_vt_in, sa_sigma, sa_cb1, sa_cb2, sa_ka, sa_cw2, sa_cw3, sa_cv1, sa_prt;
 } fp_case;
 
 /* --- integer setup (bit-exact with the reference) ------------------------------ */
@@ -79,6 +83,9 @@ typedef struct fp_case {
 int fp_neighbor_matrix(int n_elems, const int* tp, int n_bnd, const int* bnd, int* tt);
 /* ElementOrdering.reverseCuthillMcKee (ElementOrdering.java:52-142): order[n_elems] */
 int fp_rcm_order(int n_elems, const int* tt, int* order);
+/* FlowProMain.generateWallDistanceFunction / generateDistanceFunction (FlowProMain.java:931-1014):
+ * distance of every vertex from the wall faces (tt == -1) along mesh edges; dist[n_points] */
+int fp_wall_distance(int n_points, const double* pxy, int n_elems, const int* tp, const int* tt, double* dist);
 
 /* --- context --------------------------------------------------------------------- */
 /* Mesh constructor + Mesh.init + LinearSolver.factory (Mesh.java:69-224, LinearSolver.java:35-89) */

@@ -20,7 +20,8 @@ _I = ["model", "viscous", "order", "time_order", "preconditioner", "ordering", "
       "vary_cfl", "fixed_dt", "positivity_limiter", "jacobian_mode", "gmres_m", "gmres_cycles",
       "has_attack_angle", "threads"]
 _D = ["kappa", "cv", "pIn0", "rhoIn0", "pOut", "reynolds", "prandtl", "attack_angle", "cfl", "max_cfl", "dt",
-      "newton_tol", "gmres_tol", "penalty", "beta0", "h", "residuum_stop", "scalar_a0", "scalar_a1", "scalar_k"]
+      "newton_tol", "gmres_tol", "penalty", "beta0", "h", "residuum_stop", "scalar_a0", "scalar_a1", "scalar_k",
+      "sa_vtIn", "sa_sigma", "sa_cb1", "sa_cb2", "sa_ka", "sa_cw2", "sa_cw3", "sa_cv1", "sa_prt"]
 
 
 class Props(C.Structure):
@@ -61,6 +62,8 @@ def lib():
             "or_jvp": (None, [vp, C.c_double, dp, dp]), "or_vjp": (None, [vp, C.c_double, dp, dp]),
             "or_functional": (C.c_double, [vp, C.c_int]), "or_functional_grad_fd": (None, [vp, C.c_int, dp]),
             "or_elem_geometry": (None, [vp, C.c_int, dp, dp, dp, dp, dp, dp]),
+            "or_wall_distance": (None, [C.c_int, dp, C.c_int, ip, ip, dp]),
+            "or_model_pointwise": (None, [vp, dp, dp, C.c_double, dp, dp, dp]),
             "or_nqf": (C.c_int, [vp]), "or_l2_error_scalar": (C.c_double, [vp]), "or_nqv": (C.c_int, [vp]),
         }
         for name, (res, args) in sig.items():
@@ -72,7 +75,11 @@ def lib():
 
 
 _PREC = {"none": 0, "blockjacobi": 1, "blockilu0": 2}
-_MODEL = {"NavierStokes": 0, "TestScalarEquation": 2}
+_MODEL = {"NavierStokes": 0, "SpalartAllmaras": 1, "TestScalarEquation": 2}
+# SpalartAllmaras constants of the reference's template (Aerodynamics/src/main/resources/templates/
+# SpalartAllmaras.txt); parameters.txt may override each
+SA_DEFAULTS = {"sigma": 0.6667, "cb1": 0.1355, "cb2": 0.622, "ka": 0.41, "cw2": 0.3, "cw3": 2.0, "cv1": 7.1,
+               "Prt": 0.9}
 
 
 def props_from(p: dict, threads: int = 1) -> Props:
@@ -111,6 +118,10 @@ def props_from(p: dict, threads: int = 1) -> Props:
     P.residuum_stop = float(p.get("residuum", -1.0))
     a = p.get("a", (1.0, 0.5))
     P.scalar_a0, P.scalar_a1, P.scalar_k = float(a[0]), float(a[1]), float(p.get("k", 0.0))
+    P.sa_vtIn = float(p.get("vtIn", 0.0))
+    for key, field in (("sigma", "sa_sigma"), ("cb1", "sa_cb1"), ("cb2", "sa_cb2"), ("ka", "sa_ka"), ("cw2", "sa_cw2"),
+                       ("cw3", "sa_cw3"), ("cv1", "sa_cv1"), ("Prt", "sa_prt")):
+        setattr(P, field, float(p.get(key, SA_DEFAULTS[key])))
     return P
 
 
@@ -120,6 +131,17 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def wall_distance(PXY, TP, TT) -> np.ndarray:
+    """FlowProMain.generateDistanceFunction restated (vertex distance to the wall faces TT == -1
+    along mesh edges)."""
+    PXY = np.ascontiguousarray(PXY, dtype=np.float64)
+    TP = np.ascontiguousarray(TP, dtype=np.int32)
+    TT = np.ascontiguousarray(TT, dtype=np.int32)
+    out = np.zeros(PXY.shape[0])
+    lib().or_wall_distance(PXY.shape[0], _dp(PXY), TP.shape[0], _ip(TP), _ip(TT), _dp(out))
+    return out
 
 
 class Oracle:
@@ -271,6 +293,15 @@ class Oracle:
         M = np.zeros((self.nb, self.nb))
         lib().or_elem_geometry(self.h, i, C.byref(area), C.byref(size), _dp(Xs), _dp(nrm), _dp(jf), _dp(M))
         return dict(area=area.value, elemSize=size.value, Xs=Xs, normals=nrm, jacF=jf, M=M)
+
+    def model_pointwise(self, W, dW, D):
+        """Source term and diffusive fluxes (x, y) of the model at one state; dW[d*nEqs + m]."""
+        n = self.neq
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        dW = np.ascontiguousarray(dW, dtype=np.float64)
+        src, fx, fy = np.zeros(n), np.zeros(n), np.zeros(n)
+        lib().or_model_pointwise(self.h, _dp(W), _dp(dW), float(D), _dp(src), _dp(fx), _dp(fy))
+        return src, fx, fy
 
     def l2_error_scalar(self):
         return lib().or_l2_error_scalar(self.h)

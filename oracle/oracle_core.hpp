@@ -68,7 +68,7 @@ inline Dual dmax(Dual a, Dual b) { return (a.v >= b.v) ? a : b; }
 // AERO/Aerodynamics.java init); the model's derived constants are computed by
 // Model::init below.
 struct Props {
-    int model;            // 0 NavierStokes (Euler when viscous == 0), 2 TestScalarEquation
+    int model;            // 0 NavierStokes (Euler when viscous == 0), 1 SpalartAllmaras, 2 TestScalarEquation
     int viscous;          // isFlowViscous
     int order;            // element order k = p + 1 (global "order")
     int time_order;       // BDF 1 or 2
@@ -94,10 +94,12 @@ struct Props {
     double h;
     double residuum_stop;
     double scalar_a0, scalar_a1, scalar_k;   // TestScalarEquation a, k
+    // SpalartAllmaras (AERO/SpalartAllmaras.java:36-61): vtIn and the model constants
+    double sa_vtIn, sa_sigma, sa_cb1, sa_cb2, sa_ka, sa_cw2, sa_cw3, sa_cv1, sa_prt;
 };
 
 struct ModelConst {
-    int kind;        // 0 NS/Euler, 2 scalar
+    int kind;        // 0 NS/Euler, 1 Spalart-Allmaras, 2 scalar
     int dim, nEqs;
     bool diffusive;
     double kapa, cv, cp;
@@ -108,11 +110,13 @@ struct ModelConst {
     double attack;
     // scalar model
     double a[2], k, lambda;
+    // Spalart-Allmaras constants, cw1 = cb1/ka^2 + (1 + cb2)/sigma (AERO/SpalartAllmaras.java:49-55)
+    double vtIn, sigma, cb1, cb2, ka, cw1, cw2, cw3, cv1, Prt;
 };
 
 // C/Aerodynamics init restated: AERO/Aerodynamics.java:119-312, AERO/NavierStokes.java:18-21
 ModelConst model_init(const Props& P);
-// AERO/NavierStokes.java:101-138 constInitCondition
+// AERO/NavierStokes.java:101-138, AERO/SpalartAllmaras.java:96-137 constInitCondition
 std::vector<double> const_init_condition(const ModelConst& M);
 
 // ---------------------------------------------------------------- quadrature
@@ -202,6 +206,10 @@ std::vector<int> neighbor_matrix(int nElem, const int* TP, int nBnd, const int* 
 // ElementOrdering.reverseCuthillMcKee (C/ElementOrdering.java:52-142)
 std::vector<int> rcm_order(int nElem, const int* TT);
 
+// FlowProMain.generateDistanceFunction (C/FlowProMain.java:951-1014) with the wall faces
+// (Aerodynamics.isIPFace: TT == -1) as sources: distance of every vertex along mesh edges
+std::vector<double> wall_distance(int nPts, const double* PXY, int nElem, const int* TP, const int* TT);
+
 Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const int* TP,
                  int nBnd, const int* bnd, const double* wallDist, const std::string& rulesDir);
 
@@ -219,6 +227,8 @@ void jvp(Case& c, double a1, const double* v, double* out);
 void vjp(Case& c, double a1, const double* v, double* out);
 // Functionals (FOI/DragCoefficient.java, FOI/LiftCoefficient.java)
 double functional(const Case& c, int which, const double* W /*nullable*/);
+// pointwise source term and diffusive fluxes of the model (tests)
+void model_pointwise(const Case& c, const double* W, const double* dW, double D, double* source, double* fvx, double* fvy);
 void functional_grad_fd(Case& c, int which, double* dIdW);
 
 }  // namespace oracle

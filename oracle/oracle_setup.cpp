@@ -1,6 +1,7 @@
 // Oracle (test infrastructure only): setup restatement -- model constants,
 // quadrature rule files, Lagrange bases, affine maps, integration tables,
 // geometry, mass matrices, neighbour matrix and RCM numbering.
+#include <limits>
 #include "oracle_core.hpp"
 
 #include <algorithm>
@@ -101,9 +102,22 @@ ModelConst model_init(const Props& P) {
         M.diffusive = M.k > 0;
         return M;
     }
-    M.kind = 0;
-    M.nEqs = M.dim + 2;
-    M.diffusive = P.viscous != 0;
+    M.kind = P.model == 1 ? 1 : 0;
+    M.nEqs = M.kind == 1 ? M.dim + 3 : M.dim + 2;
+    // SpalartAllmaras.init calls Aerodynamics.init(props, dim, dim + 3, true): always diffusive
+    M.diffusive = M.kind == 1 ? true : P.viscous != 0;
+    if (M.kind == 1) {   // AERO/SpalartAllmaras.java:36-61
+        M.vtIn = P.sa_vtIn;
+        M.sigma = P.sa_sigma;
+        M.cb1 = P.sa_cb1;
+        M.cb2 = P.sa_cb2;
+        M.ka = P.sa_ka;
+        M.cw1 = M.cb1 / (M.ka * M.ka) + (1 + M.cb2) / M.sigma;
+        M.cw2 = P.sa_cw2;
+        M.cw3 = P.sa_cw3;
+        M.cv1 = P.sa_cv1;
+        M.Prt = P.sa_prt;
+    }
     // kappa + cv given (AERO/Aerodynamics.java:125-128)
     M.kapa = P.kappa;
     M.cv = P.cv;
@@ -146,6 +160,7 @@ std::vector<double> const_init_condition(const ModelConst& M) {
     W[0] = rhoIn;
     for (int d = 0; d < M.dim; ++d) W[d + 1] = rhoIn * vIn[d];
     W[M.dim + 1] = M.pOut / (kapa - 1) + 0.5 * rhoIn * VIn * VIn;
+    if (M.kind == 1) W[M.dim + 2] = rhoIn * M.vtIn;   // AERO/SpalartAllmaras.java:135
     return W;
 }
 
@@ -419,6 +434,44 @@ std::vector<int> rcm_order(int n, const int* TT) {
 
 // ------------------------------------------------------------------ case build
 // C/Mesh.java:69-224 (constructor and init) for straight triangles, one global order.
+std::vector<double> wall_distance(int nPts, const double* PXY, int nElem, const int* TP, const int* TT) {
+    const double INF = std::numeric_limits<double>::infinity();
+    std::vector<double> distance(nPts, INF);
+    // triangleElement.getFaceIndexes(k) = {k, (k + 1) % 3}
+    for (int i = 0; i < nElem; i++)
+        for (int k = 0; k < 3; k++)
+            if (TT[3 * i + k] == -1) {
+                distance[TP[3 * i + k]] = 0;
+                distance[TP[3 * i + (k + 1) % 3]] = 0;
+            }
+    bool changed = true;
+    for (int iter = 0; changed && iter <= nPts; iter++) {
+        changed = false;
+        for (int i = 0; i < nElem; i++)
+            for (int j = 0; j < 3; j++) {
+                const int pj = TP[3 * i + j];
+                for (int k = 0; k < 3; k++) {
+                    const int pk = TP[3 * i + k];
+                    if (j != k && distance[pk] < distance[pj]) {
+                        double L = 0;
+                        for (int d = 0; d < 2; d++) L += (PXY[2 * pj + d] - PXY[2 * pk + d]) * (PXY[2 * pj + d] - PXY[2 * pk + d]);
+                        const double y = distance[pk] + std::sqrt(L);
+                        if (y < distance[pj]) {
+                            distance[pj] = y;
+                            changed = true;
+                        }
+                    }
+                }
+            }
+    }
+    for (int i = 0; i < nPts; i++)
+        if (std::isinf(distance[i])) {
+            std::fill(distance.begin(), distance.end(), 1e30);
+            break;
+        }
+    return distance;
+}
+
 Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const int* TP,
                  int nBnd, const int* bnd, const double* wallDist, const std::string& rulesDir) {
     Case* c = new Case();

@@ -468,7 +468,8 @@ double time_step(Case& c) {
             rho = Wavg[0] < 1e-2 ? 1e-2 * std::exp((Wavg[0] - 1e-2) / 1e-2) : Wavg[0];
             Wavg[0] = rho;
             double p = (c.Mc.kapa - 1) * (Wavg[3] - m2 / (2 * Wavg[0]));
-            if (p < 1e-2) p = 1e-2 * std::exp((p - 1e-2) / 1e-2);
+            const double ptol = c.Mc.kind == 1 ? 1e-1 : 1e-2;   // SpalartAllmaras.pressure uses its own P_TOL
+            if (p < ptol) p = ptol * std::exp((p - ptol) / ptol);
             double a = std::sqrt(c.Mc.kapa * p / Wavg[0]);
             double v = 0;
             for (int d = 0; d < 2; d++) {
@@ -734,4 +735,11 @@ double or_l2_error_scalar(void* h) {
 }
 int or_nqf(void* h) { return ((Case*)h)->face.n; }
 int or_nqv(void* h) { return ((Case*)h)->vol.n; }
+void or_model_pointwise(void* h, const double* W, const double* dW, double D, double* s, double* fx, double* fy) {
+    model_pointwise(*(Case*)h, W, dW, D, s, fx, fy);
+}
+void or_wall_distance(int nPts, const double* PXY, int nElem, const int* TP, const int* TT, double* out) {
+    std::vector<double> d = wall_distance(nPts, PXY, nElem, TP, TT);
+    std::memcpy(out, d.data(), sizeof(double) * nPts);
+}
 }
