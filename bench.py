@@ -30,6 +30,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch of the FP32 ILU level kernel on the 400 x 250 cylinder (200,000 triangles):
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes, scaled by the copy kernel of known size (reads
+# 2.00 x FETCH_SIZE, writes 1.00 x WRITE_SIZE); profiles/ilu_factor_rows_ns2d_p3_cylinder200k_pmc.md
+FACTOR_TRAFFIC_400x250 = 2.00 * 6620.3 * 1024 + 4460.7 * 1024
 
 
 def log(msg):
@@ -102,16 +106,16 @@ def factor_algorithmic_bytes(case, dev):
     return float(blocks) * dev.B * dev.B * s
 
 
-def roofline_factor(case, dev, ms_levels):
-    """The dominant kernel of the step: the level-scheduled block-ILU0 factor kernel.
-    `avg_launch_ms` = HIP-event time of all level launches / number of levels."""
+def roofline_factor(case, dev, ms_levels, traffic=None):
+    """The dominant kernel of the step: the level-scheduled block-ILU0 factor kernel
+    (k_ilu_factor_rows).  `avg_launch_ms` = HIP-event time of all level launches / number of levels."""
     levels = dev.ilu_levels[0]
     total = factor_algorithmic_bytes(case, dev)
     per_launch = total / levels
     avg = ms_levels / levels
     achieved = per_launch / (avg * 1e-3) / 1e9
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_ilu_factor_level",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel="k_ilu_factor_rows",
                 algorithmic_bytes=per_launch, avg_launch_ms=round(avg, 5), launches=levels)
 
 
@@ -138,7 +142,14 @@ def cpu_baseline(threads, n_theta=192, n_r=64, nsteps=3):
     o.run(nsteps)
     el = time.perf_counter() - t0
     rate = o.dofs * nsteps / el
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return dict(value=round(rate, 1), unit="DOF-updates/s", cores=threads, kind="port",
+                host={"nproc": os.cpu_count(), "cpu": model},
                 sample=f"{nsteps} Newton steps (FD Jacobian h=1e-8 as the reference, blockilu0 GMRES) of the same "
                        f"NS p=3 Re=5000 M=0.5 case on a {case.n_elem}-triangle cylinder mesh, {el:.1f} s")
 
@@ -179,7 +190,8 @@ def main():
     kt = devs["kt"]
     main_bits = 32 if 32 in out else 64
     r = out[main_bits]
-    roof = roofline_factor(case, dev, kt["ilu_factor_levels"])
+    default_mesh = (args.n_theta, args.n_r) == (400, 250) and main_bits == 32
+    roof = roofline_factor(case, dev, kt["ilu_factor_levels"], FACTOR_TRAFFIC_400x250 if default_mesh else None)
     others = {"k_jacobian_team": roofline_jacobian(dev, kt["jacobian"]), "k_bsr_spmv": roofline_spmv(dev, kt["spmv"])}
     result = {
         "metric": "Newton-GMRES DOF-updates/s, mixed vs FP64 GPU at equal residual; % HBM peak",
