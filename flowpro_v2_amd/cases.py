@@ -197,6 +197,89 @@ def plate_channel(nx: int = 64, ny: int = 24, x0: float = -0.5, x1: float = 2.0,
     return Case(f"plate_{nx}x{ny}", PXY, TP, bnd, wd)
 
 
+# Kuhn split of a cube into 6 tetrahedra along the diagonal v000 -> v111: one per axis order
+_KUHN_PATHS = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
+
+
+def kuhn_cube(n: int = 4, jitter: float = 0.05, seed: int = 4, length: float = 1.0) -> Case:
+    """Unit cube [0, length]^3, n^3 cubes each Kuhn-split into 6 tetrahedra, interior vertices
+    jittered by a seeded fraction of the spacing.  Every tetrahedron is stored with a negative
+    determinant of (P1-P0, P2-P0, P3-P0): faceTransformation3Dtriangle's normals
+    normalize(Js2 x Js1) of the faces {0,2,1}, {0,1,3}, {1,2,3}, {2,0,3} then point outwards.
+    Boundary rows (type, v0, v1, v2): x = 0 inlet -2, x = length outlet -3, the other faces
+    inviscid walls -4."""
+    h = length / n
+    g = np.arange(n + 1)
+    I, J, K = np.meshgrid(g, g, g, indexing="ij")
+    P = np.stack([I.ravel(), J.ravel(), K.ravel()], 1).astype(np.float64) * h
+    vid = np.arange((n + 1) ** 3).reshape(n + 1, n + 1, n + 1)
+    rng = np.random.default_rng(seed)
+    inner = np.all((P > 1e-12) & (P < length - 1e-12), axis=1)
+    P[inner] += jitter * h * rng.uniform(-1, 1, size=(int(inner.sum()), 3))
+    tets = []
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                for path in _KUHN_PATHS:
+                    c = [i, j, k]
+                    vs = [vid[tuple(c)]]
+                    for ax in path:
+                        c[ax] += 1
+                        vs.append(vid[tuple(c)])
+                    tets.append(vs)
+    TP = np.array(tets, dtype=np.int32)
+    a, b, c, d = P[TP[:, 0]], P[TP[:, 1]], P[TP[:, 2]], P[TP[:, 3]]
+    det = np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a)
+    pos = det > 0
+    TP[pos, 1], TP[pos, 2] = TP[pos, 2].copy(), TP[pos, 1].copy()
+    rows = []
+    faces = ((0, 2, 1), (0, 1, 3), (1, 2, 3), (2, 0, 3))
+    Pv = P
+    seen = {}
+    for e in range(TP.shape[0]):
+        for f in faces:
+            key = tuple(sorted(int(TP[e, q]) for q in f))
+            seen[key] = seen.get(key, 0) + 1
+    for key, cnt in seen.items():
+        if cnt != 1:
+            continue
+        x = Pv[list(key)]
+        if np.all(np.abs(x[:, 0]) < 1e-12):
+            t = INLET
+        elif np.all(np.abs(x[:, 0] - length) < 1e-12):
+            t = OUTLET
+        else:
+            t = INVISCID_WALL
+        rows.append((t,) + key)
+    return Case(f"kuhn_{n}", P, TP, np.array(rows, dtype=np.int32))
+
+
+def euler3d_props(mach: float = 0.8, order: int = 2, **kw) -> dict:
+    p = euler_props(mach=mach, order=order, dimension=3)
+    p.update(kw)
+    return p
+
+
+def density_pulse_state(case: Case, nb: int, w0, amp: float = 0.05, width: float = 0.1,
+                        center=(0.5, 0.5, 0.5)) -> np.ndarray:
+    """Freestream w0 with a Gaussian density pulse (velocity and pressure kept): nodal values of
+    the P1 Lagrange basis at the vertices (nb = 4) or the element mean (nb = 1)."""
+    X = case.PXY[case.TP]                                     # (nElem, 4, 3)
+    if nb == 1:
+        X = X.mean(1, keepdims=True)
+    r2 = ((X - np.asarray(center)) ** 2).sum(-1)
+    rho0 = w0[0]
+    rho = rho0 * (1 + amp * np.exp(-r2 / (width * width)))
+    u = np.asarray(w0[1:4]) / rho0
+    p_e = w0[4] - 0.5 * rho0 * (u ** 2).sum()                  # internal energy density (pressure / (kappa - 1))
+    W = np.zeros((case.n_elem, 5, nb))
+    W[:, 0] = rho
+    for d in range(3):
+        W[:, 1 + d] = rho * u[d]
+    W[:, 4] = p_e + 0.5 * rho * (u ** 2).sum()
+    return W.reshape(case.n_elem, 5 * nb)
+
+
 def min_signed_area(case: Case) -> float:
     P, T = case.PXY, case.TP
     a, b, c = P[T[:, 0]], P[T[:, 1]], P[T[:, 2]]

@@ -25,7 +25,7 @@ _D = ["kappa", "cv", "pIn0", "rhoIn0", "pOut", "reynolds", "prandtl", "attack_an
 
 
 class Props(C.Structure):
-    _fields_ = [(n, C.c_int) for n in _I] + [(n, C.c_double) for n in _D]
+    _fields_ = [(n, C.c_int) for n in _I] + [(n, C.c_double) for n in _D] + [("dimension", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -64,7 +64,16 @@ def lib():
             "or_elem_geometry": (None, [vp, C.c_int, dp, dp, dp, dp, dp, dp]),
             "or_wall_distance": (None, [C.c_int, dp, C.c_int, ip, ip, dp]),
             "or_model_pointwise": (None, [vp, dp, dp, C.c_double, dp, dp, dp]),
-            "or_nqf": (C.c_int, [vp]), "or_l2_error_scalar": (C.c_double, [vp]), "or_nqv": (C.c_int, [vp]),
+            "or_nqf": (C.c_int, [vp]),
+            "ot_create": (vp, [C.POINTER(Props), C.c_int, dp, C.c_int, ip, C.c_int, ip, C.c_char_p]),
+            "ot_destroy": (None, [vp]), "ot_nbasis": (C.c_int, [vp]), "ot_neqs": (C.c_int, [vp]),
+            "ot_tt": (None, [vp, ip]), "ot_order": (None, [vp, ip]),
+            "ot_tet_neighbor_matrix": (None, [C.c_int, ip, C.c_int, ip, ip]),
+            "ot_rcm_order": (None, [C.c_int, ip, C.c_int, ip]),
+            "ot_set_w": (None, [vp, dp]), "ot_get_w": (None, [vp, dp]), "ot_residual": (None, [vp, dp]),
+            "ot_jacobian_dense": (None, [vp, C.c_double, dp]), "ot_mass": (None, [vp, C.c_int, dp]),
+            "ot_size_over_lambda": (C.c_double, [vp]),
+            "ot_geometry": (None, [vp, C.c_int, dp, dp, dp, dp, dp]), "or_l2_error_scalar": (C.c_double, [vp]), "or_nqv": (C.c_int, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -118,6 +127,7 @@ def props_from(p: dict, threads: int = 1) -> Props:
     P.residuum_stop = float(p.get("residuum", -1.0))
     a = p.get("a", (1.0, 0.5))
     P.scalar_a0, P.scalar_a1, P.scalar_k = float(a[0]), float(a[1]), float(p.get("k", 0.0))
+    P.dimension = int(p.get("dimension", 2))
     P.sa_vtIn = float(p.get("vtIn", 0.0))
     for key, field in (("sigma", "sa_sigma"), ("cb1", "sa_cb1"), ("cb2", "sa_cb2"), ("ka", "sa_ka"), ("cw2", "sa_cw2"),
                        ("cw3", "sa_cw3"), ("cv1", "sa_cv1"), ("Prt", "sa_prt")):
@@ -332,3 +342,90 @@ class Oracle:
     def from_dof(self, x):
         g = self.gindex()
         return np.stack([x[g[e]:g[e] + self.B] for e in range(self.n_elem)])
+
+
+def tet_neighbor_matrix(TP, bnd) -> np.ndarray:
+    """createNeighborMatrix for tetrahedra (boundary rows: type, v0, v1, v2)."""
+    TP = np.ascontiguousarray(TP, dtype=np.int32)
+    bnd = np.ascontiguousarray(bnd, dtype=np.int32)
+    TT = np.zeros_like(TP)
+    lib().ot_tet_neighbor_matrix(TP.shape[0], _ip(TP), bnd.shape[0], _ip(bnd), _ip(TT))
+    return TT
+
+
+def rcm_order(TT) -> np.ndarray:
+    """ElementOrdering.reverseCuthillMcKee on the face graph of any element type."""
+    TT = np.ascontiguousarray(TT, dtype=np.int32)
+    out = np.zeros(TT.shape[0], np.int32)
+    lib().ot_rcm_order(TT.shape[0], _ip(TT), TT.shape[1], _ip(out))
+    return out
+
+
+class OracleTet:
+    """DGpure on tetrahedra (oracle_tet.cpp): residual, exact dense Jacobian, geometry."""
+
+    def __init__(self, case, props: dict):
+        L = lib()
+        self.case = case
+        self.P = props_from(dict(props, dimension=3))
+        PXY = np.ascontiguousarray(case.PXY, dtype=np.float64)
+        TP = np.ascontiguousarray(case.TP, dtype=np.int32)
+        bnd = np.ascontiguousarray(case.bnd, dtype=np.int32)
+        self.h = L.ot_create(C.byref(self.P), PXY.shape[0], _dp(PXY), TP.shape[0], _ip(TP), bnd.shape[0], _ip(bnd),
+                             RULES.encode())
+        if not self.h:
+            raise RuntimeError("tetrahedron oracle construction failed")
+        self.n_elem = TP.shape[0]
+        self.nb = L.ot_nbasis(self.h)
+        self.neq = L.ot_neqs(self.h)
+        self.B = self.nb * self.neq
+        self.dofs = self.B * self.n_elem
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ot_destroy(self.h)
+            self.h = None
+
+    def set_w(self, W):
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(self.n_elem, self.B)
+        lib().ot_set_w(self.h, _dp(W))
+
+    def get_w(self):
+        W = np.zeros((self.n_elem, self.B))
+        lib().ot_get_w(self.h, _dp(W))
+        return W
+
+    def residual(self):
+        R = np.zeros((self.n_elem, self.B))
+        lib().ot_residual(self.h, _dp(R))
+        return R
+
+    def jacobian(self, a1: float) -> np.ndarray:
+        """Dense a1 M - dR/dW, rows and columns element-major (element, m*Nb + j)."""
+        A = np.zeros((self.dofs, self.dofs))
+        lib().ot_jacobian_dense(self.h, float(a1), _dp(A))
+        return A
+
+    def mass(self, i):
+        M = np.zeros((self.nb, self.nb))
+        lib().ot_mass(self.h, int(i), _dp(M))
+        return M
+
+    def TT(self):
+        out = np.zeros((self.n_elem, 4), np.int32)
+        lib().ot_tt(self.h, _ip(out))
+        return out
+
+    def order(self):
+        out = np.zeros(self.n_elem, np.int32)
+        lib().ot_order(self.h, _ip(out))
+        return out
+
+    def size_over_lambda(self) -> float:
+        return lib().ot_size_over_lambda(self.h)
+
+    def geometry(self, i):
+        vol, es = C.c_double(), C.c_double()
+        Xs, nrm, S = np.zeros(3), np.zeros((4, 3)), np.zeros(4)
+        lib().ot_geometry(self.h, int(i), C.byref(vol), C.byref(es), _dp(Xs), _dp(nrm), _dp(S))
+        return dict(vol=vol.value, elem_size=es.value, Xs=Xs, normals=nrm, S=S)

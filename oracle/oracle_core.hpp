@@ -96,6 +96,7 @@ struct Props {
     double scalar_a0, scalar_a1, scalar_k;   // TestScalarEquation a, k
     // SpalartAllmaras (AERO/SpalartAllmaras.java:36-61): vtIn and the model constants
     double sa_vtIn, sa_sigma, sa_cb1, sa_cb2, sa_ka, sa_cw2, sa_cw3, sa_cv1, sa_prt;
+    int dimension;        // 2 (triangles) or 3 (tetrahedra, oracle_tet.cpp)
 };
 
 struct ModelConst {
@@ -204,7 +205,9 @@ struct Case {
 // FlowProMain.createNeighborMatrix (C/FlowProMain.java:772-873)
 std::vector<int> neighbor_matrix(int nElem, const int* TP, int nBnd, const int* bnd);
 // ElementOrdering.reverseCuthillMcKee (C/ElementOrdering.java:52-142)
-std::vector<int> rcm_order(int nElem, const int* TT);
+std::vector<int> rcm_order(int nElem, const int* TT, int nF = 3);
+// Mat.invert (API/Mat.java:502-642)
+std::vector<std::vector<double>> mat_invert(const std::vector<std::vector<double>>& A);
 
 // FlowProMain.generateDistanceFunction (C/FlowProMain.java:951-1014) with the wall faces
 // (Aerodynamics.isIPFace: TT == -1) as sources: distance of every vertex along mesh edges

@@ -1,0 +1,365 @@
+// Oracle (test infrastructure only): the model functions of the reference's equation
+// classes (NavierStokes / Euler in 2D and 3D, SpalartAllmaras, TestScalarEquation), shared by
+// the triangle and the tetrahedron restatements.  Loops run over M.dim and M.nEqs.
+#pragma once
+#include <cmath>
+
+#include "oracle_core.hpp"
+
+namespace oracle {
+
+static const double RHO_TOL = 1e-2;   // AERO/Aerodynamics.java:22
+static const double P_TOL = 1e-2;     // AERO/NavierStokes.java:15
+static const double P_TOL_SA = 1e-1;  // AERO/SpalartAllmaras.java:15
+enum { WALL = -1, INLET = -2, OUTLET = -3, INVISCID_WALL = -4 };
+
+struct QPData {            // ElementData fields used on the path (API/ElementData.java)
+    double currentX[3];
+    double meshVelocity[3];
+    double wallDistance;
+};
+
+// ------------------------------------------------------------- NavierStokes
+// AERO/Aerodynamics.java:359-365
+template <class T> static T limiteRho(T rho) {
+    if (val(rho) < RHO_TOL) return RHO_TOL * dexp((rho - RHO_TOL) / RHO_TOL);
+    return rho;
+}
+// AERO/NavierStokes.java:23-37, AERO/SpalartAllmaras.java:63-76 (its own P_TOL)
+template <class T> static T pressure(const ModelConst& M, T* W) {
+    const double ptol = M.kind == 1 ? P_TOL_SA : P_TOL;
+    W[0] = limiteRho(W[0]);
+    T momentum2 = 0.0;
+    for (int d = 0; d < M.dim; ++d) momentum2 += W[d + 1] * W[d + 1];
+    T p = (M.kapa - 1) * (W[M.dim + 1] - momentum2 / (2 * W[0]));
+    if (val(p) < ptol) p = ptol * dexp((p - ptol) / ptol);
+    return p;
+}
+// AERO/NavierStokes.java:194-214
+template <class T> static void convectiveFlux(const ModelConst& M, T* W, const double* n, T* f) {
+    if (M.kind == 2) {   // T/user/equation/TestScalarEquation.java:74-81
+        double an = 0;
+        for (int d = 0; d < M.dim; d++) an += M.a[d] * n[d];
+        f[0] = an * W[0];
+        return;
+    }
+    W[0] = limiteRho(W[0]);
+    T V = 0.0;
+    for (int d = 0; d < M.dim; ++d) V += W[d + 1] * n[d];
+    V /= W[0];
+    T p = pressure(M, W);
+    f[0] = W[0] * V;
+    for (int d = 0; d < M.dim; ++d) f[d + 1] = W[d + 1] * V + p * n[d];
+    f[M.dim + 1] = (W[M.dim + 1] + p) * V;
+    if (M.kind == 1) f[M.dim + 2] = W[M.dim + 2] * V;   // AERO/SpalartAllmaras.java:196
+}
+// AERO/Aerodynamics.java:323-337
+template <class T> static T maxEigenvalue(const ModelConst& M, T* W, const QPData& q) {
+    if (M.kind == 2) return T(M.lambda);
+    W[0] = limiteRho(W[0]);
+    T p = pressure(M, W);
+    T a = dsqrt(M.kapa * p / W[0]);
+    T v = 0.0;
+    for (int d = 0; d < M.dim; ++d) {
+        T vRel = W[d + 1] / W[0] - q.meshVelocity[d];
+        v += vRel * vRel;
+    }
+    v = dsqrt(v);
+    return v + a;
+}
+// AERO/NavierStokes.java:141-192 (default = Lax-Friedrichs)
+template <class T>
+static void numericalConvectiveFlux(const ModelConst& M, T* WL, T* WR, const double* n, int TT, const QPData& q, T* f) {
+    const int nEqs = M.nEqs, dim = M.dim;
+    if (M.kind == 2) {   // T/user/equation/TestScalarEquation.java:84-88
+        T fL[1], fR[1];
+        convectiveFlux(M, WL, n, fL);
+        convectiveFlux(M, WR, n, fR);
+        f[0] = (fL[0] + fR[0]) / 2 - M.lambda * (WR[0] - WL[0]) / 2;
+        return;
+    }
+    WL[0] = limiteRho(WL[0]);
+    WR[0] = limiteRho(WR[0]);
+    for (int j = 0; j < nEqs; j++) f[j] = 0.0;
+    switch (TT) {
+        case WALL:
+        case INVISCID_WALL: {
+            // NavierStokes: p(WL); SpalartAllmaras: p(WR) (AERO/SpalartAllmaras.java:146)
+            T p = pressure(M, M.kind == 1 ? WR : WL);
+            f[0] = 0.0;
+            double V = .0;
+            for (int d = 0; d < dim; ++d) {
+                f[d + 1] = p * n[d];
+                V += q.meshVelocity[d] * n[d];
+            }
+            f[dim + 1] = p * V;
+            for (int j = 0; j < nEqs; j++) f[j] += V * WR[j];
+            break;
+        }
+        case INLET:
+        case OUTLET:
+            convectiveFlux(M, WR, n, f);
+            break;
+        default: {
+            T fL[5], fR[5];
+            convectiveFlux(M, WL, n, fL);
+            convectiveFlux(M, WR, n, fR);
+            T lamL = maxEigenvalue(M, WL, q);
+            T lamR = maxEigenvalue(M, WR, q);
+            T maxEigenValue = dmax(lamL, lamR);
+            for (int j = 0; j < nEqs; j++) f[j] = (fL[j] + fR[j] - maxEigenValue * (WR[j] - WL[j])) / 2;
+        }
+    }
+}
+// AERO/NavierStokes.java:216-315 (subsonic inlet, no attack angle unless given)
+template <class T> static void boundaryValue(const ModelConst& M, T* WL, const double* n, int TT, const QPData& q, T* WR) {
+    const int nEqs = M.nEqs, dim = M.dim;
+    if (M.kind == 2) {   // exact(currentX)
+        WR[0] = std::sin(2 * q.currentX[0] + q.currentX[1]);
+        return;
+    }
+    WL[0] = limiteRho(WL[0]);
+    for (int j = 0; j < nEqs; j++) WR[j] = 0.0;
+    T p = pressure(M, WL);
+    const double kapa = M.kapa;
+    switch (TT) {
+        case WALL:
+            if (M.diffusive) {
+                const double* u = q.meshVelocity;
+                double absVelocity2 = .0;
+                for (int d = 0; d < dim; ++d) absVelocity2 += u[d] * u[d];
+                WR[0] = WL[0];
+                for (int d = 0; d < dim; ++d) WR[d + 1] = WR[0] * u[d];
+                WR[dim + 1] = p / (kapa - 1) + WR[0] * absVelocity2 / 2;
+                // SpalartAllmaras: WR[dim + 2] = 0 (AERO/SpalartAllmaras.java:216)
+            } else {
+                for (int j = 0; j < nEqs; j++) WR[j] = WL[j];
+                T nu = 0.0;
+                for (int d = 0; d < dim; ++d) nu += WL[d + 1] * n[d];
+                for (int d = 0; d < dim; ++d) WR[d + 1] = WL[d + 1] - n[d] * nu;
+            }
+            break;
+        case INVISCID_WALL: {
+            for (int j = 0; j < nEqs; j++) WR[j] = WL[j];
+            T nu = 0.0;
+            for (int d = 0; d < dim; ++d) nu += WL[d + 1] * n[d];
+            // SpalartAllmaras adds n nu (AERO/SpalartAllmaras.java:229-231), NavierStokes subtracts
+            if (M.kind == 1)
+                for (int d = 0; d < dim; ++d) WR[d + 1] = WL[d + 1] + n[d] * nu;
+            else
+                for (int d = 0; d < dim; ++d) WR[d + 1] = WL[d + 1] - n[d] * nu;
+            break;
+        }
+        case INLET: {
+            const double pIn0 = 1, rhoIn0 = 1;
+            T mach;
+            if (val(p) > pIn0) mach = 0.0;
+            else mach = dsqrt((2 / (kapa - 1)) * (-1 + dpow(pIn0 / p, (kapa - 1) / kapa)));
+            T rhoIn = rhoIn0 * dpow((1 + ((kapa - 1) / 2) * mach * mach), 1 / (1 - kapa));
+            T inletVelocity = mach * dsqrt((kapa * p) / rhoIn);
+            T E = p / (kapa - 1) + 0.5 * rhoIn * inletVelocity * inletVelocity;
+            WR[0] = rhoIn;
+            if (!M.hasAttack) {
+                for (int d = 0; d < dim; ++d) WR[d + 1] = -rhoIn * inletVelocity * n[d];
+            } else {
+                double dir[3] = {std::cos(M.attack), std::sin(M.attack), 0.0};   // 2D: one angle
+                for (int d = 0; d < dim; ++d) WR[d + 1] = rhoIn * inletVelocity * dir[d];
+            }
+            WR[dim + 1] = E;
+            if (M.kind == 1) WR[dim + 2] = rhoIn * M.vtIn;   // AERO/SpalartAllmaras.java:262
+            break;
+        }
+        case OUTLET: {
+            T rhoOut = WL[0];
+            T absVelocity2 = .0;
+            for (int d = 0; d < dim; ++d) absVelocity2 += WL[d + 1] * WL[d + 1];
+            absVelocity2 /= rhoOut * rhoOut;
+            T a = dsqrt(kapa * pressure(M, WL) / WL[0]);
+            T mach = dsqrt(absVelocity2) / a;
+            if (val(mach) < 1) {
+                WR[0] = rhoOut;
+                for (int d = 0; d < dim; ++d) WR[d + 1] = rhoOut * WL[d + 1] / WL[0];
+                WR[dim + 1] = M.pOut / (kapa - 1) + rhoOut * absVelocity2 / 2;
+                if (M.kind == 1) WR[dim + 2] = WL[dim + 2];   // AERO/SpalartAllmaras.java:283
+            } else {
+                for (int j = 0; j < nEqs; j++) WR[j] = WL[j];
+            }
+            break;
+        }
+        default:
+            break;
+    }
+}
+// SpalartAllmaras.max (AERO/SpalartAllmaras.java:453-459): a > b ? a : b
+template <class T> static T saMax(T a, T b) { return val(a) > val(b) ? a : b; }
+// java.lang.Math.min for ordinary values: a <= b ? a : b
+template <class T> static T jmin(T a, T b) { return val(a) <= val(b) ? a : b; }
+
+// AERO/SpalartAllmaras.java:292-351 diffusiveFlux
+template <class T> static void diffusiveFluxSA(const ModelConst& M, T* W, const T* dW, const double* n, T* flux) {
+    const int nEqs = M.nEqs, dim = M.dim;
+    W[0] = limiteRho(W[0]);
+    T rho = W[0];
+    const double lam = -2. / 3;
+    T velocity[3];
+    T velocity2 = .0;
+    for (int d = 0; d < dim; ++d) {
+        velocity[d] = W[d + 1] / rho;
+        velocity2 += velocity[d] * velocity[d];
+    }
+    T velocityJac[9];
+    for (int d = 0; d < dim; ++d)
+        for (int f = 0; f < dim; ++f) velocityJac[dim * d + f] = (dW[f * nEqs + d + 1] - dW[f * nEqs] * velocity[d]) / rho;
+    T p = pressure(M, W);
+    T pOverRhoDer[3];
+    for (int d = 0; d < dim; ++d) {
+        T temp = .0;
+        for (int f = 0; f < dim; ++f) temp += velocity[f] * velocityJac[dim * f + d];
+        T pDer = (M.kapa - 1) * (dW[d * nEqs + dim + 1] - dW[d * nEqs] * velocity2 / 2 - rho * temp);
+        pOverRhoDer[d] = (rho * pDer - p * dW[d * nEqs]) / (rho * rho);
+    }
+    T stress[9];
+    T trace = .0;
+    for (int d = 0; d < dim; ++d) {
+        trace += velocityJac[dim * d + d];
+        for (int f = 0; f < dim; ++f) stress[dim * d + f] = velocityJac[dim * d + f] + velocityJac[dim * f + d];
+    }
+    for (int d = 0; d < dim; ++d) stress[dim * d + d] += lam * trace;
+    T vt = saMax(T(0.0), W[dim + 2] / rho);
+    T vtDer[3];
+    for (int d = 0; d < dim; ++d) vtDer[d] = 1 / rho * (dW[d * nEqs + dim + 2] - dW[d * nEqs] * vt);
+    T xi = rho * vt;
+    T fv1 = xi * xi * xi / (xi * xi * xi + M.cv1 * M.cv1 * M.cv1);
+    T mut = rho * vt * fv1;
+    T constant = M.kapa / (M.kapa - 1) * (1 / M.Pr + mut / M.Prt);
+    for (int j = 0; j < nEqs; j++) flux[j] = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        T tmp = .0;
+        for (int f = 0; f < dim; ++f) {
+            flux[f + 1] += (1 + mut) * stress[dim * d + f] * n[d] / M.Re;
+            tmp += velocity[f] * (1 + mut) * stress[dim * d + f];
+        }
+        flux[dim + 1] += (tmp + constant * pOverRhoDer[d]) * n[d] / M.Re;
+        flux[dim + 2] += 1 / (M.Re * M.sigma) * (1 + rho * vt) * vtDer[d] * n[d];
+    }
+}
+
+// AERO/NavierStokes.java:317-374
+template <class T> static void diffusiveFlux(const ModelConst& M, T* W, const T* dW, const double* n, T* flux) {
+    const int nEqs = M.nEqs, dim = M.dim;
+    if (M.kind == 1) {
+        diffusiveFluxSA(M, W, dW, n, flux);
+        return;
+    }
+    if (M.kind == 2) {   // T/user/equation/TestScalarEquation.java:108-114
+        T fl = 0.0;
+        for (int d = 0; d < dim; d++) fl += M.k * dW[d] * n[d];
+        flux[0] = fl;
+        return;
+    }
+    W[0] = limiteRho(W[0]);
+    T rho = W[0];
+    const double lam = -2. / 3;
+    T velocity[3];
+    T velocity2 = .0;
+    for (int d = 0; d < dim; ++d) {
+        velocity[d] = W[d + 1] / rho;
+        velocity2 += velocity[d] * velocity[d];
+    }
+    T velocityJac[9];
+    for (int d = 0; d < dim; ++d)
+        for (int f = 0; f < dim; ++f) velocityJac[dim * d + f] = (dW[f * nEqs + d + 1] - dW[f * nEqs] * velocity[d]) / rho;
+    T p = pressure(M, W);
+    T pOverRhoDer[3];
+    for (int d = 0; d < dim; ++d) {
+        T temp = .0;
+        for (int f = 0; f < dim; ++f) temp += velocity[f] * velocityJac[dim * f + d];
+        T pDer = (M.kapa - 1) * (dW[d * nEqs + dim + 1] - dW[d * nEqs] * velocity2 / 2 - rho * temp);
+        pOverRhoDer[d] = (rho * pDer - p * dW[d * nEqs]) / (rho * rho);
+    }
+    T stress[9];
+    T trace = .0;
+    for (int d = 0; d < dim; ++d) {
+        trace += velocityJac[dim * d + d];
+        for (int f = 0; f < dim; ++f) stress[dim * d + f] = velocityJac[dim * d + f] + velocityJac[dim * f + d];
+    }
+    for (int d = 0; d < dim; ++d) stress[dim * d + d] += lam * trace;
+    const double constant = M.kapa / (M.kapa - 1) / M.Pr;
+    for (int j = 0; j < nEqs; j++) flux[j] = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        T tmp = .0;
+        for (int f = 0; f < dim; ++f) {
+            flux[f + 1] += stress[dim * d + f] * n[d] / M.Re;
+            tmp += velocity[f] * stress[dim * d + f];
+        }
+        flux[dim + 1] += (tmp + constant * pOverRhoDer[d]) * n[d] / M.Re;
+    }
+}
+// AERO/NavierStokes.java:376-395
+template <class T>
+static void numericalDiffusiveFlux(const ModelConst& M, T* Wc, const T* dWc, const double* n, int TT, const QPData& q, T* flux) {
+    if (M.kind == 2) {
+        diffusiveFlux(M, Wc, dWc, n, flux);
+        return;
+    }
+    Wc[0] = limiteRho(Wc[0]);
+    diffusiveFlux(M, Wc, dWc, n, flux);
+    if (TT < 0) {
+        if (TT == WALL) {
+            T work = .0;
+            for (int d = 0; d < M.dim; ++d) work += q.meshVelocity[d] * flux[d + 1];
+            flux[M.dim + 1] = work;
+        } else {
+            for (int j = 0; j < M.nEqs; j++) flux[j] = 0.0;
+        }
+    }
+}
+static bool isIPFace(const ModelConst& M, int TT) { return M.kind == 2 ? true : TT == WALL; }
+static bool isSourcePresent(const ModelConst& M) { return M.kind == 2 || M.kind == 1; }
+// AERO/SpalartAllmaras.java:367-451 sourceTerm (2D: rotationMagnitude = |U[1] - U[2]|)
+template <class T> static void sourceTermSA(const ModelConst& M, T* W, const T* dW, const QPData& q, T* source) {
+    const int nEqs = M.nEqs, dim = M.dim;
+    W[0] = limiteRho(W[0]);
+    T rho = W[0];
+    T velocity[3];
+    for (int d = 0; d < dim; ++d) velocity[d] = W[d + 1] / rho;
+    T velocityJac[9];
+    for (int d = 0; d < dim; ++d)
+        for (int f = 0; f < dim; ++f) velocityJac[dim * d + f] = (dW[f * nEqs + d + 1] - dW[f * nEqs] * velocity[d]) / rho;
+    // the stress magnitude Smag only enters the commented production limiter: not evaluated
+    T vt = saMax(T(0.0), W[dim + 2] / rho);
+    T vtDerMag = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        T vtDer = 1 / rho * (dW[d * nEqs + dim + 2] - dW[d * nEqs] * vt);
+        vtDerMag += vtDer * vtDer;
+    }
+    if (val(vt) < 0) vt = 0.0;
+    const double D = q.wallDistance;
+    const double ka = M.ka;
+    T xi = rho * vt;
+    const double ft2 = 0;
+    T fv1 = xi * xi * xi / (xi * xi * xi + M.cv1 * M.cv1 * M.cv1);
+    T fv2 = 1 - xi / (1 + xi * fv1);
+    T Om = dabs(velocityJac[1] - velocityJac[2]);
+    T S = Om + 1 / M.Re * vt / (ka * ka * D * D) * fv2;
+    T rt = jmin(vt / (M.Re * S * ka * ka * D * D), T(10.0));
+    T g = rt + M.cw2 * (dpow(rt, 6.0) - rt);
+    T fw = g * dpow((1 + std::pow(M.cw3, 6.0)) / (dpow(g, 6.0) + std::pow(M.cw3, 6.0)), 1.0 / 6);
+    for (int j = 0; j < nEqs; j++) source[j] = 0.0;
+    T d = 1 / M.Re * rho * M.cb2 * vtDerMag + M.cb1 * (1 - ft2) * rho * S * vt -
+          1 / M.Re * (M.cw1 * fw - M.cb1 / (ka * ka) * ft2) / rho * (rho * vt / D) * (rho * vt / D);
+    if (val(d) < -100) d = -100.0;   // limitDestruction
+    source[dim + 2] = d;
+}
+// T/user/equation/TestScalarEquation.java:131-136
+template <class T> static void sourceTerm(const ModelConst& M, T* W, const T* dW, const QPData& q, T* s) {
+    if (M.kind == 1) {
+        sourceTermSA(M, W, dW, q, s);
+        return;
+    }
+    double phase = 2 * q.currentX[0] + q.currentX[1];
+    s[0] = (2 * M.a[0] + M.a[1]) * std::cos(phase) + 5 * M.k * std::sin(phase);
+}
+
+
+}  // namespace oracle

@@ -89,7 +89,7 @@ static double stagRho(const ModelConst& M, double mach) {  // AERO/Aerodynamics.
 ModelConst model_init(const Props& P) {
     ModelConst M;
     std::memset(&M, 0, sizeof(M));
-    M.dim = 2;
+    M.dim = P.dimension == 3 ? 3 : 2;
     if (P.model == 2) {   // T/user/equation/TestScalarEquation.java:28-37
         M.kind = 2;
         M.nEqs = 1;
@@ -155,7 +155,7 @@ std::vector<double> const_init_condition(const ModelConst& M) {
     double rhoIn = std::pow(1 + ((kapa - 1) / 2) * machIn * machIn, 1 / (1 - kapa));
     double VIn = machIn * std::sqrt((kapa * M.pOut) / rhoIn);
     double alpha = M.hasAttack ? M.attack : 0;
-    double vIn[2] = {VIn * std::cos(alpha), VIn * std::sin(alpha)};
+    double vIn[3] = {VIn * std::cos(alpha), VIn * std::sin(alpha), 0.0};
     std::vector<double> W(M.nEqs);
     W[0] = rhoIn;
     for (int d = 0; d < M.dim; ++d) W[d + 1] = rhoIn * vIn[d];
@@ -398,12 +398,12 @@ static int pseudo_peripheral(const std::vector<std::vector<int>>& adj, const std
     return node;
 }
 
-std::vector<int> rcm_order(int n, const int* TT) {
+std::vector<int> rcm_order(int n, const int* TT, int nF) {
     std::vector<std::vector<int>> adj(n);
     std::vector<int> degree(n);
     for (int i = 0; i < n; i++) {
-        for (int k = 0; k < 3; k++) {
-            int t = TT[3 * i + k];
+        for (int k = 0; k < nF; k++) {
+            int t = TT[nF * i + k];
             if (t > -1 && t != i && std::find(adj[i].begin(), adj[i].end(), t) == adj[i].end()) adj[i].push_back(t);
         }
         degree[i] = (int)adj[i].size();

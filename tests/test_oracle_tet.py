@@ -1,0 +1,103 @@
+"""Pins of the tetrahedron restatement (oracle/oracle_tet.cpp), CPU only.
+
+No reference test covers 3D elements; the checks are what the geometry and the scheme fix:
+  volume = |det| / 6, P1 mass matrix = V/20 (1 + delta_ij), elemSize = 4 V / sum S (closed forms;
+  relative 1e-11: the gaussTetra weights carry 12 digits, e.g. 0.166666666667 for 1/6)
+  sum_k S_k n_k = 0 and sum_k S_k (Xe_k . n_k) = 3 V (closed flat faces, outward normals)
+  the centroid quirk: Xs = X(0.25, 0.25, 0) lies on face 0 (the plane zeta = 0)
+  neighbour matrix: symmetric face adjacency, boundary codes from the boundary rows
+  uniform freestream with consistent inlet / outlet / slip walls: R = 0 to rounding
+  exact Jacobian = central differences of the residual (1e-6)
+  the generic RCM equals the triangle RCM of the product on a triangle mesh (bit-exact)
+  order 3 is rejected, as basis3Dtetra's 4-node list cannot build a 10-function basis
+"""
+import numpy as np
+import pytest
+
+from flowpro_v2_amd import cases, native
+import oracle as O
+
+
+@pytest.fixture(scope="module")
+def cube():
+    return cases.kuhn_cube(2, 0.05, 4)
+
+
+def test_neighbor_matrix(cube):
+    TT = O.tet_neighbor_matrix(cube.TP, cube.bnd)
+    n = cube.n_elem
+    for e in range(n):
+        for k in range(4):
+            t = TT[e, k]
+            if t > -1:
+                assert (TT[t] == e).sum() == 1
+    # 6 * 8 tets, 48 boundary triangles: 4 * 48 = 2 * interior + 48
+    assert (TT < 0).sum() == len(cube.bnd)
+    assert set(np.unique(TT[TT < 0])) == {-2, -3, -4}
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_geometry_closed_forms(cube, order):
+    o = O.OracleTet(cube, cases.euler3d_props(order=order))
+    P = cube.PXY
+    for e in range(cube.n_elem):
+        g = o.geometry(e)
+        a, b, c, d = (P[cube.TP[e, q]] for q in range(4))
+        vol = abs(np.dot(np.cross(b - a, c - a), d - a)) / 6
+        assert abs(g["vol"] - vol) <= 1e-11 * vol
+        S, nrm = g["S"], g["normals"]
+        assert np.abs((S[:, None] * nrm).sum(0)).max() <= 1e-14
+        # face centroids: faces {0,2,1}, {0,1,3}, {1,2,3}, {2,0,3}
+        V = np.array([a, b, c, d])
+        cen = np.array([V[list(f)].mean(0) for f in ((0, 2, 1), (0, 1, 3), (1, 2, 3), (2, 0, 3))])
+        assert abs((S * (cen * nrm).sum(1)).sum() - 3 * vol) <= 1e-11 * vol
+        assert abs(g["elem_size"] - 4 * g["vol"] / S.sum()) <= 1e-15
+        # Xs = X(0.25, 0.25, 0) = a + 0.25 (b - a) + 0.25 (c - a): on face 0
+        assert np.abs(g["Xs"] - (a + 0.25 * (b - a) + 0.25 * (c - a))).max() <= 1e-15
+        if order == 2:
+            M = o.mass(e)
+            assert np.abs(M - vol / 20 * (np.ones((4, 4)) + np.eye(4))).max() <= 1e-11 * vol / 10
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_freestream_preserved(cube, order):
+    o = O.OracleTet(cube, cases.euler3d_props(order=order))
+    R = o.residual()
+    assert np.abs(R).max() <= 1e-12
+
+
+def test_order_three_rejected(cube):
+    with pytest.raises(RuntimeError):
+        O.OracleTet(cube, cases.euler3d_props(order=3))
+
+
+def test_exact_jacobian_against_central_differences(cube):
+    o = O.OracleTet(cube, cases.euler3d_props(order=2))
+    W = cases.density_pulse_state(cube, 4, o.get_w()[0][::4], amp=0.1, width=0.3)
+    W = W * (1 + 0.01 * np.random.default_rng(3).standard_normal(W.shape))
+    o.set_w(W)
+    A = o.jacobian(0.0)                    # -dR/dW
+    rng = np.random.default_rng(0)
+    eps = 1e-6
+    worst = 0.0
+    for col in rng.choice(o.dofs, 12, replace=False):
+        Wp, Wm = W.copy().ravel(), W.copy().ravel()
+        Wp[col] += eps
+        Wm[col] -= eps
+        o.set_w(Wp); Rp = o.residual().ravel()
+        o.set_w(Wm); Rm = o.residual().ravel()
+        fd = -(Rp - Rm) / (2 * eps)
+        worst = max(worst, np.abs(A[:, col] - fd).max() / np.abs(A[:, col]).max())
+    assert worst < 1e-6
+
+
+def test_generic_rcm_equals_triangle_rcm():
+    case = cases.cylinder_mesh(16, 6)
+    TT = native.neighbor_matrix(case.TP, case.bnd)
+    assert np.array_equal(O.rcm_order(TT), native.rcm_order(TT))
+
+
+def test_rcm_on_tets_is_a_permutation(cube):
+    TT = O.tet_neighbor_matrix(cube.TP, cube.bnd)
+    order = O.rcm_order(TT)
+    assert np.array_equal(np.sort(order), np.arange(cube.n_elem))
