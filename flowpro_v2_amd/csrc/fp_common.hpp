@@ -47,6 +47,35 @@ struct Mesh {
     const double* wdist;   // [e*3 + v] wall distance of the element's vertices (Spalart-Allmaras)
 };
 
+// Straight tetrahedra (transformation3Dtetra: X = J xi + X0), element data in DOF order:
+//   geo  [e*TGEO + ...]   iT (3x3 row-major, d xi_b / d x_a = iT[b][a]), |det J|, J (3x3 row-major), X0
+//   fgeo [(e*4+k)*TFGEO]  outward unit normal (3), face Jacobian = triangle area, dL
+//   nbr  [e*4+k]          neighbour element or boundary code (< 0)
+//   blkn [e*4+k]          BSR block index of (row nbr, column e), -1 on boundary faces
+// Reference tables: basis and reference gradients at the volume points, basis and reference
+// coordinates at the face points of each local face, weights, reference mass matrix.
+constexpr int TGEO = 22, TFGEO = 5;
+
+struct TetTables {
+    const double* vb;    // [NQV][NB]
+    const double* vd;    // [NQV][NB][3]
+    const double* vw;    // [NQV]
+    const double* fb;    // [4][NQF][NB]
+    const double* fw;    // [NQF]
+    const double* fxr;   // [4][NQF][3]
+    const double* mref;  // [NB][NB]
+    const double* mrefinv;
+};
+
+struct TetMesh {
+    int nElem;
+    const double* geo;
+    const double* fgeo;
+    const int* nbr;
+    const int* blkn;
+    const int* blkd;
+};
+
 // compile-time shape of the element kernels selected at run time
 struct ElemLaunch {
     int model, nb, neq, nqv, nqf, visc;

@@ -36,6 +36,17 @@ hipError_t launch_size_over_lambda(const ElemLaunch&, hipStream_t, const ModelPa
                                    const double*, const double*, const double*, double*);
 hipError_t launch_functional(const ElemLaunch&, hipStream_t, const Tables&, const Mesh&, const double*, double, int, double*, double*);
 hipError_t launch_positivity(const ElemLaunch&, hipStream_t, const Tables&, const Mesh&, double*, const double*, int*);
+// tetrahedra (fp_tet.hip)
+bool tet_supported(const ElemLaunch& L);
+hipError_t launch_residual_tet(const ElemLaunch&, hipStream_t, const ModelParams&, const TetTables&, const TetMesh&, const double*,
+                               const double*, const double*, double, double, double, double*, double*, double*);
+hipError_t launch_jacobian_tet(const ElemLaunch&, hipStream_t, const ModelParams&, const TetTables&, const TetMesh&, const double*,
+                               double, double*, const double*, double*, int);
+hipError_t launch_jvp_tet(const ElemLaunch&, hipStream_t, const ModelParams&, const TetTables&, const TetMesh&, const double*,
+                          const double*, double, double*);
+hipError_t launch_size_over_lambda_tet(const ElemLaunch&, hipStream_t, const ModelParams&, const TetMesh&, const double*,
+                                       const double*, const double*, double*);
+hipError_t launch_positivity_tet(const ElemLaunch&, hipStream_t, const TetTables&, const TetMesh&, double*, const double*, int*);
 hipError_t launch_spmv(hipStream_t, int, int, const int*, const int*, const double*, const double*, const double*, double*, int);
 template <typename TF> hipError_t launch_ilu_copy(hipStream_t, size_t, const double*, TF*);
 template <typename TF>
@@ -68,6 +79,7 @@ struct fp_ctx {
     std::string err;
     // sizes
     int nElem = 0, order = 0, NB = 0, NEQ = 0, B = 0, NQV = 0, NQF = 0;
+    int dim = 2, NF = 3;            // triangles (3 faces) or tetrahedra (4 faces)
     long long dofs = 0;
     ElemLaunch L{};
     ModelParams mp{};
@@ -78,7 +90,6 @@ struct fp_ctx {
     std::vector<int> order_;        // order_[r] = mesh element at position r
     std::vector<int> rank;          // rank[mesh element] = r
     std::vector<int> blk_own_nb;    // block (row r, column of face k neighbour), -1 on boundary
-    std::vector<int> h_nbr;
     int nnzb = 0;
     std::vector<int> lvl_f_ptr, lvl_b_ptr;
     // device
@@ -102,6 +113,8 @@ struct fp_ctx {
     hipGraphExec_t g_factor = nullptr, g_apply = nullptr;
     Tables tb{};
     Mesh ms{};
+    TetTables ttb{};
+    TetMesh tms{};
     bool jac_valid = false;
     double last_a1 = 0;
     double kappa = 1.4;
@@ -117,6 +130,34 @@ struct fp_ctx {
             return FP_EHIP;                                                                                 \
         }                                                                                                   \
     } while (0)
+
+// element kernels of the case's cell type
+static hipError_t run_residual(fp_ctx* c, const double* a) {
+    if (c->dim == 3)
+        return launch_residual_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R, c->RHS, c->enorm);
+    return launch_residual_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R, c->RHS,
+                                c->enorm);
+}
+static hipError_t run_jacobian(fp_ctx* c, double a1) {
+    if (c->dim == 3) return launch_jacobian_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, c->val, nullptr, nullptr, 0);
+    return launch_jacobian_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, c->val);
+}
+static hipError_t run_jvp(fp_ctx* c, const double* v, double a1, double* out) {
+    if (c->dim == 3) return launch_jvp_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, v, a1, out);
+    return launch_jvp(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, v, a1, out);
+}
+static hipError_t run_jtv(fp_ctx* c, const double* v, double a1, double* out) {
+    if (c->dim == 3) return launch_jacobian_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, nullptr, v, out, 1);
+    return launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, nullptr, v, out, 1);
+}
+static hipError_t run_size_over_lambda(fp_ctx* c) {
+    if (c->dim == 3) return launch_size_over_lambda_tet(c->L, c->s, c->mp, c->tms, c->W, c->esize, c->isref, c->tmp);
+    return launch_size_over_lambda(c->L, c->s, c->mp, c->tb, c->ms, c->W, c->esize, c->isref, c->tmp);
+}
+static hipError_t run_positivity(fp_ctx* c, int* count) {
+    if (c->dim == 3) return launch_positivity_tet(c->L, c->s, c->ttb, c->tms, c->W, c->isref, count);
+    return launch_positivity(c->L, c->s, c->tb, c->ms, c->W, c->isref, count);
+}
 
 // ============================================================== integer setup
 extern "C" int fp_neighbor_matrix(int n, const int* TP, int nb, const int* bnd, int* TT) {
@@ -194,15 +235,15 @@ static int peripheral(const std::vector<int>& adjp, const std::vector<int>& adj,
     return node;
 }
 
-extern "C" int fp_rcm_order(int n, const int* TT, int* order) {
+static int rcm_impl(int n, const int* TT, int nF, int* order) {
     if (n < 0 || !TT || !order) return FP_EINVAL;
     std::vector<int> deg(n, 0), adjp(n + 1, 0), adj;
-    adj.reserve((size_t)n * 3);
+    adj.reserve((size_t)n * nF);
     for (int i = 0; i < n; i++) {
         int cnt = 0;
-        int nb[3];
-        for (int k = 0; k < 3; k++) {
-            int t = TT[3 * i + k];
+        int nb[4];
+        for (int k = 0; k < nF; k++) {
+            int t = TT[(size_t)nF * i + k];
             bool dup = false;
             for (int c = 0; c < cnt; c++) dup |= nb[c] == t;
             if (t > -1 && t != i && !dup) nb[cnt++] = t;
@@ -236,6 +277,70 @@ extern "C" int fp_rcm_order(int n, const int* TT, int* order) {
         }
     }
     std::reverse(order, order + n);
+    return FP_OK;
+}
+
+extern "C" int fp_rcm_order(int n, const int* TT, int* order) { return rcm_impl(n, TT, 3, order); }
+extern "C" int fp_rcm_order_tet(int n, const int* TT, int* order) { return rcm_impl(n, TT, 4, order); }
+
+// local faces of tetrahedralElement (vertex triples of each face)
+static const int TET_FACE[4][3] = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {2, 0, 3}};
+
+// FlowProMain.createNeighborMatrix for tetrahedra: faces are matched by their sorted vertex
+// triple; the element scanned first (element-major, then local face) owns the key, the last
+// other element sharing it is its neighbour; unmatched faces take the type of the boundary row
+// (type, v0, v1, v2) with the same vertex set, else -1.
+extern "C" int fp_neighbor_matrix_tet(int n, const int* TP, int nb, const int* bnd, int* TT) {
+    if (n < 0 || !TP || !TT || (nb > 0 && !bnd)) return FP_EINVAL;
+    struct F {
+        int v[3];
+        int idx;   // 4 * element + local face: the scan order
+    };
+    auto sorted = [](int a, int b, int c, int* v) {
+        v[0] = a; v[1] = b; v[2] = c;
+        std::sort(v, v + 3);
+    };
+    std::vector<F> faces((size_t)4 * n);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 4; k++) {
+            F& f = faces[(size_t)4 * i + k];
+            sorted(TP[4 * i + TET_FACE[k][0]], TP[4 * i + TET_FACE[k][1]], TP[4 * i + TET_FACE[k][2]], f.v);
+            f.idx = 4 * i + k;
+        }
+    auto less = [](const F& a, const F& b) {
+        if (a.v[0] != b.v[0]) return a.v[0] < b.v[0];
+        if (a.v[1] != b.v[1]) return a.v[1] < b.v[1];
+        if (a.v[2] != b.v[2]) return a.v[2] < b.v[2];
+        return a.idx < b.idx;
+    };
+    std::sort(faces.begin(), faces.end(), less);
+    auto same = [](const F& a, const F& b) { return a.v[0] == b.v[0] && a.v[1] == b.v[1] && a.v[2] == b.v[2]; };
+    for (size_t a = 0; a < faces.size();) {
+        size_t b = a + 1;
+        while (b < faces.size() && same(faces[a], faces[b])) b++;
+        const int first = faces[a].idx / 4, second = b - a > 1 ? faces[b - 1].idx / 4 : -1000;
+        for (size_t q = a; q < b; q++) {
+            const int i = faces[q].idx / 4;
+            TT[faces[q].idx] = i == first ? second : first;
+        }
+        a = b;
+    }
+    std::vector<F> bf(nb > 0 ? (size_t)nb : 0);
+    for (int r = 0; r < nb; r++) {
+        sorted(bnd[4 * r + 1], bnd[4 * r + 2], bnd[4 * r + 3], bf[r].v);
+        bf[r].idx = r;
+    }
+    std::sort(bf.begin(), bf.end(), less);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 4; k++)
+            if (TT[4 * i + k] == -1000) {
+                F key;
+                sorted(TP[4 * i + TET_FACE[k][0]], TP[4 * i + TET_FACE[k][1]], TP[4 * i + TET_FACE[k][2]], key.v);
+                key.idx = std::numeric_limits<int>::max();
+                // the last boundary row with this vertex set wins, as in a map filled in row order
+                auto it = std::upper_bound(bf.begin(), bf.end(), key, less);
+                TT[4 * i + k] = (it != bf.begin() && same(*(it - 1), key)) ? bnd[4 * (it - 1)->idx] : -1;
+            }
     return FP_OK;
 }
 
@@ -390,66 +495,11 @@ static void free_all(fp_ctx* c) {
     if (c->s) (void)hipStreamDestroy(c->s);
 }
 
-static int create_impl(const fp_case* cs, fp_ctx* c) {
-    const int n = cs->n_elems;
-    c->nElem = n;
-    c->order = cs->order;
+// triangles: neighbour matrix, numbering, basis2Dtriangle tables, transformation2Dtriangle /
+// faceTransformation2Dline geometry (Element.computeGeometry)
+static int setup_tri(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
+    const int n = c->nElem;
     Basis bs = make_basis(cs->order);
-    c->NB = bs.nb;
-    c->NEQ = cs->model == MODEL_SCALAR ? 1 : (cs->model == MODEL_SA ? 5 : 4);
-    if (cs->model == MODEL_SA && !cs->wall_distance) {
-        c->err = "the Spalart-Allmaras model needs the vertex wall distance";
-        return FP_EINVAL;
-    }
-    c->B = c->NB * c->NEQ;
-    c->NQV = cs->n_vol;
-    c->NQF = cs->n_face;
-    c->dofs = (long long)n * c->B;
-    // SpalartAllmaras.init passes isDiffusive = true to Aerodynamics.init (SpalartAllmaras.java:38)
-    const bool diffusive = cs->model == MODEL_SCALAR ? cs->scalar_k > 0 : (cs->model == MODEL_SA || cs->viscous != 0);
-    c->L = ElemLaunch{cs->model, c->NB, c->NEQ, c->NQV, c->NQF, diffusive ? 1 : 0};
-    if (!element_supported(c->L)) {
-        c->err = "no element kernel for this model / order / quadrature";
-        return FP_UNSUPPORTED;
-    }
-    if (cs->preconditioner < 0 || cs->preconditioner > 2 || (cs->precond_bits != 64 && cs->precond_bits != 32)) {
-        c->err = "bad preconditioner settings";
-        return FP_EINVAL;
-    }
-    c->precond = cs->preconditioner;
-    c->bits = cs->precond_bits;
-    c->m = cs->gmres_restart > 0 ? cs->gmres_restart : 30;
-    c->cycles = cs->gmres_cycles > 0 ? cs->gmres_cycles : 5;
-    c->tol = cs->gmres_tol;
-    c->positivity = cs->positivity_limiter;
-    ModelParams& P = c->mp;
-    P.kind = cs->model;
-    P.nEqs = c->NEQ;
-    P.diffusive = diffusive;
-    P.has_attack = cs->has_attack;
-    P.kapa = cs->kappa;
-    c->kappa = cs->kappa;
-    P.pOut = cs->p_out;
-    P.Re = cs->reynolds;
-    P.Pr = cs->prandtl;
-    P.attack = cs->attack;
-    P.a0 = cs->scalar_a0;
-    P.a1 = cs->scalar_a1;
-    P.k = cs->scalar_k;
-    P.lambda = std::sqrt(cs->scalar_a0 * cs->scalar_a0 + cs->scalar_a1 * cs->scalar_a1);
-    P.vtIn = cs->sa_vt_in;
-    P.sigma = cs->sa_sigma;
-    P.cb1 = cs->sa_cb1;
-    P.cb2 = cs->sa_cb2;
-    P.ka = cs->sa_ka;
-    P.cw1 = P.ka != 0 && P.sigma != 0 ? P.cb1 / (P.ka * P.ka) + (1 + P.cb2) / P.sigma : 0.0;   // SpalartAllmaras.java:53
-    P.cw2 = cs->sa_cw2;
-    P.cw3 = cs->sa_cw3;
-    P.cv1 = cs->sa_cv1;
-    P.Prt = cs->sa_prt;
-    c->dg.beta0 = cs->order == 1 ? 0.0 : cs->beta0;                  // DGpure.java:287-289
-    c->dg.cIP = (cs->order > 1 && diffusive) ? cs->penalty : 0.0;   // DGpure.limiter :396-398
-
     // --- topology and numbering
     std::vector<int> TT((size_t)3 * n);
     int rc = fp_neighbor_matrix(n, cs->tp, cs->n_bnd, cs->bnd, TT.data());
@@ -505,7 +555,6 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     };
     size_t o_vb = put(vb), o_vd = put(vd), o_vw = put(vw), o_vx = put(vx), o_fb = put(fb), o_fd = put(fd), o_fw = put(fw),
            o_fx = put(fx), o_m = put(mref), o_mi = put(mrefinv);
-    HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     HIPCHK(c, upload(&c->tab, tab));
     HIPCHK(c, upload(&c->isref, isref));
     c->tb = Tables{c->tab + o_vb, c->tab + o_vd, c->tab + o_vw, c->tab + o_vx, c->tab + o_fb,
@@ -515,7 +564,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     //     Element.computeGeometry)
     std::vector<double> geo((size_t)n * GEO), fgeo((size_t)n * 12), esize(n), Xs(2 * (size_t)n);
     std::vector<double> wdist(cs->wall_distance ? 3 * (size_t)n : 0);
-    std::vector<int> nbr(3 * (size_t)n), nbrf(3 * (size_t)n);
+    nbr.assign(3 * (size_t)n, 0);
+    std::vector<int> nbrf(3 * (size_t)n);
     for (int r = 0; r < n; r++) {
         int e = c->order_[r];
         const double* p0 = cs->pxy + 2 * cs->tp[3 * e];
@@ -570,15 +620,267 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         }
         esize[r] = 3 * g[4] / perim;   // elemSize = nFaces * area / sum S (Element.java:549-554)
     }
-    c->h_nbr = nbr;
+    HIPCHK(c, upload(&c->geo, geo));
+    HIPCHK(c, upload(&c->fgeo, fgeo));
+    HIPCHK(c, upload(&c->esize, esize));
+    HIPCHK(c, upload(&c->nbrf, nbrf));
+    if (!wdist.empty()) HIPCHK(c, upload(&c->wdist, wdist));
+    return FP_OK;
+}
+
+// tetrahedra: neighbour matrix, numbering, basis3Dtetra tables (orders 1 and 2: the constant,
+// then the Lagrange basis on the vertices), transformation3Dtetra / faceTransformation3Dtriangle
+// geometry (Element.computeGeometry).  The face normal is normalize(Js2 x Js1) as in
+// faceTransformation3Dtriangle: outward on elements whose vertex matrix has a negative
+// determinant, the orientation the reference's meshes carry.
+static int setup_tet(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
+    const int n = c->nElem;
+    std::vector<int> TT((size_t)4 * n);
+    int rc = fp_neighbor_matrix_tet(n, cs->tp, cs->n_bnd, cs->bnd, TT.data());
+    if (rc) return rc;
+    c->order_.resize(n);
+    if (cs->ordering == 1) rcm_impl(n, TT.data(), 4, c->order_.data());
+    else
+        for (int i = 0; i < n; i++) c->order_[i] = i;
+    c->rank.resize(n);
+    for (int r = 0; r < n; r++) c->rank[c->order_[r]] = r;
+
+    // --- reference tables
+    const int NB = c->NB, NQV = c->NQV, NQF = c->NQF;
+    static const double XI[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    static const double DPHI[4][3] = {{-1, -1, -1}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    auto phi = [&](const double* x, double* out) {
+        if (NB == 1) out[0] = 1.0;
+        else {
+            out[0] = 1.0 - x[0] - x[1] - x[2];
+            out[1] = x[0];
+            out[2] = x[1];
+            out[3] = x[2];
+        }
+    };
+    std::vector<double> vb((size_t)NQV * NB), vd((size_t)NQV * NB * 3, 0.0), vw(cs->vol_w, cs->vol_w + NQV);
+    for (int q = 0; q < NQV; q++) {
+        phi(cs->vol_xi + 3 * q, &vb[(size_t)q * NB]);
+        if (NB == 4)
+            for (int j = 0; j < NB; j++)
+                for (int d = 0; d < 3; d++) vd[((size_t)q * NB + j) * 3 + d] = DPHI[j][d];
+    }
+    std::vector<double> fb((size_t)4 * NQF * NB), fw(cs->face_w, cs->face_w + NQF), fxr((size_t)4 * NQF * 3);
+    for (int k = 0; k < 4; k++) {
+        const double* R0 = XI[TET_FACE[k][0]];
+        for (int p = 0; p < NQF; p++) {
+            const double u = cs->face_xi[2 * p], v = cs->face_xi[2 * p + 1];
+            double* xr = &fxr[((size_t)k * NQF + p) * 3];
+            for (int d = 0; d < 3; d++)
+                xr[d] = (XI[TET_FACE[k][1]][d] - R0[d]) * u + (XI[TET_FACE[k][2]][d] - R0[d]) * v + R0[d];
+            phi(xr, &fb[((size_t)k * NQF + p) * NB]);
+        }
+    }
+    std::vector<double> mref((size_t)NB * NB, 0.0), mrefinv, isref(NB, 0.0);
+    double wsum = 0;
+    for (int q = 0; q < NQV; q++) wsum += vw[q];
+    for (int i = 0; i < NB; i++) {
+        for (int j = 0; j < NB; j++)
+            for (int q = 0; q < NQV; q++) mref[i * NB + j] += vw[q] * vb[q * NB + i] * vb[q * NB + j];
+        for (int q = 0; q < NQV; q++) isref[i] += vw[q] * vb[q * NB + i];
+        isref[i] /= wsum;
+    }
+    if (!invert(mref, NB, mrefinv)) {
+        c->err = "singular reference mass matrix (volume rule)";
+        return FP_EINVAL;
+    }
+    std::vector<double> tab;
+    auto put = [&](const std::vector<double>& v) {
+        size_t off = tab.size();
+        tab.insert(tab.end(), v.begin(), v.end());
+        return off;
+    };
+    size_t o_vb = put(vb), o_vd = put(vd), o_vw = put(vw), o_fb = put(fb), o_fw = put(fw), o_fx = put(fxr), o_m = put(mref),
+           o_mi = put(mrefinv);
+    HIPCHK(c, upload(&c->tab, tab));
+    HIPCHK(c, upload(&c->isref, isref));
+    c->ttb = TetTables{c->tab + o_vb, c->tab + o_vd, c->tab + o_vw, c->tab + o_fb, c->tab + o_fw, c->tab + o_fx, c->tab + o_m,
+                       c->tab + o_mi};
+
+    // --- geometry in the DOF order
+    std::vector<double> geo((size_t)n * TGEO), fgeo((size_t)n * 4 * TFGEO), esize(n), Xs(3 * (size_t)n);
+    for (int r = 0; r < n; r++) {
+        const int e = c->order_[r];
+        const double* P[4];
+        for (int v = 0; v < 4; v++) P[v] = cs->pxy + 3 * (size_t)cs->tp[4 * e + v];
+        double J[3][3];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) J[a][b] = P[b + 1][a] - P[0][a];
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                     c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        if (!(std::fabs(det) > 0)) {
+            c->err = "degenerate tetrahedron " + std::to_string(e);
+            return FP_EINVAL;
+        }
+        double* g = &geo[(size_t)r * TGEO];
+        // iT = J^-1 by cofactors: iT[b][a] = cof(J)[a][b] / det
+        g[0] = c00 / det;
+        g[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+        g[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+        g[3] = c01 / det;
+        g[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+        g[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+        g[6] = c02 / det;
+        g[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+        g[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+        g[9] = std::fabs(det);
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) g[10 + 3 * a + b] = J[a][b];
+        for (int a = 0; a < 3; a++) g[19 + a] = P[0][a];
+        // Xs = X(0.25, 0.25, 0): transformation3Dtetra.getXs evaluates the literal {0.25, 0.25, 0, 25}
+        for (int a = 0; a < 3; a++) Xs[3 * (size_t)r + a] = J[a][0] * 0.25 + J[a][1] * 0.25 + P[0][a];
+    }
+    double fwsum = 0, vwsum = 0;
+    for (int p = 0; p < NQF; p++) fwsum += fw[p];
+    for (int q = 0; q < NQV; q++) vwsum += vw[q];
+    nbr.assign(4 * (size_t)n, 0);
+    for (int r = 0; r < n; r++) {
+        const int e = c->order_[r];
+        const double* g = &geo[(size_t)r * TGEO];
+        double sumS = 0;
+        for (int k = 0; k < 4; k++) {
+            const double* R0 = XI[TET_FACE[k][0]];
+            double s1[3], s2[3], Js1[3], Js2[3];
+            for (int d = 0; d < 3; d++) {
+                s1[d] = XI[TET_FACE[k][1]][d] - R0[d];
+                s2[d] = XI[TET_FACE[k][2]][d] - R0[d];
+            }
+            for (int a = 0; a < 3; a++) {
+                Js1[a] = g[10 + 3 * a] * s1[0] + g[11 + 3 * a] * s1[1] + g[12 + 3 * a] * s1[2];
+                Js2[a] = g[10 + 3 * a] * s2[0] + g[11 + 3 * a] * s2[1] + g[12 + 3 * a] * s2[2];
+            }
+            const double cr[3] = {Js2[1] * Js1[2] - Js2[2] * Js1[1], Js2[2] * Js1[0] - Js2[0] * Js1[2],
+                                  Js2[0] * Js1[1] - Js2[1] * Js1[0]};
+            const double len = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+            const double nn[3] = {cr[0] / len, cr[1] / len, cr[2] / len};
+            const int t = TT[4 * (size_t)e + k];
+            const double* xs = &Xs[3 * (size_t)r];
+            double dL = 0;
+            if (t > -1) {
+                const int rt = c->rank[t];
+                for (int d = 0; d < 3; d++) dL += (xs[d] - Xs[3 * (size_t)rt + d]) * nn[d];
+                nbr[4 * r + k] = rt;
+            } else {
+                double X[3];
+                for (int a = 0; a < 3; a++) {
+                    double m = 0;
+                    for (int b = 0; b < 3; b++) m += g[10 + 3 * a + b] * (R0[b] + s1[b] / 3 + s2[b] / 3);
+                    X[a] = m + g[19 + a];
+                }
+                for (int d = 0; d < 3; d++) dL += (xs[d] - X[d]) * nn[d];
+                nbr[4 * r + k] = t;
+            }
+            double* f = &fgeo[((size_t)r * 4 + k) * TFGEO];
+            f[0] = nn[0]; f[1] = nn[1]; f[2] = nn[2];
+            f[3] = len / 2;   // |Js1 x Js2| / 2
+            f[4] = std::fabs(dL);
+            sumS += len / 2 * fwsum;
+        }
+        esize[r] = 4 * (g[9] * vwsum) / sumS;   // elemSize = nFaces * volume / sum S (Element.java:549-554)
+    }
+    HIPCHK(c, upload(&c->geo, geo));
+    HIPCHK(c, upload(&c->fgeo, fgeo));
+    HIPCHK(c, upload(&c->esize, esize));
+    return FP_OK;
+}
+
+static int create_impl(const fp_case* cs, fp_ctx* c) {
+    const int n = cs->n_elems;
+    c->nElem = n;
+    c->order = cs->order;
+    c->dim = cs->dimension == 3 ? 3 : 2;
+    c->NF = c->dim + 1;
+    if (c->dim == 3) {
+        if (cs->model != MODEL_NAVIER_STOKES || cs->viscous) {
+            c->err = "tetrahedra: only the inviscid NavierStokes model has device kernels";
+            return FP_UNSUPPORTED;
+        }
+        if (cs->has_attack) {   // the device inlet of the 3D model blows along -n
+            c->err = "tetrahedra: no attack angle";
+            return FP_UNSUPPORTED;
+        }
+        if (cs->order > 2) {   // basis3Dtetra's node list has the 4 vertices only
+            c->err = "tetrahedra: basis3Dtetra builds orders 1 and 2 only";
+            return FP_UNSUPPORTED;
+        }
+        c->NB = cs->order == 1 ? 1 : 4;
+        c->NEQ = 5;
+    } else {
+        c->NB = make_basis(cs->order).nb;
+        c->NEQ = cs->model == MODEL_SCALAR ? 1 : (cs->model == MODEL_SA ? 5 : 4);
+    }
+    if (cs->model == MODEL_SA && !cs->wall_distance) {
+        c->err = "the Spalart-Allmaras model needs the vertex wall distance";
+        return FP_EINVAL;
+    }
+    c->B = c->NB * c->NEQ;
+    c->NQV = cs->n_vol;
+    c->NQF = cs->n_face;
+    c->dofs = (long long)n * c->B;
+    // SpalartAllmaras.init passes isDiffusive = true to Aerodynamics.init (SpalartAllmaras.java:38)
+    const bool diffusive = cs->model == MODEL_SCALAR ? cs->scalar_k > 0 : (cs->model == MODEL_SA || cs->viscous != 0);
+    c->L = ElemLaunch{cs->model, c->NB, c->NEQ, c->NQV, c->NQF, diffusive ? 1 : 0};
+    if (!(c->dim == 3 ? tet_supported(c->L) : element_supported(c->L))) {
+        c->err = "no element kernel for this model / order / quadrature";
+        return FP_UNSUPPORTED;
+    }
+    if (cs->preconditioner < 0 || cs->preconditioner > 2 || (cs->precond_bits != 64 && cs->precond_bits != 32)) {
+        c->err = "bad preconditioner settings";
+        return FP_EINVAL;
+    }
+    c->precond = cs->preconditioner;
+    c->bits = cs->precond_bits;
+    c->m = cs->gmres_restart > 0 ? cs->gmres_restart : 30;
+    c->cycles = cs->gmres_cycles > 0 ? cs->gmres_cycles : 5;
+    c->tol = cs->gmres_tol;
+    c->positivity = cs->positivity_limiter;
+    ModelParams& P = c->mp;
+    P.kind = cs->model;
+    P.nEqs = c->NEQ;
+    P.diffusive = diffusive;
+    P.has_attack = cs->has_attack;
+    P.kapa = cs->kappa;
+    c->kappa = cs->kappa;
+    P.pOut = cs->p_out;
+    P.Re = cs->reynolds;
+    P.Pr = cs->prandtl;
+    P.attack = cs->attack;
+    P.a0 = cs->scalar_a0;
+    P.a1 = cs->scalar_a1;
+    P.k = cs->scalar_k;
+    P.lambda = std::sqrt(cs->scalar_a0 * cs->scalar_a0 + cs->scalar_a1 * cs->scalar_a1);
+    P.vtIn = cs->sa_vt_in;
+    P.sigma = cs->sa_sigma;
+    P.cb1 = cs->sa_cb1;
+    P.cb2 = cs->sa_cb2;
+    P.ka = cs->sa_ka;
+    P.cw1 = P.ka != 0 && P.sigma != 0 ? P.cb1 / (P.ka * P.ka) + (1 + P.cb2) / P.sigma : 0.0;   // SpalartAllmaras.java:53
+    P.cw2 = cs->sa_cw2;
+    P.cw3 = cs->sa_cw3;
+    P.cv1 = cs->sa_cv1;
+    P.Prt = cs->sa_prt;
+    c->dg.beta0 = cs->order == 1 ? 0.0 : cs->beta0;                  // DGpure.java:287-289
+    c->dg.cIP = (cs->order > 1 && diffusive) ? cs->penalty : 0.0;   // DGpure.limiter :396-398
+
+    HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    std::vector<int> nbr;
+    int rc = c->dim == 3 ? setup_tet(cs, c, nbr) : setup_tri(cs, c, nbr);
+    if (rc) return rc;
+    const int NF = c->NF;
 
     // --- BSR pattern: row r = own block + face neighbours, sorted by column
     std::vector<int> rowptr(n + 1, 0), colidx, diagpos(n);
     for (int r = 0; r < n; r++) {
         std::vector<int> cols = {r};
-        for (int k = 0; k < 3; k++)
-            if (nbr[3 * r + k] > -1 && std::find(cols.begin(), cols.end(), nbr[3 * r + k]) == cols.end())
-                cols.push_back(nbr[3 * r + k]);
+        for (int k = 0; k < NF; k++)
+            if (nbr[NF * r + k] > -1 && std::find(cols.begin(), cols.end(), nbr[NF * r + k]) == cols.end())
+                cols.push_back(nbr[NF * r + k]);
         std::sort(cols.begin(), cols.end());
         for (size_t i = 0; i < cols.size(); i++) {
             if (cols[i] == r) diagpos[r] = (int)colidx.size();
@@ -592,14 +894,14 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
             if (colidx[s] == col) return s;
         return -1;
     };
-    std::vector<int> blkn(3 * (size_t)n, -1), blkd(n);
-    c->blk_own_nb.assign(3 * (size_t)n, -1);
+    std::vector<int> blkn(NF * (size_t)n, -1), blkd(n);
+    c->blk_own_nb.assign(NF * (size_t)n, -1);
     for (int r = 0; r < n; r++) {
         blkd[r] = diagpos[r];
-        for (int k = 0; k < 3; k++)
-            if (nbr[3 * r + k] > -1) {
-                blkn[3 * r + k] = find_blk(nbr[3 * r + k], r);
-                c->blk_own_nb[3 * r + k] = find_blk(r, nbr[3 * r + k]);
+        for (int k = 0; k < NF; k++)
+            if (nbr[NF * r + k] > -1) {
+                blkn[NF * r + k] = find_blk(nbr[NF * r + k], r);
+                c->blk_own_nb[NF * r + k] = find_blk(r, nbr[NF * r + k]);
             }
     }
     // --- ILU level sets (forward: lower blocks, backward: upper blocks)
@@ -674,11 +976,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         c->ilu_rows = ilu_factor_rows_supported(c->B, max_row) && !(g && g[0] == '0');
     }
 
-    HIPCHK(c, upload(&c->geo, geo));
-    HIPCHK(c, upload(&c->fgeo, fgeo));
-    HIPCHK(c, upload(&c->esize, esize));
     HIPCHK(c, upload(&c->nbr, nbr));
-    HIPCHK(c, upload(&c->nbrf, nbrf));
     HIPCHK(c, upload(&c->blkn, blkn));
     HIPCHK(c, upload(&c->blkd, blkd));
     HIPCHK(c, upload(&c->rowptr, rowptr));
@@ -690,8 +988,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     HIPCHK(c, upload(&c->meta_f, meta_f));
     HIPCHK(c, upload(&c->meta_b, meta_b));
     HIPCHK(c, upload(&c->ftask, ftask));
-    if (!wdist.empty()) HIPCHK(c, upload(&c->wdist, wdist));
     c->ms = Mesh{n, c->geo, c->fgeo, c->nbr, c->nbrf, c->blkn, c->blkd, c->wdist};
+    c->tms = TetMesh{n, c->geo, c->fgeo, c->nbr, c->blkn, c->blkd};
 
     // --- state, matrix, Krylov workspace
     const size_t N = (size_t)c->dofs, BB = (size_t)c->B * c->B;
@@ -831,7 +1129,7 @@ extern "C" int fp_restore_level(fp_ctx* c) {
 // ============================================================== Newton step
 extern "C" int fp_time_step(fp_ctx* c, double* out) {
     if (!c || !out) return FP_EINVAL;
-    HIPCHK(c, launch_size_over_lambda(c->L, c->s, c->mp, c->tb, c->ms, c->W, c->esize, c->isref, c->tmp));
+    HIPCHK(c, run_size_over_lambda(c));
     HIPCHK(c, launch_min(c->s, (size_t)c->nElem, c->tmp, c->partial, c->scal));
     HIPCHK(c, hipMemcpyAsync(out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
@@ -854,10 +1152,9 @@ static void bdf_coefficients(int time_order, double dt, double dto, double* a) {
 static int assemble_impl(fp_ctx* c, double dt, double dto, int time_order, int with_jac, double* eq_norms, double* rhs_norm) {
     double a[3];
     bdf_coefficients(time_order, dt, dto, a);
-    HIPCHK(c, launch_residual_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R, c->RHS,
-                                   c->enorm));
+    HIPCHK(c, run_residual(c, a));
     if (with_jac) {
-        HIPCHK(c, launch_jacobian_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a[0], c->val));
+        HIPCHK(c, run_jacobian(c, a[0]));
         c->jac_valid = true;
         c->last_a1 = a[0];
     }
@@ -1001,9 +1298,9 @@ static int op_apply(fp_ctx* c, const double* v, double* y) {
     if (c->op == 0) {
         HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, v, nullptr, y, 0));
     } else if (c->op == 1) {
-        HIPCHK(c, launch_jvp(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, v, c->op_a1, y));
+        HIPCHK(c, run_jvp(c, v, c->op_a1, y));
     } else {
-        HIPCHK(c, launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->op_a1, nullptr, v, y, 1));
+        HIPCHK(c, run_jtv(c, v, c->op_a1, y));
     }
     return FP_OK;
 }
@@ -1153,6 +1450,7 @@ extern "C" int fp_linear_solve_jfnk(fp_ctx* c, int* iterations, double* rel_resi
 
 extern "C" int fp_adjoint_solve(fp_ctx* c, double a1, const double* rhs, double* lambda, int* iterations, double* rel_residual) {
     if (!c || !rhs || !lambda || c->mp.kind != MODEL_NAVIER_STOKES) return FP_EINVAL;
+    if (c->dim != 2) return FP_UNSUPPORTED;
     int rc = to_device(c, rhs, c->RHS);
     if (rc) return rc;
     HIPCHK(c, launch_fill(c->s, (size_t)c->dofs, 0.0, c->x));
@@ -1174,6 +1472,7 @@ extern "C" int fp_adjoint_solve(fp_ctx* c, double a1, const double* rhs, double*
 
 extern "C" int fp_functional(fp_ctx* c, int which, double* value, double* dIdW) {
     if (!c || !value || (which != 0 && which != 1) || c->mp.kind != MODEL_NAVIER_STOKES) return FP_EINVAL;
+    if (c->dim != 2) return FP_UNSUPPORTED;   // the force coefficients integrate over wall edges
     HIPCHK(c, launch_functional(c->L, c->s, c->tb, c->ms, c->W, c->kappa, which, c->tmp, dIdW ? c->w : nullptr));
     HIPCHK(c, launch_dot(c->s, (size_t)c->nElem, c->tmp, nullptr, c->partial, c->scal, 2));
     HIPCHK(c, hipMemcpyAsync(value, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -1188,7 +1487,7 @@ extern "C" int fp_update(fp_ctx* c, int* n_limited) {
     int cnt = 0;
     if (c->positivity) {
         HIPCHK(c, hipMemsetAsync(c->istat + 1, 0, sizeof(int), c->s));
-        HIPCHK(c, launch_positivity(c->L, c->s, c->tb, c->ms, c->W, c->isref, c->istat + 1));
+        HIPCHK(c, run_positivity(c, c->istat + 1));
         HIPCHK(c, hipMemcpyAsync(&cnt, c->istat + 1, sizeof(int), hipMemcpyDeviceToHost, c->s));
     }
     HIPCHK(c, hipStreamSynchronize(c->s));
@@ -1217,9 +1516,9 @@ extern "C" int fp_export_jacobian(fp_ctx* c, double* diag, double* neigh) {
         const size_t e = c->order_[r];
         if (diag) std::memcpy(diag + e * BB, &h[(size_t)diagpos[r] * BB], BB * sizeof(double));
         if (neigh)
-            for (int k = 0; k < 3; k++) {
-                double* dst = neigh + (e * 3 + k) * BB;
-                int blk = c->blk_own_nb[3 * r + k];
+            for (int k = 0; k < c->NF; k++) {
+                double* dst = neigh + (e * c->NF + k) * BB;
+                int blk = c->blk_own_nb[(size_t)c->NF * r + k];
                 if (blk >= 0) std::memcpy(dst, &h[(size_t)blk * BB], BB * sizeof(double));
                 else std::memset(dst, 0, BB * sizeof(double));
             }
@@ -1238,7 +1537,7 @@ extern "C" int fp_jv(fp_ctx* c, double a1, const double* v, double* out) {
     if (!c || !v || !out) return FP_EINVAL;
     int rc = to_device(c, v, c->aux);
     if (rc) return rc;
-    HIPCHK(c, launch_jvp(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->aux, a1, c->w));
+    HIPCHK(c, run_jvp(c, c->aux, a1, c->w));
     return from_device(c, c->w, out);
 }
 
@@ -1246,7 +1545,7 @@ extern "C" int fp_jtv(fp_ctx* c, double a1, const double* v, double* out) {
     if (!c || !v || !out) return FP_EINVAL;
     int rc = to_device(c, v, c->aux);
     if (rc) return rc;
-    HIPCHK(c, launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, nullptr, c->aux, c->w, 1));
+    HIPCHK(c, run_jtv(c, c->aux, a1, c->w));
     return from_device(c, c->w, out);
 }
 
@@ -1261,11 +1560,10 @@ extern "C" int fp_time_kernel(fp_ctx* c, int which, int reps, double* avg_ms) {
     auto once = [&]() -> int {
         switch (which) {
             case 0:
-                HIPCHK(c, launch_residual_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R,
-                                               c->RHS, c->enorm));
+                HIPCHK(c, run_residual(c, a));
                 return FP_OK;
             case 1:
-                HIPCHK(c, launch_jacobian_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a[0], c->val));
+                HIPCHK(c, run_jacobian(c, a[0]));
                 c->jac_valid = true;
                 return FP_OK;
             case 2:
@@ -1276,7 +1574,7 @@ extern "C" int fp_time_kernel(fp_ctx* c, int which, int reps, double* avg_ms) {
             case 4:
                 return prec_apply(c, c->aux, c->RHS);
             case 5:
-                HIPCHK(c, launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a[0], nullptr, c->RHS, c->w, 1));
+                HIPCHK(c, run_jtv(c, c->RHS, a[0], c->w));
                 return FP_OK;
         }
         return FP_EINVAL;

@@ -55,9 +55,10 @@ def init(props: dict) -> ModelConstants:
         return ModelConstants(model, 2, 1, 2, k > 0, a=a, k=k)
     if model not in ("NavierStokes", "SpalartAllmaras"):
         raise UnsupportedModel(model)
-    if int(props.get("dimension", 2)) != 2:
-        raise UnsupportedModel(f"{model} in 3D")
+    dim = int(props.get("dimension", 2))
     sa = model == "SpalartAllmaras"
+    if dim not in (2, 3) or (dim == 3 and (sa or bool(props.get("isFlowViscous", False)))):
+        raise UnsupportedModel(f"{model} in {dim}D")
     kappa, cv = float(props["kappa"]), float(props["cv"])
     # subsonic stagnation inlet: pIn0, rhoIn0, pOut (Aerodynamics.java:169-182)
     p_ref, rho_ref = float(props["pIn0"]), float(props["rhoIn0"])
@@ -80,7 +81,7 @@ def init(props: dict) -> ModelConstants:
     extra = {}
     if sa:
         extra = dict(vt_in=float(props["vtIn"]), sa={k: float(props.get(k, v)) for k, v in SA_DEFAULTS.items()})
-    return ModelConstants(model, MODEL_IDS[model], 5 if sa else 4, 2, diffusive, kappa=kappa, cv=cv, p_ref=p_ref,
+    return ModelConstants(model, MODEL_IDS[model], dim + (3 if sa else 2), dim, diffusive, kappa=kappa, cv=cv, p_ref=p_ref,
                           rho_ref=rho_ref, velocity_ref=velocity_ref, p_out=p_out, reynolds=re, prandtl=pr,
                           attack=attack, **extra)
 
@@ -95,8 +96,10 @@ def const_init_condition(mc: ModelConstants) -> list[float]:
     rho_in = math.pow(1 + ((kappa - 1) / 2) * mach_in * mach_in, 1 / (1 - kappa))
     v_in = mach_in * math.sqrt((kappa * mc.p_out) / rho_in)
     alpha = mc.attack or 0.0
-    w = [rho_in, rho_in * v_in * math.cos(alpha), rho_in * v_in * math.sin(alpha),
-         mc.p_out / (kappa - 1) + 0.5 * rho_in * v_in * v_in]
+    w = [rho_in, rho_in * v_in * math.cos(alpha), rho_in * v_in * math.sin(alpha)]
+    if mc.dim == 3:
+        w.append(0.0)
+    w.append(mc.p_out / (kappa - 1) + 0.5 * rho_in * v_in * v_in)
     if mc.model_id == 1:
         w.append(rho_in * mc.vt_in)
     return w

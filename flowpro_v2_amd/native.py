@@ -23,7 +23,7 @@ EXPORTS = ["fp_neighbor_matrix", "fp_rcm_order", "fp_create", "fp_destroy", "fp_
            "fp_element_order", "fp_set_state", "fp_get_state", "fp_next_level", "fp_restore_level", "fp_time_step",
            "fp_assemble", "fp_linear_solve", "fp_update", "fp_export_residual", "fp_export_jacobian",
            "fp_export_solution", "fp_jv", "fp_jtv", "fp_time_kernel", "fp_stream", "fp_linear_solve_jfnk",
-           "fp_functional", "fp_adjoint_solve", "fp_wall_distance"]
+           "fp_functional", "fp_adjoint_solve", "fp_wall_distance", "fp_neighbor_matrix_tet", "fp_rcm_order_tet"]
 
 
 class FpCase(C.Structure):
@@ -46,6 +46,7 @@ class FpCase(C.Structure):
         ("sa_vt_in", C.c_double), ("sa_sigma", C.c_double), ("sa_cb1", C.c_double), ("sa_cb2", C.c_double),
         ("sa_ka", C.c_double), ("sa_cw2", C.c_double), ("sa_cw3", C.c_double), ("sa_cv1", C.c_double),
         ("sa_prt", C.c_double),
+        ("dimension", C.c_int),
     ]
 
 
@@ -88,6 +89,8 @@ def lib() -> C.CDLL:
         "fp_functional": (C.c_int, [vp, C.c_int, dp, dp]),
         "fp_adjoint_solve": (C.c_int, [vp, C.c_double, dp, dp, ip, dp]),
         "fp_wall_distance": (C.c_int, [C.c_int, dp, C.c_int, ip, ip, dp]),
+        "fp_neighbor_matrix_tet": (C.c_int, [C.c_int, ip, C.c_int, ip, ip]),
+        "fp_rcm_order_tet": (C.c_int, [C.c_int, ip, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -117,6 +120,26 @@ def neighbor_matrix(TP: np.ndarray, bnd: np.ndarray) -> np.ndarray:
     if rc:
         raise FatalError(f"fp_neighbor_matrix: {rc}")
     return TT
+
+
+def neighbor_matrix_tet(TP: np.ndarray, bnd: np.ndarray) -> np.ndarray:
+    """createNeighborMatrix on tetrahedra (fp_neighbor_matrix_tet): TT[n_elems][4]."""
+    TP = np.ascontiguousarray(TP, dtype=np.int32)
+    bnd = np.ascontiguousarray(bnd, dtype=np.int32).reshape(-1, 4)
+    TT = np.zeros_like(TP)
+    rc = lib().fp_neighbor_matrix_tet(TP.shape[0], iptr(TP), bnd.shape[0], iptr(bnd), iptr(TT))
+    if rc:
+        raise FatalError(f"fp_neighbor_matrix_tet: {rc}")
+    return TT
+
+
+def rcm_order_tet(TT: np.ndarray) -> np.ndarray:
+    TT = np.ascontiguousarray(TT, dtype=np.int32)
+    order = np.zeros(TT.shape[0], np.int32)
+    rc = lib().fp_rcm_order_tet(TT.shape[0], iptr(TT), iptr(order))
+    if rc:
+        raise FatalError(f"fp_rcm_order_tet: {rc}")
+    return order
 
 
 def wall_distance(PXY: np.ndarray, TP: np.ndarray, TT: np.ndarray) -> np.ndarray:

@@ -39,3 +39,10 @@ def triangle_rule(order: int):
     xy = np.stack([0.0 * bary[:, 0] + 1.0 * bary[:, 1] + 0.0 * bary[:, 2],
                    0.0 * bary[:, 0] + 0.0 * bary[:, 1] + 1.0 * bary[:, 2]], 1)
     return np.ascontiguousarray(xy), w
+
+
+def tetra_rule(order: int):
+    """Reference-tetrahedron rule of degree 2k - 1 (QuadratureTetra: the file holds the
+    (xi, eta, zeta) coordinates directly; the rules have positive weights, no swap)."""
+    x, w = _read(os.path.join(RULES_DIR, "gaussTetra", f"{2 * order - 1}.txt"), 3)
+    return np.ascontiguousarray(x), w

@@ -45,8 +45,15 @@ class DeviceCase:
         self.order = order
         self.props = props
         self.case = case
-        vxi, vw = quadrature.triangle_rule(order)
-        fxi, fw = quadrature.line_rule(order)
+        self.dim = int(self.mc.dim)
+        if self.dim == 3:
+            if self.mc.attack is not None:
+                raise UnsupportedCase("attack angle on tetrahedra")
+            vxi, vw = quadrature.tetra_rule(order)
+            fxi, fw = quadrature.triangle_rule(order)
+        else:
+            vxi, vw = quadrature.triangle_rule(order)
+            fxi, fw = quadrature.line_rule(order)
         self._keep = [np.ascontiguousarray(case.PXY, np.float64), np.ascontiguousarray(case.TP, np.int32),
                       np.ascontiguousarray(case.bnd, np.int32), np.ascontiguousarray(vxi), np.ascontiguousarray(vw),
                       np.ascontiguousarray(fxi), np.ascontiguousarray(fw)]
@@ -78,6 +85,7 @@ class DeviceCase:
         cs.gmres_cycles = int(props.get("gmresCycles", 5))
         cs.gmres_tol = float(props.get("iterativeSolverTol", 1e-2))
         cs.positivity_limiter = int(bool(props.get("positivityLimiter", True)))
+        cs.dimension = self.dim
         if mc.sa is not None:
             if wd is None:
                 raise UnsupportedCase("SpalartAllmaras needs the vertex wall distance (Case.wall_distance)")
@@ -186,7 +194,7 @@ class DeviceCase:
 
     def jacobian(self):
         d = np.zeros((self.n_elem, self.B, self.B))
-        n = np.zeros((self.n_elem, 3, self.B, self.B))
+        n = np.zeros((self.n_elem, self.dim + 1, self.B, self.B))
         self._check(native.lib().fp_export_jacobian(self.h, native.dptr(d), native.dptr(n)), "fp_export_jacobian")
         return d, n
 

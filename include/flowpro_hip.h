@@ -38,21 +38,22 @@ enum {
 typedef struct fp_ctx fp_ctx;
 
 /* Case description: the inputs of FlowProMain.solverFactory and Mesh
- * (FlowProMain.java:486-730, Mesh.java:69-224) for straight triangles of one order. */
+ * (FlowProMain.java:486-730, Mesh.java:69-224) for straight triangles of one order, or for
+ * straight tetrahedra (dimension = 3; then the array shapes in brackets after "3D:" apply). */
 typedef struct fp_case {
     int n_points;
-    const double* pxy;            /* [n_points][2] */
+    const double* pxy;            /* [n_points][2]; 3D: [n_points][3] */
     int n_elems;
-    const int* tp;                /* [n_elems][3], counter-clockwise */
+    const int* tp;                /* [n_elems][3], counter-clockwise; 3D: [n_elems][4] */
     int n_bnd;
-    const int* bnd;               /* [n_bnd][3]: boundary type, v0, v1 (boundaryType.txt) */
+    const int* bnd;               /* [n_bnd][3]: boundary type, v0, v1 (boundaryType.txt); 3D: [n_bnd][4] */
     const double* wall_distance;  /* [n_points] or NULL */
     int order;                    /* element order k = p + 1 */
     int n_vol;                    /* volume rule (QuadratureCentral: degree 2k-1, positive weights) */
-    const double* vol_xi;         /* [n_vol][2] reference-triangle coordinates */
+    const double* vol_xi;         /* [n_vol][2] reference-triangle coordinates; 3D: [n_vol][3] gaussTetra */
     const double* vol_w;          /* [n_vol] */
-    int n_face;                   /* Gauss-Legendre rule of order k on [-1, 1] */
-    const double* face_xi;        /* [n_face] */
+    int n_face;                   /* Gauss-Legendre rule of order k on [-1, 1]; 3D: the triangle rule */
+    const double* face_xi;        /* [n_face]; 3D: [n_face][2] reference-triangle coordinates */
     const double* face_w;         /* [n_face] */
     int model;                    /* 0 NavierStokes (Euler when viscous == 0), 1 SpalartAllmaras,
                                      2 TestScalarEquation */
@@ -76,6 +77,9 @@ typedef struct fp_case {
     /* SpalartAllmaras (SpalartAllmaras.java:36-61): inlet turbulence vtIn and the model constants
      * (cw1 is derived); wall_distance above is required for this model */
     double sa_vt_in, sa_sigma, sa_cb1, sa_cb2, sa_ka, sa_cw2, sa_cw3, sa_cv1, sa_prt;
+    /* Parameters.dimension: 2 (or 0) triangles, 3 tetrahedra.  Tetrahedra carry the inviscid
+     * model (viscous = 0) at orders 1 and 2, the orders basis3Dtetra can build */
+    int dimension;
 } fp_case;
 
 /* --- integer setup (bit-exact with the reference) ------------------------------ */
@@ -83,6 +87,10 @@ typedef struct fp_case {
 int fp_neighbor_matrix(int n_elems, const int* tp, int n_bnd, const int* bnd, int* tt);
 /* ElementOrdering.reverseCuthillMcKee (ElementOrdering.java:52-142): order[n_elems] */
 int fp_rcm_order(int n_elems, const int* tt, int* order);
+/* the same two for tetrahedra (faces {0,2,1}, {0,1,3}, {1,2,3}, {2,0,3} of tetrahedralElement):
+ * bnd[n_bnd][4] = type, v0, v1, v2; tt[n_elems][4] */
+int fp_neighbor_matrix_tet(int n_elems, const int* tp, int n_bnd, const int* bnd, int* tt);
+int fp_rcm_order_tet(int n_elems, const int* tt, int* order);
 /* FlowProMain.generateWallDistanceFunction / generateDistanceFunction (FlowProMain.java:931-1014):
  * distance of every vertex from the wall faces (tt == -1) along mesh edges; dist[n_points] */
 int fp_wall_distance(int n_points, const double* pxy, int n_elems, const int* tp, const int* tt, double* dist);
@@ -123,6 +131,7 @@ int fp_update(fp_ctx* ctx, int* n_limited);
 /* --- parity and adjoint ---------------------------------------------------------------- */
 int fp_export_residual(fp_ctx* ctx, double* R, double* RHS);
 /* blocks in the reference's transposed layout: diag[e][i][j] = ADiag, neigh[e][k][i][j] = ANeighs[k].A
+ * (k over the element's faces: 3 on triangles, 4 on tetrahedra)
  * (zero for boundary faces) */
 int fp_export_jacobian(fp_ctx* ctx, double* diag, double* neigh);
 /* the last Newton correction x in DOF numbering */

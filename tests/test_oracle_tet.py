@@ -10,6 +10,8 @@ No reference test covers 3D elements; the checks are what the geometry and the s
   exact Jacobian = central differences of the residual (1e-6)
   the generic RCM equals the triangle RCM of the product on a triangle mesh (bit-exact)
   order 3 is rejected, as basis3Dtetra's 4-node list cannot build a 10-function basis
+The product's integer setup on tetrahedra (fp_neighbor_matrix_tet, fp_rcm_order_tet) equals the
+oracle's bit for bit, also with duplicated and missing boundary rows.
 """
 import numpy as np
 import pytest
@@ -101,3 +103,16 @@ def test_rcm_on_tets_is_a_permutation(cube):
     TT = O.tet_neighbor_matrix(cube.TP, cube.bnd)
     order = O.rcm_order(TT)
     assert np.array_equal(np.sort(order), np.arange(cube.n_elem))
+
+
+def test_product_tet_integer_setup_bit_exact():
+    for n, seed in ((2, 4), (3, 5)):
+        cube = cases.kuhn_cube(n, 0.05, seed)
+        TT = native.neighbor_matrix_tet(cube.TP, cube.bnd)
+        assert np.array_equal(TT, O.tet_neighbor_matrix(cube.TP, cube.bnd))
+        assert np.array_equal(native.rcm_order_tet(TT), O.rcm_order(TT))
+        # a boundary row repeated with another type (the later row wins) and missing rows (-1)
+        extra = cube.bnd[:5].copy()
+        extra[:, 0] = -1
+        for bnd in (np.concatenate([cube.bnd, extra]), cube.bnd[10:]):
+            assert np.array_equal(native.neighbor_matrix_tet(cube.TP, bnd), O.tet_neighbor_matrix(cube.TP, bnd))
