@@ -216,42 +216,29 @@ def kuhn_cube(n: int = 4, jitter: float = 0.05, seed: int = 4, length: float = 1
     rng = np.random.default_rng(seed)
     inner = np.all((P > 1e-12) & (P < length - 1e-12), axis=1)
     P[inner] += jitter * h * rng.uniform(-1, 1, size=(int(inner.sum()), 3))
-    tets = []
-    for i in range(n):
-        for j in range(n):
-            for k in range(n):
-                for path in _KUHN_PATHS:
-                    c = [i, j, k]
-                    vs = [vid[tuple(c)]]
-                    for ax in path:
-                        c[ax] += 1
-                        vs.append(vid[tuple(c)])
-                    tets.append(vs)
-    TP = np.array(tets, dtype=np.int32)
+    # cube (i, j, k) in lexicographic order, then the six axis orders: the path from v000 adds
+    # one axis at a time, so the vertex ids are cumulative sums of the axis strides
+    stride = np.array([(n + 1) ** 2, n + 1, 1])
+    base = vid[:n, :n, :n].reshape(-1)
+    steps = stride[np.array(_KUHN_PATHS)]                                   # (6, 3)
+    offs = np.concatenate([np.zeros((6, 1), np.int64), np.cumsum(steps, 1)], 1)   # (6, 4)
+    TP = (base[:, None, None] + offs[None]).reshape(-1, 4).astype(np.int32)
     a, b, c, d = P[TP[:, 0]], P[TP[:, 1]], P[TP[:, 2]], P[TP[:, 3]]
     det = np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a)
     pos = det > 0
     TP[pos, 1], TP[pos, 2] = TP[pos, 2].copy(), TP[pos, 1].copy()
-    rows = []
-    faces = ((0, 2, 1), (0, 1, 3), (1, 2, 3), (2, 0, 3))
-    Pv = P
-    seen = {}
-    for e in range(TP.shape[0]):
-        for f in faces:
-            key = tuple(sorted(int(TP[e, q]) for q in f))
-            seen[key] = seen.get(key, 0) + 1
-    for key, cnt in seen.items():
-        if cnt != 1:
-            continue
-        x = Pv[list(key)]
-        if np.all(np.abs(x[:, 0]) < 1e-12):
-            t = INLET
-        elif np.all(np.abs(x[:, 0] - length) < 1e-12):
-            t = OUTLET
-        else:
-            t = INVISCID_WALL
-        rows.append((t,) + key)
-    return Case(f"kuhn_{n}", P, TP, np.array(rows, dtype=np.int32))
+    # faces seen once, in the order of their first appearance (element, then local face)
+    faces = np.array(((0, 2, 1), (0, 1, 3), (1, 2, 3), (2, 0, 3)))
+    keys = np.sort(TP[:, faces].reshape(-1, 3), axis=1)
+    uniq, first, cnt = np.unique(keys, axis=0, return_index=True, return_counts=True)
+    once = np.argsort(first[cnt == 1], kind="stable")
+    keys = uniq[cnt == 1][once]
+    x = P[keys]                                                             # (nb, 3, 3)
+    t = np.full(len(keys), INVISCID_WALL, np.int32)
+    t[np.all(np.abs(x[:, :, 0]) < 1e-12, axis=1)] = INLET
+    t[np.all(np.abs(x[:, :, 0] - length) < 1e-12, axis=1)] = OUTLET
+    rows = np.concatenate([t[:, None], keys.astype(np.int32)], 1)
+    return Case(f"kuhn_{n}", P, TP, np.ascontiguousarray(rows, dtype=np.int32))
 
 
 def euler3d_props(mach: float = 0.8, order: int = 2, **kw) -> dict:
