@@ -25,7 +25,9 @@ _D = ["kappa", "cv", "pIn0", "rhoIn0", "pOut", "reynolds", "prandtl", "attack_an
 
 
 class Props(C.Structure):
-    _fields_ = [(n, C.c_int) for n in _I] + [(n, C.c_double) for n in _D] + [("dimension", C.c_int)]
+    _fields_ = [(n, C.c_int) for n in _I] + [(n, C.c_double) for n in _D] + [("dimension", C.c_int)] + \
+        [("damp_tol", C.c_double), ("damp_const", C.c_double), ("damp_inner_tol", C.c_double),
+         ("n_damp_inner_coef", C.c_int), ("damp_inner_coef", C.c_double * 5)]
 
 
 def build(force: bool = False) -> str:
@@ -51,6 +53,7 @@ def lib():
             "or_tt": (None, [vp, ip]), "or_order": (None, [vp, ip]), "or_gindex": (None, [vp, ip]),
             "or_set_w": (None, [vp, dp, dp, dp]), "or_get_w": (None, [vp, dp]),
             "or_limiter": (None, [vp]), "or_residual": (None, [vp, dp]),
+            "or_artificial_viscosity": (None, [vp, dp, dp]), "or_trunc": (None, [vp, C.c_int, dp]),
             "or_assemble": (None, [vp, C.c_double, C.c_double]),
             "or_assemble_residuum": (None, [vp, C.c_double, C.c_double]),
             "or_get_blocks": (None, [vp, dp, dp]), "or_get_rhs": (None, [vp, dp, dp]),
@@ -128,6 +131,14 @@ def props_from(p: dict, threads: int = 1) -> Props:
     a = p.get("a", (1.0, 0.5))
     P.scalar_a0, P.scalar_a1, P.scalar_k = float(a[0]), float(a[1]), float(p.get("k", 0.0))
     P.dimension = int(p.get("dimension", 2))
+    P.damp_tol, P.damp_const = float(p.get("dampTol", 0.0)), float(p.get("dampConst", 0.0))
+    P.damp_inner_tol = float(p.get("dampInnerTol", 0.0))
+    coef = p.get("dampInnerCoef")
+    if coef is not None:
+        coef = [float(v) for v in (coef if hasattr(coef, "__len__") else [coef])]
+        P.n_damp_inner_coef = len(coef)
+        for i, v in enumerate(coef):
+            P.damp_inner_coef[i] = v
     P.sa_vtIn = float(p.get("vtIn", 0.0))
     for key, field in (("sigma", "sa_sigma"), ("cb1", "sa_cb1"), ("cb2", "sa_cb2"), ("ka", "sa_ka"), ("cw2", "sa_cw2"),
                        ("cw3", "sa_cw3"), ("cv1", "sa_cv1"), ("Prt", "sa_prt")):
@@ -213,6 +224,18 @@ class Oracle:
 
     def limiter(self):
         lib().or_limiter(self.h)
+
+    def artificial_viscosity(self):
+        """(eps[n_elem], dampInner[n_elem, nEqs]) set by the last limiter call."""
+        eps = np.zeros(self.n_elem)
+        di = np.zeros((self.n_elem, self.neq))
+        lib().or_artificial_viscosity(self.h, _dp(eps), _dp(di))
+        return eps, di
+
+    def trunc(self, i):
+        out = np.zeros((self.nb, self.nb))
+        lib().or_trunc(self.h, int(i), _dp(out))
+        return out
 
     def residual(self):
         out = np.zeros((self.n_elem, self.B))

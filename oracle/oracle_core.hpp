@@ -97,6 +97,11 @@ struct Props {
     // SpalartAllmaras (AERO/SpalartAllmaras.java:36-61): vtIn and the model constants
     double sa_vtIn, sa_sigma, sa_cb1, sa_cb2, sa_ka, sa_cw2, sa_cw3, sa_cv1, sa_prt;
     int dimension;        // 2 (triangles) or 3 (tetrahedra, oracle_tet.cpp)
+    // artificial viscosity (DGpure.initMethod, C/element/DGpure.java:40-71): dampTol, dampConst,
+    // dampInnerTol and the optional dampInnerCoef array (n_damp_inner_coef = 0: absent)
+    double damp_tol, damp_const, damp_inner_tol;
+    int n_damp_inner_coef;
+    double damp_inner_coef[5];
 };
 
 struct ModelConst {
@@ -155,7 +160,9 @@ struct Elem {
     std::vector<double> W, Wo, Wo2;
     // limiter outputs (C/element/DGpure.java:358-400)
     double eps = 0, c_IP = 0;
-    std::vector<double> dampInner;
+    std::vector<double> dampInner, innerIndicatorOld;
+    // projection onto linear functions in the nodal basis (DGpure.computeOrderTruncationMatrix, order > 2)
+    std::vector<double> TrunOrd;   // [Nb][Nb]
     int gIndex0 = 0;
     // implicit storage (C/element/Implicit.java:14-126)
     std::vector<double> ADiag;          // [B][B], ADiag[i*B + j] = A[perturbed i][residual j]
@@ -216,7 +223,7 @@ std::vector<double> wall_distance(int nPts, const double* PXY, int nElem, const 
 Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const int* TP,
                  int nBnd, const int* bnd, const double* wallDist, const std::string& rulesDir);
 
-void limiter_all(Case& c);
+void limiter_all(Case& c, bool isFirstIter = false);
 void residual_all(Case& c, double* R);          // spatial residual R(W), Java layout
 void assemble(Case& c, double dt, double dto);  // BDF1/2 assembleJacobiMatrix for every element
 void assemble_residuum(Case& c, double dt, double dto);

@@ -18,10 +18,10 @@ static void residuumWall(const Case& c, const Elem& e, int k, const T* Wown, con
     const int TT = e.TT[k];
     const int nR = nBasis;
     if (KR) for (int j = 0; j < nR * nEqs; j++) KR[j] = 0.0;
-    double beta0 = c.P.beta0;
-    if (c.P.order == 1) beta0 = 0;   // DGpure.java:287-289
     const bool convective = true;
     const bool diffusive = M.diffusive;
+    double beta0 = c.P.beta0;
+    if (diffusive && c.P.order == 1) beta0 = 0;   // DGpure.java:287-289, inside the isDiffusive branch
     for (int p = 0; p < e.nqf[k]; p++) {
         const double* baseLeft = &e.baseL[k][p * nBasis];
         const double* dBaseLeft = &e.dbaseL[k][p * nBasis * 2];
@@ -99,7 +99,7 @@ static void residuumWall(const Case& c, const Elem& e, int k, const T* Wown, con
             T avFlux = 0.0;
             if (TT > -1) {
                 const Elem& er = c.E[TT];
-                double epsFace = 0.5 * (e.eps + er.eps) + 0.0 /*dampConst*/ + 0.5 * (e.dampInner[m] + er.dampInner[m]);
+                double epsFace = 0.5 * (e.eps + er.eps) + c.P.damp_const + 0.5 * (e.dampInner[m] + er.dampInner[m]);
                 avFlux = epsFace * (dWsum + beta0 * (WR[m] - WL[m]) / dL);
             }
             for (int j = 0; j < nBasis; j++) {
@@ -190,7 +190,7 @@ static void residuum(const Case& c, const Elem& e, const T* Wown, const T* const
                         fsum += (f[d][m] - u[d] * WInt[m]) * dBase[j * 2 + d];
                         dWsum += dWInt[nEqs * d + m] * dBase[j * 2 + d];
                     }
-                    K[nBasis * m + j] += Jac * weight * fsum - (e.eps + 0.0 + e.dampInner[m]) * Jac * weight * dWsum;
+                    K[nBasis * m + j] += Jac * weight * fsum - (e.eps + c.P.damp_const + e.dampInner[m]) * Jac * weight * dWsum;
                 }
                 if (volumeFluxes && M.diffusive) {
                     T fvsum = 0.0;
@@ -203,12 +203,84 @@ static void residuum(const Case& c, const Elem& e, const T* Wown, const T* const
 }
 
 // ------------------------------------------------------------- limiter
-// C/element/DGpure.java:358-400 for dampTol = dampInnerTol = 0: eps = 0, c_IP = penalty
-void limiter_all(Case& c) {
+// Element.calculateAvgW: W_avg[m] = sum_j W[m Nb + j] Is[j]
+static void average_w(const Case& c, const Elem& e, double* Wavg) {
+    for (int m = 0; m < c.Mc.nEqs; m++) {
+        Wavg[m] = 0;
+        for (int j = 0; j < e.nBasis; j++) Wavg[m] = Wavg[m] + e.W[m * e.nBasis + j] * e.Is[j];
+    }
+}
+
+// DGpure.shock_senzor_all_eqn (C/element/DGpure.java:454-503): per equation, the L2 distance of
+// the solution from its truncation to linear functions (order > 2) or to the mean, relative to
+// its L2 norm, on a log10 scale, through a sine ramp of half-width kap around S0 = log10(1/(k-1)^4).
+// shock_senzor (:403-451) is the m = 0 entry.
+static void shock_sensors(const Case& c, const Elem& e, double kap, double* shock) {
+    const int nb = e.nBasis;
+    double Ws[5];
+    average_w(c, e, Ws);
+    for (int m = 0; m < c.Mc.nEqs; m++) {
+        std::vector<double> trunc(nb, 0.0);
+        if (e.order > 2) {
+            for (int i = 0; i < nb; i++)
+                for (int j = 0; j < nb; j++) trunc[i] += e.TrunOrd[i * nb + j] * e.W[m * nb + j];
+        } else {
+            for (int i = 0; i < nb; i++) trunc[i] = Ws[m];   // Lagrange basis: the mean in every coefficient
+        }
+        double Se = 0, pod = 0;
+        for (int p = 0; p < e.nq; p++) {
+            double vi = 0, vt = 0;
+            for (int i = 0; i < nb; i++) {
+                vi += e.W[m * nb + i] * e.baseV[p * nb + i];
+                vt += trunc[i] * e.baseV[p * nb + i];
+            }
+            Se += e.JacV[p] * e.wV[p] * (vi - vt) * (vi - vt);
+            pod += e.JacV[p] * e.wV[p] * vi * vi;
+        }
+        Se = std::log10(std::fabs(Se / pod));
+        const double S0 = std::log10(1. / std::pow(e.order - 1, 4.0));
+        double s = 0.5 * (1 + std::sin(M_PI * (Se - S0) / (2 * kap)));
+        if (Se < S0 - kap) s = 0;
+        if (Se > S0 + kap) s = 1;
+        shock[m] = std::isnan(s) ? 0 : s;
+    }
+}
+
+// DGpure.limiter (C/element/DGpure.java:358-400): eps from the density sensor (dampTol), the
+// equation-wise damping from the sensors combined by the model (Aerodynamics.combineShockSensors:
+// every equation takes the density sensor; the scalar model keeps its own) with the decaying
+// memory innerIndicatorOld, and the interior penalty c_IP.
+void limiter_all(Case& c, bool isFirstIter) {
+    (void)isFirstIter;   // the first-iteration branch of the reference has an empty body
+    const int nEqs = c.Mc.nEqs;
     for (auto& e : c.E) {
         e.eps = 0;
         e.c_IP = 0;
-        if (e.order > 1 && c.Mc.diffusive) e.c_IP = c.P.penalty;
+        if (e.order <= 1) continue;
+        double Wavg[5];
+        average_w(c, e, Wavg);
+        QPData q;
+        q.meshVelocity[0] = q.meshVelocity[1] = q.meshVelocity[2] = 0;
+        const double lam = maxEigenvalue(c.Mc, Wavg, q);
+        double shock[5];
+        if (c.P.damp_tol > 0) {
+            shock_sensors(c, e, c.P.damp_tol, shock);
+            e.eps = lam * e.elemSize / e.order * shock[0];
+        }
+        if (c.P.damp_inner_tol > 0) {
+            shock_sensors(c, e, c.P.damp_inner_tol, shock);
+            if (c.Mc.kind != 2)
+                for (int m = 1; m < nEqs; m++) shock[m] = shock[0];
+            for (int m = 0; m < nEqs; m++) {
+                const double indicator = std::max(shock[m], e.innerIndicatorOld[m]);
+                e.innerIndicatorOld[m] = 0.1 * indicator;
+                if (c.P.n_damp_inner_coef > 0)
+                    e.dampInner[m] = (c.P.n_damp_inner_coef == 1 ? c.P.damp_inner_coef[0] : c.P.damp_inner_coef[m]) * indicator;
+                else
+                    e.dampInner[m] = lam * e.elemSize / e.order * indicator;
+            }
+        }
+        if (c.Mc.diffusive) e.c_IP = c.P.penalty;
     }
 }
 

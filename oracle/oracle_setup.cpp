@@ -513,6 +513,7 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
         }
         transform_init(e);
         e.dampInner.assign(nEqs, 0.0);
+        e.innerIndicatorOld.assign(nEqs, 0.0);
     }
     // DOF numbering (C/Mesh.java:181-187, C/element/Element.java:241-248)
     if (P.ordering == 1) c->order = rcm_order(nElem, c->TT.data());
@@ -628,6 +629,40 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
                 for (int p = 0; p < e.nq; p++)
                     e.M[a * Nb + b] = e.M[a * Nb + b] + e.JacV[p] * e.wV[p] * e.baseV[p * Nb + a] * e.baseV[p * Nb + b];
             for (int p = 0; p < e.nq; p++) e.Is[a] = e.Is[a] + e.JacV[p] * e.wV[p] * e.baseV[p * Nb + a] / e.area;
+        }
+        // DGpure.computeOrderTruncationMatrix (C/element/DGpure.java:588-626): TrunOrd =
+        // M^-1 T^T Mhat^-1 T with the Taylor basis {1, x - Xs, y - Xs} at the physical points
+        if (order > 2) {
+            const int nl = 3;
+            std::vector<std::vector<double>> Mhat(nl, std::vector<double>(nl, 0.0)), T(nl, std::vector<double>(Nb, 0.0));
+            std::vector<double> Xq(2 * e.nq);
+            for (int p = 0; p < e.nq; p++) getX(e, &qv.x[2 * p], &Xq[2 * p]);
+            auto taylor = [&](int a, const double* X) { return a > 0 ? X[a - 1] - e.Xs[a - 1] : 1.0; };
+            for (int a = 0; a < nl; a++) {
+                for (int b = 0; b < nl; b++)
+                    for (int p = 0; p < e.nq; p++)
+                        Mhat[a][b] += e.JacV[p] * e.wV[p] * taylor(a, &Xq[2 * p]) * taylor(b, &Xq[2 * p]);
+                for (int b = 0; b < Nb; b++)
+                    for (int p = 0; p < e.nq; p++) T[a][b] += e.JacV[p] * e.wV[p] * taylor(a, &Xq[2 * p]) * e.baseV[p * Nb + b];
+            }
+            auto times = [](const std::vector<std::vector<double>>& A, const std::vector<std::vector<double>>& Bm) {
+                std::vector<std::vector<double>> C(A.size(), std::vector<double>(Bm[0].size(), 0.0));
+                for (size_t i = 0; i < A.size(); i++)
+                    for (size_t j = 0; j < Bm[0].size(); j++)
+                        for (size_t k = 0; k < Bm.size(); k++) C[i][j] += A[i][k] * Bm[k][j];
+                return C;
+            };
+            std::vector<std::vector<double>> Tt(Nb, std::vector<double>(nl)), Mm(Nb, std::vector<double>(Nb));
+            for (int a = 0; a < nl; a++)
+                for (int b = 0; b < Nb; b++) Tt[b][a] = T[a][b];
+            for (int a = 0; a < Nb; a++)
+                for (int b = 0; b < Nb; b++) Mm[a][b] = e.M[a * Nb + b];
+            auto tr = times(mat_invert(Mhat), T);
+            tr = times(Tt, tr);
+            tr = times(mat_invert(Mm), tr);
+            e.TrunOrd.resize((size_t)Nb * Nb);
+            for (int a = 0; a < Nb; a++)
+                for (int b = 0; b < Nb; b++) e.TrunOrd[a * Nb + b] = tr[a][b];
         }
         e.ADiag.assign(B * B, 0.0);
         for (int k = 0; k < 3; k++)

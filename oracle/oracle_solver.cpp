@@ -556,7 +556,7 @@ int run(Case& c, int nsteps) {
         }
         if (c.P.fixed_dt ? c.state_dt < c.P.dt / 1024 : (c.currentCFL > 0 && c.currentCFL < c.P.cfl / 20)) break;
         double dt = time_step(c);
-        limiter_all(c);
+        limiter_all(c, completed == 0);
         double rhsNorm0 = 0;
         converges = true;
         for (int s = 0; s < c.P.newton_iters; s++) {
@@ -644,6 +644,21 @@ void or_get_w(void* h, double* W) {
     }
 }
 void or_limiter(void* h) { limiter_all(*(Case*)h); }
+// eps[e] and dampInner[e][m] of every element (mesh order)
+void or_artificial_viscosity(void* h, double* eps, double* dampInner) {
+    Case* c = (Case*)h;
+    for (int i = 0; i < c->nElem; i++) {
+        eps[i] = c->E[i].eps;
+        for (int m = 0; m < c->Mc.nEqs; m++) dampInner[i * c->Mc.nEqs + m] = c->E[i].dampInner[m];
+    }
+}
+// TrunOrd of element i (zeros when the order is <= 2)
+void or_trunc(void* h, int i, double* out) {
+    Case* c = (Case*)h;
+    const Elem& e = c->E[i];
+    const int nb = e.nBasis;
+    for (int q = 0; q < nb * nb; q++) out[q] = e.TrunOrd.empty() ? 0.0 : e.TrunOrd[q];
+}
 void or_residual(void* h, double* R) { residual_all(*(Case*)h, R); }
 void or_assemble(void* h, double dt, double dto) { assemble(*(Case*)h, dt, dto); }
 void or_assemble_residuum(void* h, double dt, double dto) { assemble_residuum(*(Case*)h, dt, dto); }

@@ -1,0 +1,156 @@
+"""Pins of the artificial-viscosity restatement (DGpure.limiter, shock sensors, the damping
+terms of DGpure.residuum / residuumWall) in the oracle, CPU only.
+
+No reference test covers the damping; the checks are what the definitions fix:
+  TrunOrd (order > 2) is the M-orthogonal projection onto a 3-dimensional space holding the
+  constants: P^2 = P, trace P = 3, M P symmetric, P 1 = 1 (1e-12)
+  at order 2 (P1, truncation to the mean) the density sensor of a linear density follows from
+  closed-form integrals: int rho^2 = A/6 (a^2+b^2+c^2+ab+bc+ca), mean = (a+b+c)/3; eps =
+  lambda(mean state) * elemSize / 2 * g with g the ramp at Se, and the clamps g = 0 / 1 (1e-12)
+  dampConst c on the scalar equation with zero advection and a continuous linear field: the
+  damping residual is -c (grad f . n) |face| / 2 on the two vertices of every boundary face and
+  zero elsewhere (the weak Laplacian with central gradient flux; jumps vanish) (1e-12); the
+  same field with the vertex values in another node order is discontinuous and fails it
+  the inner damping memory: after a sensor value g, a state with a zero sensor gets the
+  indicator 0.1 g (dampInner = lambda h / k * 0.1 g, or dampInnerCoef * 0.1 g) (1e-12)
+"""
+import math
+
+import numpy as np
+import pytest
+
+from flowpro_v2_amd import cases
+import oracle as O
+
+
+def _tri_area(P):
+    a, b, c = P
+    return 0.5 * abs((b[0] - a[0]) * (c[1] - a[1]) - (c[0] - a[0]) * (b[1] - a[1]))
+
+
+@pytest.mark.parametrize("order", [3, 4])
+def test_truncation_matrix_is_the_linear_projection(order):
+    case = cases.channel_bump(6, 3)
+    o = O.Oracle(case, cases.euler_props(order=order, dampTol=1.0))
+    for e in (0, 7, case.n_elem - 1):
+        P = o.trunc(e)
+        M = o.geometry(e)["M"]
+        s = np.abs(P).max()
+        assert np.abs(P @ P - P).max() <= 1e-12 * s
+        assert abs(np.trace(P) - 3) <= 1e-12
+        MP = M @ P
+        assert np.abs(MP - MP.T).max() <= 1e-12 * np.abs(MP).max()
+        assert np.abs(P @ np.ones(o.nb) - 1).max() <= 1e-12
+
+
+def _linear_density_state(case, o, rng, amp):
+    """P1 state with the freestream velocity and energy, density linear in every element (vertex
+    values drawn per element, so the field is discontinuous)."""
+    w0 = o.get_w()[0][:: o.nb]
+    rho = w0[0] * (1 + amp * rng.standard_normal((case.n_elem, 3)))
+    W = np.zeros((case.n_elem, 4, 3))
+    W[:, 0] = rho
+    for m in (1, 2, 3):
+        W[:, m] = w0[m]
+    return W.reshape(case.n_elem, 12), rho
+
+
+def _lambda(w, kappa):
+    p = (kappa - 1) * (w[3] - (w[1] ** 2 + w[2] ** 2) / (2 * w[0]))
+    return math.hypot(w[1] / w[0], w[2] / w[0]) + math.sqrt(kappa * p / w[0])
+
+
+@pytest.mark.parametrize("kap,amp", [(3.0, 0.2), (1e-3, 0.05)])
+def test_density_sensor_at_order_two(kap, amp):
+    case = cases.channel_bump(8, 3)
+    o = O.Oracle(case, cases.euler_props(order=2, dampTol=kap))
+    W, rho = _linear_density_state(case, o, np.random.default_rng(5), amp)
+    o.set_w(W)
+    o.limiter()
+    eps, di = o.artificial_viscosity()
+    assert np.all(di == 0)
+    for e in range(case.n_elem):
+        a, b, c = rho[e]
+        P = case.PXY[case.TP[e]]
+        A = _tri_area(P)
+        perim = sum(np.linalg.norm(P[(k + 1) % 3] - P[k]) for k in range(3))
+        i2 = A / 6 * (a * a + b * b + c * c + a * b + b * c + c * a)
+        mean = (a + b + c) / 3
+        Se = math.log10((i2 - A * mean * mean) / i2)
+        g = 0.5 * (1 + math.sin(math.pi * Se / (2 * kap)))
+        if Se < -kap:
+            g = 0.0
+        if Se > kap:
+            g = 1.0
+        wbar = W[e].reshape(4, 3).mean(1)
+        ref = _lambda(wbar, 1.4) * (3 * A / perim) / 2 * g
+        assert abs(eps[e] - ref) <= 1e-12 * max(abs(ref), 1e-300) + 1e-15
+    # 20 % vertex variation: Se inside the ramp of kap = 3; 5 %: Se < -1e-3, the lower clamp
+    if kap == 3.0:
+        assert (eps > 0).mean() > 0.5
+    else:
+        assert (eps == 0).all()
+
+
+def test_damp_const_is_a_weak_laplacian():
+    case = cases.square_mesh(4, 1.0, -2)
+    base = dict(model="TestScalarEquation", a=(0.0, 0.0), k=0.0, order=2, dt=1e8, penalty=0.0)
+    c = 0.37
+    o0 = O.Oracle(case, base)
+    o1 = O.Oracle(case, dict(base, dampConst=c))
+    grad = np.array([0.8, -1.3])
+    f = case.PXY @ grad + 0.25
+    # P1 nodal values = vertex values; basis2Dtriangle's barycentric nodes for k = 2 are
+    # (1,0,0), (0,0,1), (0,1,0): vertices 0, 2, 1
+    node = [0, 2, 1]
+    W = f[case.TP][:, node]
+    for o in (o0, o1):
+        o.set_w(W)
+        o.limiter()
+    D = o1.residual() - o0.residual()
+    ref = np.zeros_like(D)
+    TT = o0.TT()
+    for e in range(case.n_elem):
+        for k in range(3):
+            if TT[e, k] < 0:
+                a, b = k, (k + 1) % 3
+                pa, pb = case.PXY[case.TP[e, a]], case.PXY[case.TP[e, b]]
+                t = pb - pa
+                L = np.linalg.norm(t)
+                n = np.array([t[1], -t[0]]) / L   # outward for counter-clockwise triangles
+                ref[e, node[a]] -= c * (grad @ n) * L / 2
+                ref[e, node[b]] -= c * (grad @ n) * L / 2
+    assert np.abs(D - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("coef", [None, [0.2]])
+def test_inner_damping_memory(coef):
+    case = cases.channel_bump(8, 3)
+    props = cases.euler_props(order=3, dampInnerTol=2.0)
+    if coef is not None:
+        props["dampInnerCoef"] = coef
+    o = O.Oracle(case, props)
+    W0 = o.get_w()
+    Wp = cases.perturbed_state(W0[0][:: o.nb], case.n_elem, o.nb, 0.05, 9)
+    o.set_w(Wp)
+    o.limiter()
+    _, d1 = o.artificial_viscosity()
+    assert np.all(d1 == d1[:, :1])               # every equation takes the density sensor
+    assert (d1[:, 0] > 0).any()
+    o.set_w(W0)                                   # constant state: the sensor is 0
+    o.limiter()
+    _, d2 = o.artificial_viscosity()
+    if coef is not None:
+        np.testing.assert_allclose(d1, coef[0] * np.minimum(d1 / coef[0], 1.0), rtol=0, atol=1e-15)
+        np.testing.assert_allclose(d2, 0.1 * d1, rtol=1e-12, atol=1e-300)
+    else:
+        lam1 = np.array([_lambda(Wp[e].reshape(4, -1) @ _is(o, e), 1.4) for e in range(case.n_elem)])
+        lam2 = np.array([_lambda(W0[e].reshape(4, -1) @ _is(o, e), 1.4) for e in range(case.n_elem)])
+        np.testing.assert_allclose(d2[:, 0] / lam2, 0.1 * d1[:, 0] / lam1, rtol=1e-12, atol=1e-300)
+
+
+def _is(o, e):
+    """Is_j = int phi_j / area from the element mass matrix: the coefficients of the constant 1
+    are all ones, so M 1 / area holds the basis integrals."""
+    g = o.geometry(e)
+    return g["M"] @ np.ones(o.nb) / g["area"]
