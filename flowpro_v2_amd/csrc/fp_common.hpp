@@ -18,6 +18,8 @@ namespace fp {
 //   blkn [e*3+k]         BSR block index of (row nbr, column e), -1 on boundary faces
 //   blkd [e]             BSR block index of the diagonal block
 //   wdist[e*3+v]         wall distance of vertex v (generateWallDistanceFunction), SA only
+//   av   [e*(1+NEQ) + .] artificial viscosity eps and the equation-wise dampInner[m] of the
+//                        last limiter call (DGpure.limiter), null when the case has no damping
 // Reference-element tables (tab): basis and reference gradients at the volume
 // points, per local face at the face points, weights, point coordinates and the
 // reference mass matrix and its inverse.
@@ -45,6 +47,7 @@ struct Mesh {
     const int* blkn;
     const int* blkd;
     const double* wdist;   // [e*3 + v] wall distance of the element's vertices (Spalart-Allmaras)
+    const double* av;      // [e*(1+NEQ) + .] eps, dampInner[0..NEQ) (null without damping)
 };
 
 // Straight tetrahedra (transformation3Dtetra: X = J xi + X0), element data in DOF order:
@@ -78,7 +81,7 @@ struct TetMesh {
 
 // compile-time shape of the element kernels selected at run time
 struct ElemLaunch {
-    int model, nb, neq, nqv, nqf, visc;
+    int model, nb, neq, nqv, nqf, visc, av;
 };
 
 // vectors of one preconditioner application, read by the sweep kernels through a device
@@ -89,8 +92,46 @@ struct IluIO {
 };
 
 struct DGParams {
-    double beta0;   // DDG constant (0 when the global order is 1)
+    double beta0;   // DDG constant (0 when the model is diffusive and the global order is 1)
     double cIP;     // interior penalty (penalty when diffusive and order > 1, else 0)
+    double dampConst;   // constant artificial viscosity
+    int av;             // the damping terms are on (dampTol, dampConst or dampInnerTol set)
 };
+
+// DGpure.limiter parameters of the artificial viscosity (DGpure.initMethod)
+struct AvParams {
+    double dampTol, dampInnerTol;
+    int nCoef;          // dampInnerCoef entries (0: lambda * elemSize / order is used)
+    double coef[5];
+    int order;          // element order k (the sensor's S0 = log10(1 / (k-1)^4))
+};
+
+// Artificial viscosity of DGpure (FlowPro/src/main/java/flowpro/core/element/DGpure.java:
+// 176, 303-320), with eps and dampInner fixed by the last limiter call.  Interior faces:
+// g += epsFace (avg(grad W) . n + beta0 [W] / dL), epsFace = mean eps + dampConst + mean
+// dampInner[m]; volume: the flux loses (eps + dampConst + dampInner[m]) grad W_m.
+template <int NEQ, class T>
+__device__ inline void av_face_flux(const DGParams& G, const Mesh& ms, int e, int TT, const double* n, double dL, const T* WL,
+                                    const T* WR, const T* dWL, const T* dWR, T* g) {
+    const double* ae = ms.av + (size_t)e * (1 + NEQ);
+    const double* an = ms.av + (size_t)TT * (1 + NEQ);
+#pragma unroll
+    for (int m = 0; m < NEQ; m++) {
+        const double ef = 0.5 * (ae[0] + an[0]) + G.dampConst + 0.5 * (ae[1 + m] + an[1 + m]);
+        T dWs = (dWL[m] + dWR[m]) / 2.0 * n[0];
+        dWs += (dWL[NEQ + m] + dWR[NEQ + m]) / 2.0 * n[1];
+        g[m] += ef * (dWs + G.beta0 * (WR[m] - WL[m]) / dL);
+    }
+}
+template <int NEQ, class T>
+__device__ inline void av_volume_flux(const DGParams& G, const Mesh& ms, int e, const T* dWq, T* fx, T* fy) {
+    const double* ae = ms.av + (size_t)e * (1 + NEQ);
+#pragma unroll
+    for (int m = 0; m < NEQ; m++) {
+        const double c = ae[0] + G.dampConst + ae[1 + m];
+        fx[m] -= c * dWq[m];
+        fy[m] -= c * dWq[NEQ + m];
+    }
+}
 
 }  // namespace fp

@@ -125,6 +125,7 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
                     if (ip) g[m] -= G.cIP * (WL[m] - WR[m]);
                 }
             }
+            if (G.av && TT > -1) av_face_flux<NEQ>(G, ms, e, TT, n, dL, WL, WR, dWL, dWR, g);
             const double jw = jacF * tb.fw[p];
 #pragma unroll
             for (int m = 0; m < NEQ; m++)
@@ -170,6 +171,7 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
 #pragma unroll
                     for (int m = 0; m < NEQ; m++) { fx[m] -= gx[m]; fy[m] -= gy[m]; }
                 }
+                if (G.av) av_volume_flux<NEQ>(G, ms, e, dWq, fx, fy);
             } else {
 #pragma unroll
                 for (int m = 0; m < NEQ; m++) { fx[m] = 0.0; fy[m] = 0.0; }
@@ -477,6 +479,101 @@ __global__ void k_size_over_lambda(ModelParams P, Tables tb, Mesh ms, const doub
         for (int j = 0; j < NB; j++) wa[m] = wa[m] + W[(size_t)e * B + m * NB + j] * isref[j];
     double lam = max_eigenvalue<MK>(P, wa);
     out[e] = lam > 0 ? esize[e] / lam : __builtin_huge_val();
+}
+
+// ------------------------------------------------------------------ artificial viscosity
+// DGpure.limiter (FlowPro/src/main/java/flowpro/core/element/DGpure.java:358-400) with the
+// sensors shock_senzor / shock_senzor_all_eqn (:403-503): the L2 distance of an equation's
+// solution from its truncation (projection onto linear functions at order > 2, the mean
+// otherwise) relative to its L2 norm, on a log10 scale, through a sine ramp of half-width kap
+// around S0 = log10(1 / (k-1)^4).  The Aerodynamics models give every equation the density
+// sensor (Aerodynamics.combineShockSensors); the scalar model has one equation.  trunc is the
+// projection matrix in the nodal basis, the same for every straight triangle (the map is
+// affine, so linear functions and the L2 inner product up to |det J| are preserved).
+template <int NB>
+__device__ inline double shock_sensor(const Tables& tb, int nqv, const double* w, double mean, const double* trunc, int order,
+                                      double kap) {
+    double tc[NB];
+    if (order > 2) {
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < NB; j++) s += trunc[i * NB + j] * w[j];
+            tc[i] = s;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NB; i++) tc[i] = mean;
+    }
+    double Se = 0.0, pod = 0.0;
+    for (int q = 0; q < nqv; q++) {
+        double vi = 0.0, vt = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            vi += w[i] * tb.vb[q * NB + i];
+            vt += tc[i] * tb.vb[q * NB + i];
+        }
+        Se += tb.vw[q] * (vi - vt) * (vi - vt);
+        pod += tb.vw[q] * vi * vi;
+    }
+    Se = log10(fabs(Se / pod));
+    const double S0 = log10(1.0 / pow((double)(order - 1), 4.0));
+    double s = 0.5 * (1.0 + sin(M_PI * (Se - S0) / (2.0 * kap)));
+    if (Se < S0 - kap) s = 0.0;
+    if (Se > S0 + kap) s = 1.0;
+    return isnan(s) ? 0.0 : s;
+}
+
+template <int MK, int NB, int NEQ>
+__global__ void k_av_limiter(ModelParams P, Tables tb, int nqv, Mesh ms, AvParams A, const double* __restrict__ W,
+                             const double* __restrict__ esize, const double* __restrict__ isref, const double* __restrict__ trunc,
+                             double* __restrict__ av, double* __restrict__ old) {
+    constexpr int B = NB * NEQ;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ms.nElem) return;
+    double* a = av + (size_t)e * (1 + NEQ);
+    a[0] = 0.0;
+    if (A.order <= 1) return;
+    double w[B], mean[NEQ], wa[NEQ];
+#pragma unroll
+    for (int i = 0; i < B; i++) w[i] = W[(size_t)e * B + i];
+#pragma unroll
+    for (int m = 0; m < NEQ; m++) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) s = s + w[m * NB + j] * isref[j];
+        mean[m] = s;
+        wa[m] = s;
+    }
+    const double lam = max_eigenvalue<MK>(P, wa);
+    const double scale = lam * esize[e] / A.order;
+    if (A.dampTol > 0) a[0] = scale * shock_sensor<NB>(tb, nqv, w, mean[0], trunc, A.order, A.dampTol);
+    if (A.dampInnerTol > 0) {
+        const double g0 = shock_sensor<NB>(tb, nqv, w, mean[0], trunc, A.order, A.dampInnerTol);
+#pragma unroll
+        for (int m = 0; m < NEQ; m++) {
+            const double g = m == 0 || MK != MODEL_SCALAR ? g0 : shock_sensor<NB>(tb, nqv, w + m * NB, mean[m], trunc, A.order, A.dampInnerTol);
+            double& o = old[(size_t)e * NEQ + m];
+            const double ind = fmax(g, o);
+            o = 0.1 * ind;
+            a[1 + m] = A.nCoef > 0 ? (A.nCoef == 1 ? A.coef[0] : A.coef[m]) * ind : scale * ind;
+        }
+    }
+}
+
+hipError_t launch_av_limiter(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const Tables& tb, const Mesh& ms,
+                             const AvParams& A, const double* W, const double* esize, const double* isref, const double* trunc,
+                             double* av, double* old) {
+    dim3 blk(128), grd((ms.nElem + 127) / 128);
+#define X(MK, NB, NEQ, NQV, NQF, V)                                                                                   \
+    if (FP_MATCH(MK, NB, NEQ, NQV, NQF, V)) {                                                                          \
+        hipLaunchKernelGGL((k_av_limiter<MK, NB, NEQ>), grd, blk, 0, s, P, tb, NQV, ms, A, W, esize, isref, trunc, av, old); \
+        return hipGetLastError();                                                                                      \
+    }
+    FP_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 // ------------------------------------------------------------------ positivity limiter

@@ -21,6 +21,9 @@
 namespace fp {
 // launchers (fp_element.hip, fp_linalg.hip)
 bool element_supported(const ElemLaunch& L);
+bool team_supported(const ElemLaunch& L);
+hipError_t launch_av_limiter(const ElemLaunch&, hipStream_t, const ModelParams&, const Tables&, const Mesh&, const AvParams&,
+                             const double*, const double*, const double*, const double*, double*, double*);
 hipError_t launch_residual(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
                            const double*, const double*, const double*, double, double, double, double*, double*, double*);
 template <typename BT>
@@ -84,6 +87,8 @@ struct fp_ctx {
     ElemLaunch L{};
     ModelParams mp{};
     DGParams dg{};
+    AvParams avp{};
+    double *av = nullptr, *avold = nullptr, *trunc = nullptr;   // damping state, sensor memory, truncation
     int precond = 2, bits = 64, m = 30, cycles = 5, positivity = 1;
     double tol = 1e-2;
     // host copies
@@ -483,11 +488,55 @@ template <typename T> hipError_t upload(T** p, const std::vector<T>& h) {
 }  // namespace
 
 // ============================================================== create / destroy
+// TrunOrd = M^-1 T^T Mhat^-1 T with the Taylor basis {1, xi - 1/3, eta - 1/3} on the reference
+// triangle, from the reference tables (volume basis values, weights, points) held in c->tab
+static bool truncation_matrix(fp_ctx* c, std::vector<double>& tr) {
+    const int NB = c->NB, NQV = c->NQV;
+    std::vector<double> h((size_t)NQV * NB + NQV + 2 * (size_t)NQV);
+    if (hipMemcpy(h.data(), c->tb.vb, sizeof(double) * NQV * NB, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(h.data() + NQV * NB, c->tb.vw, sizeof(double) * NQV, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(h.data() + NQV * NB + NQV, c->tb.vx, sizeof(double) * 2 * NQV, hipMemcpyDeviceToHost) != hipSuccess) {
+        c->err = "reading the reference tables back failed";
+        return false;
+    }
+    const double* vb = h.data();
+    const double* vw = vb + NQV * NB;
+    const double* vx = vw + NQV;
+    auto taylor = [&](int a, int q) { return a == 0 ? 1.0 : vx[2 * q + a - 1] - 1.0 / 3; };
+    std::vector<double> Mh(9, 0.0), T((size_t)3 * NB, 0.0), M((size_t)NB * NB, 0.0), Mhi, Mi;
+    for (int q = 0; q < NQV; q++) {
+        for (int a = 0; a < 3; a++) {
+            for (int b = 0; b < 3; b++) Mh[a * 3 + b] += vw[q] * taylor(a, q) * taylor(b, q);
+            for (int b = 0; b < NB; b++) T[a * NB + b] += vw[q] * taylor(a, q) * vb[q * NB + b];
+        }
+        for (int a = 0; a < NB; a++)
+            for (int b = 0; b < NB; b++) M[a * NB + b] += vw[q] * vb[q * NB + a] * vb[q * NB + b];
+    }
+    if (!invert(Mh, 3, Mhi) || !invert(M, NB, Mi)) {
+        c->err = "singular matrix in the truncation operator";
+        return false;
+    }
+    std::vector<double> X((size_t)3 * NB, 0.0), Y((size_t)NB * NB, 0.0);   // X = Mhat^-1 T, Y = T^T X
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < NB; b++)
+            for (int k = 0; k < 3; k++) X[a * NB + b] += Mhi[a * 3 + k] * T[k * NB + b];
+    for (int a = 0; a < NB; a++)
+        for (int b = 0; b < NB; b++)
+            for (int k = 0; k < 3; k++) Y[a * NB + b] += T[k * NB + a] * X[k * NB + b];
+    for (int a = 0; a < NB; a++)
+        for (int b = 0; b < NB; b++) {
+            double s = 0;
+            for (int k = 0; k < NB; k++) s += Mi[a * NB + k] * Y[k * NB + b];
+            tr[(size_t)a * NB + b] = s;
+        }
+    return true;
+}
+
 static void free_all(fp_ctx* c) {
     if (c->g_factor) (void)hipGraphExecDestroy(c->g_factor);
     if (c->g_apply) (void)hipGraphExecDestroy(c->g_apply);
     if (c->io) (void)hipFree(c->io);
-    void* ptrs[] = {c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
+    void* ptrs[] = {c->av, c->avold, c->trunc, c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
                     c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->val, c->F, c->iy, c->ixs, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
                     c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->istat};
     for (void* p : ptrs)
@@ -825,8 +874,18 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     c->dofs = (long long)n * c->B;
     // SpalartAllmaras.init passes isDiffusive = true to Aerodynamics.init (SpalartAllmaras.java:38)
     const bool diffusive = cs->model == MODEL_SCALAR ? cs->scalar_k > 0 : (cs->model == MODEL_SA || cs->viscous != 0);
-    c->L = ElemLaunch{cs->model, c->NB, c->NEQ, c->NQV, c->NQF, diffusive ? 1 : 0};
-    if (!(c->dim == 3 ? tet_supported(c->L) : element_supported(c->L))) {
+    const bool av = cs->damp_tol > 0 || cs->damp_const != 0 || cs->damp_inner_tol > 0;
+    if (av && (c->dim == 3 || cs->model == MODEL_SA)) {
+        c->err = "artificial viscosity: device kernels for the 2D NavierStokes and scalar models only";
+        return FP_UNSUPPORTED;
+    }
+    if (cs->n_damp_inner_coef < 0 || cs->n_damp_inner_coef > 5 ||
+        (cs->n_damp_inner_coef > 1 && cs->n_damp_inner_coef < c->NEQ)) {
+        c->err = "dampInnerCoef needs one value or one per equation";
+        return FP_EINVAL;
+    }
+    c->L = ElemLaunch{cs->model, c->NB, c->NEQ, c->NQV, c->NQF, diffusive ? 1 : 0, av ? 1 : 0};
+    if (!(c->dim == 3 ? tet_supported(c->L) : element_supported(c->L) && team_supported(c->L))) {
         c->err = "no element kernel for this model / order / quadrature";
         return FP_UNSUPPORTED;
     }
@@ -865,7 +924,16 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     P.cw3 = cs->sa_cw3;
     P.cv1 = cs->sa_cv1;
     P.Prt = cs->sa_prt;
-    c->dg.beta0 = cs->order == 1 ? 0.0 : cs->beta0;                  // DGpure.java:287-289
+    // DGpure.java:287-289 zeroes beta0 inside the isDiffusive branch; the damping flux of an
+    // inviscid model keeps it
+    c->dg.beta0 = (cs->order == 1 && diffusive) ? 0.0 : cs->beta0;
+    c->dg.dampConst = cs->damp_const;
+    c->dg.av = av ? 1 : 0;
+    c->avp.dampTol = cs->damp_tol;
+    c->avp.dampInnerTol = cs->damp_inner_tol;
+    c->avp.nCoef = cs->n_damp_inner_coef;
+    for (int i = 0; i < cs->n_damp_inner_coef; i++) c->avp.coef[i] = cs->damp_inner_coef[i];
+    c->avp.order = cs->order;
     c->dg.cIP = (cs->order > 1 && diffusive) ? cs->penalty : 0.0;   // DGpure.limiter :396-398
 
     HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
@@ -988,7 +1056,18 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     HIPCHK(c, upload(&c->meta_f, meta_f));
     HIPCHK(c, upload(&c->meta_b, meta_b));
     HIPCHK(c, upload(&c->ftask, ftask));
-    c->ms = Mesh{n, c->geo, c->fgeo, c->nbr, c->nbrf, c->blkn, c->blkd, c->wdist};
+    if (c->dg.av) {
+        HIPCHK(c, dalloc(&c->av, (size_t)n * (1 + c->NEQ)));
+        HIPCHK(c, dalloc(&c->avold, (size_t)n * c->NEQ));
+        HIPCHK(c, hipMemsetAsync(c->av, 0, sizeof(double) * (size_t)n * (1 + c->NEQ), c->s));
+        HIPCHK(c, hipMemsetAsync(c->avold, 0, sizeof(double) * (size_t)n * c->NEQ, c->s));
+        // DGpure.computeOrderTruncationMatrix on the reference triangle: for an affine map the
+        // projection onto linear functions in the nodal basis does not depend on the element
+        std::vector<double> tr((size_t)c->NB * c->NB, 0.0);
+        if (c->order > 2 && !truncation_matrix(c, tr)) return FP_EINVAL;
+        HIPCHK(c, upload(&c->trunc, tr));
+    }
+    c->ms = Mesh{n, c->geo, c->fgeo, c->nbr, c->nbrf, c->blkn, c->blkd, c->wdist, c->av};
     c->tms = TetMesh{n, c->geo, c->fgeo, c->nbr, c->blkn, c->blkd};
 
     // --- state, matrix, Krylov workspace
@@ -1522,6 +1601,31 @@ extern "C" int fp_export_jacobian(fp_ctx* c, double* diag, double* neigh) {
                 if (blk >= 0) std::memcpy(dst, &h[(size_t)blk * BB], BB * sizeof(double));
                 else std::memset(dst, 0, BB * sizeof(double));
             }
+    }
+    return FP_OK;
+}
+
+extern "C" int fp_limiter(fp_ctx* c, int first_step) {
+    (void)first_step;   // the first-iteration branch of DGpure.limiter has an empty body
+    if (!c) return FP_EINVAL;
+    if (!c->dg.av || c->dim != 2) return FP_OK;
+    HIPCHK(c, launch_av_limiter(c->L, c->s, c->mp, c->tb, c->ms, c->avp, c->W, c->esize, c->isref, c->trunc, c->av, c->avold));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return FP_OK;
+}
+
+extern "C" int fp_export_artificial_viscosity(fp_ctx* c, double* eps, double* damp_inner) {
+    if (!c || !eps || !damp_inner) return FP_EINVAL;
+    const int ne = c->NEQ;
+    std::vector<double> h((size_t)c->nElem * (1 + ne), 0.0);
+    if (c->av) {
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->av, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+    }
+    for (int r = 0; r < c->nElem; r++) {
+        const size_t e = c->order_[r];
+        eps[e] = h[(size_t)r * (1 + ne)];
+        for (int m = 0; m < ne; m++) damp_inner[e * ne + m] = h[(size_t)r * (1 + ne) + 1 + m];
     }
     return FP_OK;
 }

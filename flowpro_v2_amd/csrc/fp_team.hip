@@ -169,9 +169,9 @@ __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double
 
 // pointwise face integrand g (the reference's K update per face point divided by the
 // test function): -fn + fvn - cIP [W] for the given own-side state
-template <int MK, int NEQ, bool VISC, class T>
-__device__ inline void face_integrand(const ModelParams& P, const DGParams& G, int TT, const double* n, double dL, const double* X,
-                                      T* WL, T* dWL, const double* st, T* g) {
+template <int MK, int NEQ, bool VISC, bool AV, class T>
+__device__ inline void face_integrand(const ModelParams& P, const DGParams& G, const Mesh& ms, int e, int TT, const double* n, double dL,
+                                      const double* X, T* WL, T* dWL, const double* st, T* g) {
     T WR[NEQ], dWR[2 * NEQ];
     if (TT > -1) {
 #pragma unroll
@@ -202,6 +202,9 @@ __device__ inline void face_integrand(const ModelParams& P, const DGParams& G, i
             if (ip) g[m] -= G.cIP * (WL[m] - WR[m]);
         }
     }
+    if constexpr (AV) {
+        if (TT > -1) av_face_flux<NEQ>(G, ms, e, TT, n, dL, WL, WR, dWL, dWR, g);
+    }
 }
 
 // number of volume integrand slots: fluxes in x and y, plus the state-dependent source of
@@ -211,8 +214,9 @@ template <int MK> constexpr int volume_slots() { return MK == MODEL_SA ? 3 : 2; 
 // pointwise volume fluxes (convective minus viscous) in x and y and the model source, called
 // in the order of DGpure.residuum (convection x, y, diffusion x, y, source): each call clips
 // the density of the shared state as the reference does.  D: wall distance at the point.
-template <int MK, int NB, int NEQ, bool VISC, class T>
-__device__ inline void volume_integrand(const ModelParams& P, T* Wq, T* dWq, double D, T* fx, T* fy, T* sv) {
+template <int MK, int NB, int NEQ, bool VISC, bool AV, class T>
+__device__ inline void volume_integrand(const ModelParams& P, const DGParams& G, const Mesh& ms, int e, T* Wq, T* dWq, double D, T* fx,
+                                        T* fy, T* sv) {
     if constexpr (NB > 1) {
         const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
         convective_flux<MK>(P, Wq, ex, fx);
@@ -224,6 +228,7 @@ __device__ inline void volume_integrand(const ModelParams& P, T* Wq, T* dWq, dou
 #pragma unroll
             for (int m = 0; m < NEQ; m++) { fx[m] -= gx[m]; fy[m] -= gy[m]; }
         }
+        if constexpr (AV) av_volume_flux<NEQ>(G, ms, e, dWq, fx, fy);
     } else {
 #pragma unroll
         for (int m = 0; m < NEQ; m++) { fx[m] = 0.0; fy[m] = 0.0; }
@@ -248,7 +253,7 @@ struct TeamEPB {
 };
 
 // ------------------------------------------------------------------ residual
-template <int MK, int NB, int NEQ, int NQV, int NQF, bool VISC>
+template <int MK, int NB, int NEQ, int NQV, int NQF, bool VISC, bool AV>
 __global__ void __launch_bounds__(256) k_residual_team(ModelParams P, DGParams G, Tables tb, Mesh ms, const double* __restrict__ W,
                                                        const double* __restrict__ Wo, const double* __restrict__ Wo2, double a1,
                                                        double a2, double a3, double* __restrict__ R, double* __restrict__ RHS,
@@ -278,7 +283,7 @@ __global__ void __launch_bounds__(256) k_residual_team(ModelParams P, DGParams G
 #pragma unroll
             for (int m = 0; m < NEQ; m++) { Wq[m] = st[m]; dWq[m] = st[NEQ + m]; dWq[NEQ + m] = st[2 * NEQ + m]; }
             const double D = MK == MODEL_SA ? wall_distance_at(ms, e, &tab[LY::T_VX + 2 * qp]) : 0.0;
-            volume_integrand<MK, NB, NEQ, VISC>(P, Wq, dWq, D, fx, fy, sv);
+            volume_integrand<MK, NB, NEQ, VISC, AV>(P, G, ms, e, Wq, dWq, D, fx, fy, sv);
             const double jw = ge[4] * tab[LY::T_VW + qp];
 #pragma unroll
             for (int m = 0; m < NEQ; m++) {
@@ -297,7 +302,7 @@ __global__ void __launch_bounds__(256) k_residual_team(ModelParams P, DGParams G
             for (int m = 0; m < NEQ; m++) { WL[m] = st[m]; dWL[m] = st[NEQ + m]; dWL[NEQ + m] = st[2 * NEQ + m]; }
             const double* xi = &tab[LY::T_FX + 2 * f];
             double X[2] = {ge[5] * xi[0] + ge[6] * xi[1] + ge[7], ge[8] * xi[0] + ge[9] * xi[1] + ge[10]};
-            face_integrand<MK, NEQ, VISC>(P, G, TT, n, fg[3], X, WL, dWL, st, g);
+            face_integrand<MK, NEQ, VISC, AV>(P, G, ms, e, TT, n, fg[3], X, WL, dWL, st, g);
             const double jw = fg[2] * tab[LY::T_FW + p];
 #pragma unroll
             for (int m = 0; m < NEQ; m++) E[O_IF + f * NEQ + m] = jw * g[m];
@@ -361,12 +366,12 @@ __global__ void __launch_bounds__(256) k_residual_team(ModelParams P, DGParams G
 // every equation.  Phase 1b stores d(J w * integrand)/d(input) for every point and input;
 // phase 2 thread (element, column i = (m', j')) builds column i of the diagonal block and
 // of the three neighbour blocks.
-template <int MK, int NB, int NEQ, int NQV, int NQF, bool VISC, typename BT>
+template <int MK, int NB, int NEQ, int NQV, int NQF, bool VISC, bool AV, typename BT>
 __global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G, Tables tb, Mesh ms, const double* __restrict__ W,
                                                        double a1, BT* __restrict__ blocks) {
     using LY = TeamLayout<NB, NEQ, NQV, NQF>;
     constexpr int B = LY::B;
-    constexpr int NIN = VISC ? 3 * NEQ : NEQ;
+    constexpr int NIN = (VISC || AV) ? 3 * NEQ : NEQ;   // the damping terms see the gradients too
     constexpr int NVS = volume_slots<MK>();
     constexpr int EXTRA = NQV * NIN * NVS * NEQ + LY::NQFT * NIN * NEQ;
     constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EXTRA>::value;
@@ -395,7 +400,7 @@ __global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G
                 dWq[NEQ + m] = dual(st[2 * NEQ + m], (s == 2 && m == mp) ? 1.0 : 0.0);
             }
             const double D = MK == MODEL_SA ? wall_distance_at(ms, e, &tab[LY::T_VX + 2 * qp]) : 0.0;
-            volume_integrand<MK, NB, NEQ, VISC>(P, Wq, dWq, D, fx, fy, sv);
+            volume_integrand<MK, NB, NEQ, VISC, AV>(P, G, ms, e, Wq, dWq, D, fx, fy, sv);
             const double jw = ge[4] * tab[LY::T_VW + qp];
             double* out = &E[O_DV + ((qp * NIN + c) * NVS) * NEQ];
 #pragma unroll
@@ -419,7 +424,7 @@ __global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G
             }
             const double* xi = &tab[LY::T_FX + 2 * f];
             double X[2] = {ge[5] * xi[0] + ge[6] * xi[1] + ge[7], ge[8] * xi[0] + ge[9] * xi[1] + ge[10]};
-            face_integrand<MK, NEQ, VISC>(P, G, TT, n, fg[3], X, WL, dWL, st, g);
+            face_integrand<MK, NEQ, VISC, AV>(P, G, ms, e, TT, n, fg[3], X, WL, dWL, st, g);
             const double jw = fg[2] * tab[LY::T_FW + p];
             double* out = &E[O_DF + (f * NIN + c) * NEQ];
 #pragma unroll
@@ -448,7 +453,7 @@ __global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G
                 double t[NEQ];
 #pragma unroll
                 for (int m = 0; m < NEQ; m++) t[m] = E[O_DF + (f * NIN + mp) * NEQ + m] * psi0;
-                if constexpr (VISC) {
+                if constexpr (VISC || AV) {
                     const double psx = E[LY::E_GF + (f * NB + jp) * 2], psy = E[LY::E_GF + (f * NB + jp) * 2 + 1];
 #pragma unroll
                     for (int m = 0; m < NEQ; m++)
@@ -483,7 +488,7 @@ __global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G
                     tx[m] = d0[m] * psi0;
                     ty[m] = d0[NEQ + m] * psi0;
                     if constexpr (NVS == 3) ts[m] = d0[2 * NEQ + m] * psi0;
-                    if constexpr (VISC) {
+                    if constexpr (VISC || AV) {
                         const double* d1 = &E[O_DV + ((qv * NIN + NEQ + mp) * NVS) * NEQ];
                         const double* d2 = &E[O_DV + ((qv * NIN + 2 * NEQ + mp) * NVS) * NEQ];
                         tx[m] += d1[m] * psx + d2[m] * psy;
@@ -514,35 +519,53 @@ __global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G
 }
 
 // ------------------------------------------------------------------ dispatch
-#define FP_TEAM_CASES(X)                          \
-    X(MODEL_NAVIER_STOKES, 3, 4, 6, 2, false)     \
-    X(MODEL_NAVIER_STOKES, 6, 4, 7, 3, false)     \
-    X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, false)   \
-    X(MODEL_NAVIER_STOKES, 3, 4, 6, 2, true)      \
-    X(MODEL_NAVIER_STOKES, 6, 4, 7, 3, true)      \
-    X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, true)    \
-    X(MODEL_SA, 3, 5, 6, 2, true)                 \
-    X(MODEL_SA, 6, 5, 7, 3, true)                 \
-    X(MODEL_SCALAR, 1, 1, 1, 1, false)            \
-    X(MODEL_SCALAR, 3, 1, 6, 2, false)            \
-    X(MODEL_SCALAR, 6, 1, 7, 3, false)            \
-    X(MODEL_SCALAR, 10, 1, 16, 4, false)          \
-    X(MODEL_SCALAR, 3, 1, 6, 2, true)             \
-    X(MODEL_SCALAR, 6, 1, 7, 3, true)             \
-    X(MODEL_SCALAR, 10, 1, 16, 4, true)
+// (model, Nb, nEqs, volume points, face points, viscous, artificial viscosity)
+#define FP_TEAM_CASES(X)                                 \
+    X(MODEL_NAVIER_STOKES, 3, 4, 6, 2, false, false)     \
+    X(MODEL_NAVIER_STOKES, 6, 4, 7, 3, false, false)     \
+    X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, false, false)   \
+    X(MODEL_NAVIER_STOKES, 3, 4, 6, 2, true, false)      \
+    X(MODEL_NAVIER_STOKES, 6, 4, 7, 3, true, false)      \
+    X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, true, false)    \
+    X(MODEL_NAVIER_STOKES, 3, 4, 6, 2, false, true)      \
+    X(MODEL_NAVIER_STOKES, 6, 4, 7, 3, false, true)      \
+    X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, false, true)    \
+    X(MODEL_NAVIER_STOKES, 3, 4, 6, 2, true, true)       \
+    X(MODEL_NAVIER_STOKES, 6, 4, 7, 3, true, true)       \
+    X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, true, true)     \
+    X(MODEL_SA, 3, 5, 6, 2, true, false)                 \
+    X(MODEL_SA, 6, 5, 7, 3, true, false)                 \
+    X(MODEL_SCALAR, 1, 1, 1, 1, false, false)            \
+    X(MODEL_SCALAR, 3, 1, 6, 2, false, false)            \
+    X(MODEL_SCALAR, 6, 1, 7, 3, false, false)            \
+    X(MODEL_SCALAR, 10, 1, 16, 4, false, false)          \
+    X(MODEL_SCALAR, 3, 1, 6, 2, true, false)             \
+    X(MODEL_SCALAR, 6, 1, 7, 3, true, false)             \
+    X(MODEL_SCALAR, 10, 1, 16, 4, true, false)           \
+    X(MODEL_SCALAR, 3, 1, 6, 2, false, true)             \
+    X(MODEL_SCALAR, 6, 1, 7, 3, false, true)
 
-#define FP_TMATCH(MK, NB, NEQ, NQV, NQF, V) \
-    (L.model == MK && L.nb == NB && L.neq == NEQ && L.nqv == NQV && L.nqf == NQF && (bool)L.visc == V)
+#define FP_TMATCH(MK, NB, NEQ, NQV, NQF, V, A)                                                                    \
+    (L.model == MK && L.nb == NB && L.neq == NEQ && L.nqv == NQV && L.nqf == NQF && (bool)L.visc == V && \
+     (bool)L.av == A)
+
+bool team_supported(const ElemLaunch& L) {
+#define X(MK, NB, NEQ, NQV, NQF, V, A) \
+    if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V, A)) return true;
+    FP_TEAM_CASES(X)
+#undef X
+    return false;
+}
 
 hipError_t launch_residual_team(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const Tables& tb,
                                 const Mesh& ms, const double* W, const double* Wo, const double* Wo2, double a1, double a2,
                                 double a3, double* R, double* RHS, double* enorm) {
-#define X(MK, NB, NEQ, NQV, NQF, V)                                                                                   \
-    if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V)) {                                                                         \
+#define X(MK, NB, NEQ, NQV, NQF, V, A)                                                                                \
+    if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V, A)) {                                                                      \
         using LY = TeamLayout<NB, NEQ, NQV, NQF>;                                                                      \
         constexpr int EX = NQV * volume_slots<MK>() * NEQ + LY::NQFT * NEQ + 2 * LY::B;                                \
         constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EX>::value;                                                     \
-        hipLaunchKernelGGL((k_residual_team<MK, NB, NEQ, NQV, NQF, V>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), 0, s, P, \
+        hipLaunchKernelGGL((k_residual_team<MK, NB, NEQ, NQV, NQF, V, A>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), 0, s, P, \
                            G, tb, ms, W, Wo, Wo2, a1, a2, a3, R, RHS, enorm);                                          \
         return hipGetLastError();                                                                                      \
     }
@@ -553,13 +576,13 @@ hipError_t launch_residual_team(const ElemLaunch& L, hipStream_t s, const ModelP
 
 hipError_t launch_jacobian_team(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const Tables& tb,
                                 const Mesh& ms, const double* W, double a1, double* blocks) {
-#define X(MK, NB, NEQ, NQV, NQF, V)                                                                                   \
-    if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V)) {                                                                         \
+#define X(MK, NB, NEQ, NQV, NQF, V, A)                                                                                \
+    if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V, A)) {                                                                      \
         using LY = TeamLayout<NB, NEQ, NQV, NQF>;                                                                      \
-        constexpr int NIN = V ? 3 * NEQ : NEQ;                                                                         \
+        constexpr int NIN = (V || A) ? 3 * NEQ : NEQ;                                                                  \
         constexpr int EX = NQV * NIN * volume_slots<MK>() * NEQ + LY::NQFT * NIN * NEQ;                                \
         constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EX>::value;                                                     \
-        hipLaunchKernelGGL((k_jacobian_team<MK, NB, NEQ, NQV, NQF, V, double>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), \
+        hipLaunchKernelGGL((k_jacobian_team<MK, NB, NEQ, NQV, NQF, V, A, double>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), \
                            0, s, P, G, tb, ms, W, a1, blocks);                                                         \
         return hipGetLastError();                                                                                      \
     }

@@ -23,7 +23,8 @@ EXPORTS = ["fp_neighbor_matrix", "fp_rcm_order", "fp_create", "fp_destroy", "fp_
            "fp_element_order", "fp_set_state", "fp_get_state", "fp_next_level", "fp_restore_level", "fp_time_step",
            "fp_assemble", "fp_linear_solve", "fp_update", "fp_export_residual", "fp_export_jacobian",
            "fp_export_solution", "fp_jv", "fp_jtv", "fp_time_kernel", "fp_stream", "fp_linear_solve_jfnk",
-           "fp_functional", "fp_adjoint_solve", "fp_wall_distance", "fp_neighbor_matrix_tet", "fp_rcm_order_tet"]
+           "fp_functional", "fp_adjoint_solve", "fp_wall_distance", "fp_neighbor_matrix_tet", "fp_rcm_order_tet",
+           "fp_limiter", "fp_export_artificial_viscosity"]
 
 
 class FpCase(C.Structure):
@@ -47,6 +48,8 @@ class FpCase(C.Structure):
         ("sa_ka", C.c_double), ("sa_cw2", C.c_double), ("sa_cw3", C.c_double), ("sa_cv1", C.c_double),
         ("sa_prt", C.c_double),
         ("dimension", C.c_int),
+        ("damp_tol", C.c_double), ("damp_const", C.c_double), ("damp_inner_tol", C.c_double),
+        ("n_damp_inner_coef", C.c_int), ("damp_inner_coef", C.c_double * 5),
     ]
 
 
@@ -91,6 +94,8 @@ def lib() -> C.CDLL:
         "fp_wall_distance": (C.c_int, [C.c_int, dp, C.c_int, ip, ip, dp]),
         "fp_neighbor_matrix_tet": (C.c_int, [C.c_int, ip, C.c_int, ip, ip]),
         "fp_rcm_order_tet": (C.c_int, [C.c_int, ip, ip]),
+        "fp_limiter": (C.c_int, [vp, C.c_int]),
+        "fp_export_artificial_viscosity": (C.c_int, [vp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

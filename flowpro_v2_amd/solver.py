@@ -35,8 +35,6 @@ class DeviceCase:
             raise UnsupportedCase("spatial method")
         if props.get("movingMesh", False) or props.get("curvedBoundary", False):
             raise UnsupportedCase("moving or curved mesh")
-        if float(props.get("dampTol", 0.0)) > 0 or float(props.get("dampConst", 0.0)) != 0:
-            raise UnsupportedCase("artificial viscosity (dampTol, dampConst)")
         prec = props.get("preconditioner", "blockilu0")
         if prec not in PRECONDITIONERS:
             raise UnsupportedCase(f"preconditioner {prec}")
@@ -86,6 +84,17 @@ class DeviceCase:
         cs.gmres_tol = float(props.get("iterativeSolverTol", 1e-2))
         cs.positivity_limiter = int(bool(props.get("positivityLimiter", True)))
         cs.dimension = self.dim
+        # DGpure.initMethod: dampTol and dampConst are required keys of the DG methods, the inner
+        # damping keys optional
+        cs.damp_tol = float(props.get("dampTol", 0.0))
+        cs.damp_const = float(props.get("dampConst", 0.0))
+        cs.damp_inner_tol = float(props.get("dampInnerTol", 0.0))
+        coef = props.get("dampInnerCoef")
+        if coef is not None:
+            coef = [float(v) for v in (coef if hasattr(coef, "__len__") else [coef])]
+            cs.n_damp_inner_coef = len(coef)
+            for i, v in enumerate(coef):
+                cs.damp_inner_coef[i] = v
         if mc.sa is not None:
             if wd is None:
                 raise UnsupportedCase("SpalartAllmaras needs the vertex wall distance (Case.wall_distance)")
@@ -152,6 +161,18 @@ class DeviceCase:
         self._check(native.lib().fp_assemble(self.h, dt, dto, time_order, int(with_jacobian), native.dptr(norms),
                                              C.byref(rhs)), "fp_assemble")
         return norms, rhs.value
+
+    def limiter(self, first_step: bool = False):
+        """DGpure.limiter for all elements: the artificial viscosity of the current state."""
+        self._check(native.lib().fp_limiter(self.h, int(first_step)), "fp_limiter")
+
+    def artificial_viscosity(self):
+        """(eps[n_elem], dampInner[n_elem, nEqs]) of the last limiter call, mesh order."""
+        eps = np.zeros(self.n_elem)
+        di = np.zeros((self.n_elem, self.neq))
+        self._check(native.lib().fp_export_artificial_viscosity(self.h, native.dptr(eps), native.dptr(di)),
+                    "fp_export_artificial_viscosity")
+        return eps, di
 
     def linear_solve(self):
         it, err = C.c_int(), C.c_double()
@@ -354,6 +375,7 @@ class GpuImplicitSolver:
             if st.convergence_hopeless():
                 break
             dt = st.time_step(dev.size_over_lambda())
+            dev.limiter(first_step=not self.history)   # LocalImplicitSolver.java:170-172
             rhs0 = 0.0
             iters = []
             limited = 0

@@ -80,6 +80,12 @@ typedef struct fp_case {
     /* Parameters.dimension: 2 (or 0) triangles, 3 tetrahedra.  Tetrahedra carry the inviscid
      * model (viscous = 0) at orders 1 and 2, the orders basis3Dtetra can build */
     int dimension;
+    /* artificial viscosity (DGpure.initMethod, DGpure.java:40-71): dampTol (shock-sensor tolerance
+     * of eps), dampConst, dampInnerTol and dampInnerCoef (n_damp_inner_coef = 0: the key is absent;
+     * 1: one value for every equation; nEqs: one per equation).  Triangles only */
+    double damp_tol, damp_const, damp_inner_tol;
+    int n_damp_inner_coef;
+    double damp_inner_coef[5];
 } fp_case;
 
 /* --- integer setup (bit-exact with the reference) ------------------------------ */
@@ -120,6 +126,10 @@ int fp_time_step(fp_ctx* ctx, double* min_size_over_lambda);
  * eq_norms[nEqs] = MasterSolver.residuumNorms, rhs_norm = LocalImplicitSolver.rhsNorm. */
 int fp_assemble(fp_ctx* ctx, double dt, double dto, int time_order, int with_jacobian, double* eq_norms,
                 double* rhs_norm);
+/* DGpure.limiter for all elements (LocalImplicitSolver.java:170-172, DGpure.java:358-400): the
+ * artificial viscosity eps and dampInner of the current state, once per time step before the
+ * Newton iterations; a no-op without damping.  first_step mirrors isFirstIter. */
+int fp_limiter(fp_ctx* ctx, int first_step);
 /* LinearSolver.solve: factor the preconditioner, GMRES from x = 0 */
 int fp_linear_solve(fp_ctx* ctx, int* iterations, double* rel_residual);
 /* the same Newton system solved Jacobian-free: GMRES on the matrix-free product
@@ -134,6 +144,8 @@ int fp_export_residual(fp_ctx* ctx, double* R, double* RHS);
  * (k over the element's faces: 3 on triangles, 4 on tetrahedra)
  * (zero for boundary faces) */
 int fp_export_jacobian(fp_ctx* ctx, double* diag, double* neigh);
+/* eps[n_elems] and damp_inner[n_elems][nEqs] of the last fp_limiter call, mesh order */
+int fp_export_artificial_viscosity(fp_ctx* ctx, double* eps, double* damp_inner);
 /* the last Newton correction x in DOF numbering */
 int fp_export_solution(fp_ctx* ctx, double* x);
 /* matrix-free (a1 M - dR/dW) v and its transpose, element layout in and out */

@@ -1,0 +1,118 @@
+"""Parity of the artificial viscosity on the device (fp_limiter, the damping terms in the
+residual, the exact Jacobian and the matrix-free products) with the oracle.
+
+Tolerances:
+  eps, dampInner after one and after two limiter calls   1e-12 (relative to the largest value)
+  residual                                               1e-12 (equation-norm relative)
+  Jacobian blocks                                        1e-10
+  J v, J^T v                                             1e-10
+  Newton residual history with the limiter every step    1e-8
+The sensors pass through log10 and sin, evaluated by glibc in the oracle and by the device
+math library here; both are within an ulp, far inside these tolerances.
+"""
+import numpy as np
+import pytest
+
+from flowpro_v2_amd import cases, solver
+import oracle as O
+from helpers import block_rel_err, eq_rel_err
+
+pytestmark = pytest.mark.gpu
+
+AV_CONFIGS = [
+    ("euler_p1_damp", lambda: cases.channel_bump(16, 4), lambda: cases.euler_props(order=2, dampTol=3.0, dampConst=0.0)),
+    ("euler_p2_inner", lambda: cases.channel_bump(12, 4),
+     lambda: cases.euler_props(order=3, dampTol=2.5, dampConst=2e-3, dampInnerTol=2.0)),
+    ("ns_p3_coef", lambda: cases.cylinder_mesh(16, 5),
+     lambda: cases.ns_props(order=4, reynolds=300.0, dampTol=1.0, dampConst=1e-3, dampInnerTol=1.5,
+                            dampInnerCoef=[0.01])),
+]
+
+
+def _setup(case, props, rel=0.1, seed=3):
+    props = dict(props, jacobian="exact")
+    o = O.Oracle(case, props)
+    dev = solver.DeviceCase(case, props)
+    W0 = dev.initial_state()
+    W = cases.perturbed_state(W0[0][:: dev.nb], case.n_elem, dev.nb, rel, seed)
+    o.set_w(W)
+    dev.set_state(W)
+    o.limiter()
+    dev.limiter(True)
+    return o, dev, W
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("name,mk_case,mk_props", AV_CONFIGS, ids=[c[0] for c in AV_CONFIGS])
+def test_limiter_matches_oracle(name, mk_case, mk_props):
+    case, props = mk_case(), mk_props()
+    o, dev, W = _setup(case, props)
+    eps, di = dev.artificial_viscosity()
+    eps_ref, di_ref = o.artificial_viscosity()
+    assert (eps_ref > 0).any()
+    assert _rel(eps, eps_ref) < 1e-12
+    if (di_ref > 0).any():
+        assert _rel(di, di_ref) < 1e-12
+    # the second call sees the memory of the first (innerIndicatorOld)
+    W2 = cases.perturbed_state(W[0][:: dev.nb], case.n_elem, dev.nb, 0.002, 8)
+    o.set_w(W2)
+    dev.set_state(W2)
+    o.limiter()
+    dev.limiter(False)
+    eps, di = dev.artificial_viscosity()
+    eps_ref, di_ref = o.artificial_viscosity()
+    assert np.abs(eps - eps_ref).max() <= 1e-12 * max(np.abs(eps_ref).max(), 1e-300)
+    assert np.abs(di - di_ref).max() <= 1e-12 * max(np.abs(di_ref).max(), 1e-300)
+    dev.close()
+
+
+@pytest.mark.parametrize("name,mk_case,mk_props", AV_CONFIGS, ids=[c[0] for c in AV_CONFIGS])
+def test_damped_residual_and_jacobian(name, mk_case, mk_props):
+    case, props = mk_case(), mk_props()
+    o, dev, W = _setup(case, props)
+    dt = 0.05
+    o.assemble(dt, -1.0)
+    dev.assemble(dt, -1.0, 1, True)
+    R, _ = dev.residual()
+    assert eq_rel_err(R, o.residual(), dev.nb, dev.neq) < 1e-12
+    d, n = dev.jacobian()
+    dref, nref = o.blocks()
+    assert block_rel_err(d, dref) < 1e-10
+    assert block_rel_err(n, nref) < 1e-10
+    dev.close()
+
+
+@pytest.mark.parametrize("name,mk_case,mk_props", AV_CONFIGS[1:], ids=[c[0] for c in AV_CONFIGS[1:]])
+def test_damped_jv_jtv(name, mk_case, mk_props):
+    case, props = mk_case(), mk_props()
+    o, dev, W = _setup(case, props)
+    rng = np.random.default_rng(7)
+    v = rng.standard_normal((case.n_elem, dev.B))
+    a1 = 20.0
+    assert block_rel_err(dev.jv(a1, v), o.jvp(a1, v)) < 1e-10
+    assert block_rel_err(dev.jtv(a1, v), o.vjp(a1, v)) < 1e-10
+    dev.close()
+
+
+def test_damped_newton_history():
+    """BDF1 steps of the GAMM-style setup (Euler, order 2, dampTol 3) plus inner damping: the
+    limiter runs every step on both sides, its memory carried across steps."""
+    case = cases.channel_bump(16, 4)
+    props = cases.euler_props(order=2, CFL=5.0, dampTol=3.0, dampInnerTol=2.0, jacobian="exact",
+                              iterativeSolverTol=1e-10, newtonIters=1)
+    o = O.Oracle(case, props)
+    dev = solver.DeviceCase(case, props)
+    W = cases.perturbed_state(dev.initial_state()[0][:: dev.nb], case.n_elem, dev.nb, 0.1, 21)
+    o.set_w(W)
+    dev.set_state(W)
+    n = 6
+    hist_ref = o.run(n)
+    s = solver.GpuImplicitSolver(dev, props)
+    hist = np.array([h.residuum for h in s.solve(n)])
+    assert len(hist) == len(hist_ref) == n
+    assert np.abs(hist / hist_ref - 1).max() < 1e-8
+    assert block_rel_err(dev.get_state(), o.get_w()) < 1e-8
+    dev.close()
