@@ -924,6 +924,11 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     P.cw3 = cs->sa_cw3;
     P.cv1 = cs->sa_cv1;
     P.Prt = cs->sa_prt;
+    P.flux = cs->model == MODEL_NAVIER_STOKES && cs->numerical_flux == 1 ? 1 : 0;   // Aerodynamics.java:300-303
+    if (cs->numerical_flux < 0 || cs->numerical_flux > 1) {
+        c->err = "numerical_flux: 0 (default) or 1 (vanLeer)";
+        return FP_EINVAL;
+    }
     // DGpure.java:287-289 zeroes beta0 inside the isDiffusive branch; the damping flux of an
     // inviscid model keeps it
     c->dg.beta0 = (cs->order == 1 && diffusive) ? 0.0 : cs->beta0;

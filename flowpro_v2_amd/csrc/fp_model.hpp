@@ -76,6 +76,7 @@ struct ModelParams {
     double a0, a1, k, lambda;   // scalar model
     // Spalart-Allmaras: inlet turbulence and model constants, cw1 = cb1/ka^2 + (1 + cb2)/sigma
     double vtIn, sigma, cb1, cb2, ka, cw1, cw2, cw3, cv1, Prt;
+    int flux;          // NavierStokes interior flux: 0 Lax-Friedrichs, 1 van Leer splitting
 };
 
 constexpr double RHO_TOL = 1e-2;   // Aerodynamics.java:22
@@ -118,6 +119,39 @@ template <int MK, class T> __device__ inline void convective_flux(const ModelPar
     if constexpr (MK == MODEL_SA) f[4] = W[4] * V;
 }
 
+// NavierStokes.fplus (sign +1) / fminus (sign -1) (ModuleProjects/Equations/Aerodynamics/src/
+// main/java/flowpro/user/equation/NavierStokes.java:407-465) in DIM dimensions: van Leer's split
+// fluxes, mass flux +-rho a (M +- 1)^2 / 4 for |M| < 1, the whole flux on one side beyond
+template <int DIM, class T>
+__device__ inline void van_leer_split(const ModelParams& P, const T* W, T p, const double* n, int sign, T* f) {
+    const double kapa = P.kapa;
+#pragma unroll
+    for (int j = 0; j < DIM + 2; j++) f[j] = 0.0;
+    T vel[DIM];
+    T Vn = 0.0, q2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        vel[d] = W[d + 1] / W[0];
+        Vn += vel[d] * n[d];
+        q2 += vel[d] * vel[d];
+    }
+    T a = t_sqrt(kapa * p / W[0]);
+    T M = Vn / a;
+    if (fabs(value(M)) < 1) {
+        T fm = sign > 0 ? W[0] * a * ((M + 1.0) * (M + 1.0)) / 4.0 : -W[0] * a * ((M - 1.0) * (M - 1.0)) / 4.0;
+        f[0] = fm;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) f[d + 1] = fm * (vel[d] + (-Vn + (sign * 2.0) * a) / kapa * n[d]);
+        T c = (kapa - 1.0) * Vn + (sign * 2.0) * a;
+        f[DIM + 1] = fm * ((q2 - Vn * Vn) / 2.0 + c * c / (2.0 * (kapa * kapa - 1.0)));
+    } else if (sign > 0 ? value(M) >= 1 : value(M) <= -1) {
+        f[0] = W[0] * Vn;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) f[d + 1] = W[d + 1] * Vn + p * n[d];
+        f[DIM + 1] = (W[DIM + 1] + p) * Vn;
+    }
+}
+
 template <int MK, class T> __device__ inline T max_eigenvalue(const ModelParams& P, T* W) {
     if constexpr (MK == MODEL_SCALAR) return T(P.lambda);
     W[0] = limite_rho(W[0]);
@@ -155,6 +189,18 @@ __device__ inline void numerical_convective_flux(const ModelParams& P, T* WL, T*
     if (TT == BT_INLET || TT == BT_OUTLET) {
         convective_flux<MK>(P, WR, n, f);
         return;
+    }
+    if constexpr (MK == MODEL_NAVIER_STOKES) {
+        if (P.flux == 1) {   // NavierStokes.java:174-179: fplus(WL, pressure(WL)) + fminus(WR, pressure(WR))
+            T pL = pressure_m<MK>(P, WL);
+            T fp[NE], fm[NE];
+            van_leer_split<2>(P, WL, pL, n, +1, fp);
+            T pR = pressure_m<MK>(P, WR);
+            van_leer_split<2>(P, WR, pR, n, -1, fm);
+#pragma unroll
+            for (int j = 0; j < NE; j++) f[j] = fp[j] + fm[j];
+            return;
+        }
     }
     T fL[NE], fR[NE];
     convective_flux<MK>(P, WL, n, fL);

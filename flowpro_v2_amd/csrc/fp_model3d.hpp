@@ -60,6 +60,16 @@ __device__ inline void numerical_convective_flux3(const ModelParams& P, T* WL, T
         convective_flux3(P, WR, n, f);
         return;
     }
+    if (P.flux == 1) {   // van Leer splitting, NavierStokes.java:174-179
+        T pL = pressure3(P, WL);
+        T fp[5], fm[5];
+        van_leer_split<3>(P, WL, pL, n, +1, fp);
+        T pR = pressure3(P, WR);
+        van_leer_split<3>(P, WR, pR, n, -1, fm);
+#pragma unroll
+        for (int j = 0; j < 5; j++) f[j] = fp[j] + fm[j];
+        return;
+    }
     T fL[5], fR[5];
     convective_flux3(P, WL, n, fL);
     convective_flux3(P, WR, n, fR);

@@ -50,6 +50,7 @@ class FpCase(C.Structure):
         ("dimension", C.c_int),
         ("damp_tol", C.c_double), ("damp_const", C.c_double), ("damp_inner_tol", C.c_double),
         ("n_damp_inner_coef", C.c_int), ("damp_inner_coef", C.c_double * 5),
+        ("numerical_flux", C.c_int),
     ]
 
 

@@ -89,6 +89,10 @@ class DeviceCase:
         cs.damp_tol = float(props.get("dampTol", 0.0))
         cs.damp_const = float(props.get("dampConst", 0.0))
         cs.damp_inner_tol = float(props.get("dampInnerTol", 0.0))
+        flux = props.get("numericalFlux", "default")
+        if flux not in ("default", "vanLeer"):
+            raise UnsupportedCase(f"numerical flux {flux}")
+        cs.numerical_flux = 1 if flux == "vanLeer" else 0
         coef = props.get("dampInnerCoef")
         if coef is not None:
             coef = [float(v) for v in (coef if hasattr(coef, "__len__") else [coef])]

@@ -86,6 +86,8 @@ typedef struct fp_case {
     double damp_tol, damp_const, damp_inner_tol;
     int n_damp_inner_coef;
     double damp_inner_coef[5];
+    /* Aerodynamics.numericalFluxType of NavierStokes: 0 "default" (Lax-Friedrichs), 1 "vanLeer" */
+    int numerical_flux;
 } fp_case;
 
 /* --- integer setup (bit-exact with the reference) ------------------------------ */

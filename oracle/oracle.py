@@ -27,7 +27,7 @@ _D = ["kappa", "cv", "pIn0", "rhoIn0", "pOut", "reynolds", "prandtl", "attack_an
 class Props(C.Structure):
     _fields_ = [(n, C.c_int) for n in _I] + [(n, C.c_double) for n in _D] + [("dimension", C.c_int)] + \
         [("damp_tol", C.c_double), ("damp_const", C.c_double), ("damp_inner_tol", C.c_double),
-         ("n_damp_inner_coef", C.c_int), ("damp_inner_coef", C.c_double * 5)]
+         ("n_damp_inner_coef", C.c_int), ("damp_inner_coef", C.c_double * 5), ("numerical_flux", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -67,6 +67,7 @@ def lib():
             "or_elem_geometry": (None, [vp, C.c_int, dp, dp, dp, dp, dp, dp]),
             "or_wall_distance": (None, [C.c_int, dp, C.c_int, ip, ip, dp]),
             "or_model_pointwise": (None, [vp, dp, dp, C.c_double, dp, dp, dp]),
+            "or_flux_pointwise": (None, [vp, dp, dp, dp, C.c_int, dp, dp]),
             "or_nqf": (C.c_int, [vp]),
             "ot_create": (vp, [C.POINTER(Props), C.c_int, dp, C.c_int, ip, C.c_int, ip, C.c_char_p]),
             "ot_destroy": (None, [vp]), "ot_nbasis": (C.c_int, [vp]), "ot_neqs": (C.c_int, [vp]),
@@ -133,6 +134,7 @@ def props_from(p: dict, threads: int = 1) -> Props:
     P.dimension = int(p.get("dimension", 2))
     P.damp_tol, P.damp_const = float(p.get("dampTol", 0.0)), float(p.get("dampConst", 0.0))
     P.damp_inner_tol = float(p.get("dampInnerTol", 0.0))
+    P.numerical_flux = 1 if p.get("numericalFlux", "default") == "vanLeer" else 0
     coef = p.get("dampInnerCoef")
     if coef is not None:
         coef = [float(v) for v in (coef if hasattr(coef, "__len__") else [coef])]
@@ -335,6 +337,16 @@ class Oracle:
         src, fx, fy = np.zeros(n), np.zeros(n), np.zeros(n)
         lib().or_model_pointwise(self.h, _dp(W), _dp(dW), float(D), _dp(src), _dp(fx), _dp(fy))
         return src, fx, fy
+
+    def flux_pointwise(self, WL, WR, n, TT=0):
+        """(numerical flux of the face point, physical flux of WL) for unit normal n."""
+        k = self.neq
+        WL = np.ascontiguousarray(WL, dtype=np.float64)
+        WR = np.ascontiguousarray(WR, dtype=np.float64)
+        n = np.ascontiguousarray(n, dtype=np.float64)
+        fn, fp = np.zeros(k), np.zeros(k)
+        lib().or_flux_pointwise(self.h, _dp(WL), _dp(WR), _dp(n), int(TT), _dp(fn), _dp(fp))
+        return fn, fp
 
     def l2_error_scalar(self):
         return lib().or_l2_error_scalar(self.h)

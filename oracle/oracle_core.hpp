@@ -102,12 +102,14 @@ struct Props {
     double damp_tol, damp_const, damp_inner_tol;
     int n_damp_inner_coef;
     double damp_inner_coef[5];
+    int numerical_flux;   // Aerodynamics.numericalFluxType: 0 "default" (Lax-Friedrichs), 1 "vanLeer"
 };
 
 struct ModelConst {
     int kind;        // 0 NS/Euler, 1 Spalart-Allmaras, 2 scalar
     int dim, nEqs;
     bool diffusive;
+    bool vanLeer;    // NavierStokes interior flux: van Leer flux-vector splitting
     double kapa, cv, cp;
     double pRef, rhoRef, velocityRef, lRef, tRef;
     double pOut;     // normalised by pRef
@@ -239,6 +241,7 @@ void vjp(Case& c, double a1, const double* v, double* out);
 double functional(const Case& c, int which, const double* W /*nullable*/);
 // pointwise source term and diffusive fluxes of the model (tests)
 void model_pointwise(const Case& c, const double* W, const double* dW, double D, double* source, double* fvx, double* fvy);
+void flux_pointwise(const Case& c, const double* WL, const double* WR, const double* n, int TT, double* fnum, double* fphys);
 void functional_grad_fd(Case& c, int which, double* dIdW);
 
 }  // namespace oracle

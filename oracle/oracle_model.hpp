@@ -67,7 +67,35 @@ template <class T> static T maxEigenvalue(const ModelConst& M, T* W, const QPDat
     v = dsqrt(v);
     return v + a;
 }
-// AERO/NavierStokes.java:141-192 (default = Lax-Friedrichs)
+// AERO/NavierStokes.java:407-465 fplus (sign +1) / fminus (sign -1): van Leer's split fluxes;
+// inside |M| < 1 the mass flux is +-rho a (M +- 1)^2 / 4, beyond the sonic point one side
+// carries the whole flux and the other none
+template <class T> static void vanLeerSplit(const ModelConst& M, const T* W, T p, const double* n, int sign, T* f) {
+    const int dim = M.dim, nEqs = M.nEqs;
+    const double kapa = M.kapa;
+    for (int j = 0; j < nEqs; j++) f[j] = 0.0;
+    T vel[3];
+    T Vn = 0.0, q2 = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        vel[d] = W[d + 1] / W[0];
+        Vn += vel[d] * n[d];
+        q2 += vel[d] * vel[d];
+    }
+    T a = dsqrt(kapa * p / W[0]);
+    T Mn = Vn / a;
+    if (std::fabs(val(Mn)) < 1) {
+        T fm = sign > 0 ? W[0] * a * ((Mn + 1) * (Mn + 1)) / 4 : -W[0] * a * ((Mn - 1) * (Mn - 1)) / 4;
+        f[0] = fm;
+        for (int d = 0; d < dim; ++d) f[d + 1] = fm * (vel[d] + (-Vn + sign * 2 * a) / kapa * n[d]);
+        T c = (kapa - 1) * Vn + sign * 2 * a;
+        f[dim + 1] = fm * ((q2 - Vn * Vn) / 2 + c * c / (2 * (kapa * kapa - 1)));
+    } else if (sign > 0 ? val(Mn) >= 1 : val(Mn) <= -1) {
+        f[0] = W[0] * Vn;
+        for (int d = 0; d < dim; ++d) f[d + 1] = W[d + 1] * Vn + p * n[d];
+        f[dim + 1] = (W[dim + 1] + p) * Vn;
+    }
+}
+// AERO/NavierStokes.java:141-192 (default = Lax-Friedrichs, numericalFlux = vanLeer)
 template <class T>
 static void numericalConvectiveFlux(const ModelConst& M, T* WL, T* WR, const double* n, int TT, const QPData& q, T* f) {
     const int nEqs = M.nEqs, dim = M.dim;
@@ -101,6 +129,15 @@ static void numericalConvectiveFlux(const ModelConst& M, T* WL, T* WR, const dou
             convectiveFlux(M, WR, n, f);
             break;
         default: {
+            if (M.vanLeer) {   // AERO/NavierStokes.java:174-179
+                T pL = pressure(M, WL);
+                T fp[5], fm[5];
+                vanLeerSplit(M, WL, pL, n, +1, fp);
+                T pR = pressure(M, WR);
+                vanLeerSplit(M, WR, pR, n, -1, fm);
+                for (int j = 0; j < nEqs; j++) f[j] = fp[j] + fm[j];
+                break;
+            }
             T fL[5], fR[5];
             convectiveFlux(M, WL, n, fL);
             convectiveFlux(M, WR, n, fR);

@@ -577,4 +577,18 @@ void model_pointwise(const Case& c, const double* W, const double* dW, double D,
     for (int m = 0; m < nEqs; m++) w[m] = W[m];
     diffusiveFlux(M, w, dw, ey, fvy);
 }
+
+// numerical flux of one face point (interior when TT > -1) and the physical flux of WL
+void flux_pointwise(const Case& c, const double* WL, const double* WR, const double* n, int TT, double* fnum, double* fphys) {
+    const int nEqs = c.Mc.nEqs;
+    double wl[5], wr[5];
+    QPData q;
+    q.currentX[0] = q.currentX[1] = q.currentX[2] = 0;
+    q.meshVelocity[0] = q.meshVelocity[1] = q.meshVelocity[2] = 0;
+    q.wallDistance = 0;
+    for (int m = 0; m < nEqs; m++) { wl[m] = WL[m]; wr[m] = WR[m]; }
+    numericalConvectiveFlux(c.Mc, wl, wr, n, TT, q, fnum);
+    for (int m = 0; m < nEqs; m++) wl[m] = WL[m];
+    convectiveFlux(c.Mc, wl, n, fphys);
+}
 }  // namespace oracle

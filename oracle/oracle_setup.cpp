@@ -106,6 +106,7 @@ ModelConst model_init(const Props& P) {
     M.nEqs = M.kind == 1 ? M.dim + 3 : M.dim + 2;
     // SpalartAllmaras.init calls Aerodynamics.init(props, dim, dim + 3, true): always diffusive
     M.diffusive = M.kind == 1 ? true : P.viscous != 0;
+    M.vanLeer = M.kind == 0 && P.numerical_flux == 1;   // AERO/Aerodynamics.java:300-303
     if (M.kind == 1) {   // AERO/SpalartAllmaras.java:36-61
         M.vtIn = P.sa_vtIn;
         M.sigma = P.sa_sigma;

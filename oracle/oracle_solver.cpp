@@ -750,6 +750,9 @@ double or_l2_error_scalar(void* h) {
 }
 int or_nqf(void* h) { return ((Case*)h)->face.n; }
 int or_nqv(void* h) { return ((Case*)h)->vol.n; }
+void or_flux_pointwise(void* h, const double* WL, const double* WR, const double* n, int TT, double* fnum, double* fphys) {
+    flux_pointwise(*(Case*)h, WL, WR, n, TT, fnum, fphys);
+}
 void or_model_pointwise(void* h, const double* W, const double* dW, double D, double* s, double* fx, double* fy) {
     model_pointwise(*(Case*)h, W, dW, D, s, fx, fy);
 }

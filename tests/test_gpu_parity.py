@@ -40,6 +40,10 @@ CONFIGS = [
     ("ns_p3", lambda: cases.cylinder_mesh(16, 5), lambda: cases.ns_props(order=4)),
     ("scalar_p2_diff", lambda: cases.square_mesh(4, 1.0, -2),
      lambda: dict(model="TestScalarEquation", a=(1.0, 0.5), k=0.1, order=3, dt=1e8, penalty=0.0)),
+    # numericalFlux = vanLeer (NavierStokes.fplus / fminus) on interior faces
+    ("euler_p2_vanleer", lambda: cases.channel_bump(12, 4), lambda: cases.euler_props(order=3, numericalFlux="vanLeer")),
+    ("ns_p1_vanleer", lambda: cases.cylinder_mesh(16, 6),
+     lambda: cases.ns_props(order=2, reynolds=100.0, numericalFlux="vanLeer")),
 ]
 # Spalart-Allmaras on the flat-plate channel (wall, symmetry, inlet, outlet faces; vertex wall distance)
 SA_CONFIGS = [
@@ -69,7 +73,10 @@ def test_residual_and_jacobian(name, mk_case, mk_props):
     dev.close()
 
 
-@pytest.mark.parametrize("name,mk_case,mk_props", CONFIGS[:4] + SA_CONFIGS, ids=[c[0] for c in CONFIGS[:4] + SA_CONFIGS])
+JV_CONFIGS = CONFIGS[:4] + CONFIGS[5:] + SA_CONFIGS
+
+
+@pytest.mark.parametrize("name,mk_case,mk_props", JV_CONFIGS, ids=[c[0] for c in JV_CONFIGS])
 def test_jv_jtv(name, mk_case, mk_props):
     case, props = mk_case(), mk_props()
     o, dev, W = _setup(case, props)
@@ -171,6 +178,8 @@ def test_fp32_preconditioner_same_solution():
     # noise, which differs between any two summation orders; from a perturbed state it is smooth
     ("sa_p2_bdf2", lambda: cases.plate_channel(12, 5),
      lambda: cases.sa_props(order=3, reynolds=1e5, CFL=2.0, iterativeSolverTol=1e-12, timeOrder=2, perturb=21), 6),
+    ("euler_p1_vanleer", lambda: cases.channel_bump(16, 4),
+     lambda: cases.euler_props(order=2, CFL=2.0, iterativeSolverTol=1e-12, numericalFlux="vanLeer", perturb=5), 6),
 ])
 def test_newton_history(name, mk_case, mk_props, steps):
     case, props = mk_case(), dict(mk_props())

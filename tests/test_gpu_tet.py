@@ -110,6 +110,25 @@ def test_tet_exact_jacobian_blocks(order):
     dev.close()
 
 
+def test_tet_van_leer_residual_and_jacobian():
+    """numericalFlux = vanLeer on the tetrahedra (the 3D split fluxes of NavierStokes.fplus /
+    fminus): residual 1e-12, dense Jacobian 1e-10."""
+    case = cases.kuhn_cube(3, 0.05, 4)
+    props = cases.euler3d_props(order=2, numericalFlux="vanLeer")
+    o = O.OracleTet(case, props)
+    dev = solver.DeviceCase(case, props)
+    W = _state(case, dev.nb, dev.initial_state()[0][:: dev.nb])
+    o.set_w(W)
+    dev.set_state(W)
+    dt = 0.05
+    dev.assemble(dt, -1.0, 1, True)
+    R, _ = dev.residual()
+    assert eq_rel_err(R, o.residual(), dev.nb, dev.neq) < 1e-12
+    d, n = dev.jacobian()
+    assert block_rel_err(_dense_from_blocks(o, d, n), o.jacobian(1.0 / dt)) < 1e-10
+    dev.close()
+
+
 @pytest.mark.parametrize("order", [1, 2])
 def test_tet_jv_jtv(order):
     case, o, dev, W = _setup(order)
