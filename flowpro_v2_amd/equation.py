@@ -13,7 +13,7 @@ from __future__ import annotations
 import math
 from dataclasses import dataclass
 
-MODEL_IDS = {"NavierStokes": 0, "SpalartAllmaras": 1, "TestScalarEquation": 2}
+MODEL_IDS = {"NavierStokes": 0, "SpalartAllmaras": 1, "TestScalarEquation": 2, "AdiabaticEulers": 3}
 # model constants of the reference's SpalartAllmaras template
 # (ModuleProjects/Equations/Aerodynamics/src/main/resources/templates/SpalartAllmaras.txt)
 SA_DEFAULTS = {"sigma": 0.6667, "cb1": 0.1355, "cb2": 0.622, "ka": 0.41, "cw2": 0.3, "cw3": 2.0, "cv1": 7.1,
@@ -53,19 +53,23 @@ def init(props: dict) -> ModelConstants:
         a = tuple(float(v) for v in props.get("a", (1.0, 0.5)))
         k = float(props.get("k", 0.0))
         return ModelConstants(model, 2, 1, 2, k > 0, a=a, k=k)
-    if model not in ("NavierStokes", "SpalartAllmaras"):
+    if model not in ("NavierStokes", "SpalartAllmaras", "AdiabaticEulers"):
         raise UnsupportedModel(model)
     dim = int(props.get("dimension", 2))
     sa = model == "SpalartAllmaras"
-    if dim not in (2, 3) or (dim == 3 and (sa or bool(props.get("isFlowViscous", False)))):
+    adiabatic = model == "AdiabaticEulers"
+    if dim not in (2, 3) or (dim == 3 and (sa or adiabatic or bool(props.get("isFlowViscous", False)))):
         raise UnsupportedModel(f"{model} in {dim}D")
+    if adiabatic and bool(props.get("isInletSupersonic", False)):
+        raise UnsupportedModel("AdiabaticEulers with a supersonic inlet")
     kappa, cv = float(props["kappa"]), float(props["cv"])
     # subsonic stagnation inlet: pIn0, rhoIn0, pOut (Aerodynamics.java:169-182)
     p_ref, rho_ref = float(props["pIn0"]), float(props["rhoIn0"])
     p_out = float(props["pOut"]) / p_ref
     velocity_ref = math.sqrt(p_ref / rho_ref)
     # SpalartAllmaras.init passes isDiffusive = true (SpalartAllmaras.java:38)
-    diffusive = True if sa else bool(props.get("isFlowViscous", False))
+    # AdiabaticEulers.init: Aerodynamics.init(props, dim, dim + 1, false) (AdiabaticEulers.java:13-16)
+    diffusive = True if sa else (False if adiabatic else bool(props.get("isFlowViscous", False)))
     re = pr = -1.0
     if diffusive:
         # far-field Reynolds number rescaled to the reference values (Aerodynamics.java:262-281)
@@ -81,7 +85,7 @@ def init(props: dict) -> ModelConstants:
     extra = {}
     if sa:
         extra = dict(vt_in=float(props["vtIn"]), sa={k: float(props.get(k, v)) for k, v in SA_DEFAULTS.items()})
-    return ModelConstants(model, MODEL_IDS[model], dim + (3 if sa else 2), dim, diffusive, kappa=kappa, cv=cv, p_ref=p_ref,
+    return ModelConstants(model, MODEL_IDS[model], dim + (3 if sa else (1 if adiabatic else 2)), dim, diffusive, kappa=kappa, cv=cv, p_ref=p_ref,
                           rho_ref=rho_ref, velocity_ref=velocity_ref, p_out=p_out, reynolds=re, prandtl=pr,
                           attack=attack, **extra)
 
@@ -99,6 +103,8 @@ def const_init_condition(mc: ModelConstants) -> list[float]:
     w = [rho_in, rho_in * v_in * math.cos(alpha), rho_in * v_in * math.sin(alpha)]
     if mc.dim == 3:
         w.append(0.0)
+    if mc.model_id == 3:   # AdiabaticEulers.constInitCondition: no energy (AdiabaticEulers.java:97-126)
+        return w
     w.append(mc.p_out / (kappa - 1) + 0.5 * rho_in * v_in * v_in)
     if mc.model_id == 1:
         w.append(rho_in * mc.vt_in)

@@ -88,7 +88,7 @@ def lib():
 
 
 _PREC = {"none": 0, "blockjacobi": 1, "blockilu0": 2}
-_MODEL = {"NavierStokes": 0, "SpalartAllmaras": 1, "TestScalarEquation": 2}
+_MODEL = {"NavierStokes": 0, "SpalartAllmaras": 1, "TestScalarEquation": 2, "AdiabaticEulers": 3}
 # SpalartAllmaras constants of the reference's template (Aerodynamics/src/main/resources/templates/
 # SpalartAllmaras.txt); parameters.txt may override each
 SA_DEFAULTS = {"sigma": 0.6667, "cb1": 0.1355, "cb2": 0.622, "ka": 0.41, "cw2": 0.3, "cw3": 2.0, "cv1": 7.1,

@@ -27,6 +27,12 @@ template <class T> static T limiteRho(T rho) {
 }
 // AERO/NavierStokes.java:23-37, AERO/SpalartAllmaras.java:63-76 (its own P_TOL)
 template <class T> static T pressure(const ModelConst& M, T* W) {
+    if (M.kind == 3) {   // AERO/AdiabaticEulers.java:25-33: isentropic p = rho^kappa, P_TOL = 0.1
+        W[0] = limiteRho(W[0]);
+        T p = dpow(W[0], M.kapa);
+        if (val(p) < 1e-1) p = 1e-1 * dexp((p - 1e-1) / 1e-1);
+        return p;
+    }
     const double ptol = M.kind == 1 ? P_TOL_SA : P_TOL;
     W[0] = limiteRho(W[0]);
     T momentum2 = 0.0;
@@ -50,6 +56,7 @@ template <class T> static void convectiveFlux(const ModelConst& M, T* W, const d
     T p = pressure(M, W);
     f[0] = W[0] * V;
     for (int d = 0; d < M.dim; ++d) f[d + 1] = W[d + 1] * V + p * n[d];
+    if (M.kind == 3) return;   // AERO/AdiabaticEulers.java:177-193: no energy equation
     f[M.dim + 1] = (W[M.dim + 1] + p) * V;
     if (M.kind == 1) f[M.dim + 2] = W[M.dim + 2] * V;   // AERO/SpalartAllmaras.java:196
 }
@@ -113,14 +120,15 @@ static void numericalConvectiveFlux(const ModelConst& M, T* WL, T* WR, const dou
         case WALL:
         case INVISCID_WALL: {
             // NavierStokes: p(WL); SpalartAllmaras: p(WR) (AERO/SpalartAllmaras.java:146)
-            T p = pressure(M, M.kind == 1 ? WR : WL);
+            // AdiabaticEulers: p(WR) and no energy component (AERO/AdiabaticEulers.java:136-149)
+            T p = pressure(M, (M.kind == 1 || M.kind == 3) ? WR : WL);
             f[0] = 0.0;
             double V = .0;
             for (int d = 0; d < dim; ++d) {
                 f[d + 1] = p * n[d];
                 V += q.meshVelocity[d] * n[d];
             }
-            f[dim + 1] = p * V;
+            if (M.kind != 3) f[dim + 1] = p * V;
             for (int j = 0; j < nEqs; j++) f[j] += V * WR[j];
             break;
         }
@@ -148,11 +156,62 @@ static void numericalConvectiveFlux(const ModelConst& M, T* WL, T* WR, const dou
         }
     }
 }
+// AERO/AdiabaticEulers.java:36-95 (subsonic inlet): the wall state adds n (m . n), as
+// SpalartAllmaras does; the outlet density is pOut^(1/kappa)
+template <class T> static void boundaryValueAdiabatic(const ModelConst& M, T* WL, const double* n, int TT, T* WR) {
+    const int nEqs = M.nEqs, dim = M.dim;
+    const double kapa = M.kapa;
+    WL[0] = limiteRho(WL[0]);
+    for (int j = 0; j < nEqs; j++) WR[j] = 0.0;
+    switch (TT) {
+        case WALL:
+        case INVISCID_WALL: {
+            for (int j = 0; j < nEqs; j++) WR[j] = WL[j];
+            T nu = 0.0;
+            for (int d = 0; d < dim; ++d) nu += WL[d + 1] * n[d];
+            for (int d = 0; d < dim; ++d) WR[d + 1] = WL[d + 1] + n[d] * nu;
+            break;
+        }
+        case INLET: {
+            const double pIn0 = 1, rhoIn0 = 1;
+            T p = pressure(M, WL);
+            T mach;
+            if (val(p) > pIn0) mach = 0.0;
+            else mach = dsqrt((2 / (kapa - 1)) * (-1 + dpow(pIn0 / p, (kapa - 1) / kapa)));
+            T rhoIn = rhoIn0 * dpow((1 + ((kapa - 1) / 2) * mach * mach), 1 / (1 - kapa));
+            T normalVelocity = mach * dsqrt((kapa * p) / rhoIn);
+            WR[0] = rhoIn;
+            for (int d = 0; d < dim; ++d) WR[d + 1] = -rhoIn * normalVelocity * n[d];
+            break;
+        }
+        case OUTLET: {
+            T absVelocity = .0;
+            for (int d = 0; d < dim; ++d) absVelocity += WL[d + 1] * WL[d + 1];
+            absVelocity = dsqrt(absVelocity) / WL[0];
+            T a = dsqrt(kapa * pressure(M, WL) / WL[0]);
+            T mach = absVelocity / a;
+            if (val(mach) < 1) {
+                const double rhoOut = std::pow(M.pOut, 1. / kapa);
+                WR[0] = rhoOut;
+                for (int d = 0; d < dim; ++d) WR[d + 1] = rhoOut * WL[d + 1] / WL[0];
+            } else {
+                for (int j = 0; j < nEqs; j++) WR[j] = WL[j];
+            }
+            break;
+        }
+        default:
+            break;
+    }
+}
 // AERO/NavierStokes.java:216-315 (subsonic inlet, no attack angle unless given)
 template <class T> static void boundaryValue(const ModelConst& M, T* WL, const double* n, int TT, const QPData& q, T* WR) {
     const int nEqs = M.nEqs, dim = M.dim;
     if (M.kind == 2) {   // exact(currentX)
         WR[0] = std::sin(2 * q.currentX[0] + q.currentX[1]);
+        return;
+    }
+    if (M.kind == 3) {
+        boundaryValueAdiabatic(M, WL, n, TT, WR);
         return;
     }
     WL[0] = limiteRho(WL[0]);

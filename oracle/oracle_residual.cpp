@@ -578,7 +578,8 @@ void model_pointwise(const Case& c, const double* W, const double* dW, double D,
     diffusiveFlux(M, w, dw, ey, fvy);
 }
 
-// numerical flux of one face point (interior when TT > -1) and the physical flux of WL
+// numerical flux of one face point and the physical flux of WL; on boundary faces (TT < 0)
+// the outer state is the model's boundary value of WL, as in residuumWall
 void flux_pointwise(const Case& c, const double* WL, const double* WR, const double* n, int TT, double* fnum, double* fphys) {
     const int nEqs = c.Mc.nEqs;
     double wl[5], wr[5];
@@ -587,6 +588,7 @@ void flux_pointwise(const Case& c, const double* WL, const double* WR, const dou
     q.meshVelocity[0] = q.meshVelocity[1] = q.meshVelocity[2] = 0;
     q.wallDistance = 0;
     for (int m = 0; m < nEqs; m++) { wl[m] = WL[m]; wr[m] = WR[m]; }
+    if (TT < 0) boundaryValue(c.Mc, wl, n, TT, q, wr);
     numericalConvectiveFlux(c.Mc, wl, wr, n, TT, q, fnum);
     for (int m = 0; m < nEqs; m++) wl[m] = WL[m];
     convectiveFlux(c.Mc, wl, n, fphys);

@@ -102,10 +102,11 @@ ModelConst model_init(const Props& P) {
         M.diffusive = M.k > 0;
         return M;
     }
-    M.kind = P.model == 1 ? 1 : 0;
-    M.nEqs = M.kind == 1 ? M.dim + 3 : M.dim + 2;
+    M.kind = P.model == 1 ? 1 : (P.model == 3 ? 3 : 0);
+    // AdiabaticEulers.init: Aerodynamics.init(props, dim, dim + 1, false) (AERO/AdiabaticEulers.java:13-16)
+    M.nEqs = M.kind == 1 ? M.dim + 3 : (M.kind == 3 ? M.dim + 1 : M.dim + 2);
     // SpalartAllmaras.init calls Aerodynamics.init(props, dim, dim + 3, true): always diffusive
-    M.diffusive = M.kind == 1 ? true : P.viscous != 0;
+    M.diffusive = M.kind == 1 ? true : (M.kind == 3 ? false : P.viscous != 0);
     M.vanLeer = M.kind == 0 && P.numerical_flux == 1;   // AERO/Aerodynamics.java:300-303
     if (M.kind == 1) {   // AERO/SpalartAllmaras.java:36-61
         M.vtIn = P.sa_vtIn;
@@ -160,6 +161,7 @@ std::vector<double> const_init_condition(const ModelConst& M) {
     std::vector<double> W(M.nEqs);
     W[0] = rhoIn;
     for (int d = 0; d < M.dim; ++d) W[d + 1] = rhoIn * vIn[d];
+    if (M.kind == 3) return W;   // AERO/AdiabaticEulers.java:97-126: no energy
     W[M.dim + 1] = M.pOut / (kapa - 1) + 0.5 * rhoIn * VIn * VIn;
     if (M.kind == 1) W[M.dim + 2] = rhoIn * M.vtIn;   // AERO/SpalartAllmaras.java:135
     return W;

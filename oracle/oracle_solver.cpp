@@ -467,8 +467,9 @@ double time_step(Case& c) {
             for (int d = 0; d < 2; d++) m2 += Wavg[d + 1] * Wavg[d + 1];
             rho = Wavg[0] < 1e-2 ? 1e-2 * std::exp((Wavg[0] - 1e-2) / 1e-2) : Wavg[0];
             Wavg[0] = rho;
-            double p = (c.Mc.kapa - 1) * (Wavg[3] - m2 / (2 * Wavg[0]));
-            const double ptol = c.Mc.kind == 1 ? 1e-1 : 1e-2;   // SpalartAllmaras.pressure uses its own P_TOL
+            // AdiabaticEulers.pressure: rho^kappa (AERO/AdiabaticEulers.java:25-33)
+            double p = c.Mc.kind == 3 ? std::pow(Wavg[0], c.Mc.kapa) : (c.Mc.kapa - 1) * (Wavg[3] - m2 / (2 * Wavg[0]));
+            const double ptol = (c.Mc.kind == 1 || c.Mc.kind == 3) ? 1e-1 : 1e-2;   // SA and AdiabaticEulers P_TOL
             if (p < ptol) p = ptol * std::exp((p - ptol) / ptol);
             double a = std::sqrt(c.Mc.kapa * p / Wavg[0]);
             double v = 0;
@@ -488,9 +489,12 @@ double time_step(Case& c) {
     return c.state_dt;
 }
 
-// Element.positivityLimiter (C/element/Element.java:385-476) for NavierStokes
+// Element.positivityLimiter (C/element/Element.java:385-476) for NavierStokes; AdiabaticEulers
+// admits every positive density (AERO/AdiabaticEulers.java:19-22)
+static int g_admissible_kind = 0;
 static bool admissible(const double* W) {
     if (!(W[0] > 1e-10)) return false;
+    if (g_admissible_kind == 3) return true;
     double m2 = 0;
     for (int d = 0; d < 2; d++) m2 += W[d + 1] * W[d + 1];
     return W[3] - m2 / (2 * W[0]) > 1e-10;
@@ -498,6 +502,7 @@ static bool admissible(const double* W) {
 static bool positivity_limit(const Case& c, Elem& e) {
     const int Nb = e.nBasis, nEqs = c.Mc.nEqs;
     if (Nb == 1 || c.Mc.kind == 2) return false;
+    g_admissible_kind = c.Mc.kind;
     double Wavg[5] = {0, 0, 0, 0, 0};
     for (int j = 0; j < nEqs; j++)
         for (int m = 0; m < Nb; m++) Wavg[j] = Wavg[j] + e.W[j * Nb + m] * e.Is[m];
