@@ -287,6 +287,16 @@ def euler_props(mach: float = 0.67, order: int = 2, **kw) -> dict:
     return p
 
 
+def gamm_props(order: int = 2, **kw) -> dict:
+    """Parameters of the reference's GAMM channel example (FlowPro/simulations/examples/GAMM/
+    default/parameters.txt): AdiabaticEulers, pOut = 0.737, CFL 5, dampTol 3, ilu0."""
+    p = dict(model="AdiabaticEulers", kappa=1.4, cv=1.0, pIn0=1.0, rhoIn0=1.0, pOut=0.737, attackAngle=0.0,
+             order=order, timeOrder=1, CFL=5.0, newtonIters=1, penalty=1.0, dampTol=3.0, dampConst=0.0,
+             preconditioner="ilu0", elementOrdering="rcm", iterativeSolverTol=1e-2, newtonIterTol=1e-4)
+    p.update(kw)
+    return p
+
+
 def ns_props(mach: float = 0.5, reynolds: float = 5000.0, order: int = 4, **kw) -> dict:
     p = euler_props(mach=mach, order=order, isFlowViscous=True, reynolds=reynolds, prandtl=0.72, penalty=1.0)
     p.update(kw)

@@ -396,6 +396,9 @@ __global__ void __launch_bounds__(128) k_jvp(ModelParams P, DGParams G, Tables t
     X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, true) \
     X(MODEL_SA, 3, 5, 6, 2, true)             \
     X(MODEL_SA, 6, 5, 7, 3, true)             \
+    X(MODEL_ADIABATIC, 3, 3, 6, 2, false)     \
+    X(MODEL_ADIABATIC, 6, 3, 7, 3, false)     \
+    X(MODEL_ADIABATIC, 10, 3, 16, 4, false)   \
     X(MODEL_SCALAR, 1, 1, 1, 1, false)        \
     X(MODEL_SCALAR, 3, 1, 6, 2, false)        \
     X(MODEL_SCALAR, 6, 1, 7, 3, false)        \
@@ -579,19 +582,22 @@ hipError_t launch_av_limiter(const ElemLaunch& L, hipStream_t s, const ModelPara
 // ------------------------------------------------------------------ positivity limiter
 // Element.positivityLimiter (FlowPro/src/main/java/flowpro/core/element/Element.java:385-476) with
 // Aerodynamics.isAdmissible (:348-357); the L2 projection of the constant 1 onto the nodal basis is 1.
-__device__ inline bool admissible(const double* W) {
+// AdiabaticEulers (three equations) admits every positive density (AdiabaticEulers.java:19-22)
+template <int NEQ> __device__ inline bool admissible(const double* W) {
     if (!(W[0] > 1e-10)) return false;
+    if constexpr (NEQ == 3) return true;
     double m2 = W[1] * W[1];
     m2 += W[2] * W[2];
     return W[3] - m2 / (2 * W[0]) > 1e-10;
 }
-__device__ inline double admissible_theta(const double* Wavg, const double* Wx) {
-    if (admissible(Wx)) return 1.0;
-    double lo = 0, hi = 1, Wt[4];   // admissibility depends on rho, rho u, rho v, rho E only
+template <int NEQ> __device__ inline double admissible_theta(const double* Wavg, const double* Wx) {
+    if (admissible<NEQ>(Wx)) return 1.0;
+    constexpr int NA = NEQ < 4 ? NEQ : 4;   // admissibility depends on rho, rho u, rho v (, rho E) only
+    double lo = 0, hi = 1, Wt[NA];
     for (int it = 0; it < 30; it++) {
         double mid = 0.5 * (lo + hi);
-        for (int m = 0; m < 4; m++) Wt[m] = Wavg[m] + mid * (Wx[m] - Wavg[m]);
-        if (admissible(Wt)) lo = mid;
+        for (int m = 0; m < NA; m++) Wt[m] = Wavg[m] + mid * (Wx[m] - Wavg[m]);
+        if (admissible<NEQ>(Wt)) lo = mid;
         else hi = mid;
     }
     return lo;
@@ -611,14 +617,14 @@ __global__ void k_positivity(Tables tb, Mesh ms, double* __restrict__ W, const d
     for (int m = 0; m < NEQ; m++)
 #pragma unroll
         for (int j = 0; j < NB; j++) wa[m] = wa[m] + w[m * NB + j] * isref[j];
-    if (!admissible(wa)) return;
+    if (!admissible<NEQ>(wa)) return;
     double theta = 1.0;
     auto at = [&](const double* phi) {
         double wx[NEQ];
         for (int m = 0; m < NEQ; m++) wx[m] = 0.0;
         for (int m = 0; m < NEQ; m++)
             for (int j = 0; j < NB; j++) wx[m] += w[m * NB + j] * phi[j];
-        theta = fmin(theta, admissible_theta(wa, wx));
+        theta = fmin(theta, admissible_theta<NEQ>(wa, wx));
     };
     for (int q = 0; q < NQV; q++) at(tb.vb + q * NB);
     for (int k = 0; k < 3; k++)

@@ -862,7 +862,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         c->NEQ = 5;
     } else {
         c->NB = make_basis(cs->order).nb;
-        c->NEQ = cs->model == MODEL_SCALAR ? 1 : (cs->model == MODEL_SA ? 5 : 4);
+        c->NEQ = cs->model == MODEL_SCALAR ? 1 : (cs->model == MODEL_SA ? 5 : (cs->model == MODEL_ADIABATIC ? 3 : 4));
     }
     if (cs->model == MODEL_SA && !cs->wall_distance) {
         c->err = "the Spalart-Allmaras model needs the vertex wall distance";
@@ -873,7 +873,13 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     c->NQF = cs->n_face;
     c->dofs = (long long)n * c->B;
     // SpalartAllmaras.init passes isDiffusive = true to Aerodynamics.init (SpalartAllmaras.java:38)
-    const bool diffusive = cs->model == MODEL_SCALAR ? cs->scalar_k > 0 : (cs->model == MODEL_SA || cs->viscous != 0);
+    // AdiabaticEulers.init passes isDiffusive = false (AdiabaticEulers.java:13-16)
+    const bool diffusive = cs->model == MODEL_SCALAR ? cs->scalar_k > 0
+                                                     : (cs->model == MODEL_SA || (cs->model != MODEL_ADIABATIC && cs->viscous != 0));
+    if (cs->model < 0 || cs->model > MODEL_ADIABATIC) {
+        c->err = "unknown model";
+        return FP_EINVAL;
+    }
     const bool av = cs->damp_tol > 0 || cs->damp_const != 0 || cs->damp_inner_tol > 0;
     if (av && (c->dim == 3 || cs->model == MODEL_SA)) {
         c->err = "artificial viscosity: device kernels for the 2D NavierStokes and scalar models only";
@@ -889,11 +895,13 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         c->err = "no element kernel for this model / order / quadrature";
         return FP_UNSUPPORTED;
     }
-    if (cs->preconditioner < 0 || cs->preconditioner > 2 || (cs->precond_bits != 64 && cs->precond_bits != 32)) {
+    if (cs->preconditioner < 0 || cs->preconditioner > 3 || (cs->precond_bits != 64 && cs->precond_bits != 32)) {
         c->err = "bad preconditioner settings";
         return FP_EINVAL;
     }
-    c->precond = cs->preconditioner;
+    // ilu0 (ILU0.java: point ILU(0) on the CRS matrix) is the same incomplete factorisation as
+    // BlockILU0 on a pattern of dense element blocks, so it runs on the block factorisation
+    c->precond = cs->preconditioner == 3 ? 2 : cs->preconditioner;
     c->bits = cs->precond_bits;
     c->m = cs->gmres_restart > 0 ? cs->gmres_restart : 30;
     c->cycles = cs->gmres_cycles > 0 ? cs->gmres_cycles : 5;

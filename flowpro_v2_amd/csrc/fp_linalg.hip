@@ -1009,7 +1009,7 @@ hipError_t launch_ilu_copy(hipStream_t s, size_t n, const double* src, TF* dst) 
     return hipGetLastError();
 }
 
-#define FP_BLOCK_SIZES(X) X(1) X(3) X(5) X(6) X(10) X(12) X(15) X(20) X(24) X(30) X(40)
+#define FP_BLOCK_SIZES(X) X(1) X(3) X(5) X(6) X(9) X(10) X(12) X(15) X(18) X(20) X(24) X(30) X(40)
 
 template <typename TF>
 hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* colidx,

@@ -6,7 +6,8 @@
 // reference model classes so that the density clipping `limiteRho`, which
 // overwrites its argument, acts on the same values:
 //   ModuleProjects/Equations/Aerodynamics/src/main/java/flowpro/user/equation/
-//     Aerodynamics.java:323-370, NavierStokes.java:23-395, SpalartAllmaras.java:36-459
+//     Aerodynamics.java:323-370, NavierStokes.java:23-395, SpalartAllmaras.java:36-459,
+//     AdiabaticEulers.java:9-211 (isentropic Euler: rho, rho u, rho v with p = rho^kappa)
 //   FlowPro/src/test/java/flowpro/user/equation/TestScalarEquation.java:28-136
 #pragma once
 #include <hip/hip_runtime.h>
@@ -59,9 +60,12 @@ __device__ inline double t_abs(double a) { return fabs(a); }
 __device__ inline dual t_abs(dual a) { return a.v < 0.0 ? -a : a; }
 
 enum BoundaryType { BT_WALL = -1, BT_INLET = -2, BT_OUTLET = -3, BT_INVISCID_WALL = -4 };
-enum ModelKind { MODEL_NAVIER_STOKES = 0, MODEL_SA = 1, MODEL_SCALAR = 2 };
-// equations per model: rho, rho u, rho v, rho E (+ rho nu~ for Spalart-Allmaras); 1 for the scalar
-template <int MK> constexpr int neq_of() { return MK == MODEL_SCALAR ? 1 : (MK == MODEL_SA ? 5 : 4); }
+enum ModelKind { MODEL_NAVIER_STOKES = 0, MODEL_SA = 1, MODEL_SCALAR = 2, MODEL_ADIABATIC = 3 };
+// equations per model: rho, rho u, rho v, rho E (+ rho nu~ for Spalart-Allmaras); 1 for the scalar;
+// rho, rho u, rho v for AdiabaticEulers
+template <int MK> constexpr int neq_of() {
+    return MK == MODEL_SCALAR ? 1 : (MK == MODEL_SA ? 5 : (MK == MODEL_ADIABATIC ? 3 : 4));
+}
 
 // Derived model constants (the host computes them as Aerodynamics.init does).
 struct ModelParams {
@@ -90,6 +94,12 @@ template <class T> __device__ inline T limite_rho(T rho) {
 
 // NavierStokes.pressure / SpalartAllmaras.pressure (same formula, own P_TOL)
 template <int MK, class T> __device__ inline T pressure_m(const ModelParams& P, T* W) {
+    if constexpr (MK == MODEL_ADIABATIC) {   // AdiabaticEulers.pressure: rho^kappa, P_TOL = 0.1
+        W[0] = limite_rho(W[0]);
+        T p = t_pow(W[0], P.kapa);
+        if (value(p) < 1e-1) p = 1e-1 * t_exp((p - 1e-1) / 1e-1);
+        return p;
+    }
     constexpr double ptol = MK == MODEL_SA ? P_TOL_SA : P_TOL;
     W[0] = limite_rho(W[0]);
     T mom2 = W[1] * W[1];
@@ -115,7 +125,7 @@ template <int MK, class T> __device__ inline void convective_flux(const ModelPar
     f[0] = W[0] * V;
     f[1] = W[1] * V + p * n[0];
     f[2] = W[2] * V + p * n[1];
-    f[3] = (W[3] + p) * V;
+    if constexpr (MK != MODEL_ADIABATIC) f[3] = (W[3] + p) * V;
     if constexpr (MK == MODEL_SA) f[4] = W[4] * V;
 }
 
@@ -177,12 +187,13 @@ __device__ inline void numerical_convective_flux(const ModelParams& P, T* WL, T*
     WL[0] = limite_rho(WL[0]);
     WR[0] = limite_rho(WR[0]);
     if (TT == BT_WALL || TT == BT_INVISCID_WALL) {
-        // NavierStokes takes p(WL), SpalartAllmaras p(WR) (SpalartAllmaras.java:146)
-        T p = pressure_m<MK>(P, MK == MODEL_SA ? WR : WL);
+        // NavierStokes takes p(WL), SpalartAllmaras and AdiabaticEulers p(WR) (SpalartAllmaras.java:146,
+        // AdiabaticEulers.java:136-149)
+        T p = pressure_m<MK>(P, (MK == MODEL_SA || MK == MODEL_ADIABATIC) ? WR : WL);
         f[0] = 0.0;
         f[1] = p * n[0];
         f[2] = p * n[1];
-        f[3] = 0.0;   // p * (u_mesh . n) with u_mesh = 0
+        if constexpr (MK != MODEL_ADIABATIC) f[3] = 0.0;   // p * (u_mesh . n) with u_mesh = 0
         if constexpr (MK == MODEL_SA) f[4] = 0.0;
         return;
     }
@@ -212,11 +223,60 @@ __device__ inline void numerical_convective_flux(const ModelParams& P, T* WL, T*
     for (int j = 0; j < NE; j++) f[j] = (fL[j] + fR[j] - lam * (WR[j] - WL[j])) / 2.0;
 }
 
+// AdiabaticEulers.boundaryValue (AdiabaticEulers.java:36-95), subsonic inlet: walls add n (m . n)
+// (as SpalartAllmaras does), the subsonic outlet takes the density pOut^(1/kappa)
+template <class T> __device__ inline void boundary_value_adiabatic(const ModelParams& P, T* WL, const double* n, int TT, T* WR) {
+    const double kapa = P.kapa;
+    WL[0] = limite_rho(WL[0]);
+    if (TT == BT_WALL || TT == BT_INVISCID_WALL) {
+        T nu = WL[1] * n[0];
+        nu += WL[2] * n[1];
+        WR[0] = WL[0];
+        WR[1] = WL[1] + n[0] * nu;
+        WR[2] = WL[2] + n[1] * nu;
+    } else if (TT == BT_INLET) {
+        const double pIn0 = 1.0, rhoIn0 = 1.0;
+        T p = pressure_m<MODEL_ADIABATIC>(P, WL);
+        T mach;
+        if (value(p) > pIn0) mach = 0.0;
+        else mach = t_sqrt((2.0 / (kapa - 1.0)) * (-1.0 + t_pow(pIn0 / p, (kapa - 1.0) / kapa)));
+        T rhoIn = rhoIn0 * t_pow((1.0 + ((kapa - 1.0) / 2.0) * mach * mach), 1.0 / (1.0 - kapa));
+        T vel = mach * t_sqrt((kapa * p) / rhoIn);
+        WR[0] = rhoIn;
+        WR[1] = -rhoIn * vel * n[0];
+        WR[2] = -rhoIn * vel * n[1];
+    } else if (TT == BT_OUTLET) {
+        T av = WL[1] * WL[1];
+        av += WL[2] * WL[2];
+        av = t_sqrt(av) / WL[0];
+        T a = t_sqrt(kapa * pressure_m<MODEL_ADIABATIC>(P, WL) / WL[0]);
+        T mach = av / a;
+        if (value(mach) < 1.0) {
+            const double rhoOut = pow(P.pOut, 1.0 / kapa);
+            WR[0] = rhoOut;
+            WR[1] = rhoOut * WL[1] / WL[0];
+            WR[2] = rhoOut * WL[2] / WL[0];
+        } else {
+            WR[0] = WL[0];
+            WR[1] = WL[1];
+            WR[2] = WL[2];
+        }
+    } else {
+        WR[0] = 0.0;
+        WR[1] = 0.0;
+        WR[2] = 0.0;
+    }
+}
+
 // boundary state from the inner state (subsonic stagnation inlet, pressure outlet, walls)
 template <int MK, class T>
 __device__ inline void boundary_value(const ModelParams& P, T* WL, const double* n, int TT, const double* X, T* WR) {
     if constexpr (MK == MODEL_SCALAR) {
         WR[0] = sin(2.0 * X[0] + X[1]);
+        return;
+    }
+    if constexpr (MK == MODEL_ADIABATIC) {
+        boundary_value_adiabatic(P, WL, n, TT, WR);
         return;
     }
     constexpr int NE = neq_of<MK>();

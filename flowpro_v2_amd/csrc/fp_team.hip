@@ -535,6 +535,12 @@ __global__ void __launch_bounds__(256) k_jacobian_team(ModelParams P, DGParams G
     X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, true, true)     \
     X(MODEL_SA, 3, 5, 6, 2, true, false)                 \
     X(MODEL_SA, 6, 5, 7, 3, true, false)                 \
+    X(MODEL_ADIABATIC, 3, 3, 6, 2, false, false)         \
+    X(MODEL_ADIABATIC, 6, 3, 7, 3, false, false)         \
+    X(MODEL_ADIABATIC, 10, 3, 16, 4, false, false)       \
+    X(MODEL_ADIABATIC, 3, 3, 6, 2, false, true)          \
+    X(MODEL_ADIABATIC, 6, 3, 7, 3, false, true)          \
+    X(MODEL_ADIABATIC, 10, 3, 16, 4, false, true)        \
     X(MODEL_SCALAR, 1, 1, 1, 1, false, false)            \
     X(MODEL_SCALAR, 3, 1, 6, 2, false, false)            \
     X(MODEL_SCALAR, 6, 1, 7, 3, false, false)            \

@@ -18,7 +18,7 @@ import numpy as np
 from . import equation, native, quadrature
 from .cases import Case
 
-PRECONDITIONERS = {"none": 0, "blockjacobi": 1, "blockilu0": 2}
+PRECONDITIONERS = {"none": 0, "blockjacobi": 1, "blockilu0": 2, "ilu0": 3}
 MAX_VALUE = 1.7976931348623157e308
 
 

@@ -56,7 +56,7 @@ typedef struct fp_case {
     const double* face_xi;        /* [n_face]; 3D: [n_face][2] reference-triangle coordinates */
     const double* face_w;         /* [n_face] */
     int model;                    /* 0 NavierStokes (Euler when viscous == 0), 1 SpalartAllmaras,
-                                     2 TestScalarEquation */
+                                     2 TestScalarEquation, 3 AdiabaticEulers (subsonic inlet) */
     int viscous;
     double kappa;                 /* Aerodynamics.init derived constants */
     double p_out;                 /* outlet pressure / pRef */
@@ -68,7 +68,8 @@ typedef struct fp_case {
     double beta0;                 /* DDG constant (DGpure, default 2) */
     double penalty;               /* interior penalty (DGpure, default 0) */
     int ordering;                 /* 0 none, 1 rcm (ElementOrdering) */
-    int preconditioner;           /* 0 none, 1 blockjacobi, 2 blockilu0 */
+    int preconditioner;           /* 0 none, 1 blockjacobi, 2 blockilu0, 3 ilu0 (the same factorisation
+                                     as blockilu0 on the pattern of dense element blocks) */
     int precond_bits;             /* 64 or 32: precision of the preconditioner factors and sweeps */
     int gmres_restart;            /* 30 */
     int gmres_cycles;             /* 5 */

@@ -87,7 +87,7 @@ def lib():
     return _lib
 
 
-_PREC = {"none": 0, "blockjacobi": 1, "blockilu0": 2}
+_PREC = {"none": 0, "blockjacobi": 1, "blockilu0": 2, "ilu0": 3}
 _MODEL = {"NavierStokes": 0, "SpalartAllmaras": 1, "TestScalarEquation": 2, "AdiabaticEulers": 3}
 # SpalartAllmaras constants of the reference's template (Aerodynamics/src/main/resources/templates/
 # SpalartAllmaras.txt); parameters.txt may override each

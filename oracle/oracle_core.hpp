@@ -72,7 +72,7 @@ struct Props {
     int viscous;          // isFlowViscous
     int order;            // element order k = p + 1 (global "order")
     int time_order;       // BDF 1 or 2
-    int preconditioner;   // 0 none, 1 blockjacobi, 2 blockilu0
+    int preconditioner;   // 0 none, 1 blockjacobi, 2 blockilu0, 3 ilu0 (point ILU(0) on the CRS matrix)
     int ordering;         // 0 none, 1 rcm
     int newton_iters;
     int steps;
@@ -189,6 +189,8 @@ struct Case {
     std::vector<int> blkRowStart;   // per element: offsets into H, see build_crs
     // block ILU0 (C/LinearSolvers/preconditioners/BlockILU0.java)
     std::vector<int> ilu_elems, ilu_rank, ilu_diag;
+    std::vector<double> pilu;          // ILU0 factors on the CRS pattern
+    std::vector<int> diagind;          // CRS position of each row's diagonal entry
     std::vector<std::vector<int>> ilu_cols;
     std::vector<std::vector<std::vector<double>>> ilu_blocks;   // [i][kk] -> n x m row-major
     std::vector<std::vector<double>> ilu_dinv;

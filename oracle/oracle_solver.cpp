@@ -248,8 +248,58 @@ static void bj_apply(Case& c, double* x, const double* b) {
     }
 }
 
+// ------------------------------------------------------------- ILU0 (point)
+// C/LinearSolvers/preconditioners/ILU0.java:32-100 on the CRS matrix in DOF order, no pivoting;
+// SparseMatrix.findDiagonalIndices gives each row's diagonal position
+static bool pilu_factor(Case& c) {
+    const int n = c.dofs;
+    const int* IA = c.Icrs.data();
+    const int* JA = c.Jcrs.data();
+    if (c.diagind.empty()) {
+        c.diagind.assign(n, 0);
+        for (int r = 0; r < n; r++)
+            for (int j = IA[r]; j < IA[r + 1]; j++)
+                if (JA[j] == r) { c.diagind[r] = j; break; }
+    }
+    c.pilu = c.H;
+    double* ilu = c.pilu.data();
+    const int* diag = c.diagind.data();
+    for (int k = 1; k < n; ++k) {
+        for (int i = IA[k]; i < diag[k]; ++i) {
+            const int index = JA[i];
+            const double LUii = ilu[diag[index]];
+            if (LUii == 0) return false;   // the reference throws ArithmeticException
+            const double LUki = (ilu[i] /= LUii);
+            for (int j = diag[index] + 1, l = IA[k] + 1; j < IA[index + 1]; ++j) {
+                while (l < IA[k + 1] && JA[l] < JA[j]) l++;
+                if (l < IA[k + 1] && JA[l] == JA[j]) ilu[l] -= LUki * ilu[j];
+            }
+        }
+    }
+    return true;
+}
+static void pilu_apply(const Case& c, double* x, const double* b) {
+    const int n = c.dofs;
+    const int* IA = c.Icrs.data();
+    const int* JA = c.Jcrs.data();
+    const int* diag = c.diagind.data();
+    const double* ilu = c.pilu.data();
+    std::vector<double> y(n);
+    for (int i = 0; i < n; i++) {
+        double sum = 0;
+        for (int j = IA[i]; j < diag[i]; j++) sum = sum + ilu[j] * y[JA[j]];
+        y[i] = b[i] - sum;
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double sum = 0;
+        for (int j = diag[i] + 1; j < IA[i + 1]; j++) sum = sum + ilu[j] * x[JA[j]];
+        x[i] = (y[i] - sum) / ilu[diag[i]];
+    }
+}
+
 static void prec_apply(Case& c, double* x, const double* b) {
-    if (c.P.preconditioner == 2) ilu_apply(c, x, b);
+    if (c.P.preconditioner == 3) pilu_apply(c, x, b);
+    else if (c.P.preconditioner == 2) ilu_apply(c, x, b);
     else if (c.P.preconditioner == 1) bj_apply(c, x, b);
     else std::memcpy(x, b, sizeof(double) * c.dofs);
 }
@@ -362,7 +412,9 @@ bool linear_solve(Case& c, double* x) {
     std::vector<double> b(c.dofs);
     for (const auto& e : c.E)   // updateB (:141-152)
         for (int i = 0; i < (int)e.RHS.size(); i++) b[e.gIndex0 + i] = e.RHS[i];
-    if (c.P.preconditioner == 2) {
+    if (c.P.preconditioner == 3) {
+        if (!pilu_factor(c)) return false;
+    } else if (c.P.preconditioner == 2) {
         if (!ilu_factor(c)) return false;
     } else if (c.P.preconditioner == 1) {
         c.bj_work.assign(c.nElem, {});

@@ -43,7 +43,7 @@ def test_convergence_rates_match_reference(p, k):
 
 # ---------------------------------------------------------------- LinearSolverTest
 @pytest.mark.parametrize("ordering", ["rcm", "none"])
-@pytest.mark.parametrize("prec", ["blockilu0", "blockjacobi", "none"])
+@pytest.mark.parametrize("prec", ["ilu0", "blockilu0", "blockjacobi", "none"])
 def test_gmres_matches_direct_solution(ordering, prec):
     """T/core/LinearSolverTest.java:27-62: 4x4 mesh, DGpure order 2, k = 0.1, Random(2), assemble(0.05, -1)."""
     o = O.Oracle(cases.square_mesh(4, 1.0, -2), scalar_props(2, 0.1, iterativeSolverTol=1e-12,
@@ -56,6 +56,28 @@ def test_gmres_matches_direct_solution(ordering, prec):
     r, _ = o.rhs()
     xd = np.linalg.solve(A, o.to_dof(r))
     assert ok and np.abs(x - xd).max() / np.abs(xd).max() < 1e-8
+
+
+@pytest.mark.parametrize("props", [
+    lambda: cases.ns_props(order=3, reynolds=200.0),
+    lambda: cases.euler_props(model="AdiabaticEulers", order=3),
+])
+def test_point_ilu0_equals_block_ilu0(props):
+    """ILU0 (point, no pivoting, on the CRS pattern) and BlockILU0 (block pattern, inverted
+    diagonal blocks) are the same incomplete factorisation: the pattern is made of dense
+    element blocks, so the point elimination fills exactly the blocks the block elimination
+    updates.  The first GMRES(1) direction M^-1 b agrees to rounding (1e-10)."""
+    case = cases.channel_bump(8, 3)
+    xs = []
+    for prec in ("ilu0", "blockilu0"):
+        o = O.Oracle(case, dict(props(), preconditioner=prec, gmresRestart=1, gmresCycles=1,
+                                iterativeSolverTol=1e-30, jacobian="exact"))
+        o.set_w(cases.perturbed_state(o.get_w()[0][:: o.nb], case.n_elem, o.nb, 0.02, 4))
+        o.limiter()
+        o.assemble(0.05, -1.0)
+        _, x, _, _ = o.linear_solve()
+        xs.append(x)
+    assert np.abs(xs[0] - xs[1]).max() <= 1e-10 * np.abs(xs[1]).max()
 
 
 def test_block_ilu_on_block_diagonal_is_exact():
