@@ -89,6 +89,7 @@ struct fp_ctx {
     DGParams dg{};
     AvParams avp{};
     double *av = nullptr, *avold = nullptr, *trunc = nullptr;   // damping state, sensor memory, truncation
+    std::vector<double> h_isref;    // Is_j = int phi_j / |K| (the element mean of W is sum_j W_j Is_j)
     int precond = 2, bits = 64, m = 30, cycles = 5, positivity = 1;
     double tol = 1e-2;
     // host copies
@@ -606,6 +607,7 @@ static int setup_tri(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
            o_fx = put(fx), o_m = put(mref), o_mi = put(mrefinv);
     HIPCHK(c, upload(&c->tab, tab));
     HIPCHK(c, upload(&c->isref, isref));
+    c->h_isref = isref;
     c->tb = Tables{c->tab + o_vb, c->tab + o_vd, c->tab + o_vw, c->tab + o_vx, c->tab + o_fb,
                    c->tab + o_fd, c->tab + o_fw, c->tab + o_fx, c->tab + o_m,  c->tab + o_mi};
 
@@ -748,6 +750,7 @@ static int setup_tet(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
            o_mi = put(mrefinv);
     HIPCHK(c, upload(&c->tab, tab));
     HIPCHK(c, upload(&c->isref, isref));
+    c->h_isref = isref;
     c->ttb = TetTables{c->tab + o_vb, c->tab + o_vd, c->tab + o_vw, c->tab + o_fb, c->tab + o_fw, c->tab + o_fx, c->tab + o_m,
                        c->tab + o_mi};
 
@@ -1640,6 +1643,21 @@ extern "C" int fp_export_artificial_viscosity(fp_ctx* c, double* eps, double* da
         eps[e] = h[(size_t)r * (1 + ne)];
         for (int m = 0; m < ne; m++) damp_inner[e * ne + m] = h[(size_t)r * (1 + ne) + 1 + m];
     }
+    return FP_OK;
+}
+
+// Mesh.getAvgW (Mesh.java:263-271, Element.calculateAvgW): avg[e][m] = sum_j W[e][m Nb + j] Is_j
+extern "C" int fp_export_averages(fp_ctx* c, double* avg) {
+    if (!c || !avg) return FP_EINVAL;
+    std::vector<double> W((size_t)c->dofs);
+    int rc = from_device(c, c->W, W.data());
+    if (rc) return rc;
+    for (int e = 0; e < c->nElem; e++)
+        for (int m = 0; m < c->NEQ; m++) {
+            double s = 0;
+            for (int j = 0; j < c->NB; j++) s = s + W[(size_t)e * c->B + m * c->NB + j] * c->h_isref[j];
+            avg[(size_t)e * c->NEQ + m] = s;
+        }
     return FP_OK;
 }
 

@@ -24,7 +24,7 @@ EXPORTS = ["fp_neighbor_matrix", "fp_rcm_order", "fp_create", "fp_destroy", "fp_
            "fp_assemble", "fp_linear_solve", "fp_update", "fp_export_residual", "fp_export_jacobian",
            "fp_export_solution", "fp_jv", "fp_jtv", "fp_time_kernel", "fp_stream", "fp_linear_solve_jfnk",
            "fp_functional", "fp_adjoint_solve", "fp_wall_distance", "fp_neighbor_matrix_tet", "fp_rcm_order_tet",
-           "fp_limiter", "fp_export_artificial_viscosity"]
+           "fp_limiter", "fp_export_artificial_viscosity", "fp_export_averages"]
 
 
 class FpCase(C.Structure):
@@ -97,6 +97,7 @@ def lib() -> C.CDLL:
         "fp_rcm_order_tet": (C.c_int, [C.c_int, ip, ip]),
         "fp_limiter": (C.c_int, [vp, C.c_int]),
         "fp_export_artificial_viscosity": (C.c_int, [vp, dp, dp]),
+        "fp_export_averages": (C.c_int, [vp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -177,6 +178,12 @@ class DeviceCase:
         self._check(native.lib().fp_export_artificial_viscosity(self.h, native.dptr(eps), native.dptr(di)),
                     "fp_export_artificial_viscosity")
         return eps, di
+
+    def averages(self) -> np.ndarray:
+        """Element means of the state (Mesh.getAvgW), [n_elem, nEqs] in mesh order."""
+        out = np.zeros((self.n_elem, self.neq))
+        self._check(native.lib().fp_export_averages(self.h, native.dptr(out)), "fp_export_averages")
+        return out
 
     def linear_solve(self):
         it, err = C.c_int(), C.c_double()
@@ -350,9 +357,15 @@ class StepRecord:
 class GpuImplicitSolver:
     """LocalImplicitSolver.solve with the Newton step on the device."""
 
-    def __init__(self, dev: DeviceCase, props: dict):
+    def __init__(self, dev: DeviceCase, props: dict, simulation_path: str | None = None):
+        """simulation_path: a FlowPro simulation directory; residuum.txt gets a line per step
+        (appendResiduum) and state.txt, W.txt, We.txt, ... are written every saveRate steps
+        (saveData), in the Java tool chain's formats."""
         self.dev = dev
         self.props = props
+        self.simulation_path = simulation_path
+        self.save_rate = int(props.get("saveRate", 2 ** 31 - 1))   # Parameters.saveRate
+        self.execution_ms = 0.0
         if "dt" in props and "CFL" not in props:
             self.state = State(cfl=0.0, fixed_dt=float(props["dt"]))
         else:
@@ -367,7 +380,21 @@ class GpuImplicitSolver:
         self.newton_history: list[float] = []
         self.linear_failures = 0
 
+    def save(self) -> None:
+        """LocalImplicitSolver.saveData into the simulation directory."""
+        from . import results_io
+        results_io.save_data(self.simulation_path, self.dev, self.state, int(self.execution_ms))
+
+    def continue_from(self) -> None:
+        """continueComputation: State.load and the initial W from We.txt (W.txt when the order
+        changed) of the simulation directory."""
+        from . import results_io
+        results_io.load_initial_state(self.simulation_path, self.dev, self.state)
+
     def solve(self, steps: int) -> list[StepRecord]:
+        import time
+        from . import results_io
+        t_start = time.perf_counter() - self.execution_ms / 1000.0
         st, dev = self.state, self.dev
         dto = -1.0
         converges = True
@@ -410,10 +437,16 @@ class GpuImplicitSolver:
             if converges:
                 if not math.isfinite(st.residuum):
                     break
+                self.execution_ms = (time.perf_counter() - t_start) * 1000.0
                 st.t += dt
                 dto = dt
+                if self.simulation_path:
+                    results_io.append_residuum(os.path.join(self.simulation_path, "residuum.txt"), st.residuum, st.t,
+                                               float(int(self.execution_ms)), st.equation_residuum)
                 dev.next_level()
                 self.history.append(StepRecord(st.steps, st.residuum, dt, st.current_cfl, newton_done, iters, limited))
+                if self.simulation_path and st.steps % self.save_rate == 0:
+                    self.save()
             st.steps += 1
         st.steps -= 1
         return self.history

@@ -149,6 +149,8 @@ int fp_export_residual(fp_ctx* ctx, double* R, double* RHS);
 int fp_export_jacobian(fp_ctx* ctx, double* diag, double* neigh);
 /* eps[n_elems] and damp_inner[n_elems][nEqs] of the last fp_limiter call, mesh order */
 int fp_export_artificial_viscosity(fp_ctx* ctx, double* eps, double* damp_inner);
+/* element means of the state, avg[n_elems][nEqs] in mesh order (Mesh.getAvgW: the W.txt content) */
+int fp_export_averages(fp_ctx* ctx, double* avg);
 /* the last Newton correction x in DOF numbering */
 int fp_export_solution(fp_ctx* ctx, double* x);
 /* matrix-free (a1 M - dR/dW) v and its transpose, element layout in and out */
