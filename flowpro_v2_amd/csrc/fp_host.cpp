@@ -32,7 +32,7 @@ hipError_t launch_jacobian(const ElemLaunch&, hipStream_t, const ModelParams&, c
 hipError_t launch_residual_team(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
                                 const double*, const double*, const double*, double, double, double, double*, double*, double*);
 hipError_t launch_jacobian_team(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
-                                const double*, double, double*);
+                                const double*, double, double*, float*);
 hipError_t launch_jvp(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
                       const double*, const double*, double, double*);
 hipError_t launch_size_over_lambda(const ElemLaunch&, hipStream_t, const ModelParams&, const Tables&, const Mesh&,
@@ -106,6 +106,7 @@ struct fp_ctx {
     int *ftptr = nullptr, *ftask = nullptr;   // per-row task lists of the ILU0 factor
     int *meta_f = nullptr, *meta_b = nullptr; // per-row records of the two sweeps, in level order
     bool ilu_rows = false;                    // factor with the register-resident row kernel
+    bool f_fresh = false;                     // F holds the FP32 copy of the current matrix (written by the Jacobian)
     double* val = nullptr;        // FP64 matrix (nnzb * B * B)
     void* F = nullptr;            // preconditioner factors (FP64 or FP32)
     void *iy = nullptr, *ixs = nullptr;
@@ -145,8 +146,12 @@ static hipError_t run_residual(fp_ctx* c, const double* a) {
                                 c->enorm);
 }
 static hipError_t run_jacobian(fp_ctx* c, double a1) {
+    c->f_fresh = false;
     if (c->dim == 3) return launch_jacobian_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, c->val, nullptr, nullptr, 0);
-    return launch_jacobian_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, c->val);
+    // with FP32 block-ILU0 factors the Jacobian kernel also writes the factorisation's starting copy
+    float* f32 = (c->precond == 2 && c->bits == 32) ? (float*)c->F : nullptr;
+    c->f_fresh = f32 != nullptr;
+    return launch_jacobian_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, c->val, f32);
 }
 static hipError_t run_jvp(fp_ctx* c, const double* v, double a1, double* out) {
     if (c->dim == 3) return launch_jvp_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, v, a1, out);
@@ -1299,10 +1304,6 @@ template <typename TF> static int ilu_level_launches(fp_ctx* c) {
     }
     return FP_OK;
 }
-template <typename TF> static int ilu_factor_launches(fp_ctx* c) {
-    int rc = ilu_copy_launch<TF>(c);
-    return rc ? rc : ilu_level_launches<TF>(c);
-}
 template <typename TF> static int ilu_apply_launches(fp_ctx* c) {
     const TF* F = (const TF*)c->F;
     for (size_t l = 0; l + 1 < c->lvl_f_ptr.size(); l++) {
@@ -1334,7 +1335,14 @@ template <typename TF> static int run_captured(fp_ctx* c, hipGraphExec_t* exec, 
 }
 template <typename TF> static int ilu_factor_t(fp_ctx* c) {
     HIPCHK(c, hipMemsetAsync(c->istat, 0, sizeof(int), c->s));
-    return run_captured<TF>(c, &c->g_factor, ilu_factor_launches<TF>);
+    // the factorisation overwrites F: it starts from a copy of the assembled values, which the
+    // Jacobian kernel has already written in FP32 after an assembly
+    if (!c->f_fresh) {
+        int rc = ilu_copy_launch<TF>(c);
+        if (rc) return rc;
+    }
+    c->f_fresh = false;
+    return run_captured<TF>(c, &c->g_factor, ilu_level_launches<TF>);
 }
 template <typename TF> static int ilu_apply_t(fp_ctx* c, double* out, const double* in) {
     HIPCHK(c, launch_set_io(c->s, c->io, in, out));
