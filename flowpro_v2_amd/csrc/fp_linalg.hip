@@ -1029,8 +1029,9 @@ hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nr
     return hipErrorInvalidValue;
 }
 
-#define FP_ROWREG_SIZES(X) X(24) X(30) X(40)
-constexpr int ILU_ROW_MAXR = 4;
+// (block size, row blocks held in registers, task waves): the triangles' rows have at most 4
+// blocks, the tetrahedra's (B = 20 at P1) 5
+#define FP_ROWREG_SIZES(X) X(20, 5, 4) X(24, 4, 8) X(30, 4, 8) X(40, 4, 8)
 #ifndef ILU_ROW_WAVES
 #define ILU_ROW_WAVES 8
 #endif
@@ -1038,10 +1039,10 @@ constexpr int ILU_ROW_MAXR = 4;
 template <typename TF>
 hipError_t launch_ilu_factor_rows(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* diagpos,
                                   const int* tptr, const int* tasks, TF* F, int* status) {
-#define X(BS)                                                                                                          \
+#define X(BS, MR, NWV)                                                                                                 \
     if (B == BS) {                                                                                                     \
-        hipLaunchKernelGGL((k_ilu_factor_rows<TF, BS, ILU_ROW_MAXR, ILU_ROW_WAVES>), dim3(nrows), dim3(64 * ILU_ROW_WAVES), 0, \
-                           s, rows, nrows, rowptr,                                                                     \
+        constexpr int NW_ = NWV == 8 ? ILU_ROW_WAVES : NWV;                                                            \
+        hipLaunchKernelGGL((k_ilu_factor_rows<TF, BS, MR, NW_>), dim3(nrows), dim3(64 * NW_), 0, s, rows, nrows, rowptr, \
                            diagpos, tptr, reinterpret_cast<const int2*>(tasks), F, status);                            \
         return hipGetLastError();                                                                                      \
     }
@@ -1051,7 +1052,7 @@ hipError_t launch_ilu_factor_rows(hipStream_t s, int B, const int* rows, int nro
 }
 
 bool ilu_factor_rows_supported(int B, int max_row_blocks) {
-#define X(BS) if (B == BS) return max_row_blocks <= ILU_ROW_MAXR;
+#define X(BS, MR, NWV) if (B == BS) return max_row_blocks <= MR;
     FP_ROWREG_SIZES(X)
 #undef X
     return false;

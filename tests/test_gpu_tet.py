@@ -159,6 +159,27 @@ def test_tet_linear_solve_matches_dense(precond):
     dev.close()
 
 
+@pytest.mark.parametrize("bits", [64, 32])
+def test_tet_row_factor_kernel_matches_level_kernel(bits, monkeypatch):
+    """The register-resident row factor kernel at B = 20 with five blocks per row (P1 tetrahedra)
+    against the one-wave level kernel, which the triangle cases pin against the oracle's
+    BlockILU0: the first GMRES(1) direction M^-1 b agrees (1e-12 FP64, 1e-6 FP32)."""
+    xs = []
+    for rows in ("1", "0"):
+        monkeypatch.setenv("FLOWPRO_ILU_ROWS", rows)
+        case = cases.kuhn_cube(3, 0.05, 4)
+        props = cases.euler3d_props(order=2, gmresRestart=1, gmresCycles=1, iterativeSolverTol=1e-14)
+        dev = solver.DeviceCase(case, props, precond_bits=bits)
+        dev.set_state(_state(case, dev.nb, dev.initial_state()[0][:: dev.nb]))
+        dev.assemble(0.05, -1.0, 1, True)
+        rc, it, err = dev.linear_solve()
+        assert rc in (0, 1) and it == 1
+        xs.append(dev.solution())
+        dev.close()
+    tol = 1e-12 if bits == 64 else 1e-6
+    assert np.abs(xs[0] - xs[1]).max() <= tol * np.abs(xs[1]).max()
+
+
 @pytest.mark.parametrize("order", [1, 2])
 def test_tet_size_over_lambda(order):
     case, o, dev, W = _setup(order)
