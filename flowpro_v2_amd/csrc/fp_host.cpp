@@ -47,6 +47,8 @@ hipError_t launch_jacobian_tet(const ElemLaunch&, hipStream_t, const ModelParams
                                double, double*, const double*, double*, int);
 hipError_t launch_jvp_tet(const ElemLaunch&, hipStream_t, const ModelParams&, const TetTables&, const TetMesh&, const double*,
                           const double*, double, double*);
+hipError_t launch_jacobian_tet_team(const ElemLaunch&, hipStream_t, const ModelParams&, const TetTables&, const TetMesh&, const double*,
+                                    double, double*, float*);
 hipError_t launch_size_over_lambda_tet(const ElemLaunch&, hipStream_t, const ModelParams&, const TetMesh&, const double*,
                                        const double*, const double*, double*);
 hipError_t launch_positivity_tet(const ElemLaunch&, hipStream_t, const TetTables&, const TetMesh&, double*, const double*, int*);
@@ -146,11 +148,10 @@ static hipError_t run_residual(fp_ctx* c, const double* a) {
                                 c->enorm);
 }
 static hipError_t run_jacobian(fp_ctx* c, double a1) {
-    c->f_fresh = false;
-    if (c->dim == 3) return launch_jacobian_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, c->val, nullptr, nullptr, 0);
     // with FP32 block-ILU0 factors the Jacobian kernel also writes the factorisation's starting copy
     float* f32 = (c->precond == 2 && c->bits == 32) ? (float*)c->F : nullptr;
     c->f_fresh = f32 != nullptr;
+    if (c->dim == 3) return launch_jacobian_tet_team(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, c->val, f32);
     return launch_jacobian_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, c->val, f32);
 }
 static hipError_t run_jvp(fp_ctx* c, const double* v, double a1, double* out) {

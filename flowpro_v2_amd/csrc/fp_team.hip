@@ -436,94 +436,84 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
         }
     }
     __syncthreads();
-    // phase 2: thread (element, column i = (mp, jp))
+    // phase 2: thread (element, column i = (mp, jp), equation m): the NB rows of equation m in
+    // column i of the diagonal block and of the three neighbour blocks.  One equation per thread
+    // keeps the accumulators small and gives the contraction NEQ times the threads of a
+    // column-per-thread split (the element count per workgroup is small: the LDS holds the
+    // point derivatives); consecutive threads write consecutive rows of a column.
     const size_t BB = (size_t)B * B;
-    for (int q = tid; q < nel * B; q += nt) {
-        const int el = q / B, i = q % B, mp = i / NB, jp = i % NB, e = e0 + el;
+    for (int q = tid; q < nel * B * NEQ; q += nt) {
+        const int el = q / (B * NEQ), r2 = q % (B * NEQ), i = r2 / NEQ, m = r2 % NEQ;
+        const int mp = i / NB, jp = i % NB, e = e0 + el;
         const double* E = eb + el * STRIDE;
-        double acc[B];
+        double acc[NB];
 #pragma unroll
-        for (int o = 0; o < B; o++) acc[o] = 0.0;
+        for (int j = 0; j < NB; j++) acc[j] = 0.0;
+#pragma unroll 1
         for (int k = 0; k < 3; k++) {
             const int blkn = ms.blkn[e * 3 + k];
             const int kr = ms.nbrf[e * 3 + k];
-            double accR[B];
+            double accR[NB];
 #pragma unroll
-            for (int o = 0; o < B; o++) accR[o] = 0.0;
+            for (int j = 0; j < NB; j++) accR[j] = 0.0;
+#pragma unroll 1
             for (int p = 0; p < NQF; p++) {
                 const int f = k * NQF + p;
-                // t_m = sum_s d g_m / d u_(s, mp) * psi_s(jp)
-                const double psi0 = tab[LY::T_FB + f * NB + jp];
-                double t[NEQ];
-#pragma unroll
-                for (int m = 0; m < NEQ; m++) t[m] = E[O_DF + (f * NIN + mp) * NEQ + m] * psi0;
+                // t = sum_s d g_m / d u_(s, mp) * psi_s(jp)
+                double t = E[O_DF + (f * NIN + mp) * NEQ + m] * tab[LY::T_FB + f * NB + jp];
                 if constexpr (VISC || AV) {
                     const double psx = E[LY::E_GF + (f * NB + jp) * 2], psy = E[LY::E_GF + (f * NB + jp) * 2 + 1];
-#pragma unroll
-                    for (int m = 0; m < NEQ; m++)
-                        t[m] += E[O_DF + (f * NIN + NEQ + mp) * NEQ + m] * psx + E[O_DF + (f * NIN + 2 * NEQ + mp) * NEQ + m] * psy;
+                    t += E[O_DF + (f * NIN + NEQ + mp) * NEQ + m] * psx + E[O_DF + (f * NIN + 2 * NEQ + mp) * NEQ + m] * psy;
                 }
 #pragma unroll
-                for (int m = 0; m < NEQ; m++)
-#pragma unroll
-                    for (int j = 0; j < NB; j++) acc[m * NB + j] += tab[LY::T_FB + f * NB + j] * t[m];
+                for (int j = 0; j < NB; j++) acc[j] += tab[LY::T_FB + f * NB + j] * t;
                 if (blkn >= 0) {
                     const double* phR = &tab[LY::T_FB + (kr * NQF + (NQF - 1 - p)) * NB];
 #pragma unroll
-                    for (int m = 0; m < NEQ; m++)
-#pragma unroll
-                        for (int j = 0; j < NB; j++) accR[m * NB + j] += phR[j] * t[m];
+                    for (int j = 0; j < NB; j++) accR[j] += phR[j] * t;
                 }
             }
             if (blkn >= 0) {
-                BT* col = blocks + (size_t)blkn * BB + (size_t)i * B;
+                const size_t o0 = (size_t)blkn * BB + (size_t)i * B + (size_t)m * NB;
 #pragma unroll
-                for (int o = 0; o < B; o++) col[o] = (BT)accR[o];
+                for (int j = 0; j < NB; j++) blocks[o0 + j] = (BT)accR[j];
                 if (fblocks) {   // the FP32 copy the block-ILU0 factorisation starts from
-                    float* fcol = fblocks + (size_t)blkn * BB + (size_t)i * B;
 #pragma unroll
-                    for (int o = 0; o < B; o++) fcol[o] = (float)accR[o];
+                    for (int j = 0; j < NB; j++) fblocks[o0 + j] = (float)accR[j];
                 }
             }
         }
         if constexpr (NB > 1) {
+#pragma unroll 1
             for (int qv = 0; qv < NQV; qv++) {
                 const double psi0 = tab[LY::T_VB + qv * NB + jp];
-                const double psx = E[LY::E_GV + (qv * NB + jp) * 2], psy = E[LY::E_GV + (qv * NB + jp) * 2 + 1];
-                double tx[NEQ], ty[NEQ], ts[NEQ];
-#pragma unroll
-                for (int m = 0; m < NEQ; m++) {
-                    const double* d0 = &E[O_DV + ((qv * NIN + mp) * NVS) * NEQ];
-                    tx[m] = d0[m] * psi0;
-                    ty[m] = d0[NEQ + m] * psi0;
-                    if constexpr (NVS == 3) ts[m] = d0[2 * NEQ + m] * psi0;
-                    if constexpr (VISC || AV) {
-                        const double* d1 = &E[O_DV + ((qv * NIN + NEQ + mp) * NVS) * NEQ];
-                        const double* d2 = &E[O_DV + ((qv * NIN + 2 * NEQ + mp) * NVS) * NEQ];
-                        tx[m] += d1[m] * psx + d2[m] * psy;
-                        ty[m] += d1[NEQ + m] * psx + d2[NEQ + m] * psy;
-                        if constexpr (NVS == 3) ts[m] += d1[2 * NEQ + m] * psx + d2[2 * NEQ + m] * psy;
-                    }
+                const double* d0 = &E[O_DV + ((qv * NIN + mp) * NVS) * NEQ];
+                double tx = d0[m] * psi0, ty = d0[NEQ + m] * psi0, ts = 0.0;
+                if constexpr (NVS == 3) ts = d0[2 * NEQ + m] * psi0;
+                if constexpr (VISC || AV) {
+                    const double psx = E[LY::E_GV + (qv * NB + jp) * 2], psy = E[LY::E_GV + (qv * NB + jp) * 2 + 1];
+                    const double* d1 = &E[O_DV + ((qv * NIN + NEQ + mp) * NVS) * NEQ];
+                    const double* d2 = &E[O_DV + ((qv * NIN + 2 * NEQ + mp) * NVS) * NEQ];
+                    tx += d1[m] * psx + d2[m] * psy;
+                    ty += d1[NEQ + m] * psx + d2[NEQ + m] * psy;
+                    if constexpr (NVS == 3) ts += d1[2 * NEQ + m] * psx + d2[2 * NEQ + m] * psy;
                 }
 #pragma unroll
                 for (int j = 0; j < NB; j++) {
                     const double gx = E[LY::E_GV + (qv * NB + j) * 2], gy = E[LY::E_GV + (qv * NB + j) * 2 + 1];
-#pragma unroll
-                    for (int m = 0; m < NEQ; m++) {
-                        acc[m * NB + j] += tx[m] * gx + ty[m] * gy;
-                        if constexpr (NVS == 3) acc[m * NB + j] += ts[m] * tab[LY::T_VB + qv * NB + j];
-                    }
+                    acc[j] += tx * gx + ty * gy;
+                    if constexpr (NVS == 3) acc[j] += ts * tab[LY::T_VB + qv * NB + j];
                 }
             }
         }
         const double jac = ms.geo[(size_t)e * GEO + 4];
-        BT* col = blocks + (size_t)ms.blkd[e] * BB + (size_t)i * B;
+        const size_t o0 = (size_t)ms.blkd[e] * BB + (size_t)i * B + (size_t)m * NB;
 #pragma unroll
-        for (int o = 0; o < B; o++) {
-            double v = -acc[o];
-            if (o / NB == mp) v += a1 * (jac * tab[LY::T_M + jp * NB + (o % NB)]);
-            col[o] = (BT)v;
-            if (fblocks) fblocks[(size_t)ms.blkd[e] * BB + (size_t)i * B + o] = (float)v;
+        for (int j = 0; j < NB; j++) {
+            double v = -acc[j];
+            if (m == mp) v += a1 * (jac * tab[LY::T_M + jp * NB + j]);
+            blocks[o0 + j] = (BT)v;
+            if (fblocks) fblocks[o0 + j] = (float)v;
         }
     }
 }

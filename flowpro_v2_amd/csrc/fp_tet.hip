@@ -267,6 +267,211 @@ __global__ void __launch_bounds__(128) k_jacobian_tet(ModelParams P, TetTables t
     }
 }
 
+// Team form of the exact Jacobian (assembly mode): a workgroup owns EPB elements.  Phase 1
+// interpolates the own and the neighbour state at every face and volume point; phase 2 takes the
+// derivative of each point's integrand (face: g = -fn; volume: the fluxes in x, y, z) with respect
+// to each of the five own-state values with one dual pass per (point, value); phase 3 thread
+// (element, column dir) contracts those with the test functions into column dir of the diagonal
+// block (-dK/dW + a1 M) and of the four neighbour blocks (the neighbour-side integrand, KR).
+template <int NB, int NQV, int NQF>
+struct TetTeam {
+    static constexpr int NE = 5, B = NB * NE, NQFT = 4 * NQF, NQ = NQV + NQFT;
+    // per element (doubles): own coefficients, point states, neighbour basis at the face points,
+    // integrand derivatives
+    static constexpr int E_W = 0;
+    static constexpr int E_SV = E_W + B;                       // [NQV][NE]
+    static constexpr int E_SF = E_SV + NQV * NE;               // [NQFT][2 NE] WL, WR
+    static constexpr int E_PR = E_SF + NQFT * 2 * NE;          // [NQFT][NB] neighbour basis
+    static constexpr int E_DV = E_PR + NQFT * NB;              // [NQV][NE inputs][3 NE]
+    static constexpr int E_DF = E_DV + NQV * NE * 3 * NE;      // [NQFT][NE inputs][NE]
+    static constexpr int PER = E_DF + NQFT * NE * NE;
+    static constexpr int EPB0 = (56 * 1024 / 8) / PER;
+    static constexpr int EPB = EPB0 < 1 ? 1 : (EPB0 > 8 ? 8 : EPB0);
+};
+
+template <int NB, int NQV, int NQF>
+__global__ void __launch_bounds__(256) k_jacobian_tet_team(ModelParams P, TetTables tb, TetMesh mt, const double* __restrict__ W,
+                                                           double a1, double* __restrict__ blocks, float* __restrict__ fblocks) {
+    using LY = TetTeam<NB, NQV, NQF>;
+    constexpr int NE = LY::NE, B = LY::B, NQ = LY::NQ, EPB = LY::EPB;
+    __shared__ double eb[EPB * LY::PER];
+    const int e0 = blockIdx.x * EPB;
+    const int nel = min(EPB, mt.nElem - e0);
+    if (nel <= 0) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int q = tid; q < nel * B; q += nt) {
+        const int el = q / B, i = q % B;
+        eb[el * LY::PER + LY::E_W + i] = W[(size_t)(e0 + el) * B + i];
+    }
+    __syncthreads();
+    // phase 1: states at the points
+    for (int q = tid; q < nel * NQ; q += nt) {
+        const int el = q / NQ, qp = q % NQ, e = e0 + el;
+        double* E = eb + el * LY::PER;
+        const double* w = E + LY::E_W;
+        if (qp < NQV) {
+            const double* phi = tb.vb + qp * NB;
+#pragma unroll
+            for (int m = 0; m < NE; m++) {
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; j++) s += w[m * NB + j] * phi[j];
+                E[LY::E_SV + qp * NE + m] = s;
+            }
+        } else {
+            const int f = qp - NQV, k = f / NQF;
+            const double* phiL = tb.fb + f * NB;
+            double* st = E + LY::E_SF + f * 2 * NE;
+#pragma unroll
+            for (int m = 0; m < NE; m++) {
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; j++) s += w[m * NB + j] * phiL[j];
+                st[m] = s;
+            }
+            const int TT = mt.nbr[e * 4 + k];
+            double* pr = E + LY::E_PR + f * NB;
+            if (TT > -1) {
+                const double* ge = mt.geo + (size_t)e * TGEO;
+                const double* gn = mt.geo + (size_t)TT * TGEO;
+                const double* xr = tb.fxr + f * 3;
+                double X[3], xn[3], phiR[NB];
+#pragma unroll
+                for (int a = 0; a < 3; a++) X[a] = ge[10 + 3 * a] * xr[0] + ge[11 + 3 * a] * xr[1] + ge[12 + 3 * a] * xr[2] + ge[19 + a];
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+                    xn[a] = gn[3 * a] * (X[0] - gn[19]) + gn[3 * a + 1] * (X[1] - gn[20]) + gn[3 * a + 2] * (X[2] - gn[21]);
+                tet_basis<NB>(xn, phiR);
+                const double* wn = W + (size_t)TT * B;
+#pragma unroll
+                for (int m = 0; m < NE; m++) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int j = 0; j < NB; j++) s += wn[m * NB + j] * phiR[j];
+                    st[NE + m] = s;
+                }
+#pragma unroll
+                for (int j = 0; j < NB; j++) pr[j] = phiR[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < NB; j++) pr[j] = 0.0;
+            }
+        }
+    }
+    __syncthreads();
+    // phase 2: derivatives of the weighted integrands, one (point, own value) pair per task
+    for (int q = tid; q < nel * NQ * NE; q += nt) {
+        const int el = q / (NQ * NE), r = q % (NQ * NE), qp = r / NE, c = r % NE, e = e0 + el;
+        double* E = eb + el * LY::PER;
+        if (qp < NQV) {
+            if constexpr (NB > 1) {
+                dual Wq[NE];
+#pragma unroll
+                for (int m = 0; m < NE; m++) Wq[m] = dual(E[LY::E_SV + qp * NE + m], m == c ? 1.0 : 0.0);
+                dual f[3][NE];
+                const double ex[3] = {1.0, 0.0, 0.0}, ey[3] = {0.0, 1.0, 0.0}, ez[3] = {0.0, 0.0, 1.0};
+                convective_flux3(P, Wq, ex, f[0]);
+                convective_flux3(P, Wq, ey, f[1]);
+                convective_flux3(P, Wq, ez, f[2]);
+                const double jw = mt.geo[(size_t)e * TGEO + 9] * tb.vw[qp];
+                double* out = E + LY::E_DV + (qp * NE + c) * 3 * NE;
+#pragma unroll
+                for (int d = 0; d < 3; d++)
+#pragma unroll
+                    for (int m = 0; m < NE; m++) out[d * NE + m] = jw * f[d][m].d;
+            }
+        } else {
+            const int f = qp - NQV, k = f / NQF, p = f % NQF;
+            const int TT = mt.nbr[e * 4 + k];
+            const double* fg = mt.fgeo + ((size_t)e * 4 + k) * TFGEO;
+            const double n[3] = {fg[0], fg[1], fg[2]};
+            const double* st = E + LY::E_SF + f * 2 * NE;
+            dual WL[NE], WR[NE];
+#pragma unroll
+            for (int m = 0; m < NE; m++) WL[m] = dual(st[m], m == c ? 1.0 : 0.0);
+            if (TT > -1) {
+#pragma unroll
+                for (int m = 0; m < NE; m++) WR[m] = dual(st[NE + m], 0.0);
+            } else {
+                boundary_value3(P, WL, n, TT, WR);
+            }
+            dual fn[NE];
+            numerical_convective_flux3(P, WL, WR, n, TT, fn);
+            const double jw = fg[3] * tb.fw[p];
+            double* out = E + LY::E_DF + (f * NE + c) * NE;
+#pragma unroll
+            for (int m = 0; m < NE; m++) out[m] = -jw * fn[m].d;
+        }
+    }
+    __syncthreads();
+    // phase 3: thread (element, column dir = (mp, jp))
+    const size_t BB = (size_t)B * B;
+    for (int q = tid; q < nel * B; q += nt) {
+        const int el = q / B, dir = q % B, mp = dir / NB, jp = dir % NB, e = e0 + el;
+        const double* E = eb + el * LY::PER;
+        double acc[B];
+#pragma unroll
+        for (int o = 0; o < B; o++) acc[o] = 0.0;
+        for (int k = 0; k < 4; k++) {
+            const int blk = mt.blkn[e * 4 + k];
+            double accR[B];
+#pragma unroll
+            for (int o = 0; o < B; o++) accR[o] = 0.0;
+            for (int p = 0; p < NQF; p++) {
+                const int f = k * NQF + p;
+                const double psi = tb.fb[f * NB + jp];
+                const double* dg = E + LY::E_DF + (f * NE + mp) * NE;
+                const double* phiL = tb.fb + f * NB;
+                const double* phiR = E + LY::E_PR + f * NB;
+#pragma unroll
+                for (int m = 0; m < NE; m++) {
+                    const double t = dg[m] * psi;
+#pragma unroll
+                    for (int j = 0; j < NB; j++) {
+                        acc[m * NB + j] += phiL[j] * t;
+                        accR[m * NB + j] += phiR[j] * t;
+                    }
+                }
+            }
+            if (blk >= 0) {
+                double* col = blocks + (size_t)blk * BB + (size_t)dir * B;
+#pragma unroll
+                for (int o = 0; o < B; o++) col[o] = accR[o];
+                if (fblocks) {
+                    float* fc = fblocks + (size_t)blk * BB + (size_t)dir * B;
+#pragma unroll
+                    for (int o = 0; o < B; o++) fc[o] = (float)accR[o];
+                }
+            }
+        }
+        if constexpr (NB > 1) {
+            const double* ge = mt.geo + (size_t)e * TGEO;
+            for (int qv = 0; qv < NQV; qv++) {
+                const double psi = tb.vb[qv * NB + jp];
+                const double* df = E + LY::E_DV + (qv * NE + mp) * 3 * NE;
+                const double* dphi = tb.vd + qv * NB * 3;
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    double gr[3];
+#pragma unroll
+                    for (int a = 0; a < 3; a++) gr[a] = ge[a] * dphi[3 * j] + ge[3 + a] * dphi[3 * j + 1] + ge[6 + a] * dphi[3 * j + 2];
+#pragma unroll
+                    for (int m = 0; m < NE; m++) acc[m * NB + j] += (df[m] * gr[0] + df[NE + m] * gr[1] + df[2 * NE + m] * gr[2]) * psi;
+                }
+            }
+        }
+        const double jac = mt.geo[(size_t)e * TGEO + 9];
+        double* col = blocks + (size_t)mt.blkd[e] * BB + (size_t)dir * B;
+#pragma unroll
+        for (int o = 0; o < B; o++) {
+            double a = -acc[o];
+            if (o / NB == mp) a += a1 * (jac * tb.mref[jp * NB + (o % NB)]);
+            col[o] = a;
+            if (fblocks) fblocks[(size_t)mt.blkd[e] * BB + (size_t)dir * B + o] = (float)a;
+        }
+    }
+}
+
 // out = (a1 M - dR/dW) v by one forward tangent pass per element
 template <int NB, int NQV, int NQF>
 __global__ void __launch_bounds__(128) k_jvp_tet(ModelParams P, TetTables tb, TetMesh mt, const double* __restrict__ W,
@@ -400,6 +605,20 @@ hipError_t launch_jacobian_tet(const ElemLaunch& L, hipStream_t s, const ModelPa
     if (FP_TMATCH3(NB, NQV, NQF)) {                                                                                         \
         hipLaunchKernelGGL((k_jacobian_tet<NB, NQV, NQF, double>), grd, blk, 0, s, P, tb, mt, W, a1, blocks, v, out, mode); \
         return hipGetLastError();                                                                                           \
+    }
+    FP_TET_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_jacobian_tet_team(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const TetTables& tb, const TetMesh& mt,
+                                    const double* W, double a1, double* blocks, float* fblocks) {
+#define X(NB, NQV, NQF)                                                                                          \
+    if (FP_TMATCH3(NB, NQV, NQF)) {                                                                              \
+        constexpr int EPB = TetTeam<NB, NQV, NQF>::EPB;                                                          \
+        hipLaunchKernelGGL((k_jacobian_tet_team<NB, NQV, NQF>), dim3((mt.nElem + EPB - 1) / EPB), dim3(256), 0, s, P, tb, mt, W, \
+                           a1, blocks, fblocks);                                                                 \
+        return hipGetLastError();                                                                                \
     }
     FP_TET_CASES(X)
 #undef X
