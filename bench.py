@@ -13,6 +13,11 @@ and sweeps, and mixed (FP32 factors/sweeps under the FP64 residual, Jacobian and
 GMRES).  `value` is the mixed-precision DOF-updates/s; the FP64 rate, their ratio
 and both normalised residual histories are reported beside it.
 
+`--workload euler3d_tet` measures the 3D configuration instead: compressible Euler on the
+Kuhn-split unit cube (64^3 cubes, 1,572,864 tetrahedra, 5 % seeded vertex jitter), M = 0.8 with a
+Gaussian density and pressure pulse (amplitudes 0.05, width 0.1), element order 2 (P1: the highest order
+the reference's basis3Dtetra builds), BDF1 Newton steps with blockilu0 GMRES.
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -45,11 +50,11 @@ def make_case(n_theta, n_r):
     return cases.cylinder_mesh(n_theta, n_r)
 
 
-def run_solver(case, props, bits, warmup, steps):
+def run_solver(case, props, bits, warmup, steps, state=None):
     from flowpro_v2_amd import solver
     t0 = time.perf_counter()
     dev = solver.DeviceCase(case, props, precond_bits=bits)
-    dev.set_state(dev.initial_state())
+    dev.set_state(dev.initial_state() if state is None else state(dev))
     log(f"fp{bits}: context ready in {time.perf_counter() - t0:.1f} s, dofs {dev.dofs}, "
         f"ILU levels {dev.ilu_levels}, block nonzeros {dev.nnzb}")
     s = solver.GpuImplicitSolver(dev, props)
@@ -74,10 +79,13 @@ def kernel_table(dev, reps=5):
 
 def roofline_jacobian(dev, ms):
     """Algorithmic bytes of one Jacobian-assembly launch: per element row 8 B^2 (1 + n_nbr)
-    block bytes written + residual-input bytes 8 B (2 + 1) + 112 B geometry + 12 B indices."""
+    block bytes written (plus 4 B^2 (1 + n_nbr) for the FP32 copy the block-ILU0 factorisation
+    starts from, which the kernel writes with FP32 factors) + residual-input bytes 8 B (2 + 1)
+    + geometry and neighbour indices (112 + 12 B for a triangle, 208 + 16 B for a tetrahedron)."""
     B = dev.B
-    blocks_bytes = 8.0 * B * B * dev.nnzb
-    elem_bytes = dev.n_elem * (8.0 * B * 3 + 112 + 12)
+    blocks_bytes = (8.0 + (4.0 if dev.precond_bits == 32 else 0.0)) * B * B * dev.nnzb
+    geometry = 112 + 12 if dev.dim == 2 else 208 + 16   # affine map, faces, neighbour indices
+    elem_bytes = dev.n_elem * (8.0 * B * 3 + geometry)
     traffic = blocks_bytes + elem_bytes
     achieved = traffic / (ms * 1e-3) / 1e9
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
@@ -85,13 +93,14 @@ def roofline_jacobian(dev, ms):
                 algorithmic_bytes=traffic, avg_launch_ms=ms)
 
 
-def factor_algorithmic_bytes(case, dev):
+def factor_algorithmic_bytes(case, dev, TT=None):
     """Minimum HBM bytes of one block-ILU0 factorisation in DOF order (rows processed in
     increasing order, lower neighbours k of row i eliminated in increasing order): every block
     of row i read and written once, D_k^-1 read once per lower neighbour k, and U_kj read once
     per update A_ij -= L_ik U_kj (j = i and the common neighbours of i and k above k)."""
     from flowpro_v2_amd import native
-    TT = native.neighbor_matrix(case.TP, case.bnd)
+    if TT is None:
+        TT = native.neighbor_matrix(case.TP, case.bnd)
     order = dev.element_order()
     rank = np.empty_like(order)
     rank[order] = np.arange(order.size, dtype=order.dtype)
@@ -106,11 +115,11 @@ def factor_algorithmic_bytes(case, dev):
     return float(blocks) * dev.B * dev.B * s
 
 
-def roofline_factor(case, dev, ms_levels, traffic=None):
+def roofline_factor(case, dev, ms_levels, traffic=None, TT=None):
     """The dominant kernel of the step: the level-scheduled block-ILU0 factor kernel
     (k_ilu_factor_rows).  `avg_launch_ms` = HIP-event time of all level launches / number of levels."""
     levels = dev.ilu_levels[0]
-    total = factor_algorithmic_bytes(case, dev)
+    total = factor_algorithmic_bytes(case, dev, TT)
     per_launch = total / levels
     avg = ms_levels / levels
     achieved = per_launch / (avg * 1e-3) / 1e9
@@ -164,19 +173,34 @@ def main():
     ap.add_argument("--cfl", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--workload", choices=["ns2d_cylinder", "euler3d_tet"], default="ns2d_cylinder")
+    ap.add_argument("--cube", type=int, default=64, help="cubes per edge of the euler3d_tet workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is "
               "element partitioning with a halo layer, not built)", file=sys.stderr)
         sys.exit(2)
-    from flowpro_v2_amd import cases
-    case = make_case(args.n_theta, args.n_r)
-    props = cases.ns_props(order=4, CFL=args.cfl)
-    log(f"mesh {case.name}: {case.n_elem} triangles")
+    from flowpro_v2_amd import cases, native
+    tet = args.workload == "euler3d_tet"
+    state = None
+    TT = None
+    if tet:
+        case = cases.kuhn_cube(args.cube, 0.05, 4)
+        props = cases.euler3d_props(mach=0.8, order=2, CFL=args.cfl)
+        TT = native.neighbor_matrix_tet(case.TP, case.bnd)
+
+        def state(dev):
+            return cases.density_pulse_state(case, dev.nb, dev.initial_state()[0][:: dev.nb], amp=0.05, width=0.1,
+                                             pressure_amp=0.05)
+        log(f"mesh {case.name}: {case.n_elem} tetrahedra")
+    else:
+        case = make_case(args.n_theta, args.n_r)
+        props = cases.ns_props(order=4, CFL=args.cfl)
+        log(f"mesh {case.name}: {case.n_elem} triangles")
     out = {}
     devs = {}
     for bits in ([64, 32] if args.only is None else ([64] if args.only == "fp64" else [32])):
-        dev, res = run_solver(case, props, bits, args.warmup, args.steps)
+        dev, res = run_solver(case, props, bits, args.warmup, args.steps, state)
         res["rate"] = dev.dofs * res["updates"] / res["elapsed"]
         out[bits] = res
         if bits == 32 or args.only is not None:
@@ -190,9 +214,12 @@ def main():
     kt = devs["kt"]
     main_bits = 32 if 32 in out else 64
     r = out[main_bits]
-    default_mesh = (args.n_theta, args.n_r) == (400, 250) and main_bits == 32
-    roof = roofline_factor(case, dev, kt["ilu_factor_levels"], FACTOR_TRAFFIC_400x250 if default_mesh else None)
+    default_mesh = not tet and (args.n_theta, args.n_r) == (400, 250) and main_bits == 32
+    roof = roofline_factor(case, dev, kt["ilu_factor_levels"], FACTOR_TRAFFIC_400x250 if default_mesh else None, TT)
     others = {"k_jacobian_team": roofline_jacobian(dev, kt["jacobian"]), "k_bsr_spmv": roofline_spmv(dev, kt["spmv"])}
+    if tet:
+        others = {"k_jacobian_tet_team": dict(roofline_jacobian(dev, kt["jacobian"]), kernel="k_jacobian_tet_team"),
+                  "k_bsr_spmv": roofline_spmv(dev, kt["spmv"])}
     result = {
         "metric": "Newton-GMRES DOF-updates/s, mixed vs FP64 GPU at equal residual; % HBM peak",
         "value": round(r["rate"], 1),
@@ -206,9 +233,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 residual/Jacobian/GMRES, f32 preconditioner" if main_bits == 32 else "f64",
         "data": "synthetic (freestream state, seeded mesh)",
-        "config": {"workload": f"NS2D DG p=3 Re=5000 M=0.5 cylinder, {case.n_elem} triangles, "
-                               f"{dev.dofs} DOFs, BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) GMRES(30) tol 1e-2",
-                   "elements": case.n_elem, "dofs": dev.dofs, "order": 4, "precond_bits": main_bits},
+        "config": ({"workload": f"Euler3D DG P1 (order 2) Kuhn cube {args.cube}^3 x 6 = {case.n_elem} tetrahedra, "
+                                f"{dev.dofs} DOFs, M=0.8 density pulse, BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) "
+                                "GMRES(30) tol 1e-2",
+                    "elements": case.n_elem, "dofs": dev.dofs, "order": 2, "precond_bits": main_bits} if tet else
+                   {"workload": f"NS2D DG p=3 Re=5000 M=0.5 cylinder, {case.n_elem} triangles, "
+                                f"{dev.dofs} DOFs, BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) GMRES(30) tol 1e-2",
+                    "elements": case.n_elem, "dofs": dev.dofs, "order": 4, "precond_bits": main_bits}),
         "fp64_value": round(out[64]["rate"], 1) if 64 in out else None,
         "mixed_over_fp64": round(out[32]["rate"] / out[64]["rate"], 3) if (32 in out and 64 in out) else None,
         "residuals": {f"fp{b}": [float(f"{v:.6e}") for v in out[b]["residuals"]] for b in out},
@@ -218,7 +249,10 @@ def main():
         "other_kernels": others,
     }
     dev.close()
-    if not args.no_cpu_baseline:
+    if tet:
+        result["data"] = "synthetic (freestream with a seeded density pulse, seeded jittered mesh)"
+        result["cpu_baseline"] = None   # the CPU restatement of the tetrahedra covers residual and Jacobian only
+    elif not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")
         result["cpu_baseline"] = cpu_baseline(threads)

@@ -248,17 +248,21 @@ def euler3d_props(mach: float = 0.8, order: int = 2, **kw) -> dict:
 
 
 def density_pulse_state(case: Case, nb: int, w0, amp: float = 0.05, width: float = 0.1,
-                        center=(0.5, 0.5, 0.5)) -> np.ndarray:
-    """Freestream w0 with a Gaussian density pulse (velocity and pressure kept): nodal values of
-    the P1 Lagrange basis at the vertices (nb = 4) or the element mean (nb = 1)."""
+                        center=(0.5, 0.5, 0.5), pressure_amp: float = 0.0) -> np.ndarray:
+    """Freestream w0 with a Gaussian density pulse (velocity kept; pressure kept, or raised by the
+    same Gaussian times pressure_amp): nodal values of the P1 Lagrange basis at the vertices
+    (nb = 4) or the element mean (nb = 1).  Without a pressure pulse the cross-stream momentum
+    residuals of a flow along x stay at round-off; a pressure pulse sends an acoustic wave into
+    every direction, so each equation's residual is of the order of the amplitudes."""
     X = case.PXY[case.TP]                                     # (nElem, 4, 3)
     if nb == 1:
         X = X.mean(1, keepdims=True)
     r2 = ((X - np.asarray(center)) ** 2).sum(-1)
     rho0 = w0[0]
-    rho = rho0 * (1 + amp * np.exp(-r2 / (width * width)))
+    g = np.exp(-r2 / (width * width))
+    rho = rho0 * (1 + amp * g)
     u = np.asarray(w0[1:4]) / rho0
-    p_e = w0[4] - 0.5 * rho0 * (u ** 2).sum()                  # internal energy density (pressure / (kappa - 1))
+    p_e = (w0[4] - 0.5 * rho0 * (u ** 2).sum()) * (1 + pressure_amp * g)   # pressure / (kappa - 1)
     W = np.zeros((case.n_elem, 5, nb))
     W[:, 0] = rho
     for d in range(3):

@@ -180,6 +180,50 @@ def test_tet_row_factor_kernel_matches_level_kernel(bits, monkeypatch):
     assert np.abs(xs[0] - xs[1]).max() <= tol * np.abs(xs[1]).max()
 
 
+@pytest.mark.parametrize("n", [3, 8])
+def test_tet_fp32_preconditioner_same_solution(n):
+    """FP32 block-ILU0 factors and sweeps (the factorisation starting from the FP32 copy the
+    Jacobian kernel writes) solve the same FP64 system: at a tight GMRES tolerance the correction
+    agrees with the FP64-preconditioned one to the FP32 rounding of the preconditioned operator
+    (1e-6)."""
+    xs = []
+    for bits in (64, 32):
+        case = cases.kuhn_cube(n, 0.05, 4)
+        props = cases.euler3d_props(order=2, iterativeSolverTol=1e-12)
+        dev = solver.DeviceCase(case, props, precond_bits=bits)
+        dev.set_state(_state(case, dev.nb, dev.initial_state()[0][:: dev.nb]))
+        dev.assemble(0.05, -1.0, 1, True)
+        rc, it, err = dev.linear_solve()
+        assert rc == 0
+        xs.append(dev.solution())
+        dev.close()
+    assert np.abs(xs[0] - xs[1]).max() <= 1e-6 * np.abs(xs[0]).max()
+
+
+def test_tet_fp32_preconditioner_same_history():
+    """Newton steps with FP32 and with FP64 preconditioner factors reach the same states and the
+    same normalised residual history.  The state is a density and pressure pulse, so that every
+    equation's residual is of the order of the pulse: the history is the largest per-equation
+    ratio to its first-steps maximum, and an equation whose residual sits at round-off (the
+    cross-stream momenta of a pure density pulse) turns the FP32 rounding of the GMRES
+    correction into an O(1) change of that ratio.  States 1e-7 relative, history 1e-5."""
+    case = cases.kuhn_cube(8, 0.05, 4)
+    props = cases.euler3d_props(mach=0.8, order=2, CFL=10.0)
+    hist, Ws = [], []
+    for bits in (64, 32):
+        dev = solver.DeviceCase(case, props, precond_bits=bits)
+        dev.set_state(cases.density_pulse_state(case, dev.nb, dev.initial_state()[0][:: dev.nb], amp=0.05,
+                                                width=0.1, pressure_amp=0.05))
+        s = solver.GpuImplicitSolver(dev, props)
+        hist.append(np.array([h.residuum for h in s.solve(8)]))
+        Ws.append(dev.get_state())
+        dev.close()
+    print("histories", hist)
+    assert hist[0][-1] < 0.9 * hist[0][0]
+    assert np.abs(Ws[1] - Ws[0]).max() <= 1e-7 * np.abs(Ws[0]).max()
+    assert np.abs(hist[1] / hist[0] - 1).max() < 1e-5
+
+
 @pytest.mark.parametrize("order", [1, 2])
 def test_tet_size_over_lambda(order):
     case, o, dev, W = _setup(order)
