@@ -24,7 +24,7 @@ WALL, INLET, OUTLET, INVISCID_WALL = -1, -2, -3, -4
 class Case:
     name: str
     PXY: np.ndarray            # (nPts, 2) float64
-    TP: np.ndarray             # (nElem, 3) int32, counter-clockwise
+    TP: np.ndarray             # (nElem, 3) int32 triangles or (nElem, 4) quadrilaterals, counter-clockwise
     bnd: np.ndarray            # (nBnd, 3) int32: type, v0, v1
     wall_distance: np.ndarray | None = None
     props: dict = field(default_factory=dict)
@@ -81,9 +81,29 @@ def _split_quads(vid: np.ndarray) -> np.ndarray:
     return np.stack([t1, t2], 1).reshape(-1, 3).astype(np.int32)
 
 
-def square_mesh(n: int, size: float = 1.0, boundary_type: int = INLET) -> Case:
+def _quads(vid: np.ndarray) -> np.ndarray:
+    """Structured (ni+1) x (nj+1) vertex grid -> ni*nj quadrilaterals (a, b, c, d) =
+    (i, j), (i+1, j), (i+1, j+1), (i, j+1), j outer and i inner."""
+    ni, nj = vid.shape[0] - 1, vid.shape[1] - 1
+    I, J = np.meshgrid(np.arange(ni), np.arange(nj), indexing="ij")
+    I, J = I.T.ravel(), J.T.ravel()
+    return np.stack([vid[I, J], vid[I + 1, J], vid[I + 1, J + 1], vid[I, J + 1]], 1).astype(np.int32)
+
+
+def _orient_ccw_quads(PXY: np.ndarray, TP: np.ndarray) -> np.ndarray:
+    """Reverse (a, b, c, d) -> (a, d, c, b) where the shoelace area is negative."""
+    X, Y = PXY[TP, 0], PXY[TP, 1]
+    area2 = (X * np.roll(Y, -1, 1) - np.roll(X, -1, 1) * Y).sum(1)
+    TP = TP.copy()
+    neg = area2 < 0
+    TP[neg, 1], TP[neg, 3] = TP[neg, 3].copy(), TP[neg, 1].copy()
+    return TP
+
+
+def square_mesh(n: int, size: float = 1.0, boundary_type: int = INLET, quads: bool = False) -> Case:
     """TestSimulation.writeSquareMesh (TestSimulation.java:28-80): [0,size]^2, n x n
-    cells split with alternating diagonals, all boundary faces of one type."""
+    cells split with alternating diagonals, all boundary faces of one type (quads: the cells
+    themselves, squareElement)."""
     h = size / n
     j, i = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
     PXY = np.stack([i.ravel() * h, j.ravel() * h], 1).astype(np.float64)
@@ -92,28 +112,35 @@ def square_mesh(n: int, size: float = 1.0, boundary_type: int = INLET) -> Case:
         return j * (n + 1) + i
 
     vid = np.array([[v(ii, jj) for jj in range(n + 1)] for ii in range(n + 1)])
-    TP = _split_quads(vid)
+    TP = _quads(vid) if quads else _split_quads(vid)
     rows = []
     for k in range(n):
         rows += [(boundary_type, v(k, 0), v(k + 1, 0)), (boundary_type, v(n, k), v(n, k + 1)),
                  (boundary_type, v(k + 1, n), v(k, n)), (boundary_type, v(0, k + 1), v(0, k))]
     wd = np.array([min(min(ii, jj), min(n - ii, n - jj)) * h for jj in range(n + 1) for ii in range(n + 1)])
-    return Case(f"square{n}", PXY, TP, np.array(rows, dtype=np.int32), wd)
+    return Case(f"square{n}" + ("q" if quads else ""), PXY, TP, np.array(rows, dtype=np.int32), wd)
 
 
-def channel_bump(nx: int = 64, ny: int = 16) -> Case:
+def channel_bump(nx: int = 64, ny: int = 16, quads: bool = False, jitter: float = 0.0, seed: int = 5) -> Case:
     """Channel [0,3]x[0,1] with a 10%-height circular-arc bump on x in [1,2]
     (y_b = sqrt(1.69 - (x-1.5)^2) - 1.2, the GAMM geometry), (nx+1)x(ny+1) vertices,
-    y = y_b(x) + eta (1 - y_b(x)); walls -1 bottom/top, inlet -2 left, outlet -3 right."""
+    y = y_b(x) + eta (1 - y_b(x)); walls -1 bottom/top, inlet -2 left, outlet -3 right.
+    quads: the cells as quadrilaterals (general bilinear maps over the bump); jitter moves the
+    interior vertices by a seeded fraction of the spacing."""
     x = np.linspace(0.0, 3.0, nx + 1)
     yb = np.where((x >= 1.0) & (x <= 2.0), np.sqrt(np.maximum(1.69 - (x - 1.5) ** 2, 0.0)) - 1.2, 0.0)
     yb = np.maximum(yb, 0.0)
     eta = np.linspace(0.0, 1.0, ny + 1)
     X = np.repeat(x[:, None], ny + 1, 1)
     Y = yb[:, None] + eta[None, :] * (1.0 - yb[:, None])
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        hx, hy = 3.0 / nx, (1.0 - yb.max()) / ny
+        X[1:-1, 1:-1] += jitter * hx * rng.uniform(-1, 1, size=X[1:-1, 1:-1].shape)
+        Y[1:-1, 1:-1] += jitter * hy * rng.uniform(-1, 1, size=Y[1:-1, 1:-1].shape)
     vid = np.arange((nx + 1) * (ny + 1)).reshape(nx + 1, ny + 1)
     PXY = np.stack([X.ravel(), Y.ravel()], 1)
-    TP = _orient_ccw(PXY, _split_quads(vid))
+    TP = _orient_ccw_quads(PXY, _quads(vid)) if quads else _orient_ccw(PXY, _split_quads(vid))
     rows = []
     for i in range(nx):
         rows.append((WALL, vid[i, 0], vid[i + 1, 0]))
@@ -121,11 +148,11 @@ def channel_bump(nx: int = 64, ny: int = 16) -> Case:
     for j in range(ny):
         rows.append((INLET, vid[0, j + 1], vid[0, j]))
         rows.append((OUTLET, vid[nx, j], vid[nx, j + 1]))
-    return Case(f"channel_bump_{nx}x{ny}", PXY, TP, np.array(rows, dtype=np.int32))
+    return Case(f"channel_bump_{nx}x{ny}" + ("q" if quads else ""), PXY, TP, np.array(rows, dtype=np.int32))
 
 
 def cylinder_mesh(n_theta: int = 400, n_r: int = 250, growth: float = 1.01, jitter: float = 0.15,
-                  seed: int = 2) -> Case:
+                  seed: int = 2, quads: bool = False) -> Case:
     """[-5,5]^2 around a unit-diameter cylinder: an O-grid from the circle r = 0.5 to
     the square boundary, geometric radial spacing, each cell split into two triangles
     (alternating diagonals), interior vertices jittered by a seeded fraction of the
@@ -147,7 +174,7 @@ def cylinder_mesh(n_theta: int = 400, n_r: int = 250, growth: float = 1.01, jitt
     PXY = P.reshape(-1, 2)
     vid = np.arange(n_theta * (n_r + 1)).reshape(n_theta, n_r + 1)
     vid = np.concatenate([vid, vid[:1]], 0)                # periodic in theta
-    TP = _orient_ccw(PXY, _split_quads(vid))
+    TP = _orient_ccw_quads(PXY, _quads(vid)) if quads else _orient_ccw(PXY, _split_quads(vid))
     rows = []
     for i in range(n_theta):
         a, b = vid[i, 0], vid[i + 1, 0]
@@ -161,7 +188,7 @@ def cylinder_mesh(n_theta: int = 400, n_r: int = 250, growth: float = 1.01, jitt
         else:
             t = INVISCID_WALL
         rows.append((t, a, b))
-    return Case(f"cylinder_{n_theta}x{n_r}", PXY, TP, np.array(rows, dtype=np.int32))
+    return Case(f"cylinder_{n_theta}x{n_r}" + ("q" if quads else ""), PXY, TP, np.array(rows, dtype=np.int32))
 
 
 def plate_channel(nx: int = 64, ny: int = 24, x0: float = -0.5, x1: float = 2.0, first: float = 1e-5,

@@ -79,6 +79,42 @@ struct TetMesh {
     const int* blkd;
 };
 
+// Quadrilaterals (squareElement: basis2Dsquare, QuadratureSquare, transformation2Dsquare).  The
+// bilinear map makes the Jacobian vary over the element, so the geometry is stored per point,
+// element data in DOF order:
+//   qv   [(e*NQV + q)*(1+2NB)]        |det J| w at volume point q, then the physical basis gradients
+//                                     (d phi_j/dx, d phi_j/dy) pairs
+//   qf   [((e*4 + k)*NQF + p)*2NB]    physical basis gradients at face point p of face k
+//   qn   [((e*4 + k)*NQF + p)*3NB]    the neighbour's basis (NB) and physical gradients (2NB) at that
+//                                     point, from the neighbour's inverse map (transformation2Dsquare.getXi)
+//   xq   [(e*(NQV+4NQF) + i)*2]       physical coordinates of the volume points, then the face points
+//   fgeo [(e*4 + k)*4]                face normal nx ny, face Jacobian (edge length / 2), dL
+//   mass, minv [e*NB*NB]              element mass matrix and its inverse
+//   is   [e*NB]                       Is_j = int phi_j / |K| (element mean = sum_j W_j Is_j)
+//   nbr, nbrf, blkn [e*4 + k], blkd [e]  as for triangles
+struct QuadTables {
+    const double* vb;    // [NQV][NB]
+    const double* vw;    // [NQV]
+    const double* fb;    // [4][NQF][NB]
+    const double* fw;    // [NQF]
+};
+
+struct QuadMesh {
+    int nElem;
+    const double* qv;
+    const double* qf;
+    const double* qn;
+    const double* xq;
+    const double* fgeo;
+    const double* mass;
+    const double* minv;
+    const double* is;
+    const int* nbr;
+    const int* nbrf;
+    const int* blkn;
+    const int* blkd;
+};
+
 // compile-time shape of the element kernels selected at run time
 struct ElemLaunch {
     int model, nb, neq, nqv, nqf, visc, av;
@@ -134,4 +170,26 @@ __device__ inline void av_volume_flux(const DGParams& G, const Mesh& ms, int e, 
     }
 }
 
+// Element.positivityLimiter (FlowPro/src/main/java/flowpro/core/element/Element.java:385-476) with
+// Aerodynamics.isAdmissible (:348-357); the L2 projection of the constant 1 onto the nodal basis is 1.
+// AdiabaticEulers (three equations) admits every positive density (AdiabaticEulers.java:19-22)
+template <int NEQ> __device__ inline bool admissible(const double* W) {
+    if (!(W[0] > 1e-10)) return false;
+    if constexpr (NEQ == 3) return true;
+    double m2 = W[1] * W[1];
+    m2 += W[2] * W[2];
+    return W[3] - m2 / (2 * W[0]) > 1e-10;
+}
+template <int NEQ> __device__ inline double admissible_theta(const double* Wavg, const double* Wx) {
+    if (admissible<NEQ>(Wx)) return 1.0;
+    constexpr int NA = NEQ < 4 ? NEQ : 4;   // admissibility depends on rho, rho u, rho v (, rho E) only
+    double lo = 0, hi = 1, Wt[NA];
+    for (int it = 0; it < 30; it++) {
+        double mid = 0.5 * (lo + hi);
+        for (int m = 0; m < NA; m++) Wt[m] = Wavg[m] + mid * (Wx[m] - Wavg[m]);
+        if (admissible<NEQ>(Wt)) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 }  // namespace fp

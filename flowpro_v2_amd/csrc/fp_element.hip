@@ -580,28 +580,6 @@ hipError_t launch_av_limiter(const ElemLaunch& L, hipStream_t s, const ModelPara
 }
 
 // ------------------------------------------------------------------ positivity limiter
-// Element.positivityLimiter (FlowPro/src/main/java/flowpro/core/element/Element.java:385-476) with
-// Aerodynamics.isAdmissible (:348-357); the L2 projection of the constant 1 onto the nodal basis is 1.
-// AdiabaticEulers (three equations) admits every positive density (AdiabaticEulers.java:19-22)
-template <int NEQ> __device__ inline bool admissible(const double* W) {
-    if (!(W[0] > 1e-10)) return false;
-    if constexpr (NEQ == 3) return true;
-    double m2 = W[1] * W[1];
-    m2 += W[2] * W[2];
-    return W[3] - m2 / (2 * W[0]) > 1e-10;
-}
-template <int NEQ> __device__ inline double admissible_theta(const double* Wavg, const double* Wx) {
-    if (admissible<NEQ>(Wx)) return 1.0;
-    constexpr int NA = NEQ < 4 ? NEQ : 4;   // admissibility depends on rho, rho u, rho v (, rho E) only
-    double lo = 0, hi = 1, Wt[NA];
-    for (int it = 0; it < 30; it++) {
-        double mid = 0.5 * (lo + hi);
-        for (int m = 0; m < NA; m++) Wt[m] = Wavg[m] + mid * (Wx[m] - Wavg[m]);
-        if (admissible<NEQ>(Wt)) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
 template <int NB, int NEQ, int NQV, int NQF>
 __global__ void k_positivity(Tables tb, Mesh ms, double* __restrict__ W, const double* __restrict__ isref, int* __restrict__ count) {
     constexpr int B = NB * NEQ;

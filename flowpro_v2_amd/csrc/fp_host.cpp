@@ -67,7 +67,17 @@ hipError_t launch_set_io(hipStream_t, IluIO*, const double*, double*);
 hipError_t launch_block_jacobi(hipStream_t, int, int, double*, const double*, double*);
 hipError_t launch_gather_diag(hipStream_t, int, int, const int*, const double*, double*);
 hipError_t launch_min(hipStream_t, size_t, const double*, double*, double*);
-hipError_t launch_colsum(hipStream_t, int, int, const double*, double*);
+hipError_t launch_colsum(hipStream_t, int, int, const double*, double*, double*);
+bool quad_supported(const ElemLaunch&);
+hipError_t launch_residual_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const QuadTables&, const QuadMesh&,
+                                const double*, const double*, const double*, double, double, double, double*, double*, double*);
+hipError_t launch_jacobian_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const QuadTables&, const QuadMesh&,
+                                const double*, double, double*, float*, const double*, double*, int);
+hipError_t launch_jvp_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const QuadTables&, const QuadMesh&,
+                           const double*, const double*, double, double*);
+hipError_t launch_size_over_lambda_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const QuadMesh&, const double*,
+                                        const double*, double*);
+hipError_t launch_positivity_quad(const ElemLaunch&, hipStream_t, const QuadTables&, const QuadMesh&, double*, int*);
 hipError_t launch_dot(hipStream_t, size_t, const double*, const double*, double*, double*, int);
 hipError_t launch_axpy_dot(hipStream_t, size_t, double*, const double*, const double*, const double*, double*, double*, int);
 hipError_t launch_scale_div(hipStream_t, size_t, double*, const double*, const double*);
@@ -84,7 +94,8 @@ struct fp_ctx {
     std::string err;
     // sizes
     int nElem = 0, order = 0, NB = 0, NEQ = 0, B = 0, NQV = 0, NQF = 0;
-    int dim = 2, NF = 3;            // triangles (3 faces) or tetrahedra (4 faces)
+    int dim = 2, NF = 3;            // triangles (3 faces), quadrilaterals or tetrahedra (4 faces)
+    bool quad = false;              // 2D quadrilaterals (squareElement)
     long long dofs = 0;
     ElemLaunch L{};
     ModelParams mp{};
@@ -92,6 +103,7 @@ struct fp_ctx {
     AvParams avp{};
     double *av = nullptr, *avold = nullptr, *trunc = nullptr;   // damping state, sensor memory, truncation
     std::vector<double> h_isref;    // Is_j = int phi_j / |K| (the element mean of W is sum_j W_j Is_j)
+    std::vector<double> h_qis;      // quadrilaterals: Is of every element, mesh order [e*NB + j]
     int precond = 2, bits = 64, m = 30, cycles = 5, positivity = 1;
     double tol = 1e-2;
     // host copies
@@ -124,6 +136,9 @@ struct fp_ctx {
     Mesh ms{};
     TetTables ttb{};
     TetMesh tms{};
+    QuadTables qtb{};
+    QuadMesh qms{};
+    double *qv = nullptr, *qf = nullptr, *qn = nullptr, *xq = nullptr, *qmass = nullptr, *qminv = nullptr, *qis = nullptr;
     bool jac_valid = false;
     double last_a1 = 0;
     double kappa = 1.4;
@@ -142,6 +157,9 @@ struct fp_ctx {
 
 // element kernels of the case's cell type
 static hipError_t run_residual(fp_ctx* c, const double* a) {
+    if (c->quad)
+        return launch_residual_quad(c->L, c->s, c->mp, c->dg, c->qtb, c->qms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R, c->RHS,
+                                    c->enorm);
     if (c->dim == 3)
         return launch_residual_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R, c->RHS, c->enorm);
     return launch_residual_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R, c->RHS,
@@ -151,44 +169,51 @@ static hipError_t run_jacobian(fp_ctx* c, double a1) {
     // with FP32 block-ILU0 factors the Jacobian kernel also writes the factorisation's starting copy
     float* f32 = (c->precond == 2 && c->bits == 32) ? (float*)c->F : nullptr;
     c->f_fresh = f32 != nullptr;
+    if (c->quad) return launch_jacobian_quad(c->L, c->s, c->mp, c->dg, c->qtb, c->qms, c->W, a1, c->val, f32, nullptr, nullptr, 0);
     if (c->dim == 3) return launch_jacobian_tet_team(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, c->val, f32);
     return launch_jacobian_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, c->val, f32);
 }
 static hipError_t run_jvp(fp_ctx* c, const double* v, double a1, double* out) {
+    if (c->quad) return launch_jvp_quad(c->L, c->s, c->mp, c->dg, c->qtb, c->qms, c->W, v, a1, out);
     if (c->dim == 3) return launch_jvp_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, v, a1, out);
     return launch_jvp(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, v, a1, out);
 }
 static hipError_t run_jtv(fp_ctx* c, const double* v, double a1, double* out) {
+    if (c->quad) return launch_jacobian_quad(c->L, c->s, c->mp, c->dg, c->qtb, c->qms, c->W, a1, nullptr, nullptr, v, out, 1);
     if (c->dim == 3) return launch_jacobian_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, nullptr, v, out, 1);
     return launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, nullptr, v, out, 1);
 }
 static hipError_t run_size_over_lambda(fp_ctx* c) {
+    if (c->quad) return launch_size_over_lambda_quad(c->L, c->s, c->mp, c->qms, c->W, c->esize, c->tmp);
     if (c->dim == 3) return launch_size_over_lambda_tet(c->L, c->s, c->mp, c->tms, c->W, c->esize, c->isref, c->tmp);
     return launch_size_over_lambda(c->L, c->s, c->mp, c->tb, c->ms, c->W, c->esize, c->isref, c->tmp);
 }
 static hipError_t run_positivity(fp_ctx* c, int* count) {
+    if (c->quad) return launch_positivity_quad(c->L, c->s, c->qtb, c->qms, c->W, count);
     if (c->dim == 3) return launch_positivity_tet(c->L, c->s, c->ttb, c->tms, c->W, c->isref, count);
     return launch_positivity(c->L, c->s, c->tb, c->ms, c->W, c->isref, count);
 }
 
 // ============================================================== integer setup
-extern "C" int fp_neighbor_matrix(int n, const int* TP, int nb, const int* bnd, int* TT) {
+// 2D elements with nF faces, face k between vertices k and (k + 1) % nF (triangleElement,
+// squareElement.getFaceIndexes)
+static int neighbor_matrix_2d(int n, const int* TP, int nF, int nb, const int* bnd, int* TT) {
     if (n < 0 || !TP || !TT || (nb > 0 && !bnd)) return FP_EINVAL;
     auto key = [&](int i, int k) {
-        long long a = TP[3 * i + k], b = TP[3 * i + (k + 1) % 3];
+        long long a = TP[nF * i + k], b = TP[nF * i + (k + 1) % nF];
         if (a > b) std::swap(a, b);
         return (a << 32) | b;
     };
     std::unordered_map<long long, std::pair<int, int>> faces;
-    faces.reserve((size_t)n * 2);
+    faces.reserve((size_t)n * nF);
     for (int i = 0; i < n; i++)
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < nF; k++) {
             auto it = faces.find(key(i, k));
             if (it == faces.end()) faces.emplace(key(i, k), std::make_pair(i, -1000));
             else it->second.second = i;
         }
     for (int i = 0; i < n; i++)
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < nF; k++) {
             const auto& f = faces[key(i, k)];
             int t = -1000;
             if (f.first == i) {
@@ -196,7 +221,7 @@ extern "C" int fp_neighbor_matrix(int n, const int* TP, int nb, const int* bnd, 
             } else {
                 t = f.first;
             }
-            TT[3 * i + k] = t;
+            TT[nF * i + k] = t;
         }
     std::unordered_map<long long, int> bmap;
     for (int r = 0; r < nb; r++) {
@@ -205,12 +230,18 @@ extern "C" int fp_neighbor_matrix(int n, const int* TP, int nb, const int* bnd, 
         bmap[(a << 32) | b] = bnd[3 * r];
     }
     for (int i = 0; i < n; i++)
-        for (int k = 0; k < 3; k++)
-            if (TT[3 * i + k] == -1000) {
+        for (int k = 0; k < nF; k++)
+            if (TT[nF * i + k] == -1000) {
                 auto it = bmap.find(key(i, k));
-                TT[3 * i + k] = it != bmap.end() ? it->second : -1;
+                TT[nF * i + k] = it != bmap.end() ? it->second : -1;
             }
     return FP_OK;
+}
+extern "C" int fp_neighbor_matrix(int n, const int* TP, int nb, const int* bnd, int* TT) {
+    return neighbor_matrix_2d(n, TP, 3, nb, bnd, TT);
+}
+extern "C" int fp_neighbor_matrix_quad(int n, const int* TP, int nb, const int* bnd, int* TT) {
+    return neighbor_matrix_2d(n, TP, 4, nb, bnd, TT);
 }
 
 // breadth-first search from `node` over unvisited elements; returns the last element of the
@@ -294,6 +325,7 @@ static int rcm_impl(int n, const int* TT, int nF, int* order) {
 
 extern "C" int fp_rcm_order(int n, const int* TT, int* order) { return rcm_impl(n, TT, 3, order); }
 extern "C" int fp_rcm_order_tet(int n, const int* TT, int* order) { return rcm_impl(n, TT, 4, order); }
+extern "C" int fp_rcm_order_quad(int n, const int* TT, int* order) { return rcm_impl(n, TT, 4, order); }
 
 // local faces of tetrahedralElement (vertex triples of each face)
 static const int TET_FACE[4][3] = {{0, 2, 1}, {0, 1, 3}, {1, 2, 3}, {2, 0, 3}};
@@ -466,6 +498,82 @@ Basis make_basis(int order) {
     b.coef = inv;
     return b;
 }
+// Mat.invert (FlowProAPI/src/main/java/flowpro/api/Mat.java:502-642) in double: Gaussian
+// elimination with scaled partial pivoting (row scale = largest |entry|, the pivot row
+// maximises |a_ij| / scale, rows exchanged through an index vector), then forward and back
+// substitution for each column of the identity.  The square basis takes its coefficients from it
+// so that they carry the reference's rounding: the Vandermonde matrix of order 4 has
+// cond ~ 1e4, and another elimination order moves the coefficients by ~1e-12.
+bool invert_scaled_pivot(const std::vector<double>& A0, int n, std::vector<double>& X) {
+    std::vector<double> a(A0), c(n);
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; i++) {
+        idx[i] = i;
+        double c1 = 0;
+        for (int j = 0; j < n; j++) c1 = std::max(c1, std::fabs(a[(size_t)i * n + j]));
+        c[i] = c1;
+    }
+    for (int j = 0; j < n - 1; j++) {
+        double best = 0;
+        int k = j;
+        for (int i = j; i < n; i++) {
+            const double r = std::fabs(a[(size_t)idx[i] * n + j]) / c[idx[i]];
+            if (r > best) { best = r; k = i; }
+        }
+        std::swap(idx[j], idx[k]);
+        for (int i = j + 1; i < n; i++) {
+            const double pj = a[(size_t)idx[i] * n + j] / a[(size_t)idx[j] * n + j];
+            a[(size_t)idx[i] * n + j] = pj;
+            for (int l = j + 1; l < n; l++) a[(size_t)idx[i] * n + l] -= pj * a[(size_t)idx[j] * n + l];
+        }
+    }
+    std::vector<double> b((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) b[(size_t)i * n + i] = 1;
+    for (int i = 0; i < n - 1; i++)
+        for (int j = i + 1; j < n; j++)
+            for (int k = 0; k < n; k++) b[(size_t)idx[j] * n + k] -= a[(size_t)idx[j] * n + i] * b[(size_t)idx[i] * n + k];
+    X.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) {
+        const double last = a[(size_t)idx[n - 1] * n + n - 1];
+        if (last == 0) return false;
+        X[(size_t)(n - 1) * n + i] = b[(size_t)idx[n - 1] * n + i] / last;
+        for (int j = n - 2; j >= 0; j--) {
+            double x = b[(size_t)idx[j] * n + i];
+            for (int k = j + 1; k < n; k++) x -= a[(size_t)idx[j] * n + k] * X[(size_t)k * n + i];
+            X[(size_t)j * n + i] = x / a[(size_t)idx[j] * n + j];
+        }
+    }
+    return true;
+}
+
+// basis2Dsquare (FlowPro/src/main/java/flowpro/core/basis/basis2Dsquare.java:17-73): the same
+// monomial exponents, all order^2 of them; nodes (b_i, b_j), b = linspace(0, 1, order), i outer
+Basis make_basis_square(int order) {
+    Basis b;
+    b.order = order;
+    b.nb = order * order;
+    b.xe.assign(order * order, 0);
+    b.ye.assign(order * order, 0);
+    int s = 0;
+    for (int i = 0; i < order; i++)
+        for (int j = 0; j <= i; j++) { b.xe[s] = i - j; b.ye[s] = j; s++; }
+    for (int i = order - 1; i > 0; i--)
+        for (int j = 0; j < i; j++) { b.xe[s] = order - 1 - j; b.ye[s] = order - i + j; s++; }
+    if (order == 1) {
+        b.coef = {1.0};
+        return b;
+    }
+    std::vector<double> V((size_t)b.nb * b.nb), inv;
+    int r = 0;
+    for (int i = 0; i < order; i++)
+        for (int j = 0; j < order; j++, r++) {
+            const double x = (double)i / (order - 1), y = (double)j / (order - 1);
+            for (int k = 0; k < b.nb; k++) V[(size_t)r * b.nb + k] = std::pow(x, b.xe[k]) * std::pow(y, b.ye[k]);
+        }
+    invert_scaled_pivot(V, b.nb, inv);
+    b.coef = inv;
+    return b;
+}
 double basis_fun(const Basis& b, int m, double x, double y) {
     double v = 0;
     for (int i = 0; i < b.nb; i++) v += b.coef[(size_t)i * b.nb + m] * std::pow(x, b.xe[i]) * std::pow(y, b.ye[i]);
@@ -543,7 +651,7 @@ static void free_all(fp_ctx* c) {
     if (c->g_factor) (void)hipGraphExecDestroy(c->g_factor);
     if (c->g_apply) (void)hipGraphExecDestroy(c->g_apply);
     if (c->io) (void)hipFree(c->io);
-    void* ptrs[] = {c->av, c->avold, c->trunc, c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
+    void* ptrs[] = {c->qv, c->qf, c->qn, c->xq, c->qmass, c->qminv, c->qis, c->av, c->avold, c->trunc, c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
                     c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->val, c->F, c->iy, c->ixs, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
                     c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->istat};
     for (void* p : ptrs)
@@ -682,6 +790,252 @@ static int setup_tri(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
     HIPCHK(c, upload(&c->esize, esize));
     HIPCHK(c, upload(&c->nbrf, nbrf));
     if (!wdist.empty()) HIPCHK(c, upload(&c->wdist, wdist));
+    return FP_OK;
+}
+
+// transformation2Dsquare.getX (FlowPro/src/main/java/flowpro/core/transformation/
+// transformation2Dsquare.java:50-53) with the reference's evaluation order and no fused
+// multiply-add, ab = (A0..A3, B0..B3)
+static void quad_getX(const double* ab, const double* xi, double* X) {
+#pragma clang fp contract(off)
+    X[0] = ab[0] * xi[0] * xi[1] + ab[1] * xi[0] + ab[2] * xi[1] + ab[3];
+    X[1] = ab[4] * xi[0] * xi[1] + ab[5] * xi[0] + ab[6] * xi[1] + ab[7];
+}
+
+// transformation2Dsquare.getXi (:64-91): Newton from (0.5, 0.5) with the forward-difference
+// Jacobian J[j] = (getX(Xi + h e_j) - getX(Xi)) * (1 / h), h = 1e-8, transposed and inverted by
+// Mat.invert, at most 50 steps, stopping when |Xi_new - Xi| < 1e-13.  The neighbour-side basis of a
+// face point is evaluated at this iterate, as Integration.Face does; its last step leaves the
+// point off by up to ~1e-15, which moves a residual at order 4 by ~3e-12 relative, so the product
+// takes the same iterate rather than the exact point.
+static void quad_getXi(const double* ab, const double* X, double* out) {
+#pragma clang fp contract(off)
+    double Xi[2] = {0.5, 0.5}, Xin[2] = {0.5, 0.5};
+    const double h = 1e-8;
+    for (int i = 0; i < 50; i++) {
+        double X0[2];
+        quad_getX(ab, Xi, X0);
+        std::vector<double> J(4), iJ;
+        for (int j = 0; j < 2; j++) {
+            double Xp[2] = {Xi[0] + (j == 0 ? h : 0.0), Xi[1] + (j == 1 ? h : 0.0)}, Xh[2];
+            quad_getX(ab, Xp, Xh);
+            for (int d = 0; d < 2; d++) J[(size_t)d * 2 + j] = (Xh[d] - X0[d]) * (1 / h);   // row j, transposed
+        }
+        invert_scaled_pivot(J, 2, iJ);
+        const double r0 = X0[0] - X[0], r1 = X0[1] - X[1];
+        for (int d = 0; d < 2; d++) {
+            double t = 0;
+            t += iJ[(size_t)d * 2] * r0;
+            t += iJ[(size_t)d * 2 + 1] * r1;
+            Xin[d] = Xi[d] - t;
+        }
+        double n2 = 0;
+        n2 += (Xin[0] - Xi[0]) * (Xin[0] - Xi[0]);
+        n2 += (Xin[1] - Xi[1]) * (Xin[1] - Xi[1]);
+        if (std::sqrt(n2) < 1e-13) break;
+        if (std::isnan(Xin[0]) || std::isnan(Xin[1])) {
+            Xin[0] = Xin[1] = 0.5;
+            break;
+        }
+        Xi[0] = Xin[0];
+        Xi[1] = Xin[1];
+    }
+    out[0] = Xin[0];
+    out[1] = Xin[1];
+}
+
+// quadrilaterals: neighbour matrix, numbering, basis2Dsquare tables, transformation2Dsquare
+// (x = A0 xi eta + A1 xi + A2 eta + A3 over the reference square (0,0), (1,0), (1,1), (0,1)) and
+// faceTransformation2Dline geometry (Element.computeGeometry), per point where the bilinear map
+// makes it vary: |det J| w, the physical basis gradients d_x = iT^T d_xi, the element mass matrix
+static int setup_quad(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
+    const int n = c->nElem;
+    Basis bs = make_basis_square(cs->order);
+    std::vector<int> TT((size_t)4 * n);
+    int rc = fp_neighbor_matrix_quad(n, cs->tp, cs->n_bnd, cs->bnd, TT.data());
+    if (rc) return rc;
+    c->order_.resize(n);
+    if (cs->ordering == 1) fp_rcm_order_quad(n, TT.data(), c->order_.data());
+    else
+        for (int i = 0; i < n; i++) c->order_[i] = i;
+    c->rank.resize(n);
+    for (int r = 0; r < n; r++) c->rank[c->order_[r]] = r;
+
+    const int NB = c->NB, NQV = c->NQV, NQF = c->NQF, NQT = NQV + 4 * NQF;
+    static const double CX[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};   // transformation2Dsquare.coordsXi
+    // reference points: volume (the tensor rule on [0,1]^2), then face k point p
+    std::vector<double> xr((size_t)NQT * 2);
+    for (int q = 0; q < NQV; q++) { xr[2 * q] = cs->vol_xi[2 * q]; xr[2 * q + 1] = cs->vol_xi[2 * q + 1]; }
+    for (int k = 0; k < 4; k++)
+        for (int p = 0; p < NQF; p++) {
+            const double t = cs->face_xi[p];
+            const int a = k, b = (k + 1) % 4;
+            const size_t i = NQV + (size_t)k * NQF + p;
+            xr[2 * i] = (1 + t) / 2 * CX[b][0] + (1 - t) / 2 * CX[a][0];
+            xr[2 * i + 1] = (1 + t) / 2 * CX[b][1] + (1 - t) / 2 * CX[a][1];
+        }
+    std::vector<double> phi((size_t)NQT * NB), dxi((size_t)NQT * NB * 2);
+    for (int i = 0; i < NQT; i++)
+        for (int j = 0; j < NB; j++) {
+            phi[(size_t)i * NB + j] = basis_fun(bs, j, xr[2 * i], xr[2 * i + 1]);
+            dxi[((size_t)i * NB + j) * 2] = basis_der(bs, j, xr[2 * i], xr[2 * i + 1], 0);
+            dxi[((size_t)i * NB + j) * 2 + 1] = basis_der(bs, j, xr[2 * i], xr[2 * i + 1], 1);
+        }
+    std::vector<double> vb(phi.begin(), phi.begin() + (size_t)NQV * NB), fb(phi.begin() + (size_t)NQV * NB, phi.end());
+    std::vector<double> vw(cs->vol_w, cs->vol_w + NQV), fw(cs->face_w, cs->face_w + NQF);
+    std::vector<double> tab;
+    auto put = [&](const std::vector<double>& v) {
+        size_t off = tab.size();
+        tab.insert(tab.end(), v.begin(), v.end());
+        return off;
+    };
+    size_t o_vb = put(vb), o_vw = put(vw), o_fb = put(fb), o_fw = put(fw);
+    HIPCHK(c, upload(&c->tab, tab));
+    c->qtb = QuadTables{c->tab + o_vb, c->tab + o_vw, c->tab + o_fb, c->tab + o_fw};
+
+    // per element, DOF order
+    std::vector<double> qv((size_t)n * NQV * (1 + 2 * NB)), qf((size_t)n * 4 * NQF * 2 * NB), xq((size_t)n * NQT * 2);
+    std::vector<double> fgeo((size_t)n * 16), esize(n), mass((size_t)n * NB * NB), minv, qis((size_t)n * NB), Xs(2 * (size_t)n);
+    std::vector<double> AB((size_t)n * 8);
+    c->h_qis.assign((size_t)n * NB, 0.0);
+    auto getX = [&](const double* ab, double x, double y, double* X) {
+        X[0] = ab[0] * x * y + ab[1] * x + ab[2] * y + ab[3];
+        X[1] = ab[4] * x * y + ab[5] * x + ab[6] * y + ab[7];
+    };
+    auto jacobi = [&](const double* ab, double x, double y, double J[2][2]) {
+        J[0][0] = ab[0] * y + ab[1];
+        J[0][1] = ab[0] * x + ab[2];
+        J[1][0] = ab[4] * y + ab[5];
+        J[1][1] = ab[4] * x + ab[6];
+    };
+    for (int r = 0; r < n; r++) {
+        const int e = c->order_[r];
+        double vx[4], vy[4];
+        for (int v = 0; v < 4; v++) {
+            vx[v] = cs->pxy[2 * cs->tp[4 * e + v]];
+            vy[v] = cs->pxy[2 * cs->tp[4 * e + v] + 1];
+        }
+        // the solution of (xi eta, xi, eta, 1) . A = x at the four reference vertices
+        double* ab = &AB[(size_t)r * 8];
+        ab[0] = vx[0] - vx[1] + vx[2] - vx[3]; ab[1] = vx[1] - vx[0]; ab[2] = vx[3] - vx[0]; ab[3] = vx[0];
+        ab[4] = vy[0] - vy[1] + vy[2] - vy[3]; ab[5] = vy[1] - vy[0]; ab[6] = vy[3] - vy[0]; ab[7] = vy[0];
+        const double centre[2] = {0.5, 0.5};
+        quad_getX(ab, centre, &Xs[2 * (size_t)r]);   // transformation2Dsquare.getXs
+        for (int i = 0; i < NQT; i++) {
+            double J[2][2];
+            jacobi(ab, xr[2 * i], xr[2 * i + 1], J);
+            const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+            const double iT[2][2] = {{J[1][1] / det, -J[0][1] / det}, {-J[1][0] / det, J[0][0] / det}};
+            double* g = i < NQV ? &qv[((size_t)r * NQV + i) * (1 + 2 * NB) + 1] : &qf[((size_t)r * 4 * NQF + (i - NQV)) * 2 * NB];
+            if (i < NQV) g[-1] = std::fabs(det) * vw[i];
+            for (int j = 0; j < NB; j++) {
+                const double* d = &dxi[((size_t)i * NB + j) * 2];
+                g[2 * j] = iT[0][0] * d[0] + iT[1][0] * d[1];
+                g[2 * j + 1] = iT[0][1] * d[0] + iT[1][1] * d[1];
+            }
+            quad_getX(ab, &xr[2 * i], &xq[((size_t)r * NQT + i) * 2]);
+        }
+        double area = 0;
+        for (int q = 0; q < NQV; q++) area += qv[((size_t)r * NQV + q) * (1 + 2 * NB)];
+        double* M = &mass[(size_t)r * NB * NB];
+        for (int a = 0; a < NB; a++) {
+            for (int b = 0; b < NB; b++) {
+                double sum = 0;
+                for (int q = 0; q < NQV; q++) sum += qv[((size_t)r * NQV + q) * (1 + 2 * NB)] * vb[(size_t)q * NB + a] * vb[(size_t)q * NB + b];
+                M[a * NB + b] = sum;
+            }
+            double is = 0;
+            for (int q = 0; q < NQV; q++) is += qv[((size_t)r * NQV + q) * (1 + 2 * NB)] * vb[(size_t)q * NB + a];
+            qis[(size_t)r * NB + a] = is / area;
+            c->h_qis[(size_t)e * NB + a] = is / area;
+        }
+        esize[r] = area;   // completed below with the perimeter
+    }
+    minv.assign(mass.size(), 0.0);
+    for (int r = 0; r < n; r++) {
+        std::vector<double> M(mass.begin() + (size_t)r * NB * NB, mass.begin() + (size_t)(r + 1) * NB * NB), Mi;
+        if (!invert(M, NB, Mi)) {
+            c->err = "singular element mass matrix";
+            return FP_EINVAL;
+        }
+        std::copy(Mi.begin(), Mi.end(), minv.begin() + (size_t)r * NB * NB);
+    }
+    nbr.assign(4 * (size_t)n, 0);
+    std::vector<int> nbrf(4 * (size_t)n, 0);
+    // the neighbour's basis and physical gradients at the own face points (Integration.Face:
+    // coordinatesFaceXiRight = elemRight.transform.getXi(coordinatesFace))
+    std::vector<double> qn((size_t)n * 4 * NQF * 3 * NB, 0.0);
+    for (int r = 0; r < n; r++) {
+        const int e = c->order_[r];
+        for (int k = 0; k < 4; k++) {
+            const int t = TT[4 * (size_t)e + k];
+            if (t < 0) continue;
+            const double* abt = &AB[(size_t)c->rank[t] * 8];
+            for (int p = 0; p < NQF; p++) {
+                const double* X = &xq[((size_t)r * NQT + NQV + (size_t)k * NQF + p) * 2];
+                double xi[2], J[2][2];
+                quad_getXi(abt, X, xi);
+                jacobi(abt, xi[0], xi[1], J);
+                const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+                const double iT[2][2] = {{J[1][1] / det, -J[0][1] / det}, {-J[1][0] / det, J[0][0] / det}};
+                double* o = &qn[(((size_t)r * 4 + k) * NQF + p) * 3 * NB];
+                for (int j = 0; j < NB; j++) {
+                    const double d0 = basis_der(bs, j, xi[0], xi[1], 0), d1 = basis_der(bs, j, xi[0], xi[1], 1);
+                    o[j] = basis_fun(bs, j, xi[0], xi[1]);
+                    o[NB + 2 * j] = iT[0][0] * d0 + iT[1][0] * d1;
+                    o[NB + 2 * j + 1] = iT[0][1] * d0 + iT[1][1] * d1;
+                }
+            }
+        }
+    }
+    double fwsum = 0;
+    for (int p = 0; p < NQF; p++) fwsum += fw[p];
+    for (int r = 0; r < n; r++) {
+        const int e = c->order_[r];
+        const double* ab = &AB[(size_t)r * 8];
+        double perim = 0;
+        for (int k = 0; k < 4; k++) {
+            const int a = k, b = (k + 1) % 4;
+            const double s0 = CX[b][0] - CX[a][0], s1 = CX[b][1] - CX[a][1];
+            // J s is the same at every point of a straight edge of the square
+            double J[2][2];
+            jacobi(ab, CX[a][0], CX[a][1], J);
+            const double Js0 = J[0][0] * s0 + J[0][1] * s1, Js1 = J[1][0] * s0 + J[1][1] * s1;
+            const double len = std::sqrt(Js0 * Js0 + Js1 * Js1);
+            const double nx = Js1 / len, ny = -Js0 / len;
+            const int t = TT[4 * (size_t)e + k];
+            double dL;
+            if (t > -1) {
+                const int rt = c->rank[t];
+                dL = (Xs[2 * r] - Xs[2 * rt]) * nx + (Xs[2 * r + 1] - Xs[2 * rt + 1]) * ny;
+                nbr[4 * r + k] = rt;
+                int kr = 0;
+                for (int s2 = 0; s2 < 4; s2++)
+                    if (TT[4 * (size_t)t + s2] == e) { kr = s2; break; }
+                nbrf[4 * r + k] = kr;
+            } else {
+                double mid[2], xm[2] = {(CX[a][0] + CX[b][0]) / 2, (CX[a][1] + CX[b][1]) / 2};
+                quad_getX(ab, xm, mid);
+                dL = (Xs[2 * r] - mid[0]) * nx + (Xs[2 * r + 1] - mid[1]) * ny;
+                nbr[4 * r + k] = t;
+                nbrf[4 * r + k] = 0;
+            }
+            double* f = &fgeo[((size_t)r * 4 + k) * 4];
+            f[0] = nx; f[1] = ny; f[2] = len / 2; f[3] = std::fabs(dL);
+            perim += len / 2 * fwsum;   // S_k = sum_p JacobianFace w_p
+        }
+        esize[r] = 4 * esize[r] / perim;   // elemSize = nFaces * area / sum S (Element.java:549-554)
+    }
+    HIPCHK(c, upload(&c->qv, qv));
+    HIPCHK(c, upload(&c->qf, qf));
+    HIPCHK(c, upload(&c->qn, qn));
+    HIPCHK(c, upload(&c->xq, xq));
+    HIPCHK(c, upload(&c->fgeo, fgeo));
+    HIPCHK(c, upload(&c->esize, esize));
+    HIPCHK(c, upload(&c->qmass, mass));
+    HIPCHK(c, upload(&c->qminv, minv));
+    HIPCHK(c, upload(&c->qis, qis));
+    HIPCHK(c, upload(&c->nbrf, nbrf));
     return FP_OK;
 }
 
@@ -853,7 +1207,16 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     c->nElem = n;
     c->order = cs->order;
     c->dim = cs->dimension == 3 ? 3 : 2;
-    c->NF = c->dim + 1;
+    c->quad = c->dim == 2 && cs->elem_type == 4;
+    c->NF = c->quad ? 4 : c->dim + 1;
+    if (cs->elem_type != 0 && cs->elem_type != 3 && cs->elem_type != 4) {
+        c->err = "elem_type: 3 (or 0) triangles, 4 quadrilaterals";
+        return FP_EINVAL;
+    }
+    if (c->quad && (cs->model == MODEL_SA || cs->model == MODEL_SCALAR)) {
+        c->err = "quadrilaterals: device kernels for the NavierStokes and AdiabaticEulers models";
+        return FP_UNSUPPORTED;
+    }
     if (c->dim == 3) {
         if (cs->model != MODEL_NAVIER_STOKES || cs->viscous) {
             c->err = "tetrahedra: only the inviscid NavierStokes model has device kernels";
@@ -869,6 +1232,9 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         }
         c->NB = cs->order == 1 ? 1 : 4;
         c->NEQ = 5;
+    } else if (c->quad) {
+        c->NB = cs->order * cs->order;
+        c->NEQ = cs->model == MODEL_ADIABATIC ? 3 : 4;
     } else {
         c->NB = make_basis(cs->order).nb;
         c->NEQ = cs->model == MODEL_SCALAR ? 1 : (cs->model == MODEL_SA ? 5 : (cs->model == MODEL_ADIABATIC ? 3 : 4));
@@ -890,7 +1256,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         return FP_EINVAL;
     }
     const bool av = cs->damp_tol > 0 || cs->damp_const != 0 || cs->damp_inner_tol > 0;
-    if (av && (c->dim == 3 || cs->model == MODEL_SA)) {
+    if (av && (c->dim == 3 || c->quad || cs->model == MODEL_SA)) {
         c->err = "artificial viscosity: device kernels for the 2D NavierStokes and scalar models only";
         return FP_UNSUPPORTED;
     }
@@ -900,7 +1266,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         return FP_EINVAL;
     }
     c->L = ElemLaunch{cs->model, c->NB, c->NEQ, c->NQV, c->NQF, diffusive ? 1 : 0, av ? 1 : 0};
-    if (!(c->dim == 3 ? tet_supported(c->L) : element_supported(c->L) && team_supported(c->L))) {
+    if (!(c->dim == 3 ? tet_supported(c->L) : c->quad ? quad_supported(c->L) : element_supported(c->L) && team_supported(c->L))) {
         c->err = "no element kernel for this model / order / quadrature";
         return FP_UNSUPPORTED;
     }
@@ -960,7 +1326,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
 
     HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     std::vector<int> nbr;
-    int rc = c->dim == 3 ? setup_tet(cs, c, nbr) : setup_tri(cs, c, nbr);
+    int rc = c->dim == 3 ? setup_tet(cs, c, nbr) : c->quad ? setup_quad(cs, c, nbr) : setup_tri(cs, c, nbr);
     if (rc) return rc;
     const int NF = c->NF;
 
@@ -1091,6 +1457,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     }
     c->ms = Mesh{n, c->geo, c->fgeo, c->nbr, c->nbrf, c->blkn, c->blkd, c->wdist, c->av};
     c->tms = TetMesh{n, c->geo, c->fgeo, c->nbr, c->blkn, c->blkd};
+    c->qms = QuadMesh{n, c->qv, c->qf, c->qn, c->xq, c->fgeo, c->qmass, c->qminv, c->qis, c->nbr, c->nbrf, c->blkn, c->blkd};
 
     // --- state, matrix, Krylov workspace
     const size_t N = (size_t)c->dofs, BB = (size_t)c->B * c->B;
@@ -1261,7 +1628,7 @@ static int assemble_impl(fp_ctx* c, double dt, double dto, int time_order, int w
     }
     if (eq_norms || rhs_norm) {
         const int ne = c->NEQ;
-        HIPCHK(c, launch_colsum(c->s, c->nElem, 2 * ne, c->enorm, c->scal));
+        HIPCHK(c, launch_colsum(c->s, c->nElem, 2 * ne, c->enorm, c->partial, c->scal));   // 256 x 2 NEQ <= 4096 partials
         double h[10];
         HIPCHK(c, hipMemcpyAsync(h, c->scal, sizeof(double) * 2 * ne, hipMemcpyDeviceToHost, c->s));
         HIPCHK(c, hipStreamSynchronize(c->s));
@@ -1576,7 +1943,7 @@ extern "C" int fp_adjoint_solve(fp_ctx* c, double a1, const double* rhs, double*
 
 extern "C" int fp_functional(fp_ctx* c, int which, double* value, double* dIdW) {
     if (!c || !value || (which != 0 && which != 1) || c->mp.kind != MODEL_NAVIER_STOKES) return FP_EINVAL;
-    if (c->dim != 2) return FP_UNSUPPORTED;   // the force coefficients integrate over wall edges
+    if (c->dim != 2 || c->quad) return FP_UNSUPPORTED;   // the force coefficients integrate over triangle wall edges
     HIPCHK(c, launch_functional(c->L, c->s, c->tb, c->ms, c->W, c->kappa, which, c->tmp, dIdW ? c->w : nullptr));
     HIPCHK(c, launch_dot(c->s, (size_t)c->nElem, c->tmp, nullptr, c->partial, c->scal, 2));
     HIPCHK(c, hipMemcpyAsync(value, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -1664,7 +2031,8 @@ extern "C" int fp_export_averages(fp_ctx* c, double* avg) {
     for (int e = 0; e < c->nElem; e++)
         for (int m = 0; m < c->NEQ; m++) {
             double s = 0;
-            for (int j = 0; j < c->NB; j++) s = s + W[(size_t)e * c->B + m * c->NB + j] * c->h_isref[j];
+            for (int j = 0; j < c->NB; j++)
+                s = s + W[(size_t)e * c->B + m * c->NB + j] * (c->quad ? c->h_qis[(size_t)e * c->NB + j] : c->h_isref[j]);
             avg[(size_t)e * c->NEQ + m] = s;
         }
     return FP_OK;

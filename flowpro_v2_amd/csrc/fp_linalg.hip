@@ -1009,7 +1009,7 @@ hipError_t launch_ilu_copy(hipStream_t s, size_t n, const double* src, TF* dst) 
     return hipGetLastError();
 }
 
-#define FP_BLOCK_SIZES(X) X(1) X(3) X(5) X(6) X(9) X(10) X(12) X(15) X(18) X(20) X(24) X(30) X(40)
+#define FP_BLOCK_SIZES(X) X(1) X(3) X(4) X(5) X(6) X(9) X(10) X(12) X(15) X(16) X(18) X(20) X(24) X(27) X(30) X(36) X(40) X(48) X(64)
 
 template <typename TF>
 hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* colidx,
@@ -1151,16 +1151,39 @@ hipError_t launch_min(hipStream_t s, size_t n, const double* a, double* partial,
     return hipGetLastError();
 }
 
-// sum of per-element norms: out[c] = sum_e en[e*nc + c] (deterministic, one block per component)
-__global__ void __launch_bounds__(RED_THREADS) k_colsum(int nElem, int nc, const double* __restrict__ en, double* __restrict__ out) {
+// sum of per-element norms: out[c] = sum_e en[e*nc + c], deterministic (fixed grid and order):
+// COLSUM_BLOCKS blocks accumulate every component of a grid-strided set of elements, then one
+// block per component adds the block partials
+constexpr int COLSUM_BLOCKS = 256;
+constexpr int COLSUM_MAXC = 16;
+__global__ void __launch_bounds__(RED_THREADS) k_colsum_partial(int nElem, int nc, const double* __restrict__ en,
+                                                                double* __restrict__ partial) {
+    double acc[COLSUM_MAXC];
+#pragma unroll
+    for (int c = 0; c < COLSUM_MAXC; c++) acc[c] = 0.0;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nElem; e += gridDim.x * blockDim.x) {
+        const double* row = en + (size_t)e * nc;
+#pragma unroll
+        for (int c = 0; c < COLSUM_MAXC; c++)
+            if (c < nc) acc[c] += row[c];
+    }
+    for (int c = 0; c < nc; c++) {
+        const double t = block_sum(acc[c]);
+        if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = t;
+    }
+}
+__global__ void __launch_bounds__(RED_THREADS) k_colsum_final(int np, const double* __restrict__ partial, double* __restrict__ out) {
     const int c = blockIdx.x;
     double s = 0.0;
-    for (int e = threadIdx.x; e < nElem; e += blockDim.x) s += en[(size_t)e * nc + c];
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s += partial[(size_t)c * np + i];
     s = block_sum(s);
     if (threadIdx.x == 0) out[c] = s;
 }
-hipError_t launch_colsum(hipStream_t s, int nElem, int nc, const double* en, double* out) {
-    hipLaunchKernelGGL(k_colsum, dim3(nc), dim3(RED_THREADS), 0, s, nElem, nc, en, out);
+// partial: COLSUM_BLOCKS * nc doubles
+hipError_t launch_colsum(hipStream_t s, int nElem, int nc, const double* en, double* partial, double* out) {
+    if (nc < 1 || nc > COLSUM_MAXC) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_colsum_partial, dim3(COLSUM_BLOCKS), dim3(RED_THREADS), 0, s, nElem, nc, en, partial);
+    hipLaunchKernelGGL(k_colsum_final, dim3(nc), dim3(RED_THREADS), 0, s, COLSUM_BLOCKS, partial, out);
     return hipGetLastError();
 }
 

@@ -24,7 +24,8 @@ EXPORTS = ["fp_neighbor_matrix", "fp_rcm_order", "fp_create", "fp_destroy", "fp_
            "fp_assemble", "fp_linear_solve", "fp_update", "fp_export_residual", "fp_export_jacobian",
            "fp_export_solution", "fp_jv", "fp_jtv", "fp_time_kernel", "fp_stream", "fp_linear_solve_jfnk",
            "fp_functional", "fp_adjoint_solve", "fp_wall_distance", "fp_neighbor_matrix_tet", "fp_rcm_order_tet",
-           "fp_limiter", "fp_export_artificial_viscosity", "fp_export_averages"]
+           "fp_limiter", "fp_export_artificial_viscosity", "fp_export_averages", "fp_neighbor_matrix_quad",
+           "fp_rcm_order_quad"]
 
 
 class FpCase(C.Structure):
@@ -51,6 +52,7 @@ class FpCase(C.Structure):
         ("damp_tol", C.c_double), ("damp_const", C.c_double), ("damp_inner_tol", C.c_double),
         ("n_damp_inner_coef", C.c_int), ("damp_inner_coef", C.c_double * 5),
         ("numerical_flux", C.c_int),
+        ("elem_type", C.c_int),
     ]
 
 
@@ -95,6 +97,8 @@ def lib() -> C.CDLL:
         "fp_wall_distance": (C.c_int, [C.c_int, dp, C.c_int, ip, ip, dp]),
         "fp_neighbor_matrix_tet": (C.c_int, [C.c_int, ip, C.c_int, ip, ip]),
         "fp_rcm_order_tet": (C.c_int, [C.c_int, ip, ip]),
+        "fp_neighbor_matrix_quad": (C.c_int, [C.c_int, ip, C.c_int, ip, ip]),
+        "fp_rcm_order_quad": (C.c_int, [C.c_int, ip, ip]),
         "fp_limiter": (C.c_int, [vp, C.c_int]),
         "fp_export_artificial_viscosity": (C.c_int, [vp, dp, dp]),
         "fp_export_averages": (C.c_int, [vp, dp]),
@@ -146,6 +150,26 @@ def rcm_order_tet(TT: np.ndarray) -> np.ndarray:
     rc = lib().fp_rcm_order_tet(TT.shape[0], iptr(TT), iptr(order))
     if rc:
         raise FatalError(f"fp_rcm_order_tet: {rc}")
+    return order
+
+
+def neighbor_matrix_quad(TP: np.ndarray, bnd: np.ndarray) -> np.ndarray:
+    """createNeighborMatrix on quadrilaterals (fp_neighbor_matrix_quad): TT[n_elems][4]."""
+    TP = np.ascontiguousarray(TP, dtype=np.int32)
+    bnd = np.ascontiguousarray(bnd, dtype=np.int32).reshape(-1, 3)
+    TT = np.zeros_like(TP)
+    rc = lib().fp_neighbor_matrix_quad(TP.shape[0], iptr(TP), bnd.shape[0], iptr(bnd), iptr(TT))
+    if rc:
+        raise FatalError(f"fp_neighbor_matrix_quad: {rc}")
+    return TT
+
+
+def rcm_order_quad(TT: np.ndarray) -> np.ndarray:
+    TT = np.ascontiguousarray(TT, dtype=np.int32)
+    order = np.zeros(TT.shape[0], np.int32)
+    rc = lib().fp_rcm_order_quad(TT.shape[0], iptr(TT), iptr(order))
+    if rc:
+        raise FatalError(f"fp_rcm_order_quad: {rc}")
     return order
 
 

@@ -41,6 +41,16 @@ def triangle_rule(order: int):
     return np.ascontiguousarray(xy), w
 
 
+def square_rule(order: int):
+    """QuadratureSquare (FlowPro/src/main/java/flowpro/core/quadrature/QuadratureSquare.java:7-25):
+    the tensor product of the Gauss rule of the order on [0, 1]^2, point (i, j) ->
+    ((1 + x_i) / 2, (1 + x_j) / 2), weight w_i w_j / 4, i outer."""
+    x, w = line_rule(order)
+    X = np.array([[(1 + x[i]) / 2, (1 + x[j]) / 2] for i in range(len(x)) for j in range(len(x))])
+    W = np.array([w[i] * w[j] / 4 for i in range(len(x)) for j in range(len(x))])
+    return np.ascontiguousarray(X), W
+
+
 def tetra_rule(order: int):
     """Reference-tetrahedron rule of degree 2k - 1 (QuadratureTetra: the file holds the
     (xi, eta, zeta) coordinates directly; the rules have positive weights, no swap)."""

@@ -50,9 +50,13 @@ class DeviceCase:
                 raise UnsupportedCase("attack angle on tetrahedra")
             vxi, vw = quadrature.tetra_rule(order)
             fxi, fw = quadrature.triangle_rule(order)
+        elif case.TP.shape[1] == 4:   # squareElement
+            vxi, vw = quadrature.square_rule(order)
+            fxi, fw = quadrature.line_rule(order)
         else:
             vxi, vw = quadrature.triangle_rule(order)
             fxi, fw = quadrature.line_rule(order)
+        self.quad = self.dim == 2 and case.TP.shape[1] == 4
         self._keep = [np.ascontiguousarray(case.PXY, np.float64), np.ascontiguousarray(case.TP, np.int32),
                       np.ascontiguousarray(case.bnd, np.int32), np.ascontiguousarray(vxi), np.ascontiguousarray(vw),
                       np.ascontiguousarray(fxi), np.ascontiguousarray(fw)]
@@ -85,6 +89,7 @@ class DeviceCase:
         cs.gmres_tol = float(props.get("iterativeSolverTol", 1e-2))
         cs.positivity_limiter = int(bool(props.get("positivityLimiter", True)))
         cs.dimension = self.dim
+        cs.elem_type = 4 if self.quad else 3
         # DGpure.initMethod: dampTol and dampConst are required keys of the DG methods, the inner
         # damping keys optional
         cs.damp_tol = float(props.get("dampTol", 0.0))
@@ -226,7 +231,7 @@ class DeviceCase:
 
     def jacobian(self):
         d = np.zeros((self.n_elem, self.B, self.B))
-        n = np.zeros((self.n_elem, self.dim + 1, self.B, self.B))
+        n = np.zeros((self.n_elem, self.case.TP.shape[1] if self.dim == 2 else 4, self.B, self.B))
         self._check(native.lib().fp_export_jacobian(self.h, native.dptr(d), native.dptr(n)), "fp_export_jacobian")
         return d, n
 

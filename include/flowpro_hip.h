@@ -38,19 +38,21 @@ enum {
 typedef struct fp_ctx fp_ctx;
 
 /* Case description: the inputs of FlowProMain.solverFactory and Mesh
- * (FlowProMain.java:486-730, Mesh.java:69-224) for straight triangles of one order, or for
- * straight tetrahedra (dimension = 3; then the array shapes in brackets after "3D:" apply). */
+ * (FlowProMain.java:486-730, Mesh.java:69-224) for straight triangles of one order, for
+ * quadrilaterals (elem_type = 4, elementType.txt type 4; "quad:" shapes), or for straight
+ * tetrahedra (dimension = 3; then the array shapes in brackets after "3D:" apply). */
 typedef struct fp_case {
     int n_points;
     const double* pxy;            /* [n_points][2]; 3D: [n_points][3] */
     int n_elems;
-    const int* tp;                /* [n_elems][3], counter-clockwise; 3D: [n_elems][4] */
+    const int* tp;                /* [n_elems][3], counter-clockwise; quad: [n_elems][4]; 3D: [n_elems][4] */
     int n_bnd;
     const int* bnd;               /* [n_bnd][3]: boundary type, v0, v1 (boundaryType.txt); 3D: [n_bnd][4] */
     const double* wall_distance;  /* [n_points] or NULL */
     int order;                    /* element order k = p + 1 */
     int n_vol;                    /* volume rule (QuadratureCentral: degree 2k-1, positive weights) */
-    const double* vol_xi;         /* [n_vol][2] reference-triangle coordinates; 3D: [n_vol][3] gaussTetra */
+    const double* vol_xi;         /* [n_vol][2] reference-triangle coordinates; quad: QuadratureSquare of
+                                     order k on [0,1]^2 (k^2 points); 3D: [n_vol][3] gaussTetra */
     const double* vol_w;          /* [n_vol] */
     int n_face;                   /* Gauss-Legendre rule of order k on [-1, 1]; 3D: the triangle rule */
     const double* face_xi;        /* [n_face]; 3D: [n_face][2] reference-triangle coordinates */
@@ -89,6 +91,10 @@ typedef struct fp_case {
     double damp_inner_coef[5];
     /* Aerodynamics.numericalFluxType of NavierStokes: 0 "default" (Lax-Friedrichs), 1 "vanLeer" */
     int numerical_flux;
+    /* 2D element type of every element (elementType.txt): 3 (or 0) triangleElement, 4 squareElement
+     * (basis2Dsquare, transformation2Dsquare); quadrilaterals carry the NavierStokes and
+     * AdiabaticEulers models at orders 1-4, without artificial viscosity */
+    int elem_type;
 } fp_case;
 
 /* --- integer setup (bit-exact with the reference) ------------------------------ */
@@ -100,6 +106,10 @@ int fp_rcm_order(int n_elems, const int* tt, int* order);
  * bnd[n_bnd][4] = type, v0, v1, v2; tt[n_elems][4] */
 int fp_neighbor_matrix_tet(int n_elems, const int* tp, int n_bnd, const int* bnd, int* tt);
 int fp_rcm_order_tet(int n_elems, const int* tt, int* order);
+/* the same two for quadrilaterals (squareElement.getFaceIndexes(k) = {k, (k + 1) % 4}):
+ * tp[n_elems][4], bnd[n_bnd][3], tt[n_elems][4] */
+int fp_neighbor_matrix_quad(int n_elems, const int* tp, int n_bnd, const int* bnd, int* tt);
+int fp_rcm_order_quad(int n_elems, const int* tt, int* order);
 /* FlowProMain.generateWallDistanceFunction / generateDistanceFunction (FlowProMain.java:931-1014):
  * distance of every vertex from the wall faces (tt == -1) along mesh edges; dist[n_points] */
 int fp_wall_distance(int n_points, const double* pxy, int n_elems, const int* tp, const int* tt, double* dist);

@@ -27,7 +27,8 @@ _D = ["kappa", "cv", "pIn0", "rhoIn0", "pOut", "reynolds", "prandtl", "attack_an
 class Props(C.Structure):
     _fields_ = [(n, C.c_int) for n in _I] + [(n, C.c_double) for n in _D] + [("dimension", C.c_int)] + \
         [("damp_tol", C.c_double), ("damp_const", C.c_double), ("damp_inner_tol", C.c_double),
-         ("n_damp_inner_coef", C.c_int), ("damp_inner_coef", C.c_double * 5), ("numerical_flux", C.c_int)]
+         ("n_damp_inner_coef", C.c_int), ("damp_inner_coef", C.c_double * 5), ("numerical_flux", C.c_int),
+         ("elem_type", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -65,7 +66,7 @@ def lib():
             "or_jvp": (None, [vp, C.c_double, dp, dp]), "or_vjp": (None, [vp, C.c_double, dp, dp]),
             "or_functional": (C.c_double, [vp, C.c_int]), "or_functional_grad_fd": (None, [vp, C.c_int, dp]),
             "or_elem_geometry": (None, [vp, C.c_int, dp, dp, dp, dp, dp, dp]),
-            "or_wall_distance": (None, [C.c_int, dp, C.c_int, ip, ip, dp]),
+            "or_wall_distance": (None, [C.c_int, dp, C.c_int, ip, ip, C.c_int, dp]), "or_nfaces": (C.c_int, [vp]),
             "or_model_pointwise": (None, [vp, dp, dp, C.c_double, dp, dp, dp]),
             "or_flux_pointwise": (None, [vp, dp, dp, dp, C.c_int, dp, dp]),
             "or_nqf": (C.c_int, [vp]),
@@ -163,7 +164,7 @@ def wall_distance(PXY, TP, TT) -> np.ndarray:
     TP = np.ascontiguousarray(TP, dtype=np.int32)
     TT = np.ascontiguousarray(TT, dtype=np.int32)
     out = np.zeros(PXY.shape[0])
-    lib().or_wall_distance(PXY.shape[0], _dp(PXY), TP.shape[0], _ip(TP), _ip(TT), _dp(out))
+    lib().or_wall_distance(PXY.shape[0], _dp(PXY), TP.shape[0], _ip(TP), _ip(TT), TP.shape[1], _dp(out))
     return out
 
 
@@ -176,6 +177,7 @@ class Oracle:
         self.P = props_from(props, threads)
         PXY = np.ascontiguousarray(case.PXY, dtype=np.float64)
         TP = np.ascontiguousarray(case.TP, dtype=np.int32)
+        self.P.elem_type = 4 if TP.shape[1] == 4 else 3
         bnd = np.ascontiguousarray(case.bnd, dtype=np.int32)
         wd = None if case.wall_distance is None else np.ascontiguousarray(case.wall_distance, dtype=np.float64)
         self.h = L.or_create(C.byref(self.P), PXY.shape[0], _dp(PXY), TP.shape[0], _ip(TP), bnd.shape[0], _ip(bnd),
@@ -187,6 +189,7 @@ class Oracle:
         self.neq = L.or_neqs(self.h)
         self.B = self.nb * self.neq
         self.dofs = L.or_dofs(self.h)
+        self.nf = L.or_nfaces(self.h)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -199,7 +202,7 @@ class Oracle:
         return dict(zip(["kapa", "pOut", "Re", "Pr", "pRef", "rhoRef", "velocityRef"], out))
 
     def TT(self):
-        out = np.zeros((self.n_elem, 3), np.int32)
+        out = np.zeros((self.n_elem, self.nf), np.int32)
         lib().or_tt(self.h, _ip(out))
         return out
 
@@ -252,7 +255,7 @@ class Oracle:
 
     def blocks(self):
         d = np.zeros((self.n_elem, self.B, self.B))
-        n = np.zeros((self.n_elem, 3, self.B, self.B))
+        n = np.zeros((self.n_elem, self.nf, self.B, self.B))
         lib().or_get_blocks(self.h, _dp(d), _dp(n))
         return d, n
 
@@ -323,8 +326,8 @@ class Oracle:
         nqf = lib().or_nqf(self.h)
         area, size = C.c_double(), C.c_double()
         Xs = np.zeros(2)
-        nrm = np.zeros((3, nqf, 2))
-        jf = np.zeros((3, nqf))
+        nrm = np.zeros((self.nf, nqf, 2))
+        jf = np.zeros((self.nf, nqf))
         M = np.zeros((self.nb, self.nb))
         lib().or_elem_geometry(self.h, i, C.byref(area), C.byref(size), _dp(Xs), _dp(nrm), _dp(jf), _dp(M))
         return dict(area=area.value, elemSize=size.value, Xs=Xs, normals=nrm, jacF=jf, M=M)
@@ -360,7 +363,7 @@ class Oracle:
         for e in range(self.n_elem):
             ge = g[e]
             A[ge:ge + self.B, ge:ge + self.B] += d[e].T
-            for k in range(3):
+            for k in range(self.nf):
                 t = TT[e, k]
                 if t > -1:
                     A[ge:ge + self.B, g[t]:g[t] + self.B] += nb[e, k].T

@@ -103,6 +103,7 @@ struct Props {
     int n_damp_inner_coef;
     double damp_inner_coef[5];
     int numerical_flux;   // Aerodynamics.numericalFluxType: 0 "default" (Lax-Friedrichs), 1 "vanLeer"
+    int elem_type;        // 2D element type of every element: 3 triangles (or 0), 4 quadrilaterals
 };
 
 struct ModelConst {
@@ -138,24 +139,29 @@ Rule load_line_rule(const std::string& dir, int order);
 Rule load_triangle_rule(const std::string& dir, int order);
 
 // ---------------------------------------------------------------- mesh / elements
+constexpr int MAXF = 4;   // faces (= vertices) of a 2D element: 3 triangle, 4 quadrilateral
+
 struct Elem {
     int index;
     int order, nBasis;
-    int TT[3];
-    int TP[3];
-    double vert[3][2];
-    double wallDist[3];
-    // transformation2Dtriangle coefficients (C/transformation/transformation2Dtriangle.java:62-83)
-    double A[3], B[3], iA[3], iB[3];
+    int nF = 3;               // faces = vertices (triangleElement / squareElement)
+    int TT[MAXF];
+    int TP[MAXF];
+    double vert[MAXF][2];
+    double wallDist[MAXF];
+    // transformation2Dtriangle coefficients (C/transformation/transformation2Dtriangle.java:62-83):
+    // x = A0 xi + A1 eta + A2; transformation2Dsquare (C/transformation/transformation2Dsquare.java:22-48):
+    // x = A0 xi eta + A1 xi + A2 eta + A3 (the same for y with B)
+    double A[4], B[4], iA[3], iB[3];
     // integration tables (C/Integration.java:40-60, 108-153)
     int nq;
     std::vector<double> JacV, wV, baseV, dbaseV, interpV;   // baseV[q*Nb+j], dbaseV[(q*Nb+j)*2+d], interpV[q*3+v]
-    int nqf[3];
-    std::vector<double> JacF[3], wF[3], baseL[3], dbaseL[3], baseR[3], dbaseR[3], interpF[3];
+    int nqf[MAXF];
+    std::vector<double> JacF[MAXF], wF[MAXF], baseL[MAXF], dbaseL[MAXF], baseR[MAXF], dbaseR[MAXF], interpF[MAXF];
     // geometry (C/element/Element.java:516-567)
-    double n[3][16][2];
-    double S[3], Xs[2], Xes[3][2], area, elemSize;
-    int faceIndexReverse[3];
+    double n[MAXF][16][2];
+    double S[MAXF], Xs[2], Xes[MAXF][2], area, elemSize;
+    int faceIndexReverse[MAXF];
     // mass (C/element/DGpure.java:547-573)
     std::vector<double> M, Is;
     // state
@@ -168,7 +174,7 @@ struct Elem {
     int gIndex0 = 0;
     // implicit storage (C/element/Implicit.java:14-126)
     std::vector<double> ADiag;          // [B][B], ADiag[i*B + j] = A[perturbed i][residual j]
-    std::vector<double> ANeigh[3];      // [B_nbr][B]
+    std::vector<double> ANeigh[MAXF];   // [B_nbr][B]
     std::vector<double> RHS, spatialR;
 };
 
@@ -178,7 +184,8 @@ struct Case {
     int nPts = 0, nElem = 0;
     std::vector<double> PXY;
     std::vector<int> TP;
-    std::vector<int> TT;    // nElem * 3
+    int nF = 3;             // faces per element (elem_type)
+    std::vector<int> TT;    // nElem * nF
     std::vector<Elem> E;
     std::vector<int> order; // element visiting order for DOF numbering
     int dofs = 0;
@@ -213,8 +220,9 @@ struct Case {
     int lin_fail = 0;
 };
 
-// FlowProMain.createNeighborMatrix (C/FlowProMain.java:772-873)
-std::vector<int> neighbor_matrix(int nElem, const int* TP, int nBnd, const int* bnd);
+// FlowProMain.createNeighborMatrix (C/FlowProMain.java:772-873); nF faces per element, face k
+// between vertices k and (k + 1) % nF (triangleElement / squareElement.getFaceIndexes)
+std::vector<int> neighbor_matrix(int nElem, const int* TP, int nBnd, const int* bnd, int nF = 3);
 // ElementOrdering.reverseCuthillMcKee (C/ElementOrdering.java:52-142)
 std::vector<int> rcm_order(int nElem, const int* TT, int nF = 3);
 // Mat.invert (API/Mat.java:502-642)
@@ -222,7 +230,10 @@ std::vector<std::vector<double>> mat_invert(const std::vector<std::vector<double
 
 // FlowProMain.generateDistanceFunction (C/FlowProMain.java:951-1014) with the wall faces
 // (Aerodynamics.isIPFace: TT == -1) as sources: distance of every vertex along mesh edges
-std::vector<double> wall_distance(int nPts, const double* PXY, int nElem, const int* TP, const int* TT);
+std::vector<double> wall_distance(int nPts, const double* PXY, int nElem, const int* TP, const int* TT, int nF = 3);
+// Transformation.getX of the element's map and the reference vertex j (transform.coordsXi)
+void elem_getX(const Elem& e, const double* Xi, double* X);
+const double* elem_ref_vertex(const Elem& e, int j);
 
 Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const int* TP,
                  int nBnd, const int* bnd, const double* wallDist, const std::string& rulesDir);

@@ -39,18 +39,16 @@ static void residuumWall(const Case& c, const Elem& e, int k, const T* Wown, con
         // interpolateVelocityAndFillElementDataObjectOnFace (Element.java:300-332)
         QPData q;
         {
-            int ei[2] = {k, (k + 1) % 3};
-            static const double CX[3][2] = {{0, 0}, {1, 0}, {0, 1}};
+            int ei[2] = {k, (k + 1) % e.nF};
             double cXi[2] = {0, 0};
             q.meshVelocity[0] = q.meshVelocity[1] = 0;
             q.wallDistance = 0;
             for (int j = 0; j < 2; j++) {
                 double ip = e.interpF[k][2 * p + j];
-                for (int d = 0; d < dim; d++) cXi[d] += ip * CX[ei[j]][d];
+                for (int d = 0; d < dim; d++) cXi[d] += ip * elem_ref_vertex(e, ei[j])[d];
                 q.wallDistance += ip * e.wallDist[ei[j]];
             }
-            q.currentX[0] = e.A[0] * cXi[0] + e.A[1] * cXi[1] + e.A[2];
-            q.currentX[1] = e.B[0] * cXi[0] + e.B[1] * cXi[1] + e.B[2];
+            elem_getX(e, cXi, q.currentX);
         }
         T WL[5], WR[5], dWL[10], dWR[10];
         for (int m = 0; m < nEqs; m++) { WL[m] = 0.0; WR[m] = 0.0; }
@@ -136,10 +134,9 @@ template <class T>
 static void residuum(const Case& c, const Elem& e, const T* Wown, const T* const* Wnb, T* K, T* const* KR) {
     const ModelConst& M = c.Mc;
     const int nEqs = M.nEqs, dim = 2, nBasis = e.nBasis;
-    for (int k = 0; k < 3; k++) residuumWall(c, e, k, Wown, Wnb[k], K, KR[k]);
+    for (int k = 0; k < e.nF; k++) residuumWall(c, e, k, Wown, Wnb[k], K, KR[k]);
     const bool volumeFluxes = e.order > 1;
     if (!(volumeFluxes || isSourcePresent(M))) return;
-    static const double CX[3][2] = {{0, 0}, {1, 0}, {0, 1}};
     for (int p = 0; p < e.nq; p++) {
         const double* base = &e.baseV[p * nBasis];
         const double* dBase = &e.dbaseV[p * nBasis * 2];
@@ -150,13 +147,12 @@ static void residuum(const Case& c, const Elem& e, const T* Wown, const T* const
             double cXi[2] = {0, 0};
             q.meshVelocity[0] = q.meshVelocity[1] = 0;
             q.wallDistance = 0;
-            for (int j = 0; j < 3; j++) {
-                double ip = e.interpV[3 * p + j];
-                for (int d = 0; d < dim; d++) cXi[d] += ip * CX[j][d];
+            for (int j = 0; j < e.nF; j++) {
+                double ip = e.interpV[e.nF * p + j];
+                for (int d = 0; d < dim; d++) cXi[d] += ip * elem_ref_vertex(e, j)[d];
                 q.wallDistance += ip * e.wallDist[j];
             }
-            q.currentX[0] = e.A[0] * cXi[0] + e.A[1] * cXi[1] + e.A[2];
-            q.currentX[1] = e.B[0] * cXi[0] + e.B[1] * cXi[1] + e.B[2];
+            elem_getX(e, cXi, q.currentX);
         }
         const double u[2] = {0, 0};
         T WInt[5], dWInt[10];
@@ -291,9 +287,9 @@ void residual_all(Case& c, double* R) {
     for (int i = 0; i < c.nElem; i++) {
         const Elem& e = c.E[i];
         const int B = e.nBasis * nEqs;
-        const double* Wnb[3];
-        double* KR[3] = {nullptr, nullptr, nullptr};
-        for (int k = 0; k < 3; k++) Wnb[k] = e.TT[k] > -1 ? c.E[e.TT[k]].W.data() : nullptr;
+        const double* Wnb[MAXF];
+        double* KR[MAXF] = {nullptr, nullptr, nullptr, nullptr};
+        for (int k = 0; k < e.nF; k++) Wnb[k] = e.TT[k] > -1 ? c.E[e.TT[k]].W.data() : nullptr;
         double* K = R + (size_t)i * B;
         for (int j = 0; j < B; j++) K[j] = 0;
         residuum<double>(c, e, e.W.data(), Wnb, K, KR);
@@ -337,9 +333,9 @@ void assemble_residuum(Case& c, double dt, double dto) {
         Elem& e = c.E[i];
         const int B = e.nBasis * nEqs;
         std::vector<double> Rw(B, 0.0);
-        const double* Wnb[3];
-        double* KR[3] = {nullptr, nullptr, nullptr};
-        for (int k = 0; k < 3; k++) Wnb[k] = e.TT[k] > -1 ? c.E[e.TT[k]].W.data() : nullptr;
+        const double* Wnb[MAXF];
+        double* KR[MAXF] = {nullptr, nullptr, nullptr, nullptr};
+        for (int k = 0; k < e.nF; k++) Wnb[k] = e.TT[k] > -1 ? c.E[e.TT[k]].W.data() : nullptr;
         residuum<double>(c, e, e.W.data(), Wnb, Rw.data(), KR);
         assemble_rhs(c, e, Rw.data(), a);
     }
@@ -356,11 +352,11 @@ void assemble(Case& c, double dt, double dto) {
         Elem& e = c.E[i];
         const int Nb = e.nBasis, B = Nb * nEqs;
         std::vector<double> Rw(B, 0.0);
-        std::vector<double> RwNeigh[3], RwNeighH[3];
-        double* KR[3] = {nullptr, nullptr, nullptr};
-        double* KRh[3] = {nullptr, nullptr, nullptr};
-        const double* Wnb[3];
-        for (int k = 0; k < 3; k++) {
+        std::vector<double> RwNeigh[MAXF], RwNeighH[MAXF];
+        double* KR[MAXF] = {nullptr, nullptr, nullptr, nullptr};
+        double* KRh[MAXF] = {nullptr, nullptr, nullptr, nullptr};
+        const double* Wnb[MAXF];
+        for (int k = 0; k < e.nF; k++) {
             Wnb[k] = e.TT[k] > -1 ? c.E[e.TT[k]].W.data() : nullptr;
             if (e.TT[k] > -1) {
                 RwNeigh[k].assign(B, 0.0);
@@ -379,7 +375,7 @@ void assemble(Case& c, double dt, double dto) {
                 for (int j = 0; j < B; j++) Ai[j] = -Rw[j];
                 for (int j = 0; j < B; j++) Wh[j] = e.W[j] + (j == ii ? h : 0.0);
                 residuum<double>(c, e, Wh.data(), Wnb, Ai, KRh);
-                for (int k = 0; k < 3; k++) {
+                for (int k = 0; k < e.nF; k++) {
                     if (e.TT[k] > -1) {
                         Elem& nb = c.E[e.TT[k]];
                         double* Aaux = &nb.ANeigh[e.faceIndexReverse[k]][(size_t)ii * B];
@@ -390,11 +386,11 @@ void assemble(Case& c, double dt, double dto) {
             for (int j = 0; j < B * B; j++) e.ADiag[j] /= -h;
         } else {
             // exact derivative: forward mode, one direction per perturbed DOF
-            std::vector<Dual> Wd(B), K(B), KRd[3];
-            std::vector<Dual> Wnbd[3];
-            const Dual* Wnbp[3];
-            Dual* KRp[3];
-            for (int k = 0; k < 3; k++) {
+            std::vector<Dual> Wd(B), K(B), KRd[MAXF];
+            std::vector<Dual> Wnbd[MAXF];
+            const Dual* Wnbp[MAXF];
+            Dual* KRp[MAXF];
+            for (int k = 0; k < e.nF; k++) {
                 Wnbp[k] = nullptr;
                 KRp[k] = nullptr;
                 if (e.TT[k] > -1) {
@@ -411,7 +407,7 @@ void assemble(Case& c, double dt, double dto) {
                 residuum<Dual>(c, e, Wd.data(), Wnbp, K.data(), KRp);
                 double* Ai = &e.ADiag[(size_t)ii * B];
                 for (int j = 0; j < B; j++) Ai[j] = -K[j].d;
-                for (int k = 0; k < 3; k++) {
+                for (int k = 0; k < e.nF; k++) {
                     if (e.TT[k] > -1) {
                         Elem& nb = c.E[e.TT[k]];
                         double* Aaux = &nb.ANeigh[e.faceIndexReverse[k]][(size_t)ii * B];
@@ -434,11 +430,11 @@ void jvp(Case& c, double a1, const double* v, double* out) {
         const Elem& e = c.E[i];
         const int Nb = e.nBasis, B = Nb * nEqs;
         std::vector<Dual> Wd(B), K(B);
-        std::vector<Dual> Wnbd[3];
-        const Dual* Wnbp[3];
-        Dual* KRp[3] = {nullptr, nullptr, nullptr};
+        std::vector<Dual> Wnbd[MAXF];
+        const Dual* Wnbp[MAXF];
+        Dual* KRp[MAXF] = {nullptr, nullptr, nullptr, nullptr};
         for (int j = 0; j < B; j++) { Wd[j] = Dual(e.W[j], v[(size_t)i * B + j]); K[j] = Dual(0.0, 0.0); }
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < e.nF; k++) {
             Wnbp[k] = nullptr;
             int t = e.TT[k];
             if (t > -1) {
@@ -466,11 +462,11 @@ void vjp(Case& c, double a1, const double* v, double* out) {
     for (int i = 0; i < c.nElem; i++) {
         const Elem& e = c.E[i];
         const int Nb = e.nBasis, B = Nb * nEqs;
-        std::vector<Dual> Wd(B), K(B), KRd[3];
-        std::vector<Dual> Wnbd[3];
-        const Dual* Wnbp[3];
-        Dual* KRp[3];
-        for (int k = 0; k < 3; k++) {
+        std::vector<Dual> Wd(B), K(B), KRd[MAXF];
+        std::vector<Dual> Wnbd[MAXF];
+        const Dual* Wnbp[MAXF];
+        Dual* KRp[MAXF];
+        for (int k = 0; k < e.nF; k++) {
             Wnbp[k] = nullptr;
             KRp[k] = nullptr;
             int t = e.TT[k];
@@ -487,7 +483,7 @@ void vjp(Case& c, double a1, const double* v, double* out) {
             residuum<Dual>(c, e, Wd.data(), Wnbp, K.data(), KRp);
             double s = 0;
             for (int j = 0; j < B; j++) s += -K[j].d * v[(size_t)i * B + j];
-            for (int k = 0; k < 3; k++) {
+            for (int k = 0; k < e.nF; k++) {
                 int t = e.TT[k];
                 if (t > -1)
                     for (int j = 0; j < B; j++) s += KRd[k][j].d * v[(size_t)t * B + j];
@@ -506,7 +502,7 @@ void vjp(Case& c, double a1, const double* v, double* out) {
 static double functional_elem(const Case& c, const Elem& e, int which, const double* W) {
     const int nEqs = c.Mc.nEqs, Nb = e.nBasis;
     double f = 0;
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < e.nF; k++) {
         double fk = 0;
         for (int p = 0; p < e.nqf[k]; p++) {
             const double* baseLeft = &e.baseL[k][p * Nb];

@@ -200,6 +200,24 @@ Rule load_line_rule(const std::string& dir, int order) {
     return read_rule(dir + "/gauss/" + std::to_string(order) + ".txt", 1);
 }
 
+// C/quadrature/QuadratureSquare.java:7-25: the tensor product of the Gauss rule of the order on
+// [0, 1]^2, point (i, j) -> ((1 + x_i) / 2, (1 + x_j) / 2), weight w_i w_j / 4, i outer
+Rule load_square_rule(const std::string& dir, int order) {
+    Rule l = load_line_rule(dir, order), r;
+    r.n = l.n * l.n;
+    r.x.resize(2 * r.n);
+    r.w.resize(r.n);
+    int s = 0;
+    for (int i = 0; i < l.n; i++)
+        for (int j = 0; j < l.n; j++) {
+            r.x[2 * s] = (1 + l.x[i]) / 2;
+            r.x[2 * s + 1] = (1 + l.x[j]) / 2;
+            r.w[s] = l.w[i] * l.w[j] / 4;
+            ++s;
+        }
+    return r;
+}
+
 Rule load_triangle_rule(const std::string& dir, int order) {
     int deg = 2 * order - 1;   // C/quadrature/QuadratureCentral.java:46-60
     Rule r = read_rule(dir + "/gaussTri/" + std::to_string(deg) + ".txt", 2);
@@ -210,13 +228,15 @@ Rule load_triangle_rule(const std::string& dir, int order) {
 }
 
 // ------------------------------------------------------------------ basis
-// C/basis/basis2Dtriangle.java:22-118
+// C/basis/basis2Dtriangle.java:22-118; with square = true C/basis/basis2Dsquare.java:17-104 (the
+// same monomial exponents, all order^2 of them, and the Vandermonde matrix at the tensor nodes
+// X = (b_i, b_j), b = linspace(0, 1, order), i outer)
 struct BasisTri {
     int order, nBasis;
     std::vector<int> xExp, yExp;
     std::vector<std::vector<double>> coeffs;
-    explicit BasisTri(int ord) : order(ord) {
-        nBasis = order * (1 + order) / 2;
+    explicit BasisTri(int ord, bool square = false) : order(ord) {
+        nBasis = square ? order * order : order * (1 + order) / 2;
         xExp.assign(order * order, 0);
         yExp.assign(order * order, 0);
         int s = 0;
@@ -226,6 +246,16 @@ struct BasisTri {
             for (int j = 0; j < i; j++) { xExp[s] = order - 1 - j; yExp[s] = order - i + j; ++s; }
         if (order == 1) {
             coeffs = {{1.0}};
+        } else if (square) {
+            auto b = linspace(0, 1, order);
+            std::vector<std::vector<double>> V(nBasis, std::vector<double>(nBasis));
+            int r = 0;
+            for (int i = 0; i < order; i++)
+                for (int j = 0; j < order; j++) {
+                    for (int k = 0; k < nBasis; k++) V[r][k] = std::pow(b[i], xExp[k]) * std::pow(b[j], yExp[k]);
+                    ++r;
+                }
+            coeffs = mat_invert(V);
         } else {
             auto b = barycentr(order, nBasis);
             std::vector<std::vector<double>> V(nBasis, std::vector<double>(nBasis));
@@ -282,6 +312,68 @@ struct BasisTri {
 };
 
 // ------------------------------------------------------------------ transformation
+static const double CX_TRI[3][2] = {{0, 0}, {1, 0}, {0, 1}};            // transformation2Dtriangle.coordsXi
+static const double CX_SQ[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};     // transformation2Dsquare.coordsXi
+const double* elem_ref_vertex(const Elem& e, int j) { return e.nF == 4 ? CX_SQ[j] : CX_TRI[j]; }
+
+// C/transformation/transformation2Dsquare.java:12-48: rows (xi eta, xi, eta, 1) at the reference
+// vertices, A = invV x, B = invV y
+static void transform_init_square(Elem& e) {
+    std::vector<std::vector<double>> V(4, std::vector<double>(4));
+    for (int i = 0; i < 4; i++) {
+        V[i][0] = CX_SQ[i][0] * CX_SQ[i][1];
+        V[i][1] = CX_SQ[i][0];
+        V[i][2] = CX_SQ[i][1];
+        V[i][3] = 1;
+    }
+    auto invV = mat_invert(V);
+    for (int i = 0; i < 4; i++) {
+        double a = 0, b = 0;
+        for (int j = 0; j < 4; j++) { a += invV[i][j] * e.vert[j][0]; b += invV[i][j] * e.vert[j][1]; }
+        e.A[i] = a;
+        e.B[i] = b;
+    }
+}
+static void getX_square(const Elem& e, const double* Xi, double* X) {   // transformation2Dsquare.java:50-53
+    X[0] = e.A[0] * Xi[0] * Xi[1] + e.A[1] * Xi[0] + e.A[2] * Xi[1] + e.A[3];
+    X[1] = e.B[0] * Xi[0] * Xi[1] + e.B[1] * Xi[0] + e.B[2] * Xi[1] + e.B[3];
+}
+// getDX / jacobiMatrix (transformation2Dsquare.java:94-125): J[i][j] = dX_i / dXi_j
+static void jacobi_square(const Elem& e, const double* Xi, double J[2][2]) {
+    J[0][0] = e.A[0] * Xi[1] + e.A[1];
+    J[0][1] = e.A[0] * Xi[0] + e.A[2];
+    J[1][0] = e.B[0] * Xi[1] + e.B[1];
+    J[1][1] = e.B[0] * Xi[0] + e.B[2];
+}
+// transformation2Dsquare.getXi (:64-91): Newton from (0.5, 0.5) with a forward-difference
+// Jacobian (h = 1e-8), at most 50 steps, stop when |Xi_new - Xi| < 1e-13
+static void getXi_square(const Elem& e, const double* X, double* out) {
+    double Xi[2] = {0.5, 0.5}, Xin[2] = {0.5, 0.5};
+    const double h = 1e-8;
+    for (int i = 0; i < 50; i++) {
+        std::vector<std::vector<double>> J(2, std::vector<double>(2));
+        double X0[2];
+        getX_square(e, Xi, X0);
+        for (int j = 0; j < 2; j++) {
+            double Xp[2] = {Xi[0], Xi[1]}, Xh[2];
+            Xp[j] = Xi[j] + h;   // Mat.plusVec(Xi, V) with V = h e_j
+            getX_square(e, Xp, Xh);
+            for (int d = 0; d < 2; d++) J[j][d] = (Xh[d] - X0[d]) * (1 / h);
+        }
+        std::swap(J[0][1], J[1][0]);   // Mat.transposeInPlace
+        auto iJ = mat_invert(J);
+        double r[2] = {X0[0] - X[0], X0[1] - X[1]};
+        for (int d = 0; d < 2; d++) Xin[d] = Xi[d] - (iJ[d][0] * r[0] + iJ[d][1] * r[1]);
+        double dd = std::sqrt((Xin[0] - Xi[0]) * (Xin[0] - Xi[0]) + (Xin[1] - Xi[1]) * (Xin[1] - Xi[1]));
+        if (dd < 1e-13) break;
+        if (std::isnan(Xin[0]) || std::isnan(Xin[1])) { Xin[0] = Xin[1] = 0.5; break; }
+        Xi[0] = Xin[0];
+        Xi[1] = Xin[1];
+    }
+    out[0] = Xin[0];
+    out[1] = Xin[1];
+}
+
 // C/transformation/transformation2Dtriangle.java:40-119
 static void transform_init(Elem& e) {
     static const double CX[3][2] = {{0, 0}, {1, 0}, {0, 1}};
@@ -310,20 +402,37 @@ static void transform_init(Elem& e) {
     times(invVi, aux, e.iB);
 }
 static void getX(const Elem& e, const double* Xi, double* X) {
+    if (e.nF == 4) { getX_square(e, Xi, X); return; }
     X[0] = e.A[0] * Xi[0] + e.A[1] * Xi[1] + e.A[2];
     X[1] = e.B[0] * Xi[0] + e.B[1] * Xi[1] + e.B[2];
 }
+void elem_getX(const Elem& e, const double* Xi, double* X) { getX(e, Xi, X); }
 static void getXi(const Elem& e, const double* X, double* Xi) {
+    if (e.nF == 4) { getXi_square(e, X, Xi); return; }
     Xi[0] = e.iA[0] * X[0] + e.iA[1] * X[1] + e.iA[2];
     Xi[1] = e.iB[0] * X[0] + e.iB[1] * X[1] + e.iB[2];
 }
-static double jacobianVol(const Elem& e) {
+// Jacobian of the map at Xi: |det| / 2 on the triangle (transformation2Dtriangle), |det| on the
+// square (transformation2Dsquare.jacobian, :128-134)
+static double jacobianVol(const Elem& e, const double* Xi) {
+    if (e.nF == 4) {
+        double J[2][2];
+        jacobi_square(e, Xi, J);
+        return std::fabs(J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+    }
     return std::fabs(e.A[0] * e.B[1] - e.A[1] * e.B[0]) / 2;
 }
-// Transformation.transformBasis: iT = invert(jacobiMatrix), d_x = sum_j iT[j][i] d_xi_j
+static void jacobi_matrix(const Elem& e, const double* Xi, double J[2][2]) {
+    if (e.nF == 4) { jacobi_square(e, Xi, J); return; }
+    J[0][0] = e.A[0]; J[0][1] = e.A[1];
+    J[1][0] = e.B[0]; J[1][1] = e.B[1];
+}
+// Transformation.transformBasis: iT = invert(jacobiMatrix(Xi)), d_x = sum_j iT[j][i] d_xi_j
 // (C/transformation/Transformation.java:146-163)
-static void transform_basis(const Elem& e, const double* dXi, double* dX, int nb) {
-    std::vector<std::vector<double>> J = {{e.A[0], e.A[1]}, {e.B[0], e.B[1]}};
+static void transform_basis(const Elem& e, const double* Xi, const double* dXi, double* dX, int nb) {
+    double Jm[2][2];
+    jacobi_matrix(e, Xi, Jm);
+    std::vector<std::vector<double>> J = {{Jm[0][0], Jm[0][1]}, {Jm[1][0], Jm[1][1]}};
     auto iT = mat_invert(J);
     for (int m = 0; m < nb; m++)
         for (int i = 0; i < 2; i++) {
@@ -334,28 +443,28 @@ static void transform_basis(const Elem& e, const double* dXi, double* dX, int nb
 }
 
 // ------------------------------------------------------------------ neighbour matrix
-std::vector<int> neighbor_matrix(int nElem, const int* TP, int nBnd, const int* bnd) {
+std::vector<int> neighbor_matrix(int nElem, const int* TP, int nBnd, const int* bnd, int nF) {
     // key = sorted vertex pair (C/FlowProMain.java:786-806 uses the string "a$b$")
     std::map<std::pair<int, int>, std::pair<int, int>> faces;
     auto key = [&](int i, int k) {
-        int a = TP[3 * i + k], b = TP[3 * i + (k + 1) % 3];
+        int a = TP[nF * i + k], b = TP[nF * i + (k + 1) % nF];
         return a < b ? std::make_pair(a, b) : std::make_pair(b, a);
     };
     for (int i = 0; i < nElem; i++)
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < nF; k++) {
             auto kk = key(i, k);
             auto it = faces.find(kk);
             if (it == faces.end()) faces[kk] = {i, -1000};
             else it->second.second = i;
         }
-    std::vector<int> TT(3 * nElem, -1000);
+    std::vector<int> TT((size_t)nF * nElem, -1000);
     for (int i = 0; i < nElem; i++)
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < nF; k++) {
             auto& f = faces[key(i, k)];
             if (f.first == i) {
-                if (f.second != -1000) TT[3 * i + k] = f.second;
+                if (f.second != -1000) TT[nF * i + k] = f.second;
             } else {
-                TT[3 * i + k] = f.first;
+                TT[nF * i + k] = f.first;
             }
         }
     std::map<std::pair<int, int>, int> bmap;   // :834-848
@@ -364,10 +473,10 @@ std::vector<int> neighbor_matrix(int nElem, const int* TP, int nBnd, const int* 
         bmap[a < b ? std::make_pair(a, b) : std::make_pair(b, a)] = bnd[3 * r];
     }
     for (int i = 0; i < nElem; i++)
-        for (int k = 0; k < 3; k++)
-            if (TT[3 * i + k] == -1000) {
+        for (int k = 0; k < nF; k++)
+            if (TT[nF * i + k] == -1000) {
                 auto it = bmap.find(key(i, k));
-                TT[3 * i + k] = it != bmap.end() ? it->second : -1;
+                TT[nF * i + k] = it != bmap.end() ? it->second : -1;
             }
     return TT;
 }
@@ -437,24 +546,24 @@ std::vector<int> rcm_order(int n, const int* TT, int nF) {
 
 // ------------------------------------------------------------------ case build
 // C/Mesh.java:69-224 (constructor and init) for straight triangles, one global order.
-std::vector<double> wall_distance(int nPts, const double* PXY, int nElem, const int* TP, const int* TT) {
+std::vector<double> wall_distance(int nPts, const double* PXY, int nElem, const int* TP, const int* TT, int nF) {
     const double INF = std::numeric_limits<double>::infinity();
     std::vector<double> distance(nPts, INF);
-    // triangleElement.getFaceIndexes(k) = {k, (k + 1) % 3}
+    // triangleElement / squareElement.getFaceIndexes(k) = {k, (k + 1) % nF}
     for (int i = 0; i < nElem; i++)
-        for (int k = 0; k < 3; k++)
-            if (TT[3 * i + k] == -1) {
-                distance[TP[3 * i + k]] = 0;
-                distance[TP[3 * i + (k + 1) % 3]] = 0;
+        for (int k = 0; k < nF; k++)
+            if (TT[nF * i + k] == -1) {
+                distance[TP[nF * i + k]] = 0;
+                distance[TP[nF * i + (k + 1) % nF]] = 0;
             }
     bool changed = true;
     for (int iter = 0; changed && iter <= nPts; iter++) {
         changed = false;
         for (int i = 0; i < nElem; i++)
-            for (int j = 0; j < 3; j++) {
-                const int pj = TP[3 * i + j];
-                for (int k = 0; k < 3; k++) {
-                    const int pk = TP[3 * i + k];
+            for (int j = 0; j < nF; j++) {
+                const int pj = TP[nF * i + j];
+                for (int k = 0; k < nF; k++) {
+                    const int pk = TP[nF * i + k];
                     if (j != k && distance[pk] < distance[pj]) {
                         double L = 0;
                         for (int d = 0; d < 2; d++) L += (PXY[2 * pj + d] - PXY[2 * pk + d]) * (PXY[2 * pj + d] - PXY[2 * pk + d]);
@@ -483,14 +592,18 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
     c->state_dt = P.fixed_dt ? P.dt : 0.0;   // State constructor (C/State.java:75-79)
     c->nPts = nPts;
     c->nElem = nElem;
+    const bool square = P.elem_type == 4;
+    const int nF = square ? 4 : 3;
+    c->nF = nF;
     c->PXY.assign(PXY, PXY + 2 * nPts);
-    c->TP.assign(TP, TP + 3 * nElem);
-    c->TT = neighbor_matrix(nElem, TP, nBnd, bnd);
+    c->TP.assign(TP, TP + (size_t)nF * nElem);
+    c->TT = neighbor_matrix(nElem, TP, nBnd, bnd, nF);
     const int nEqs = c->Mc.nEqs;
     const int order = P.order;
-    c->vol = load_triangle_rule(rulesDir, order);
+    // volume and face rules of the element order (Parameters.volumeQuadratureOrder "auto")
+    c->vol = square ? load_square_rule(rulesDir, order) : load_triangle_rule(rulesDir, order);
     c->face = load_line_rule(rulesDir, order);
-    BasisTri basis(order);
+    BasisTri basis(order, square);
     const int Nb = basis.nBasis;
     const int B = Nb * nEqs;
     // reference-element tables (same for every straight triangle)
@@ -507,19 +620,21 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
         e.index = i;
         e.order = order;
         e.nBasis = Nb;
-        for (int k = 0; k < 3; k++) {
-            e.TT[k] = c->TT[3 * i + k];
-            e.TP[k] = TP[3 * i + k];
-            e.vert[k][0] = PXY[2 * TP[3 * i + k]];
-            e.vert[k][1] = PXY[2 * TP[3 * i + k] + 1];
-            e.wallDist[k] = wallDist ? wallDist[TP[3 * i + k]] : 0.0;
+        e.nF = nF;
+        for (int k = 0; k < nF; k++) {
+            e.TT[k] = c->TT[nF * i + k];
+            e.TP[k] = TP[nF * i + k];
+            e.vert[k][0] = PXY[2 * TP[nF * i + k]];
+            e.vert[k][1] = PXY[2 * TP[nF * i + k] + 1];
+            e.wallDist[k] = wallDist ? wallDist[TP[nF * i + k]] : 0.0;
         }
-        transform_init(e);
+        if (square) transform_init_square(e);
+        else transform_init(e);
         e.dampInner.assign(nEqs, 0.0);
         e.innerIndicatorOld.assign(nEqs, 0.0);
     }
     // DOF numbering (C/Mesh.java:181-187, C/element/Element.java:241-248)
-    if (P.ordering == 1) c->order = rcm_order(nElem, c->TT.data());
+    if (P.ordering == 1) c->order = rcm_order(nElem, c->TT.data(), nF);
     else {
         c->order.resize(nElem);
         for (int i = 0; i < nElem; i++) c->order[i] = i;
@@ -530,29 +645,39 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
     // integration tables, geometry, mass (C/Integration.java, C/element/Element.java:516-567)
     for (int i = 0; i < nElem; i++) {
         Elem& e = c->E[i];
-        const double jac = jacobianVol(e);
         e.nq = qv.n;
-        e.JacV.assign(qv.n, jac);
+        e.JacV.resize(qv.n);
+        for (int p = 0; p < qv.n; p++) e.JacV[p] = jacobianVol(e, &qv.x[2 * p]);
         e.wV = qv.w;
         e.baseV = baseRef;
         e.dbaseV.resize(qv.n * Nb * 2);
-        for (int p = 0; p < qv.n; p++) transform_basis(e, &dXiRef[p * Nb * 2], &e.dbaseV[p * Nb * 2], Nb);
-        e.interpV.resize(qv.n * 3);
+        for (int p = 0; p < qv.n; p++) transform_basis(e, &qv.x[2 * p], &dXiRef[p * Nb * 2], &e.dbaseV[p * Nb * 2], Nb);
+        // getInterpolant: barycentric on the triangle, bilinear on the square
+        // (transformation2Dsquare.java:137-150)
+        e.interpV.resize(qv.n * nF);
         for (int p = 0; p < qv.n; p++) {
             double xi = qv.x[2 * p], eta = qv.x[2 * p + 1];
-            e.interpV[3 * p] = 1 - xi - eta;
-            e.interpV[3 * p + 1] = xi;
-            e.interpV[3 * p + 2] = eta;
+            if (square) {
+                e.interpV[4 * p] = (1 - xi) * (1 - eta);
+                e.interpV[4 * p + 1] = xi * (1 - eta);
+                e.interpV[4 * p + 2] = xi * eta;
+                e.interpV[4 * p + 3] = (1 - xi) * eta;
+            } else {
+                e.interpV[3 * p] = 1 - xi - eta;
+                e.interpV[3 * p + 1] = xi;
+                e.interpV[3 * p + 2] = eta;
+            }
         }
     }
-    static const double CX[3][2] = {{0, 0}, {1, 0}, {0, 1}};
     const Rule& qf = c->face;
     for (int i = 0; i < nElem; i++) {
         Elem& e = c->E[i];
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < nF; k++) {
             // faceTransformation2Dline (C/transformation/faceTransformation2Dline.java:23-60)
-            int f0 = k, f1 = (k + 1) % 3;
-            double ref0[2] = {CX[f0][0], CX[f0][1]}, ref1[2] = {CX[f1][0], CX[f1][1]};
+            int f0 = k, f1 = (k + 1) % nF;
+            const double* CX0 = elem_ref_vertex(e, f0);
+            const double* CX1 = elem_ref_vertex(e, f1);
+            double ref0[2] = {CX0[0], CX0[1]}, ref1[2] = {CX1[0], CX1[1]};
             int nf = qf.n;
             e.nqf[k] = nf;
             e.wF[k] = qf.w;
@@ -561,14 +686,18 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
             e.dbaseL[k].resize(nf * Nb * 2);
             e.interpF[k].resize(nf * 2);
             double s[2] = {ref1[0] - ref0[0], ref1[1] - ref0[1]};
-            // Js = J * s with J = [[A0,A1],[B0,B1]]
-            double Js[2] = {e.A[0] * s[0] + e.A[1] * s[1], e.B[0] * s[0] + e.B[1] * s[1]};
-            double Jsn = std::sqrt(Js[0] * Js[0] + Js[1] * Js[1]);
             for (int p = 0; p < nf; p++) {
                 double xi = qf.x[p];
                 double XiRef[2];
                 XiRef[0] = (1 + xi) / 2 * ref1[0] + (1 - xi) / 2 * ref0[0];
                 XiRef[1] = (1 + xi) / 2 * ref1[1] + (1 - xi) / 2 * ref0[1];
+                // Js = jacobiMatrix(XiRef) s (Transformation.jacobianMatrixDirection)
+                double Jm[2][2];
+                jacobi_matrix(e, XiRef, Jm);
+                double Js[2] = {0, 0};
+                for (int a = 0; a < 2; a++)
+                    for (int b = 0; b < 2; b++) Js[a] += Jm[a][b] * s[b];
+                double Jsn = std::sqrt(Js[0] * Js[0] + Js[1] * Js[1]);
                 e.JacF[k][p] = Jsn / 2;
                 double nrm[2] = {Js[1], -Js[0]};
                 double nn = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1]);
@@ -577,10 +706,10 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
                 e.interpF[k][2 * p] = (1 - xi) / 2;
                 e.interpF[k][2 * p + 1] = (1 + xi) / 2;
                 for (int m = 0; m < Nb; m++) e.baseL[k][p * Nb + m] = basis.fun(m, XiRef);
-                double dXi[64];
+                double dXi[128];
                 for (int d = 0; d < 2; d++)
                     for (int m = 0; m < Nb; m++) dXi[m * 2 + d] = basis.der(m, XiRef, d);
-                transform_basis(e, dXi, &e.dbaseL[k][p * Nb * 2], Nb);
+                transform_basis(e, XiRef, dXi, &e.dbaseL[k][p * Nb * 2], Nb);
             }
             int t = e.TT[k];
             if (t > -1) {
@@ -596,34 +725,37 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
                     getX(e, XiRef, X);
                     getXi(er, X, XiR);
                     for (int m = 0; m < Nb; m++) e.baseR[k][p * Nb + m] = basis.fun(m, XiR);
-                    double dXi[64];
+                    double dXi[128];
                     for (int d = 0; d < 2; d++)
                         for (int m = 0; m < Nb; m++) dXi[m * 2 + d] = basis.der(m, XiR, d);
-                    transform_basis(er, dXi, &e.dbaseR[k][p * Nb * 2], Nb);
+                    transform_basis(er, XiR, dXi, &e.dbaseR[k][p * Nb * 2], Nb);
                 }
             }
         }
         // computeGeometry
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < nF; k++) {
             double mid[2] = {0.5 * (0), 0};
-            int f0 = k, f1 = (k + 1) % 3;
+            int f0 = k, f1 = (k + 1) % nF;
+            const double* CX0 = elem_ref_vertex(e, f0);
+            const double* CX1 = elem_ref_vertex(e, f1);
             double XiRef[2];
             // faceTransform.getXs = getX(Xi = 0): (1+0)/2*ref1 + (1-0)/2*ref0
-            XiRef[0] = (1 + 0.0) / 2 * CX[f1][0] + (1 - 0.0) / 2 * CX[f0][0];
-            XiRef[1] = (1 + 0.0) / 2 * CX[f1][1] + (1 - 0.0) / 2 * CX[f0][1];
+            XiRef[0] = (1 + 0.0) / 2 * CX1[0] + (1 - 0.0) / 2 * CX0[0];
+            XiRef[1] = (1 + 0.0) / 2 * CX1[1] + (1 - 0.0) / 2 * CX0[1];
             getX(e, XiRef, mid);
             e.Xes[k][0] = mid[0];
             e.Xes[k][1] = mid[1];
             e.S[k] = 0;
             for (int p = 0; p < e.nqf[k]; p++) e.S[k] = e.S[k] + e.JacF[k][p] * e.wF[k][p];
         }
-        double third[2] = {1.0 / 3, 1.0 / 3};
-        getX(e, third, e.Xs);
+        // transform.getXs: the centroid of the reference triangle, the centre of the square
+        double centre[2] = {square ? 0.5 : 1.0 / 3, square ? 0.5 : 1.0 / 3};
+        getX(e, centre, e.Xs);
         e.area = 0;
         for (int p = 0; p < e.nq; p++) e.area = e.area + e.JacV[p] * e.wV[p];
         e.elemSize = 0;
-        for (int k = 0; k < 3; k++) e.elemSize += e.S[k];
-        e.elemSize = 3 * e.area / e.elemSize;
+        for (int k = 0; k < nF; k++) e.elemSize += e.S[k];
+        e.elemSize = nF * e.area / e.elemSize;
         // mass matrix and basis weights (C/element/DGpure.java:547-573)
         e.M.assign(Nb * Nb, 0.0);
         e.Is.assign(Nb, 0.0);
@@ -668,17 +800,17 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
                 for (int b = 0; b < Nb; b++) e.TrunOrd[a * Nb + b] = tr[a][b];
         }
         e.ADiag.assign(B * B, 0.0);
-        for (int k = 0; k < 3; k++)
+        for (int k = 0; k < nF; k++)
             if (e.TT[k] > -1) e.ANeigh[k].assign(B * B, 0.0);
         e.RHS.assign(B, 0.0);
         e.spatialR.assign(B, 0.0);
     }
     for (int i = 0; i < nElem; i++) {
         Elem& e = c->E[i];
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < nF; k++) {
             e.faceIndexReverse[k] = 0;
             if (e.TT[k] > -1)
-                for (int s = 0; s < 3; s++)
+                for (int s = 0; s < nF; s++)
                     if (c->E[e.TT[k]].TT[s] == i) { e.faceIndexReverse[k] = s; break; }
         }
     }

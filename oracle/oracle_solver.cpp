@@ -23,7 +23,7 @@ static void build_crs(Case& c) {
     for (const auto& e : c.E) {
         int B = e.nBasis * nEqs;
         int len = B;
-        for (int k = 0; k < 3; k++)
+        for (int k = 0; k < e.nF; k++)
             if (e.TT[k] > -1) len += c.E[e.TT[k]].nBasis * nEqs;
         for (int i = 0; i < B; i++) rowLen[e.gIndex0 + i] = len;
     }
@@ -33,7 +33,7 @@ static void build_crs(Case& c) {
     for (const auto& e : c.E) {
         int B = e.nBasis * nEqs;
         std::vector<int> blk = {e.gIndex0};
-        for (int k = 0; k < 3; k++)
+        for (int k = 0; k < e.nF; k++)
             if (e.TT[k] > -1) blk.push_back(c.E[e.TT[k]].gIndex0);
         std::sort(blk.begin(), blk.end());
         for (int i = 0; i < B; i++) {
@@ -57,7 +57,7 @@ static void update_data(Case& c) {
                 c.H[p - &c.Jcrs[0]] = v;
             };
             for (int j = 0; j < B; j++) put(e.gIndex0 + j, e.ADiag[(size_t)j * B + i]);
-            for (int k = 0; k < 3; k++)
+            for (int k = 0; k < e.nF; k++)
                 if (e.TT[k] > -1) {
                     const Elem& nb = c.E[e.TT[k]];
                     for (int j = 0; j < B; j++) put(nb.gIndex0 + j, e.ANeigh[k][(size_t)j * B + i]);
@@ -102,7 +102,7 @@ static void ilu_set_matrix(Case& c) {
     for (int i = 0; i < n; i++) {
         const Elem& e = c.E[c.ilu_elems[i]];
         std::vector<int> row = {i};
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < e.nF; k++) {
             int t = e.TT[k];
             if (t > -1 && std::find(row.begin(), row.end(), c.ilu_rank[t]) == row.end()) row.push_back(c.ilu_rank[t]);
         }
@@ -127,7 +127,7 @@ static bool ilu_factor(Case& c) {
         auto& D = c.ilu_blocks[i][c.ilu_diag[i]];
         for (int r = 0; r < B; r++)
             for (int cc = 0; cc < B; cc++) D[(size_t)r * B + cc] += e.ADiag[(size_t)cc * B + r];
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < e.nF; k++) {
             int t = e.TT[k];
             if (t > -1) {
                 int pos = (int)(std::lower_bound(c.ilu_cols[i].begin(), c.ilu_cols[i].end(), c.ilu_rank[t]) - c.ilu_cols[i].begin());
@@ -574,7 +574,7 @@ static bool positivity_limit(const Case& c, Elem& e) {
     };
     double theta = 1;
     for (int p = 0; p < e.nq; p++) theta = std::min(theta, theta_at(&e.baseV[p * Nb]));
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < e.nF; k++)
         for (int p = 0; p < e.nqf[k]; p++) theta = std::min(theta, theta_at(&e.baseL[k][p * Nb]));
     if (theta >= 1) return false;
     std::vector<std::vector<double>> Mg(Nb, std::vector<double>(Nb, 0.0));
@@ -681,6 +681,7 @@ void or_model_const(void* h, double* out) {   // kapa, pOut, Re, Pr, pRef, rhoRe
     out[0] = M.kapa; out[1] = M.pOut; out[2] = M.Re; out[3] = M.Pr;
     out[4] = M.pRef; out[5] = M.rhoRef; out[6] = M.velocityRef;
 }
+int or_nfaces(void* h) { return ((Case*)h)->nF; }
 void or_tt(void* h, int* out) { Case* c = (Case*)h; std::memcpy(out, c->TT.data(), sizeof(int) * c->TT.size()); }
 void or_order(void* h, int* out) { Case* c = (Case*)h; std::memcpy(out, c->order.data(), sizeof(int) * c->nElem); }
 void or_gindex(void* h, int* out) { Case* c = (Case*)h; for (int i = 0; i < c->nElem; i++) out[i] = c->E[i].gIndex0; }
@@ -726,8 +727,8 @@ void or_get_blocks(void* h, double* diag, double* neigh) {
         const Elem& e = c->E[i];
         size_t BB = e.ADiag.size();
         std::memcpy(diag + i * BB, e.ADiag.data(), sizeof(double) * BB);
-        for (int k = 0; k < 3; k++) {
-            double* dst = neigh + (i * 3 + k) * BB;
+        for (int k = 0; k < c->nF; k++) {
+            double* dst = neigh + ((size_t)i * c->nF + k) * BB;
             if (e.TT[k] > -1) std::memcpy(dst, e.ANeigh[k].data(), sizeof(double) * BB);
             else std::memset(dst, 0, sizeof(double) * BB);
         }
@@ -781,7 +782,7 @@ void or_elem_geometry(void* h, int i, double* area, double* elemSize, double* Xs
     *area = e.area;
     *elemSize = e.elemSize;
     Xs[0] = e.Xs[0]; Xs[1] = e.Xs[1];
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < e.nF; k++)
         for (int p = 0; p < e.nqf[k]; p++) {
             normals[(k * e.nqf[k] + p) * 2] = e.n[k][p][0];
             normals[(k * e.nqf[k] + p) * 2 + 1] = e.n[k][p][1];
@@ -796,7 +797,8 @@ double or_l2_error_scalar(void* h) {
     for (const auto& e : c->E) {
         for (int p = 0; p < e.nq; p++) {
             double Xi[2] = {c->vol.x[2 * p], c->vol.x[2 * p + 1]};
-            double X[2] = {e.A[0] * Xi[0] + e.A[1] * Xi[1] + e.A[2], e.B[0] * Xi[0] + e.B[1] * Xi[1] + e.B[2]};
+            double X[2];
+            elem_getX(e, Xi, X);
             double u = 0;
             for (int j = 0; j < e.nBasis; j++) u += e.W[j] * e.baseV[p * e.nBasis + j];
             double diff = u - std::sin(2 * X[0] + X[1]);
@@ -813,8 +815,8 @@ void or_flux_pointwise(void* h, const double* WL, const double* WR, const double
 void or_model_pointwise(void* h, const double* W, const double* dW, double D, double* s, double* fx, double* fy) {
     model_pointwise(*(Case*)h, W, dW, D, s, fx, fy);
 }
-void or_wall_distance(int nPts, const double* PXY, int nElem, const int* TP, const int* TT, double* out) {
-    std::vector<double> d = wall_distance(nPts, PXY, nElem, TP, TT);
+void or_wall_distance(int nPts, const double* PXY, int nElem, const int* TP, const int* TT, int nF, double* out) {
+    std::vector<double> d = wall_distance(nPts, PXY, nElem, TP, TT, nF);
     std::memcpy(out, d.data(), sizeof(double) * nPts);
 }
 }

@@ -1,0 +1,171 @@
+"""Parity of the quadrilateral kernels (squareElement, bilinear maps) with the oracle, through
+the C ABI.
+
+Meshes: the channel with the bump split into quadrilaterals (every cell over the bump is a
+general bilinear quadrilateral) with seeded interior jitter, and the cylinder O-grid of
+quadrilaterals.  Tolerances (equation-norm relative unless said otherwise):
+  residual, BDF right-hand side             1e-12
+  residual norms, rhs norm                  1e-11 (relative)
+  exact Jacobian blocks (diagonal, neighbour) 1e-10 (max |diff| / max |entry|)
+  J v, J^T v                                1e-10
+  GMRES correction (tol 1e-13) vs oracle and vs dense   1e-8 (max |diff| / max |x|)
+  Newton residual history, state            1e-8
+  min elemSize / lambda                     1e-13
+The neighbour side of a face point is the neighbour's basis at the point found by the
+reference's Newton inverse of the bilinear map (transformation2Dsquare.getXi, restated in the
+oracle); the device takes the neighbour's own face point in reversed order, the same point to
+rounding.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from flowpro_v2_amd import cases, native, solver
+import oracle as O
+from helpers import block_rel_err, eq_rel_err
+
+pytestmark = pytest.mark.gpu
+
+QUAD_CONFIGS = [
+    ("euler_q1", lambda: cases.channel_bump(12, 4, quads=True, jitter=0.2), lambda: cases.euler_props(order=2)),
+    ("euler_q2", lambda: cases.channel_bump(10, 4, quads=True, jitter=0.2), lambda: cases.euler_props(order=3)),
+    ("euler_q3_vanleer", lambda: cases.channel_bump(8, 3, quads=True, jitter=0.2),
+     lambda: cases.euler_props(order=4, numericalFlux="vanLeer")),
+    ("ns_q1", lambda: cases.cylinder_mesh(16, 5, quads=True), lambda: cases.ns_props(order=2, reynolds=100.0, penalty=1.0)),
+    ("ns_q3", lambda: cases.cylinder_mesh(16, 4, quads=True), lambda: cases.ns_props(order=4, reynolds=300.0)),
+    ("ae_q2", lambda: cases.channel_bump(10, 4, quads=True, jitter=0.2),
+     lambda: cases.euler_props(model="AdiabaticEulers", order=3)),
+    ("euler_q0", lambda: cases.channel_bump(12, 4, quads=True, jitter=0.2), lambda: cases.euler_props(order=1)),
+]
+
+
+def _setup(case, props, seed=3, rel=0.01):
+    props = dict(props, jacobian="exact")
+    o = O.Oracle(case, props)
+    dev = solver.DeviceCase(case, props)
+    assert (dev.nb, dev.neq) == (o.nb, o.neq)
+    W = cases.perturbed_state(dev.initial_state()[0][:: dev.nb], case.n_elem, dev.nb, rel, seed)
+    o.set_w(W)
+    o.limiter()
+    dev.set_state(W)
+    return o, dev, W
+
+
+def test_quad_integer_setup_matches_oracle():
+    case = cases.cylinder_mesh(16, 5, quads=True)
+    TT = native.neighbor_matrix_quad(case.TP, case.bnd)
+    o = O.Oracle(case, cases.euler_props(order=2))
+    assert np.array_equal(TT, o.TT())
+    assert np.array_equal(native.rcm_order_quad(TT), o.order())
+
+
+@pytest.mark.parametrize("name,mk_case,mk_props", QUAD_CONFIGS, ids=[c[0] for c in QUAD_CONFIGS])
+def test_quad_residual_and_jacobian(name, mk_case, mk_props):
+    case, props = mk_case(), mk_props()
+    o, dev, W = _setup(case, props)
+    dt, dto = 0.05, -1.0
+    o.assemble(dt, dto)
+    norms, rhs = dev.assemble(dt, dto, 1, True)
+    R, RHS = dev.residual()
+    Rref = o.residual()
+    RHSref, _ = o.rhs()
+    assert eq_rel_err(R, Rref, dev.nb, dev.neq) < 1e-12
+    assert eq_rel_err(RHS, RHSref, dev.nb, dev.neq) < 1e-12
+    np.testing.assert_allclose(norms, o.residuum_norms(), rtol=1e-11)
+    assert math.isclose(rhs, o.rhs_norm(), rel_tol=1e-11)
+    d, n = dev.jacobian()
+    dref, nref = o.blocks()
+    assert block_rel_err(d, dref) < 1e-10
+    assert block_rel_err(n, nref) < 1e-10
+    dev.close()
+
+
+@pytest.mark.parametrize("name,mk_case,mk_props", QUAD_CONFIGS[:6], ids=[c[0] for c in QUAD_CONFIGS[:6]])
+def test_quad_jv_jtv(name, mk_case, mk_props):
+    case, props = mk_case(), mk_props()
+    o, dev, W = _setup(case, props)
+    rng = np.random.default_rng(7)
+    v = rng.standard_normal((case.n_elem, dev.B))
+    a1 = 20.0
+    assert block_rel_err(dev.jv(a1, v), o.jvp(a1, v)) < 1e-10
+    assert block_rel_err(dev.jtv(a1, v), o.vjp(a1, v)) < 1e-10
+    dev.close()
+
+
+@pytest.mark.parametrize("prec", ["blockilu0", "blockjacobi"])
+def test_quad_linear_solve(prec):
+    case = cases.channel_bump(12, 4, quads=True, jitter=0.2)
+    props = cases.euler_props(order=3, preconditioner=prec, iterativeSolverTol=1e-13)
+    o, dev, W = _setup(case, props)
+    o.assemble(0.05, -1.0)
+    dev.assemble(0.05, -1.0, 1, True)
+    ok, xref, _, _ = o.linear_solve()
+    rc, it, err = dev.linear_solve()
+    x = dev.solution()
+    assert ok and rc == 0
+    assert np.abs(x - xref).max() / np.abs(xref).max() < 1e-8
+    A = o.global_matrix()
+    r, _ = o.rhs()
+    xd = np.linalg.solve(A, o.to_dof(r))
+    assert np.abs(x - xd).max() / np.abs(xd).max() < 1e-8
+    dev.close()
+
+
+@pytest.mark.parametrize("name,mk_case,mk_props,steps", [
+    ("euler_q1", lambda: cases.channel_bump(12, 4, quads=True, jitter=0.2),
+     lambda: cases.euler_props(order=2, CFL=5.0, newtonIters=2, iterativeSolverTol=1e-10), 6),
+    ("ns_q2_bdf2", lambda: cases.cylinder_mesh(16, 5, quads=True),
+     lambda: cases.ns_props(order=3, reynolds=200.0, CFL=2.0, timeOrder=2, iterativeSolverTol=1e-10), 5),
+], ids=["euler_q1", "ns_q2_bdf2"])
+def test_quad_newton_history(name, mk_case, mk_props, steps):
+    case, props = mk_case(), dict(mk_props(), jacobian="exact")
+    o = O.Oracle(case, props)
+    W0 = cases.perturbed_state(o.get_w()[0][:: o.nb], case.n_elem, o.nb, 0.01, 5)
+    o.set_w(W0)
+    hist_ref = o.run(steps)
+    dev = solver.DeviceCase(case, props)
+    dev.set_state(W0)
+    s = solver.GpuImplicitSolver(dev, props)
+    hist = [r.residuum for r in s.solve(steps)]
+    assert len(hist) == len(hist_ref) == steps
+    np.testing.assert_allclose(hist, hist_ref, rtol=1e-8)
+    Wg, Wr = dev.get_state(), o.get_w()
+    assert np.abs(Wg - Wr).max() / np.abs(Wr).max() < 1e-8
+    dev.close()
+
+
+def test_quad_time_step_and_averages():
+    case = cases.channel_bump(12, 4, quads=True, jitter=0.2)
+    props = cases.euler_props(order=3)
+    o, dev, W = _setup(case, props)
+    o.set_cfl(1.0)
+    assert math.isclose(dev.size_over_lambda(), o.time_step(), rel_tol=1e-13)
+    # element means: sum_j W_j Is_j with Is_j = int phi_j / area of the element's own map
+    avg = dev.averages()
+    for e in range(0, case.n_elem, 5):
+        M = o.geometry(e)["M"]
+        Is = M @ np.ones(dev.nb) / o.geometry(e)["area"]
+        ref = W[e].reshape(dev.neq, dev.nb) @ Is
+        assert np.abs(avg[e] - ref).max() <= 1e-13 * np.abs(ref).max()
+    dev.close()
+
+
+def test_quad_freestream_residual_vanishes():
+    case = cases.channel_bump(8, 4, quads=True)
+    case.PXY[:, 1] = np.repeat(np.linspace(0.0, 1.0, 5)[None, :], 9, 0).ravel()
+    rng = np.random.default_rng(7)
+    inner = (case.PXY[:, 0] > 1e-9) & (case.PXY[:, 0] < 3 - 1e-9) & (case.PXY[:, 1] > 1e-9) & (case.PXY[:, 1] < 1 - 1e-9)
+    case.PXY[inner] += 0.08 * rng.uniform(-1, 1, size=(inner.sum(), 2))
+    dev = solver.DeviceCase(case, cases.euler_props(order=3))
+    dev.set_state(dev.initial_state())
+    dev.assemble(0.05, -1.0, 1, False)
+    R, _ = dev.residual()
+    assert np.abs(R).max() < 1e-12
+    dev.close()
+
+
+def test_quad_unsupported_features():
+    case = cases.channel_bump(6, 2, quads=True)
+    with pytest.raises(solver.UnsupportedCase):
+        solver.DeviceCase(case, cases.euler_props(order=2, dampTol=1.0))
