@@ -234,7 +234,7 @@ def main():
         "dtype": "f64 residual/Jacobian/GMRES, f32 preconditioner" if main_bits == 32 else "f64",
         "data": "synthetic (freestream state, seeded mesh)",
         "config": ({"workload": f"Euler3D DG P1 (order 2) Kuhn cube {args.cube}^3 x 6 = {case.n_elem} tetrahedra, "
-                                f"{dev.dofs} DOFs, M=0.8 density pulse, BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) "
+                                f"{dev.dofs} DOFs, M=0.8 density and pressure pulse, BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) "
                                 "GMRES(30) tol 1e-2",
                     "elements": case.n_elem, "dofs": dev.dofs, "order": 2, "precond_bits": main_bits} if tet else
                    {"workload": f"NS2D DG p=3 Re=5000 M=0.5 cylinder, {case.n_elem} triangles, "
@@ -250,7 +250,7 @@ def main():
     }
     dev.close()
     if tet:
-        result["data"] = "synthetic (freestream with a seeded density pulse, seeded jittered mesh)"
+        result["data"] = "synthetic (freestream with a Gaussian density and pressure pulse, seeded jittered mesh)"
         result["cpu_baseline"] = None   # the CPU restatement of the tetrahedra covers residual and Jacobian only
     elif not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
