@@ -18,6 +18,12 @@ Kuhn-split unit cube (64^3 cubes, 1,572,864 tetrahedra, 5 % seeded vertex jitter
 Gaussian density and pressure pulse (amplitudes 0.05, width 0.1), element order 2 (P1: the highest order
 the reference's basis3Dtetra builds), BDF1 Newton steps with blockilu0 GMRES.
 
+`--workload batched_ns` measures many independent small solves sharing the GPU: 64 seeded 2D
+Navier-Stokes cylinder instances (5k-50k triangles each, element order 2-4, Re in [1e3, 1e4],
+M in [0.2, 0.6]), each with its own context and stream, driven concurrently; `value` is the
+aggregate DOF-updates/s, and the lift coefficient with its exact gradient is evaluated per
+instance after the steps.
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -163,6 +169,73 @@ def cpu_baseline(threads, n_theta=192, n_r=64, nsteps=3):
                        f"NS p=3 Re=5000 M=0.5 case on a {case.n_elem}-triangle cylinder mesh, {el:.1f} s")
 
 
+def bench_batched(args):
+    """Independent instances solved concurrently (flowpro_v2_amd.batch), FP64 then mixed."""
+    from flowpro_v2_amd import batch
+    t0 = time.perf_counter()
+    inst = batch.seeded_ns_instances(n=args.instances, cfl=args.cfl)
+    n_elem = sum(c.n_elem for c, _ in inst)
+    log(f"{len(inst)} instances, {n_elem} triangles in total, meshes built in {time.perf_counter() - t0:.1f} s")
+    out = {}
+    for bits in ([64, 32] if args.only is None else ([64] if args.only == "fp64" else [32])):
+        t0 = time.perf_counter()
+        b = batch.BatchedSolver(inst, precond_bits=bits)
+        dofs = b.dofs
+        log(f"fp{bits}: {len(inst)} contexts ready in {time.perf_counter() - t0:.1f} s, {dofs} DOFs")
+        b.solve(args.warmup)
+        t0 = time.perf_counter()
+        recs = b.solve(args.steps)
+        elapsed = time.perf_counter() - t0
+        updates = sum(it.dev.dofs * sum(r.newton for r in rs) for it, rs in zip(b.instances, recs))
+        t1 = time.perf_counter()
+        lifts = b.functionals(1)
+        t_lift = time.perf_counter() - t1
+        iters = [sum(it for r in rs for it in r.gmres_iters) for rs in recs]
+        out[bits] = dict(rate=updates / elapsed, elapsed=elapsed, updates=updates, dofs=dofs, iters=iters,
+                         final=[rs[-1].residuum for rs in recs], lift_ms=1e3 * t_lift,
+                         lift=[v for v, _ in lifts])
+        log(f"fp{bits}: {args.steps} steps of every instance in {elapsed:.3f} s, {updates / elapsed:.4g} DOF-updates/s; "
+            f"lift and gradient of all instances in {1e3 * t_lift:.1f} ms")
+        b.close()
+    main_bits = 32 if 32 in out else 64
+    r = out[main_bits]
+    # the same steps with the instances one after another (one host thread): the gain of sharing the GPU
+    b = batch.BatchedSolver(inst, precond_bits=main_bits, workers=1)
+    b.solve(args.warmup)
+    t0 = time.perf_counter()
+    recs = b.solve(args.steps)
+    serial = sum(it.dev.dofs * sum(x.newton for x in rs) for it, rs in zip(b.instances, recs)) / (time.perf_counter() - t0)
+    b.close()
+    log(f"fp{main_bits}, instances one after another: {serial:.4g} DOF-updates/s")
+    result = {
+        "metric": "Newton-GMRES DOF-updates/s, mixed vs FP64 GPU at equal residual; % HBM peak",
+        "value": round(r["rate"], 1),
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * r["elapsed"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 residual/Jacobian/GMRES, f32 preconditioner" if main_bits == 32 else "f64",
+        "data": "synthetic (seeded cylinder meshes, freestream states, seeded Re and M per instance)",
+        "config": {"workload": f"{len(inst)} independent NS2D cylinder instances, 5k-50k triangles each, element order "
+                               f"2-4, Re U[1e3,1e4], M U[0.2,0.6], BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) GMRES(30) "
+                               "tol 1e-2, one context and stream per instance",
+                   "instances": len(inst), "elements": n_elem, "dofs": r["dofs"], "precond_bits": main_bits},
+        "fp64_value": round(out[64]["rate"], 1) if 64 in out else None,
+        "mixed_over_fp64": round(out[32]["rate"] / out[64]["rate"], 3) if (32 in out and 64 in out) else None,
+        "serial_value": round(serial, 1),
+        "final_residual_max_rel_diff": (max(abs(a / b - 1) for a, b in zip(out[32]["final"], out[64]["final"]))
+                                        if (32 in out and 64 in out) else None),
+        "lift_and_gradient_ms": round(r["lift_ms"], 2),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,13 +246,17 @@ def main():
     ap.add_argument("--cfl", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
-    ap.add_argument("--workload", choices=["ns2d_cylinder", "euler3d_tet"], default="ns2d_cylinder")
+    ap.add_argument("--workload", choices=["ns2d_cylinder", "euler3d_tet", "batched_ns"], default="ns2d_cylinder")
+    ap.add_argument("--instances", type=int, default=64, help="instances of the batched_ns workload")
     ap.add_argument("--cube", type=int, default=64, help="cubes per edge of the euler3d_tet workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is "
               "element partitioning with a halo layer, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched_ns":
+        bench_batched(args)
+        return
     from flowpro_v2_amd import cases, native
     tet = args.workload == "euler3d_tet"
     state = None
