@@ -1797,7 +1797,7 @@ static int op_prec(fp_ctx* c, double* out, const double* in) {
 
 // Gmres.solve (FlowPro/src/main/java/flowpro/core/LinearSolvers/Gmres.java:42-122): left
 // preconditioned, restart m, modified Gram-Schmidt, residual relative to |M^-1 b|.
-static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_out, bool* ok) {
+static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_out, bool* ok, bool x_zero) {
     const size_t N = (size_t)c->dofs;
     const int m = c->m;
     int rc;
@@ -1807,17 +1807,21 @@ static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_
     double bnrm2, nr, error;
     if ((rc = op_prec(c, c->r, b))) return rc;
     if ((rc = dnorm(c, c->r, &bnrm2))) return rc;
+    nr = bnrm2;
     if (bnrm2 == 0) bnrm2 = 1;
-    if ((rc = op_residual(c, x, b, c->aux))) return rc;
-    if ((rc = op_prec(c, c->r, c->aux))) return rc;
-    if ((rc = dnorm(c, c->r, &nr))) return rc;
+    // r = M^-1 (b - A x): every solve starts from x = 0, where b - A x is b itself, so the
+    // preconditioned residual is the vector just computed (the reference applies M^-1 twice)
+    if (!x_zero) {
+        if ((rc = op_residual(c, x, b, c->aux))) return rc;
+        if ((rc = op_prec(c, c->r, c->aux))) return rc;
+        if ((rc = dnorm(c, c->r, &nr))) return rc;
+    }
     error = nr / bnrm2;
     if (error < c->tol) { *ok = true; *err_out = error; return FP_OK; }
     double* hdev = c->scal + 8;   // H column on the device
     std::vector<double> hcol(m + 2);
     for (int it = 0; it < c->cycles; it++) {
-        double norm_r;
-        if ((rc = dnorm(c, c->r, &norm_r))) return rc;
+        const double norm_r = nr;   // |r| of the r computed last (before the loop or at the cycle's end)
         HIPCHK(c, hipMemcpyAsync(c->scal + 1, &norm_r, sizeof(double), hipMemcpyHostToDevice, c->s));
         HIPCHK(c, launch_scale_div(c->s, N, c->V, c->r, c->scal + 1));
         std::fill(s.begin(), s.end(), 0.0);
@@ -1896,7 +1900,7 @@ static int linear_solve_impl(fp_ctx* c, int op, int* iterations, double* rel_res
     bool ok = false;
     c->op = op;
     c->op_a1 = c->last_a1;
-    rc = gmres(c, c->RHS, c->x, &it, &err, &ok);
+    rc = gmres(c, c->RHS, c->x, &it, &err, &ok, true);
     c->op = 0;
     if (rc) return rc;
     if (iterations) *iterations = it;
@@ -1930,7 +1934,7 @@ extern "C" int fp_adjoint_solve(fp_ctx* c, double a1, const double* rhs, double*
     bool ok = false;
     c->op = 2;
     c->op_a1 = a1;
-    rc = gmres(c, c->RHS, c->x, &it, &err, &ok);
+    rc = gmres(c, c->RHS, c->x, &it, &err, &ok, true);
     c->op = 0;
     c->jac_valid = false;   // RHS was overwritten
     if (rc) return rc;

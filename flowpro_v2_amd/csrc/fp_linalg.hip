@@ -540,6 +540,9 @@ __device__ inline double pivot_reciprocal(double x) { return 1.0 / x; }
 // of A_ik and of L_ik are exchanged between the waves through LDS.  The diagonal block is
 // then inverted by four of the waves (Gauss-Jordan with implicit partial pivoting; pivot row
 // = largest |M[.][p]| via a DPP wave maximum on |x| with the lane index in the low bits).
+#ifndef ILU_INV_WAVES
+#define ILU_INV_WAVES 4
+#endif
 template <typename TF, int B, int MAXR, int NW>
 __global__ void __launch_bounds__(64 * NW) k_ilu_factor_rows(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
                                                          const int* __restrict__ diagpos, const int* __restrict__ tptr,
@@ -548,7 +551,7 @@ __global__ void __launch_bounds__(64 * NW) k_ilu_factor_rows(const int* __restri
     constexpr int BB = B * B;
     constexpr int NT = 64 * NW;
     constexpr int CW = (B + NW - 1) / NW;       // columns per wave in the tasks
-    constexpr int NG = 4;                       // waves of the inversion
+    constexpr int NG = ILU_INV_WAVES < NW ? ILU_INV_WAVES : NW;   // waves of the inversion
     constexpr int CG = (B + NG - 1) / NG;       // identity columns per inversion wave
     constexpr int PER = (BB + NT - 1) / NT;
     __shared__ TF S[2][BB];
@@ -793,8 +796,10 @@ __global__ void __launch_bounds__(64) k_ilu_fwd_level(const int* __restrict__ me
     y[(size_t)i * B + t] = s;
 }
 
-// backward: x_i = D_i^-1 (y_i - sum_{j>i} U_ij x_j); the row of D_i^-1 is loaded together
-// with the U blocks, before the exchange of the right-hand side through LDS
+// backward: x_i = D_i^-1 (y_i - sum_{j>i} U_ij x_j).  FP32: the row of D_i^-1 is loaded together
+// with the U blocks, before the exchange of the right-hand side through LDS.  FP64: the 80 registers
+// of that row held across the U products cost 3.4x at B = 40 (32 -> 9 us per level), so the row is
+// loaded after the exchange.
 template <typename TF, int B>
 __global__ void __launch_bounds__(64) k_ilu_bwd_level(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
                                                       const TF* __restrict__ y, TF* __restrict__ xs,
@@ -805,8 +810,10 @@ __global__ void __launch_bounds__(64) k_ilu_bwd_level(const int* __restrict__ me
     const int slot = threadIdx.x / B, t = threadIdx.x % B;
     const int li = RPW == 1 ? (int)blockIdx.x : (int)blockIdx.x * RPW + slot;   // uniform record index as above
     const bool active = slot < RPW && li < nrows;
+    constexpr bool early_d = sizeof(TF) == 4;
     int i = 0;
     TF drow[B];
+    const TF* dp = nullptr;
     if (active) {
         const int4* m = reinterpret_cast<const int4*>(meta) + (size_t)li * 4;
         const int4 m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
@@ -814,9 +821,11 @@ __global__ void __launch_bounds__(64) k_ilu_bwd_level(const int* __restrict__ me
         const int cnt = m0.y;
         const int blk[ILU_MAXN] = {m1.x, m1.y, m1.z, m1.w, m2.x, m2.y};
         const int col[ILU_MAXN] = {m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
-        const TF* D = F + (size_t)m0.z * BB + t;
+        dp = F + (size_t)m0.z * BB + t;
+        if constexpr (early_d) {
 #pragma unroll
-        for (int c = 0; c < B; c++) drow[c] = D[(size_t)c * B];
+            for (int c = 0; c < B; c++) drow[c] = dp[(size_t)c * B];
+        }
         TF s = y[(size_t)i * B + t];
 #pragma unroll
         for (int n = 0; n < ILU_MAXN; n++) {
@@ -840,6 +849,10 @@ __global__ void __launch_bounds__(64) k_ilu_bwd_level(const int* __restrict__ me
     }
     __syncthreads();
     if (!active) return;
+    if constexpr (!early_d) {
+#pragma unroll
+        for (int c = 0; c < B; c++) drow[c] = dp[(size_t)c * B];
+    }
     const TF* rr = rhs + slot * B;
     TF a0 = 0, a1 = 0, a2 = 0, a3 = 0;
 #pragma unroll
