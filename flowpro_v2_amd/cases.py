@@ -28,6 +28,8 @@ class Case:
     bnd: np.ndarray            # (nBnd, 3) int32: type, v0, v1
     wall_distance: np.ndarray | None = None
     props: dict = field(default_factory=dict)
+    types: np.ndarray | None = None    # per-element type (elementType.txt: 3 triangle, 4 square); TP then has 4 columns
+    orders: np.ndarray | None = None   # per-element order (order.txt)
 
     @property
     def n_elem(self) -> int:
@@ -149,6 +151,40 @@ def channel_bump(nx: int = 64, ny: int = 16, quads: bool = False, jitter: float 
         rows.append((INLET, vid[0, j + 1], vid[0, j]))
         rows.append((OUTLET, vid[nx, j], vid[nx, j + 1]))
     return Case(f"channel_bump_{nx}x{ny}" + ("q" if quads else ""), PXY, TP, np.array(rows, dtype=np.int32))
+
+
+def mixed_channel(nx: int = 12, ny: int = 4, quad_fraction: float = 0.5, jitter: float = 0.2, seed: int = 7,
+                  flat: bool = False) -> Case:
+    """channel_bump's cells, each kept as a quadrilateral (squareElement, type 4) with probability
+    quad_fraction or split into two triangles (type 3) with the alternating diagonals; TP has four
+    columns (-1 in the fourth for triangles) and `types` the element types.  flat: no bump."""
+    base = channel_bump(nx, ny, quads=True, jitter=jitter, seed=seed)
+    PXY = base.PXY.copy()
+    if flat:
+        PXY[:, 1] = np.repeat(np.linspace(0.0, 1.0, ny + 1)[None, :], nx + 1, 0).ravel()
+        if jitter > 0:
+            rng = np.random.default_rng(seed)
+            inner = (PXY[:, 0] > 1e-9) & (PXY[:, 0] < 3 - 1e-9) & (PXY[:, 1] > 1e-9) & (PXY[:, 1] < 1 - 1e-9)
+            PXY[inner] += jitter * min(3.0 / nx, 1.0 / ny) * rng.uniform(-0.5, 0.5, size=(inner.sum(), 2))
+    rng = np.random.default_rng(seed + 1)
+    rows, types = [], []
+    for c, q in enumerate(base.TP):
+        if rng.uniform() < quad_fraction:
+            rows.append(list(q))
+            types.append(4)
+        else:
+            a, b, cc, d = q
+            tris = [(a, b, cc), (a, cc, d)] if c % 2 == 0 else [(a, b, d), (b, cc, d)]
+            for t in tris:
+                rows.append(list(t) + [-1])
+                types.append(3)
+    TP = np.array(rows, dtype=np.int32)
+    return Case(f"mixed_{nx}x{ny}", PXY, TP, base.bnd.copy(), types=np.array(types, dtype=np.int32))
+
+
+def seeded_orders(n_elem: int, choices=(1, 2, 3, 4), seed: int = 3) -> np.ndarray:
+    """Per-element orders (an order.txt) drawn from `choices` with numpy default_rng(seed)."""
+    return np.random.default_rng(seed).choice(np.asarray(choices), size=n_elem).astype(np.int32)
 
 
 def cylinder_mesh(n_theta: int = 400, n_r: int = 250, growth: float = 1.01, jitter: float = 0.15,

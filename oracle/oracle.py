@@ -47,7 +47,9 @@ def lib():
         L = C.CDLL(LIB)
         vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
         sig = {
-            "or_create": (vp, [C.POINTER(Props), C.c_int, dp, C.c_int, ip, C.c_int, ip, dp, C.c_char_p]),
+            "or_create": (vp, [C.POINTER(Props), C.c_int, dp, C.c_int, ip, C.c_int, ip, dp, C.c_char_p, ip, ip]),
+            "or_get_blocks_flat": (None, [vp, dp, dp]), "or_nbasis_of": (C.c_int, [vp, C.c_int]),
+            "or_mass": (None, [vp, C.c_int, dp]),
             "or_destroy": (None, [vp]),
             "or_dofs": (C.c_int, [vp]), "or_nbasis": (C.c_int, [vp]), "or_neqs": (C.c_int, [vp]),
             "or_model_const": (None, [vp, dp]),
@@ -169,7 +171,11 @@ def wall_distance(PXY, TP, TT) -> np.ndarray:
 
 
 class Oracle:
-    """One case in the oracle: mirrors the reference's Mesh + LocalImplicitSolver."""
+    """One case in the oracle: mirrors the reference's Mesh + LocalImplicitSolver.
+
+    A case with per-element types or orders (`case.types`, `case.orders`) has elements of
+    different sizes: element-layout arrays (states, residuals, J v) are then flat, element e
+    owning [off[e], off[e+1])."""
 
     def __init__(self, case, props: dict, threads: int = 1):
         L = lib()
@@ -177,11 +183,20 @@ class Oracle:
         self.P = props_from(props, threads)
         PXY = np.ascontiguousarray(case.PXY, dtype=np.float64)
         TP = np.ascontiguousarray(case.TP, dtype=np.int32)
-        self.P.elem_type = 4 if TP.shape[1] == 4 else 3
+        types = getattr(case, "types", None)
+        orders = getattr(case, "orders", None)
+        self.mixed = types is not None or orders is not None
+        self.P.elem_type = 4 if (TP.shape[1] == 4 and types is None) else 3
+        ty = None if types is None else np.ascontiguousarray(types, dtype=np.int32)
+        od = None if orders is None else np.ascontiguousarray(orders, dtype=np.int32)
+        if od is not None:
+            self.P.order = int(od.max())   # Parameters.setOrder(max of order.txt) (FlowProMain.java:545)
         bnd = np.ascontiguousarray(case.bnd, dtype=np.int32)
         wd = None if case.wall_distance is None else np.ascontiguousarray(case.wall_distance, dtype=np.float64)
+        self._keep = (ty, od)
         self.h = L.or_create(C.byref(self.P), PXY.shape[0], _dp(PXY), TP.shape[0], _ip(TP), bnd.shape[0], _ip(bnd),
-                             _dp(wd) if wd is not None else None, RULES.encode())
+                             _dp(wd) if wd is not None else None, RULES.encode(),
+                             _ip(ty) if ty is not None else None, _ip(od) if od is not None else None)
         if not self.h:
             raise RuntimeError("oracle case construction failed")
         self.n_elem = TP.shape[0]
@@ -190,6 +205,18 @@ class Oracle:
         self.B = self.nb * self.neq
         self.dofs = L.or_dofs(self.h)
         self.nf = L.or_nfaces(self.h)
+        self.sizes = np.array([L.or_nbasis_of(self.h, i) * self.neq for i in range(self.n_elem)])
+        self.off = np.concatenate([[0], np.cumsum(self.sizes)])
+
+    def mass(self, i):
+        """Element mass matrix (DGpure: sum_q |J| w phi_a phi_b)."""
+        nb = int(self.sizes[i]) // self.neq
+        out = np.zeros((nb, nb))
+        lib().or_mass(self.h, int(i), _dp(out))
+        return out
+
+    def _shape(self):
+        return (self.dofs,) if self.mixed else (self.n_elem, self.B)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -217,13 +244,13 @@ class Oracle:
         return out
 
     def set_w(self, W, Wo=None, Wo2=None):
-        W = np.ascontiguousarray(W, dtype=np.float64).reshape(self.n_elem, self.B)
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(self._shape())
         Wo = None if Wo is None else np.ascontiguousarray(Wo, dtype=np.float64)
         Wo2 = None if Wo2 is None else np.ascontiguousarray(Wo2, dtype=np.float64)
         lib().or_set_w(self.h, _dp(W), _dp(Wo) if Wo is not None else None, _dp(Wo2) if Wo2 is not None else None)
 
     def get_w(self):
-        out = np.zeros((self.n_elem, self.B))
+        out = np.zeros(self._shape())
         lib().or_get_w(self.h, _dp(out))
         return out
 
@@ -243,7 +270,7 @@ class Oracle:
         return out
 
     def residual(self):
-        out = np.zeros((self.n_elem, self.B))
+        out = np.zeros(self._shape())
         lib().or_residual(self.h, _dp(out))
         return out
 
@@ -253,6 +280,26 @@ class Oracle:
     def assemble_residuum(self, dt, dto=-1.0):
         lib().or_assemble_residuum(self.h, dt, dto)
 
+    def blocks_flat(self):
+        """Mixed sizes: diag[e] (B_e x B_e) and, per element and face with a neighbour t, the
+        [B_t][B_e] block (transposed storage), as lists."""
+        TT = self.TT()
+        nd = int((self.sizes ** 2).sum())
+        nn = sum(int(self.sizes[t] * self.sizes[e]) for e in range(self.n_elem) for t in TT[e] if t > -1)
+        d, n = np.zeros(nd), np.zeros(max(nn, 1))
+        lib().or_get_blocks_flat(self.h, _dp(d), _dp(n))
+        diag, neigh, od, on = [], {}, 0, 0
+        for e in range(self.n_elem):
+            b = int(self.sizes[e])
+            diag.append(d[od:od + b * b].reshape(b, b))
+            od += b * b
+            for k, t in enumerate(TT[e]):
+                if t > -1:
+                    bt = int(self.sizes[t])
+                    neigh[(e, k)] = n[on:on + bt * b].reshape(bt, b)
+                    on += bt * b
+        return diag, neigh
+
     def blocks(self):
         d = np.zeros((self.n_elem, self.B, self.B))
         n = np.zeros((self.n_elem, self.nf, self.B, self.B))
@@ -260,8 +307,8 @@ class Oracle:
         return d, n
 
     def rhs(self):
-        r = np.zeros((self.n_elem, self.B))
-        s = np.zeros((self.n_elem, self.B))
+        r = np.zeros(self._shape())
+        s = np.zeros(self._shape())
         lib().or_get_rhs(self.h, _dp(r), _dp(s))
         return r, s
 
@@ -303,13 +350,13 @@ class Oracle:
         lib().or_set_cfl(self.h, cfl)
 
     def jvp(self, a1, v):
-        v = np.ascontiguousarray(v, dtype=np.float64).reshape(self.n_elem, self.B)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(self._shape())
         out = np.zeros_like(v)
         lib().or_jvp(self.h, a1, _dp(v), _dp(out))
         return out
 
     def vjp(self, a1, v):
-        v = np.ascontiguousarray(v, dtype=np.float64).reshape(self.n_elem, self.B)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(self._shape())
         out = np.zeros_like(v)
         lib().or_vjp(self.h, a1, _dp(v), _dp(out))
         return out
@@ -318,7 +365,7 @@ class Oracle:
         return lib().or_functional(self.h, which)
 
     def functional_grad_fd(self, which=0):
-        out = np.zeros((self.n_elem, self.B))
+        out = np.zeros(self._shape())
         lib().or_functional_grad_fd(self.h, which, _dp(out))
         return out
 
@@ -356,10 +403,20 @@ class Oracle:
 
     def global_matrix(self):
         """Dense (a1 M - dR/dW) in DOF numbering from the element blocks (small cases only)."""
-        d, nb = self.blocks()
         g = self.gindex()
         TT = self.TT()
         A = np.zeros((self.dofs, self.dofs))
+        if self.mixed:
+            diag, neigh = self.blocks_flat()
+            for e in range(self.n_elem):
+                ge, b = g[e], int(self.sizes[e])
+                A[ge:ge + b, ge:ge + b] += diag[e].T
+                for (ee, k), blk in neigh.items():
+                    if ee == e:
+                        t = TT[e, k]
+                        A[ge:ge + b, g[t]:g[t] + int(self.sizes[t])] += blk.T
+            return A
+        d, nb = self.blocks()
         for e in range(self.n_elem):
             ge = g[e]
             A[ge:ge + self.B, ge:ge + self.B] += d[e].T
@@ -373,12 +430,19 @@ class Oracle:
         """Element-major Java layout -> vector in DOF numbering (gIndex)."""
         g = self.gindex()
         x = np.zeros(self.dofs)
+        if self.mixed:
+            W = np.asarray(W).ravel()
+            for e in range(self.n_elem):
+                x[g[e]:g[e] + self.sizes[e]] = W[self.off[e]:self.off[e + 1]]
+            return x
         for e in range(self.n_elem):
             x[g[e]:g[e] + self.B] = W[e]
         return x
 
     def from_dof(self, x):
         g = self.gindex()
+        if self.mixed:
+            return np.concatenate([x[g[e]:g[e] + self.sizes[e]] for e in range(self.n_elem)])
         return np.stack([x[g[e]:g[e] + self.B] for e in range(self.n_elem)])
 
 

@@ -158,6 +158,7 @@ struct Elem {
     std::vector<double> JacV, wV, baseV, dbaseV, interpV;   // baseV[q*Nb+j], dbaseV[(q*Nb+j)*2+d], interpV[q*3+v]
     int nqf[MAXF];
     std::vector<double> JacF[MAXF], wF[MAXF], baseL[MAXF], dbaseL[MAXF], baseR[MAXF], dbaseR[MAXF], interpF[MAXF];
+    std::vector<double> xiV, xiF[MAXF];   // reference coordinates of the volume points, face-rule points on [-1, 1]
     // geometry (C/element/Element.java:516-567)
     double n[MAXF][16][2];
     double S[MAXF], Xs[2], Xes[MAXF][2], area, elemSize;
@@ -184,7 +185,9 @@ struct Case {
     int nPts = 0, nElem = 0;
     std::vector<double> PXY;
     std::vector<int> TP;
-    int nF = 3;             // faces per element (elem_type)
+    int nF = 3;             // entries per TT / TP row: the element type's faces, 4 with mixed types
+    bool mixed = false;     // per-element types or orders given
+    std::vector<int> off;   // element-layout offsets: element i owns [off[i], off[i+1]) of W, R, v
     std::vector<int> TT;    // nElem * nF
     std::vector<Elem> E;
     std::vector<int> order; // element visiting order for DOF numbering
@@ -236,7 +239,8 @@ void elem_getX(const Elem& e, const double* Xi, double* X);
 const double* elem_ref_vertex(const Elem& e, int j);
 
 Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const int* TP,
-                 int nBnd, const int* bnd, const double* wallDist, const std::string& rulesDir);
+                 int nBnd, const int* bnd, const double* wallDist, const std::string& rulesDir,
+                 const int* types = nullptr, const int* orders = nullptr);
 
 void limiter_all(Case& c, bool isFirstIter = false);
 void residual_all(Case& c, double* R);          // spatial residual R(W), Java layout

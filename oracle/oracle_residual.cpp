@@ -16,7 +16,7 @@ static void residuumWall(const Case& c, const Elem& e, int k, const T* Wown, con
     const ModelConst& M = c.Mc;
     const int nEqs = M.nEqs, dim = 2, nBasis = e.nBasis;
     const int TT = e.TT[k];
-    const int nR = nBasis;
+    const int nR = TT > -1 ? c.E[TT].nBasis : nBasis;   // the neighbour's basis size
     if (KR) for (int j = 0; j < nR * nEqs; j++) KR[j] = 0.0;
     const bool convective = true;
     const bool diffusive = M.diffusive;
@@ -290,7 +290,7 @@ void residual_all(Case& c, double* R) {
         const double* Wnb[MAXF];
         double* KR[MAXF] = {nullptr, nullptr, nullptr, nullptr};
         for (int k = 0; k < e.nF; k++) Wnb[k] = e.TT[k] > -1 ? c.E[e.TT[k]].W.data() : nullptr;
-        double* K = R + (size_t)i * B;
+        double* K = R + c.off[i];
         for (int j = 0; j < B; j++) K[j] = 0;
         residuum<double>(c, e, e.W.data(), Wnb, K, KR);
     }
@@ -359,8 +359,9 @@ void assemble(Case& c, double dt, double dto) {
         for (int k = 0; k < e.nF; k++) {
             Wnb[k] = e.TT[k] > -1 ? c.E[e.TT[k]].W.data() : nullptr;
             if (e.TT[k] > -1) {
-                RwNeigh[k].assign(B, 0.0);
-                RwNeighH[k].assign(B, 0.0);
+                const int Bn = c.E[e.TT[k]].nBasis * nEqs;
+                RwNeigh[k].assign(Bn, 0.0);
+                RwNeighH[k].assign(Bn, 0.0);
                 KR[k] = RwNeigh[k].data();
                 KRh[k] = RwNeighH[k].data();
             }
@@ -378,8 +379,9 @@ void assemble(Case& c, double dt, double dto) {
                 for (int k = 0; k < e.nF; k++) {
                     if (e.TT[k] > -1) {
                         Elem& nb = c.E[e.TT[k]];
-                        double* Aaux = &nb.ANeigh[e.faceIndexReverse[k]][(size_t)ii * B];
-                        for (int j = 0; j < B; j++) Aaux[j] = (RwNeighH[k][j] - RwNeigh[k][j]) / h;
+                        const int Bn = nb.nBasis * nEqs;
+                        double* Aaux = &nb.ANeigh[e.faceIndexReverse[k]][(size_t)ii * Bn];   // [B][B_nbr]
+                        for (int j = 0; j < Bn; j++) Aaux[j] = (RwNeighH[k][j] - RwNeigh[k][j]) / h;
                     }
                 }
             }
@@ -395,9 +397,10 @@ void assemble(Case& c, double dt, double dto) {
                 KRp[k] = nullptr;
                 if (e.TT[k] > -1) {
                     const auto& Wn = c.E[e.TT[k]].W;
-                    Wnbd[k].resize(B);
-                    for (int j = 0; j < B; j++) Wnbd[k][j] = Dual(Wn[j], 0.0);
-                    KRd[k].resize(B);
+                    const int Bn = (int)Wn.size();
+                    Wnbd[k].resize(Bn);
+                    for (int j = 0; j < Bn; j++) Wnbd[k][j] = Dual(Wn[j], 0.0);
+                    KRd[k].resize(Bn);
                     Wnbp[k] = Wnbd[k].data();
                     KRp[k] = KRd[k].data();
                 }
@@ -410,8 +413,9 @@ void assemble(Case& c, double dt, double dto) {
                 for (int k = 0; k < e.nF; k++) {
                     if (e.TT[k] > -1) {
                         Elem& nb = c.E[e.TT[k]];
-                        double* Aaux = &nb.ANeigh[e.faceIndexReverse[k]][(size_t)ii * B];
-                        for (int j = 0; j < B; j++) Aaux[j] = KRd[k][j].d;
+                        const int Bn = nb.nBasis * nEqs;
+                        double* Aaux = &nb.ANeigh[e.faceIndexReverse[k]][(size_t)ii * Bn];
+                        for (int j = 0; j < Bn; j++) Aaux[j] = KRd[k][j].d;
                     }
                 }
             }
@@ -433,13 +437,15 @@ void jvp(Case& c, double a1, const double* v, double* out) {
         std::vector<Dual> Wnbd[MAXF];
         const Dual* Wnbp[MAXF];
         Dual* KRp[MAXF] = {nullptr, nullptr, nullptr, nullptr};
-        for (int j = 0; j < B; j++) { Wd[j] = Dual(e.W[j], v[(size_t)i * B + j]); K[j] = Dual(0.0, 0.0); }
+        const size_t oi = c.off[i];
+        for (int j = 0; j < B; j++) { Wd[j] = Dual(e.W[j], v[oi + j]); K[j] = Dual(0.0, 0.0); }
         for (int k = 0; k < e.nF; k++) {
             Wnbp[k] = nullptr;
             int t = e.TT[k];
             if (t > -1) {
-                Wnbd[k].resize(B);
-                for (int j = 0; j < B; j++) Wnbd[k][j] = Dual(c.E[t].W[j], v[(size_t)t * B + j]);
+                const int Bn = (int)c.E[t].W.size();
+                Wnbd[k].resize(Bn);
+                for (int j = 0; j < Bn; j++) Wnbd[k][j] = Dual(c.E[t].W[j], v[(size_t)c.off[t] + j]);
                 Wnbp[k] = Wnbd[k].data();
             }
         }
@@ -447,8 +453,8 @@ void jvp(Case& c, double a1, const double* v, double* out) {
         for (int m = 0; m < nEqs; m++)
             for (int r = 0; r < Nb; r++) {
                 double mv = 0;
-                for (int s = 0; s < Nb; s++) mv += e.M[r * Nb + s] * v[(size_t)i * B + m * Nb + s];
-                out[(size_t)i * B + m * Nb + r] = a1 * mv - K[m * Nb + r].d;
+                for (int s = 0; s < Nb; s++) mv += e.M[r * Nb + s] * v[oi + m * Nb + s];
+                out[oi + m * Nb + r] = a1 * mv - K[m * Nb + r].d;
             }
     }
 }
@@ -471,9 +477,10 @@ void vjp(Case& c, double a1, const double* v, double* out) {
             KRp[k] = nullptr;
             int t = e.TT[k];
             if (t > -1) {
-                Wnbd[k].resize(B);
-                for (int j = 0; j < B; j++) Wnbd[k][j] = Dual(c.E[t].W[j], 0.0);
-                KRd[k].resize(B);
+                const int Bn = (int)c.E[t].W.size();
+                Wnbd[k].resize(Bn);
+                for (int j = 0; j < Bn; j++) Wnbd[k][j] = Dual(c.E[t].W[j], 0.0);
+                KRd[k].resize(Bn);
                 Wnbp[k] = Wnbd[k].data();
                 KRp[k] = KRd[k].data();
             }
@@ -482,16 +489,16 @@ void vjp(Case& c, double a1, const double* v, double* out) {
             for (int j = 0; j < B; j++) { Wd[j] = Dual(e.W[j], j == ii ? 1.0 : 0.0); K[j] = Dual(0.0, 0.0); }
             residuum<Dual>(c, e, Wd.data(), Wnbp, K.data(), KRp);
             double s = 0;
-            for (int j = 0; j < B; j++) s += -K[j].d * v[(size_t)i * B + j];
+            for (int j = 0; j < B; j++) s += -K[j].d * v[(size_t)c.off[i] + j];
             for (int k = 0; k < e.nF; k++) {
                 int t = e.TT[k];
                 if (t > -1)
-                    for (int j = 0; j < B; j++) s += KRd[k][j].d * v[(size_t)t * B + j];
+                    for (int j = 0; j < (int)KRd[k].size(); j++) s += KRd[k][j].d * v[(size_t)c.off[t] + j];
             }
             int m = ii / Nb, r = ii % Nb;
             double mv = 0;
-            for (int q = 0; q < Nb; q++) mv += e.M[q * Nb + r] * v[(size_t)i * B + m * Nb + q];
-            out[(size_t)i * B + ii] = s + a1 * mv;
+            for (int q = 0; q < Nb; q++) mv += e.M[q * Nb + r] * v[(size_t)c.off[i] + m * Nb + q];
+            out[(size_t)c.off[i] + ii] = s + a1 * mv;
         }
     }
 }
@@ -528,8 +535,7 @@ double functional(const Case& c, int which, const double* W) {
     double I = 0;
     for (int i = 0; i < c.nElem; i++) {
         const Elem& e = c.E[i];
-        const int B = e.nBasis * c.Mc.nEqs;
-        I += functional_elem(c, e, which, W ? W + (size_t)i * B : e.W.data());
+        I += functional_elem(c, e, which, W ? W + c.off[i] : e.W.data());
     }
     return I;
 }
@@ -546,7 +552,7 @@ void functional_grad_fd(Case& c, int which, double* dIdW) {
         for (int j = 0; j < B; j++) {
             Wh[j] = e.W[j] + h;
             double Iwh = functional_elem(c, e, which, Wh.data());
-            dIdW[(size_t)i * B + j] = (Iwh - Iw) / h;
+            dIdW[(size_t)c.off[i] + j] = (Iwh - Iw) / h;
             Wh[j] = e.W[j];
         }
     }

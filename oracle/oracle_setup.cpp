@@ -584,77 +584,153 @@ std::vector<double> wall_distance(int nPts, const double* PXY, int nElem, const 
     return distance;
 }
 
+// FlowProMain.createNeighborMatrix with a vertex count per element (elementType.txt: 3 triangle,
+// 4 square; face k between vertices k and (k + 1) % nV); TP and TT rows have `stride` entries,
+// the unused entries of TT are -1000
+static std::vector<int> neighbor_matrix_v(int nElem, const int* TP, int stride, const std::vector<int>& nV, int nBnd,
+                                          const int* bnd) {
+    std::map<std::pair<int, int>, std::pair<int, int>> faces;
+    auto key = [&](int i, int k) {
+        int a = TP[stride * i + k], b = TP[stride * i + (k + 1) % nV[i]];
+        return a < b ? std::make_pair(a, b) : std::make_pair(b, a);
+    };
+    for (int i = 0; i < nElem; i++)
+        for (int k = 0; k < nV[i]; k++) {
+            auto kk = key(i, k);
+            auto it = faces.find(kk);
+            if (it == faces.end()) faces[kk] = {i, -1000};
+            else it->second.second = i;
+        }
+    std::vector<int> TT((size_t)stride * nElem, -1000);
+    std::vector<char> open((size_t)stride * nElem, 0);
+    for (int i = 0; i < nElem; i++)
+        for (int k = 0; k < nV[i]; k++) {
+            auto& f = faces[key(i, k)];
+            open[stride * i + k] = 1;
+            if (f.first == i) {
+                if (f.second != -1000) { TT[stride * i + k] = f.second; open[stride * i + k] = 0; }
+            } else {
+                TT[stride * i + k] = f.first;
+                open[stride * i + k] = 0;
+            }
+        }
+    std::map<std::pair<int, int>, int> bmap;
+    for (int r = 0; r < nBnd; r++) {
+        int a = bnd[3 * r + 1], b = bnd[3 * r + 2];
+        bmap[a < b ? std::make_pair(a, b) : std::make_pair(b, a)] = bnd[3 * r];
+    }
+    for (int i = 0; i < nElem; i++)
+        for (int k = 0; k < nV[i]; k++)
+            if (open[stride * i + k]) {
+                auto it = bmap.find(key(i, k));
+                TT[stride * i + k] = it != bmap.end() ? it->second : -1;
+            }
+    return TT;
+}
+
+// C/Mesh.java:69-224 (constructor and init) for straight triangles and squares.  types / orders
+// (nullable) give every element's type (3, 4) and order (order.txt); with either of them TP has 4
+// entries per element (the fourth unused on triangles).  Quadrature orders follow the element
+// order (Parameters.volumeQuadratureOrder "auto"); a face takes the larger order of its two sides
+// (Integration.Face, C/Integration.java:114-117).
 Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const int* TP,
-                 int nBnd, const int* bnd, const double* wallDist, const std::string& rulesDir) {
+                 int nBnd, const int* bnd, const double* wallDist, const std::string& rulesDir,
+                 const int* types, const int* orders) {
     Case* c = new Case();
     c->P = P;
     c->Mc = model_init(P);
     c->state_dt = P.fixed_dt ? P.dt : 0.0;   // State constructor (C/State.java:75-79)
     c->nPts = nPts;
     c->nElem = nElem;
-    const bool square = P.elem_type == 4;
-    const int nF = square ? 4 : 3;
-    c->nF = nF;
+    const bool mixed = types || orders;
+    std::vector<int> nV(nElem), ord(nElem);
+    for (int i = 0; i < nElem; i++) {
+        nV[i] = types ? (types[i] == 4 ? 4 : 3) : (P.elem_type == 4 ? 4 : 3);
+        ord[i] = orders ? orders[i] : P.order;
+    }
+    const int stride = mixed ? 4 : nV[0];
+    c->nF = stride;
+    c->mixed = mixed;
     c->PXY.assign(PXY, PXY + 2 * nPts);
-    c->TP.assign(TP, TP + (size_t)nF * nElem);
-    c->TT = neighbor_matrix(nElem, TP, nBnd, bnd, nF);
+    c->TP.assign(TP, TP + (size_t)stride * nElem);
+    c->TT = neighbor_matrix_v(nElem, TP, stride, nV, nBnd, bnd);
     const int nEqs = c->Mc.nEqs;
-    const int order = P.order;
-    // volume and face rules of the element order (Parameters.volumeQuadratureOrder "auto")
-    c->vol = square ? load_square_rule(rulesDir, order) : load_triangle_rule(rulesDir, order);
-    c->face = load_line_rule(rulesDir, order);
-    BasisTri basis(order, square);
-    const int Nb = basis.nBasis;
-    const int B = Nb * nEqs;
-    // reference-element tables (same for every straight triangle)
-    const Rule& qv = c->vol;
-    std::vector<double> baseRef(qv.n * Nb), dXiRef(qv.n * Nb * 2);
-    for (int p = 0; p < qv.n; p++)
-        for (int m = 0; m < Nb; m++) baseRef[p * Nb + m] = basis.fun(m, &qv.x[2 * p]);
-    for (int d = 0; d < 2; d++)
-        for (int p = 0; p < qv.n; p++)
-            for (int m = 0; m < Nb; m++) dXiRef[(p * Nb + m) * 2 + d] = basis.der(m, &qv.x[2 * p], d);
+    // bases and rules per (square, order)
+    std::map<std::pair<bool, int>, BasisTri*> bases;
+    std::map<std::pair<bool, int>, Rule> vrules;
+    std::map<int, Rule> lrules;
+    auto basis_of = [&](bool sq, int k) -> const BasisTri& {
+        auto it = bases.find({sq, k});
+        if (it == bases.end()) it = bases.emplace(std::make_pair(sq, k), new BasisTri(k, sq)).first;
+        return *it->second;
+    };
+    auto vrule_of = [&](bool sq, int k) -> const Rule& {
+        auto it = vrules.find({sq, k});
+        if (it == vrules.end())
+            it = vrules.emplace(std::make_pair(sq, k), sq ? load_square_rule(rulesDir, k) : load_triangle_rule(rulesDir, k)).first;
+        return it->second;
+    };
+    auto lrule_of = [&](int k) -> const Rule& {
+        auto it = lrules.find(k);
+        if (it == lrules.end()) it = lrules.emplace(k, load_line_rule(rulesDir, k)).first;
+        return it->second;
+    };
+    c->vol = vrule_of(nV[0] == 4, ord[0]);
+    c->face = lrule_of(ord[0]);
     c->E.resize(nElem);
     for (int i = 0; i < nElem; i++) {
         Elem& e = c->E[i];
         e.index = i;
-        e.order = order;
-        e.nBasis = Nb;
-        e.nF = nF;
-        for (int k = 0; k < nF; k++) {
-            e.TT[k] = c->TT[nF * i + k];
-            e.TP[k] = TP[nF * i + k];
-            e.vert[k][0] = PXY[2 * TP[nF * i + k]];
-            e.vert[k][1] = PXY[2 * TP[nF * i + k] + 1];
-            e.wallDist[k] = wallDist ? wallDist[TP[nF * i + k]] : 0.0;
+        e.order = ord[i];
+        e.nF = nV[i];
+        e.nBasis = basis_of(e.nF == 4, e.order).nBasis;
+        for (int k = 0; k < e.nF; k++) {
+            e.TT[k] = c->TT[stride * i + k];
+            e.TP[k] = TP[stride * i + k];
+            e.vert[k][0] = PXY[2 * TP[stride * i + k]];
+            e.vert[k][1] = PXY[2 * TP[stride * i + k] + 1];
+            e.wallDist[k] = wallDist ? wallDist[TP[stride * i + k]] : 0.0;
         }
-        if (square) transform_init_square(e);
+        if (e.nF == 4) transform_init_square(e);
         else transform_init(e);
         e.dampInner.assign(nEqs, 0.0);
         e.innerIndicatorOld.assign(nEqs, 0.0);
     }
     // DOF numbering (C/Mesh.java:181-187, C/element/Element.java:241-248)
-    if (P.ordering == 1) c->order = rcm_order(nElem, c->TT.data(), nF);
+    if (P.ordering == 1) c->order = rcm_order(nElem, c->TT.data(), stride);
     else {
         c->order.resize(nElem);
         for (int i = 0; i < nElem; i++) c->order[i] = i;
     }
     int dofs0 = 0;
-    for (int idx : c->order) { c->E[idx].gIndex0 = dofs0; dofs0 += B; }
+    for (int idx : c->order) { c->E[idx].gIndex0 = dofs0; dofs0 += c->E[idx].nBasis * nEqs; }
     c->dofs = dofs0;
+    c->off.assign(nElem + 1, 0);
+    for (int i = 0; i < nElem; i++) c->off[i + 1] = c->off[i] + c->E[i].nBasis * nEqs;
     // integration tables, geometry, mass (C/Integration.java, C/element/Element.java:516-567)
     for (int i = 0; i < nElem; i++) {
         Elem& e = c->E[i];
+        const bool square = e.nF == 4;
+        const BasisTri& basis = basis_of(square, e.order);
+        const Rule& qv = vrule_of(square, e.order);
+        const int Nb = e.nBasis;
         e.nq = qv.n;
+        e.xiV = qv.x;
         e.JacV.resize(qv.n);
         for (int p = 0; p < qv.n; p++) e.JacV[p] = jacobianVol(e, &qv.x[2 * p]);
         e.wV = qv.w;
-        e.baseV = baseRef;
+        e.baseV.resize(qv.n * Nb);
         e.dbaseV.resize(qv.n * Nb * 2);
-        for (int p = 0; p < qv.n; p++) transform_basis(e, &qv.x[2 * p], &dXiRef[p * Nb * 2], &e.dbaseV[p * Nb * 2], Nb);
+        std::vector<double> dXiRef(Nb * 2);
+        for (int p = 0; p < qv.n; p++) {
+            for (int m = 0; m < Nb; m++) e.baseV[p * Nb + m] = basis.fun(m, &qv.x[2 * p]);
+            for (int d = 0; d < 2; d++)
+                for (int m = 0; m < Nb; m++) dXiRef[m * 2 + d] = basis.der(m, &qv.x[2 * p], d);
+            transform_basis(e, &qv.x[2 * p], dXiRef.data(), &e.dbaseV[p * Nb * 2], Nb);
+        }
         // getInterpolant: barycentric on the triangle, bilinear on the square
         // (transformation2Dsquare.java:137-150)
-        e.interpV.resize(qv.n * nF);
+        e.interpV.resize(qv.n * e.nF);
         for (int p = 0; p < qv.n; p++) {
             double xi = qv.x[2 * p], eta = qv.x[2 * p + 1];
             if (square) {
@@ -669,15 +745,19 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
             }
         }
     }
-    const Rule& qf = c->face;
     for (int i = 0; i < nElem; i++) {
         Elem& e = c->E[i];
-        for (int k = 0; k < nF; k++) {
-            // faceTransformation2Dline (C/transformation/faceTransformation2Dline.java:23-60)
-            int f0 = k, f1 = (k + 1) % nF;
+        const int Nb = e.nBasis;
+        const BasisTri& basis = basis_of(e.nF == 4, e.order);
+        for (int k = 0; k < e.nF; k++) {
+            // faceTransformation2Dline (C/transformation/faceTransformation2Dline.java:23-60) with the
+            // face rule of the larger order of the two sides
+            int f0 = k, f1 = (k + 1) % e.nF;
             const double* CX0 = elem_ref_vertex(e, f0);
             const double* CX1 = elem_ref_vertex(e, f1);
             double ref0[2] = {CX0[0], CX0[1]}, ref1[2] = {CX1[0], CX1[1]};
+            const int t = e.TT[k];
+            const Rule& qf = lrule_of(t > -1 ? std::max(e.order, c->E[t].order) : e.order);
             int nf = qf.n;
             e.nqf[k] = nf;
             e.wF[k] = qf.w;
@@ -685,9 +765,11 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
             e.baseL[k].resize(nf * Nb);
             e.dbaseL[k].resize(nf * Nb * 2);
             e.interpF[k].resize(nf * 2);
+            e.xiF[k].resize(nf);
             double s[2] = {ref1[0] - ref0[0], ref1[1] - ref0[1]};
             for (int p = 0; p < nf; p++) {
                 double xi = qf.x[p];
+                e.xiF[k][p] = xi;
                 double XiRef[2];
                 XiRef[0] = (1 + xi) / 2 * ref1[0] + (1 - xi) / 2 * ref0[0];
                 XiRef[1] = (1 + xi) / 2 * ref1[1] + (1 - xi) / 2 * ref0[1];
@@ -711,11 +793,14 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
                     for (int m = 0; m < Nb; m++) dXi[m * 2 + d] = basis.der(m, XiRef, d);
                 transform_basis(e, XiRef, dXi, &e.dbaseL[k][p * Nb * 2], Nb);
             }
-            int t = e.TT[k];
             if (t > -1) {
+                // the neighbour's own basis at the same physical points (elemRight.basis at
+                // elemRight.transform.getXi)
                 Elem& er = c->E[t];
-                e.baseR[k].resize(nf * Nb);
-                e.dbaseR[k].resize(nf * Nb * 2);
+                const BasisTri& bR = basis_of(er.nF == 4, er.order);
+                const int nR = er.nBasis;
+                e.baseR[k].resize(nf * nR);
+                e.dbaseR[k].resize(nf * nR * 2);
                 for (int p = 0; p < nf; p++) {
                     double xi = qf.x[p];
                     double XiRef[2];
@@ -724,18 +809,18 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
                     double X[2], XiR[2];
                     getX(e, XiRef, X);
                     getXi(er, X, XiR);
-                    for (int m = 0; m < Nb; m++) e.baseR[k][p * Nb + m] = basis.fun(m, XiR);
+                    for (int m = 0; m < nR; m++) e.baseR[k][p * nR + m] = bR.fun(m, XiR);
                     double dXi[128];
                     for (int d = 0; d < 2; d++)
-                        for (int m = 0; m < Nb; m++) dXi[m * 2 + d] = basis.der(m, XiR, d);
-                    transform_basis(er, XiR, dXi, &e.dbaseR[k][p * Nb * 2], Nb);
+                        for (int m = 0; m < nR; m++) dXi[m * 2 + d] = bR.der(m, XiR, d);
+                    transform_basis(er, XiR, dXi, &e.dbaseR[k][p * nR * 2], nR);
                 }
             }
         }
         // computeGeometry
-        for (int k = 0; k < nF; k++) {
+        for (int k = 0; k < e.nF; k++) {
             double mid[2] = {0.5 * (0), 0};
-            int f0 = k, f1 = (k + 1) % nF;
+            int f0 = k, f1 = (k + 1) % e.nF;
             const double* CX0 = elem_ref_vertex(e, f0);
             const double* CX1 = elem_ref_vertex(e, f1);
             double XiRef[2];
@@ -749,13 +834,14 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
             for (int p = 0; p < e.nqf[k]; p++) e.S[k] = e.S[k] + e.JacF[k][p] * e.wF[k][p];
         }
         // transform.getXs: the centroid of the reference triangle, the centre of the square
+        const bool square = e.nF == 4;
         double centre[2] = {square ? 0.5 : 1.0 / 3, square ? 0.5 : 1.0 / 3};
         getX(e, centre, e.Xs);
         e.area = 0;
         for (int p = 0; p < e.nq; p++) e.area = e.area + e.JacV[p] * e.wV[p];
         e.elemSize = 0;
-        for (int k = 0; k < nF; k++) e.elemSize += e.S[k];
-        e.elemSize = nF * e.area / e.elemSize;
+        for (int k = 0; k < e.nF; k++) e.elemSize += e.S[k];
+        e.elemSize = e.nF * e.area / e.elemSize;
         // mass matrix and basis weights (C/element/DGpure.java:547-573)
         e.M.assign(Nb * Nb, 0.0);
         e.Is.assign(Nb, 0.0);
@@ -767,11 +853,11 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
         }
         // DGpure.computeOrderTruncationMatrix (C/element/DGpure.java:588-626): TrunOrd =
         // M^-1 T^T Mhat^-1 T with the Taylor basis {1, x - Xs, y - Xs} at the physical points
-        if (order > 2) {
+        if (e.order > 2) {
             const int nl = 3;
             std::vector<std::vector<double>> Mhat(nl, std::vector<double>(nl, 0.0)), T(nl, std::vector<double>(Nb, 0.0));
             std::vector<double> Xq(2 * e.nq);
-            for (int p = 0; p < e.nq; p++) getX(e, &qv.x[2 * p], &Xq[2 * p]);
+            for (int p = 0; p < e.nq; p++) getX(e, &e.xiV[2 * p], &Xq[2 * p]);
             auto taylor = [&](int a, const double* X) { return a > 0 ? X[a - 1] - e.Xs[a - 1] : 1.0; };
             for (int a = 0; a < nl; a++) {
                 for (int b = 0; b < nl; b++)
@@ -799,30 +885,33 @@ Case* build_case(const Props& P, int nPts, const double* PXY, int nElem, const i
             for (int a = 0; a < Nb; a++)
                 for (int b = 0; b < Nb; b++) e.TrunOrd[a * Nb + b] = tr[a][b];
         }
+        const int B = Nb * nEqs;
         e.ADiag.assign(B * B, 0.0);
-        for (int k = 0; k < nF; k++)
-            if (e.TT[k] > -1) e.ANeigh[k].assign(B * B, 0.0);
+        for (int k = 0; k < e.nF; k++)
+            if (e.TT[k] > -1) e.ANeigh[k].assign((size_t)c->E[e.TT[k]].nBasis * nEqs * B, 0.0);   // [B_nbr][B]
         e.RHS.assign(B, 0.0);
         e.spatialR.assign(B, 0.0);
     }
     for (int i = 0; i < nElem; i++) {
         Elem& e = c->E[i];
-        for (int k = 0; k < nF; k++) {
+        for (int k = 0; k < e.nF; k++) {
             e.faceIndexReverse[k] = 0;
             if (e.TT[k] > -1)
-                for (int s = 0; s < nF; s++)
+                for (int s = 0; s < c->E[e.TT[k]].nF; s++)
                     if (c->E[e.TT[k]].TT[s] == i) { e.faceIndexReverse[k] = s; break; }
         }
     }
     // initial condition (C/element/DGpure.java:74-107, lagrange branch)
     auto W0 = const_init_condition(c->Mc);
     for (auto& e : c->E) {
+        const int B = e.nBasis * nEqs;
         e.W.assign(B, 0.0);
-        for (int j = 0; j < Nb; j++)
-            for (int m = 0; m < nEqs; m++) e.W[m * Nb + j] = W0[m];
+        for (int j = 0; j < e.nBasis; j++)
+            for (int m = 0; m < nEqs; m++) e.W[m * e.nBasis + j] = W0[m];
         e.Wo = e.W;
         e.Wo2 = e.W;
     }
+    for (auto& kv : bases) delete kv.second;
     return c;
 }
 

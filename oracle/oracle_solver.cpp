@@ -32,14 +32,14 @@ static void build_crs(Case& c) {
     c.H.assign(c.Icrs[n], 0.0);
     for (const auto& e : c.E) {
         int B = e.nBasis * nEqs;
-        std::vector<int> blk = {e.gIndex0};
+        std::vector<std::pair<int, int>> blk = {{e.gIndex0, B}};   // (first DOF, block size)
         for (int k = 0; k < e.nF; k++)
-            if (e.TT[k] > -1) blk.push_back(c.E[e.TT[k]].gIndex0);
+            if (e.TT[k] > -1) blk.push_back({c.E[e.TT[k]].gIndex0, c.E[e.TT[k]].nBasis * nEqs});
         std::sort(blk.begin(), blk.end());
         for (int i = 0; i < B; i++) {
             int pos = c.Icrs[e.gIndex0 + i];
-            for (int g : blk)
-                for (int j = 0; j < B; j++) c.Jcrs[pos++] = g + j;
+            for (const auto& g : blk)
+                for (int j = 0; j < g.second; j++) c.Jcrs[pos++] = g.first + j;
         }
     }
 }
@@ -60,7 +60,7 @@ static void update_data(Case& c) {
             for (int k = 0; k < e.nF; k++)
                 if (e.TT[k] > -1) {
                     const Elem& nb = c.E[e.TT[k]];
-                    for (int j = 0; j < B; j++) put(nb.gIndex0 + j, e.ANeigh[k][(size_t)j * B + i]);
+                    for (int j = 0; j < nb.nBasis * nEqs; j++) put(nb.gIndex0 + j, e.ANeigh[k][(size_t)j * B + i]);
                 }
         }
     }
@@ -110,7 +110,9 @@ static void ilu_set_matrix(Case& c) {
         c.ilu_cols[i] = row;
         c.ilu_diag[i] = (int)(std::lower_bound(row.begin(), row.end(), i) - row.begin());
         int B = e.nBasis * nEqs;
-        c.ilu_blocks[i].assign(row.size(), std::vector<double>((size_t)B * B, 0.0));
+        c.ilu_blocks[i].resize(row.size());   // [B_i][B_col], row-major
+        for (size_t q = 0; q < row.size(); q++)
+            c.ilu_blocks[i][q].assign((size_t)B * c.E[c.ilu_elems[row[q]]].nBasis * nEqs, 0.0);
         c.ilu_y[i].assign(B, 0.0);
     }
 }
@@ -132,8 +134,9 @@ static bool ilu_factor(Case& c) {
             if (t > -1) {
                 int pos = (int)(std::lower_bound(c.ilu_cols[i].begin(), c.ilu_cols[i].end(), c.ilu_rank[t]) - c.ilu_cols[i].begin());
                 auto& Bk = c.ilu_blocks[i][pos];
+                const int Bt = c.E[t].nBasis * nEqs;
                 for (int r = 0; r < B; r++)
-                    for (int cc = 0; cc < B; cc++) Bk[(size_t)r * B + cc] += e.ANeigh[k][(size_t)cc * B + r];
+                    for (int cc = 0; cc < Bt; cc++) Bk[(size_t)r * Bt + cc] += e.ANeigh[k][(size_t)cc * B + r];
             }
         }
     }
@@ -142,26 +145,28 @@ static bool ilu_factor(Case& c) {
         int B = c.E[c.ilu_elems[i]].nBasis * nEqs;
         for (int kk = 0; kk < c.ilu_diag[i]; kk++) {
             int k = row[kk];
+            const int Bk = c.E[c.ilu_elems[k]].nBasis * nEqs;
             const auto& Dk = c.ilu_dinv[k];
-            std::vector<double> L((size_t)B * B, 0.0);
+            std::vector<double> L((size_t)B * Bk, 0.0);   // Mat.times(blocks[i][kk], diagInverse[k])
             const auto& A = c.ilu_blocks[i][kk];
             for (int r = 0; r < B; r++)
-                for (int cc = 0; cc < B; cc++) {
+                for (int cc = 0; cc < Bk; cc++) {
                     double s = 0;
-                    for (int m = 0; m < B; m++) s = s + A[(size_t)r * B + m] * Dk[(size_t)m * B + cc];
-                    L[(size_t)r * B + cc] = s;
+                    for (int m = 0; m < Bk; m++) s = s + A[(size_t)r * Bk + m] * Dk[(size_t)m * Bk + cc];
+                    L[(size_t)r * Bk + cc] = s;
                 }
             c.ilu_blocks[i][kk] = L;
             for (int jj = kk + 1; jj < (int)row.size(); jj++) {
                 auto it = std::lower_bound(c.ilu_cols[k].begin(), c.ilu_cols[k].end(), row[jj]);
                 if (it != c.ilu_cols[k].end() && *it == row[jj]) {
-                    const auto& U = c.ilu_blocks[k][it - c.ilu_cols[k].begin()];
-                    auto& C = c.ilu_blocks[i][jj];
-                    for (int r = 0; r < B; r++)
-                        for (int m = 0; m < B; m++) {
-                            double a = L[(size_t)r * B + m];
+                    const int Bj = c.E[c.ilu_elems[row[jj]]].nBasis * nEqs;
+                    const auto& U = c.ilu_blocks[k][it - c.ilu_cols[k].begin()];   // [Bk][Bj]
+                    auto& C = c.ilu_blocks[i][jj];                                 // [B][Bj]
+                    for (int r = 0; r < B; r++)                                    // subtractProduct (:222-233)
+                        for (int m = 0; m < Bk; m++) {
+                            double a = L[(size_t)r * Bk + m];
                             if (a != 0)
-                                for (int cc = 0; cc < B; cc++) C[(size_t)r * B + cc] -= a * U[(size_t)m * B + cc];
+                                for (int cc = 0; cc < Bj; cc++) C[(size_t)r * Bj + cc] -= a * U[(size_t)m * Bj + cc];
                         }
                 }
             }
@@ -187,9 +192,10 @@ static void ilu_apply(Case& c, double* x, const double* b) {
         for (int kk = 0; kk < c.ilu_diag[i]; kk++) {
             const auto& L = c.ilu_blocks[i][kk];
             const auto& yk = c.ilu_y[row[kk]];
+            const int Bk = (int)yk.size();
             for (int r = 0; r < B; r++) {
                 double sum = 0;
-                for (int cc = 0; cc < (int)yk.size(); cc++) sum += L[(size_t)r * B + cc] * yk[cc];
+                for (int cc = 0; cc < Bk; cc++) sum += L[(size_t)r * Bk + cc] * yk[cc];
                 yi[r] -= sum;
             }
         }
@@ -204,7 +210,7 @@ static void ilu_apply(Case& c, double* x, const double* b) {
             int Bj = c.E[c.ilu_elems[row[jj]]].nBasis * nEqs;
             for (int r = 0; r < B; r++) {
                 double sum = 0;
-                for (int cc = 0; cc < Bj; cc++) sum += U[(size_t)r * B + cc] * x[gj + cc];
+                for (int cc = 0; cc < Bj; cc++) sum += U[(size_t)r * Bj + cc] * x[gj + cc];
                 rhs[r] -= sum;
             }
         }
@@ -663,10 +669,12 @@ using namespace oracle;
 
 extern "C" {
 
+// types / orders: per-element element type (3, 4) and order, nullable; with either, TP has 4 entries
+// per element
 void* or_create(const Props* P, int nPts, const double* PXY, int nElem, const int* TP, int nBnd, const int* bnd,
-                const double* wallDist, const char* rulesDir) {
+                const double* wallDist, const char* rulesDir, const int* types, const int* orders) {
     try {
-        return build_case(*P, nPts, PXY, nElem, TP, nBnd, bnd, wallDist, rulesDir);
+        return build_case(*P, nPts, PXY, nElem, TP, nBnd, bnd, wallDist, rulesDir, types, orders);
     } catch (const std::exception& ex) {
         std::fprintf(stderr, "oracle: %s\n", ex.what());
         return nullptr;
@@ -689,16 +697,17 @@ void or_set_w(void* h, const double* W, const double* Wo, const double* Wo2) {
     Case* c = (Case*)h;
     for (int i = 0; i < c->nElem; i++) {
         int B = (int)c->E[i].W.size();
-        std::memcpy(c->E[i].W.data(), W + (size_t)i * B, sizeof(double) * B);
-        std::memcpy(c->E[i].Wo.data(), (Wo ? Wo : W) + (size_t)i * B, sizeof(double) * B);
-        std::memcpy(c->E[i].Wo2.data(), (Wo2 ? Wo2 : (Wo ? Wo : W)) + (size_t)i * B, sizeof(double) * B);
+        const size_t o = c->off[i];
+        std::memcpy(c->E[i].W.data(), W + o, sizeof(double) * B);
+        std::memcpy(c->E[i].Wo.data(), (Wo ? Wo : W) + o, sizeof(double) * B);
+        std::memcpy(c->E[i].Wo2.data(), (Wo2 ? Wo2 : (Wo ? Wo : W)) + o, sizeof(double) * B);
     }
 }
 void or_get_w(void* h, double* W) {
     Case* c = (Case*)h;
     for (int i = 0; i < c->nElem; i++) {
         int B = (int)c->E[i].W.size();
-        std::memcpy(W + (size_t)i * B, c->E[i].W.data(), sizeof(double) * B);
+        std::memcpy(W + c->off[i], c->E[i].W.data(), sizeof(double) * B);
     }
 }
 void or_limiter(void* h) { limiter_all(*(Case*)h); }
@@ -734,12 +743,33 @@ void or_get_blocks(void* h, double* diag, double* neigh) {
         }
     }
 }
+// blocks with the elements' own sizes, flat: diag[e] (B_e x B_e) one after another, then for every
+// element and face with a neighbour t the [B_t][B_e] block, in element and face order
+void or_get_blocks_flat(void* h, double* diag, double* neigh) {
+    Case* c = (Case*)h;
+    size_t od = 0, on = 0;
+    for (int i = 0; i < c->nElem; i++) {
+        const Elem& e = c->E[i];
+        std::memcpy(diag + od, e.ADiag.data(), sizeof(double) * e.ADiag.size());
+        od += e.ADiag.size();
+        for (int k = 0; k < e.nF; k++)
+            if (e.TT[k] > -1) {
+                std::memcpy(neigh + on, e.ANeigh[k].data(), sizeof(double) * e.ANeigh[k].size());
+                on += e.ANeigh[k].size();
+            }
+    }
+}
+int or_nbasis_of(void* h, int i) { return ((Case*)h)->E[i].nBasis; }
+void or_mass(void* h, int i, double* M) {
+    const Elem& e = ((Case*)h)->E[i];
+    std::memcpy(M, e.M.data(), sizeof(double) * e.M.size());
+}
 void or_get_rhs(void* h, double* rhs, double* spatial) {
     Case* c = (Case*)h;
     for (int i = 0; i < c->nElem; i++) {
         const Elem& e = c->E[i];
-        std::memcpy(rhs + i * e.RHS.size(), e.RHS.data(), sizeof(double) * e.RHS.size());
-        if (spatial) std::memcpy(spatial + i * e.RHS.size(), e.spatialR.data(), sizeof(double) * e.RHS.size());
+        std::memcpy(rhs + c->off[i], e.RHS.data(), sizeof(double) * e.RHS.size());
+        if (spatial) std::memcpy(spatial + c->off[i], e.spatialR.data(), sizeof(double) * e.RHS.size());
     }
 }
 int or_linear_solve(void* h, double* x, int* iters, double* err) {
@@ -796,7 +826,7 @@ double or_l2_error_scalar(void* h) {
     double error = 0;
     for (const auto& e : c->E) {
         for (int p = 0; p < e.nq; p++) {
-            double Xi[2] = {c->vol.x[2 * p], c->vol.x[2 * p + 1]};
+            double Xi[2] = {e.xiV[2 * p], e.xiV[2 * p + 1]};
             double X[2];
             elem_getX(e, Xi, X);
             double u = 0;
