@@ -9,7 +9,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libflowpro_hip.so")
-SOURCES = ["fp_element.hip", "fp_team.hip", "fp_tet.hip", "fp_quad.hip", "fp_linalg.hip", "fp_host.cpp"]
+SOURCES = ["fp_element.hip", "fp_team.hip", "fp_tet.hip", "fp_quad.hip", "fp_mixed.hip", "fp_linalg.hip", "fp_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FLOWPRO_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

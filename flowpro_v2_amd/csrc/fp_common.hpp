@@ -115,6 +115,33 @@ struct QuadMesh {
     const int* blkd;
 };
 
+// Meshes that mix element types (triangles, quadrilaterals) and orders.  Every element is padded
+// to the largest basis of the mesh, NBM functions: coefficient m*NBM + j of element e is the
+// reference's m*nb_e + j for j < nb_e and zero otherwise; the padded unknowns are decoupled by an
+// identity block in the mass matrix, so the padded system has the same solution and the
+// factorisation the same real entries.  Per element (DOF order), all tables zero padded:
+//   vol  [(e*NQVM + q)*(1 + 3NBM)]            |det J| w, basis (NBM), physical gradients (2NBM)
+//   face [((e*4 + k)*NQFM + p)*(3 + 6NBM)]   JacobianFace w, X (2), own basis, own gradients,
+//                                             the neighbour's basis and gradients at the point
+//   fgeo [(e*4 + k)*4]                        nx, ny, 0, dL
+//   nv, nqv, nbe [e]; nqf [e*4 + k]           faces, volume points, basis functions; face points
+struct MixedMesh {
+    int nElem, nqvm, nqfm;
+    const int* nv;
+    const int* nqv;
+    const int* nbe;
+    const int* nqf;
+    const double* vol;
+    const double* face;
+    const double* fgeo;
+    const double* mass;
+    const double* minv;
+    const double* is;
+    const int* nbr;
+    const int* blkn;
+    const int* blkd;
+};
+
 // compile-time shape of the element kernels selected at run time
 struct ElemLaunch {
     int model, nb, neq, nqv, nqf, visc, av;

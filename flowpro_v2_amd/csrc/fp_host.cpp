@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <fstream>
 #include <map>
 #include <string>
 #include <unordered_map>
@@ -69,6 +70,16 @@ hipError_t launch_gather_diag(hipStream_t, int, int, const int*, const double*, 
 hipError_t launch_min(hipStream_t, size_t, const double*, double*, double*);
 hipError_t launch_colsum(hipStream_t, int, int, const double*, double*, double*);
 bool quad_supported(const ElemLaunch&);
+bool mixed_supported(const ElemLaunch&);
+hipError_t launch_residual_mixed(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const MixedMesh&, const double*,
+                                 const double*, const double*, double, double, double, double*, double*, double*);
+hipError_t launch_jacobian_mixed(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const MixedMesh&, const double*,
+                                 double, double*, float*, const double*, double*, int);
+hipError_t launch_jvp_mixed(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const MixedMesh&, const double*,
+                            const double*, double, double*);
+hipError_t launch_size_over_lambda_mixed(const ElemLaunch&, hipStream_t, const ModelParams&, const MixedMesh&, const double*,
+                                         const double*, double*);
+hipError_t launch_positivity_mixed(const ElemLaunch&, hipStream_t, const MixedMesh&, double*, int*);
 hipError_t launch_residual_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const QuadTables&, const QuadMesh&,
                                 const double*, const double*, const double*, double, double, double, double*, double*, double*);
 hipError_t launch_jacobian_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const QuadTables&, const QuadMesh&,
@@ -96,6 +107,11 @@ struct fp_ctx {
     int nElem = 0, order = 0, NB = 0, NEQ = 0, B = 0, NQV = 0, NQF = 0;
     int dim = 2, NF = 3;            // triangles (3 faces), quadrilaterals or tetrahedra (4 faces)
     bool quad = false;              // 2D quadrilaterals (squareElement)
+    bool mixed = false;             // per-element types or orders: elements padded to NB functions
+    std::vector<int> m_nb;          // mixed: basis functions of every mesh element
+    std::vector<long long> m_off;   // mixed: offset of every mesh element in the reference's element layout
+    std::vector<long long> m_g;     // mixed: the reference's gIndex of the element at DOF position r
+    long long ref_dofs = 0;         // mixed: the reference's number of unknowns
     long long dofs = 0;
     ElemLaunch L{};
     ModelParams mp{};
@@ -110,6 +126,7 @@ struct fp_ctx {
     std::vector<int> order_;        // order_[r] = mesh element at position r
     std::vector<int> rank;          // rank[mesh element] = r
     std::vector<int> blk_own_nb;    // block (row r, column of face k neighbour), -1 on boundary
+    std::vector<int> h_nbr;         // mixed: face neighbours by position (nbr on the device)
     int nnzb = 0;
     std::vector<int> lvl_f_ptr, lvl_b_ptr;
     // device
@@ -138,6 +155,9 @@ struct fp_ctx {
     TetMesh tms{};
     QuadTables qtb{};
     QuadMesh qms{};
+    MixedMesh mms{};
+    int *m_nvd = nullptr, *m_nqvd = nullptr, *m_nbd = nullptr, *m_nqfd = nullptr;
+    double *m_vol = nullptr, *m_face = nullptr;
     double *qv = nullptr, *qf = nullptr, *qn = nullptr, *xq = nullptr, *qmass = nullptr, *qminv = nullptr, *qis = nullptr;
     bool jac_valid = false;
     double last_a1 = 0;
@@ -157,6 +177,8 @@ struct fp_ctx {
 
 // element kernels of the case's cell type
 static hipError_t run_residual(fp_ctx* c, const double* a) {
+    if (c->mixed)
+        return launch_residual_mixed(c->L, c->s, c->mp, c->dg, c->mms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R, c->RHS, c->enorm);
     if (c->quad)
         return launch_residual_quad(c->L, c->s, c->mp, c->dg, c->qtb, c->qms, c->W, c->Wo, c->Wo2, a[0], a[1], a[2], c->R, c->RHS,
                                     c->enorm);
@@ -169,26 +191,31 @@ static hipError_t run_jacobian(fp_ctx* c, double a1) {
     // with FP32 block-ILU0 factors the Jacobian kernel also writes the factorisation's starting copy
     float* f32 = (c->precond == 2 && c->bits == 32) ? (float*)c->F : nullptr;
     c->f_fresh = f32 != nullptr;
+    if (c->mixed) return launch_jacobian_mixed(c->L, c->s, c->mp, c->dg, c->mms, c->W, a1, c->val, f32, nullptr, nullptr, 0);
     if (c->quad) return launch_jacobian_quad(c->L, c->s, c->mp, c->dg, c->qtb, c->qms, c->W, a1, c->val, f32, nullptr, nullptr, 0);
     if (c->dim == 3) return launch_jacobian_tet_team(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, c->val, f32);
     return launch_jacobian_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, c->val, f32);
 }
 static hipError_t run_jvp(fp_ctx* c, const double* v, double a1, double* out) {
+    if (c->mixed) return launch_jvp_mixed(c->L, c->s, c->mp, c->dg, c->mms, c->W, v, a1, out);
     if (c->quad) return launch_jvp_quad(c->L, c->s, c->mp, c->dg, c->qtb, c->qms, c->W, v, a1, out);
     if (c->dim == 3) return launch_jvp_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, v, a1, out);
     return launch_jvp(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, v, a1, out);
 }
 static hipError_t run_jtv(fp_ctx* c, const double* v, double a1, double* out) {
+    if (c->mixed) return launch_jacobian_mixed(c->L, c->s, c->mp, c->dg, c->mms, c->W, a1, nullptr, nullptr, v, out, 1);
     if (c->quad) return launch_jacobian_quad(c->L, c->s, c->mp, c->dg, c->qtb, c->qms, c->W, a1, nullptr, nullptr, v, out, 1);
     if (c->dim == 3) return launch_jacobian_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, nullptr, v, out, 1);
     return launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, nullptr, v, out, 1);
 }
 static hipError_t run_size_over_lambda(fp_ctx* c) {
+    if (c->mixed) return launch_size_over_lambda_mixed(c->L, c->s, c->mp, c->mms, c->W, c->esize, c->tmp);
     if (c->quad) return launch_size_over_lambda_quad(c->L, c->s, c->mp, c->qms, c->W, c->esize, c->tmp);
     if (c->dim == 3) return launch_size_over_lambda_tet(c->L, c->s, c->mp, c->tms, c->W, c->esize, c->isref, c->tmp);
     return launch_size_over_lambda(c->L, c->s, c->mp, c->tb, c->ms, c->W, c->esize, c->isref, c->tmp);
 }
 static hipError_t run_positivity(fp_ctx* c, int* count) {
+    if (c->mixed) return launch_positivity_mixed(c->L, c->s, c->mms, c->W, count);
     if (c->quad) return launch_positivity_quad(c->L, c->s, c->qtb, c->qms, c->W, count);
     if (c->dim == 3) return launch_positivity_tet(c->L, c->s, c->ttb, c->tms, c->W, c->isref, count);
     return launch_positivity(c->L, c->s, c->tb, c->ms, c->W, c->isref, count);
@@ -242,6 +269,50 @@ extern "C" int fp_neighbor_matrix(int n, const int* TP, int nb, const int* bnd, 
 }
 extern "C" int fp_neighbor_matrix_quad(int n, const int* TP, int nb, const int* bnd, int* TT) {
     return neighbor_matrix_2d(n, TP, 4, nb, bnd, TT);
+}
+// mixed triangles and squares: TP rows of 4 (the fourth unused on triangles), types[e] = 3 or 4;
+// TT rows of 4, the fourth entry of a triangle is -1000
+extern "C" int fp_neighbor_matrix_mixed(int n, const int* TP, const int* types, int nb, const int* bnd, int* TT) {
+    if (n < 0 || !TP || !types || !TT || (nb > 0 && !bnd)) return FP_EINVAL;
+    auto nv = [&](int i) { return types[i] == 4 ? 4 : 3; };
+    auto key = [&](int i, int k) {
+        long long a = TP[4 * i + k], b = TP[4 * i + (k + 1) % nv(i)];
+        if (a > b) std::swap(a, b);
+        return (a << 32) | b;
+    };
+    std::unordered_map<long long, std::pair<int, int>> faces;
+    faces.reserve((size_t)n * 4);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < nv(i); k++) {
+            auto it = faces.find(key(i, k));
+            if (it == faces.end()) faces.emplace(key(i, k), std::make_pair(i, -1000));
+            else it->second.second = i;
+        }
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 4; k++) {
+            if (k >= nv(i)) { TT[4 * i + k] = -1000; continue; }
+            const auto& f = faces[key(i, k)];
+            int t = -1000;
+            if (f.first == i) {
+                if (f.second != -1000) t = f.second;
+            } else {
+                t = f.first;
+            }
+            TT[4 * i + k] = t == -1000 ? -2000 : t;   // -2000: open face, resolved below
+        }
+    std::unordered_map<long long, int> bmap;
+    for (int r = 0; r < nb; r++) {
+        long long a = bnd[3 * r + 1], b = bnd[3 * r + 2];
+        if (a > b) std::swap(a, b);
+        bmap[(a << 32) | b] = bnd[3 * r];
+    }
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < nv(i); k++)
+            if (TT[4 * i + k] == -2000) {
+                auto it = bmap.find(key(i, k));
+                TT[4 * i + k] = it != bmap.end() ? it->second : -1;
+            }
+    return FP_OK;
 }
 
 // breadth-first search from `node` over unvisited elements; returns the last element of the
@@ -651,7 +722,7 @@ static void free_all(fp_ctx* c) {
     if (c->g_factor) (void)hipGraphExecDestroy(c->g_factor);
     if (c->g_apply) (void)hipGraphExecDestroy(c->g_apply);
     if (c->io) (void)hipFree(c->io);
-    void* ptrs[] = {c->qv, c->qf, c->qn, c->xq, c->qmass, c->qminv, c->qis, c->av, c->avold, c->trunc, c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
+    void* ptrs[] = {c->m_nvd, c->m_nqvd, c->m_nbd, c->m_nqfd, c->m_vol, c->m_face, c->qv, c->qf, c->qn, c->xq, c->qmass, c->qminv, c->qis, c->av, c->avold, c->trunc, c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
                     c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->val, c->F, c->iy, c->ixs, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
                     c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->istat};
     for (void* p : ptrs)
@@ -1039,6 +1110,325 @@ static int setup_quad(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
     return FP_OK;
 }
 
+// FlowPro's gaussRules files (QuadratureCentral, FlowPro/src/main/java/flowpro/core/quadrature/
+// QuadratureCentral.java:34-91): "n", then n rows of coordinates, then n weights
+namespace {
+struct FileRule {
+    int n = 0;
+    std::vector<double> x, w;   // x: n * dim
+};
+bool read_rule_file(const std::string& path, int cols, FileRule& r) {
+    std::ifstream f(path);
+    if (!f) return false;
+    f >> r.n;
+    r.x.resize((size_t)r.n * cols);
+    r.w.resize(r.n);
+    for (auto& v : r.x) f >> v;
+    for (auto& v : r.w) f >> v;
+    return (bool)f;
+}
+// the triangle rule of degree 2k - 1 (the next degree when it has a negative weight), barycentric
+// (b0, b1, b2) -> (b1, b2) on the reference triangle (QuadratureTriangle)
+bool triangle_rule_file(const std::string& dir, int k, FileRule& r) {
+    FileRule b;
+    int deg = 2 * k - 1;
+    if (!read_rule_file(dir + "/gaussTri/" + std::to_string(deg) + ".txt", 3, b)) return false;
+    for (double w : b.w)
+        if (w < 0) {
+            if (!read_rule_file(dir + "/gaussTri/" + std::to_string(deg + 1) + ".txt", 3, b)) return false;
+            break;
+        }
+    r.n = b.n;
+    r.w = b.w;
+    r.x.resize(2 * (size_t)b.n);
+    for (int i = 0; i < b.n; i++) {
+        r.x[2 * i] = b.x[3 * i + 1];
+        r.x[2 * i + 1] = b.x[3 * i + 2];
+    }
+    return true;
+}
+// QuadratureSquare: the tensor Gauss rule of order k on [0, 1]^2, i outer
+bool square_rule_file(const std::string& dir, int k, FileRule& r) {
+    FileRule l;
+    if (!read_rule_file(dir + "/gauss/" + std::to_string(k) + ".txt", 1, l)) return false;
+    r.n = l.n * l.n;
+    r.x.resize(2 * (size_t)r.n);
+    r.w.resize(r.n);
+    int s = 0;
+    for (int i = 0; i < l.n; i++)
+        for (int j = 0; j < l.n; j++, s++) {
+            r.x[2 * s] = (1 + l.x[i]) / 2;
+            r.x[2 * s + 1] = (1 + l.x[j]) / 2;
+            r.w[s] = l.w[i] * l.w[j] / 4;
+        }
+    return true;
+}
+}  // namespace
+
+// meshes mixing triangles and squares and element orders: every element is padded to c->NB basis
+// functions; the tables of each element are computed with its own basis, rule and map, the face
+// rule is the line rule of the larger order of the two sides (Integration.Face), and the
+// neighbour-side basis at a face point is the neighbour's at the point its inverse map returns
+// (affine on triangles, the Newton iteration of transformation2Dsquare.getXi on squares)
+static int setup_mixed(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
+    const int n = c->nElem, NB = c->NB, NEQ = c->NEQ;
+    if (!cs->gauss_rules_dir) {
+        c->err = "per-element types or orders need gauss_rules_dir";
+        return FP_EINVAL;
+    }
+    const std::string dir = cs->gauss_rules_dir;
+    std::vector<int> nV(n), ord(n), typ(n, cs->elem_type == 4 ? 4 : 3);
+    for (int i = 0; i < n; i++) {
+        if (cs->elem_types) typ[i] = cs->elem_types[i] == 4 ? 4 : 3;
+        nV[i] = typ[i];
+        ord[i] = cs->elem_orders ? cs->elem_orders[i] : cs->order;
+    }
+    // TP with 4 entries per element
+    std::vector<int> tp4((size_t)4 * n, -1);
+    const int stride = (cs->elem_types || cs->elem_type == 4) ? 4 : 3;
+    for (int i = 0; i < n; i++)
+        for (int v = 0; v < nV[i]; v++) tp4[4 * (size_t)i + v] = cs->tp[(size_t)stride * i + v];
+    std::vector<int> TT((size_t)4 * n);
+    int rc = fp_neighbor_matrix_mixed(n, tp4.data(), typ.data(), cs->n_bnd, cs->bnd, TT.data());
+    if (rc) return rc;
+    c->order_.resize(n);
+    if (cs->ordering == 1) fp_rcm_order_quad(n, TT.data(), c->order_.data());
+    else
+        for (int i = 0; i < n; i++) c->order_[i] = i;
+    c->rank.resize(n);
+    for (int r = 0; r < n; r++) c->rank[c->order_[r]] = r;
+    // bases and rules per (type, order)
+    std::map<std::pair<int, int>, Basis> bases;
+    std::map<std::pair<int, int>, FileRule> vrules;
+    std::map<int, FileRule> lrules;
+    auto basis_of = [&](int t, int k) -> const Basis& {
+        auto it = bases.find({t, k});
+        if (it == bases.end()) it = bases.emplace(std::make_pair(t, k), t == 4 ? make_basis_square(k) : make_basis(k)).first;
+        return it->second;
+    };
+    auto vrule_of = [&](int t, int k) -> const FileRule* {
+        auto it = vrules.find({t, k});
+        if (it == vrules.end()) {
+            FileRule r;
+            if (!(t == 4 ? square_rule_file(dir, k, r) : triangle_rule_file(dir, k, r))) return nullptr;
+            it = vrules.emplace(std::make_pair(t, k), r).first;
+        }
+        return &it->second;
+    };
+    auto lrule_of = [&](int k) -> const FileRule* {
+        auto it = lrules.find(k);
+        if (it == lrules.end()) {
+            FileRule r;
+            if (!read_rule_file(dir + "/gauss/" + std::to_string(k) + ".txt", 1, r)) return nullptr;
+            it = lrules.emplace(k, r).first;
+        }
+        return &it->second;
+    };
+    c->m_nb.resize(n);
+    int nqvm = 1, nqfm = 1;
+    for (int i = 0; i < n; i++) {
+        c->m_nb[i] = basis_of(typ[i], ord[i]).nb;
+        const FileRule* vr = vrule_of(typ[i], ord[i]);
+        if (!vr) {
+            c->err = "missing quadrature rule file under gauss_rules_dir";
+            return FP_EINVAL;
+        }
+        nqvm = std::max(nqvm, vr->n);
+        for (int k = 0; k < nV[i]; k++) {
+            const int t = TT[4 * (size_t)i + k];
+            const int qk = t > -1 ? std::max(ord[i], ord[t]) : ord[i];
+            const FileRule* lr = lrule_of(qk);
+            if (!lr) {
+                c->err = "missing quadrature rule file under gauss_rules_dir";
+                return FP_EINVAL;
+            }
+            nqfm = std::max(nqfm, lr->n);
+        }
+    }
+    // maps: triangle x = x0 + A0 xi + A1 eta (affine), square transformation2Dsquare (bilinear)
+    static const double CXT[3][2] = {{0, 0}, {1, 0}, {0, 1}};
+    static const double CXS[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+    std::vector<double> AB((size_t)n * 8);
+    for (int i = 0; i < n; i++) {
+        double vx[4], vy[4];
+        for (int v = 0; v < nV[i]; v++) {
+            vx[v] = cs->pxy[2 * tp4[4 * (size_t)i + v]];
+            vy[v] = cs->pxy[2 * tp4[4 * (size_t)i + v] + 1];
+        }
+        double* ab = &AB[(size_t)i * 8];
+        if (typ[i] == 4) {
+            ab[0] = vx[0] - vx[1] + vx[2] - vx[3]; ab[1] = vx[1] - vx[0]; ab[2] = vx[3] - vx[0]; ab[3] = vx[0];
+            ab[4] = vy[0] - vy[1] + vy[2] - vy[3]; ab[5] = vy[1] - vy[0]; ab[6] = vy[3] - vy[0]; ab[7] = vy[0];
+        } else {   // the bilinear form with a zero twist term
+            ab[0] = 0; ab[1] = vx[1] - vx[0]; ab[2] = vx[2] - vx[0]; ab[3] = vx[0];
+            ab[4] = 0; ab[5] = vy[1] - vy[0]; ab[6] = vy[2] - vy[0]; ab[7] = vy[0];
+        }
+    }
+    auto getX = [&](int i, const double* xi, double* X) {
+        const double* ab = &AB[(size_t)i * 8];
+        if (typ[i] == 4) { quad_getX(ab, xi, X); return; }
+        X[0] = ab[1] * xi[0] + ab[2] * xi[1] + ab[3];
+        X[1] = ab[5] * xi[0] + ab[6] * xi[1] + ab[7];
+    };
+    auto jac = [&](int i, const double* xi, double J[2][2]) {
+        const double* ab = &AB[(size_t)i * 8];
+        J[0][0] = ab[0] * xi[1] + ab[1];
+        J[0][1] = ab[0] * xi[0] + ab[2];
+        J[1][0] = ab[4] * xi[1] + ab[5];
+        J[1][1] = ab[4] * xi[0] + ab[6];
+    };
+    auto getXi = [&](int i, const double* X, double* xi) {
+        const double* ab = &AB[(size_t)i * 8];
+        if (typ[i] == 4) { quad_getXi(ab, X, xi); return; }
+        const double det = ab[1] * ab[6] - ab[2] * ab[5];
+        const double dx = X[0] - ab[3], dy = X[1] - ab[7];
+        xi[0] = (ab[6] * dx - ab[2] * dy) / det;
+        xi[1] = (-ab[5] * dx + ab[1] * dy) / det;
+    };
+    // basis values and physical gradients of element i at reference point xi, padded to NB
+    auto tabulate = [&](int i, const double* xi, double* phi, double* gr) {
+        const Basis& b = basis_of(typ[i], ord[i]);
+        double J[2][2];
+        jac(i, xi, J);
+        const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        const double iT[2][2] = {{J[1][1] / det, -J[0][1] / det}, {-J[1][0] / det, J[0][0] / det}};
+        for (int j = 0; j < NB; j++) {
+            phi[j] = 0;
+            gr[2 * j] = gr[2 * j + 1] = 0;
+            if (j >= b.nb) continue;
+            const double d0 = basis_der(b, j, xi[0], xi[1], 0), d1 = basis_der(b, j, xi[0], xi[1], 1);
+            phi[j] = basis_fun(b, j, xi[0], xi[1]);
+            gr[2 * j] = iT[0][0] * d0 + iT[1][0] * d1;
+            gr[2 * j + 1] = iT[0][1] * d0 + iT[1][1] * d1;
+        }
+        return std::fabs(det) / (typ[i] == 4 ? 1.0 : 2.0);   // transformation jacobian (|det| / 2 on the triangle)
+    };
+    const int VS = 1 + 3 * NB, FS = 3 + 6 * NB;
+    std::vector<double> vol((size_t)n * nqvm * VS, 0.0), face((size_t)n * 4 * nqfm * FS, 0.0), fgeo((size_t)n * 16, 0.0);
+    std::vector<double> mass((size_t)n * NB * NB, 0.0), minv((size_t)n * NB * NB, 0.0), qis((size_t)n * NB, 0.0), esize(n), Xs(2 * (size_t)n);
+    std::vector<int> nvd(n), nqvd(n), nbd(n), nqfd((size_t)4 * n, 0);
+    nbr.assign(4 * (size_t)n, -1000);
+    c->h_qis.assign((size_t)n * NB, 0.0);
+    for (int r = 0; r < n; r++) {
+        const int e = c->order_[r];
+        const FileRule& vr = *vrule_of(typ[e], ord[e]);
+        nvd[r] = nV[e];
+        nqvd[r] = vr.n;
+        nbd[r] = c->m_nb[e];
+        const double centre[2] = {typ[e] == 4 ? 0.5 : 1.0 / 3, typ[e] == 4 ? 0.5 : 1.0 / 3};
+        getX(e, centre, &Xs[2 * (size_t)r]);
+        double area = 0;
+        for (int q = 0; q < vr.n; q++) {
+            double* v = &vol[((size_t)r * nqvm + q) * VS];
+            const double jacv = tabulate(e, &vr.x[2 * q], v + 1, v + 1 + NB);
+            v[0] = jacv * vr.w[q];
+            area += v[0];
+        }
+        const int nb = c->m_nb[e];
+        double* M = &mass[(size_t)r * NB * NB];
+        for (int a = 0; a < NB; a++) {
+            for (int b = 0; b < NB; b++) {
+                double sum = 0;
+                if (a < nb && b < nb)
+                    for (int q = 0; q < vr.n; q++) {
+                        const double* v = &vol[((size_t)r * nqvm + q) * VS];
+                        sum += v[0] * v[1 + a] * v[1 + b];
+                    }
+                M[a * NB + b] = a < nb || b < nb ? sum : (a == b ? 1.0 : 0.0);   // identity on the padded unknowns
+            }
+            double is = 0;
+            if (a < nb)
+                for (int q = 0; q < vr.n; q++) is += vol[((size_t)r * nqvm + q) * VS] * vol[((size_t)r * nqvm + q) * VS + 1 + a];
+            qis[(size_t)r * NB + a] = is / area;
+            c->h_qis[(size_t)e * NB + a] = is / area;
+        }
+        std::vector<double> Mr((size_t)nb * nb), Mi;
+        for (int a = 0; a < nb; a++)
+            for (int b = 0; b < nb; b++) Mr[(size_t)a * nb + b] = M[a * NB + b];
+        if (!invert(Mr, nb, Mi)) {
+            c->err = "singular element mass matrix";
+            return FP_EINVAL;
+        }
+        for (int a = 0; a < NB; a++)
+            for (int b = 0; b < NB; b++)
+                minv[(size_t)r * NB * NB + a * NB + b] = (a < nb && b < nb) ? Mi[(size_t)a * nb + b] : (a == b ? 1.0 : 0.0);
+        esize[r] = area;
+    }
+    for (int r = 0; r < n; r++) {
+        const int e = c->order_[r];
+        double perim = 0;
+        for (int k = 0; k < nV[e]; k++) {
+            const int t = TT[4 * (size_t)e + k];
+            const FileRule& lr = *lrule_of(t > -1 ? std::max(ord[e], ord[t]) : ord[e]);
+            nqfd[4 * r + k] = lr.n;
+            const double* CA = typ[e] == 4 ? CXS[k] : CXT[k];
+            const double* CB = typ[e] == 4 ? CXS[(k + 1) % 4] : CXT[(k + 1) % 3];
+            const double s0 = CB[0] - CA[0], s1 = CB[1] - CA[1];
+            double nx = 0, ny = 0, S = 0;
+            for (int p = 0; p < lr.n; p++) {
+                const double tt = lr.x[p];
+                double xi[2] = {(1 + tt) / 2 * CB[0] + (1 - tt) / 2 * CA[0], (1 + tt) / 2 * CB[1] + (1 - tt) / 2 * CA[1]};
+                double J[2][2];
+                jac(e, xi, J);
+                const double Js0 = J[0][0] * s0 + J[0][1] * s1, Js1 = J[1][0] * s0 + J[1][1] * s1;
+                const double len = std::sqrt(Js0 * Js0 + Js1 * Js1);
+                nx = Js1 / len;
+                ny = -Js0 / len;
+                double* f = &face[(((size_t)r * 4 + k) * nqfm + p) * FS];
+                f[0] = len / 2 * lr.w[p];
+                S += len / 2 * lr.w[p];
+                getX(e, xi, f + 1);
+                tabulate(e, xi, f + 3, f + 3 + NB);
+                if (t > -1) {
+                    double xin[2];
+                    getXi(t, f + 1, xin);
+                    tabulate(t, xin, f + 3 + 3 * NB, f + 3 + 4 * NB);
+                }
+            }
+            double dL;
+            if (t > -1) {
+                const int rt = c->rank[t];
+                dL = (Xs[2 * r] - Xs[2 * rt]) * nx + (Xs[2 * r + 1] - Xs[2 * rt + 1]) * ny;
+                nbr[4 * r + k] = rt;
+            } else {
+                double mid[2], xm[2] = {(CA[0] + CB[0]) / 2, (CA[1] + CB[1]) / 2};
+                getX(e, xm, mid);
+                dL = (Xs[2 * r] - mid[0]) * nx + (Xs[2 * r + 1] - mid[1]) * ny;
+                nbr[4 * r + k] = t;
+            }
+            double* g = &fgeo[((size_t)r * 4 + k) * 4];
+            g[0] = nx; g[1] = ny; g[2] = 0; g[3] = std::fabs(dL);
+            perim += S;
+        }
+        esize[r] = nV[e] * esize[r] / perim;   // elemSize = nFaces * area / sum S
+    }
+    // the reference's element layout and DOF numbering
+    c->m_off.assign(n + 1, 0);
+    for (int i = 0; i < n; i++) c->m_off[i + 1] = c->m_off[i] + (long long)c->m_nb[i] * NEQ;
+    c->m_g.assign(n, 0);
+    long long g = 0;
+    for (int r = 0; r < n; r++) {
+        c->m_g[r] = g;
+        g += (long long)c->m_nb[c->order_[r]] * NEQ;
+    }
+    c->ref_dofs = g;
+    HIPCHK(c, upload(&c->m_vol, vol));
+    HIPCHK(c, upload(&c->m_face, face));
+    HIPCHK(c, upload(&c->fgeo, fgeo));
+    HIPCHK(c, upload(&c->esize, esize));
+    HIPCHK(c, upload(&c->qmass, mass));
+    HIPCHK(c, upload(&c->qminv, minv));
+    HIPCHK(c, upload(&c->qis, qis));
+    HIPCHK(c, upload(&c->m_nvd, nvd));
+    HIPCHK(c, upload(&c->m_nqvd, nqvd));
+    HIPCHK(c, upload(&c->m_nbd, nbd));
+    HIPCHK(c, upload(&c->m_nqfd, nqfd));
+    c->mms.nElem = n;
+    c->mms.nqvm = nqvm;
+    c->mms.nqfm = nqfm;
+    return FP_OK;
+}
+
 // tetrahedra: neighbour matrix, numbering, basis3Dtetra tables (orders 1 and 2: the constant,
 // then the Lagrange basis on the vertices), transformation3Dtetra / faceTransformation3Dtriangle
 // geometry (Element.computeGeometry).  The face normal is normalize(Js2 x Js1) as in
@@ -1207,11 +1597,32 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     c->nElem = n;
     c->order = cs->order;
     c->dim = cs->dimension == 3 ? 3 : 2;
-    c->quad = c->dim == 2 && cs->elem_type == 4;
-    c->NF = c->quad ? 4 : c->dim + 1;
+    c->mixed = c->dim == 2 && (cs->elem_types || cs->elem_orders);
+    c->quad = c->dim == 2 && cs->elem_type == 4 && !c->mixed;
+    c->NF = c->quad || c->mixed ? 4 : c->dim + 1;
     if (cs->elem_type != 0 && cs->elem_type != 3 && cs->elem_type != 4) {
         c->err = "elem_type: 3 (or 0) triangles, 4 quadrilaterals";
         return FP_EINVAL;
+    }
+    if (c->mixed) {
+        if (cs->model == MODEL_SA || cs->model == MODEL_SCALAR) {
+            c->err = "mixed element types or orders: device kernels for the NavierStokes and AdiabaticEulers models";
+            return FP_UNSUPPORTED;
+        }
+        int maxo = 0;
+        for (int i = 0; i < n; i++) {
+            const int o = cs->elem_orders ? cs->elem_orders[i] : cs->order;
+            const int t = cs->elem_types ? cs->elem_types[i] : (cs->elem_type == 4 ? 4 : 3);
+            if (o < 1 || o > 4 || (t != 0 && t != 3 && t != 4)) {
+                c->err = "mixed meshes: element orders 1-4, element types 3 or 4";
+                return FP_UNSUPPORTED;
+            }
+            maxo = std::max(maxo, o);
+        }
+        if (maxo != cs->order) {   // Parameters.order: the largest element order (beta0, c_IP)
+            c->err = "order must be the largest element order";
+            return FP_EINVAL;
+        }
     }
     if (c->quad && (cs->model == MODEL_SA || cs->model == MODEL_SCALAR)) {
         c->err = "quadrilaterals: device kernels for the NavierStokes and AdiabaticEulers models";
@@ -1235,6 +1646,16 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     } else if (c->quad) {
         c->NB = cs->order * cs->order;
         c->NEQ = cs->model == MODEL_ADIABATIC ? 3 : 4;
+    } else if (c->mixed) {
+        // padded basis size: the largest basis of the mesh rounded up to 4, 9 or 16
+        int nbmax = 0;
+        for (int i = 0; i < n; i++) {
+            const int o = cs->elem_orders ? cs->elem_orders[i] : cs->order;
+            const bool sq = cs->elem_types ? cs->elem_types[i] == 4 : cs->elem_type == 4;
+            nbmax = std::max(nbmax, sq ? o * o : o * (o + 1) / 2);
+        }
+        c->NB = nbmax <= 4 ? 4 : nbmax <= 9 ? 9 : 16;
+        c->NEQ = cs->model == MODEL_ADIABATIC ? 3 : 4;
     } else {
         c->NB = make_basis(cs->order).nb;
         c->NEQ = cs->model == MODEL_SCALAR ? 1 : (cs->model == MODEL_SA ? 5 : (cs->model == MODEL_ADIABATIC ? 3 : 4));
@@ -1256,7 +1677,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         return FP_EINVAL;
     }
     const bool av = cs->damp_tol > 0 || cs->damp_const != 0 || cs->damp_inner_tol > 0;
-    if (av && (c->dim == 3 || c->quad || cs->model == MODEL_SA)) {
+    if (av && (c->dim == 3 || c->quad || c->mixed || cs->model == MODEL_SA)) {
         c->err = "artificial viscosity: device kernels for the 2D NavierStokes and scalar models only";
         return FP_UNSUPPORTED;
     }
@@ -1266,7 +1687,10 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         return FP_EINVAL;
     }
     c->L = ElemLaunch{cs->model, c->NB, c->NEQ, c->NQV, c->NQF, diffusive ? 1 : 0, av ? 1 : 0};
-    if (!(c->dim == 3 ? tet_supported(c->L) : c->quad ? quad_supported(c->L) : element_supported(c->L) && team_supported(c->L))) {
+    if (!(c->dim == 3    ? tet_supported(c->L)
+          : c->mixed     ? mixed_supported(c->L)
+          : c->quad      ? quad_supported(c->L)
+                         : element_supported(c->L) && team_supported(c->L))) {
         c->err = "no element kernel for this model / order / quadrature";
         return FP_UNSUPPORTED;
     }
@@ -1326,9 +1750,10 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
 
     HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     std::vector<int> nbr;
-    int rc = c->dim == 3 ? setup_tet(cs, c, nbr) : c->quad ? setup_quad(cs, c, nbr) : setup_tri(cs, c, nbr);
+    int rc = c->dim == 3 ? setup_tet(cs, c, nbr) : c->mixed ? setup_mixed(cs, c, nbr) : c->quad ? setup_quad(cs, c, nbr) : setup_tri(cs, c, nbr);
     if (rc) return rc;
     const int NF = c->NF;
+    if (c->mixed) c->h_nbr = nbr;
 
     // --- BSR pattern: row r = own block + face neighbours, sorted by column
     std::vector<int> rowptr(n + 1, 0), colidx, diagpos(n);
@@ -1458,6 +1883,9 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     c->ms = Mesh{n, c->geo, c->fgeo, c->nbr, c->nbrf, c->blkn, c->blkd, c->wdist, c->av};
     c->tms = TetMesh{n, c->geo, c->fgeo, c->nbr, c->blkn, c->blkd};
     c->qms = QuadMesh{n, c->qv, c->qf, c->qn, c->xq, c->fgeo, c->qmass, c->qminv, c->qis, c->nbr, c->nbrf, c->blkn, c->blkd};
+    if (c->mixed)
+        c->mms = MixedMesh{n, c->mms.nqvm, c->mms.nqfm, c->m_nvd, c->m_nqvd, c->m_nbd, c->m_nqfd, c->m_vol, c->m_face,
+                           c->fgeo, c->qmass, c->qminv, c->qis, c->nbr, c->blkn, c->blkd};
 
     // --- state, matrix, Krylov workspace
     const size_t N = (size_t)c->dofs, BB = (size_t)c->B * c->B;
@@ -1500,8 +1928,10 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
 }
 
 extern "C" int fp_create(const fp_case* cs, fp_ctx** out) {
-    if (!cs || !out || cs->n_elems <= 0 || !cs->tp || !cs->pxy || cs->order < 1 || cs->n_vol <= 0 || cs->n_face <= 0 ||
-        !cs->vol_xi || !cs->vol_w || !cs->face_xi || !cs->face_w)
+    if (!cs || !out || cs->n_elems <= 0 || !cs->tp || !cs->pxy || cs->order < 1)
+        return FP_EINVAL;
+    const bool mixed = cs->dimension != 3 && (cs->elem_types || cs->elem_orders);   // rules read from gauss_rules_dir
+    if (!mixed && (cs->n_vol <= 0 || cs->n_face <= 0 || !cs->vol_xi || !cs->vol_w || !cs->face_xi || !cs->face_w))
         return FP_EINVAL;
     *out = nullptr;
     fp_ctx* c = new fp_ctx();
@@ -1530,7 +1960,7 @@ extern "C" int fp_info(const fp_ctx* c, long long* info) {
     if (!c || !info) return FP_EINVAL;
     info[0] = c->NB;
     info[1] = c->NEQ;
-    info[2] = c->dofs;
+    info[2] = c->mixed ? c->ref_dofs : c->dofs;
     info[3] = (long long)c->lvl_f_ptr.size() - 1;
     info[4] = (long long)c->lvl_b_ptr.size() - 1;
     info[5] = c->nnzb;
@@ -1547,10 +1977,20 @@ extern "C" void* fp_stream(fp_ctx* c) { return c ? (void*)c->s : nullptr; }
 
 // ============================================================== state
 // element layout, mesh order <-> device (DOF order)
+// mixed meshes: the reference's layout holds element e at m_off[e] as [m][nb_e]; the device pads
+// every element to [m][NB] with zeros
 static int to_device(fp_ctx* c, const double* src, double* dst) {
     const size_t B = c->B;
-    std::vector<double> h((size_t)c->dofs);
-    for (int r = 0; r < c->nElem; r++) std::memcpy(&h[r * B], src + (size_t)c->order_[r] * B, B * sizeof(double));
+    std::vector<double> h((size_t)c->dofs, 0.0);
+    if (c->mixed) {
+        for (int r = 0; r < c->nElem; r++) {
+            const int e = c->order_[r], nb = c->m_nb[e];
+            for (int m = 0; m < c->NEQ; m++)
+                std::memcpy(&h[r * B + (size_t)m * c->NB], src + c->m_off[e] + (size_t)m * nb, nb * sizeof(double));
+        }
+    } else {
+        for (int r = 0; r < c->nElem; r++) std::memcpy(&h[r * B], src + (size_t)c->order_[r] * B, B * sizeof(double));
+    }
     HIPCHK(c, hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     return FP_OK;
@@ -1560,6 +2000,14 @@ static int from_device(fp_ctx* c, const double* src, double* dst) {
     std::vector<double> h((size_t)c->dofs);
     HIPCHK(c, hipMemcpyAsync(h.data(), src, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
+    if (c->mixed) {
+        for (int r = 0; r < c->nElem; r++) {
+            const int e = c->order_[r], nb = c->m_nb[e];
+            for (int m = 0; m < c->NEQ; m++)
+                std::memcpy(dst + c->m_off[e] + (size_t)m * nb, &h[r * B + (size_t)m * c->NB], nb * sizeof(double));
+        }
+        return FP_OK;
+    }
     for (int r = 0; r < c->nElem; r++) std::memcpy(dst + (size_t)c->order_[r] * B, &h[r * B], B * sizeof(double));
     return FP_OK;
 }
@@ -1947,7 +2395,7 @@ extern "C" int fp_adjoint_solve(fp_ctx* c, double a1, const double* rhs, double*
 
 extern "C" int fp_functional(fp_ctx* c, int which, double* value, double* dIdW) {
     if (!c || !value || (which != 0 && which != 1) || c->mp.kind != MODEL_NAVIER_STOKES) return FP_EINVAL;
-    if (c->dim != 2 || c->quad) return FP_UNSUPPORTED;   // the force coefficients integrate over triangle wall edges
+    if (c->dim != 2 || c->quad || c->mixed) return FP_UNSUPPORTED;   // the force coefficients integrate over triangle wall edges
     HIPCHK(c, launch_functional(c->L, c->s, c->tb, c->ms, c->W, c->kappa, which, c->tmp, dIdW ? c->w : nullptr));
     HIPCHK(c, launch_dot(c->s, (size_t)c->nElem, c->tmp, nullptr, c->partial, c->scal, 2));
     HIPCHK(c, hipMemcpyAsync(value, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->s));
@@ -1987,6 +2435,34 @@ extern "C" int fp_export_jacobian(fp_ctx* c, double* diag, double* neigh) {
     HIPCHK(c, hipStreamSynchronize(c->s));
     std::vector<int> diagpos(c->nElem);
     HIPCHK(c, hipMemcpy(diagpos.data(), c->diagpos, sizeof(int) * c->nElem, hipMemcpyDeviceToHost));
+    if (c->mixed) {
+        // flat: diag element after element (B_e x B_e), neigh per element and face with a
+        // neighbour t (B_t x B_e), both in mesh order; block index [column][row] over the
+        // unpadded [m][nb] numbering of the two elements
+        const int NB = c->NB, NEQ = c->NEQ, B = c->B;
+        auto unpad = [&](const double* blk, int nbc, int nbr_, double* out) {
+            for (int mc = 0; mc < NEQ; mc++)
+                for (int jc = 0; jc < nbc; jc++)
+                    for (int mr = 0; mr < NEQ; mr++)
+                        for (int jr = 0; jr < nbr_; jr++)
+                            out[(size_t)(mc * nbc + jc) * nbr_ * NEQ + mr * nbr_ + jr] = blk[(size_t)(mc * NB + jc) * B + mr * NB + jr];
+        };
+        const size_t BB2 = (size_t)B * B;
+        size_t od = 0, on = 0;
+        for (int e = 0; e < c->nElem; e++) {
+            const int r = c->rank[e], nbe = c->m_nb[e];
+            if (diag) unpad(&h[(size_t)diagpos[r] * BB2], nbe, nbe, diag + od);
+            od += (size_t)nbe * NEQ * nbe * NEQ;
+            for (int k = 0; k < 4; k++) {
+                const int rt = c->h_nbr[4 * (size_t)r + k];
+                if (rt < 0) continue;
+                const int nbt = c->m_nb[c->order_[rt]];
+                if (neigh) unpad(&h[(size_t)c->blk_own_nb[4 * (size_t)r + k] * BB2], nbt, nbe, neigh + on);
+                on += (size_t)nbt * NEQ * nbe * NEQ;
+            }
+        }
+        return FP_OK;
+    }
     for (int r = 0; r < c->nElem; r++) {
         const size_t e = c->order_[r];
         if (diag) std::memcpy(diag + e * BB, &h[(size_t)diagpos[r] * BB], BB * sizeof(double));
@@ -2029,21 +2505,35 @@ extern "C" int fp_export_artificial_viscosity(fp_ctx* c, double* eps, double* da
 // Mesh.getAvgW (Mesh.java:263-271, Element.calculateAvgW): avg[e][m] = sum_j W[e][m Nb + j] Is_j
 extern "C" int fp_export_averages(fp_ctx* c, double* avg) {
     if (!c || !avg) return FP_EINVAL;
-    std::vector<double> W((size_t)c->dofs);
+    std::vector<double> W((size_t)(c->mixed ? c->ref_dofs : c->dofs));
     int rc = from_device(c, c->W, W.data());
     if (rc) return rc;
-    for (int e = 0; e < c->nElem; e++)
+    for (int e = 0; e < c->nElem; e++) {
+        const int nb = c->mixed ? c->m_nb[e] : c->NB;
+        const size_t o = c->mixed ? (size_t)c->m_off[e] : (size_t)e * c->B;
         for (int m = 0; m < c->NEQ; m++) {
             double s = 0;
-            for (int j = 0; j < c->NB; j++)
-                s = s + W[(size_t)e * c->B + m * c->NB + j] * (c->quad ? c->h_qis[(size_t)e * c->NB + j] : c->h_isref[j]);
+            for (int j = 0; j < nb; j++)
+                s = s + W[o + (size_t)m * nb + j] * (c->quad || c->mixed ? c->h_qis[(size_t)e * c->NB + j] : c->h_isref[j]);
             avg[(size_t)e * c->NEQ + m] = s;
         }
+    }
     return FP_OK;
 }
 
 extern "C" int fp_export_solution(fp_ctx* c, double* x) {
     if (!c || !x) return FP_EINVAL;
+    if (c->mixed) {   // the reference's DOF numbering: element at position r from gIndex m_g[r], [m][nb]
+        std::vector<double> h((size_t)c->dofs);
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->x, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        for (int r = 0; r < c->nElem; r++) {
+            const int nb = c->m_nb[c->order_[r]];
+            for (int m = 0; m < c->NEQ; m++)
+                std::memcpy(x + c->m_g[r] + (size_t)m * nb, &h[(size_t)r * c->B + (size_t)m * c->NB], nb * sizeof(double));
+        }
+        return FP_OK;
+    }
     HIPCHK(c, hipMemcpyAsync(x, c->x, (size_t)c->dofs * sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     return FP_OK;

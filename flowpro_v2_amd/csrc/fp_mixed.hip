@@ -1,0 +1,475 @@
+// DG kernels on meshes that mix element types (triangleElement, squareElement) and element
+// orders (order.txt), for the NavierStokes (Euler when inviscid) and AdiabaticEulers models.
+//
+// Every element is padded to NBM basis functions with zero tables, so one compile-time shape
+// serves all elements; the point counts (volume rule of the element, face rule of the larger order
+// of the two sides, Integration.Face, FlowPro/src/main/java/flowpro/core/Integration.java:114-117)
+// are runtime loop bounds.  The neighbour-side basis and gradients at a face point are tabulated at
+// setup from the neighbour's own basis at the point its inverse map returns.  One thread per
+// element (residual, J v) or per (element, own padded DOF) (exact Jacobian columns, J^T v); the
+// arithmetic is the one of DGpure.residuum / residuumWall (DGpure.java:108-355): faces first, then
+// the volume.
+#include <hip/hip_runtime.h>
+
+#include "fp_common.hpp"
+
+namespace fp {
+
+namespace {
+struct MPlain {
+    const double* W;
+    int B;
+    __device__ double operator()(int el, int i) const { return W[(size_t)el * B + i]; }
+};
+struct MTangent {
+    const double* W;
+    const double* V;
+    int B;
+    __device__ dual operator()(int el, int i) const {
+        const size_t o = (size_t)el * B + i;
+        return dual(W[o], V[o]);
+    }
+};
+struct MConst {
+    const double* W;
+    int B;
+    __device__ dual operator()(int el, int i) const { return dual(W[(size_t)el * B + i], 0.0); }
+};
+struct MColumn {
+    const double* W;
+    int B, dir;
+    __device__ dual operator()(int el, int i) const { return dual(W[(size_t)el * B + i], i == dir ? 1.0 : 0.0); }
+};
+}  // namespace
+
+template <int MK, int NE, bool VISC, class T>
+__device__ inline void mixed_face_flux(const ModelParams& P, const DGParams& G, double cIP, int TT, const double* n, double dL,
+                                       const double* X, T* WL, T* dWL, T* WR, T* dWR, T* g) {
+    if (TT < 0) {
+        boundary_value<MK>(P, WL, n, TT, X, WR);
+#pragma unroll
+        for (int m = 0; m < 2 * NE; m++) dWR[m] = dWL[m];
+    }
+    T fn[NE];
+    numerical_convective_flux<MK>(P, WL, WR, n, TT, fn);
+#pragma unroll
+    for (int m = 0; m < NE; m++) g[m] = -fn[m];
+    if constexpr (VISC) {
+        T Wc[NE], dWc[2 * NE], fvn[NE];
+#pragma unroll
+        for (int m = 0; m < NE; m++) {
+            Wc[m] = TT > -1 ? (WL[m] + WR[m]) / 2.0 : WR[m];
+            T jump = G.beta0 * (WR[m] - WL[m]) / dL;
+            dWc[m] = (dWL[m] + dWR[m]) / 2.0 + jump * n[0];
+            dWc[NE + m] = (dWL[NE + m] + dWR[NE + m]) / 2.0 + jump * n[1];
+        }
+        numerical_diffusive_flux<MK>(P, Wc, dWc, n, TT, fvn);
+        const bool ip = TT > -1 || is_ip_face<MK>(TT);
+#pragma unroll
+        for (int m = 0; m < NE; m++) {
+            g[m] += fvn[m];
+            if (ip) g[m] -= cIP * (WL[m] - WR[m]);
+        }
+    }
+}
+
+template <int NB, int NE, class T, class Acc>
+__device__ inline void mixed_interp(Acc c, int el, const double* phi, const double* gr, T* Wp, T* dWp) {
+#pragma unroll
+    for (int m = 0; m < NE; m++) {
+        T s = 0.0, sx = 0.0, sy = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const T a = c(el, m * NB + j);
+            s += a * phi[j];
+            sx += a * gr[2 * j];
+            sy += a * gr[2 * j + 1];
+        }
+        Wp[m] = s;
+        dWp[m] = sx;
+        dWp[NE + m] = sy;
+    }
+}
+
+// DGpure.limiter's c_IP: the penalty on elements of order > 1 of a diffusive model
+__device__ inline double mixed_cip(const DGParams& G, const MixedMesh& mm, int e) { return mm.nbe[e] > 1 ? G.cIP : 0.0; }
+
+template <int MK, int NB, bool VISC, class T, class Own, class Nb, class Sink>
+__device__ void mixed_residual(const ModelParams& P, const DGParams& G, const MixedMesh& mm, int e, Own own, Nb nbw, Sink& sk) {
+    constexpr int NE = neq_of<MK>();
+    constexpr int FS = 3 + 6 * NB;
+    const double cIP = mixed_cip(G, mm, e);
+    const int nv = mm.nv[e];
+#pragma unroll 1
+    for (int k = 0; k < nv; k++) {
+        const int TT = mm.nbr[e * 4 + k];
+        const double* fg = mm.fgeo + ((size_t)e * 4 + k) * 4;
+        const double n[2] = {fg[0], fg[1]};
+        const double dL = fg[3];
+        const int nq = mm.nqf[e * 4 + k];
+#pragma unroll 1
+        for (int p = 0; p < nq; p++) {
+            const double* fp = mm.face + (((size_t)e * 4 + k) * mm.nqfm + p) * FS;
+            const double* phiL = fp + 3;
+            T WL[NE], dWL[2 * NE], WR[NE], dWR[2 * NE];
+            mixed_interp<NB, NE, T>(own, e, phiL, phiL + NB, WL, dWL);
+            const double* phiR = nullptr;
+            if (TT > -1) {
+                phiR = fp + 3 + 3 * NB;
+                mixed_interp<NB, NE, T>(nbw, TT, phiR, phiR + NB, WR, dWR);
+            }
+            T g[NE];
+            mixed_face_flux<MK, NE, VISC>(P, G, cIP, TT, n, dL, fp + 1, WL, dWL, WR, dWR, g);
+            sk.face(k, p, phiL, phiR, fp[0], g);
+        }
+    }
+    if (mm.nbe[e] > 1) {   // the volume fluxes vanish at order 1 (DGpure.residuum)
+        const int nq = mm.nqv[e];
+#pragma unroll 1
+        for (int q = 0; q < nq; q++) {
+            const double* vp = mm.vol + ((size_t)e * mm.nqvm + q) * (1 + 3 * NB);
+            T Wq[NE], dWq[2 * NE], fx[NE], fy[NE];
+            mixed_interp<NB, NE, T>(own, e, vp + 1, vp + 1 + NB, Wq, dWq);
+            const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
+            convective_flux<MK>(P, Wq, ex, fx);
+            convective_flux<MK>(P, Wq, ey, fy);
+            if constexpr (VISC) {
+                T gx[NE], gy[NE];
+                diffusive_flux<MK>(P, Wq, dWq, ex, gx);
+                diffusive_flux<MK>(P, Wq, dWq, ey, gy);
+#pragma unroll
+                for (int m = 0; m < NE; m++) { fx[m] -= gx[m]; fy[m] -= gy[m]; }
+            }
+            sk.vol(vp[0], vp + 1 + NB, fx, fy);
+        }
+    }
+}
+
+template <int NB, int NE>
+struct MValueSink {
+    double K[NB * NE];
+    __device__ void face(int, int, const double* phiL, const double*, double jw, const double* g) {
+#pragma unroll
+        for (int m = 0; m < NE; m++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) K[m * NB + j] += (jw * phiL[j]) * g[m];
+    }
+    __device__ void vol(double jw, const double* gr, const double* fx, const double* fy) {
+#pragma unroll
+        for (int m = 0; m < NE; m++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) K[m * NB + j] += jw * (fx[m] * gr[2 * j] + fy[m] * gr[2 * j + 1]);
+    }
+};
+template <int NB, int NE>
+struct MTangentSink {
+    double K[NB * NE];
+    __device__ void face(int, int, const double* phiL, const double*, double jw, const dual* g) {
+#pragma unroll
+        for (int m = 0; m < NE; m++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) K[m * NB + j] += (jw * phiL[j]) * g[m].d;
+    }
+    __device__ void vol(double jw, const double* gr, const dual* fx, const dual* fy) {
+#pragma unroll
+        for (int m = 0; m < NE; m++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) K[m * NB + j] += jw * (fx[m].d * gr[2 * j] + fy[m].d * gr[2 * j + 1]);
+    }
+};
+
+template <int MK, int NB, bool VISC>
+__global__ void __launch_bounds__(128) k_residual_mixed(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
+                                                        const double* __restrict__ Wo, const double* __restrict__ Wo2, double a1,
+                                                        double a2, double a3, double* __restrict__ R, double* __restrict__ RHS,
+                                                        double* __restrict__ enorm) {
+    constexpr int NE = neq_of<MK>(), B = NB * NE;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mm.nElem) return;
+    MValueSink<NB, NE> sk;
+#pragma unroll
+    for (int i = 0; i < B; i++) sk.K[i] = 0.0;
+    mixed_residual<MK, NB, VISC, double>(P, G, mm, e, MPlain{W, B}, MPlain{W, B}, sk);
+    const double* M = mm.mass + (size_t)e * NB * NB;
+    double H[B];
+#pragma unroll
+    for (int m = 0; m < NE; m++)
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const size_t o = (size_t)e * B + m * NB + i;
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                const size_t oj = (size_t)e * B + m * NB + j;
+                const double mij = M[i * NB + j];
+                s += mij * a1 * W[oj] + mij * a2 * Wo[oj] + mij * a3 * Wo2[oj];
+            }
+            R[o] = sk.K[m * NB + i];
+            H[m * NB + i] = sk.K[m * NB + i] - s;
+            RHS[o] = H[m * NB + i];
+        }
+    if (!enorm) return;
+    const double* Mi = mm.minv + (size_t)e * NB * NB;
+#pragma unroll
+    for (int which = 0; which < 2; which++)
+#pragma unroll
+        for (int m = 0; m < NE; m++) {
+            const double* v = (which ? H : sk.K) + m * NB;
+            double sum = 0.0;
+#pragma unroll
+            for (int a = 0; a < NB; a++) {
+                double t = 0.0;
+#pragma unroll
+                for (int b = 0; b < NB; b++) t += Mi[a * NB + b] * v[b];
+                sum += v[a] * t;
+            }
+            enorm[(size_t)e * 2 * NE + which * NE + m] = sum;
+        }
+}
+
+// Thread (e, dir) over the padded DOFs.  mode 0: column dir of the diagonal block (-dK/dW + a1 M)
+// and of the neighbour blocks (the neighbour-side integrand, KR); mode 1: ((a1 M - dR/dW)^T v)[e, dir].
+template <int NB, int NE, typename BT>
+struct MColumnSink : MTangentSink<NB, NE> {
+    static constexpr int B = NB * NE;
+    const MixedMesh* mm;
+    BT* blocks;
+    float* fblocks;
+    const double* v;
+    int e, dir, mode;
+    double dot;
+    __device__ void face(int k, int p, const double* phiL, const double* phiR, double jw, const dual* g) {
+        MTangentSink<NB, NE>::face(k, p, phiL, phiR, jw, g);
+        if (!phiR) return;
+        if (mode == 0) {
+            const size_t base = (size_t)mm->blkn[e * 4 + k] * B * B + (size_t)dir * B;
+#pragma unroll
+            for (int m = 0; m < NE; m++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const double a = (jw * phiR[j]) * g[m].d;
+                    BT* c = blocks + base + m * NB + j;
+                    *c = p == 0 ? (BT)a : (BT)(*c + a);
+                    if (fblocks && p + 1 == mm->nqf[e * 4 + k]) fblocks[base + m * NB + j] = (float)*c;
+                }
+        } else {
+            const double* vn = v + (size_t)mm->nbr[e * 4 + k] * B;
+#pragma unroll
+            for (int m = 0; m < NE; m++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) dot += ((jw * phiR[j]) * g[m].d) * vn[m * NB + j];
+        }
+    }
+};
+
+template <int MK, int NB, bool VISC, typename BT>
+__global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W, double a1,
+                                                       BT* __restrict__ blocks, float* __restrict__ fblocks,
+                                                       const double* __restrict__ v, double* __restrict__ out, int mode) {
+    constexpr int NE = neq_of<MK>(), B = NB * NE;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = (int)(t / B), dir = (int)(t % B);
+    if (e >= mm.nElem) return;
+    MColumnSink<NB, NE, BT> sk;
+#pragma unroll
+    for (int i = 0; i < B; i++) sk.K[i] = 0.0;
+    sk.mm = &mm;
+    sk.blocks = blocks;
+    sk.fblocks = fblocks;
+    sk.v = v;
+    sk.e = e;
+    sk.dir = dir;
+    sk.mode = mode;
+    sk.dot = 0.0;
+    mixed_residual<MK, NB, VISC, dual>(P, G, mm, e, MColumn{W, B, dir}, MConst{W, B}, sk);
+    const double* M = mm.mass + (size_t)e * NB * NB;
+    const int md = dir / NB, jd = dir % NB;
+    if (mode == 0) {
+        const size_t base = (size_t)mm.blkd[e] * B * B + (size_t)dir * B;
+#pragma unroll
+        for (int i = 0; i < B; i++) {
+            double a = -sk.K[i];
+            if (i / NB == md) a += a1 * M[jd * NB + (i % NB)];
+            blocks[base + i] = (BT)a;
+            if (fblocks) fblocks[base + i] = (float)a;
+        }
+    } else {
+        const double* ve = v + (size_t)e * B;
+        double s = sk.dot;
+#pragma unroll
+        for (int i = 0; i < B; i++) {
+            double a = -sk.K[i];
+            if (i / NB == md) a += a1 * M[jd * NB + (i % NB)];
+            s += a * ve[i];
+        }
+        out[(size_t)e * B + dir] = s;
+    }
+}
+
+template <int MK, int NB, bool VISC>
+__global__ void __launch_bounds__(64) k_jvp_mixed(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
+                                                  const double* __restrict__ v, double a1, double* __restrict__ out) {
+    constexpr int NE = neq_of<MK>(), B = NB * NE;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mm.nElem) return;
+    MTangentSink<NB, NE> sk;
+#pragma unroll
+    for (int i = 0; i < B; i++) sk.K[i] = 0.0;
+    mixed_residual<MK, NB, VISC, dual>(P, G, mm, e, MTangent{W, v, B}, MTangent{W, v, B}, sk);
+    const double* M = mm.mass + (size_t)e * NB * NB;
+#pragma unroll
+    for (int m = 0; m < NE; m++)
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < NB; j++) s += M[i * NB + j] * v[(size_t)e * B + m * NB + j];
+            out[(size_t)e * B + m * NB + i] = a1 * s - sk.K[m * NB + i];
+        }
+}
+
+template <int MK, int NB>
+__global__ void k_size_over_lambda_mixed(ModelParams P, MixedMesh mm, const double* __restrict__ W, const double* __restrict__ esize,
+                                         double* __restrict__ out) {
+    constexpr int NE = neq_of<MK>(), B = NB * NE;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mm.nElem) return;
+    const double* is = mm.is + (size_t)e * NB;
+    double wa[NE];
+#pragma unroll
+    for (int m = 0; m < NE; m++) {
+        wa[m] = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) wa[m] = wa[m] + W[(size_t)e * B + m * NB + j] * is[j];
+    }
+    const double lam = max_eigenvalue<MK>(P, wa);
+    out[e] = lam > 0 ? esize[e] / lam : __builtin_huge_val();
+}
+
+// Element.positivityLimiter (Element.java:385-476) over the element's volume and face points; only
+// the element's own coefficients change (the padded ones stay zero)
+template <int MK, int NB>
+__global__ void k_positivity_mixed(MixedMesh mm, double* __restrict__ W, int* __restrict__ count) {
+    constexpr int NE = neq_of<MK>(), B = NB * NE;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mm.nElem || mm.nbe[e] == 1) return;
+    const int nb = mm.nbe[e];
+    const double* is = mm.is + (size_t)e * NB;
+    double w[B], wa[NE];
+#pragma unroll
+    for (int i = 0; i < B; i++) w[i] = W[(size_t)e * B + i];
+#pragma unroll
+    for (int m = 0; m < NE; m++) {
+        wa[m] = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) wa[m] = wa[m] + w[m * NB + j] * is[j];
+    }
+    if (!admissible<NE>(wa)) return;
+    double theta = 1.0;
+    auto at = [&](const double* phi) {
+        double wx[NE];
+#pragma unroll
+        for (int m = 0; m < NE; m++) {
+            wx[m] = 0.0;
+#pragma unroll
+            for (int j = 0; j < NB; j++) wx[m] += w[m * NB + j] * phi[j];
+        }
+        theta = fmin(theta, admissible_theta<NE>(wa, wx));
+    };
+    for (int q = 0; q < mm.nqv[e]; q++) at(mm.vol + ((size_t)e * mm.nqvm + q) * (1 + 3 * NB) + 1);
+    for (int k = 0; k < mm.nv[e]; k++)
+        for (int p = 0; p < mm.nqf[e * 4 + k]; p++) at(mm.face + (((size_t)e * 4 + k) * mm.nqfm + p) * (3 + 6 * NB) + 3);
+    if (theta >= 1) return;
+#pragma unroll
+    for (int m = 0; m < NE; m++)
+#pragma unroll
+        for (int i = 0; i < NB; i++)
+            if (i < nb) W[(size_t)e * B + m * NB + i] = wa[m] + theta * (w[m * NB + i] - wa[m]);
+    atomicAdd(count, 1);
+}
+
+// ------------------------------------------------------------------ dispatch
+// (model, padded basis size, viscous): the padded size is the largest basis of the mesh rounded up
+// to 4, 9 or 16 (element orders up to 4)
+#define FP_MIXED_CASES(X)                                                                                            \
+    X(MODEL_NAVIER_STOKES, 4, false) X(MODEL_NAVIER_STOKES, 9, false) X(MODEL_NAVIER_STOKES, 16, false)              \
+    X(MODEL_NAVIER_STOKES, 4, true) X(MODEL_NAVIER_STOKES, 9, true) X(MODEL_NAVIER_STOKES, 16, true)                 \
+    X(MODEL_ADIABATIC, 4, false) X(MODEL_ADIABATIC, 9, false) X(MODEL_ADIABATIC, 16, false)
+#define FP_MMATCH(MK, NB, V) (L.model == MK && L.nb == NB && (L.visc != 0) == V && !L.av)
+
+bool mixed_supported(const ElemLaunch& L) {
+#define X(MK, NB, V) \
+    if (FP_MMATCH(MK, NB, V)) return true;
+    FP_MIXED_CASES(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_residual_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
+                                 const double* W, const double* Wo, const double* Wo2, double a1, double a2, double a3, double* R,
+                                 double* RHS, double* enorm) {
+    dim3 blk(128), grd((mm.nElem + 127) / 128);
+#define X(MK, NB, V)                                                                                                      \
+    if (FP_MMATCH(MK, NB, V)) {                                                                                           \
+        hipLaunchKernelGGL((k_residual_mixed<MK, NB, V>), grd, blk, 0, s, P, G, mm, W, Wo, Wo2, a1, a2, a3, R, RHS, enorm); \
+        return hipGetLastError();                                                                                         \
+    }
+    FP_MIXED_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_jacobian_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
+                                 const double* W, double a1, double* blocks, float* fblocks, const double* v, double* out, int mode) {
+    const long long threads = (long long)mm.nElem * L.nb * L.neq;
+    dim3 blk(64), grd((unsigned)((threads + 63) / 64));
+#define X(MK, NB, V)                                                                                                   \
+    if (FP_MMATCH(MK, NB, V)) {                                                                                        \
+        hipLaunchKernelGGL((k_jacobian_mixed<MK, NB, V, double>), grd, blk, 0, s, P, G, mm, W, a1, blocks, fblocks, v, out, \
+                           mode);                                                                                      \
+        return hipGetLastError();                                                                                      \
+    }
+    FP_MIXED_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_jvp_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
+                            const double* W, const double* v, double a1, double* out) {
+    dim3 blk(64), grd((mm.nElem + 63) / 64);
+#define X(MK, NB, V)                                                                              \
+    if (FP_MMATCH(MK, NB, V)) {                                                                   \
+        hipLaunchKernelGGL((k_jvp_mixed<MK, NB, V>), grd, blk, 0, s, P, G, mm, W, v, a1, out);   \
+        return hipGetLastError();                                                                 \
+    }
+    FP_MIXED_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_size_over_lambda_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const MixedMesh& mm, const double* W,
+                                         const double* esize, double* out) {
+    dim3 blk(256), grd((mm.nElem + 255) / 256);
+#define X(MK, NB, V)                                                                                   \
+    if (FP_MMATCH(MK, NB, V)) {                                                                        \
+        hipLaunchKernelGGL((k_size_over_lambda_mixed<MK, NB>), grd, blk, 0, s, P, mm, W, esize, out);  \
+        return hipGetLastError();                                                                      \
+    }
+    FP_MIXED_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_positivity_mixed(const ElemLaunch& L, hipStream_t s, const MixedMesh& mm, double* W, int* count) {
+    dim3 blk(128), grd((mm.nElem + 127) / 128);
+#define X(MK, NB, V)                                                                      \
+    if (FP_MMATCH(MK, NB, V)) {                                                           \
+        hipLaunchKernelGGL((k_positivity_mixed<MK, NB>), grd, blk, 0, s, mm, W, count);   \
+        return hipGetLastError();                                                         \
+    }
+    FP_MIXED_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+}  // namespace fp

@@ -25,7 +25,7 @@ EXPORTS = ["fp_neighbor_matrix", "fp_rcm_order", "fp_create", "fp_destroy", "fp_
            "fp_export_solution", "fp_jv", "fp_jtv", "fp_time_kernel", "fp_stream", "fp_linear_solve_jfnk",
            "fp_functional", "fp_adjoint_solve", "fp_wall_distance", "fp_neighbor_matrix_tet", "fp_rcm_order_tet",
            "fp_limiter", "fp_export_artificial_viscosity", "fp_export_averages", "fp_neighbor_matrix_quad",
-           "fp_rcm_order_quad"]
+           "fp_rcm_order_quad", "fp_neighbor_matrix_mixed"]
 
 
 class FpCase(C.Structure):
@@ -53,6 +53,8 @@ class FpCase(C.Structure):
         ("n_damp_inner_coef", C.c_int), ("damp_inner_coef", C.c_double * 5),
         ("numerical_flux", C.c_int),
         ("elem_type", C.c_int),
+        ("elem_types", C.POINTER(C.c_int)), ("elem_orders", C.POINTER(C.c_int)),
+        ("gauss_rules_dir", C.c_char_p),
     ]
 
 
@@ -99,6 +101,7 @@ def lib() -> C.CDLL:
         "fp_rcm_order_tet": (C.c_int, [C.c_int, ip, ip]),
         "fp_neighbor_matrix_quad": (C.c_int, [C.c_int, ip, C.c_int, ip, ip]),
         "fp_rcm_order_quad": (C.c_int, [C.c_int, ip, ip]),
+        "fp_neighbor_matrix_mixed": (C.c_int, [C.c_int, ip, ip, C.c_int, ip, ip]),
         "fp_limiter": (C.c_int, [vp, C.c_int]),
         "fp_export_artificial_viscosity": (C.c_int, [vp, dp, dp]),
         "fp_export_averages": (C.c_int, [vp, dp]),
@@ -192,3 +195,16 @@ def rcm_order(TT: np.ndarray) -> np.ndarray:
     if rc:
         raise FatalError(f"fp_rcm_order: {rc}")
     return order
+
+
+def neighbor_matrix_mixed(TP: np.ndarray, types: np.ndarray, bnd: np.ndarray) -> np.ndarray:
+    """createNeighborMatrix on a mesh of triangles and squares (fp_neighbor_matrix_mixed):
+    TP[n_elems][4] (the fourth vertex unused on a triangle), TT[n_elems][4]."""
+    TP = np.ascontiguousarray(TP, dtype=np.int32)
+    types = np.ascontiguousarray(types, dtype=np.int32)
+    bnd = np.ascontiguousarray(bnd, dtype=np.int32)
+    TT = np.zeros((TP.shape[0], 4), np.int32)
+    rc = lib().fp_neighbor_matrix_mixed(TP.shape[0], iptr(TP), iptr(types), bnd.shape[0], iptr(bnd), iptr(TT))
+    if rc:
+        raise FatalError(f"fp_neighbor_matrix_mixed: {rc}")
+    return TT

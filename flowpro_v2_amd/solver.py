@@ -41,10 +41,24 @@ class DeviceCase:
             raise UnsupportedCase(f"preconditioner {prec}")
         self.mc = equation.init(props)
         order = int(props["order"])
-        self.order = order
         self.props = props
         self.case = case
         self.dim = int(self.mc.dim)
+        types, orders = getattr(case, "types", None), getattr(case, "orders", None)
+        # elementType.txt / order.txt: per-element types and orders (FlowProMain.java:520-560)
+        self.mixed = self.dim == 2 and (types is not None or orders is not None)
+        tp_in = np.asarray(case.TP)
+        if self.mixed:
+            if orders is not None:
+                order = int(np.max(orders))   # Parameters.order: the largest element order (FlowProMain.java:545)
+            ty = (np.full(case.n_elem, 4 if tp_in.shape[1] == 4 else 3, np.int32) if types is None
+                  else np.where(np.asarray(types) == 4, 4, 3).astype(np.int32))
+            od = (np.full(case.n_elem, order, np.int32) if orders is None
+                  else np.ascontiguousarray(orders, np.int32))
+            nbs = np.where(ty == 4, od * od, od * (od + 1) // 2)
+            if tp_in.shape[1] == 3:
+                tp_in = np.concatenate([tp_in, -np.ones((tp_in.shape[0], 1), tp_in.dtype)], 1)
+        self.order = order
         if self.dim == 3:
             if self.mc.attack is not None:
                 raise UnsupportedCase("attack angle on tetrahedra")
@@ -56,8 +70,8 @@ class DeviceCase:
         else:
             vxi, vw = quadrature.triangle_rule(order)
             fxi, fw = quadrature.line_rule(order)
-        self.quad = self.dim == 2 and case.TP.shape[1] == 4
-        self._keep = [np.ascontiguousarray(case.PXY, np.float64), np.ascontiguousarray(case.TP, np.int32),
+        self.quad = self.dim == 2 and case.TP.shape[1] == 4 and not self.mixed
+        self._keep = [np.ascontiguousarray(case.PXY, np.float64), np.ascontiguousarray(tp_in, np.int32),
                       np.ascontiguousarray(case.bnd, np.int32), np.ascontiguousarray(vxi), np.ascontiguousarray(vw),
                       np.ascontiguousarray(fxi), np.ascontiguousarray(fw)]
         pxy, tp, bnd, vxi, vw, fxi, fw = self._keep
@@ -90,6 +104,11 @@ class DeviceCase:
         cs.positivity_limiter = int(bool(props.get("positivityLimiter", True)))
         cs.dimension = self.dim
         cs.elem_type = 4 if self.quad else 3
+        if self.mixed:
+            self._types = ty
+            self._keep += [ty, od, quadrature.RULES_DIR.encode()]
+            cs.elem_types, cs.elem_orders = native.iptr(ty), native.iptr(od)
+            cs.gauss_rules_dir = self._keep[-1]
         # DGpure.initMethod: dampTol and dampConst are required keys of the DG methods, the inner
         # damping keys optional
         cs.damp_tol = float(props.get("dampTol", 0.0))
@@ -126,6 +145,14 @@ class DeviceCase:
         self.B = self.nb * self.neq
         self.n_elem = case.n_elem
         self.precond_bits = precond_bits
+        if self.mixed:
+            # the reference's layout: element after element, each [m][nb_e]
+            self.sizes = nbs * self.neq
+            self.off = np.concatenate([[0], np.cumsum(self.sizes)])
+            self.elem_nb = nbs
+
+    def _shape(self):
+        return (self.dofs,) if self.mixed else (self.n_elem, self.B)
 
     def close(self):
         if getattr(self, "h", None):
@@ -143,13 +170,13 @@ class DeviceCase:
 
     # ---- state
     def set_state(self, W, Wo=None, Wo2=None):
-        arrs = [None if a is None else np.ascontiguousarray(a, np.float64).reshape(self.n_elem, self.B)
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64).reshape(self._shape())
                 for a in (W, Wo, Wo2)]
         ptr = [native.dptr(a) if a is not None else None for a in arrs]
         self._check(native.lib().fp_set_state(self.h, *ptr), "fp_set_state")
 
     def get_state(self) -> np.ndarray:
-        out = np.zeros((self.n_elem, self.B))
+        out = np.zeros(self._shape())
         self._check(native.lib().fp_get_state(self.h, native.dptr(out)), "fp_get_state")
         return out
 
@@ -210,7 +237,7 @@ class DeviceCase:
 
     def adjoint_solve(self, rhs, a1: float = 0.0):
         """(a1 M - dR/dW)^T lambda = rhs with GMRES on the matrix-free J^T v."""
-        rhs = np.ascontiguousarray(rhs, np.float64).reshape(self.n_elem, self.B)
+        rhs = np.ascontiguousarray(rhs, np.float64).reshape(self._shape())
         lam = np.zeros_like(rhs)
         it, err = C.c_int(), C.c_double()
         rc = self._check(native.lib().fp_adjoint_solve(self.h, a1, native.dptr(rhs), native.dptr(lam), C.byref(it),
@@ -224,16 +251,38 @@ class DeviceCase:
 
     # ---- exports
     def residual(self):
-        R = np.zeros((self.n_elem, self.B))
-        RHS = np.zeros((self.n_elem, self.B))
+        R = np.zeros(self._shape())
+        RHS = np.zeros(self._shape())
         self._check(native.lib().fp_export_residual(self.h, native.dptr(R), native.dptr(RHS)), "fp_export_residual")
         return R, RHS
 
     def jacobian(self):
+        if self.mixed:
+            return self._jacobian_flat()
         d = np.zeros((self.n_elem, self.B, self.B))
         n = np.zeros((self.n_elem, self.case.TP.shape[1] if self.dim == 2 else 4, self.B, self.B))
         self._check(native.lib().fp_export_jacobian(self.h, native.dptr(d), native.dptr(n)), "fp_export_jacobian")
         return d, n
+
+    def _jacobian_flat(self):
+        """Mixed meshes: diag[e] (B_e x B_e) and neigh[(e, k)] (B_t x B_e, t the face-k neighbour),
+        the layout of the reference's ADiag / ANeighs[k].A."""
+        TT = native.neighbor_matrix_mixed(self._keep[1], self._types, self._keep[2])
+        sz = self.sizes
+        nd = int((sz ** 2).sum())
+        nn = sum(int(sz[t] * sz[e]) for e in range(self.n_elem) for t in TT[e] if t > -1)
+        d, n = np.zeros(nd), np.zeros(max(nn, 1))
+        self._check(native.lib().fp_export_jacobian(self.h, native.dptr(d), native.dptr(n)), "fp_export_jacobian")
+        diag, neigh, od, on = [], {}, 0, 0
+        for e in range(self.n_elem):
+            b = int(sz[e])
+            diag.append(d[od:od + b * b].reshape(b, b))
+            od += b * b
+            for k, t in enumerate(TT[e]):
+                if t > -1:
+                    neigh[(e, k)] = n[on:on + int(sz[t]) * b].reshape(int(sz[t]), b)
+                    on += int(sz[t]) * b
+        return diag, neigh
 
     def solution(self) -> np.ndarray:
         x = np.zeros(self.dofs)
@@ -246,13 +295,13 @@ class DeviceCase:
         return o
 
     def jv(self, a1: float, v):
-        v = np.ascontiguousarray(v, np.float64).reshape(self.n_elem, self.B)
+        v = np.ascontiguousarray(v, np.float64).reshape(self._shape())
         out = np.zeros_like(v)
         self._check(native.lib().fp_jv(self.h, a1, native.dptr(v), native.dptr(out)), "fp_jv")
         return out
 
     def jtv(self, a1: float, v):
-        v = np.ascontiguousarray(v, np.float64).reshape(self.n_elem, self.B)
+        v = np.ascontiguousarray(v, np.float64).reshape(self._shape())
         out = np.zeros_like(v)
         self._check(native.lib().fp_jtv(self.h, a1, native.dptr(v), native.dptr(out)), "fp_jtv")
         return out
@@ -264,6 +313,8 @@ class DeviceCase:
 
     def initial_state(self) -> np.ndarray:
         w0 = equation.const_init_condition(self.mc)
+        if self.mixed:
+            return np.concatenate([np.repeat(np.asarray(w0), nb) for nb in self.elem_nb])
         return np.tile(np.repeat(np.asarray(w0), self.nb), (self.n_elem, 1))
 
 

@@ -95,6 +95,16 @@ typedef struct fp_case {
      * (basis2Dsquare, transformation2Dsquare); quadrilaterals carry the NavierStokes and
      * AdiabaticEulers models at orders 1-4, without artificial viscosity */
     int elem_type;
+    /* meshes mixing triangles and squares (elementType.txt) or element orders (order.txt): when
+     * elem_types or elem_orders is not NULL, tp holds 4 vertices per element (the fourth unused on a
+     * triangle), every element takes its own basis, volume rule and map, faces take the line rule of
+     * the larger order of their two sides, and the rules are read from the reference's gaussRules
+     * folder gauss_rules_dir (gauss/<k>.txt, gaussTri/<d>.txt).  Solution vectors then use the
+     * reference's layout: element after element, each [m][nb_e].  Models NavierStokes and
+     * AdiabaticEulers, orders 1-4, no artificial viscosity; vol_* and face_* are unused */
+    const int* elem_types;        /* [n_elems]: 3 or 4, or NULL (all elem_type) */
+    const int* elem_orders;       /* [n_elems]: 1..4, or NULL (all order) */
+    const char* gauss_rules_dir;
 } fp_case;
 
 /* --- integer setup (bit-exact with the reference) ------------------------------ */
@@ -109,6 +119,9 @@ int fp_rcm_order_tet(int n_elems, const int* tt, int* order);
 /* the same two for quadrilaterals (squareElement.getFaceIndexes(k) = {k, (k + 1) % 4}):
  * tp[n_elems][4], bnd[n_bnd][3], tt[n_elems][4] */
 int fp_neighbor_matrix_quad(int n_elems, const int* tp, int n_bnd, const int* bnd, int* tt);
+/* the same on a mesh of triangles and squares (elementType.txt): types[e] 3 or 4, tp[n_elems][4]
+ * (the fourth vertex unused on a triangle), tt[n_elems][4] with -1000 after a triangle's 3 faces */
+int fp_neighbor_matrix_mixed(int n_elems, const int* tp, const int* types, int n_bnd, const int* bnd, int* tt);
 int fp_rcm_order_quad(int n_elems, const int* tt, int* order);
 /* FlowProMain.generateWallDistanceFunction / generateDistanceFunction (FlowProMain.java:931-1014):
  * distance of every vertex from the wall faces (tt == -1) along mesh edges; dist[n_points] */
@@ -119,7 +132,7 @@ int fp_wall_distance(int n_points, const double* pxy, int n_elems, const int* tp
 int fp_create(const fp_case* c, fp_ctx** out);
 int fp_destroy(fp_ctx* ctx);
 const char* fp_last_error(const fp_ctx* ctx);
-/* info: [0] Nb, [1] nEqs, [2] dofs, [3] forward ILU levels, [4] backward ILU levels, [5] block nonzeros */
+/* info: [0] Nb (mixed meshes: the padded size), [1] nEqs, [2] dofs (mixed: the reference's count), [3] forward ILU levels, [4] backward ILU levels, [5] block nonzeros */
 int fp_info(const fp_ctx* ctx, long long* info);
 /* the element order of the DOF numbering: order[r] = mesh element at position r */
 int fp_element_order(const fp_ctx* ctx, int* order);
@@ -155,13 +168,16 @@ int fp_update(fp_ctx* ctx, int* n_limited);
 int fp_export_residual(fp_ctx* ctx, double* R, double* RHS);
 /* blocks in the reference's transposed layout: diag[e][i][j] = ADiag, neigh[e][k][i][j] = ANeighs[k].A
  * (k over the element's faces: 3 on triangles, 4 on tetrahedra)
- * (zero for boundary faces) */
+ * (zero for boundary faces).  Mixed types or orders: flat, mesh order, diag = the B_e x B_e blocks
+ * one after the other, neigh = for every element and face with a neighbour t the B_t x B_e block
+ * (B_e = nb_e nEqs) */
 int fp_export_jacobian(fp_ctx* ctx, double* diag, double* neigh);
 /* eps[n_elems] and damp_inner[n_elems][nEqs] of the last fp_limiter call, mesh order */
 int fp_export_artificial_viscosity(fp_ctx* ctx, double* eps, double* damp_inner);
 /* element means of the state, avg[n_elems][nEqs] in mesh order (Mesh.getAvgW: the W.txt content) */
 int fp_export_averages(fp_ctx* ctx, double* avg);
-/* the last Newton correction x in DOF numbering */
+/* the last Newton correction x in DOF numbering (mixed meshes: the reference's gIndex numbering,
+ * elements in the numbering's order, each [m][nb_e]) */
 int fp_export_solution(fp_ctx* ctx, double* x);
 /* matrix-free (a1 M - dR/dW) v and its transpose, element layout in and out */
 int fp_jv(fp_ctx* ctx, double a1, const double* v, double* out);

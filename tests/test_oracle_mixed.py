@@ -129,3 +129,22 @@ def test_gmres_with_rectangular_blocks_matches_dense_solve(prec):
     r, _ = o.rhs()
     xd = np.linalg.solve(A, o.to_dof(r))
     assert ok and np.abs(x - xd).max() / np.abs(xd).max() < 1e-8
+
+
+def test_product_mixed_integer_setup_bit_exact():
+    """fp_neighbor_matrix_mixed and fp_rcm_order_quad (the face graph of 4-slot rows) equal the
+    oracle's createNeighborMatrix and reverseCuthillMcKee on triangles and squares together, also
+    with a repeated boundary row of another type (the later row wins) and with missing rows (-1)."""
+    from flowpro_v2_amd import native
+    for seed in (7, 8):
+        c = cases.mixed_channel(9, 4, jitter=0.2, seed=seed)
+        o = O.Oracle(c, cases.euler_props(order=2))
+        TT = native.neighbor_matrix_mixed(c.TP, c.types, c.bnd)
+        assert np.array_equal(TT, o.TT())
+        assert (TT[c.types == 3, 3] == -1000).all()
+        assert np.array_equal(native.rcm_order_quad(TT), o.order())
+        extra = c.bnd[:5].copy()
+        extra[:, 0] = -4
+        for bnd in (np.concatenate([c.bnd, extra]), c.bnd[6:]):
+            c2 = cases.Case("b", c.PXY, c.TP, bnd, types=c.types)
+            assert np.array_equal(native.neighbor_matrix_mixed(c.TP, c.types, bnd), O.Oracle(c2, cases.euler_props(order=2)).TT())
