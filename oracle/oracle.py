@@ -187,6 +187,8 @@ class Oracle:
         orders = getattr(case, "orders", None)
         self.mixed = types is not None or orders is not None
         self.P.elem_type = 4 if (TP.shape[1] == 4 and types is None) else 3
+        if self.mixed and TP.shape[1] == 3:   # per-element orders on triangles: rows of 4 (the fourth unused)
+            TP = np.ascontiguousarray(np.concatenate([TP, -np.ones((TP.shape[0], 1), np.int32)], 1))
         ty = None if types is None else np.ascontiguousarray(types, dtype=np.int32)
         od = None if orders is None else np.ascontiguousarray(orders, dtype=np.int32)
         if od is not None:

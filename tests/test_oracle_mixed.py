@@ -148,3 +148,24 @@ def test_product_mixed_integer_setup_bit_exact():
         for bnd in (np.concatenate([c.bnd, extra]), c.bnd[6:]):
             c2 = cases.Case("b", c.PXY, c.TP, bnd, types=c.types)
             assert np.array_equal(native.neighbor_matrix_mixed(c.TP, c.types, bnd), O.Oracle(c2, cases.euler_props(order=2)).TT())
+
+
+def test_triangle_orders_with_one_order_equal_uniform():
+    """Per-element orders on triangles only (order.txt without elementType.txt), all equal: the
+    residual and Jacobian blocks of the uniform triangle build, bit for bit."""
+    u = cases.channel_bump(6, 3, jitter=0.2)
+    m = cases.Case("m", u.PXY, u.TP, u.bnd, orders=np.full(u.n_elem, 3, np.int32))
+    props = cases.ns_props(order=3, reynolds=200.0, jacobian="exact")
+    ou, om = O.Oracle(u, props), O.Oracle(m, props)
+    assert om.mixed and (om.sizes == ou.B).all()
+    W = cases.perturbed_state(ou.get_w()[0][:: ou.nb], u.n_elem, ou.nb, 0.02, 4)
+    ou.set_w(W)
+    om.set_w(W.ravel())
+    assert np.array_equal(ou.residual().ravel(), om.residual())
+    for o in (ou, om):
+        o.limiter()
+        o.assemble(0.1, -1.0)
+    d, n = ou.blocks()
+    dm, nm = om.blocks_flat()
+    assert all(np.array_equal(d[e], dm[e]) for e in range(u.n_elem))
+    assert all(np.array_equal(n[e, k], blk) for (e, k), blk in nm.items())
