@@ -58,8 +58,11 @@ template <typename TF> hipError_t launch_ilu_copy(hipStream_t, size_t, const dou
 template <typename TF>
 hipError_t launch_ilu_factor_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, int*);
 template <typename TF>
-hipError_t launch_ilu_factor_rows(hipStream_t, int, const int*, int, const int*, const int*, const int*, const int*, TF*, int*);
+hipError_t launch_ilu_factor_rows(hipStream_t, int, int, const int*, int, const int*, const int*, const int*, const int*, TF*, int*);
 bool ilu_factor_rows_supported(int, int);
+int ilu_factor_rows_variants();
+int ilu_factor_rows_default_variant(int B);
+void ilu_factor_rows_shape(int variant, int* nw, int* ngw);
 template <typename TF>
 hipError_t launch_ilu_fwd_level(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);
 template <typename TF>
@@ -137,6 +140,8 @@ struct fp_ctx {
     int *ftptr = nullptr, *ftask = nullptr;   // per-row task lists of the ILU0 factor
     int *meta_f = nullptr, *meta_b = nullptr; // per-row records of the two sweeps, in level order
     bool ilu_rows = false;                    // factor with the register-resident row kernel
+    int ilu_variant = -1;                     // launch shape of the row kernel (-1: not tuned yet)
+    std::vector<double> tune_ms;              // the tuner's time of every shape (ms per factorisation)
     bool f_fresh = false;                     // F holds the FP32 copy of the current matrix (written by the Jacobian)
     double* val = nullptr;        // FP64 matrix (nnzb * B * B)
     void* F = nullptr;            // preconditioner factors (FP64 or FP32)
@@ -1975,6 +1980,17 @@ extern "C" int fp_element_order(const fp_ctx* c, int* order) {
 
 extern "C" void* fp_stream(fp_ctx* c) { return c ? (void*)c->s : nullptr; }
 
+extern "C" int fp_autotune_report(const fp_ctx* c, int* variant, int* shapes, double* ms, int n) {
+    if (!c || n < 0) return FP_EINVAL;
+    const int nv = ilu_factor_rows_variants();
+    if (variant) *variant = c->ilu_rows ? c->ilu_variant : -1;
+    for (int v = 0; v < std::min(n, nv); v++) {
+        if (shapes) ilu_factor_rows_shape(v, &shapes[2 * v], &shapes[2 * v + 1]);
+        if (ms) ms[v] = v < (int)c->tune_ms.size() ? c->tune_ms[v] : 0.0;
+    }
+    return nv;
+}
+
 // ============================================================== state
 // element layout, mesh order <-> device (DOF order)
 // mixed meshes: the reference's layout holds element e at m_off[e] as [m][nb_e]; the device pads
@@ -2112,8 +2128,8 @@ template <typename TF> static int ilu_level_launches(fp_ctx* c) {
     for (size_t l = 0; l + 1 < c->lvl_f_ptr.size(); l++) {
         int b = c->lvl_f_ptr[l], e = c->lvl_f_ptr[l + 1];
         if (c->ilu_rows)
-            HIPCHK(c, launch_ilu_factor_rows<TF>(c->s, c->B, c->lvl_f + b, e - b, c->rowptr, c->diagpos, c->ftptr, c->ftask, F,
-                                                 c->istat));
+            HIPCHK(c, launch_ilu_factor_rows<TF>(c->s, c->B, c->ilu_variant >= 0 ? c->ilu_variant : ilu_factor_rows_default_variant(c->B),
+                                                 c->lvl_f + b, e - b, c->rowptr, c->diagpos, c->ftptr, c->ftask, F, c->istat));
         else
             HIPCHK(c, launch_ilu_factor_level<TF>(c->s, c->B, c->lvl_f + b, e - b, c->rowptr, c->colidx, c->diagpos, F,
                                                   c->istat));
@@ -2149,7 +2165,55 @@ template <typename TF> static int run_captured(fp_ctx* c, hipGraphExec_t* exec, 
     HIPCHK(c, hipGraphLaunch(*exec, c->s));
     return FP_OK;
 }
+// Autotuner of the row kernel's launch shape (task waves, inversion waves).  On the first
+// factorisation of a context every shape factors the case's own assembled matrix twice, each
+// time from a fresh copy of the values, and the second run is timed with HIP events on the
+// library's stream; the fastest shape is kept for the context and captured into the factor
+// graph.  The shapes give bit-identical factors, so the choice changes time only.
+// FLOWPRO_AUTOTUNE=0 keeps the default shape.
+template <typename TF> static int ilu_autotune(fp_ctx* c) {
+    const int nv = ilu_factor_rows_variants();
+    c->tune_ms.assign(nv, 0.0);
+    c->ilu_variant = ilu_factor_rows_default_variant(c->B);
+    const char* g = std::getenv("FLOWPRO_AUTOTUNE");
+    if (!c->ilu_rows || (g && g[0] == '0')) return FP_OK;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    int best = c->ilu_variant, rc = FP_OK;
+    double best_ms = std::numeric_limits<double>::infinity();
+    for (int v = 0; v < nv && rc == FP_OK; v++) {
+        c->ilu_variant = v;
+        for (int rep = 0; rep < 2 && rc == FP_OK; rep++) {
+            if ((rc = ilu_copy_launch<TF>(c))) break;
+            if (hipEventRecord(e0, c->s) != hipSuccess) { rc = FP_EHIP; break; }
+            if ((rc = ilu_level_launches<TF>(c))) break;
+            float ms = 0;
+            if (hipEventRecord(e1, c->s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = FP_EHIP;
+                break;
+            }
+            c->tune_ms[v] = ms;
+        }
+        if (rc == FP_OK && c->tune_ms[v] < best_ms) {
+            best_ms = c->tune_ms[v];
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->ilu_variant = best;
+    c->f_fresh = false;   // F holds a factor now: the factorisation starts from a fresh copy
+    if (rc != FP_OK && c->err.empty()) c->err = "ILU autotuning failed";
+    return rc;
+}
+
 template <typename TF> static int ilu_factor_t(fp_ctx* c) {
+    if (c->ilu_variant < 0) {
+        int rc = ilu_autotune<TF>(c);
+        if (rc) return rc;
+    }
     HIPCHK(c, hipMemsetAsync(c->istat, 0, sizeof(int), c->s));
     // the factorisation overwrites F: it starts from a copy of the assembled values, which the
     // Jacobian kernel has already written in FP32 after an assembly

@@ -538,12 +538,11 @@ __device__ inline double pivot_reciprocal(double x) { return 1.0 / x; }
 // block of each task (D_k^-1 or U_kj) is staged through a double-buffered LDS tile, with the
 // next task's block already in flight in registers while the current one is used.  The rows
 // of A_ik and of L_ik are exchanged between the waves through LDS.  The diagonal block is
-// then inverted by four of the waves (Gauss-Jordan with implicit partial pivoting; pivot row
+// then inverted by NGW of the waves (Gauss-Jordan with implicit partial pivoting; pivot row
 // = largest |M[.][p]| via a DPP wave maximum on |x| with the lane index in the low bits).
-#ifndef ILU_INV_WAVES
-#define ILU_INV_WAVES 4
-#endif
-template <typename TF, int B, int MAXR, int NW>
+// Every (NW, NGW) computes every entry with the same operations in the same order, so the
+// variants give bit-identical factors and differ in time only.
+template <typename TF, int B, int MAXR, int NW, int NGW>
 __global__ void __launch_bounds__(64 * NW) k_ilu_factor_rows(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
                                                          const int* __restrict__ diagpos, const int* __restrict__ tptr,
                                                          const int2* __restrict__ tasks, TF* __restrict__ F,
@@ -551,7 +550,7 @@ __global__ void __launch_bounds__(64 * NW) k_ilu_factor_rows(const int* __restri
     constexpr int BB = B * B;
     constexpr int NT = 64 * NW;
     constexpr int CW = (B + NW - 1) / NW;       // columns per wave in the tasks
-    constexpr int NG = ILU_INV_WAVES < NW ? ILU_INV_WAVES : NW;   // waves of the inversion
+    constexpr int NG = NGW < NW ? NGW : NW;     // waves of the inversion
     constexpr int CG = (B + NG - 1) / NG;       // identity columns per inversion wave
     constexpr int PER = (BB + NT - 1) / NT;
     __shared__ TF S[2][BB];
@@ -1042,30 +1041,46 @@ hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nr
     return hipErrorInvalidValue;
 }
 
-// (block size, row blocks held in registers, task waves): the triangles' rows have at most 4
-// blocks, the tetrahedra's (B = 20 at P1) 5
-#define FP_ROWREG_SIZES(X) X(20, 5, 4) X(24, 4, 8) X(30, 4, 8) X(40, 4, 8)
-#ifndef ILU_ROW_WAVES
-#define ILU_ROW_WAVES 8
-#endif
+// (block size, row blocks held in registers): the triangles' rows have at most 4 blocks, the
+// tetrahedra's (B = 20 at P1) 5
+#define FP_ROWREG_SIZES(X) X(20, 5) X(24, 4) X(30, 4) X(40, 4)
+// launch shapes of the row kernel, picked per context by timing them on the case's own matrix:
+// (task waves NW, inversion waves NGW)
+#define FP_ROWREG_VARIANTS(Y, BS, MR) Y(BS, MR, 0, 4, 4) Y(BS, MR, 1, 8, 4) Y(BS, MR, 2, 8, 2) Y(BS, MR, 3, 8, 8) Y(BS, MR, 4, 4, 2)
+constexpr int ILU_ROW_VARIANTS = 5;
+
+int ilu_factor_rows_variants() { return ILU_ROW_VARIANTS; }
+// the shape used without tuning: 4 task waves for B = 20 (5 columns each), 8 above
+int ilu_factor_rows_default_variant(int B) { return B == 20 ? 0 : 1; }
+void ilu_factor_rows_shape(int variant, int* nw, int* ngw) {
+    static const int T[ILU_ROW_VARIANTS][2] = {{4, 4}, {8, 4}, {8, 2}, {8, 8}, {4, 2}};
+    const int v = variant >= 0 && variant < ILU_ROW_VARIANTS ? variant : 0;
+    *nw = T[v][0];
+    *ngw = T[v][1];
+}
 
 template <typename TF>
-hipError_t launch_ilu_factor_rows(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* diagpos,
-                                  const int* tptr, const int* tasks, TF* F, int* status) {
-#define X(BS, MR, NWV)                                                                                                 \
-    if (B == BS) {                                                                                                     \
-        constexpr int NW_ = NWV == 8 ? ILU_ROW_WAVES : NWV;                                                            \
-        hipLaunchKernelGGL((k_ilu_factor_rows<TF, BS, MR, NW_>), dim3(nrows), dim3(64 * NW_), 0, s, rows, nrows, rowptr, \
-                           diagpos, tptr, reinterpret_cast<const int2*>(tasks), F, status);                            \
-        return hipGetLastError();                                                                                      \
+hipError_t launch_ilu_factor_rows(hipStream_t s, int B, int variant, const int* rows, int nrows, const int* rowptr,
+                                  const int* diagpos, const int* tptr, const int* tasks, TF* F, int* status) {
+#define Y(BS, MR, V, NW_, NG_)                                                                                          \
+    if (variant == V) {                                                                                                 \
+        hipLaunchKernelGGL((k_ilu_factor_rows<TF, BS, MR, NW_, NG_>), dim3(nrows), dim3(64 * NW_), 0, s, rows, nrows,   \
+                           rowptr, diagpos, tptr, reinterpret_cast<const int2*>(tasks), F, status);                     \
+        return hipGetLastError();                                                                                       \
+    }
+#define X(BS, MR)                                  \
+    if (B == BS) {                                 \
+        FP_ROWREG_VARIANTS(Y, BS, MR)              \
+        return hipErrorInvalidValue;               \
     }
     FP_ROWREG_SIZES(X)
 #undef X
+#undef Y
     return hipErrorInvalidValue;
 }
 
 bool ilu_factor_rows_supported(int B, int max_row_blocks) {
-#define X(BS, MR, NWV) if (B == BS) return max_row_blocks <= MR;
+#define X(BS, MR) if (B == BS) return max_row_blocks <= MR;
     FP_ROWREG_SIZES(X)
 #undef X
     return false;
@@ -1108,7 +1123,7 @@ hipError_t launch_set_io(hipStream_t s, IluIO* io, const double* b, double* x) {
     template hipError_t launch_ilu_copy<TF>(hipStream_t, size_t, const double*, TF*);                                          \
     template hipError_t launch_ilu_factor_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, \
                                                     int*);                                                                     \
-    template hipError_t launch_ilu_factor_rows<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*,      \
+    template hipError_t launch_ilu_factor_rows<TF>(hipStream_t, int, int, const int*, int, const int*, const int*, const int*, \
                                                    const int*, TF*, int*);                                                     \
     template hipError_t launch_ilu_fwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);             \
     template hipError_t launch_ilu_bwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*);

@@ -25,7 +25,7 @@ EXPORTS = ["fp_neighbor_matrix", "fp_rcm_order", "fp_create", "fp_destroy", "fp_
            "fp_export_solution", "fp_jv", "fp_jtv", "fp_time_kernel", "fp_stream", "fp_linear_solve_jfnk",
            "fp_functional", "fp_adjoint_solve", "fp_wall_distance", "fp_neighbor_matrix_tet", "fp_rcm_order_tet",
            "fp_limiter", "fp_export_artificial_viscosity", "fp_export_averages", "fp_neighbor_matrix_quad",
-           "fp_rcm_order_quad", "fp_neighbor_matrix_mixed"]
+           "fp_rcm_order_quad", "fp_neighbor_matrix_mixed", "fp_autotune_report"]
 
 
 class FpCase(C.Structure):
@@ -102,6 +102,7 @@ def lib() -> C.CDLL:
         "fp_neighbor_matrix_quad": (C.c_int, [C.c_int, ip, C.c_int, ip, ip]),
         "fp_rcm_order_quad": (C.c_int, [C.c_int, ip, ip]),
         "fp_neighbor_matrix_mixed": (C.c_int, [C.c_int, ip, ip, C.c_int, ip, ip]),
+        "fp_autotune_report": (C.c_int, [vp, ip, ip, dp, C.c_int]),
         "fp_limiter": (C.c_int, [vp, C.c_int]),
         "fp_export_artificial_viscosity": (C.c_int, [vp, dp, dp]),
         "fp_export_averages": (C.c_int, [vp, dp]),

@@ -306,6 +306,16 @@ class DeviceCase:
         self._check(native.lib().fp_jtv(self.h, a1, native.dptr(v), native.dptr(out)), "fp_jtv")
         return out
 
+    def autotune_report(self):
+        """(chosen shape index or -1, [(task waves, inversion waves)], [ms per factorisation]) of the
+        block-ILU0 factor kernel's launch-shape autotuner."""
+        v = C.c_int(-1)
+        shapes = np.zeros((8, 2), np.int32)
+        ms = np.zeros(8)
+        n = self._check(native.lib().fp_autotune_report(self.h, C.byref(v), native.iptr(shapes), native.dptr(ms), 8),
+                        "fp_autotune_report")
+        return v.value, [tuple(int(x) for x in r) for r in shapes[:n]], ms[:n].tolist()
+
     def time_kernel(self, which: int, reps: int = 10) -> float:
         ms = C.c_double()
         self._check(native.lib().fp_time_kernel(self.h, which, reps, C.byref(ms)), "fp_time_kernel")

@@ -201,6 +201,14 @@ int fp_adjoint_solve(fp_ctx* ctx, double a1, const double* rhs, double* lambda, 
 int fp_time_kernel(fp_ctx* ctx, int which, int reps, double* avg_ms);
 /* hipStream_t of the context (for callers that record their own events) */
 void* fp_stream(fp_ctx* ctx);
+/* the launch-shape autotuner of the block-ILU0 factor kernel (no reference counterpart: the
+ * reference has one CPU loop): on the first factorisation of a context every shape
+ * (shapes[v] = {task waves, inversion waves}) factors the case's own matrix, timed with HIP
+ * events, and the fastest is kept; the shapes give bit-identical factors.  variant = the chosen
+ * shape (-1 before the first factorisation or when the row kernel does not serve this block
+ * size), ms[v] = its time per factorisation (0 when FLOWPRO_AUTOTUNE=0).  Fills min(n, count)
+ * entries and returns the number of shapes */
+int fp_autotune_report(const fp_ctx* ctx, int* variant, int* shapes, double* ms, int n);
 
 #ifdef __cplusplus
 }
