@@ -77,6 +77,16 @@ def run_solver(case, props, bits, warmup, steps, state=None):
                      failures=s.linear_failures)
 
 
+def autotune_summary(dev):
+    """The factor kernel's launch shape picked by the library's autotuner, with every shape's time
+    per factorisation on this case's matrix."""
+    v, shapes, ms = dev.autotune_report()
+    if v < 0:
+        return None
+    return {"kernel": "k_ilu_factor_rows", "chosen": {"task_waves": shapes[v][0], "inversion_waves": shapes[v][1]},
+            "factor_ms": {f"{nw}x{ng}": round(t, 3) for (nw, ng), t in zip(shapes, ms)}}
+
+
 def kernel_table(dev, reps=5):
     names = {0: "residual", 1: "jacobian", 2: "spmv", 3: "ilu_factor", 4: "ilu_apply", 5: "jtv",
              6: "ilu_factor_levels"}
@@ -324,6 +334,7 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
         "roofline": roof,
         "other_kernels": others,
+        "autotune": autotune_summary(dev),
     }
     dev.close()
     if tet:
