@@ -256,7 +256,9 @@ def main():
     ap.add_argument("--cfl", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
-    ap.add_argument("--workload", choices=["ns2d_cylinder", "euler3d_tet", "batched_ns"], default="ns2d_cylinder")
+    ap.add_argument("--workload", choices=["ns2d_cylinder", "euler3d_tet", "batched_ns", "mixed_ns"], default="ns2d_cylinder")
+    ap.add_argument("--mx", type=int, default=520, help="channel cells along x of the mixed_ns workload")
+    ap.add_argument("--my", type=int, default=256, help="channel cells along y of the mixed_ns workload")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batched_ns workload")
     ap.add_argument("--cube", type=int, default=64, help="cubes per edge of the euler3d_tet workload")
     args = ap.parse_args()
@@ -269,9 +271,17 @@ def main():
         return
     from flowpro_v2_amd import cases, native
     tet = args.workload == "euler3d_tet"
+    mixed = args.workload == "mixed_ns"
     state = None
     TT = None
-    if tet:
+    if mixed:
+        # squares and triangles (elementType.txt) with per-element orders 2-4 (order.txt)
+        case = cases.mixed_channel(args.mx, args.my, jitter=0.2, seed=7)
+        case.orders = cases.seeded_orders(case.n_elem, (2, 3, 4), 3)
+        props = cases.ns_props(order=4, CFL=args.cfl)
+        TT = native.neighbor_matrix_mixed(case.TP, case.types, case.bnd)
+        log(f"mesh {case.name}: {case.n_elem} elements ({int((case.types == 4).sum())} squares), orders 2-4")
+    elif tet:
         case = cases.kuhn_cube(args.cube, 0.05, 4)
         props = cases.euler3d_props(mach=0.8, order=2, CFL=args.cfl)
         TT = native.neighbor_matrix_tet(case.TP, case.bnd)
@@ -301,11 +311,16 @@ def main():
     kt = devs["kt"]
     main_bits = 32 if 32 in out else 64
     r = out[main_bits]
-    default_mesh = not tet and (args.n_theta, args.n_r) == (400, 250) and main_bits == 32
+    default_mesh = not tet and not mixed and (args.n_theta, args.n_r) == (400, 250) and main_bits == 32
     roof = roofline_factor(case, dev, kt["ilu_factor_levels"], FACTOR_TRAFFIC_400x250 if default_mesh else None, TT)
     others = {"k_jacobian_team": roofline_jacobian(dev, kt["jacobian"]), "k_bsr_spmv": roofline_spmv(dev, kt["spmv"])}
     if tet:
         others = {"k_jacobian_tet_team": dict(roofline_jacobian(dev, kt["jacobian"]), kernel="k_jacobian_tet_team"),
+                  "k_bsr_spmv": roofline_spmv(dev, kt["spmv"])}
+    if mixed:
+        # blocks of the padded size B = 16 nEqs; the factor runs the per-level block kernels at B = 64
+        roof["kernel"] = "k_ilu_factor_level"
+        others = {"k_jacobian_mixed": dict(roofline_jacobian(dev, kt["jacobian"]), kernel="k_jacobian_mixed"),
                   "k_bsr_spmv": roofline_spmv(dev, kt["spmv"])}
     result = {
         "metric": "Newton-GMRES DOF-updates/s, mixed vs FP64 GPU at equal residual; % HBM peak",
@@ -320,7 +335,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 residual/Jacobian/GMRES, f32 preconditioner" if main_bits == 32 else "f64",
         "data": "synthetic (freestream state, seeded mesh)",
-        "config": ({"workload": f"Euler3D DG P1 (order 2) Kuhn cube {args.cube}^3 x 6 = {case.n_elem} tetrahedra, "
+        "config": ({"workload": f"NS2D DG Re=5000 M=0.5 channel with a bump, {case.n_elem} elements mixing squares and "
+                                f"triangles, per-element orders 2-4 (padded to 16 basis functions), {dev.dofs} DOFs, "
+                                f"BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) GMRES(30) tol 1e-2",
+                    "elements": case.n_elem, "dofs": dev.dofs, "order": "2-4 per element", "precond_bits": main_bits}
+                   if mixed else
+                   {"workload": f"Euler3D DG P1 (order 2) Kuhn cube {args.cube}^3 x 6 = {case.n_elem} tetrahedra, "
                                 f"{dev.dofs} DOFs, M=0.8 density and pressure pulse, BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) "
                                 "GMRES(30) tol 1e-2",
                     "elements": case.n_elem, "dofs": dev.dofs, "order": 2, "precond_bits": main_bits} if tet else
@@ -340,6 +360,9 @@ def main():
     if tet:
         result["data"] = "synthetic (freestream with a Gaussian density and pressure pulse, seeded jittered mesh)"
         result["cpu_baseline"] = None   # the CPU restatement of the tetrahedra covers residual and Jacobian only
+    elif mixed:
+        result["data"] = "synthetic (freestream state, seeded mesh, element types and orders)"
+        result["cpu_baseline"] = None
     elif not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")

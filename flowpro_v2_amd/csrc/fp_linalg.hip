@@ -795,10 +795,11 @@ __global__ void __launch_bounds__(64) k_ilu_fwd_level(const int* __restrict__ me
     y[(size_t)i * B + t] = s;
 }
 
-// backward: x_i = D_i^-1 (y_i - sum_{j>i} U_ij x_j).  FP32: the row of D_i^-1 is loaded together
-// with the U blocks, before the exchange of the right-hand side through LDS.  FP64: the 80 registers
-// of that row held across the U products cost 3.4x at B = 40 (32 -> 9 us per level), so the row is
-// loaded after the exchange.
+// backward: x_i = D_i^-1 (y_i - sum_{j>i} U_ij x_j).  FP32 up to B = 40: the row of D_i^-1 is loaded
+// together with the U blocks, before the exchange of the right-hand side through LDS.  FP64, and FP32
+// at B = 64: the registers of that row held across the U products cost 3.4x at B = 40 in FP64
+// (32 -> 9 us per level) and 10x at B = 64 in FP32 (68 us against 6 us for the forward level), so the
+// row is loaded after the exchange.
 template <typename TF, int B>
 __global__ void __launch_bounds__(64) k_ilu_bwd_level(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
                                                       const TF* __restrict__ y, TF* __restrict__ xs,
@@ -809,7 +810,7 @@ __global__ void __launch_bounds__(64) k_ilu_bwd_level(const int* __restrict__ me
     const int slot = threadIdx.x / B, t = threadIdx.x % B;
     const int li = RPW == 1 ? (int)blockIdx.x : (int)blockIdx.x * RPW + slot;   // uniform record index as above
     const bool active = slot < RPW && li < nrows;
-    constexpr bool early_d = sizeof(TF) == 4;
+    constexpr bool early_d = sizeof(TF) == 4 && B <= 40;
     int i = 0;
     TF drow[B];
     const TF* dp = nullptr;
@@ -1042,8 +1043,8 @@ hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nr
 }
 
 // (block size, row blocks held in registers): the triangles' rows have at most 4 blocks, the
-// tetrahedra's (B = 20 at P1) 5
-#define FP_ROWREG_SIZES(X) X(20, 5) X(24, 4) X(30, 4) X(40, 4)
+// tetrahedra's (B = 20 at P1) 5, those of meshes with squares (B = 64: order 4, or padded) 5
+#define FP_ROWREG_SIZES(X) X(20, 5) X(24, 4) X(30, 4) X(40, 4) X(64, 5)
 // launch shapes of the row kernel, picked per context by timing them on the case's own matrix:
 // (task waves NW, inversion waves NGW)
 #define FP_ROWREG_VARIANTS(Y, BS, MR) Y(BS, MR, 0, 4, 4) Y(BS, MR, 1, 8, 4) Y(BS, MR, 2, 8, 2) Y(BS, MR, 3, 8, 8) Y(BS, MR, 4, 4, 2)

@@ -262,14 +262,42 @@ struct MColumnSink : MTangentSink<NB, NE> {
     }
 };
 
+// cols != nullptr (assembly): thread t takes the column cols[t] = e B + dir of the list of the
+// elements' own (unpadded) unknowns, ncols of them; the columns of padded unknowns are written by
+// k_mixed_padded_diagonal.  cols == nullptr: thread t takes column t % B of element t / B.
 template <int MK, int NB, bool VISC, typename BT>
 __global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W, double a1,
                                                        BT* __restrict__ blocks, float* __restrict__ fblocks,
-                                                       const double* __restrict__ v, double* __restrict__ out, int mode) {
+                                                       const double* __restrict__ v, double* __restrict__ out, int mode,
+                                                       const int* __restrict__ cols, int ncols) {
     constexpr int NE = neq_of<MK>(), B = NB * NE;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int e = (int)(t / B), dir = (int)(t % B);
+    if (cols && t >= ncols) return;
+    const long long col = cols ? (long long)cols[t] : t;
+    const int e = (int)(col / B), dir = (int)(col % B);
     if (e >= mm.nElem) return;
+    if (dir % NB >= mm.nbe[e]) {
+        // a padded unknown: nothing depends on it, so its column is a1 e_dir in the diagonal
+        // block (the identity mass) and zero in the neighbour blocks
+        if (mode == 0) {
+            const size_t base = (size_t)mm.blkd[e] * B * B + (size_t)dir * B;
+            for (int i = 0; i < B; i++) {
+                blocks[base + i] = (BT)(i == dir ? a1 : 0.0);
+                if (fblocks) fblocks[base + i] = (float)(i == dir ? a1 : 0.0);
+            }
+            for (int k = 0; k < mm.nv[e]; k++)
+                if (mm.nbr[e * 4 + k] > -1) {
+                    const size_t nb = (size_t)mm.blkn[e * 4 + k] * B * B + (size_t)dir * B;
+                    for (int i = 0; i < B; i++) {
+                        blocks[nb + i] = (BT)0;
+                        if (fblocks) fblocks[nb + i] = 0.0f;
+                    }
+                }
+        } else {
+            out[(size_t)e * B + dir] = a1 * v[(size_t)e * B + dir];
+        }
+        return;
+    }
     MColumnSink<NB, NE, BT> sk;
 #pragma unroll
     for (int i = 0; i < B; i++) sk.K[i] = 0.0;
@@ -304,6 +332,21 @@ __global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G
         }
         out[(size_t)e * B + dir] = s;
     }
+}
+
+// The padded unknowns' columns of the matrix: zero except a1 on the diagonal (the identity mass).
+// The zeros are written once (the matrix and the FP32 factor storage are cleared at set-up, and
+// neither the Jacobian kernel nor the factorisation writes a non-zero into these columns), so each
+// assembly rewrites only the diagonal entry, in the FP64 matrix and in the FP32 copy the
+// factorisation starts from (where the previous factorisation left 1/a1).
+__global__ void k_mixed_padded_diagonal(int n, const int* __restrict__ pdofs, const int* __restrict__ blkd, int B, double a1,
+                                        double* __restrict__ blocks, float* __restrict__ fblocks) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int p = pdofs[t], r = p / B, dir = p % B;
+    const size_t o = (size_t)blkd[r] * B * B + (size_t)dir * B + dir;
+    blocks[o] = a1;
+    if (fblocks) fblocks[o] = (float)a1;
 }
 
 template <int MK, int NB, bool VISC>
@@ -419,14 +462,23 @@ hipError_t launch_residual_mixed(const ElemLaunch& L, hipStream_t s, const Model
     return hipErrorInvalidValue;
 }
 
+// mode 0 (assembly) runs over the own unknowns listed in cols (ncols) and then sets the padded
+// unknowns' diagonal (pdofs, npad); mode 1 (J^T v) runs over every padded column
 hipError_t launch_jacobian_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
-                                 const double* W, double a1, double* blocks, float* fblocks, const double* v, double* out, int mode) {
-    const long long threads = (long long)mm.nElem * L.nb * L.neq;
+                                 const double* W, double a1, double* blocks, float* fblocks, const double* v, double* out, int mode,
+                                 const int* cols, int ncols, const int* pdofs, int npad) {
+    const bool compact = mode == 0 && cols;
+    const long long threads = compact ? (long long)ncols : (long long)mm.nElem * L.nb * L.neq;
     dim3 blk(64), grd((unsigned)((threads + 63) / 64));
+    if (threads == 0) return hipSuccess;
 #define X(MK, NB, V)                                                                                                   \
     if (FP_MMATCH(MK, NB, V)) {                                                                                        \
         hipLaunchKernelGGL((k_jacobian_mixed<MK, NB, V, double>), grd, blk, 0, s, P, G, mm, W, a1, blocks, fblocks, v, out, \
-                           mode);                                                                                      \
+                           mode, compact ? cols : nullptr, compact ? ncols : 0);                                       \
+        hipError_t e_ = hipGetLastError();                                                                             \
+        if (e_ != hipSuccess || !compact || npad == 0) return e_;                                                      \
+        hipLaunchKernelGGL(k_mixed_padded_diagonal, dim3((npad + 255) / 256), dim3(256), 0, s, npad, pdofs, mm.blkd,   \
+                           L.nb * L.neq, a1, blocks, fblocks);                                                         \
         return hipGetLastError();                                                                                      \
     }
     FP_MIXED_CASES(X)
