@@ -334,6 +334,181 @@ __global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G
     }
 }
 
+// Exact Jacobian, one 64-lane workgroup per element (assembly, mode 0 of k_jacobian_mixed).
+// Phase 1: the derivatives of the face integrand g(WL, dWL; WR, dWR) and of the volume fluxes
+// (fx, fy)(W, dW) at every point with respect to the 3 nEqs own inputs (W_m, dW_m/dx, dW_m/dy;
+// nEqs when inviscid), one dual evaluation per (point, input), spread over the lanes and kept
+// in LDS.  Phase 2: lane = own column (m, j); the column's tangent at a point is the chain rule
+// over those inputs (phi_j, dphi_j/dx, dphi_j/dy in equation m), contracted with the own test
+// functions into the diagonal block and with the neighbour's into the neighbour blocks.  The
+// dual pass per (column, point) of k_jacobian_mixed becomes one per (input, point).
+constexpr int MJ_MAXF = 16, MJ_MAXV = 16;   // face and volume points per element (orders <= 4)
+
+template <int MK, int NB, bool VISC>
+__global__ void __launch_bounds__(64) k_jacobian_mixed_elem(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
+                                                            double a1, double* __restrict__ blocks, float* __restrict__ fblocks) {
+    constexpr int NE = neq_of<MK>(), B = NB * NE;
+    constexpr int NIN = VISC ? 3 * NE : NE;
+    __shared__ double JF[MJ_MAXF][NIN][NE];
+    __shared__ double JV[MJ_MAXV][NIN][2 * NE];
+    const int e = blockIdx.x, lane = threadIdx.x;
+    if (e >= mm.nElem) return;
+    const int nv = mm.nv[e], nbe = mm.nbe[e], nqv = nbe > 1 ? mm.nqv[e] : 0;
+    const double cIP = mixed_cip(G, mm, e);
+    int fstart[5];
+    fstart[0] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) fstart[k + 1] = fstart[k] + (k < nv ? mm.nqf[e * 4 + k] : 0);
+    const int nfp = fstart[4];
+    const MPlain own{W, B};
+    // phase 1
+    for (int it = lane; it < (nfp + nqv) * NIN; it += 64) {
+        const int pt = it / NIN, d = it % NIN;
+        if (pt < nfp) {
+            int k = 0;
+            while (pt >= fstart[k + 1]) k++;
+            const int q = pt - fstart[k];
+            const int TT = mm.nbr[e * 4 + k];
+            const double* fg = mm.fgeo + ((size_t)e * 4 + k) * 4;
+            const double n[2] = {fg[0], fg[1]};
+            const double* fp = mm.face + (((size_t)e * 4 + k) * mm.nqfm + q) * (3 + 6 * NB);
+            double wl[NE], dwl[2 * NE];
+            mixed_interp<NB, NE, double>(own, e, fp + 3, fp + 3 + NB, wl, dwl);
+            dual WL[NE], dWL[2 * NE], WR[NE], dWR[2 * NE], g[NE];
+#pragma unroll
+            for (int m = 0; m < NE; m++) WL[m] = dual(wl[m], d == m ? 1.0 : 0.0);
+#pragma unroll
+            for (int m = 0; m < 2 * NE; m++) dWL[m] = dual(dwl[m], d == NE + m ? 1.0 : 0.0);
+            if (TT > -1) {
+                double wr[NE], dwr[2 * NE];
+                mixed_interp<NB, NE, double>(own, TT, fp + 3 + 3 * NB, fp + 3 + 4 * NB, wr, dwr);
+#pragma unroll
+                for (int m = 0; m < NE; m++) WR[m] = dual(wr[m], 0.0);
+#pragma unroll
+                for (int m = 0; m < 2 * NE; m++) dWR[m] = dual(dwr[m], 0.0);
+            }
+            mixed_face_flux<MK, NE, VISC>(P, G, cIP, TT, n, fg[3], fp + 1, WL, dWL, WR, dWR, g);
+#pragma unroll
+            for (int m = 0; m < NE; m++) JF[pt][d][m] = g[m].d;
+        } else {
+            const int q = pt - nfp;
+            const double* vp = mm.vol + ((size_t)e * mm.nqvm + q) * (1 + 3 * NB);
+            double wq[NE], dwq[2 * NE];
+            mixed_interp<NB, NE, double>(own, e, vp + 1, vp + 1 + NB, wq, dwq);
+            dual Wq[NE], dWq[2 * NE], fx[NE], fy[NE];
+#pragma unroll
+            for (int m = 0; m < NE; m++) Wq[m] = dual(wq[m], d == m ? 1.0 : 0.0);
+#pragma unroll
+            for (int m = 0; m < 2 * NE; m++) dWq[m] = dual(dwq[m], d == NE + m ? 1.0 : 0.0);
+            const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
+            convective_flux<MK>(P, Wq, ex, fx);
+            convective_flux<MK>(P, Wq, ey, fy);
+            if constexpr (VISC) {
+                dual gx[NE], gy[NE];
+                diffusive_flux<MK>(P, Wq, dWq, ex, gx);
+                diffusive_flux<MK>(P, Wq, dWq, ey, gy);
+#pragma unroll
+                for (int m = 0; m < NE; m++) { fx[m] -= gx[m]; fy[m] -= gy[m]; }
+            }
+#pragma unroll
+            for (int m = 0; m < NE; m++) {
+                JV[q][d][m] = fx[m].d;
+                JV[q][d][NE + m] = fy[m].d;
+            }
+        }
+    }
+    __syncthreads();
+    // phase 2
+    if (lane >= B) return;
+    const int dir = lane, md = dir / NB, jd = dir % NB;
+    if (jd >= nbe) {   // a padded unknown: a1 e_dir in the diagonal block, zero in the neighbour blocks
+        const size_t base = (size_t)mm.blkd[e] * B * B + (size_t)dir * B;
+        for (int i = 0; i < B; i++) {
+            blocks[base + i] = i == dir ? a1 : 0.0;
+            if (fblocks) fblocks[base + i] = (float)(i == dir ? a1 : 0.0);
+        }
+        for (int k = 0; k < nv; k++)
+            if (mm.nbr[e * 4 + k] > -1) {
+                const size_t nb = (size_t)mm.blkn[e * 4 + k] * B * B + (size_t)dir * B;
+                for (int i = 0; i < B; i++) {
+                    blocks[nb + i] = 0.0;
+                    if (fblocks) fblocks[nb + i] = 0.0f;
+                }
+            }
+        return;
+    }
+    double K[B];
+#pragma unroll
+    for (int i = 0; i < B; i++) K[i] = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < nv; k++) {
+        const int TT = mm.nbr[e * 4 + k];
+        const int nq = mm.nqf[e * 4 + k];
+        const size_t nbase = TT > -1 ? (size_t)mm.blkn[e * 4 + k] * B * B + (size_t)dir * B : 0;
+#pragma unroll 1
+        for (int q = 0; q < nq; q++) {
+            const double* fp = mm.face + (((size_t)e * 4 + k) * mm.nqfm + q) * (3 + 6 * NB);
+            const double jw = fp[0];
+            const double* phiL = fp + 3;
+            const double* grL = fp + 3 + NB;
+            const double tw = phiL[jd], tx = grL[2 * jd], ty = grL[2 * jd + 1];
+            const int pt = fstart[k] + q;
+            double T[NE];
+#pragma unroll
+            for (int m = 0; m < NE; m++) {
+                T[m] = JF[pt][md][m] * tw;
+                if constexpr (VISC) T[m] += JF[pt][NE + md][m] * tx + JF[pt][2 * NE + md][m] * ty;
+            }
+#pragma unroll
+            for (int m = 0; m < NE; m++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) K[m * NB + j] += (jw * phiL[j]) * T[m];
+            if (TT > -1) {
+                const double* phiR = fp + 3 + 3 * NB;
+#pragma unroll
+                for (int m = 0; m < NE; m++)
+#pragma unroll
+                    for (int j = 0; j < NB; j++) {
+                        const double a = (jw * phiR[j]) * T[m];
+                        double* c = blocks + nbase + m * NB + j;
+                        *c = q == 0 ? a : *c + a;
+                        if (fblocks && q + 1 == nq) fblocks[nbase + m * NB + j] = (float)*c;
+                    }
+            }
+        }
+    }
+#pragma unroll 1
+    for (int q = 0; q < nqv; q++) {
+        const double* vp = mm.vol + ((size_t)e * mm.nqvm + q) * (1 + 3 * NB);
+        const double jw = vp[0];
+        const double* gr = vp + 1 + NB;
+        const double tw = vp[1 + jd], tx = gr[2 * jd], ty = gr[2 * jd + 1];
+        double Tx[NE], Ty[NE];
+#pragma unroll
+        for (int m = 0; m < NE; m++) {
+            Tx[m] = JV[q][md][m] * tw;
+            Ty[m] = JV[q][md][NE + m] * tw;
+            if constexpr (VISC) {
+                Tx[m] += JV[q][NE + md][m] * tx + JV[q][2 * NE + md][m] * ty;
+                Ty[m] += JV[q][NE + md][NE + m] * tx + JV[q][2 * NE + md][NE + m] * ty;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NE; m++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) K[m * NB + j] += jw * (Tx[m] * gr[2 * j] + Ty[m] * gr[2 * j + 1]);
+    }
+    const double* M = mm.mass + (size_t)e * NB * NB;
+    const size_t base = (size_t)mm.blkd[e] * B * B + (size_t)dir * B;
+#pragma unroll
+    for (int i = 0; i < B; i++) {
+        double a = -K[i];
+        if (i / NB == md) a += a1 * M[jd * NB + (i % NB)];
+        blocks[base + i] = a;
+        if (fblocks) fblocks[base + i] = (float)a;
+    }
+}
+
 // The padded unknowns' columns of the matrix: zero except a1 on the diagonal (the identity mass).
 // The zeros are written once (the matrix and the FP32 factor storage are cleared at set-up, and
 // neither the Jacobian kernel nor the factorisation writes a non-zero into these columns), so each
@@ -468,10 +643,16 @@ hipError_t launch_jacobian_mixed(const ElemLaunch& L, hipStream_t s, const Model
                                  const double* W, double a1, double* blocks, float* fblocks, const double* v, double* out, int mode,
                                  const int* cols, int ncols, const int* pdofs, int npad) {
     const bool compact = mode == 0 && cols;
+    const bool elem = mode == 0 && mm.nqfm <= 4 && mm.nqvm <= MJ_MAXV && mm.nElem > 0;
     const long long threads = compact ? (long long)ncols : (long long)mm.nElem * L.nb * L.neq;
     dim3 blk(64), grd((unsigned)((threads + 63) / 64));
     if (threads == 0) return hipSuccess;
 #define X(MK, NB, V)                                                                                                   \
+    if (FP_MMATCH(MK, NB, V) && elem) {                                                                                \
+        hipLaunchKernelGGL((k_jacobian_mixed_elem<MK, NB, V>), dim3(mm.nElem), dim3(64), 0, s, P, G, mm, W, a1, blocks, \
+                           fblocks);                                                                                   \
+        return hipGetLastError();                                                                                      \
+    }                                                                                                                  \
     if (FP_MMATCH(MK, NB, V)) {                                                                                        \
         hipLaunchKernelGGL((k_jacobian_mixed<MK, NB, V, double>), grd, blk, 0, s, P, G, mm, W, a1, blocks, fblocks, v, out, \
                            mode, compact ? cols : nullptr, compact ? ncols : 0);                                       \
