@@ -64,11 +64,14 @@ class BatchedSolver:
 
 
 def seeded_ns_instances(n: int = 64, seed: int = 11, min_elems: int = 5000, max_elems: int = 50000,
-                        orders=(2, 3, 4), cfl: float = 10.0) -> list[tuple[cases.Case, dict]]:
+                        orders=(2, 3, 4), cfl: float = 10.0,
+                        per_element_orders: bool = False) -> list[tuple[cases.Case, dict]]:
     """n independent 2D NavierStokes cylinder cases: the number of triangles drawn uniformly from
     [min_elems, max_elems], the element order from `orders`, Re from [1e3, 1e4] and M from
     [0.2, 0.6], all from numpy default_rng(seed).  Each mesh is the cylinder O-grid with
-    n_theta ~ 1.6 n_r (n_theta a multiple of 8) and its own jitter seed."""
+    n_theta ~ 1.6 n_r (n_theta a multiple of 8) and its own jitter seed.  per_element_orders:
+    every element draws its order from `orders` (an order.txt per instance, seed 200 + i) and the
+    instance's order is the largest."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
@@ -79,5 +82,8 @@ def seeded_ns_instances(n: int = 64, seed: int = 11, min_elems: int = 5000, max_
         n_theta = max(16, 8 * int(round(math.sqrt(target / 2 * 1.6) / 8)))
         n_r = max(4, int(round(target / (2 * n_theta))))
         case = cases.cylinder_mesh(n_theta, n_r, seed=100 + i)
+        if per_element_orders:
+            case.orders = cases.seeded_orders(case.n_elem, orders, 200 + i)
+            order = int(case.orders.max())
         out.append((case, cases.ns_props(mach=mach, reynolds=re, order=order, CFL=cfl)))
     return out

@@ -83,6 +83,7 @@ hipError_t launch_jvp_mixed(const ElemLaunch&, hipStream_t, const ModelParams&, 
 hipError_t launch_size_over_lambda_mixed(const ElemLaunch&, hipStream_t, const ModelParams&, const MixedMesh&, const double*,
                                          const double*, double*);
 hipError_t launch_positivity_mixed(const ElemLaunch&, hipStream_t, const MixedMesh&, double*, int*);
+hipError_t launch_functional_mixed(const ElemLaunch&, hipStream_t, const MixedMesh&, const double*, double, int, double*, double*);
 hipError_t launch_residual_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const QuadTables&, const QuadMesh&,
                                 const double*, const double*, const double*, double, double, double, double*, double*, double*);
 hipError_t launch_jacobian_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const QuadTables&, const QuadMesh&,
@@ -2483,8 +2484,11 @@ extern "C" int fp_adjoint_solve(fp_ctx* c, double a1, const double* rhs, double*
 
 extern "C" int fp_functional(fp_ctx* c, int which, double* value, double* dIdW) {
     if (!c || !value || (which != 0 && which != 1) || c->mp.kind != MODEL_NAVIER_STOKES) return FP_EINVAL;
-    if (c->dim != 2 || c->quad || c->mixed) return FP_UNSUPPORTED;   // the force coefficients integrate over triangle wall edges
-    HIPCHK(c, launch_functional(c->L, c->s, c->tb, c->ms, c->W, c->kappa, which, c->tmp, dIdW ? c->w : nullptr));
+    if (c->dim != 2 || c->quad) return FP_UNSUPPORTED;   // the force coefficients integrate over triangle wall edges
+    if (c->mixed)
+        HIPCHK(c, launch_functional_mixed(c->L, c->s, c->mms, c->W, c->kappa, which, c->tmp, dIdW ? c->w : nullptr));
+    else
+        HIPCHK(c, launch_functional(c->L, c->s, c->tb, c->ms, c->W, c->kappa, which, c->tmp, dIdW ? c->w : nullptr));
     HIPCHK(c, launch_dot(c->s, (size_t)c->nElem, c->tmp, nullptr, c->partial, c->scal, 2));
     HIPCHK(c, hipMemcpyAsync(value, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));

@@ -606,6 +606,53 @@ __global__ void k_positivity_mixed(MixedMesh mm, double* __restrict__ W, int* __
     atomicAdd(count, 1);
 }
 
+// DragCoefficient (which 0) / LiftCoefficient (which 1) contribution of every element: the
+// pressure over its wall faces times the normal component, at the face rule of the element's own
+// order (a wall face has no second side), and the exact gradient with respect to the element's
+// coefficients (zero on the padded ones)
+template <int NB>
+__global__ void k_functional_mixed(MixedMesh mm, const double* __restrict__ W, double kappa, int which, double* __restrict__ ival,
+                                   double* __restrict__ grad) {
+    constexpr int NEQ = 4, B = NB * NEQ;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mm.nElem) return;
+    double g[B];
+#pragma unroll
+    for (int i = 0; i < B; i++) g[i] = 0.0;
+    double I = 0.0;
+    for (int k = 0; k < mm.nv[e]; k++) {
+        if (mm.nbr[e * 4 + k] != BT_WALL) continue;
+        const double nk = mm.fgeo[((size_t)e * 4 + k) * 4 + which];
+        for (int p = 0; p < mm.nqf[e * 4 + k]; p++) {
+            const double* fp = mm.face + (((size_t)e * 4 + k) * mm.nqfm + p) * (3 + 6 * NB);
+            const double* phi = fp + 3;
+            double w[NEQ];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) {
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; j++) s += W[(size_t)e * B + m * NB + j] * phi[j];
+                w[m] = s;
+            }
+            const double mom2 = w[1] * w[1] + w[2] * w[2];
+            const double pr = (kappa - 1) * (w[3] - mom2 / (2 * w[0]));
+            const double jw = fp[0];
+            I += jw * pr * nk;
+            const double dp[NEQ] = {(kappa - 1) * mom2 / (2 * w[0] * w[0]), -(kappa - 1) * w[1] / w[0], -(kappa - 1) * w[2] / w[0],
+                                    kappa - 1};
+#pragma unroll
+            for (int m = 0; m < NEQ; m++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) g[m * NB + j] += jw * nk * dp[m] * phi[j];
+        }
+    }
+    ival[e] = I;
+    if (grad) {
+#pragma unroll
+        for (int i = 0; i < B; i++) grad[(size_t)e * B + i] = g[i];
+    }
+}
+
 // ------------------------------------------------------------------ dispatch
 // (model, padded basis size, viscous): the padded size is the largest basis of the mesh rounded up
 // to 4, 9 or 16 (element orders up to 4)
@@ -703,6 +750,17 @@ hipError_t launch_positivity_mixed(const ElemLaunch& L, hipStream_t s, const Mix
     FP_MIXED_CASES(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_functional_mixed(const ElemLaunch& L, hipStream_t s, const MixedMesh& mm, const double* W, double kappa, int which,
+                                   double* ival, double* grad) {
+    if (L.model != MODEL_NAVIER_STOKES) return hipErrorInvalidValue;
+    dim3 blk(128), grd((mm.nElem + 127) / 128);
+    if (L.nb == 4) hipLaunchKernelGGL((k_functional_mixed<4>), grd, blk, 0, s, mm, W, kappa, which, ival, grad);
+    else if (L.nb == 9) hipLaunchKernelGGL((k_functional_mixed<9>), grd, blk, 0, s, mm, W, kappa, which, ival, grad);
+    else if (L.nb == 16) hipLaunchKernelGGL((k_functional_mixed<16>), grd, blk, 0, s, mm, W, kappa, which, ival, grad);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 }  // namespace fp

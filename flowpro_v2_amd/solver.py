@@ -230,7 +230,7 @@ class DeviceCase:
     def functional(self, which: int = 0, gradient: bool = True):
         """DragCoefficient (0) / LiftCoefficient (1) value and exact dI/dW."""
         val = C.c_double()
-        g = np.zeros((self.n_elem, self.B)) if gradient else None
+        g = np.zeros(self._shape()) if gradient else None
         self._check(native.lib().fp_functional(self.h, which, C.byref(val), native.dptr(g) if g is not None else None),
                     "fp_functional")
         return val.value, g

@@ -194,6 +194,24 @@ def test_mixed_freestream_residual_vanishes():
     dev.close()
 
 
+@pytest.mark.parametrize("which", [0, 1])
+def test_mixed_functional_and_gradient(which):
+    """Drag / lift on the cylinder with per-element orders 2-4: the value against the oracle, the
+    exact gradient against the oracle's forward differences (OptimisationToolExport, h = 1e-8) and
+    zero away from the wall."""
+    c = cases.cylinder_mesh(32, 8)
+    case = cases.Case("cyl_orders", c.PXY, c.TP, c.bnd, orders=cases.seeded_orders(c.n_elem, (2, 3, 4), 4))
+    o, dev, W = _setup(case, cases.ns_props(order=4, reynolds=300.0))
+    val, g = dev.functional(which)
+    assert abs(val - o.functional(which)) <= 1e-12
+    gfd = o.functional_grad_fd(which)
+    assert np.abs(g - gfd).max() <= 1e-6 * np.abs(gfd).max()
+    TT = o.TT()
+    wall = np.concatenate([np.full(o.sizes[e], (TT[e] == -1).any()) for e in range(case.n_elem)])
+    assert np.abs(g[~wall]).max() == 0.0 and np.abs(g[wall]).max() > 0
+    dev.close()
+
+
 def test_mixed_unsupported_features():
     case = _mixed(6, 2)
     with pytest.raises(solver.UnsupportedCase):
