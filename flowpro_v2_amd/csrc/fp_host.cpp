@@ -84,6 +84,8 @@ hipError_t launch_size_over_lambda_mixed(const ElemLaunch&, hipStream_t, const M
                                          const double*, double*);
 hipError_t launch_positivity_mixed(const ElemLaunch&, hipStream_t, const MixedMesh&, double*, int*);
 hipError_t launch_functional_mixed(const ElemLaunch&, hipStream_t, const MixedMesh&, const double*, double, int, double*, double*);
+hipError_t launch_functional_quad(const ElemLaunch&, hipStream_t, const QuadTables&, const QuadMesh&, const double*, double, int, double*,
+                                  double*);
 hipError_t launch_residual_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const QuadTables&, const QuadMesh&,
                                 const double*, const double*, const double*, double, double, double, double*, double*, double*);
 hipError_t launch_jacobian_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const QuadTables&, const QuadMesh&,
@@ -2484,8 +2486,10 @@ extern "C" int fp_adjoint_solve(fp_ctx* c, double a1, const double* rhs, double*
 
 extern "C" int fp_functional(fp_ctx* c, int which, double* value, double* dIdW) {
     if (!c || !value || (which != 0 && which != 1) || c->mp.kind != MODEL_NAVIER_STOKES) return FP_EINVAL;
-    if (c->dim != 2 || c->quad) return FP_UNSUPPORTED;   // the force coefficients integrate over triangle wall edges
-    if (c->mixed)
+    if (c->dim != 2) return FP_UNSUPPORTED;   // the force coefficients integrate over the wall edges of 2D meshes
+    if (c->quad)
+        HIPCHK(c, launch_functional_quad(c->L, c->s, c->qtb, c->qms, c->W, c->kappa, which, c->tmp, dIdW ? c->w : nullptr));
+    else if (c->mixed)
         HIPCHK(c, launch_functional_mixed(c->L, c->s, c->mms, c->W, c->kappa, which, c->tmp, dIdW ? c->w : nullptr));
     else
         HIPCHK(c, launch_functional(c->L, c->s, c->tb, c->ms, c->W, c->kappa, which, c->tmp, dIdW ? c->w : nullptr));

@@ -411,6 +411,51 @@ __global__ void k_positivity_quad(QuadTables tb, QuadMesh mq, double* __restrict
 
 // ------------------------------------------------------------------ dispatch
 // (model, NB, NQV, NQF, viscous) for element orders 1-4: NB = NQV = k^2, NQF = k
+// DragCoefficient (which 0) / LiftCoefficient (which 1) contribution of every quadrilateral:
+// the pressure over its wall faces times the normal component, and the exact gradient
+template <int NB, int NQF>
+__global__ void k_functional_quad(QuadTables tb, QuadMesh mq, const double* __restrict__ W, double kappa, int which,
+                                  double* __restrict__ ival, double* __restrict__ grad) {
+    constexpr int NEQ = 4, B = NB * NEQ;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mq.nElem) return;
+    double g[B];
+#pragma unroll
+    for (int i = 0; i < B; i++) g[i] = 0.0;
+    double I = 0.0;
+    for (int k = 0; k < 4; k++) {
+        if (mq.nbr[e * 4 + k] != BT_WALL) continue;
+        const double* fg = mq.fgeo + ((size_t)e * 4 + k) * 4;
+        const double nk = fg[which];
+        for (int p = 0; p < NQF; p++) {
+            const double* phi = tb.fb + (k * NQF + p) * NB;
+            double w[NEQ];
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) {
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; j++) s += W[(size_t)e * B + m * NB + j] * phi[j];
+                w[m] = s;
+            }
+            const double mom2 = w[1] * w[1] + w[2] * w[2];
+            const double pr = (kappa - 1) * (w[3] - mom2 / (2 * w[0]));
+            const double jw = fg[2] * tb.fw[p];
+            I += jw * pr * nk;
+            const double dp[NEQ] = {(kappa - 1) * mom2 / (2 * w[0] * w[0]), -(kappa - 1) * w[1] / w[0], -(kappa - 1) * w[2] / w[0],
+                                    kappa - 1};
+#pragma unroll
+            for (int m = 0; m < NEQ; m++)
+#pragma unroll
+                for (int j = 0; j < NB; j++) g[m * NB + j] += jw * nk * dp[m] * phi[j];
+        }
+    }
+    ival[e] = I;
+    if (grad) {
+#pragma unroll
+        for (int i = 0; i < B; i++) grad[(size_t)e * B + i] = g[i];
+    }
+}
+
 #define FP_QUAD_CASES(X)                               \
     X(MODEL_NAVIER_STOKES, 1, 1, 1, false)             \
     X(MODEL_NAVIER_STOKES, 4, 4, 2, false)             \
@@ -498,6 +543,20 @@ hipError_t launch_positivity_quad(const ElemLaunch& L, hipStream_t s, const Quad
     if (FP_QMATCH(MK, NB, NQV, NQF, V)) {                                                               \
         hipLaunchKernelGGL((k_positivity_quad<MK, NB, NQV, NQF>), grd, blk, 0, s, tb, mq, W, count);    \
         return hipGetLastError();                                                                       \
+    }
+    FP_QUAD_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_functional_quad(const ElemLaunch& L, hipStream_t s, const QuadTables& tb, const QuadMesh& mq, const double* W,
+                                  double kappa, int which, double* ival, double* grad) {
+    if (L.model != MODEL_NAVIER_STOKES) return hipErrorInvalidValue;
+    dim3 blk(128), grd((mq.nElem + 127) / 128);
+#define X(MK, NB, NQV, NQF, V)                                                                                   \
+    if (MK == MODEL_NAVIER_STOKES && FP_QMATCH(MK, NB, NQV, NQF, V)) {                                            \
+        hipLaunchKernelGGL((k_functional_quad<NB, NQF>), grd, blk, 0, s, tb, mq, W, kappa, which, ival, grad);   \
+        return hipGetLastError();                                                                                 \
     }
     FP_QUAD_CASES(X)
 #undef X

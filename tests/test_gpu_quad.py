@@ -165,6 +165,22 @@ def test_quad_freestream_residual_vanishes():
     dev.close()
 
 
+@pytest.mark.parametrize("which", [0, 1])
+def test_quad_functional_and_gradient(which):
+    """Drag / lift on the cylinder O-grid of quadrilaterals: the value against the oracle (1e-12),
+    the exact gradient against the oracle's forward differences (OptimisationToolExport, h = 1e-8)
+    and zero away from the wall."""
+    case = cases.cylinder_mesh(16, 5, quads=True)
+    o, dev, W = _setup(case, cases.ns_props(order=3, reynolds=300.0))
+    val, g = dev.functional(which)
+    assert abs(val - o.functional(which)) <= 1e-12
+    gfd = o.functional_grad_fd(which)
+    assert np.abs(g - gfd).max() <= 1e-6 * np.abs(gfd).max()
+    wall = (o.TT() == -1).any(1)
+    assert np.abs(g[~wall]).max() == 0.0 and np.abs(g[wall]).max() > 0
+    dev.close()
+
+
 def test_quad_unsupported_features():
     case = cases.channel_bump(6, 2, quads=True)
     with pytest.raises(solver.UnsupportedCase):
