@@ -77,7 +77,7 @@ bool mixed_supported(const ElemLaunch&);
 hipError_t launch_residual_mixed(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const MixedMesh&, const double*,
                                  const double*, const double*, double, double, double, double*, double*, double*);
 hipError_t launch_jacobian_mixed(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const MixedMesh&, const double*,
-                                 double, double*, float*, const double*, double*, int, const int*, int, const int*, int);
+                                 double, double*, float*, const double*, double*, int);
 hipError_t launch_jvp_mixed(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const MixedMesh&, const double*,
                             const double*, double, double*);
 hipError_t launch_size_over_lambda_mixed(const ElemLaunch&, hipStream_t, const ModelParams&, const MixedMesh&, const double*,
@@ -165,8 +165,6 @@ struct fp_ctx {
     QuadMesh qms{};
     MixedMesh mms{};
     int *m_nvd = nullptr, *m_nqvd = nullptr, *m_nbd = nullptr, *m_nqfd = nullptr;
-    int *m_cols = nullptr, *m_pdofs = nullptr;   // mixed: own and padded unknowns, position r B + m NB + j
-    int m_ncols = 0, m_npad = 0;
     double *m_vol = nullptr, *m_face = nullptr;
     double *qv = nullptr, *qf = nullptr, *qn = nullptr, *xq = nullptr, *qmass = nullptr, *qminv = nullptr, *qis = nullptr;
     bool jac_valid = false;
@@ -201,9 +199,7 @@ static hipError_t run_jacobian(fp_ctx* c, double a1) {
     // with FP32 block-ILU0 factors the Jacobian kernel also writes the factorisation's starting copy
     float* f32 = (c->precond == 2 && c->bits == 32) ? (float*)c->F : nullptr;
     c->f_fresh = f32 != nullptr;
-    if (c->mixed)
-        return launch_jacobian_mixed(c->L, c->s, c->mp, c->dg, c->mms, c->W, a1, c->val, f32, nullptr, nullptr, 0, c->m_cols,
-                                     c->m_ncols, c->m_pdofs, c->m_npad);
+    if (c->mixed) return launch_jacobian_mixed(c->L, c->s, c->mp, c->dg, c->mms, c->W, a1, c->val, f32, nullptr, nullptr, 0);
     if (c->quad) return launch_jacobian_quad(c->L, c->s, c->mp, c->dg, c->qtb, c->qms, c->W, a1, c->val, f32, nullptr, nullptr, 0);
     if (c->dim == 3) return launch_jacobian_tet_team(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, c->val, f32);
     return launch_jacobian_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, c->val, f32);
@@ -215,8 +211,7 @@ static hipError_t run_jvp(fp_ctx* c, const double* v, double a1, double* out) {
     return launch_jvp(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, v, a1, out);
 }
 static hipError_t run_jtv(fp_ctx* c, const double* v, double a1, double* out) {
-    if (c->mixed)
-        return launch_jacobian_mixed(c->L, c->s, c->mp, c->dg, c->mms, c->W, a1, nullptr, nullptr, v, out, 1, nullptr, 0, nullptr, 0);
+    if (c->mixed) return launch_jacobian_mixed(c->L, c->s, c->mp, c->dg, c->mms, c->W, a1, nullptr, nullptr, v, out, 1);
     if (c->quad) return launch_jacobian_quad(c->L, c->s, c->mp, c->dg, c->qtb, c->qms, c->W, a1, nullptr, nullptr, v, out, 1);
     if (c->dim == 3) return launch_jacobian_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, nullptr, v, out, 1);
     return launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, nullptr, v, out, 1);
@@ -735,7 +730,7 @@ static void free_all(fp_ctx* c) {
     if (c->g_factor) (void)hipGraphExecDestroy(c->g_factor);
     if (c->g_apply) (void)hipGraphExecDestroy(c->g_apply);
     if (c->io) (void)hipFree(c->io);
-    void* ptrs[] = {c->m_cols, c->m_pdofs, c->m_nvd, c->m_nqvd, c->m_nbd, c->m_nqfd, c->m_vol, c->m_face, c->qv, c->qf, c->qn, c->xq, c->qmass, c->qminv, c->qis, c->av, c->avold, c->trunc, c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
+    void* ptrs[] = {c->m_nvd, c->m_nqvd, c->m_nbd, c->m_nqfd, c->m_vol, c->m_face, c->qv, c->qf, c->qn, c->xq, c->qmass, c->qminv, c->qis, c->av, c->avold, c->trunc, c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
                     c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->val, c->F, c->iy, c->ixs, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
                     c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->istat};
     for (void* p : ptrs)
@@ -1429,16 +1424,6 @@ static int setup_mixed(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
         g += (long long)c->m_nb[c->order_[r]] * NEQ;
     }
     c->ref_dofs = g;
-    // the own and the padded unknowns, in DOF position order
-    std::vector<int> cols, pdofs;
-    cols.reserve((size_t)g);
-    for (int r = 0; r < n; r++)
-        for (int m = 0; m < NEQ; m++)
-            for (int j = 0; j < NB; j++) (j < nbd[r] ? cols : pdofs).push_back(r * NB * NEQ + m * NB + j);
-    c->m_ncols = (int)cols.size();
-    c->m_npad = (int)pdofs.size();
-    if (!cols.empty()) HIPCHK(c, upload(&c->m_cols, cols));
-    if (!pdofs.empty()) HIPCHK(c, upload(&c->m_pdofs, pdofs));
     HIPCHK(c, upload(&c->m_vol, vol));
     HIPCHK(c, upload(&c->m_face, face));
     HIPCHK(c, upload(&c->fgeo, fgeo));
@@ -1945,11 +1930,6 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     {
         const char* g = std::getenv("FLOWPRO_GRAPH");
         c->use_graph = !(g && g[0] == '0');
-    }
-    if (c->mixed) {
-        // the padded unknowns' columns stay zero off the diagonal from here on (k_mixed_padded_diagonal)
-        HIPCHK(c, hipMemsetAsync(c->val, 0, (size_t)c->nnzb * BB * sizeof(double), c->s));
-        if (c->F) HIPCHK(c, hipMemsetAsync(c->F, 0, (size_t)c->nnzb * BB * (c->bits == 32 ? sizeof(float) : sizeof(double)), c->s));
     }
     HIPCHK(c, hipMemsetAsync(c->W, 0, N * sizeof(double), c->s));
     HIPCHK(c, hipMemsetAsync(c->Wo, 0, N * sizeof(double), c->s));

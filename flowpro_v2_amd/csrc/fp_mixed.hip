@@ -227,6 +227,114 @@ __global__ void __launch_bounds__(128) k_residual_mixed(ModelParams P, DGParams 
         }
 }
 
+constexpr int MJ_MAXF = 16, MJ_MAXV = 16;   // face and volume points per element (orders <= 4)
+
+// The residual, one 64-lane workgroup per element: lane = point evaluates the face integrand
+// or the volume fluxes there (into LDS), then lane = own DOF (m, j) sums its test-function
+// contributions in the same order as k_residual_mixed (faces, then volume), adds the BDF mass
+// terms, and lanes (which, m) form the equation norms.  Same arithmetic as k_residual_mixed.
+template <int MK, int NB, bool VISC>
+__global__ void __launch_bounds__(64) k_residual_mixed_elem(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
+                                                            const double* __restrict__ Wo, const double* __restrict__ Wo2, double a1,
+                                                            double a2, double a3, double* __restrict__ R, double* __restrict__ RHS,
+                                                            double* __restrict__ enorm) {
+    constexpr int NE = neq_of<MK>(), B = NB * NE;
+    __shared__ double GF[MJ_MAXF][NE];
+    __shared__ double GV[MJ_MAXV][2 * NE];
+    __shared__ double KS[B], HS[B];
+    const int e = blockIdx.x, lane = threadIdx.x;
+    if (e >= mm.nElem) return;
+    const int nv = mm.nv[e], nqv = mm.nbe[e] > 1 ? mm.nqv[e] : 0;
+    const double cIP = mixed_cip(G, mm, e);
+    int fstart[5];
+    fstart[0] = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) fstart[k + 1] = fstart[k] + (k < nv ? mm.nqf[e * 4 + k] : 0);
+    const int nfp = fstart[4];
+    const MPlain own{W, B};
+    for (int pt = lane; pt < nfp + nqv; pt += 64) {
+        if (pt < nfp) {
+            int k = 0;
+            while (pt >= fstart[k + 1]) k++;
+            const int q = pt - fstart[k];
+            const int TT = mm.nbr[e * 4 + k];
+            const double* fg = mm.fgeo + ((size_t)e * 4 + k) * 4;
+            const double n[2] = {fg[0], fg[1]};
+            const double* fp = mm.face + (((size_t)e * 4 + k) * mm.nqfm + q) * (3 + 6 * NB);
+            double WL[NE], dWL[2 * NE], WR[NE], dWR[2 * NE], g[NE];
+            mixed_interp<NB, NE, double>(own, e, fp + 3, fp + 3 + NB, WL, dWL);
+            if (TT > -1) mixed_interp<NB, NE, double>(own, TT, fp + 3 + 3 * NB, fp + 3 + 4 * NB, WR, dWR);
+            mixed_face_flux<MK, NE, VISC>(P, G, cIP, TT, n, fg[3], fp + 1, WL, dWL, WR, dWR, g);
+#pragma unroll
+            for (int m = 0; m < NE; m++) GF[pt][m] = g[m];
+        } else {
+            const int q = pt - nfp;
+            const double* vp = mm.vol + ((size_t)e * mm.nqvm + q) * (1 + 3 * NB);
+            double Wq[NE], dWq[2 * NE], fx[NE], fy[NE];
+            mixed_interp<NB, NE, double>(own, e, vp + 1, vp + 1 + NB, Wq, dWq);
+            const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
+            convective_flux<MK>(P, Wq, ex, fx);
+            convective_flux<MK>(P, Wq, ey, fy);
+            if constexpr (VISC) {
+                double gx[NE], gy[NE];
+                diffusive_flux<MK>(P, Wq, dWq, ex, gx);
+                diffusive_flux<MK>(P, Wq, dWq, ey, gy);
+#pragma unroll
+                for (int m = 0; m < NE; m++) { fx[m] -= gx[m]; fy[m] -= gy[m]; }
+            }
+#pragma unroll
+            for (int m = 0; m < NE; m++) {
+                GV[q][m] = fx[m];
+                GV[q][NE + m] = fy[m];
+            }
+        }
+    }
+    __syncthreads();
+    if (lane < B) {
+        const int m = lane / NB, j = lane % NB;
+        double K = 0.0;
+        for (int k = 0; k < nv; k++)
+            for (int q = 0; q < mm.nqf[e * 4 + k]; q++) {
+                const double* fp = mm.face + (((size_t)e * 4 + k) * mm.nqfm + q) * (3 + 6 * NB);
+                K += (fp[0] * fp[3 + j]) * GF[fstart[k] + q][m];
+            }
+        for (int q = 0; q < nqv; q++) {
+            const double* vp = mm.vol + ((size_t)e * mm.nqvm + q) * (1 + 3 * NB);
+            const double* gr = vp + 1 + NB;
+            K += vp[0] * (GV[q][m] * gr[2 * j] + GV[q][NE + m] * gr[2 * j + 1]);
+        }
+        const double* M = mm.mass + (size_t)e * NB * NB;
+        double sm = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < NB; jj++) {
+            const size_t oj = (size_t)e * B + m * NB + jj;
+            const double mij = M[j * NB + jj];
+            sm += mij * a1 * W[oj] + mij * a2 * Wo[oj] + mij * a3 * Wo2[oj];
+        }
+        const size_t o = (size_t)e * B + lane;
+        R[o] = K;
+        RHS[o] = K - sm;
+        KS[lane] = K;
+        HS[lane] = K - sm;
+    }
+    if (!enorm) return;
+    __syncthreads();
+    if (lane < 2 * NE) {
+        const int which = lane / NE, m = lane % NE;
+        const double* Mi = mm.minv + (size_t)e * NB * NB;
+        const double* v = (which ? HS : KS) + m * NB;
+        double sum = 0.0;
+#pragma unroll
+        for (int a = 0; a < NB; a++) {
+            double t = 0.0;
+#pragma unroll
+            for (int b = 0; b < NB; b++) t += Mi[a * NB + b] * v[b];
+            sum += v[a] * t;
+        }
+        enorm[(size_t)e * 2 * NE + which * NE + m] = sum;
+    }
+}
+
 // Thread (e, dir) over the padded DOFs.  mode 0: column dir of the diagonal block (-dK/dW + a1 M)
 // and of the neighbour blocks (the neighbour-side integrand, KR); mode 1: ((a1 M - dR/dW)^T v)[e, dir].
 template <int NB, int NE, typename BT>
@@ -262,19 +370,15 @@ struct MColumnSink : MTangentSink<NB, NE> {
     }
 };
 
-// cols != nullptr (assembly): thread t takes the column cols[t] = e B + dir of the list of the
-// elements' own (unpadded) unknowns, ncols of them; the columns of padded unknowns are written by
-// k_mixed_padded_diagonal.  cols == nullptr: thread t takes column t % B of element t / B.
+// Thread t takes column t % B of element t / B.  Assembly (mode 0) runs k_jacobian_mixed_elem;
+// this per-column form serves J^T v (mode 1).
 template <int MK, int NB, bool VISC, typename BT>
 __global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W, double a1,
                                                        BT* __restrict__ blocks, float* __restrict__ fblocks,
-                                                       const double* __restrict__ v, double* __restrict__ out, int mode,
-                                                       const int* __restrict__ cols, int ncols) {
+                                                       const double* __restrict__ v, double* __restrict__ out, int mode) {
     constexpr int NE = neq_of<MK>(), B = NB * NE;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cols && t >= ncols) return;
-    const long long col = cols ? (long long)cols[t] : t;
-    const int e = (int)(col / B), dir = (int)(col % B);
+    const int e = (int)(t / B), dir = (int)(t % B);
     if (e >= mm.nElem) return;
     if (dir % NB >= mm.nbe[e]) {
         // a padded unknown: nothing depends on it, so its column is a1 e_dir in the diagonal
@@ -342,7 +446,6 @@ __global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G
 // over those inputs (phi_j, dphi_j/dx, dphi_j/dy in equation m), contracted with the own test
 // functions into the diagonal block and with the neighbour's into the neighbour blocks.  The
 // dual pass per (column, point) of k_jacobian_mixed becomes one per (input, point).
-constexpr int MJ_MAXF = 16, MJ_MAXV = 16;   // face and volume points per element (orders <= 4)
 
 template <int MK, int NB, bool VISC>
 __global__ void __launch_bounds__(64) k_jacobian_mixed_elem(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
@@ -509,21 +612,6 @@ __global__ void __launch_bounds__(64) k_jacobian_mixed_elem(ModelParams P, DGPar
     }
 }
 
-// The padded unknowns' columns of the matrix: zero except a1 on the diagonal (the identity mass).
-// The zeros are written once (the matrix and the FP32 factor storage are cleared at set-up, and
-// neither the Jacobian kernel nor the factorisation writes a non-zero into these columns), so each
-// assembly rewrites only the diagonal entry, in the FP64 matrix and in the FP32 copy the
-// factorisation starts from (where the previous factorisation left 1/a1).
-__global__ void k_mixed_padded_diagonal(int n, const int* __restrict__ pdofs, const int* __restrict__ blkd, int B, double a1,
-                                        double* __restrict__ blocks, float* __restrict__ fblocks) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int p = pdofs[t], r = p / B, dir = p % B;
-    const size_t o = (size_t)blkd[r] * B * B + (size_t)dir * B + dir;
-    blocks[o] = a1;
-    if (fblocks) fblocks[o] = (float)a1;
-}
-
 template <int MK, int NB, bool VISC>
 __global__ void __launch_bounds__(64) k_jvp_mixed(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
                                                   const double* __restrict__ v, double a1, double* __restrict__ out) {
@@ -674,7 +762,13 @@ hipError_t launch_residual_mixed(const ElemLaunch& L, hipStream_t s, const Model
                                  const double* W, const double* Wo, const double* Wo2, double a1, double a2, double a3, double* R,
                                  double* RHS, double* enorm) {
     dim3 blk(128), grd((mm.nElem + 127) / 128);
+    const bool elem = mm.nqfm <= 4 && mm.nqvm <= MJ_MAXV && mm.nElem > 0;
 #define X(MK, NB, V)                                                                                                      \
+    if (FP_MMATCH(MK, NB, V) && elem) {                                                                                   \
+        hipLaunchKernelGGL((k_residual_mixed_elem<MK, NB, V>), dim3(mm.nElem), dim3(64), 0, s, P, G, mm, W, Wo, Wo2, a1, a2, \
+                           a3, R, RHS, enorm);                                                                            \
+        return hipGetLastError();                                                                                         \
+    }                                                                                                                     \
     if (FP_MMATCH(MK, NB, V)) {                                                                                           \
         hipLaunchKernelGGL((k_residual_mixed<MK, NB, V>), grd, blk, 0, s, P, G, mm, W, Wo, Wo2, a1, a2, a3, R, RHS, enorm); \
         return hipGetLastError();                                                                                         \
@@ -684,29 +778,22 @@ hipError_t launch_residual_mixed(const ElemLaunch& L, hipStream_t s, const Model
     return hipErrorInvalidValue;
 }
 
-// mode 0 (assembly) runs over the own unknowns listed in cols (ncols) and then sets the padded
-// unknowns' diagonal (pdofs, npad); mode 1 (J^T v) runs over every padded column
+// mode 0 (assembly): one workgroup per element (k_jacobian_mixed_elem, every column including the
+// padded ones); mode 1 (J^T v): one thread per padded column
 hipError_t launch_jacobian_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
-                                 const double* W, double a1, double* blocks, float* fblocks, const double* v, double* out, int mode,
-                                 const int* cols, int ncols, const int* pdofs, int npad) {
-    const bool compact = mode == 0 && cols;
-    const bool elem = mode == 0 && mm.nqfm <= 4 && mm.nqvm <= MJ_MAXV && mm.nElem > 0;
-    const long long threads = compact ? (long long)ncols : (long long)mm.nElem * L.nb * L.neq;
+                                 const double* W, double a1, double* blocks, float* fblocks, const double* v, double* out, int mode) {
+    if (mm.nElem == 0) return hipSuccess;
+    if (mode == 0 && (mm.nqfm > 4 || mm.nqvm > MJ_MAXV)) return hipErrorInvalidValue;   // orders <= 4 only
+    const long long threads = (long long)mm.nElem * L.nb * L.neq;
     dim3 blk(64), grd((unsigned)((threads + 63) / 64));
-    if (threads == 0) return hipSuccess;
 #define X(MK, NB, V)                                                                                                   \
-    if (FP_MMATCH(MK, NB, V) && elem) {                                                                                \
-        hipLaunchKernelGGL((k_jacobian_mixed_elem<MK, NB, V>), dim3(mm.nElem), dim3(64), 0, s, P, G, mm, W, a1, blocks, \
-                           fblocks);                                                                                   \
-        return hipGetLastError();                                                                                      \
-    }                                                                                                                  \
     if (FP_MMATCH(MK, NB, V)) {                                                                                        \
-        hipLaunchKernelGGL((k_jacobian_mixed<MK, NB, V, double>), grd, blk, 0, s, P, G, mm, W, a1, blocks, fblocks, v, out, \
-                           mode, compact ? cols : nullptr, compact ? ncols : 0);                                       \
-        hipError_t e_ = hipGetLastError();                                                                             \
-        if (e_ != hipSuccess || !compact || npad == 0) return e_;                                                      \
-        hipLaunchKernelGGL(k_mixed_padded_diagonal, dim3((npad + 255) / 256), dim3(256), 0, s, npad, pdofs, mm.blkd,   \
-                           L.nb * L.neq, a1, blocks, fblocks);                                                         \
+        if (mode == 0)                                                                                                 \
+            hipLaunchKernelGGL((k_jacobian_mixed_elem<MK, NB, V>), dim3(mm.nElem), dim3(64), 0, s, P, G, mm, W, a1,     \
+                               blocks, fblocks);                                                                       \
+        else                                                                                                           \
+            hipLaunchKernelGGL((k_jacobian_mixed<MK, NB, V, double>), grd, blk, 0, s, P, G, mm, W, a1, blocks, fblocks, \
+                               v, out, mode);                                                                          \
         return hipGetLastError();                                                                                      \
     }
     FP_MIXED_CASES(X)
