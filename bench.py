@@ -247,6 +247,72 @@ def bench_batched(args):
     print(json.dumps(result), flush=True)
 
 
+def bench_steady(args):
+    """Time to a steady state: the 200k-triangle cylinder at Re = 40 (steady flow), NS p=3, M = 0.5,
+    CFL varied by the residual (State/CFLSetup with CFL <= 0, maxCFL 1e3) until the relative residual
+    (State.residuum) falls below --tol, from the freestream; FP64 preconditioner, then FP32.  Each
+    context factors once before the timed solve so that the autotuner's trials are not timed."""
+    from flowpro_v2_amd import cases, solver
+    case = make_case(args.n_theta, args.n_r)
+    props = cases.ns_props(order=4, reynolds=40.0, CFL=-1.0, maxCFL=1e3, residuum=args.tol)
+    log(f"mesh {case.name}: {case.n_elem} triangles, Re 40, to residuum {args.tol:g}")
+    out, states = {}, {}
+    for bits in (64, 32):
+        dev = solver.DeviceCase(case, props, precond_bits=bits)
+        w0 = dev.initial_state()
+        dev.set_state(w0)
+        dev.assemble(1e-3, -1.0, 1, True)
+        dev.linear_solve()
+        dev.set_state(w0)
+        s = solver.GpuImplicitSolver(dev, props)
+        t0 = time.perf_counter()
+        while len(s.history) < args.max_steps and (not s.history or s.history[-1].residuum > args.tol):
+            n0 = len(s.history)
+            s.solve(min(10, args.max_steps - n0))   # GpuImplicitSolver.solve continues and returns the whole history
+            if len(s.history) == n0:
+                break
+            log(f"fp{bits}: step {len(s.history)}, residuum {s.history[-1].residuum:.3e}, CFL {s.state.current_cfl:.3g}, "
+                f"{time.perf_counter() - t0:.1f} s")
+        elapsed = time.perf_counter() - t0
+        recs = s.history
+        iters = sum(it for r in recs for it in r.gmres_iters)
+        out[bits] = dict(seconds=elapsed, steps=len(recs), residuum=recs[-1].residuum, gmres=iters,
+                         updates=sum(r.newton for r in recs), dofs=dev.dofs)
+        states[bits] = dev.get_state()
+        log(f"fp{bits}: {len(recs)} steps, {iters} GMRES iterations, residuum {recs[-1].residuum:.3e} in {elapsed:.2f} s")
+        dev.close()
+    diff = float(np.abs(states[32] - states[64]).max() / np.abs(states[64]).max())
+    reached = all(out[b]["residuum"] <= args.tol for b in out)
+    result = {
+        "metric": f"Newton-GMRES wall time to a {args.tol:g} relative steady residual, mixed vs FP64 preconditioner",
+        "value": round(out[32]["seconds"], 3),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": out[32]["steps"],
+        "warmup": 0,
+        "ms_per_step": round(1e3 * out[32]["seconds"] / max(out[32]["steps"], 1), 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 residual/Jacobian/GMRES, f32 preconditioner",
+        "data": "synthetic (freestream start, seeded mesh)",
+        "config": {"workload": f"NS2D DG p=3 Re=40 M=0.5 cylinder (steady), {case.n_elem} triangles, {out[32]['dofs']} DOFs, "
+                               f"BDF1, CFL varied by the residual (maxCFL 1e3), blockilu0 (RCM) GMRES(30) tol 1e-2, "
+                               f"until residuum <= {args.tol:g}",
+                   "elements": case.n_elem, "dofs": out[32]["dofs"], "order": 4},
+        "reached_tolerance": reached,
+        "fp64_seconds": round(out[64]["seconds"], 3),
+        "speedup_mixed_over_fp64": round(out[64]["seconds"] / out[32]["seconds"], 3),
+        "steps_fp64": out[64]["steps"],
+        "gmres_iterations": {"fp64": out[64]["gmres"], "fp32": out[32]["gmres"]},
+        "final_residuum": {"fp64": out[64]["residuum"], "fp32": out[32]["residuum"]},
+        "state_max_rel_diff_mixed_vs_fp64": diff,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,7 +323,10 @@ def main():
     ap.add_argument("--cfl", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
-    ap.add_argument("--workload", choices=["ns2d_cylinder", "euler3d_tet", "batched_ns", "mixed_ns"], default="ns2d_cylinder")
+    ap.add_argument("--workload", choices=["ns2d_cylinder", "euler3d_tet", "batched_ns", "mixed_ns", "ns2d_steady"],
+                    default="ns2d_cylinder")
+    ap.add_argument("--tol", type=float, default=1e-8, help="ns2d_steady: the relative steady residual to reach")
+    ap.add_argument("--max-steps", type=int, default=400, help="ns2d_steady: step limit")
     ap.add_argument("--mx", type=int, default=520, help="channel cells along x of the mixed_ns workload")
     ap.add_argument("--my", type=int, default=256, help="channel cells along y of the mixed_ns workload")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batched_ns workload")
@@ -271,6 +340,9 @@ def main():
         sys.exit(2)
     if args.workload == "batched_ns":
         bench_batched(args)
+        return
+    if args.workload == "ns2d_steady":
+        bench_steady(args)
         return
     from flowpro_v2_amd import cases, native
     tet = args.workload == "euler3d_tet"
