@@ -221,7 +221,10 @@ def save_data(simulation_path: str, dev, state, cpu_ms: int = 0) -> None:
     if dev.mc.model_id != 2:
         save_reference_values(os.path.join(simulation_path, "referenceValues.txt"), dev.mc)
     save_matrix(os.path.join(simulation_path, "W.txt"), dev.averages())
-    save_matrix(os.path.join(simulation_path, "We.txt"), dev.get_state())
+    W = dev.get_state()
+    if getattr(dev, "mixed", False):   # one row per element, each of its own length nEqs nBasis_e
+        W = [W[dev.off[e]:dev.off[e + 1]] for e in range(dev.n_elem)]
+    save_matrix(os.path.join(simulation_path, "We.txt"), W)
     eps, di = dev.artificial_viscosity()
     save_matrix(os.path.join(simulation_path, "artificialViscosity.txt"), eps[:, None] + di)
 
@@ -232,10 +235,14 @@ def load_initial_state(simulation_path: str, dev, state) -> None:
     W.txt holds element means, spread over the coefficients of the nodal basis)."""
     old_order = load_state(os.path.join(simulation_path, "state.txt"), state)
     we = os.path.join(simulation_path, "We.txt")
+    mixed = getattr(dev, "mixed", False)
     if old_order == dev.order and os.path.exists(we):
         rows = load_matrix(we)
-        W = np.array(rows)
+        W = np.concatenate(rows) if mixed else np.array(rows)
     else:
         means = np.array(load_matrix(os.path.join(simulation_path, "W.txt")))
-        W = np.repeat(means, dev.nb, axis=1)
+        if mixed:
+            W = np.concatenate([np.repeat(means[e], dev.elem_nb[e]) for e in range(dev.n_elem)])
+        else:
+            W = np.repeat(means, dev.nb, axis=1)
     dev.set_state(W)

@@ -52,3 +52,30 @@ def test_save_and_continue(tmp_path):
     rest = [h.residuum for h in s2.solve(3)]
     np.testing.assert_allclose(first + rest, ref, rtol=1e-12)
     dev2.close()
+
+
+def test_save_and_continue_mixed_orders(tmp_path):
+    """A mesh of squares and triangles with per-element orders: We.txt holds one row per element of
+    its own length (nEqs nBasis_e), and continuing from the files reproduces the uninterrupted run."""
+    case = cases.mixed_channel(8, 4, jitter=0.2)
+    case.orders = cases.seeded_orders(case.n_elem, (1, 2, 3), 6)
+    props = cases.euler_props(order=3, CFL=5.0, saveRate=6, iterativeSolverTol=1e-10)
+    dev = solver.DeviceCase(case, props)
+    W0 = dev.initial_state() * (1 + 0.01 * np.random.default_rng(2).standard_normal(dev.dofs))
+    dev.set_state(W0)
+    ref = [h.residuum for h in solver.GpuImplicitSolver(dev, props).solve(9)]
+    dev.close()
+    path = str(tmp_path)
+    dev = solver.DeviceCase(case, props)
+    dev.set_state(W0)
+    first = [h.residuum for h in solver.GpuImplicitSolver(dev, props, simulation_path=path).solve(6)]
+    rows = results_io.load_matrix(os.path.join(path, "We.txt"))
+    assert [len(r) for r in rows] == list(dev.sizes)
+    assert np.array_equal(np.concatenate(rows), dev.get_state())
+    dev.close()
+    dev2 = solver.DeviceCase(case, props)
+    s2 = solver.GpuImplicitSolver(dev2, props, simulation_path=path)
+    s2.continue_from()
+    rest = [h.residuum for h in s2.solve(3)]
+    np.testing.assert_allclose(first + rest, ref, rtol=1e-12)
+    dev2.close()
