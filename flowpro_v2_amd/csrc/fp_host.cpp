@@ -13,6 +13,7 @@
 #include <fstream>
 #include <map>
 #include <string>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -1297,9 +1298,33 @@ static int setup_mixed(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
         xi[0] = (ab[6] * dx - ab[2] * dy) / det;
         xi[1] = (-ab[5] * dx + ab[1] * dy) / det;
     };
+    // the reference basis (value, d/dxi, d/deta) at a reference point; the rule points every
+    // element of a (type, order) shares are evaluated once (cache = true)
+    std::map<std::tuple<int, int, unsigned long long, unsigned long long>, std::vector<double>> refcache;
+    auto ref_values = [&](int t, int k, const double* xi, bool cache, std::vector<double>& tmp) -> const std::vector<double>& {
+        const Basis& b = basis_of(t, k);
+        unsigned long long bx, by;
+        std::memcpy(&bx, &xi[0], 8);
+        std::memcpy(&by, &xi[1], 8);
+        const auto key = std::make_tuple(t, k, bx, by);
+        if (cache) {
+            auto it = refcache.find(key);
+            if (it != refcache.end()) return it->second;
+        }
+        tmp.resize(3 * (size_t)b.nb);
+        for (int j = 0; j < b.nb; j++) {
+            tmp[3 * j] = basis_fun(b, j, xi[0], xi[1]);
+            tmp[3 * j + 1] = basis_der(b, j, xi[0], xi[1], 0);
+            tmp[3 * j + 2] = basis_der(b, j, xi[0], xi[1], 1);
+        }
+        if (!cache) return tmp;
+        return refcache.emplace(key, tmp).first->second;
+    };
+    std::vector<double> reftmp;
     // basis values and physical gradients of element i at reference point xi, padded to NB
-    auto tabulate = [&](int i, const double* xi, double* phi, double* gr) {
+    auto tabulate = [&](int i, const double* xi, double* phi, double* gr, bool cache) {
         const Basis& b = basis_of(typ[i], ord[i]);
+        const std::vector<double>& rv = ref_values(typ[i], ord[i], xi, cache, reftmp);
         double J[2][2];
         jac(i, xi, J);
         const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
@@ -1308,8 +1333,8 @@ static int setup_mixed(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
             phi[j] = 0;
             gr[2 * j] = gr[2 * j + 1] = 0;
             if (j >= b.nb) continue;
-            const double d0 = basis_der(b, j, xi[0], xi[1], 0), d1 = basis_der(b, j, xi[0], xi[1], 1);
-            phi[j] = basis_fun(b, j, xi[0], xi[1]);
+            const double d0 = rv[3 * j + 1], d1 = rv[3 * j + 2];
+            phi[j] = rv[3 * j];
             gr[2 * j] = iT[0][0] * d0 + iT[1][0] * d1;
             gr[2 * j + 1] = iT[0][1] * d0 + iT[1][1] * d1;
         }
@@ -1332,7 +1357,7 @@ static int setup_mixed(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
         double area = 0;
         for (int q = 0; q < vr.n; q++) {
             double* v = &vol[((size_t)r * nqvm + q) * VS];
-            const double jacv = tabulate(e, &vr.x[2 * q], v + 1, v + 1 + NB);
+            const double jacv = tabulate(e, &vr.x[2 * q], v + 1, v + 1 + NB, true);
             v[0] = jacv * vr.w[q];
             area += v[0];
         }
@@ -1390,11 +1415,11 @@ static int setup_mixed(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
                 f[0] = len / 2 * lr.w[p];
                 S += len / 2 * lr.w[p];
                 getX(e, xi, f + 1);
-                tabulate(e, xi, f + 3, f + 3 + NB);
+                tabulate(e, xi, f + 3, f + 3 + NB, true);
                 if (t > -1) {
                     double xin[2];
                     getXi(t, f + 1, xin);
-                    tabulate(t, xin, f + 3 + 3 * NB, f + 3 + 4 * NB);
+                    tabulate(t, xin, f + 3 + 3 * NB, f + 3 + 4 * NB, false);
                 }
             }
             double dL;
