@@ -183,7 +183,8 @@ def bench_batched(args):
     """Independent instances solved concurrently (flowpro_v2_amd.batch), FP64 then mixed."""
     from flowpro_v2_amd import batch
     t0 = time.perf_counter()
-    inst = batch.seeded_ns_instances(n=args.instances, cfl=args.cfl, per_element_orders=args.per_element_orders)
+    inst = batch.seeded_ns_instances(n=args.instances, cfl=args.cfl, per_element_orders=args.per_element_orders,
+                                     orders=(2, 3, 4, 5) if args.per_element_orders else (2, 3, 4))
     n_elem = sum(c.n_elem for c, _ in inst)
     log(f"{len(inst)} instances, {n_elem} triangles in total, meshes built in {time.perf_counter() - t0:.1f} s")
     out = {}
@@ -231,7 +232,7 @@ def bench_batched(args):
         "dtype": "f64 residual/Jacobian/GMRES, f32 preconditioner" if main_bits == 32 else "f64",
         "data": "synthetic (seeded cylinder meshes, freestream states, seeded Re and M per instance)",
         "config": {"workload": f"{len(inst)} independent NS2D cylinder instances, 5k-50k triangles each, element order "
-                               f"2-4 {'per element' if args.per_element_orders else 'per instance'}, Re U[1e3,1e4], "
+                               f"{'2-5 per element' if args.per_element_orders else '2-4 per instance'}, Re U[1e3,1e4], "
                                f"M U[0.2,0.6], BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) GMRES(30) "
                                "tol 1e-2, one context and stream per instance",
                    "instances": len(inst), "elements": n_elem, "dofs": r["dofs"], "precond_bits": main_bits},
@@ -331,7 +332,7 @@ def main():
     ap.add_argument("--my", type=int, default=256, help="channel cells along y of the mixed_ns workload")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batched_ns workload")
     ap.add_argument("--per-element-orders", action="store_true",
-                    help="batched_ns: every element of an instance draws its order from 2-4 (order.txt)")
+                    help="batched_ns: every element of an instance draws its order from 2-5 (order.txt)")
     ap.add_argument("--cube", type=int, default=64, help="cubes per edge of the euler3d_tet workload")
     args = ap.parse_args()
     if args.gpus != 1:

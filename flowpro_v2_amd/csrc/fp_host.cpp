@@ -1650,8 +1650,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         for (int i = 0; i < n; i++) {
             const int o = cs->elem_orders ? cs->elem_orders[i] : cs->order;
             const int t = cs->elem_types ? cs->elem_types[i] : (cs->elem_type == 4 ? 4 : 3);
-            if (o < 1 || o > 4 || (t != 0 && t != 3 && t != 4)) {
-                c->err = "mixed meshes: element orders 1-4, element types 3 or 4";
+            if (o < 1 || o > (t == 4 ? 4 : 5) || (t != 0 && t != 3 && t != 4)) {
+                c->err = "mixed meshes: element orders 1-5 on triangles, 1-4 on squares, element types 3 or 4";
                 return FP_UNSUPPORTED;
             }
             maxo = std::max(maxo, o);

@@ -227,7 +227,9 @@ __global__ void __launch_bounds__(128) k_residual_mixed(ModelParams P, DGParams 
         }
 }
 
-constexpr int MJ_MAXF = 16, MJ_MAXV = 16;   // face and volume points per element (orders <= 4)
+// face and volume points per element: 4 faces of up to 5 points (a face next to an order-5
+// triangle), 19 volume points (the triangle rule of order 5, degree 9)
+constexpr int MJ_MAXF = 20, MJ_MAXV = 19;
 
 // The residual, one 64-lane workgroup per element: lane = point evaluates the face integrand
 // or the volume fluxes there (into LDS), then lane = own DOF (m, j) sums its test-function
@@ -743,7 +745,7 @@ __global__ void k_functional_mixed(MixedMesh mm, const double* __restrict__ W, d
 
 // ------------------------------------------------------------------ dispatch
 // (model, padded basis size, viscous): the padded size is the largest basis of the mesh rounded up
-// to 4, 9 or 16 (element orders up to 4)
+// to 4, 9 or 16 (element orders up to 5 on triangles, 4 on squares)
 #define FP_MIXED_CASES(X)                                                                                            \
     X(MODEL_NAVIER_STOKES, 4, false) X(MODEL_NAVIER_STOKES, 9, false) X(MODEL_NAVIER_STOKES, 16, false)              \
     X(MODEL_NAVIER_STOKES, 4, true) X(MODEL_NAVIER_STOKES, 9, true) X(MODEL_NAVIER_STOKES, 16, true)                 \
@@ -762,7 +764,7 @@ hipError_t launch_residual_mixed(const ElemLaunch& L, hipStream_t s, const Model
                                  const double* W, const double* Wo, const double* Wo2, double a1, double a2, double a3, double* R,
                                  double* RHS, double* enorm) {
     dim3 blk(128), grd((mm.nElem + 127) / 128);
-    const bool elem = mm.nqfm <= 4 && mm.nqvm <= MJ_MAXV && mm.nElem > 0;
+    const bool elem = mm.nqfm * 4 <= MJ_MAXF && mm.nqvm <= MJ_MAXV && mm.nElem > 0;
 #define X(MK, NB, V)                                                                                                      \
     if (FP_MMATCH(MK, NB, V) && elem) {                                                                                   \
         hipLaunchKernelGGL((k_residual_mixed_elem<MK, NB, V>), dim3(mm.nElem), dim3(64), 0, s, P, G, mm, W, Wo, Wo2, a1, a2, \
@@ -783,7 +785,7 @@ hipError_t launch_residual_mixed(const ElemLaunch& L, hipStream_t s, const Model
 hipError_t launch_jacobian_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
                                  const double* W, double a1, double* blocks, float* fblocks, const double* v, double* out, int mode) {
     if (mm.nElem == 0) return hipSuccess;
-    if (mode == 0 && (mm.nqfm > 4 || mm.nqvm > MJ_MAXV)) return hipErrorInvalidValue;   // orders <= 4 only
+    if (mode == 0 && (mm.nqfm * 4 > MJ_MAXF || mm.nqvm > MJ_MAXV)) return hipErrorInvalidValue;   // orders <= 5 only
     const long long threads = (long long)mm.nElem * L.nb * L.neq;
     dim3 blk(64), grd((unsigned)((threads + 63) / 64));
 #define X(MK, NB, V)                                                                                                   \

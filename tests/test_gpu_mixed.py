@@ -49,6 +49,9 @@ MIXED_CONFIGS = [
     ("ns_mixed_234", lambda: _mixed(6, 3, orders=(2, 3, 4), seed=8), lambda: cases.ns_props(order=4, reynolds=300.0)),
     ("ae_mixed_12", lambda: _mixed(8, 4, orders=(1, 2), seed=9),
      lambda: cases.euler_props(model="AdiabaticEulers", order=2)),
+    # order 5 (15 functions, the degree-9 triangle rule of 19 points) on triangles
+    ("euler_tri_orders_345", lambda: _tri_orders(8, 4, (3, 4, 5), 3), lambda: cases.euler_props(order=5)),
+    ("ns_tri_orders_45", lambda: _tri_orders(6, 3, (4, 5), 4), lambda: cases.ns_props(order=5, reynolds=300.0)),
 ]
 
 
