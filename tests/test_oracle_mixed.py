@@ -169,3 +169,35 @@ def test_triangle_orders_with_one_order_equal_uniform():
     dm, nm = om.blocks_flat()
     assert all(np.array_equal(d[e], dm[e]) for e in range(u.n_elem))
     assert all(np.array_equal(n[e, k], blk) for (e, k), blk in nm.items())
+
+
+def test_order_five_triangles_pins():
+    """Order 5 (15 functions, the degree-9 triangle rule of 19 points, 5-point face rules): a
+    uniform order.txt of 5 equals the uniform order-5 build bit for bit, and free-stream
+    preservation and conservation hold on triangles with per-element orders 3-5."""
+    u = cases.channel_bump(4, 2, jitter=0.2)
+    m = cases.Case("m", u.PXY, u.TP, u.bnd, orders=np.full(u.n_elem, 5, np.int32))
+    props = cases.euler_props(order=5)
+    ou, om = O.Oracle(u, props), O.Oracle(m, props)
+    assert ou.nb == 15 and (om.sizes == 60).all()
+    W = cases.perturbed_state(ou.get_w()[0][:: ou.nb], u.n_elem, ou.nb, 0.02, 4)
+    ou.set_w(W)
+    om.set_w(W.ravel())
+    assert np.array_equal(ou.residual().ravel(), om.residual())
+    b = cases.channel_bump(6, 3, jitter=0.2, seed=2)
+    b.PXY[:, 1] = np.repeat(np.linspace(0.0, 1.0, 4)[None, :], 7, 0).ravel()
+    c = cases.Case("t", b.PXY, b.TP, b.bnd, orders=cases.seeded_orders(b.n_elem, (3, 4, 5), 2))
+    o = O.Oracle(c, props)
+    assert np.abs(o.residual()).max() < 1e-12
+    bnd = c.bnd.copy()
+    bnd[:, 0] = -4
+    cs = cases.Case("slip", c.PXY, c.TP, bnd, orders=c.orders)
+    o = O.Oracle(cs, props)
+    rng = np.random.default_rng(5)
+    W = np.concatenate([np.repeat([1.0, 0.1, -0.05, 2.5], s // 4) for s in o.sizes])
+    W *= 1 + 0.02 * rng.standard_normal(W.shape)
+    o.set_w(W)
+    R = o.residual()
+    for mq in (0, 3):
+        parts = [R[o.off[e] + mq * (o.sizes[e] // 4): o.off[e] + (mq + 1) * (o.sizes[e] // 4)] for e in range(cs.n_elem)]
+        assert abs(sum(p.sum() for p in parts)) < 1e-13 * sum(np.abs(p).sum() for p in parts)
