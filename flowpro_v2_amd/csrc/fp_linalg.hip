@@ -42,6 +42,34 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int nrows, int B, const int* _
     y[g] = s;
 }
 
+// the same product with the block size fixed at compile time: the column loop unrolls and its
+// loads issue ahead of the (unchanged, sequential) accumulation
+template <int B>
+__global__ void __launch_bounds__(256) k_bsr_spmv_b(int nrows, const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                                    const double* __restrict__ val, const double* __restrict__ x,
+                                                    const double* __restrict__ b, double* __restrict__ y, int mode) {
+    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int r = (int)(g / B), t = (int)(g % B);
+    if (r >= nrows) return;
+    double s = mode ? b[g] : 0.0;
+    constexpr size_t BB = (size_t)B * B;
+    for (int blk = rowptr[r]; blk < rowptr[r + 1]; blk++) {
+        const double* a = val + (size_t)blk * BB + t;
+        const double* xc = x + (size_t)colidx[blk] * B;
+        double av[B];
+#pragma unroll
+        for (int c = 0; c < B; c++) av[c] = a[(size_t)c * B];
+        if (mode) {
+#pragma unroll
+            for (int c = 0; c < B; c++) s -= av[c] * xc[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < B; c++) s += av[c] * xc[c];
+        }
+    }
+    y[g] = s;
+}
+
 // ------------------------------------------------------------------ ILU copy
 template <typename TF>
 __global__ void k_ilu_copy(size_t n, const double* __restrict__ src, TF* __restrict__ dst) {
@@ -1011,8 +1039,15 @@ static inline unsigned grid_for(size_t n, int t) {
 hipError_t launch_spmv(hipStream_t s, int nrows, int B, const int* rowptr, const int* colidx, const double* val,
                        const double* x, const double* b, double* y, int mode) {
     long long n = (long long)nrows * B;
-    hipLaunchKernelGGL(k_bsr_spmv, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nrows, B, rowptr, colidx, val, x, b, y,
-                       mode);
+    const dim3 grd((unsigned)((n + 255) / 256)), blk(256);
+    switch (B) {
+        case 20: hipLaunchKernelGGL(k_bsr_spmv_b<20>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
+        case 24: hipLaunchKernelGGL(k_bsr_spmv_b<24>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
+        case 30: hipLaunchKernelGGL(k_bsr_spmv_b<30>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
+        case 40: hipLaunchKernelGGL(k_bsr_spmv_b<40>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
+        case 64: hipLaunchKernelGGL(k_bsr_spmv_b<64>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
+        default: hipLaunchKernelGGL(k_bsr_spmv, grd, blk, 0, s, nrows, B, rowptr, colidx, val, x, b, y, mode);
+    }
     return hipGetLastError();
 }
 
