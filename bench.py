@@ -149,7 +149,8 @@ def roofline_spmv(dev, ms):
     traffic = 8.0 * B * B * dev.nnzb + 8.0 * B * 2 * dev.n_elem + 4.0 * dev.nnzb
     achieved = traffic / (ms * 1e-3) / 1e9
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel="k_bsr_spmv",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel=f"k_bsr_spmv_b<{B}>" if B in (20, 24, 30, 40, 64) else "k_bsr_spmv",
                 algorithmic_bytes=traffic, avg_launch_ms=ms)
 
 
