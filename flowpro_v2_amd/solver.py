@@ -23,6 +23,17 @@ PRECONDITIONERS = {"none": 0, "blockjacobi": 1, "blockilu0": 2, "ilu0": 3}
 MAX_VALUE = 1.7976931348623157e308
 
 
+STOP_FILE_NAME = "STOP"   # MasterSolver.STOP_FILE_NAME
+
+
+def first_time_save_index(t: float, time_save_rate: float) -> int:
+    """MasterSolver.firstTimeSaveIndex (MasterSolver.java:76-81): index of the next time level
+    at which results are saved."""
+    if time_save_rate <= 0 or time_save_rate == MAX_VALUE:
+        return 1
+    return int(min(2 ** 31 - 1, math.floor(t / time_save_rate) + 1))
+
+
 class UnsupportedCase(RuntimeError):
     """The device path has no kernel for this case (the reference's Java path applies)."""
 
@@ -431,13 +442,18 @@ class GpuImplicitSolver:
         self.props = props
         self.simulation_path = simulation_path
         self.save_rate = int(props.get("saveRate", 2 ** 31 - 1))   # Parameters.saveRate
+        self.time_save_rate = float(props.get("timeSaveRate", MAX_VALUE))   # Parameters.timeSaveRate
+        self.t_end = float(props.get("tEnd", MAX_VALUE))   # Parameters.tEnd (Parameters.java:143-147)
         self.execution_ms = 0.0
-        if "dt" in props and "CFL" not in props:
-            self.state = State(cfl=0.0, fixed_dt=float(props["dt"]))
-        else:
-            cfl = float(props.get("CFL", 1.0))
+        # Parameters.java:207-221: CFL wins over dt, one of the two is required
+        if "CFL" in props:
+            cfl = float(props["CFL"])
             self.state = State(cfl=1.0 if cfl <= 0 else cfl, vary_cfl=cfl <= 0,
                                max_vary_cfl=float(props.get("maxCFL", 1e4)))
+        elif "dt" in props:
+            self.state = State(cfl=0.0, fixed_dt=float(props["dt"]))
+        else:
+            raise ValueError("either parameter CFL or dt must be defined")
         self.time_order = int(props.get("timeOrder", 1))
         self.newton_iters = int(props.get("newtonIters", 1))
         self.newton_tol = float(props.get("newtonIterTol", 1e-4))
@@ -465,13 +481,20 @@ class GpuImplicitSolver:
         dto = -1.0
         converges = True
         total = st.steps + steps
+        tsr = first_time_save_index(st.t, self.time_save_rate)
         st.steps += 1
-        while st.steps <= total and st.residuum > self.stop_residuum:
+        while st.steps <= total and st.residuum > self.stop_residuum and st.t < self.t_end:
             if converges:
                 st.update_cfl()
             if st.convergence_hopeless():
                 break
             dt = st.time_step(dev.size_over_lambda())
+            # LocalImplicitSolver.java:151-159: tEnd reached up to the round-off of the summed time
+            # (a remaining step of ~1e-12 dt would spoil the BDF solution); the last step is clipped
+            if self.t_end - st.t <= 1e-6 * dt:
+                break
+            if st.t + dt > self.t_end:
+                dt = self.t_end - st.t
             dev.limiter(first_step=not self.history)   # LocalImplicitSolver.java:170-172
             rhs0 = 0.0
             iters = []
@@ -511,8 +534,16 @@ class GpuImplicitSolver:
                                                float(int(self.execution_ms)), st.equation_residuum)
                 dev.next_level()
                 self.history.append(StepRecord(st.steps, st.residuum, dt, st.current_cfl, newton_done, iters, limited))
-                if self.simulation_path and st.steps % self.save_rate == 0:
-                    self.save()
+                if st.steps % self.save_rate == 0 or st.t > tsr * self.time_save_rate:
+                    if self.simulation_path:
+                        self.save()
+                    tsr = first_time_save_index(st.t, self.time_save_rate)
+                # MasterSolver.isStopRequested (MasterSolver.java:61-68): a file STOP in the
+                # simulation directory ends the computation after this step and is deleted
+                if self.simulation_path and os.path.exists(os.path.join(self.simulation_path, STOP_FILE_NAME)):
+                    os.remove(os.path.join(self.simulation_path, STOP_FILE_NAME))
+                    st.steps += 1
+                    break
             st.steps += 1
         st.steps -= 1
         return self.history

@@ -28,7 +28,7 @@ class Props(C.Structure):
     _fields_ = [(n, C.c_int) for n in _I] + [(n, C.c_double) for n in _D] + [("dimension", C.c_int)] + \
         [("damp_tol", C.c_double), ("damp_const", C.c_double), ("damp_inner_tol", C.c_double),
          ("n_damp_inner_coef", C.c_int), ("damp_inner_coef", C.c_double * 5), ("numerical_flux", C.c_int),
-         ("elem_type", C.c_int)]
+         ("elem_type", C.c_int), ("t_end", C.c_double)]
 
 
 def build(force: bool = False) -> str:
@@ -72,6 +72,10 @@ def lib():
             "or_model_pointwise": (None, [vp, dp, dp, C.c_double, dp, dp, dp]),
             "or_flux_pointwise": (None, [vp, dp, dp, dp, C.c_int, dp, dp]),
             "or_nqf": (C.c_int, [vp]),
+            "or_bdf_coefficients": (None, [vp, C.c_double, C.c_double, dp]),
+            "or_cfl_update": (C.c_double, [vp, C.c_double]), "or_cfl_reduce": (C.c_double, [vp]),
+            "or_set_residuum": (C.c_double, [vp, dp, dp]), "or_l2_norms_sq": (None, [vp, dp, dp]),
+            "or_positivity": (C.c_int, [vp]), "or_bj_start": (None, [vp]), "or_bj_apply": (None, [vp, dp, dp]),
             "ot_create": (vp, [C.POINTER(Props), C.c_int, dp, C.c_int, ip, C.c_int, ip, C.c_char_p]),
             "ot_destroy": (None, [vp]), "ot_nbasis": (C.c_int, [vp]), "ot_neqs": (C.c_int, [vp]),
             "ot_tt": (None, [vp, ip]), "ot_order": (None, [vp, ip]),
@@ -132,6 +136,7 @@ def props_from(p: dict, threads: int = 1) -> Props:
     P.penalty, P.beta0 = float(p.get("penalty", 0.0)), float(p.get("beta0", 2.0))
     P.h = 1e-8
     P.residuum_stop = float(p.get("residuum", -1.0))
+    P.t_end = float(p.get("tEnd", 1.7976931348623157e308))
     a = p.get("a", (1.0, 0.5))
     P.scalar_a0, P.scalar_a1, P.scalar_k = float(a[0]), float(a[1]), float(p.get("k", 0.0))
     P.dimension = int(p.get("dimension", 2))
@@ -344,6 +349,41 @@ class Oracle:
         out = np.zeros(6)
         lib().or_state(self.h, _dp(out))
         return dict(zip(["t", "CFL", "dt", "residuum", "lin_fail", "steps"], out))
+
+    def bdf_coefficients(self, dt, dto):
+        a = np.zeros(3)
+        lib().or_bdf_coefficients(self.h, dt, dto, _dp(a))
+        return a
+
+    def cfl_update(self, residuum):
+        return lib().or_cfl_update(self.h, residuum)
+
+    def cfl_reduce(self):
+        return lib().or_cfl_reduce(self.h)
+
+    def set_residuum(self, norms):
+        norms = np.ascontiguousarray(norms, dtype=np.float64)
+        eq = np.zeros(self.neq)
+        r = lib().or_set_residuum(self.h, _dp(norms), _dp(eq))
+        return r, eq
+
+    def l2_norms_sq(self, R):
+        R = np.ascontiguousarray(R, dtype=np.float64).ravel()
+        out = np.zeros(self.neq)
+        lib().or_l2_norms_sq(self.h, _dp(R), _dp(out))
+        return out
+
+    def positivity(self):
+        return lib().or_positivity(self.h)
+
+    def bj_start(self):
+        lib().or_bj_start(self.h)
+
+    def bj_apply(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        lib().or_bj_apply(self.h, _dp(b), _dp(x))
+        return x
 
     def time_step(self):
         return lib().or_time_step(self.h)

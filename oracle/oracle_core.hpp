@@ -104,6 +104,7 @@ struct Props {
     double damp_inner_coef[5];
     int numerical_flux;   // Aerodynamics.numericalFluxType: 0 "default" (Lax-Friedrichs), 1 "vanLeer"
     int elem_type;        // 2D element type of every element: 3 triangles (or 0), 4 quadrilaterals
+    double t_end;         // Parameters.tEnd (Double.MAX_VALUE when absent, C/Parameters.java:143-147)
 };
 
 struct ModelConst {
@@ -260,5 +261,7 @@ double functional(const Case& c, int which, const double* W /*nullable*/);
 void model_pointwise(const Case& c, const double* W, const double* dW, double D, double* source, double* fvx, double* fvy);
 void flux_pointwise(const Case& c, const double* WL, const double* WR, const double* n, int TT, double* fnum, double* fphys);
 void functional_grad_fd(Case& c, int which, double* dIdW);
+// BDF2.coefficients / BDF1 (a1, a2, a3) of the case's time order
+void bdf_coefficients(const Case& c, double dt, double dto, double* a);
 
 }  // namespace oracle

@@ -308,6 +308,7 @@ static void bdf_coeffs(const Case& c, double dt, double dto, double* a) {
         a[2] = 0;
     }
 }
+void bdf_coefficients(const Case& c, double dt, double dto, double* a) { bdf_coeffs(c, dt, dto, a); }
 
 // BDF1.assembleRHS / BDF2.assembleRHS (C/element/BDF1.java:34-49, BDF2.java:32-51)
 static void assemble_rhs(const Case& c, Elem& e, const double* Rw, const double* a) {

@@ -612,13 +612,16 @@ int run(Case& c, int nsteps) {
     bool converges = true;
     int total = c.steps_done + nsteps;
     int completed = 0;
-    for (++c.steps_done; c.steps_done <= total && c.residuum > c.P.residuum_stop; ++c.steps_done) {
+    for (++c.steps_done; c.steps_done <= total && c.residuum > c.P.residuum_stop && c.t < c.P.t_end; ++c.steps_done) {
         if (converges) {
             if (c.P.fixed_dt) c.state_dt = std::min(2 * c.state_dt, c.P.dt);
             else c.currentCFL = get_cfl(c, c.currentCFL, c.residuum);
         }
         if (c.P.fixed_dt ? c.state_dt < c.P.dt / 1024 : (c.currentCFL > 0 && c.currentCFL < c.P.cfl / 20)) break;
         double dt = time_step(c);
+        // LocalImplicitSolver.java:151-159: stop at tEnd up to 1e-6 dt, clip the last step
+        if (c.P.t_end - c.t <= 1e-6 * dt) break;
+        if (c.t + dt > c.P.t_end) dt = c.P.t_end - c.t;
         limiter_all(c, completed == 0);
         double rhsNorm0 = 0;
         converges = true;
@@ -845,6 +848,56 @@ void or_flux_pointwise(void* h, const double* WL, const double* WR, const double
 void or_model_pointwise(void* h, const double* W, const double* dW, double D, double* s, double* fx, double* fy) {
     model_pointwise(*(Case*)h, W, dW, D, s, fx, fy);
 }
+// ---- driver pieces, for the pins of tests/test_oracle_driver_pins.py
+void or_bdf_coefficients(void* h, double dt, double dto, double* a) { bdf_coefficients(*(Case*)h, dt, dto, a); }
+// State.updateCFL with CFLSetup.getCFL on the case's CFL state; returns the new CFL number
+double or_cfl_update(void* h, double residuum) {
+    Case* c = (Case*)h;
+    c->currentCFL = get_cfl(*c, c->currentCFL, residuum);
+    return c->currentCFL;
+}
+// State.reduceCFL with CFLSetup.reduceCFL
+double or_cfl_reduce(void* h) {
+    Case* c = (Case*)h;
+    c->cflResOld = -1;
+    c->currentCFL = c->currentCFL / 1.5;
+    return c->currentCFL;
+}
+// State.setResiduum for one step's norms: returns the residuum, eq[m] = equationResiduum
+double or_set_residuum(void* h, const double* norms, double* eq) {
+    Case* c = (Case*)h;
+    set_residuum(*c, std::vector<double>(norms, norms + c->Mc.nEqs));
+    for (int m = 0; m < c->Mc.nEqs; m++) eq[m] = c->eqResiduum[m];
+    return c->residuum;
+}
+// Element.l2NormsSquared of the vector R (Java layout), summed over the elements
+void or_l2_norms_sq(void* h, const double* R, double* out) {
+    Case* c = (Case*)h;
+    const int nEqs = c->Mc.nEqs;
+    for (int m = 0; m < nEqs; m++) out[m] = 0;
+    double tmp[5];
+    for (int i = 0; i < c->nElem; i++) {
+        const Elem& e = c->E[i];
+        std::vector<double> Re(R + c->off[i], R + c->off[i] + (size_t)e.nBasis * nEqs);
+        l2_norms_sq(*c, e, Re, tmp);
+        for (int m = 0; m < nEqs; m++) out[m] += tmp[m];
+    }
+}
+// Element.positivityLimiter on every element (the state of or_set_w); returns the number limited
+int or_positivity(void* h) {
+    Case* c = (Case*)h;
+    int n = 0;
+    for (auto& e : c->E) n += positivity_limit(*c, e) ? 1 : 0;
+    return n;
+}
+// blockJacobi: working copies of the diagonal blocks as at the start of a linear solve
+// (LinearSolver.solve -> factor), then applies that eliminate in place (b, x in DOF numbering)
+void or_bj_start(void* h) {
+    Case* c = (Case*)h;
+    c->bj_work.assign(c->nElem, {});
+    for (int i = 0; i < c->nElem; i++) c->bj_work[i] = c->E[i].ADiag;
+}
+void or_bj_apply(void* h, const double* b, double* x) { bj_apply(*(Case*)h, x, b); }
 void or_wall_distance(int nPts, const double* PXY, int nElem, const int* TP, const int* TT, int nF, double* out) {
     std::vector<double> d = wall_distance(nPts, PXY, nElem, TP, TT, nF);
     std::memcpy(out, d.data(), sizeof(double) * nPts);
