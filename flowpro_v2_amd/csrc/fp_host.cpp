@@ -2331,6 +2331,8 @@ static int op_residual(fp_ctx* c, const double* x, const double* b, double* y) {
     HIPCHK(c, launch_axpy(c->s, (size_t)c->dofs, -1.0, c->tmp, y));
     return FP_OK;
 }
+// blockJacobi overwrites its diagonal blocks on the first apply of a solve (blockJacobi.java:74-90)
+static bool prec_changes_on_apply(const fp_ctx* c) { return c->op != 2 && c->precond == 1; }
 static int op_prec(fp_ctx* c, double* out, const double* in) {
     if (c->op == 2) {   // the adjoint system is solved without preconditioner
         HIPCHK(c, launch_copy(c->s, (size_t)c->dofs, in, out));
@@ -2353,11 +2355,17 @@ static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_
     if ((rc = dnorm(c, c->r, &bnrm2))) return rc;
     nr = bnrm2;
     if (bnrm2 == 0) bnrm2 = 1;
-    // r = M^-1 (b - A x): every solve starts from x = 0, where b - A x is b itself, so the
-    // preconditioned residual is the vector just computed (the reference applies M^-1 twice)
+    // r = M^-1 (b - A x): every solve starts from x = 0, where b - A x is b itself, so for a
+    // preconditioner that is a fixed linear map the preconditioned residual is the vector just
+    // computed.  blockJacobi is not: its first apply eliminates the diagonal blocks in place
+    // (Mat.lsolveT), so the second apply of the reference (Gmres.java:49-57) sees the upper
+    // factors and is repeated here.
     if (!x_zero) {
         if ((rc = op_residual(c, x, b, c->aux))) return rc;
         if ((rc = op_prec(c, c->r, c->aux))) return rc;
+        if ((rc = dnorm(c, c->r, &nr))) return rc;
+    } else if (prec_changes_on_apply(c)) {
+        if ((rc = op_prec(c, c->r, b))) return rc;
         if ((rc = dnorm(c, c->r, &nr))) return rc;
     }
     error = nr / bnrm2;

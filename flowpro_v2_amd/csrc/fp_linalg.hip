@@ -79,135 +79,13 @@ __global__ void k_ilu_copy(size_t n, const double* __restrict__ src, TF* __restr
 }
 
 // ------------------------------------------------------------------ ILU factor
-// One workgroup (256 threads) per block row of the level; B is a compile-time constant
-// so that the block products unroll.  LDS: three B x B tiles.
+// The inverse of every diagonal block picks its pivots by the rule of the reference's
+// Mat.gaussian (FlowProAPI/src/main/java/flowpro/api/Mat.java:501-555): in column p the unused
+// row with the largest |M[r][p]| / s_r, s_r = max_j |M[r][j]| of the block before elimination
+// (scaled partial pivoting).  The keys are compared in FP32.
 template <typename TF>
-__device__ inline void wave_argmax(TF& v, int& idx) {
-    for (int off = 32; off > 0; off >>= 1) {
-        TF v2 = __shfl_xor(v, off);
-        int i2 = __shfl_xor(idx, off);
-        if (v2 > v || (v2 == v && i2 < idx)) { v = v2; idx = i2; }
-    }
-}
-
-template <typename TF, int B>
-__global__ void __launch_bounds__(256) k_ilu_factor_level(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
-                                                          const int* __restrict__ colidx, const int* __restrict__ diagpos,
-                                                          TF* __restrict__ F, int* __restrict__ status) {
-    constexpr int BB = B * B;
-    __shared__ TF As[BB];
-    __shared__ TF Ds[BB];
-    __shared__ TF Ls[BB > 3 * B ? BB : 3 * B];
-    __shared__ int piv;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    if ((int)blockIdx.x >= nrows) return;
-    const int i = rows[blockIdx.x];
-    const int r0 = rowptr[i], r1 = rowptr[i + 1], di = diagpos[i];
-    for (int kk = r0; kk < di; kk++) {
-        const int k = colidx[kk];
-        const TF* Aik = F + (size_t)kk * BB;
-        const TF* Dk = F + (size_t)diagpos[k] * BB;   // holds D_k^-1 after row k was factored
-        for (int q = tid; q < BB; q += nt) { As[q] = Aik[q]; Ds[q] = Dk[q]; }
-        __syncthreads();
-        // L = A_ik * D_k^-1   (column-major: X[c*B + r])
-        TF* Lg = F + (size_t)kk * BB;
-        for (int q = tid; q < BB; q += nt) {
-            const int r = q % B, c = q / B;
-            TF s0 = 0, s1 = 0;
-#pragma unroll
-            for (int m = 0; m + 1 < B; m += 2) {
-                s0 += As[m * B + r] * Ds[c * B + m];
-                s1 += As[(m + 1) * B + r] * Ds[c * B + m + 1];
-            }
-            if (B % 2) s0 += As[(B - 1) * B + r] * Ds[c * B + B - 1];
-            Ls[q] = s0 + s1;
-            Lg[q] = s0 + s1;
-        }
-        __syncthreads();
-        // A_ij -= L * U_kj for the blocks j > k present in both rows
-        for (int jj = kk + 1; jj < r1; jj++) {
-            const int j = colidx[jj];
-            int kj = -1;
-            for (int s = rowptr[k]; s < rowptr[k + 1]; s++)
-                if (colidx[s] == j) kj = s;
-            if (kj < 0) continue;
-            const TF* U = F + (size_t)kj * BB;
-            for (int q = tid; q < BB; q += nt) As[q] = U[q];
-            __syncthreads();
-            TF* C = F + (size_t)jj * BB;
-            for (int q = tid; q < BB; q += nt) {
-                const int r = q % B, c = q / B;
-                TF s0 = 0, s1 = 0;
-#pragma unroll
-                for (int m = 0; m + 1 < B; m += 2) {
-                    s0 += Ls[m * B + r] * As[c * B + m];
-                    s1 += Ls[(m + 1) * B + r] * As[c * B + m + 1];
-                }
-                if (B % 2) s0 += Ls[(B - 1) * B + r] * As[c * B + B - 1];
-                C[q] -= s0 + s1;
-            }
-            __syncthreads();
-        }
-    }
-    // D_i^-1 by Gauss-Jordan elimination with partial pivoting (the reference inverts with
-    // Mat.invert, API/Mat.java:605-642)
-    TF* M = As;
-    TF* Inv = Ds;
-    TF* Dg = F + (size_t)di * BB;
-    for (int q = tid; q < BB; q += nt) {
-        M[q] = Dg[q];
-        Inv[q] = (q % B == q / B) ? (TF)1 : (TF)0;
-    }
-    __syncthreads();
-    for (int p = 0; p < B; p++) {
-        if (tid < 64) {
-            TF v = -1;
-            int idx = p;
-            for (int r = p + tid; r < B; r += 64) {
-                TF a = fabs(M[p * B + r]);
-                if (a > v) { v = a; idx = r; }
-            }
-            wave_argmax(v, idx);
-            if (tid == 0) piv = idx;
-        }
-        __syncthreads();
-        const int pr = piv;
-        if (pr != p)
-            for (int c = tid; c < B; c += nt) {
-                TF t1 = M[c * B + p]; M[c * B + p] = M[c * B + pr]; M[c * B + pr] = t1;
-                TF t2 = Inv[c * B + p]; Inv[c * B + p] = Inv[c * B + pr]; Inv[c * B + pr] = t2;
-            }
-        __syncthreads();
-        const TF dinv = (TF)1 / M[p * B + p];
-        // multipliers of column p (row p scaled below); Ls[B + c] keeps row p of M and Inv
-        for (int r = tid; r < B; r += nt) Ls[r] = (r == p) ? (TF)0 : M[p * B + r];
-        for (int c = tid; c < B; c += nt) {
-            Ls[B + c] = M[c * B + p] * dinv;
-            Ls[2 * B + c] = Inv[c * B + p] * dinv;
-        }
-        __syncthreads();
-        for (int q = tid; q < BB; q += nt) {
-            const int r = q % B, c = q / B;
-            if (r == p) {
-                M[q] = Ls[B + c];
-                Inv[q] = Ls[2 * B + c];
-            } else {
-                const TF f = Ls[r];
-                if (f != (TF)0) {
-                    M[q] -= f * Ls[B + c];
-                    Inv[q] -= f * Ls[2 * B + c];
-                }
-            }
-        }
-        __syncthreads();
-    }
-    int bad = 0;
-    for (int q = tid; q < BB; q += nt) {
-        TF v = Inv[q];
-        if (!isfinite((double)v)) bad = 1;
-        Dg[q] = v;
-    }
-    if (bad) atomicOr(status, 1);
+__device__ inline TF row_scale_reciprocal(TF rowmax) {
+    return rowmax > (TF)0 ? (TF)1 / rowmax : (TF)0;
 }
 
 // One 64-lane wavefront per block row (B <= 64): lane r keeps row r of the blocks in
@@ -313,7 +191,7 @@ __global__ void __launch_bounds__(64) k_ilu_factor_level_w(const int* __restrict
         }
     }
     // D_i^-1: Gauss-Jordan with partial pivoting done implicitly (no row exchanges): at step p
-    // the unused row with the largest |M[.][p]| is normalised and eliminated from all other
+    // the unused row with the largest scaled |M[.][p]| is normalised and eliminated from all other
     // rows; the rows of the result are then the rows of the inverse in pivot order.  Lane r
     // holds row r of M (in a) and of the identity (in l); the pivot row reaches the other
     // lanes through readlane (uniform lane index), so the loop needs no LDS and no barrier.
@@ -325,12 +203,16 @@ __global__ void __launch_bounds__(64) k_ilu_factor_level_w(const int* __restrict
             l[c] = (c == r) ? (TF)1 : (TF)0;
         }
     }
+    TF rmax = 0;
+#pragma unroll
+    for (int c = 0; c < B; c++) rmax = fmax(rmax, fabs(a[c]));
+    const TF rsc = row_scale_reciprocal(rmax);
     bool used = !act;
     int dest = 0;
     bool singular = false;
 #pragma unroll
     for (int p = 0; p < B; p++) {
-        const float key = used ? -1.0f : (float)fabs(a[p]);
+        const float key = used ? -1.0f : (float)(fabs(a[p]) * rsc);
         float kmax = key;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) kmax = fmaxf(kmax, __shfl_xor(kmax, off));
@@ -383,6 +265,7 @@ __global__ void __launch_bounds__(256) k_ilu_factor_level_4w(const int* __restri
     __shared__ TF S[BB];
     __shared__ TF Lsh[BB];
     __shared__ TF fbuf[2][64];
+    __shared__ TF rowmax[NW][64];
     __shared__ int pivs[2];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, tid = threadIdx.x;
     const int r = lane;
@@ -477,6 +360,17 @@ __global__ void __launch_bounds__(256) k_ilu_factor_level_4w(const int* __restri
         mc[cc] = (act && c < B) ? Dg[c * B + r] : (TF)0;
         ic[cc] = (act && c == r) ? (TF)1 : (TF)0;
     }
+    {
+        TF m = 0;
+#pragma unroll
+        for (int cc = 0; cc < CW; cc++) m = fmax(m, fabs(mc[cc]));
+        rowmax[wv][r] = m;
+    }
+    __syncthreads();
+    TF rmax = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) rmax = fmax(rmax, rowmax[w][r]);
+    const TF rsc = row_scale_reciprocal(rmax);
     bool used = !act;
     int dest = 0;
     bool singular = false;
@@ -487,7 +381,7 @@ __global__ void __launch_bounds__(256) k_ilu_factor_level_4w(const int* __restri
             TF x = 0;
 #pragma unroll
             for (int cc = 0; cc < CW; cc++) x = (cc == cp) ? mc[cc] : x;
-            const float key = used ? -1.0f : (float)fabs(x);
+            const float key = used ? -1.0f : (float)(fabs(x) * rsc);
             float kmax = key;
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) kmax = fmaxf(kmax, __shfl_xor(kmax, off));
@@ -567,7 +461,7 @@ __device__ inline double pivot_reciprocal(double x) { return 1.0 / x; }
 // next task's block already in flight in registers while the current one is used.  The rows
 // of A_ik and of L_ik are exchanged between the waves through LDS.  The diagonal block is
 // then inverted by NGW of the waves (Gauss-Jordan with implicit partial pivoting; pivot row
-// = largest |M[.][p]| via a DPP wave maximum on |x| with the lane index in the low bits).
+// = largest scaled |M[.][p]| via a DPP wave maximum on the key with the lane index in the low bits).
 // Every (NW, NGW) computes every entry with the same operations in the same order, so the
 // variants give bit-identical factors and differ in time only.
 template <typename TF, int B, int MAXR, int NW, int NGW>
@@ -732,6 +626,10 @@ __global__ void __launch_bounds__(64 * NW) k_ilu_factor_rows(const int* __restri
     for (int c = 0; c < B; c++) mr[c] = act ? LA[c * B + r] : (TF)0;
 #pragma unroll
     for (int cc = 0; cc < CG; cc++) ic[cc] = (act && g0 + cc == r) ? (TF)1 : (TF)0;
+    TF rmax = 0;
+#pragma unroll
+    for (int c = 0; c < B; c++) rmax = fmax(rmax, fabs(mr[c]));
+    const TF rsc = row_scale_reciprocal(rmax);
     bool used = !act;
     int dest = 0;
     bool singular = false;
@@ -740,7 +638,7 @@ __global__ void __launch_bounds__(64 * NW) k_ilu_factor_rows(const int* __restri
     // reciprocal of its own pivot at the end
 #pragma unroll
     for (int p = 0; p < B; p++) {
-        const unsigned mag = __float_as_uint(fabsf((float)mr[p])) & ~63u;
+        const unsigned mag = __float_as_uint(fabsf((float)(mr[p] * rsc))) & ~63u;
         const int key = used ? 0 : (int)(mag | (unsigned)(63 - lane));
         const int kmax = wave_max_key(key);
         const int pr = 63 - (kmax & 63);

@@ -114,15 +114,15 @@ def test_linear_solve_matches_oracle(prec):
 
 
 # (name, case, props, tolerance with FP64 factors, with FP32 factors).  The differences are
-# rounding of the block inverses (pivot order) amplified by the conditioning of the system:
-# cond(A) ~ 6e2 for the NS cases, ~1e8 for the SA plate (wall-distance scaling of the SA
-# destruction term, diagonal blocks up to cond 5e3).  u64 cond(A) ~ 1e-8 bounds the SA case in
-# FP64; in FP32 (u32 cond(A) ~ 6) the SA factor is a different approximate preconditioner and no
-# parity with the FP64 oracle is expected.
+# rounding of the block inverses amplified by the conditioning of the system: cond(A) ~ 6e2 for
+# the NS cases, ~1e8 for the SA plate (wall-distance scaling of the SA destruction term,
+# diagonal blocks up to cond 5e3).  u64 cond(A) ~ 1e-8 bounds the SA case in FP64; in FP32 every
+# block inverse carries u32 cond(D) ~ 3e-4, compounded along the factorisation's chains
+# (measured 6.3e-3 with the reference's scaled pivoting), hence 2e-2.
 ILU_CASES = [
     ("B24", lambda: cases.cylinder_mesh(16, 5), lambda **kw: cases.ns_props(order=3, reynolds=300.0, **kw), 1e-10, 1e-5),
     ("B40", lambda: cases.cylinder_mesh(16, 5), lambda **kw: cases.ns_props(order=4, reynolds=300.0, **kw), 1e-10, 1e-5),
-    ("B30", lambda: cases.plate_channel(12, 5), lambda **kw: cases.sa_props(order=3, reynolds=1e5, **kw), 1e-8, None),
+    ("B30", lambda: cases.plate_channel(12, 5), lambda **kw: cases.sa_props(order=3, reynolds=1e5, **kw), 1e-8, 2e-2),
 ]
 
 
@@ -134,8 +134,6 @@ def test_block_ilu0_first_krylov_direction(name, mk_case, mk_props, tol64, tol32
     the block-ILU0 factor and both sweeps (M^-1 b) at the NS block sizes B = 24 and 40, for
     both factor kernels, against the oracle's sequential BlockILU0 (B = 30: Spalart-Allmaras p2).
     Tolerances per case in ILU_CASES."""
-    if bits == 32 and tol32 is None:
-        pytest.skip("FP32 factors of the SA plate system (cond ~1e8) are not comparable with the FP64 oracle")
     monkeypatch.setenv("FLOWPRO_ILU_ROWS", rows_kernel)
     case = mk_case()
     props = mk_props(gmresRestart=1, gmresCycles=1, iterativeSolverTol=1e-14)
