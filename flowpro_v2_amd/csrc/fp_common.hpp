@@ -152,6 +152,11 @@ struct ElemLaunch {
 struct IluIO {
     const double* b;
     double* x;
+    // dataflow sweeps: the application's number (tags the published rows), the row tickets of
+    // the forward and the backward sweep, and a flag set when a wait exceeded its spin limit
+    unsigned epoch;
+    unsigned ticket[2];
+    int err;
 };
 
 struct DGParams {

@@ -67,6 +67,9 @@ void ilu_factor_rows_shape(int variant, int* nw, int* ngw);
 template <typename TF>
 hipError_t launch_ilu_fwd_level(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);
 template <typename TF>
+hipError_t launch_ilu_sweep_flow(hipStream_t, int, int, const int*, int, const TF*, IluIO*, const unsigned long long*,
+                                 unsigned long long*, int);
+template <typename TF>
 hipError_t launch_ilu_bwd_level(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*);
 hipError_t launch_set_io(hipStream_t, IluIO*, const double*, double*);
 hipError_t launch_block_jacobi(hipStream_t, int, int, double*, const double*, double*);
@@ -150,6 +153,10 @@ struct fp_ctx {
     double* val = nullptr;        // FP64 matrix (nnzb * B * B)
     void* F = nullptr;            // preconditioner factors (FP64 or FP32)
     void *iy = nullptr, *ixs = nullptr;
+    // dataflow sweeps: published rows {value bits, epoch} of y and x (FP64: two words a value)
+    unsigned long long *ypk = nullptr, *xpk = nullptr;
+    bool ilu_flow = true;                     // one dataflow launch per sweep (else one launch per level)
+    int flow_groups = 1024;                   // workgroups of the dataflow sweeps
     double* bjw = nullptr;        // block Jacobi working copies
     double *W = nullptr, *Wo = nullptr, *Wo2 = nullptr, *R = nullptr, *RHS = nullptr, *enorm = nullptr, *tmp = nullptr;
     double *x = nullptr, *V = nullptr, *w = nullptr, *r = nullptr, *aux = nullptr, *partial = nullptr, *scal = nullptr;
@@ -732,7 +739,7 @@ static void free_all(fp_ctx* c) {
     if (c->g_apply) (void)hipGraphExecDestroy(c->g_apply);
     if (c->io) (void)hipFree(c->io);
     void* ptrs[] = {c->m_nvd, c->m_nqvd, c->m_nbd, c->m_nqfd, c->m_vol, c->m_face, c->qv, c->qf, c->qn, c->xq, c->qmass, c->qminv, c->qis, c->av, c->avold, c->trunc, c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
-                    c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->val, c->F, c->iy, c->ixs, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
+                    c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->val, c->F, c->iy, c->ixs, c->ypk, c->xpk, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
                     c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->istat};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1939,6 +1946,11 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         HIPCHK(c, hipMalloc(&c->F, (size_t)c->nnzb * BB * es));
         HIPCHK(c, hipMalloc(&c->iy, N * es));
         HIPCHK(c, hipMalloc(&c->ixs, N * es));
+        const size_t words = N * (es / 4);
+        HIPCHK(c, hipMalloc(&c->ypk, words * sizeof(unsigned long long)));
+        HIPCHK(c, hipMalloc(&c->xpk, words * sizeof(unsigned long long)));
+        HIPCHK(c, hipMemsetAsync(c->ypk, 0, words * sizeof(unsigned long long), c->s));
+        HIPCHK(c, hipMemsetAsync(c->xpk, 0, words * sizeof(unsigned long long), c->s));
     } else if (c->precond == 1) {
         HIPCHK(c, dalloc(&c->bjw, (size_t)n * BB));
     }
@@ -1952,9 +1964,14 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     HIPCHK(c, dalloc(&c->y, (size_t)c->m + 1));
     HIPCHK(c, dalloc(&c->istat, 4));
     HIPCHK(c, hipMalloc((void**)&c->io, sizeof(IluIO)));
+    HIPCHK(c, hipMemsetAsync(c->io, 0, sizeof(IluIO), c->s));
     {
         const char* g = std::getenv("FLOWPRO_GRAPH");
         c->use_graph = !(g && g[0] == '0');
+        const char* f = std::getenv("FLOWPRO_ILU_FLOW");   // "0": one sweep launch per level
+        c->ilu_flow = !(f && f[0] == '0');
+        const char* fg = std::getenv("FLOWPRO_FLOW_GROUPS");
+        if (fg && std::atoi(fg) > 0) c->flow_groups = std::atoi(fg);
     }
     HIPCHK(c, hipMemsetAsync(c->W, 0, N * sizeof(double), c->s));
     HIPCHK(c, hipMemsetAsync(c->Wo, 0, N * sizeof(double), c->s));
@@ -2170,6 +2187,11 @@ template <typename TF> static int ilu_level_launches(fp_ctx* c) {
 }
 template <typename TF> static int ilu_apply_launches(fp_ctx* c) {
     const TF* F = (const TF*)c->F;
+    if (c->ilu_flow) {
+        HIPCHK(c, launch_ilu_sweep_flow<TF>(c->s, c->B, 0, c->meta_f, c->nElem, F, c->io, nullptr, c->ypk, c->flow_groups));
+        HIPCHK(c, launch_ilu_sweep_flow<TF>(c->s, c->B, 1, c->meta_b, c->nElem, F, c->io, c->ypk, c->xpk, c->flow_groups));
+        return FP_OK;
+    }
     for (size_t l = 0; l + 1 < c->lvl_f_ptr.size(); l++) {
         int b = c->lvl_f_ptr[l], e = c->lvl_f_ptr[l + 1];
         HIPCHK(c, launch_ilu_fwd_level<TF>(c->s, c->B, c->meta_f + 16 * (size_t)b, e - b, F, c->io, (TF*)c->iy));
@@ -2259,6 +2281,21 @@ template <typename TF> static int ilu_factor_t(fp_ctx* c) {
 template <typename TF> static int ilu_apply_t(fp_ctx* c, double* out, const double* in) {
     HIPCHK(c, launch_set_io(c->s, c->io, in, out));
     return run_captured<TF>(c, &c->g_apply, ilu_apply_launches<TF>);
+}
+
+// a wait of a dataflow sweep that exceeded its spin limit (a broken dependency order) voids the
+// applications of the solve
+static int check_sweep_waits(fp_ctx* c) {
+    if (c->precond != 2 || !c->ilu_flow) return FP_OK;
+    int e = 0;
+    HIPCHK(c, hipMemcpyAsync(&e, &c->io->err, sizeof(int), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (e) {
+        HIPCHK(c, hipMemsetAsync(&c->io->err, 0, sizeof(int), c->s));
+        c->err = "block-ILU0 sweep: a dependency wait exceeded its spin limit";
+        return FP_EHIP;
+    }
+    return FP_OK;
 }
 
 static int prec_factor(fp_ctx* c) {
@@ -2455,6 +2492,7 @@ static int linear_solve_impl(fp_ctx* c, int op, int* iterations, double* rel_res
     rc = gmres(c, c->RHS, c->x, &it, &err, &ok, true);
     c->op = 0;
     if (rc) return rc;
+    if ((rc = check_sweep_waits(c))) return rc;
     if (iterations) *iterations = it;
     if (rel_residual) *rel_residual = err;
     HIPCHK(c, hipMemsetAsync(c->istat, 0, sizeof(int), c->s));
@@ -2721,6 +2759,8 @@ extern "C" int fp_time_kernel(fp_ctx* c, int which, int reps, double* avg_ms) {
     for (int i = 0; i < reps && rc == FP_OK; i++) rc = once();
     HIPCHK(c, hipEventRecord(e1, c->s));
     HIPCHK(c, hipEventSynchronize(e1));
+    if (rc == FP_OK && which == 4) rc = check_sweep_waits(c);
+    if (rc) return rc;
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);

@@ -13,6 +13,7 @@
 // of every row is the reference's).  Factors in FP64 or FP32 (TF).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "fp_common.hpp"
@@ -798,6 +799,189 @@ __global__ void __launch_bounds__(64) k_ilu_bwd_level(const int* __restrict__ me
 __global__ void k_set_io(IluIO* io, const double* b, double* x) {
     io->b = b;
     io->x = x;
+    io->epoch += 1;
+    io->ticket[0] = 0;
+    io->ticket[1] = 0;
+}
+
+// ------------------------------------------------------------------ dataflow sweeps
+// One launch per sweep direction instead of one per level.  Rows are handed out in level order
+// (a topological order of the block dependencies) by a ticket counter, WPG rows per ticket, one
+// wavefront per row (lane t = row t of the block row).  A row loads its off-diagonal blocks (and
+// for the backward sweep its row of D_i^-1) first, then waits for each neighbour row it needs,
+// multiplies, and publishes its own result.  Published values carry the application's number:
+// every value is stored as 64-bit words {32 value bits, epoch} (FP64: two words, low and high
+// half), written and polled with single-copy atomic accesses at agent scope, so a reader sees
+// a value and its tag together and needs no fence or separate flag.  Deadlock freedom: a row
+// waits only for rows earlier in the ticket order; those were handed to workgroups that are
+// already running, so the earliest unfinished row can always proceed, whatever the number of
+// resident workgroups.  Every wait is bounded: past FLOW_SPIN_LIMIT polls the row sets io->err
+// and goes on (the result is then discarded by the host), so every wave reaches the exit.
+// The arithmetic per row is that of k_ilu_fwd_level / k_ilu_bwd_level: four partial sums over
+// the columns of each block, combined as (a0 + a1) + (a2 + a3).
+constexpr int FLOW_SPIN_LIMIT = 1 << 22;
+
+template <typename TF> struct FlowWords { static constexpr int value = sizeof(TF) / 4; };
+
+__device__ inline unsigned long long flow_load(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void flow_store(unsigned long long* p, unsigned long long w) {
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// publish v (lane t < B) as row r's value t
+__device__ inline void flow_publish(unsigned long long* buf, size_t idx, float v, unsigned epoch) {
+    flow_store(buf + idx, ((unsigned long long)epoch << 32) | (unsigned)__float_as_uint(v));
+}
+__device__ inline void flow_publish(unsigned long long* buf, size_t idx, double v, unsigned epoch) {
+    const unsigned long long x = (unsigned long long)__double_as_longlong(v);
+    flow_store(buf + 2 * idx, ((unsigned long long)epoch << 32) | (x & 0xffffffffull));
+    flow_store(buf + 2 * idx + 1, ((unsigned long long)epoch << 32) | (x >> 32));
+}
+// value t of a published row, read without waiting (the row is complete)
+__device__ inline float flow_value(const unsigned long long* buf, size_t idx, float) {
+    return __uint_as_float((unsigned)(buf[idx] & 0xffffffffull));
+}
+__device__ inline double flow_value(const unsigned long long* buf, size_t idx, double) {
+    return __longlong_as_double((long long)((buf[2 * idx] & 0xffffffffull) | (buf[2 * idx + 1] << 32)));
+}
+
+// Waits until lanes 0..B-1 see row `row`'s values of this epoch; returns the raw value bits of
+// lane t (two halves for FP64).
+template <typename TF, int B>
+__device__ inline void flow_wait(const unsigned long long* buf, int row, unsigned epoch, int t, unsigned* lo, unsigned* hi,
+                                 int* err) {
+    constexpr int WD = FlowWords<TF>::value;
+    // lanes t >= B poll the row's last value (in bounds, result unused): no divergent branch
+    const unsigned long long* p = buf + ((size_t)row * B + (t < B ? t : B - 1)) * WD;
+    for (int spin = 0;; spin++) {
+        const unsigned long long w0 = flow_load(p);
+        *lo = (unsigned)(w0 & 0xffffffffull);
+        bool ok = (unsigned)(w0 >> 32) == epoch;
+        if constexpr (WD == 2) {
+            const unsigned long long w1 = flow_load(p + 1);
+            *hi = (unsigned)(w1 & 0xffffffffull);
+            ok = ok && (unsigned)(w1 >> 32) == epoch;
+        }
+        if (__all(ok || t >= B)) return;
+        // after a wait has failed anywhere, nothing waits any more: the launch drains
+        if ((spin & 1023) == 1023 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+        if (spin >= FLOW_SPIN_LIMIT) {
+            if (t == 0) atomicOr(err, 1);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+// value of lane c (compile-time) from the bits returned by flow_wait
+template <typename TF> __device__ inline TF flow_lane(unsigned lo, unsigned hi, int c);
+template <> __device__ inline float flow_lane<float>(unsigned lo, unsigned, int c) {
+    return __uint_as_float((unsigned)__builtin_amdgcn_readlane((int)lo, c));
+}
+template <> __device__ inline double flow_lane<double>(unsigned lo, unsigned hi, int c) {
+    const unsigned l = (unsigned)__builtin_amdgcn_readlane((int)lo, c);
+    const unsigned h = (unsigned)__builtin_amdgcn_readlane((int)hi, c);
+    return __longlong_as_double((long long)(((unsigned long long)h << 32) | l));
+}
+
+// blocks of a row prefetched into registers before the first wait: about 160 VGPRs
+template <typename TF, int B> struct FlowPrefetch {
+    static constexpr int v = (160 * 4) / (B * (int)sizeof(TF));
+    static constexpr int value = v < 1 ? 1 : (v > 4 ? 4 : v);
+};
+
+// s -= (a0 + a1) + (a2 + a3) of one block row product: lane t holds row t of the block in Lr,
+// the vector is read from lane c of (lo, hi)
+template <typename TF, int B>
+__device__ inline TF flow_block_product(const TF (&Lr)[B], unsigned lo, unsigned hi) {
+    TF a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+    for (int c = 0; c + 3 < B; c += 4) {
+        a0 += Lr[c] * flow_lane<TF>(lo, hi, c);
+        a1 += Lr[c + 1] * flow_lane<TF>(lo, hi, c + 1);
+        a2 += Lr[c + 2] * flow_lane<TF>(lo, hi, c + 2);
+        a3 += Lr[c + 3] * flow_lane<TF>(lo, hi, c + 3);
+    }
+#pragma unroll
+    for (int c = B - B % 4; c < B; c++) a0 += Lr[c] * flow_lane<TF>(lo, hi, c);
+    return (a0 + a1) + (a2 + a3);
+}
+
+// row t of a block into registers; lanes t >= B read row B-1 (in bounds; their results are
+// never stored), which keeps the loads free of divergent branches
+template <typename TF, int B>
+__device__ inline void flow_load_row(TF (&Lr)[B], const TF* blk, int t) {
+    const int tc = t < B ? t : B - 1;
+#pragma unroll
+    for (int c = 0; c < B; c++) Lr[c] = blk[(size_t)c * B + tc];
+}
+
+// forward: y_i = b_i - sum_{k<i} L_ik y_k  (dir 0), backward: x_i = D_i^-1 (y_i - sum_{j>i} U_ij x_j)
+// (dir 1; y read from the forward sweep's published values, complete when this launch starts)
+template <typename TF, int B, int WPG, int DIR>
+__global__ void __launch_bounds__(64 * WPG) k_ilu_sweep_flow(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
+                                                             IluIO* __restrict__ io, const unsigned long long* __restrict__ ypk,
+                                                             unsigned long long* __restrict__ out) {
+    constexpr int NPF = FlowPrefetch<TF, B>::value;
+    constexpr size_t BB = (size_t)B * B;
+    __shared__ int base_sh[2];
+    const int t = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const unsigned epoch = io->epoch;
+    const double* __restrict__ b = io->b;
+    double* __restrict__ xo = io->x;
+    for (int it = 0;; it++) {
+        if (threadIdx.x == 0) base_sh[it & 1] = (int)atomicAdd(&io->ticket[DIR], (unsigned)WPG);
+        __syncthreads();
+        const int base = base_sh[it & 1];
+        if (base >= nrows) break;
+        // wave-uniform row position: the record is then read through the scalar cache, and the
+        // block loads of one block do not wait behind those of the previous ones
+        const int li = __builtin_amdgcn_readfirstlane(base + wv);
+        if (li < nrows) {
+            const int4* m = reinterpret_cast<const int4*>(meta) + (size_t)li * 4;
+            const int4 m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+            const int i = m0.x, cnt = m0.y;
+            const int blk[ILU_MAXN] = {m1.x, m1.y, m1.z, m1.w, m2.x, m2.y};
+            const int col[ILU_MAXN] = {m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
+            TF Lr[NPF][B];
+#pragma unroll
+            for (int n = 0; n < NPF; n++)
+                if (n < cnt) flow_load_row<TF, B>(Lr[n], F + (size_t)blk[n] * BB, t);
+            TF Dr[DIR == 1 ? B : 1];
+            if constexpr (DIR == 1) flow_load_row<TF, B>(Dr, F + (size_t)m0.z * BB, t);
+            const int tc = t < B ? t : B - 1;
+            TF s = DIR == 0 ? (TF)b[(size_t)i * B + tc] : flow_value(ypk, (size_t)i * B + tc, (TF)0);
+#pragma unroll
+            for (int n = 0; n < ILU_MAXN; n++) {
+                if (n < cnt) {
+                    unsigned lo = 0, hi = 0;
+                    flow_wait<TF, B>(out, col[n], epoch, t, &lo, &hi, &io->err);
+                    if (n < NPF) {
+                        s -= flow_block_product<TF, B>(Lr[n < NPF ? n : 0], lo, hi);
+                    } else {
+                        TF Lx[B];
+                        flow_load_row<TF, B>(Lx, F + (size_t)blk[n] * BB, t);
+                        s -= flow_block_product<TF, B>(Lx, lo, hi);
+                    }
+                }
+            }
+            if constexpr (DIR == 1) {
+                // x_i = D_i^-1 s: the vector s is read from the lanes
+                unsigned lo, hi = 0;
+                if constexpr (sizeof(TF) == 4) {
+                    lo = __float_as_uint((float)s);
+                } else {
+                    const unsigned long long xb = (unsigned long long)__double_as_longlong((double)s);
+                    lo = (unsigned)(xb & 0xffffffffull);
+                    hi = (unsigned)(xb >> 32);
+                }
+                s = flow_block_product<TF, B>(Dr, lo, hi);
+                if (t < B) xo[(size_t)i * B + t] = (double)s;
+            }
+            if (t < B) flow_publish(out, (size_t)i * B + t, s, epoch);
+        }
+    }
 }
 
 // ------------------------------------------------------------------ block Jacobi
@@ -1048,12 +1232,34 @@ hipError_t launch_ilu_bwd_level(hipStream_t s, int B, const int* meta, int nrows
     return hipErrorInvalidValue;
 }
 
+template <typename TF>
+hipError_t launch_ilu_sweep_flow(hipStream_t s, int B, int dir, const int* meta, int nrows, const TF* F, IluIO* io,
+                                 const unsigned long long* ypk, unsigned long long* out, int max_groups) {
+    constexpr int WPG = 4;
+    const int groups = std::max(1, std::min((nrows + WPG - 1) / WPG, max_groups));
+#define X(BS)                                                                                                        \
+    if (B == BS) {                                                                                                   \
+        if (dir == 0)                                                                                                \
+            hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 0>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
+                               io, ypk, out);                                                                        \
+        else                                                                                                         \
+            hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 1>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
+                               io, ypk, out);                                                                        \
+        return hipGetLastError();                                                                                    \
+    }
+    FP_BLOCK_SIZES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_set_io(hipStream_t s, IluIO* io, const double* b, double* x) {
     hipLaunchKernelGGL(k_set_io, dim3(1), dim3(1), 0, s, io, b, x);
     return hipGetLastError();
 }
 
 #define FP_ILU_INST(TF)                                                                                                       \
+    template hipError_t launch_ilu_sweep_flow<TF>(hipStream_t, int, int, const int*, int, const TF*, IluIO*,                  \
+                                                  const unsigned long long*, unsigned long long*, int);                        \
     template hipError_t launch_ilu_copy<TF>(hipStream_t, size_t, const double*, TF*);                                          \
     template hipError_t launch_ilu_factor_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, \
                                                     int*);                                                                     \
