@@ -60,6 +60,9 @@ template <typename TF>
 hipError_t launch_ilu_factor_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, int*);
 template <typename TF>
 hipError_t launch_ilu_factor_rows(hipStream_t, int, int, const int*, int, const int*, const int*, const int*, const int*, TF*, int*);
+template <typename TF>
+hipError_t launch_ilu_factor_flow(hipStream_t, int, int, const int*, int, const int*, const int*, const int*, const int*, TF*, int*,
+                                  unsigned*, unsigned*, int);
 bool ilu_factor_rows_supported(int, int);
 int ilu_factor_rows_variants();
 int ilu_factor_rows_default_variant(int B);
@@ -156,6 +159,9 @@ struct fp_ctx {
     // dataflow sweeps: published rows {value bits, epoch} of y and x (FP64: two words a value)
     unsigned long long *ypk = nullptr, *xpk = nullptr;
     bool ilu_flow = true;                     // one dataflow launch per sweep (else one launch per level)
+    bool factor_flow = false;                 // one dataflow launch for the row-kernel factorisation
+    unsigned *fflag = nullptr, *fctl = nullptr;   // per-row completion epochs, {epoch, ticket}
+    int factor_groups = 2048;                 // workgroups of the dataflow factorisation
     int flow_groups = 1024;                   // workgroups of the dataflow sweeps
     double* bjw = nullptr;        // block Jacobi working copies
     double *W = nullptr, *Wo = nullptr, *Wo2 = nullptr, *R = nullptr, *RHS = nullptr, *enorm = nullptr, *tmp = nullptr;
@@ -739,7 +745,7 @@ static void free_all(fp_ctx* c) {
     if (c->g_apply) (void)hipGraphExecDestroy(c->g_apply);
     if (c->io) (void)hipFree(c->io);
     void* ptrs[] = {c->m_nvd, c->m_nqvd, c->m_nbd, c->m_nqfd, c->m_vol, c->m_face, c->qv, c->qf, c->qn, c->xq, c->qmass, c->qminv, c->qis, c->av, c->avold, c->trunc, c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
-                    c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->val, c->F, c->iy, c->ixs, c->ypk, c->xpk, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
+                    c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->val, c->F, c->iy, c->ixs, c->ypk, c->xpk, c->fflag, c->fctl, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
                     c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->istat};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1876,8 +1882,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         }
     }
     // --- ILU0 factor tasks per row, in the order of BlockILU0.factor (BlockILU0.java:101-127):
-    // for each lower k: {slot of A_ik, block of D_k^-1}, then for each later block j of row i
-    // that row k holds: {slot of A_ij | 256, block U_kj}
+    // for each lower k: {slot of A_ik | k << 9, block of D_k^-1}, then for each later block j of
+    // row i that row k holds: {slot of A_ij | 256, block U_kj}
     std::vector<int> ftptr(n + 1, 0), ftask;
     int max_row = 0;
     for (int r = 0; r < n; r++) {
@@ -1885,7 +1891,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         max_row = std::max(max_row, re - rs);
         for (int kk = rs; kk < diagpos[r]; kk++) {
             const int k = colidx[kk];
-            ftask.push_back(kk - rs);
+            ftask.push_back((kk - rs) | (k < (1 << 22) ? k << 9 : 0));
             ftask.push_back(diagpos[k]);
             for (int jj = kk + 1; jj < re; jj++) {
                 const int kj = find_blk(k, colidx[jj]);
@@ -1899,6 +1905,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     {
         const char* g = std::getenv("FLOWPRO_ILU_ROWS");   // "0" selects the per-level block kernels
         c->ilu_rows = ilu_factor_rows_supported(c->B, max_row) && !(g && g[0] == '0');
+        const char* f = std::getenv("FLOWPRO_ILU_FLOW");   // "0": one launch per level (factor and sweeps)
+        c->factor_flow = c->ilu_rows && n < (1 << 22) && !(f && f[0] == '0');
     }
 
     HIPCHK(c, upload(&c->nbr, nbr));
@@ -1951,6 +1959,10 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         HIPCHK(c, hipMalloc(&c->xpk, words * sizeof(unsigned long long)));
         HIPCHK(c, hipMemsetAsync(c->ypk, 0, words * sizeof(unsigned long long), c->s));
         HIPCHK(c, hipMemsetAsync(c->xpk, 0, words * sizeof(unsigned long long), c->s));
+        HIPCHK(c, hipMalloc(&c->fflag, (size_t)n * sizeof(unsigned)));
+        HIPCHK(c, hipMalloc(&c->fctl, 4 * sizeof(unsigned)));
+        HIPCHK(c, hipMemsetAsync(c->fflag, 0, (size_t)n * sizeof(unsigned), c->s));
+        HIPCHK(c, hipMemsetAsync(c->fctl, 0, 4 * sizeof(unsigned), c->s));
     } else if (c->precond == 1) {
         HIPCHK(c, dalloc(&c->bjw, (size_t)n * BB));
     }
@@ -1972,6 +1984,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         c->ilu_flow = !(f && f[0] == '0');
         const char* fg = std::getenv("FLOWPRO_FLOW_GROUPS");
         if (fg && std::atoi(fg) > 0) c->flow_groups = std::atoi(fg);
+        const char* ff = std::getenv("FLOWPRO_FACTOR_GROUPS");
+        if (ff && std::atoi(ff) > 0) c->factor_groups = std::atoi(ff);
     }
     HIPCHK(c, hipMemsetAsync(c->W, 0, N * sizeof(double), c->s));
     HIPCHK(c, hipMemsetAsync(c->Wo, 0, N * sizeof(double), c->s));
@@ -2174,6 +2188,12 @@ template <typename TF> static int ilu_copy_launch(fp_ctx* c) {
 }
 template <typename TF> static int ilu_level_launches(fp_ctx* c) {
     TF* F = (TF*)c->F;
+    if (c->factor_flow) {
+        HIPCHK(c, launch_ilu_factor_flow<TF>(c->s, c->B, c->ilu_variant >= 0 ? c->ilu_variant : ilu_factor_rows_default_variant(c->B),
+                                             c->lvl_f, c->nElem, c->rowptr, c->diagpos, c->ftptr, c->ftask, F, c->istat,
+                                             c->fflag, c->fctl, c->factor_groups));
+        return FP_OK;
+    }
     for (size_t l = 0; l + 1 < c->lvl_f_ptr.size(); l++) {
         int b = c->lvl_f_ptr[l], e = c->lvl_f_ptr[l + 1];
         if (c->ilu_rows)
@@ -2478,6 +2498,10 @@ static int linear_solve_impl(fp_ctx* c, int op, int* iterations, double* rel_res
     if (c->precond == 2) {
         HIPCHK(c, hipMemcpyAsync(&st, c->istat, sizeof(int), hipMemcpyDeviceToHost, c->s));
         HIPCHK(c, hipStreamSynchronize(c->s));
+        if (st & 2) {
+            c->err = "block-ILU0 factorisation: a dependency wait exceeded its spin limit";
+            return FP_EHIP;
+        }
         if (st) {
             c->err = "singular diagonal block during block ILU";
             return FP_SINGULAR_BLOCK;

@@ -465,26 +465,40 @@ __device__ inline double pivot_reciprocal(double x) { return 1.0 / x; }
 // = largest scaled |M[.][p]| via a DPP wave maximum on the key with the lane index in the low bits).
 // Every (NW, NGW) computes every entry with the same operations in the same order, so the
 // variants give bit-identical factors and differ in time only.
-template <typename TF, int B, int MAXR, int NW, int NGW>
-__global__ void __launch_bounds__(64 * NW) k_ilu_factor_rows(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
-                                                         const int* __restrict__ diagpos, const int* __restrict__ tptr,
-                                                         const int2* __restrict__ tasks, TF* __restrict__ F,
-                                                         int* __restrict__ status) {
+// Dataflow mode (one persistent launch for the whole factorisation): rows are handed out in
+// forward level order by a ticket; before the operand of the first task with a lower neighbour
+// k is fetched, the row waits until row k has published its completion (flag[k] == epoch, an
+// agent-scope acquire); after its last store a row releases at agent scope and publishes its
+// own flag.  As in the sweeps, the waits only point to rows earlier in the ticket order, so the
+// earliest unfinished row can always proceed, and every wait is bounded (status bit 2 and an
+// early exit of all later waits when a limit is hit).
+// polls of one dependency wait before it gives up (seconds of waiting; never reached when the
+// dependency order is right)
+constexpr int FLOW_SPIN_LIMIT = 1 << 22;
+
+struct FactorFlow {
+    unsigned* flag;   // per row: the epoch of the factorisation that completed it
+    unsigned* ctl;    // [0] epoch, [1] ticket
+};
+
+// the row-task record: x = slot in the row | 256 for "A_ij -= L_ik U_kj" | k << 9 for "L_ik = A_ik D_k^-1"
+__device__ inline int task_lower_row(int x) { return x >> 9; }
+
+template <typename TF, int B, int MAXR, int NW, int NGW, bool FLOW>
+__device__ __attribute__((always_inline)) inline void factor_row(
+    int i, const int* __restrict__ rowptr, const int* __restrict__ diagpos, const int* __restrict__ tptr,
+    const int2* __restrict__ tasks, TF* __restrict__ F, int* __restrict__ status, FactorFlow fl, unsigned epoch,
+    TF (&S)[2][B * B], TF (&LA)[B * B], TF (&LL)[B * B]) {
     constexpr int BB = B * B;
     constexpr int NT = 64 * NW;
     constexpr int CW = (B + NW - 1) / NW;       // columns per wave in the tasks
     constexpr int NG = NGW < NW ? NGW : NW;     // waves of the inversion
     constexpr int CG = (B + NG - 1) / NG;       // identity columns per inversion wave
     constexpr int PER = (BB + NT - 1) / NT;
-    __shared__ TF S[2][BB];
-    __shared__ TF LA[BB];
-    __shared__ TF LL[BB];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, tid = threadIdx.x;
     const int r = lane;
     const bool act = r < B;
     const int c0 = wv * CW;
-    if ((int)blockIdx.x >= nrows) return;
-    const int i = rows[blockIdx.x];
     const int rs = rowptr[i], nb = rowptr[i + 1] - rs, qd = diagpos[i] - rs;
     const int t0 = tptr[i], nt = tptr[i + 1] - t0;
 
@@ -498,7 +512,28 @@ __global__ void __launch_bounds__(64 * NW) k_ilu_factor_rows(const int* __restri
         }
     TF pf[PER];
     auto load_item = [&](int t) {
-        const TF* src = F + (size_t)tasks[t0 + t].y * BB;
+        const int2 tk = tasks[t0 + t];
+        if constexpr (FLOW) {
+            if (!(tk.x & 256)) {
+                // first task with row k: wave 0 waits until row k is complete and acquires at agent
+                // scope; the workgroup barrier then orders the other waves' loads after it
+                if (wv == 0) {
+                    const unsigned* f = fl.flag + task_lower_row(tk.x);
+                    for (int spin = 0;; spin++) {
+                        if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) break;
+                        if ((spin & 1023) == 1023 && (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 2)) break;
+                        if (spin >= FLOW_SPIN_LIMIT) {
+                            if (tid == 0) atomicOr(status, 2);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                }
+                __syncthreads();
+            }
+        }
+        const TF* src = F + (size_t)tk.y * BB;
 #pragma unroll
         for (int u = 0; u < PER; u++) {
             const int q = tid + NT * u;
@@ -620,55 +655,101 @@ __global__ void __launch_bounds__(64 * NW) k_ilu_factor_rows(const int* __restri
         }
     }
     lds_barrier();
-    if (wv >= NG) return;
-    const int g0 = wv * CG;
-    TF mr[B], ic[CG];
+    if (wv < NG) {
+        const int g0 = wv * CG;
+        TF mr[B], ic[CG];
 #pragma unroll
-    for (int c = 0; c < B; c++) mr[c] = act ? LA[c * B + r] : (TF)0;
+        for (int c = 0; c < B; c++) mr[c] = act ? LA[c * B + r] : (TF)0;
 #pragma unroll
-    for (int cc = 0; cc < CG; cc++) ic[cc] = (act && g0 + cc == r) ? (TF)1 : (TF)0;
-    TF rmax = 0;
+        for (int cc = 0; cc < CG; cc++) ic[cc] = (act && g0 + cc == r) ? (TF)1 : (TF)0;
+        TF rmax = 0;
 #pragma unroll
-    for (int c = 0; c < B; c++) rmax = fmax(rmax, fabs(mr[c]));
-    const TF rsc = row_scale_reciprocal(rmax);
-    bool used = !act;
-    int dest = 0;
-    bool singular = false;
-    TF myd = 0;
-    // elimination without normalising the pivot row: row r of the result is scaled by the
-    // reciprocal of its own pivot at the end
+        for (int c = 0; c < B; c++) rmax = fmax(rmax, fabs(mr[c]));
+        const TF rsc = row_scale_reciprocal(rmax);
+        bool used = !act;
+        int dest = 0;
+        bool singular = false;
+        TF myd = 0;
+        // elimination without normalising the pivot row: row r of the result is scaled by the
+        // reciprocal of its own pivot at the end
 #pragma unroll
-    for (int p = 0; p < B; p++) {
-        const unsigned mag = __float_as_uint(fabsf((float)(mr[p] * rsc))) & ~63u;
-        const int key = used ? 0 : (int)(mag | (unsigned)(63 - lane));
-        const int kmax = wave_max_key(key);
-        const int pr = 63 - (kmax & 63);
-        singular = singular || (kmax >> 6) == 0;
-        const TF d = pivot_reciprocal(lane_read(mr[p], pr));
-        const bool isp = r == pr;
-        const TF beta = isp ? (TF)0 : -mr[p] * d;   // 0 on lanes r >= B
+        for (int p = 0; p < B; p++) {
+            const unsigned mag = __float_as_uint(fabsf((float)(mr[p] * rsc))) & ~63u;
+            const int key = used ? 0 : (int)(mag | (unsigned)(63 - lane));
+            const int kmax = wave_max_key(key);
+            const int pr = 63 - (kmax & 63);
+            singular = singular || (kmax >> 6) == 0;
+            const TF d = pivot_reciprocal(lane_read(mr[p], pr));
+            const bool isp = r == pr;
+            const TF beta = isp ? (TF)0 : -mr[p] * d;   // 0 on lanes r >= B
 #pragma unroll
-        for (int c = p + 1; c < B; c++) mr[c] = fma(beta, lane_read(mr[c], pr), mr[c]);
+            for (int c = p + 1; c < B; c++) mr[c] = fma(beta, lane_read(mr[c], pr), mr[c]);
 #pragma unroll
-        for (int cc = 0; cc < CG; cc++) ic[cc] = fma(beta, lane_read(ic[cc], pr), ic[cc]);
-        used = used || isp;
-        dest = isp ? p : dest;
-        myd = isp ? d : myd;
-    }
-    if (act) {
-        int bad = singular ? 1 : 0;
-        TF* Dg = F + (size_t)(rs + qd) * BB;
-#pragma unroll
-        for (int cc = 0; cc < CG; cc++) {
-            const int c = g0 + cc;
-            if (c < B) {
-                const TF v = ic[cc] * myd;
-                if (!isfinite((double)v)) bad = 1;
-                Dg[c * B + dest] = v;
-            }
+            for (int cc = 0; cc < CG; cc++) ic[cc] = fma(beta, lane_read(ic[cc], pr), ic[cc]);
+            used = used || isp;
+            dest = isp ? p : dest;
+            myd = isp ? d : myd;
         }
-        if (bad) atomicOr(status, 1);
+        if (act) {
+            int bad = singular ? 1 : 0;
+            TF* Dg = F + (size_t)(rs + qd) * BB;
+#pragma unroll
+            for (int cc = 0; cc < CG; cc++) {
+                const int c = g0 + cc;
+                if (c < B) {
+                    const TF v = ic[cc] * myd;
+                    if (!isfinite((double)v)) bad = 1;
+                    Dg[c * B + dest] = v;
+                }
+            }
+            if (bad) atomicOr(status, 1);
+        }
     }
+    if constexpr (FLOW) {
+        __syncthreads();   // every store of the row is complete
+        if (tid == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(fl.flag + i, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template <typename TF, int B, int MAXR, int NW, int NGW>
+__global__ void __launch_bounds__(64 * NW) k_ilu_factor_rows(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
+                                                         const int* __restrict__ diagpos, const int* __restrict__ tptr,
+                                                         const int2* __restrict__ tasks, TF* __restrict__ F,
+                                                         int* __restrict__ status) {
+    __shared__ TF S[2][B * B];
+    __shared__ TF LA[B * B];
+    __shared__ TF LL[B * B];
+    if ((int)blockIdx.x >= nrows) return;
+    factor_row<TF, B, MAXR, NW, NGW, false>(rows[blockIdx.x], rowptr, diagpos, tptr, tasks, F, status, FactorFlow{}, 0u, S,
+                                           LA, LL);
+}
+
+// the whole factorisation in one launch: rows[] = all rows in forward level order
+template <typename TF, int B, int MAXR, int NW, int NGW>
+__global__ void __launch_bounds__(64 * NW) k_ilu_factor_flow(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
+                                                         const int* __restrict__ diagpos, const int* __restrict__ tptr,
+                                                         const int2* __restrict__ tasks, TF* __restrict__ F,
+                                                         int* __restrict__ status, FactorFlow fl) {
+    __shared__ TF S[2][B * B];
+    __shared__ TF LA[B * B];
+    __shared__ TF LL[B * B];
+    __shared__ int pos_sh[2];
+    const unsigned epoch = fl.ctl[0];
+    for (int it = 0;; it++) {
+        if (threadIdx.x == 0) pos_sh[it & 1] = (int)atomicAdd(fl.ctl + 1, 1u);
+        __syncthreads();
+        const int pos = pos_sh[it & 1];
+        if (pos >= nrows) break;
+        factor_row<TF, B, MAXR, NW, NGW, true>(rows[pos], rowptr, diagpos, tptr, tasks, F, status, fl, epoch, S, LA, LL);
+    }
+}
+
+__global__ void k_factor_flow_begin(unsigned* ctl) {
+    ctl[0] += 1;
+    ctl[1] = 0;
 }
 
 // ------------------------------------------------------------------ ILU sweeps
@@ -819,8 +900,6 @@ __global__ void k_set_io(IluIO* io, const double* b, double* x) {
 // and goes on (the result is then discarded by the host), so every wave reaches the exit.
 // The arithmetic per row is that of k_ilu_fwd_level / k_ilu_bwd_level: four partial sums over
 // the columns of each block, combined as (a0 + a1) + (a2 + a3).
-constexpr int FLOW_SPIN_LIMIT = 1 << 22;
-
 template <typename TF> struct FlowWords { static constexpr int value = sizeof(TF) / 4; };
 
 __device__ inline unsigned long long flow_load(const unsigned long long* p) {
@@ -1197,6 +1276,30 @@ hipError_t launch_ilu_factor_rows(hipStream_t s, int B, int variant, const int* 
     return hipErrorInvalidValue;
 }
 
+template <typename TF>
+hipError_t launch_ilu_factor_flow(hipStream_t s, int B, int variant, const int* rows, int nrows, const int* rowptr,
+                                  const int* diagpos, const int* tptr, const int* tasks, TF* F, int* status,
+                                  unsigned* flag, unsigned* ctl, int max_groups) {
+    const int groups = std::max(1, std::min(nrows, max_groups));
+    const FactorFlow fl{flag, ctl};
+    hipLaunchKernelGGL(k_factor_flow_begin, dim3(1), dim3(1), 0, s, ctl);
+#define Y(BS, MR, V, NW_, NG_)                                                                                          \
+    if (variant == V) {                                                                                                 \
+        hipLaunchKernelGGL((k_ilu_factor_flow<TF, BS, MR, NW_, NG_>), dim3(groups), dim3(64 * NW_), 0, s, rows, nrows,  \
+                           rowptr, diagpos, tptr, reinterpret_cast<const int2*>(tasks), F, status, fl);                \
+        return hipGetLastError();                                                                                       \
+    }
+#define X(BS, MR)                                  \
+    if (B == BS) {                                 \
+        FP_ROWREG_VARIANTS(Y, BS, MR)              \
+        return hipErrorInvalidValue;               \
+    }
+    FP_ROWREG_SIZES(X)
+#undef X
+#undef Y
+    return hipErrorInvalidValue;
+}
+
 bool ilu_factor_rows_supported(int B, int max_row_blocks) {
 #define X(BS, MR) if (B == BS) return max_row_blocks <= MR;
     FP_ROWREG_SIZES(X)
@@ -1263,6 +1366,8 @@ hipError_t launch_set_io(hipStream_t s, IluIO* io, const double* b, double* x) {
     template hipError_t launch_ilu_copy<TF>(hipStream_t, size_t, const double*, TF*);                                          \
     template hipError_t launch_ilu_factor_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, \
                                                     int*);                                                                     \
+    template hipError_t launch_ilu_factor_flow<TF>(hipStream_t, int, int, const int*, int, const int*, const int*, const int*, \
+                                                   const int*, TF*, int*, unsigned*, unsigned*, int);                          \
     template hipError_t launch_ilu_factor_rows<TF>(hipStream_t, int, int, const int*, int, const int*, const int*, const int*, \
                                                    const int*, TF*, int*);                                                     \
     template hipError_t launch_ilu_fwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);             \
