@@ -12,6 +12,7 @@
 #include <deque>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <unordered_map>
@@ -78,6 +79,7 @@ hipError_t launch_set_io(hipStream_t, IluIO*, const double*, double*);
 hipError_t launch_block_jacobi(hipStream_t, int, int, double*, const double*, double*);
 hipError_t launch_gather_diag(hipStream_t, int, int, const int*, const double*, double*);
 hipError_t launch_min(hipStream_t, size_t, const double*, double*, double*);
+int min_reduction_partials();
 hipError_t launch_colsum(hipStream_t, int, int, const double*, double*, double*);
 bool quad_supported(const ElemLaunch&);
 bool mixed_supported(const ElemLaunch&);
@@ -116,6 +118,7 @@ using namespace fp;
 
 struct fp_ctx {
     std::string err;
+    int device = 0;                 // the GPU the context lives on
     // sizes
     int nElem = 0, order = 0, NB = 0, NEQ = 0, B = 0, NQV = 0, NQF = 0;
     int dim = 2, NF = 3;            // triangles (3 faces), quadrilaterals or tetrahedra (4 faces)
@@ -230,11 +233,12 @@ static hipError_t run_jtv(fp_ctx* c, const double* v, double a1, double* out) {
     if (c->dim == 3) return launch_jacobian_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, nullptr, v, out, 1);
     return launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, nullptr, v, out, 1);
 }
-static hipError_t run_size_over_lambda(fp_ctx* c) {
-    if (c->mixed) return launch_size_over_lambda_mixed(c->L, c->s, c->mp, c->mms, c->W, c->esize, c->tmp);
-    if (c->quad) return launch_size_over_lambda_quad(c->L, c->s, c->mp, c->qms, c->W, c->esize, c->tmp);
-    if (c->dim == 3) return launch_size_over_lambda_tet(c->L, c->s, c->mp, c->tms, c->W, c->esize, c->isref, c->tmp);
-    return launch_size_over_lambda(c->L, c->s, c->mp, c->tb, c->ms, c->W, c->esize, c->isref, c->tmp);
+// elemSize / lambda_max of every element into out[nElem]
+static hipError_t run_size_over_lambda(fp_ctx* c, double* out) {
+    if (c->mixed) return launch_size_over_lambda_mixed(c->L, c->s, c->mp, c->mms, c->W, c->esize, out);
+    if (c->quad) return launch_size_over_lambda_quad(c->L, c->s, c->mp, c->qms, c->W, c->esize, out);
+    if (c->dim == 3) return launch_size_over_lambda_tet(c->L, c->s, c->mp, c->tms, c->W, c->esize, c->isref, out);
+    return launch_size_over_lambda(c->L, c->s, c->mp, c->tb, c->ms, c->W, c->esize, c->isref, out);
 }
 static hipError_t run_positivity(fp_ctx* c, int* count) {
     if (c->mixed) return launch_positivity_mixed(c->L, c->s, c->mms, c->W, count);
@@ -1642,6 +1646,36 @@ static int setup_tet(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
     return FP_OK;
 }
 
+// ============================================================== allocation-free time step
+// fp_time_step_noalloc works in `scratch`, a file-scope device array that every GPU the library
+// runs on holds until the process ends, instead of the context's buffers: per-element values
+// [0, nElem), the block minima [nElem, nElem + P), the result at nElem + P (P = the reduction's
+// block count).  Host code never passes the array's name to a kernel: on the host it is only a
+// stand-in for the symbol; the device address of each GPU's copy is taken once per device with
+// hipGetSymbolAddress (at the first fp_create on that device) and kept in g_scratch.
+constexpr size_t FP_SCRATCH_BYTES = size_t(32) << 20;   // 32 MiB: up to ~4.19 M elements
+constexpr int FP_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[FP_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch[FP_MAX_DEVICES] = {};
+
+static int scratch_setup(int device, std::string* err) {
+    if (device < 0 || device >= FP_MAX_DEVICES) {
+        *err = "device index beyond the scratch table";
+        return FP_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return FP_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));   // the current device's copy
+    if (e != hipSuccess || !p) {
+        *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return FP_EHIP;
+    }
+    g_scratch[device] = p;
+    return FP_OK;
+}
+
 static int create_impl(const fp_case* cs, fp_ctx* c) {
     const int n = cs->n_elems;
     c->nElem = n;
@@ -1798,6 +1832,11 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     c->avp.order = cs->order;
     c->dg.cIP = (cs->order > 1 && diffusive) ? cs->penalty : 0.0;   // DGpure.limiter :396-398
 
+    HIPCHK(c, hipGetDevice(&c->device));
+    {
+        const int rc = scratch_setup(c->device, &c->err);
+        if (rc) return rc;
+    }
     HIPCHK(c, hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     std::vector<int> nbr;
     int rc = c->dim == 3 ? setup_tet(cs, c, nbr) : c->mixed ? setup_mixed(cs, c, nbr) : c->quad ? setup_quad(cs, c, nbr) : setup_tri(cs, c, nbr);
@@ -2124,9 +2163,37 @@ extern "C" int fp_restore_level(fp_ctx* c) {
 // ============================================================== Newton step
 extern "C" int fp_time_step(fp_ctx* c, double* out) {
     if (!c || !out) return FP_EINVAL;
-    HIPCHK(c, run_size_over_lambda(c));
+    HIPCHK(c, run_size_over_lambda(c, c->tmp));
     HIPCHK(c, launch_min(c->s, (size_t)c->nElem, c->tmp, c->partial, c->scal));
     HIPCHK(c, hipMemcpyAsync(out, c->scal, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return FP_OK;
+}
+
+extern "C" int fp_time_step_noalloc(fp_ctx* c, double* out) {
+    if (!c || !out) return FP_EINVAL;
+    const size_t P = (size_t)min_reduction_partials();
+    const size_t need = sizeof(double) * ((size_t)c->nElem + P + 1);
+    if (need > FP_SCRATCH_BYTES) {
+        c->err = "fp_time_step_noalloc: " + std::to_string(c->nElem) + " elements need " + std::to_string(need) +
+                 " bytes of scratch, more than its " + std::to_string(FP_SCRATCH_BYTES);
+        return FP_EINVAL;
+    }
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = c->device >= 0 && c->device < FP_MAX_DEVICES ? g_scratch[c->device] : nullptr;
+    }
+    if (!base) {
+        c->err = "fp_time_step_noalloc: no scratch address for the context's device";
+        return FP_EHIP;
+    }
+    double* per_elem = static_cast<double*>(base);
+    double* partial = per_elem + c->nElem;
+    double* result = partial + P;
+    HIPCHK(c, run_size_over_lambda(c, per_elem));
+    HIPCHK(c, launch_min(c->s, (size_t)c->nElem, per_elem, partial, result));
+    HIPCHK(c, hipMemcpyAsync(out, result, sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     return FP_OK;
 }

@@ -1418,6 +1418,7 @@ __global__ void __launch_bounds__(RED_THREADS) k_min_final(int np, const double*
     s = block_min(s);
     if (threadIdx.x == 0) *out = s;
 }
+int min_reduction_partials() { return RED_BLOCKS; }
 hipError_t launch_min(hipStream_t s, size_t n, const double* a, double* partial, double* out) {
     hipLaunchKernelGGL(k_min_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, n, a, partial);
     hipLaunchKernelGGL(k_min_final, dim3(1), dim3(RED_THREADS), 0, s, RED_BLOCKS, partial, out);

@@ -25,7 +25,7 @@ EXPORTS = ["fp_neighbor_matrix", "fp_rcm_order", "fp_create", "fp_destroy", "fp_
            "fp_export_solution", "fp_jv", "fp_jtv", "fp_time_kernel", "fp_stream", "fp_linear_solve_jfnk",
            "fp_functional", "fp_adjoint_solve", "fp_wall_distance", "fp_neighbor_matrix_tet", "fp_rcm_order_tet",
            "fp_limiter", "fp_export_artificial_viscosity", "fp_export_averages", "fp_neighbor_matrix_quad",
-           "fp_rcm_order_quad", "fp_neighbor_matrix_mixed", "fp_autotune_report"]
+           "fp_rcm_order_quad", "fp_neighbor_matrix_mixed", "fp_autotune_report", "fp_time_step_noalloc"]
 
 
 class FpCase(C.Structure):
@@ -82,7 +82,7 @@ def lib() -> C.CDLL:
         "fp_get_state": (C.c_int, [vp, dp]),
         "fp_next_level": (C.c_int, [vp]),
         "fp_restore_level": (C.c_int, [vp]),
-        "fp_time_step": (C.c_int, [vp, dp]),
+        "fp_time_step": (C.c_int, [vp, dp]), "fp_time_step_noalloc": (C.c_int, [vp, dp]),
         "fp_assemble": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp]),
         "fp_linear_solve": (C.c_int, [vp, ip, dp]),
         "fp_update": (C.c_int, [vp, ip]),

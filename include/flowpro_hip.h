@@ -148,6 +148,14 @@ int fp_restore_level(fp_ctx* ctx);
 /* --- Newton step --------------------------------------------------------------------- */
 /* State.getTimeStep without the CFL factor (State.java:177-192): min_e elemSize / lambda_max */
 int fp_time_step(fp_ctx* ctx, double* min_size_over_lambda);
+/* The same minimum computed without any allocation: no counterpart in the reference.  For callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its scratch (the per-element values and the reduction's partial minima) lives in
+ * a 32 MiB device array of the library that each GPU holds until the process ends, so it serves
+ * cases of up to about 4.19 M elements (FP_EINVAL with a message beyond that).  Every call on one
+ * GPU uses that GPU's array: calls that overlap on one GPU (contexts on several threads or
+ * streams) race and must be serialised by the caller. */
+int fp_time_step_noalloc(fp_ctx* ctx, double* min_size_over_lambda);
 /* BDF1/BDF2.assembleJacobiMatrix for all elements (JacobiAssembler.assemble) or only the
  * right-hand side (assembleResiduum) when with_jacobian == 0.  dto <= 0: first step (BDF1).
  * eq_norms[nEqs] = MasterSolver.residuumNorms, rhs_norm = LocalImplicitSolver.rhsNorm. */
