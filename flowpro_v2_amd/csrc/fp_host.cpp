@@ -152,6 +152,8 @@ struct fp_ctx {
     int *rowptr = nullptr, *colidx = nullptr, *diagpos = nullptr, *lvl_f = nullptr, *lvl_b = nullptr;
     int *ftptr = nullptr, *ftask = nullptr;   // per-row task lists of the ILU0 factor
     int *meta_f = nullptr, *meta_b = nullptr; // per-row records of the two sweeps, in level order
+    int *meta_tf = nullptr, *meta_tb = nullptr;   // the same for the transposed sweeps
+    bool adj_prec = false;                    // the factors in F belong to the adjoint system being solved
     bool ilu_rows = false;                    // factor with the register-resident row kernel
     int ilu_variant = -1;                     // launch shape of the row kernel (-1: not tuned yet)
     std::vector<double> tune_ms;              // the tuner's time of every shape (ms per factorisation)
@@ -749,7 +751,7 @@ static void free_all(fp_ctx* c) {
     if (c->g_apply) (void)hipGraphExecDestroy(c->g_apply);
     if (c->io) (void)hipFree(c->io);
     void* ptrs[] = {c->m_nvd, c->m_nqvd, c->m_nbd, c->m_nqfd, c->m_vol, c->m_face, c->qv, c->qf, c->qn, c->xq, c->qmass, c->qminv, c->qis, c->av, c->avold, c->trunc, c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
-                    c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->val, c->F, c->iy, c->ixs, c->ypk, c->xpk, c->fflag, c->fctl, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
+                    c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->meta_tf, c->meta_tb, c->val, c->F, c->iy, c->ixs, c->ypk, c->xpk, c->fflag, c->fctl, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
                     c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->istat};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1920,6 +1922,29 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
             }
         }
     }
+    // records of the transposed sweeps (adjoint preconditioning), same layout: in forward level
+    // order the blocks (k, i) of the lower neighbours' rows, in backward level order the blocks
+    // (j, i) of the upper neighbours' rows
+    std::vector<int> meta_tf((size_t)16 * n, 0), meta_tb((size_t)16 * n, 0);
+    for (int pos = 0; pos < n; pos++) {
+        for (int side = 0; side < 2; side++) {
+            const int r = side == 0 ? rowsf[pos] : rowsb[pos];
+            int* m = (side == 0 ? meta_tf.data() : meta_tb.data()) + (size_t)16 * pos;
+            const int s0 = side == 0 ? rowptr[r] : diagpos[r] + 1, s1 = side == 0 ? diagpos[r] : rowptr[r + 1];
+            m[0] = r;
+            m[1] = s1 - s0;
+            m[2] = diagpos[r];
+            for (int q = s0; q < s1; q++) {
+                const int tb = find_blk(colidx[q], r);
+                if (tb < 0) {
+                    c->err = "block pattern is not symmetric";
+                    return FP_EINVAL;
+                }
+                m[4 + q - s0] = tb;
+                m[10 + q - s0] = colidx[q];
+            }
+        }
+    }
     // --- ILU0 factor tasks per row, in the order of BlockILU0.factor (BlockILU0.java:101-127):
     // for each lower k: {slot of A_ik | k << 9, block of D_k^-1}, then for each later block j of
     // row i that row k holds: {slot of A_ij | 256, block U_kj}
@@ -1959,6 +1984,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     HIPCHK(c, upload(&c->ftptr, ftptr));
     HIPCHK(c, upload(&c->meta_f, meta_f));
     HIPCHK(c, upload(&c->meta_b, meta_b));
+    HIPCHK(c, upload(&c->meta_tf, meta_tf));
+    HIPCHK(c, upload(&c->meta_tb, meta_tb));
     HIPCHK(c, upload(&c->ftask, ftask));
     if (c->dg.av) {
         HIPCHK(c, dalloc(&c->av, (size_t)n * (1 + c->NEQ)));
@@ -2455,10 +2482,21 @@ static int op_residual(fp_ctx* c, const double* x, const double* b, double* y) {
     HIPCHK(c, launch_axpy(c->s, (size_t)c->dofs, -1.0, c->tmp, y));
     return FP_OK;
 }
+// M^-T v with the block-ILU0 factors (U^T sweep with D^-T, then the unit L^T sweep): the
+// preconditioner of the adjoint system, whose matrix is the transpose of the factored one
+template <typename TF> static int ilu_apply_trans_t(fp_ctx* c, double* out, const double* in) {
+    const TF* F = (const TF*)c->F;
+    HIPCHK(c, launch_set_io(c->s, c->io, in, out));
+    HIPCHK(c, launch_ilu_sweep_flow<TF>(c->s, c->B, 2, c->meta_tf, c->nElem, F, c->io, nullptr, c->ypk, c->flow_groups));
+    HIPCHK(c, launch_ilu_sweep_flow<TF>(c->s, c->B, 3, c->meta_tb, c->nElem, F, c->io, c->ypk, c->xpk, c->flow_groups));
+    return FP_OK;
+}
+
 // blockJacobi overwrites its diagonal blocks on the first apply of a solve (blockJacobi.java:74-90)
 static bool prec_changes_on_apply(const fp_ctx* c) { return c->op != 2 && c->precond == 1; }
 static int op_prec(fp_ctx* c, double* out, const double* in) {
-    if (c->op == 2) {   // the adjoint system is solved without preconditioner
+    if (c->op == 2) {   // adjoint system: transposed block-ILU0 when factored for it, else none
+        if (c->adj_prec) return c->bits == 32 ? ilu_apply_trans_t<float>(c, out, in) : ilu_apply_trans_t<double>(c, out, in);
         HIPCHK(c, launch_copy(c->s, (size_t)c->dofs, in, out));
         return FP_OK;
     }
@@ -2609,6 +2647,23 @@ extern "C" int fp_adjoint_solve(fp_ctx* c, double a1, const double* rhs, double*
     if (c->dim != 2) return FP_UNSUPPORTED;
     int rc = to_device(c, rhs, c->RHS);
     if (rc) return rc;
+    // preconditioner: block-ILU0 of the assembled a1 M - dR/dW at the current state, applied
+    // transposed (dataflow sweeps); the assembled matrix and factors of the Newton step are
+    // overwritten
+    c->adj_prec = false;
+    c->jac_valid = false;
+    if (c->precond == 2 && c->ilu_flow) {
+        HIPCHK(c, run_jacobian(c, a1));
+        if ((rc = prec_factor(c))) return rc;
+        int st = 0;
+        HIPCHK(c, hipMemcpyAsync(&st, c->istat, sizeof(int), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        if (st) {
+            c->err = "singular diagonal block in the adjoint preconditioner";
+            return (st & 2) ? FP_EHIP : FP_SINGULAR_BLOCK;
+        }
+        c->adj_prec = true;
+    }
     HIPCHK(c, launch_fill(c->s, (size_t)c->dofs, 0.0, c->x));
     int it = 0;
     double err = 0;
@@ -2617,8 +2672,9 @@ extern "C" int fp_adjoint_solve(fp_ctx* c, double a1, const double* rhs, double*
     c->op_a1 = a1;
     rc = gmres(c, c->RHS, c->x, &it, &err, &ok, true);
     c->op = 0;
-    c->jac_valid = false;   // RHS was overwritten
+    c->adj_prec = false;
     if (rc) return rc;
+    if ((rc = check_sweep_waits(c))) return rc;
     if (iterations) *iterations = it;
     if (rel_residual) *rel_residual = err;
     rc = from_device(c, c->x, lambda);

@@ -995,22 +995,43 @@ __device__ inline void flow_load_row(TF (&Lr)[B], const TF* blk, int t) {
 #pragma unroll
     for (int c = 0; c < B; c++) Lr[c] = blk[(size_t)c * B + tc];
 }
+// row t of the transposed block (column t of the stored block: contiguous per lane)
+template <typename TF, int B>
+__device__ inline void flow_load_col(TF (&Lr)[B], const TF* blk, int t) {
+    const int tc = t < B ? t : B - 1;
+#pragma unroll
+    for (int c = 0; c < B; c++) Lr[c] = blk[(size_t)tc * B + c];
+}
+template <typename TF, int B, bool TRANS>
+__device__ inline void flow_load(TF (&Lr)[B], const TF* blk, int t) {
+    if constexpr (TRANS) flow_load_col<TF, B>(Lr, blk, t);
+    else flow_load_row<TF, B>(Lr, blk, t);
+}
 
-// forward: y_i = b_i - sum_{k<i} L_ik y_k  (dir 0), backward: x_i = D_i^-1 (y_i - sum_{j>i} U_ij x_j)
-// (dir 1; y read from the forward sweep's published values, complete when this launch starts)
+// M = L U with L unit lower and U = D + upper blocks (D_i^-1 stored in the diagonal slot).
+//   DIR 0, M^-1 forward:  y_i = b_i - sum_{k<i} L_ik y_k
+//   DIR 1, M^-1 backward: x_i = D_i^-1 (y_i - sum_{j>i} U_ij x_j)
+//   DIR 2, M^-T forward (U^T z = b):  z_i = D_i^-T (b_i - sum_{k<i} U_ki^T z_k)
+//   DIR 3, M^-T backward (L^T x = z): x_i = z_i - sum_{j>i} L_ji^T x_j
+// The second sweep of each pair reads the first's published values (complete when it starts)
+// and writes io->x.  The records of DIR 2 / 3 list the blocks (k, i) / (j, i) of the
+// neighbours' rows, read transposed.
 template <typename TF, int B, int WPG, int DIR>
 __global__ void __launch_bounds__(64 * WPG) k_ilu_sweep_flow(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
                                                              IluIO* __restrict__ io, const unsigned long long* __restrict__ ypk,
                                                              unsigned long long* __restrict__ out) {
     constexpr int NPF = FlowPrefetch<TF, B>::value;
     constexpr size_t BB = (size_t)B * B;
+    constexpr bool FIRST = DIR == 0 || DIR == 2;    // reads b (else the first sweep's values)
+    constexpr bool DIAG = DIR == 1 || DIR == 2;     // ends with the diagonal inverse
+    constexpr bool TRANS = DIR >= 2;
     __shared__ int base_sh[2];
     const int t = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const unsigned epoch = io->epoch;
     const double* __restrict__ b = io->b;
     double* __restrict__ xo = io->x;
     for (int it = 0;; it++) {
-        if (threadIdx.x == 0) base_sh[it & 1] = (int)atomicAdd(&io->ticket[DIR], (unsigned)WPG);
+        if (threadIdx.x == 0) base_sh[it & 1] = (int)atomicAdd(&io->ticket[FIRST ? 0 : 1], (unsigned)WPG);
         __syncthreads();
         const int base = base_sh[it & 1];
         if (base >= nrows) break;
@@ -1026,11 +1047,11 @@ __global__ void __launch_bounds__(64 * WPG) k_ilu_sweep_flow(const int* __restri
             TF Lr[NPF][B];
 #pragma unroll
             for (int n = 0; n < NPF; n++)
-                if (n < cnt) flow_load_row<TF, B>(Lr[n], F + (size_t)blk[n] * BB, t);
-            TF Dr[DIR == 1 ? B : 1];
-            if constexpr (DIR == 1) flow_load_row<TF, B>(Dr, F + (size_t)m0.z * BB, t);
+                if (n < cnt) flow_load<TF, B, TRANS>(Lr[n], F + (size_t)blk[n] * BB, t);
+            TF Dr[DIAG ? B : 1];
+            if constexpr (DIAG) flow_load<TF, B, TRANS>(Dr, F + (size_t)m0.z * BB, t);
             const int tc = t < B ? t : B - 1;
-            TF s = DIR == 0 ? (TF)b[(size_t)i * B + tc] : flow_value(ypk, (size_t)i * B + tc, (TF)0);
+            TF s = FIRST ? (TF)b[(size_t)i * B + tc] : flow_value(ypk, (size_t)i * B + tc, (TF)0);
 #pragma unroll
             for (int n = 0; n < ILU_MAXN; n++) {
                 if (n < cnt) {
@@ -1040,13 +1061,13 @@ __global__ void __launch_bounds__(64 * WPG) k_ilu_sweep_flow(const int* __restri
                         s -= flow_block_product<TF, B>(Lr[n < NPF ? n : 0], lo, hi);
                     } else {
                         TF Lx[B];
-                        flow_load_row<TF, B>(Lx, F + (size_t)blk[n] * BB, t);
+                        flow_load<TF, B, TRANS>(Lx, F + (size_t)blk[n] * BB, t);
                         s -= flow_block_product<TF, B>(Lx, lo, hi);
                     }
                 }
             }
-            if constexpr (DIR == 1) {
-                // x_i = D_i^-1 s: the vector s is read from the lanes
+            if constexpr (DIAG) {
+                // D_i^-1 s (or D_i^-T s): the vector s is read from the lanes
                 unsigned lo, hi = 0;
                 if constexpr (sizeof(TF) == 4) {
                     lo = __float_as_uint((float)s);
@@ -1056,6 +1077,8 @@ __global__ void __launch_bounds__(64 * WPG) k_ilu_sweep_flow(const int* __restri
                     hi = (unsigned)(xb >> 32);
                 }
                 s = flow_block_product<TF, B>(Dr, lo, hi);
+            }
+            if constexpr (!FIRST) {
                 if (t < B) xo[(size_t)i * B + t] = (double)s;
             }
             if (t < B) flow_publish(out, (size_t)i * B + t, s, epoch);
@@ -1345,8 +1368,14 @@ hipError_t launch_ilu_sweep_flow(hipStream_t s, int B, int dir, const int* meta,
         if (dir == 0)                                                                                                \
             hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 0>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
                                io, ypk, out);                                                                        \
-        else                                                                                                         \
+        else if (dir == 1)                                                                                           \
             hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 1>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
+                               io, ypk, out);                                                                        \
+        else if (dir == 2)                                                                                           \
+            hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 2>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
+                               io, ypk, out);                                                                        \
+        else                                                                                                         \
+            hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 3>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
                                io, ypk, out);                                                                        \
         return hipGetLastError();                                                                                    \
     }

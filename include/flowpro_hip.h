@@ -199,7 +199,10 @@ int fp_jtv(fp_ctx* ctx, double a1, const double* v, double* out);
 int fp_functional(fp_ctx* ctx, int which, double* value, double* dIdW);
 /* adjoint system (a1 M - dR/dW)^T lambda = rhs by GMRES on the matrix-free fp_jtv (element
  * layout in and out); a1 = 0 gives the steady adjoint lambda = (-dR/dW)^-T dI/dW that the
- * reference computes in MATLAB from J.txt (toolbox/optimisation/getFunctionalGradient.m:5-12) */
+ * reference computes in MATLAB from J.txt (toolbox/optimisation/getFunctionalGradient.m:5-12).
+ * With blockilu0 the system is preconditioned by the block-ILU0 of the assembled a1 M - dR/dW at
+ * the current state applied transposed (U^T, then L^T sweeps); this reassembles and refactors,
+ * so the Newton step's matrix must be assembled again (fp_assemble) before the next fp_linear_solve */
 int fp_adjoint_solve(fp_ctx* ctx, double a1, const double* rhs, double* lambda, int* iterations, double* rel_residual);
 
 /* --- measurement ------------------------------------------------------------------------ */

@@ -76,6 +76,7 @@ def lib():
             "or_cfl_update": (C.c_double, [vp, C.c_double]), "or_cfl_reduce": (C.c_double, [vp]),
             "or_set_residuum": (C.c_double, [vp, dp, dp]), "or_l2_norms_sq": (None, [vp, dp, dp]),
             "or_positivity": (C.c_int, [vp]), "or_bj_start": (None, [vp]), "or_bj_apply": (None, [vp, dp, dp]),
+            "or_ilu_apply": (None, [vp, C.c_int, dp, dp]),
             "ot_create": (vp, [C.POINTER(Props), C.c_int, dp, C.c_int, ip, C.c_int, ip, C.c_char_p]),
             "ot_destroy": (None, [vp]), "ot_nbasis": (C.c_int, [vp]), "ot_neqs": (C.c_int, [vp]),
             "ot_tt": (None, [vp, ip]), "ot_order": (None, [vp, ip]),
@@ -383,6 +384,13 @@ class Oracle:
         b = np.ascontiguousarray(b, dtype=np.float64)
         x = np.zeros_like(b)
         lib().or_bj_apply(self.h, _dp(b), _dp(x))
+        return x
+
+    def ilu_apply(self, b, trans=False):
+        """block-ILU0 M^-1 b or M^-T b (DOF numbering) with the factors of the last linear solve."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b)
+        lib().or_ilu_apply(self.h, int(trans), _dp(b), _dp(x))
         return x
 
     def time_step(self):

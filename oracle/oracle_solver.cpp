@@ -224,6 +224,55 @@ static void ilu_apply(Case& c, double* x, const double* b) {
     }
 }
 
+// M^-T b with the BlockILU0 factors M = L U (no reference counterpart: the preconditioner of the
+// transposed (adjoint) system): U^T z = b forward with the transposed diagonal inverses, then
+// L^T x = z backward (unit diagonal), both as column sweeps over the stored row blocks.
+static void ilu_apply_transposed(Case& c, double* x, const double* b) {
+    const int n = c.nElem;
+    std::vector<std::vector<double>> acc(n), z(n);
+    for (int i = 0; i < n; i++) {
+        const int g = c.E[c.ilu_elems[i]].gIndex0, B = (int)c.ilu_y[i].size();
+        acc[i].assign(b + g, b + g + B);
+    }
+    for (int i = 0; i < n; i++) {
+        const auto& row = c.ilu_cols[i];
+        const int B = (int)acc[i].size();
+        const auto& iU = c.ilu_dinv[i];
+        z[i].assign(B, 0.0);
+        for (int r = 0; r < B; r++) {
+            double sum = 0;
+            for (int cc = 0; cc < B; cc++) sum += iU[(size_t)cc * B + r] * acc[i][cc];
+            z[i][r] = sum;
+        }
+        for (int jj = c.ilu_diag[i] + 1; jj < (int)row.size(); jj++) {
+            const auto& U = c.ilu_blocks[i][jj];
+            auto& aj = acc[row[jj]];
+            const int Bj = (int)aj.size();
+            for (int cc = 0; cc < Bj; cc++) {
+                double sum = 0;
+                for (int r = 0; r < B; r++) sum += U[(size_t)r * Bj + cc] * z[i][r];
+                aj[cc] -= sum;
+            }
+        }
+    }
+    acc = z;
+    for (int i = n - 1; i >= 0; i--) {
+        const auto& row = c.ilu_cols[i];
+        const int B = (int)acc[i].size(), g = c.E[c.ilu_elems[i]].gIndex0;
+        for (int r = 0; r < B; r++) x[g + r] = acc[i][r];
+        for (int kk = 0; kk < c.ilu_diag[i]; kk++) {
+            const auto& L = c.ilu_blocks[i][kk];
+            auto& ak = acc[row[kk]];
+            const int Bk = (int)ak.size();
+            for (int cc = 0; cc < Bk; cc++) {
+                double sum = 0;
+                for (int r = 0; r < B; r++) sum += L[(size_t)r * Bk + cc] * x[g + r];
+                ak[cc] -= sum;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------- blockJacobi
 // C/LinearSolvers/preconditioners/blockJacobi.java:74-90 + API/Mat.java:1307-1329 lsolveT.
 // The elimination works IN PLACE on the element's ADiag (kept in bj_work between applies
@@ -898,6 +947,12 @@ void or_bj_start(void* h) {
     for (int i = 0; i < c->nElem; i++) c->bj_work[i] = c->E[i].ADiag;
 }
 void or_bj_apply(void* h, const double* b, double* x) { bj_apply(*(Case*)h, x, b); }
+// block-ILU0 M^-1 b (trans = 0) or M^-T b (trans = 1) with the factors of the last linear solve
+void or_ilu_apply(void* h, int trans, const double* b, double* x) {
+    Case* c = (Case*)h;
+    if (trans) ilu_apply_transposed(*c, x, b);
+    else ilu_apply(*c, x, b);
+}
 void or_wall_distance(int nPts, const double* PXY, int nElem, const int* TP, const int* TT, int nF, double* out) {
     std::vector<double> d = wall_distance(nPts, PXY, nElem, TP, TT, nF);
     std::memcpy(out, d.data(), sizeof(double) * nPts);

@@ -8,6 +8,9 @@ Checks (tolerances written per check):
     directions, step 1e-7                                               1e-6 relative
   J^T v vs the transpose of the assembled GPU Jacobian                 1e-12 relative
   adjoint solve vs a dense solve of the oracle's transposed matrix     1e-8 relative
+    (unsteady a1 = 1/dt, and the steady a1 = 0 adjoint at a converged NS p3 state)
+  the adjoint preconditioner M^-T b (transposed block-ILU0 sweeps)
+    vs the oracle's transposed apply, as a direction                    1e-10
   Jacobian-free Newton correction vs the assembled-matrix correction   1e-8 relative
 """
 import numpy as np
@@ -125,4 +128,54 @@ def test_jacobian_free_newton_correction():
     x_mf = dev.solution()
     assert rc == 0 and rc2 == 0
     assert np.abs(x_mf - x_asm).max() <= 1e-8 * np.abs(x_asm).max()
+    dev.close()
+
+
+def test_adjoint_preconditioner_first_direction():
+    """GMRES(1), one cycle, on the adjoint system: the correction is parallel to M^-T b, with M
+    the block-ILU0 of a1 M - dR/dW at the current state; the oracle's M^-T (ilu_apply, trans)."""
+    case = cyl()
+    props = cases.ns_props(order=4, reynolds=300.0, gmresRestart=1, gmresCycles=1, iterativeSolverTol=1e-14)
+    o, dev, W = _pair(case, props)
+    a1 = 20.0
+    _, g = dev.functional(0)
+    rc, lam, it, err = dev.adjoint_solve(g, a1=a1)
+    assert rc in (0, 1) and it == 1
+    o.assemble(1.0 / a1, -1.0)
+    o.linear_solve()
+    r = o.ilu_apply(o.to_dof(g), trans=True)
+    x = o.to_dof(lam)
+    x, r = x / np.linalg.norm(x), r / np.linalg.norm(r)
+    assert np.abs(x - np.sign(x @ r) * r).max() <= 1e-10
+    dev.close()
+
+
+def test_steady_adjoint_matches_dense():
+    """The steady adjoint of FlowPro/toolbox/optimisation/getFunctionalGradient.m:5-12,
+    lambda = (-dR/dW)^-T dI/dW (a1 = 0), at a converged NS p3 state (cylinder, Re 40, residuum
+    1e-10), preconditioned by the transposed block-ILU0 of -dR/dW, against a dense solve of the
+    oracle's matrix at the same state."""
+    case = cases.cylinder_mesh(16, 4)
+    props = dict(cases.ns_props(order=4, reynolds=40.0, CFL=-1.0, maxCFL=1e3, residuum=1e-10), jacobian="exact")
+    dev = solver.DeviceCase(case, props)
+    dev.set_state(dev.initial_state())
+    recs = solver.GpuImplicitSolver(dev, props).solve(400)
+    assert recs[-1].residuum <= 1e-10
+    W = dev.get_state()
+    dev.close()
+    props_adj = dict(props, iterativeSolverTol=1e-12)
+    dev = solver.DeviceCase(case, props_adj)
+    dev.set_state(W)
+    _, g = dev.functional(0)
+    rc, lam, it, err = dev.adjoint_solve(g, a1=0.0)
+    assert rc == 0, (it, err)
+    o = O.Oracle(case, props_adj)
+    o.set_w(W)
+    o.limiter()
+    o.assemble(float("inf"), -1.0)     # a1 = 1/dt = 0: the spatial Jacobian -dR/dW
+    A = o.global_matrix()
+    lam_ref = o.from_dof(np.linalg.solve(A.T, o.to_dof(g)))
+    print(f"steady adjoint: {it} GMRES iterations, rel. residual {err:.2e}, "
+          f"error {np.abs(lam - lam_ref).max() / np.abs(lam_ref).max():.2e}")
+    assert np.abs(lam - lam_ref).max() <= 1e-8 * np.abs(lam_ref).max()
     dev.close()

@@ -327,3 +327,22 @@ def test_tend_round_off_stops_without_a_tiny_step():
     o = O.Oracle(cases.square_mesh(3, 1.0, -2), _scalar_props(tEnd=0.9 + 1e-9))
     n = len(o.run(50))
     assert n == 3 and o.state()["t"] < 0.9 + 1e-9
+
+
+# ---------------------------------------------------------------- transposed block-ILU0
+@pytest.mark.parametrize("mk_case,props", [
+    (lambda: cases.channel_bump(8, 3), lambda: cases.ns_props(order=3, reynolds=300.0)),
+    (lambda: cases.mixed_channel(6, 3), lambda: cases.euler_props(order=3)),
+], ids=["ns_p2", "mixed"])
+def test_transposed_ilu_apply_is_the_adjoint(mk_case, props):
+    """The adjoint preconditioner M^-T (no reference counterpart) against M^-1, which the
+    reference's LinearSolverTest pins: <u, M^-1 v> = <M^-T u, v> for random u, v."""
+    case = mk_case()
+    o = O.Oracle(case, dict(props(), jacobian="exact", gmresRestart=1, gmresCycles=1))
+    o.assemble(0.05, -1.0)
+    o.linear_solve()
+    rng = np.random.default_rng(2)
+    u, v = rng.standard_normal(o.dofs), rng.standard_normal(o.dofs)
+    lhs = float(u @ o.ilu_apply(v))
+    rhs = float(o.ilu_apply(u, trans=True) @ v)
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), np.abs(u).max() * np.abs(o.ilu_apply(v)).sum())
