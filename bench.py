@@ -44,7 +44,7 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the FP32 ILU level kernel on the 400 x 250 cylinder (200,000 triangles):
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes, scaled by the copy kernel of known size (reads
 # 2.00 x FETCH_SIZE, writes 1.00 x WRITE_SIZE); profiles/ilu_factor_rows_ns2d_p3_cylinder200k_pmc.md
-FACTOR_TRAFFIC_400x250 = 2.00 * 6620.3 * 1024 + 4460.7 * 1024
+FACTOR_TRAFFIC_400x250 = None   # set from the FETCH_SIZE / WRITE_SIZE passes of the dataflow factor kernel
 
 
 def log(msg):
@@ -74,7 +74,7 @@ def run_solver(case, props, bits, warmup, steps, state=None):
     log(f"fp{bits}: {steps} steps in {elapsed:.3f} s, GMRES iterations {iters}, residuals "
         + " ".join(f"{r.residuum:.4e}" for r in recs))
     return dev, dict(elapsed=elapsed, updates=updates, iters=iters, residuals=[r.residuum for r in recs],
-                     failures=s.linear_failures)
+                     failures=s.linear_failures, last_dt=recs[-1].dt)
 
 
 def autotune_summary(dev):
@@ -131,17 +131,51 @@ def factor_algorithmic_bytes(case, dev, TT=None):
     return float(blocks) * dev.B * dev.B * s
 
 
-def roofline_factor(case, dev, ms_levels, traffic=None, TT=None):
-    """The dominant kernel of the step: the level-scheduled block-ILU0 factor kernel
-    (k_ilu_factor_rows).  `avg_launch_ms` = HIP-event time of all level launches / number of levels."""
-    levels = dev.ilu_levels[0]
-    total = factor_algorithmic_bytes(case, dev, TT)
-    per_launch = total / levels
-    avg = ms_levels / levels
-    achieved = per_launch / (avg * 1e-3) / 1e9
+def factor_kernel_name(dev):
+    """The factor kernel the context runs, from the autotuner's report: the dataflow row kernel
+    (one launch per factorisation) when the row kernel serves the block size, else the per-level
+    block kernels."""
+    v, shapes, _ = dev.autotune_report()
+    if v < 0:
+        return "k_ilu_factor_level_4w" if dev.B >= 24 else "k_ilu_factor_level_w", False
+    nw, ng = shapes[v]
+    flow = os.environ.get("FLOWPRO_ILU_FLOW", "1") != "0"
+    mr = 5 if dev.B in (20, 64) else 4
+    tf = "float" if dev.precond_bits == 32 else "double"
+    return f"k_ilu_factor_{'flow' if flow else 'rows'}<{tf}, {dev.B}, {mr}, {nw}, {ng}>", flow
+
+
+def roofline_factor(case, dev, ms_factor, traffic=None, TT=None):
+    """The dominant kernel of the step: the block-ILU0 factorisation.  With the dataflow kernel one
+    launch factors the whole matrix (`avg_launch_ms` = the HIP-event time of the launch); with the
+    per-level kernels the launch is one level (time / number of levels).  Algorithmic bytes per
+    row: 2 w B^2 (1 + n_nbr), every block of the row read and written once; the count that adds one
+    read of D_k^-1 and of U_kj per use is reported beside it."""
+    name, flow = factor_kernel_name(dev)
+    launches = 1 if flow else dev.ilu_levels[0]
+    w = 4 if dev.precond_bits == 32 else 8
+    total = 2.0 * w * dev.B * dev.B * dev.nnzb
+    wide = factor_algorithmic_bytes(case, dev, TT)
+    avg = ms_factor / launches
+    achieved = total / launches / (avg * 1e-3) / 1e9
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel="k_ilu_factor_rows",
-                algorithmic_bytes=per_launch, avg_launch_ms=round(avg, 5), launches=levels)
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
+                algorithmic_bytes=total / launches, algorithmic_bytes_with_operand_reads=wide / launches,
+                frac_with_operand_reads=round(wide / launches / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                avg_launch_ms=round(avg, 5), launches=launches)
+
+
+def roofline_apply(dev, ms):
+    """One block-ILU0 application (forward and backward dataflow sweeps): every factor block read
+    once, w B^2 (1 + n_nbr) per row, plus the right-hand side, the two published vectors and the
+    result (8 B + 2 x 8 B + 8 B per DOF in FP32; the packed values carry their tag)."""
+    w = 4 if dev.precond_bits == 32 else 8
+    traffic = w * dev.B * dev.B * dev.nnzb + 8.0 * dev.dofs * 4
+    achieved = traffic / (ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel=f"k_ilu_sweep_flow<{'float' if w == 4 else 'double'}, {dev.B}, 4, 0|1>",
+                algorithmic_bytes=traffic, avg_launch_ms=round(ms, 4), launches=2)
 
 
 def roofline_spmv(dev, ms):
@@ -152,6 +186,28 @@ def roofline_spmv(dev, ms):
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
                 kernel=f"k_bsr_spmv_b<{B}>" if B in (20, 24, 30, 40, 64) else "k_bsr_spmv",
                 algorithmic_bytes=traffic, avg_launch_ms=ms)
+
+
+def adjoint_measurement(dev, kt, last_dt):
+    """The adjoint path at the workload's size, at the state the timed steps reached: the
+    matrix-free J^T v launch, and drag adjoints lambda = (a1 M - dR/dW)^-T dI/dW by GMRES on J^T v
+    preconditioned with the transposed block-ILU0 of a1 M - dR/dW (assembly and factorisation
+    included in the time): a1 = 1/dt of the last step (the adjoint of that BDF1 step) and a1 = 0
+    (the steady form of FlowPro/toolbox/optimisation/getFunctionalGradient.m:5-12, which the
+    reference applies at a converged state; this state is a transient one)."""
+    jtv_ms = dev.time_kernel(5, 5)
+    res = {"jtv_ms": round(jtv_ms, 4), "functional": "DragCoefficient",
+           "preconditioner": "transposed block-ILU0 of a1 M - dR/dW (U^T then L^T dataflow sweeps)"}
+    for name, a1 in (("step_adjoint", 1.0 / last_dt), ("steady_adjoint", 0.0)):
+        t0 = time.perf_counter()
+        _, g = dev.functional(0)
+        rc, lam, it, err = dev.adjoint_solve(g, a1=a1)
+        el = time.perf_counter() - t0
+        log(f"adjoint ({name}, a1 = {a1:.4g}): rc {rc}, {it} GMRES iterations, rel. residual {err:.2e}, {el:.3f} s")
+        res[name] = {"a1": a1, "seconds": round(el, 3), "gmres_iterations": it, "relative_residual": err,
+                     "converged": rc == 0}
+    log(f"adjoint: J^T v {jtv_ms:.3f} ms")
+    return res
 
 
 def cpu_baseline(threads, n_theta=192, n_r=64, nsteps=3):
@@ -239,24 +295,37 @@ def bench_batched(args):
                    "instances": len(inst), "elements": n_elem, "dofs": r["dofs"], "precond_bits": main_bits},
         "fp64_value": round(out[64]["rate"], 1) if 64 in out else None,
         "mixed_over_fp64": round(out[32]["rate"] / out[64]["rate"], 3) if (32 in out and 64 in out) else None,
+        "state_max_rel_diff_mixed_vs_fp64": (float(np.abs(out[32]["state"] - out[64]["state"]).max()
+                                                   / np.abs(out[64]["state"]).max()) if (32 in out and 64 in out) else None),
         "serial_value": round(serial, 1),
         "final_residual_max_rel_diff": (max(abs(a / b - 1) for a, b in zip(out[32]["final"], out[64]["final"]))
                                         if (32 in out and 64 in out) else None),
         "lift_and_gradient_ms": round(r["lift_ms"], 2),
-        "roofline": None,
+        "kernel_ms": out[32]["kernel_ms"],
+        "roofline": out[32]["roofline"],
         "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads ...")
+        cb = cpu_baseline(threads)
+        cb["note"] = ("the CPU path's Newton-update rate on a bounded sample of the same NS p3 case; at that rate the "
+                      f"{out[32]['updates']} updates of {out[32]['dofs']} DOFs the mixed run needed would take "
+                      f"{out[32]['updates'] * out[32]['dofs'] / cb['value']:.0f} s")
+        result["cpu_baseline"] = cb
     print(json.dumps(result), flush=True)
 
 
 def bench_steady(args):
     """Time to a steady state: the 200k-triangle cylinder at Re = 40 (steady flow), NS p=3, M = 0.5,
-    CFL varied by the residual (State/CFLSetup with CFL <= 0, maxCFL 1e3) until the relative residual
-    (State.residuum) falls below --tol, from the freestream; FP64 preconditioner, then FP32.  Each
-    context factors once before the timed solve so that the autotuner's trials are not timed."""
+    CFL varied by the residual (State/CFLSetup with CFL <= 0, SER up to --max-cfl) until the relative
+    residual (State.residuum) falls below --tol, from the freestream; FP64 preconditioner, then FP32.
+    Each context factors once before the timed solve so that the autotuner's trials are not timed.
+    The roofline entry is the block-ILU0 application (the two dataflow sweeps), which with the
+    SpMV dominates the GMRES iterations of the high-CFL steps."""
     from flowpro_v2_amd import cases, solver
     case = make_case(args.n_theta, args.n_r)
-    props = cases.ns_props(order=4, reynolds=40.0, CFL=-1.0, maxCFL=1e3, residuum=args.tol)
+    props = cases.ns_props(order=4, reynolds=40.0, CFL=-1.0, maxCFL=args.max_cfl, residuum=args.tol)
     log(f"mesh {case.name}: {case.n_elem} triangles, Re 40, to residuum {args.tol:g}")
     out, states = {}, {}
     for bits in (64, 32):
@@ -282,6 +351,10 @@ def bench_steady(args):
                          updates=sum(r.newton for r in recs), dofs=dev.dofs)
         states[bits] = dev.get_state()
         log(f"fp{bits}: {len(recs)} steps, {iters} GMRES iterations, residuum {recs[-1].residuum:.3e} in {elapsed:.2f} s")
+        if bits == 32:
+            kt = kernel_table(dev)
+            out[bits]["kernel_ms"] = {k: round(v, 4) for k, v in kt.items()}
+            out[bits]["roofline"] = roofline_apply(dev, kt["ilu_apply"])
         dev.close()
     diff = float(np.abs(states[32] - states[64]).max() / np.abs(states[64]).max())
     reached = all(out[b]["residuum"] <= args.tol for b in out)
@@ -299,7 +372,7 @@ def bench_steady(args):
         "dtype": "f64 residual/Jacobian/GMRES, f32 preconditioner",
         "data": "synthetic (freestream start, seeded mesh)",
         "config": {"workload": f"NS2D DG p=3 Re=40 M=0.5 cylinder (steady), {case.n_elem} triangles, {out[32]['dofs']} DOFs, "
-                               f"BDF1, CFL varied by the residual (maxCFL 1e3), blockilu0 (RCM) GMRES(30) tol 1e-2, "
+                               f"BDF1, CFL varied by the residual (SER, maxCFL {args.max_cfl:g}), blockilu0 (RCM) GMRES(30) tol 1e-2, "
                                f"until residuum <= {args.tol:g}",
                    "elements": case.n_elem, "dofs": out[32]["dofs"], "order": 4},
         "reached_tolerance": reached,
@@ -325,16 +398,20 @@ def main():
     ap.add_argument("--cfl", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
-    ap.add_argument("--workload", choices=["ns2d_cylinder", "euler3d_tet", "batched_ns", "mixed_ns", "ns2d_steady"],
+    ap.add_argument("--workload", choices=["ns2d_cylinder", "euler3d_tet", "batched_ns", "mixed_ns", "ns2d_steady", "sa_plate"],
                     default="ns2d_cylinder")
     ap.add_argument("--tol", type=float, default=1e-8, help="ns2d_steady: the relative steady residual to reach")
-    ap.add_argument("--max-steps", type=int, default=400, help="ns2d_steady: step limit")
+    ap.add_argument("--max-steps", type=int, default=1200, help="ns2d_steady: step limit")
+    ap.add_argument("--max-cfl", type=float, default=300.0, help="ns2d_steady: maxCFL of the SER law")
     ap.add_argument("--mx", type=int, default=520, help="channel cells along x of the mixed_ns workload")
     ap.add_argument("--my", type=int, default=256, help="channel cells along y of the mixed_ns workload")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batched_ns workload")
     ap.add_argument("--per-element-orders", action="store_true",
                     help="batched_ns: every element of an instance draws its order from 2-5 (order.txt)")
     ap.add_argument("--cube", type=int, default=64, help="cubes per edge of the euler3d_tet workload")
+    ap.add_argument("--plate-nx", type=int, default=3907, help="cells along the sa_plate channel (3907 x 64 x 2 = 500k)")
+    ap.add_argument("--plate-ny", type=int, default=64, help="wall-normal cells of the sa_plate channel")
+    ap.add_argument("--sa-cfl", type=float, default=2.0, help="CFL of the sa_plate workload (ramped over 20 steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is "
@@ -349,6 +426,7 @@ def main():
     from flowpro_v2_amd import cases, native
     tet = args.workload == "euler3d_tet"
     mixed = args.workload == "mixed_ns"
+    sa = args.workload == "sa_plate"
     state = None
     TT = None
     if mixed:
@@ -367,6 +445,12 @@ def main():
             return cases.density_pulse_state(case, dev.nb, dev.initial_state()[0][:: dev.nb], amp=0.05, width=0.1,
                                              pressure_amp=0.05)
         log(f"mesh {case.name}: {case.n_elem} tetrahedra")
+    elif sa:
+        # turbulent flat plate: stretched channel, first cell 1e-5, growth 1.15 (wall distance by
+        # generateDistanceFunction), SpalartAllmaras p=2, Re 1e6, M 0.3, BDF2
+        case = cases.plate_channel(args.plate_nx, args.plate_ny, first=1e-5, growth=1.15)
+        props = cases.sa_props(mach=0.3, reynolds=1e6, order=3, CFL=args.sa_cfl, timeOrder=2)
+        log(f"mesh {case.name}: {case.n_elem} triangles, wall spacing 1e-5, growth 1.15")
     else:
         case = make_case(args.n_theta, args.n_r)
         props = cases.ns_props(order=4, CFL=args.cfl)
@@ -376,6 +460,7 @@ def main():
     for bits in ([64, 32] if args.only is None else ([64] if args.only == "fp64" else [32])):
         dev, res = run_solver(case, props, bits, args.warmup, args.steps, state)
         res["rate"] = dev.dofs * res["updates"] / res["elapsed"]
+        res["state"] = dev.get_state()
         out[bits] = res
         if bits == 32 or args.only is not None:
             kt = kernel_table(dev)
@@ -388,15 +473,19 @@ def main():
     kt = devs["kt"]
     main_bits = 32 if 32 in out else 64
     r = out[main_bits]
-    default_mesh = not tet and not mixed and (args.n_theta, args.n_r) == (400, 250) and main_bits == 32
+    default_mesh = not tet and not mixed and not sa and (args.n_theta, args.n_r) == (400, 250) and main_bits == 32
     roof = roofline_factor(case, dev, kt["ilu_factor_levels"], FACTOR_TRAFFIC_400x250 if default_mesh else None, TT)
+    adjoint = None
+    if not tet and not mixed and not sa:
+        adjoint = adjoint_measurement(dev, kt, out[main_bits]["last_dt"])
     others = {"k_jacobian_team": roofline_jacobian(dev, kt["jacobian"]), "k_bsr_spmv": roofline_spmv(dev, kt["spmv"])}
+    if sa:
+        others = {"k_jacobian_team": dict(roofline_jacobian(dev, kt["jacobian"]), kernel="k_jacobian_team (SA)"),
+                  "k_bsr_spmv": roofline_spmv(dev, kt["spmv"])}
     if tet:
         others = {"k_jacobian_tet_team": dict(roofline_jacobian(dev, kt["jacobian"]), kernel="k_jacobian_tet_team"),
                   "k_bsr_spmv": roofline_spmv(dev, kt["spmv"])}
     if mixed:
-        # blocks of the padded size B = 16 nEqs; the factor runs the per-level block kernels at B = 64
-        roof["kernel"] = "k_ilu_factor_level"
         others = {"k_jacobian_mixed": dict(roofline_jacobian(dev, kt["jacobian"]), kernel="k_jacobian_mixed"),
                   "k_bsr_spmv": roofline_spmv(dev, kt["spmv"])}
     result = {
@@ -421,6 +510,10 @@ def main():
                                 f"{dev.dofs} DOFs, M=0.8 density and pressure pulse, BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) "
                                 "GMRES(30) tol 1e-2",
                     "elements": case.n_elem, "dofs": dev.dofs, "order": 2, "precond_bits": main_bits} if tet else
+                   {"workload": f"RANS Spalart-Allmaras DG p=2 Re=1e6 M=0.3 flat plate, stretched channel {case.n_elem} "
+                                f"triangles (first cell 1e-5, growth 1.15), {dev.dofs} DOFs, BDF2 CFL {args.sa_cfl} ramp, "
+                                "blockilu0 (RCM) GMRES(30) tol 1e-2",
+                    "elements": case.n_elem, "dofs": dev.dofs, "order": 3, "precond_bits": main_bits} if sa else
                    {"workload": f"NS2D DG p=3 Re=5000 M=0.5 cylinder, {case.n_elem} triangles, "
                                 f"{dev.dofs} DOFs, BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) GMRES(30) tol 1e-2",
                     "elements": case.n_elem, "dofs": dev.dofs, "order": 4, "precond_bits": main_bits}),
@@ -432,6 +525,7 @@ def main():
         "roofline": roof,
         "other_kernels": others,
         "autotune": autotune_summary(dev),
+        "adjoint": adjoint,
     }
     dev.close()
     if tet:
@@ -439,6 +533,9 @@ def main():
         result["cpu_baseline"] = None   # the CPU restatement of the tetrahedra covers residual and Jacobian only
     elif mixed:
         result["data"] = "synthetic (freestream state, seeded mesh, element types and orders)"
+        result["cpu_baseline"] = None
+    elif sa:
+        result["data"] = "synthetic (freestream state with inlet turbulence vtIn 3, seeded mesh)"
         result["cpu_baseline"] = None
     elif not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)

@@ -36,6 +36,8 @@ hipError_t launch_residual_team(const ElemLaunch&, hipStream_t, const ModelParam
                                 const double*, const double*, const double*, double, double, double, double*, double*, double*);
 hipError_t launch_jacobian_team(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
                                 const double*, double, double*, float*);
+hipError_t launch_jtv_team(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
+                           const double*, const double*, double, double*);
 hipError_t launch_jvp(const ElemLaunch&, hipStream_t, const ModelParams&, const DGParams&, const Tables&, const Mesh&,
                       const double*, const double*, double, double*);
 hipError_t launch_size_over_lambda(const ElemLaunch&, hipStream_t, const ModelParams&, const Tables&, const Mesh&,
@@ -233,6 +235,7 @@ static hipError_t run_jtv(fp_ctx* c, const double* v, double a1, double* out) {
     if (c->mixed) return launch_jacobian_mixed(c->L, c->s, c->mp, c->dg, c->mms, c->W, a1, nullptr, nullptr, v, out, 1);
     if (c->quad) return launch_jacobian_quad(c->L, c->s, c->mp, c->dg, c->qtb, c->qms, c->W, a1, nullptr, nullptr, v, out, 1);
     if (c->dim == 3) return launch_jacobian_tet(c->L, c->s, c->mp, c->ttb, c->tms, c->W, a1, nullptr, v, out, 1);
+    if (team_supported(c->L)) return launch_jtv_team(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, v, a1, out);
     return launch_jacobian<double>(c->L, c->s, c->mp, c->dg, c->tb, c->ms, c->W, a1, nullptr, v, out, 1);
 }
 // elemSize / lambda_max of every element into out[nElem]
@@ -2390,10 +2393,13 @@ template <typename TF> static int ilu_factor_t(fp_ctx* c) {
         if (rc) return rc;
     }
     c->f_fresh = false;
+    // the dataflow factorisation is two launches: replayed from a graph only in the per-level mode
+    if (c->factor_flow) return ilu_level_launches<TF>(c);
     return run_captured<TF>(c, &c->g_factor, ilu_level_launches<TF>);
 }
 template <typename TF> static int ilu_apply_t(fp_ctx* c, double* out, const double* in) {
     HIPCHK(c, launch_set_io(c->s, c->io, in, out));
+    if (c->ilu_flow) return ilu_apply_launches<TF>(c);   // two launches, no graph needed
     return run_captured<TF>(c, &c->g_apply, ilu_apply_launches<TF>);
 }
 

@@ -369,10 +369,14 @@ __global__ void __launch_bounds__(256, 3) k_residual_team(ModelParams P, DGParam
 // NIN local inputs per point: the value (and with viscosity the x and y derivatives) of
 // every equation.  Phase 1b stores d(J w * integrand)/d(input) for every point and input;
 // phase 2 thread (element, column i = (m', j')) builds column i of the diagonal block and
-// of the three neighbour blocks.
-template <int MK, int NB, int NEQ, int NQV, int NQF, bool VISC, bool AV, typename BT>
+// of the three neighbour blocks.  JTV: the columns are not stored but contracted with v, which
+// gives the transposed product (a1 M - dR/dW)^T v without any write to another element's rows:
+// column i of element e meets the rows of e (diagonal block) and of its face neighbours (the
+// neighbour-side blocks its own face integrand yields), so out[e] is complete in e's team.
+template <int MK, int NB, int NEQ, int NQV, int NQF, bool VISC, bool AV, bool JTV>
 __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParams G, Tables tb, Mesh ms, const double* __restrict__ W,
-                                                       double a1, BT* __restrict__ blocks, float* __restrict__ fblocks) {
+                                                       double a1, double* __restrict__ blocks, float* __restrict__ fblocks,
+                                                       const double* __restrict__ v, double* __restrict__ out) {
     using LY = TeamLayout<NB, NEQ, NQV, NQF>;
     constexpr int B = LY::B;
     constexpr int NIN = (VISC || AV) ? 3 * NEQ : NEQ;   // the damping terms see the gradients too
@@ -383,6 +387,7 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
     constexpr int O_DV = LY::ELEM0, O_DF = O_DV + NQV * NIN * NVS * NEQ;
     __shared__ double tab[LY::TAB];
     __shared__ double eb[EPB * STRIDE];
+    __shared__ double red[JTV ? EPB * B * NEQ : 1];   // JTV: per (element, column, equation) partial sums
     const int e0 = blockIdx.x * EPB;
     const int nel = min(EPB, ms.nElem - e0);
     if (nel <= 0) return;
@@ -447,6 +452,7 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
         const int mp = i / NB, jp = i % NB, e = e0 + el;
         const double* E = eb + el * STRIDE;
         double acc[NB];
+        double part = 0.0;   // JTV
 #pragma unroll
         for (int j = 0; j < NB; j++) acc[j] = 0.0;
 #pragma unroll 1
@@ -474,12 +480,18 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
                 }
             }
             if (blkn >= 0) {
-                const size_t o0 = (size_t)blkn * BB + (size_t)i * B + (size_t)m * NB;
+                if constexpr (JTV) {
+                    const double* vt = v + (size_t)ms.nbr[e * 3 + k] * B + (size_t)m * NB;
 #pragma unroll
-                for (int j = 0; j < NB; j++) blocks[o0 + j] = (BT)accR[j];
-                if (fblocks) {   // the FP32 copy the block-ILU0 factorisation starts from
+                    for (int j = 0; j < NB; j++) part += accR[j] * vt[j];
+                } else {
+                    const size_t o0 = (size_t)blkn * BB + (size_t)i * B + (size_t)m * NB;
 #pragma unroll
-                    for (int j = 0; j < NB; j++) fblocks[o0 + j] = (float)accR[j];
+                    for (int j = 0; j < NB; j++) blocks[o0 + j] = accR[j];
+                    if (fblocks) {   // the FP32 copy the block-ILU0 factorisation starts from
+#pragma unroll
+                        for (int j = 0; j < NB; j++) fblocks[o0 + j] = (float)accR[j];
+                    }
                 }
             }
         }
@@ -507,13 +519,33 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
             }
         }
         const double jac = ms.geo[(size_t)e * GEO + 4];
-        const size_t o0 = (size_t)ms.blkd[e] * BB + (size_t)i * B + (size_t)m * NB;
+        if constexpr (JTV) {
+            const double* ve = v + (size_t)e * B + (size_t)m * NB;
 #pragma unroll
-        for (int j = 0; j < NB; j++) {
-            double v = -acc[j];
-            if (m == mp) v += a1 * (jac * tab[LY::T_M + jp * NB + j]);
-            blocks[o0 + j] = (BT)v;
-            if (fblocks) fblocks[o0 + j] = (float)v;
+            for (int j = 0; j < NB; j++) {
+                double a = -acc[j];
+                if (m == mp) a += a1 * (jac * tab[LY::T_M + jp * NB + j]);
+                part += a * ve[j];
+            }
+            red[(el * B + i) * NEQ + m] = part;
+        } else {
+            const size_t o0 = (size_t)ms.blkd[e] * BB + (size_t)i * B + (size_t)m * NB;
+#pragma unroll
+            for (int j = 0; j < NB; j++) {
+                double a = -acc[j];
+                if (m == mp) a += a1 * (jac * tab[LY::T_M + jp * NB + j]);
+                blocks[o0 + j] = a;
+                if (fblocks) fblocks[o0 + j] = (float)a;
+            }
+        }
+    }
+    if constexpr (JTV) {
+        __syncthreads();
+        for (int q = tid; q < nel * B; q += nt) {
+            double sum = 0.0;
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) sum += red[q * NEQ + m];
+            out[(size_t)e0 * B + q] = sum;
         }
     }
 }
@@ -588,8 +620,26 @@ hipError_t launch_jacobian_team(const ElemLaunch& L, hipStream_t s, const ModelP
         constexpr int NIN = (V || A) ? 3 * NEQ : NEQ;                                                                  \
         constexpr int EX = NQV * NIN * volume_slots<MK>() * NEQ + LY::NQFT * NIN * NEQ;                                \
         constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EX>::value;                                                     \
-        hipLaunchKernelGGL((k_jacobian_team<MK, NB, NEQ, NQV, NQF, V, A, double>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), \
-                           0, s, P, G, tb, ms, W, a1, blocks, fblocks);                                                \
+        hipLaunchKernelGGL((k_jacobian_team<MK, NB, NEQ, NQV, NQF, V, A, false>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), \
+                           0, s, P, G, tb, ms, W, a1, blocks, fblocks, nullptr, nullptr);                              \
+        return hipGetLastError();                                                                                      \
+    }
+    FP_TEAM_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+// out = (a1 M - dR/dW)^T v (device layout) by the Jacobian team without storing a block
+hipError_t launch_jtv_team(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const Tables& tb,
+                           const Mesh& ms, const double* W, const double* v, double a1, double* out) {
+#define X(MK, NB, NEQ, NQV, NQF, V, A)                                                                                \
+    if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V, A)) {                                                                      \
+        using LY = TeamLayout<NB, NEQ, NQV, NQF>;                                                                      \
+        constexpr int NIN = (V || A) ? 3 * NEQ : NEQ;                                                                  \
+        constexpr int EX = NQV * NIN * volume_slots<MK>() * NEQ + LY::NQFT * NIN * NEQ;                                \
+        constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EX>::value;                                                     \
+        hipLaunchKernelGGL((k_jacobian_team<MK, NB, NEQ, NQV, NQF, V, A, true>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), \
+                           0, s, P, G, tb, ms, W, a1, nullptr, nullptr, v, out);                                       \
         return hipGetLastError();                                                                                      \
     }
     FP_TEAM_CASES(X)
