@@ -41,10 +41,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per launch of the FP32 ILU level kernel on the 400 x 250 cylinder (200,000 triangles):
-# rocprofv3 FETCH_SIZE and WRITE_SIZE passes, scaled by the copy kernel of known size (reads
-# 2.00 x FETCH_SIZE, writes 1.00 x WRITE_SIZE); profiles/ilu_factor_rows_ns2d_p3_cylinder200k_pmc.md
-FACTOR_TRAFFIC_400x250 = None   # set from the FETCH_SIZE / WRITE_SIZE passes of the dataflow factor kernel
+# HBM bytes of one launch of the FP32 dataflow factor kernel (the whole factorisation) on the 400 x 250
+# cylinder (200,000 triangles): rocprofv3 FETCH_SIZE and WRITE_SIZE passes, scaled by the copy kernel of
+# known size (reads 2.00 x FETCH_SIZE, writes 1.00 x WRITE_SIZE): 4,259,063.5 KiB fetched, 3,135,064 KiB
+# written per dispatch; profiles/ilu_factor_flow_ns2d_p3_cylinder200k_pmc.md
+FACTOR_TRAFFIC_400x250 = 2.00 * 4259063.5 * 1024 + 3135064.0 * 1024
 
 
 def log(msg):
