@@ -379,6 +379,19 @@ def sa_props(mach: float = 0.3, reynolds: float = 1e6, order: int = 3, vt_in: fl
     return p
 
 
+def kw_props(mach: float = 0.3, reynolds: float = 1e5, order: int = 3, k_in: float = 1e-3, om_in: float = -4.0,
+             **kw) -> dict:
+    """KOmegaSST parameters with the template's model constants (Aerodynamics/src/main/resources/templates/
+    KOmegaSST.txt); the inlet k and log(omega) give a turbulent viscosity of rho k exp(-omega) = 0.055 times
+    the laminar one at the inlet."""
+    p = euler_props(mach=mach, order=order, model="KOmegaSST", isFlowViscous=True, reynolds=reynolds,
+                    prandtl=0.72, penalty=1.0, kIn=k_in, omIn=om_in, SuthConst=110.0, a1=0.31, sk1=0.85, som1=0.5,
+                    alpha1=0.5532, beta1=0.075, betast=0.09, sk2=1.0, som2=0.856, alpha2=0.4404, beta2=0.0828,
+                    Prt=0.9)
+    p.update(kw)
+    return p
+
+
 def perturbed_state(w0: np.ndarray, n_elem: int, n_basis: int, rel: float = 0.01, seed: int = 1) -> np.ndarray:
     """Element-major Java layout W[e][m*Nb + j]: constant state w0 times (1 + rel*N(0,1))
     per coefficient (numpy default_rng(seed))."""

@@ -36,6 +36,7 @@ struct Tables {
     const double* fx;    // [3][NQF][2]
     const double* mref;  // [NB][NB]
     const double* mrefinv;
+    const double* fint;  // [NQF][2]: the face rule's interpolants (1 - t)/2, (1 + t)/2 of a face's two vertices
 };
 
 struct Mesh {
@@ -46,7 +47,7 @@ struct Mesh {
     const int* nbrf;
     const int* blkn;
     const int* blkd;
-    const double* wdist;   // [e*3 + v] wall distance of the element's vertices (Spalart-Allmaras)
+    const double* wdist;   // [e*3 + v] wall distance of the element's vertices (Spalart-Allmaras, k-omega SST)
     const double* av;      // [e*(1+NEQ) + .] eps, dampInner[0..NEQ) (null without damping)
 };
 

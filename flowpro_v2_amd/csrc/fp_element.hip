@@ -117,7 +117,12 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
                     dWc[m] = (dWL[m] + dWR[m]) / 2.0 + jump * n[0];
                     dWc[NEQ + m] = (dWL[NEQ + m] + dWR[NEQ + m]) / 2.0 + jump * n[1];
                 }
-                numerical_diffusive_flux<MK>(P, Wc, dWc, n, TT, fvn);
+                double Df = 0.0;   // k-omega SST: the wall distance along the face (Element.java:299-312)
+                if constexpr (MK == MODEL_KOMEGA) {
+                    const double* wd = ms.wdist + (size_t)e * 3;
+                    Df = tb.fint[2 * p] * wd[k] + tb.fint[2 * p + 1] * wd[k == 2 ? 0 : k + 1];
+                }
+                numerical_diffusive_flux<MK>(P, Wc, dWc, n, TT, fvn, Df);
                 const bool ip = TT > -1 || is_ip_face<MK>(TT);
 #pragma unroll
                 for (int m = 0; m < NEQ; m++) {
@@ -136,7 +141,7 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
         kr.face_end(k);
     }
     // ---------------- volume (DGpure.residuum) ----------------
-    if constexpr (NB > 1 || MK == MODEL_SCALAR || MK == MODEL_SA) {
+    if constexpr (NB > 1 || MK == MODEL_SCALAR || has_rans_source<MK>()) {
 #pragma unroll 1
         for (int q = 0; q < NQV; q++) {
             const double* phi = tb.vb + q * NB;
@@ -159,6 +164,12 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
                 dWq[NEQ + m] = sy;
             }
             const double jw = jac * tb.vw[q];
+            double Dq = 0.0;   // wall distance at the point (Element.java:263-275)
+            if constexpr (has_rans_source<MK>()) {
+                const double* xi = tb.vx + q * 2;
+                const double* wd = ms.wdist + (size_t)e * 3;
+                Dq = (1.0 - xi[0] - xi[1]) * wd[0] + xi[0] * wd[1] + xi[1] * wd[2];
+            }
             T fx[NEQ], fy[NEQ];
             if constexpr (NB > 1) {
                 const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
@@ -166,8 +177,8 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
                 convective_flux<MK>(P, Wq, ey, fy);
                 if constexpr (VISC) {
                     T gx[NEQ], gy[NEQ];
-                    diffusive_flux<MK>(P, Wq, dWq, ex, gx);
-                    diffusive_flux<MK>(P, Wq, dWq, ey, gy);
+                    diffusive_flux<MK>(P, Wq, dWq, ex, gx, Dq);
+                    diffusive_flux<MK>(P, Wq, dWq, ey, gy, Dq);
 #pragma unroll
                     for (int m = 0; m < NEQ; m++) { fx[m] -= gx[m]; fy[m] -= gy[m]; }
                 }
@@ -182,12 +193,8 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
                 double X[2] = {ge[5] * xi[0] + ge[6] * xi[1] + ge[7], ge[8] * xi[0] + ge[9] * xi[1] + ge[10]};
                 source_term(P, X, src);
             }
-            if constexpr (MK == MODEL_SA) {
-                const double* xi = tb.vx + q * 2;
-                const double* wd = ms.wdist + (size_t)e * 3;
-                const double D = (1.0 - xi[0] - xi[1]) * wd[0] + xi[0] * wd[1] + xi[1] * wd[2];
-                sa_source(P, Wq, dWq, D, src);
-            }
+            if constexpr (MK == MODEL_SA) sa_source(P, Wq, dWq, Dq, src);
+            if constexpr (MK == MODEL_KOMEGA) kw_source(P, Wq, dWq, Dq, src);
 #pragma unroll
             for (int j = 0; j < NB; j++) {
                 double gx, gy;
@@ -195,7 +202,7 @@ __device__ void dg_residual(const ModelParams& P, const DGParams& G, const Table
 #pragma unroll
                 for (int m = 0; m < NEQ; m++) {
                     T v = fx[m] * gx + fy[m] * gy;
-                    if constexpr (MK == MODEL_SCALAR || MK == MODEL_SA) v += src[m] * phi[j];
+                    if constexpr (MK == MODEL_SCALAR || has_rans_source<MK>()) v += src[m] * phi[j];
                     K[m * NB + j] += jw * v;
                 }
             }
@@ -396,6 +403,8 @@ __global__ void __launch_bounds__(128) k_jvp(ModelParams P, DGParams G, Tables t
     X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, true) \
     X(MODEL_SA, 3, 5, 6, 2, true)             \
     X(MODEL_SA, 6, 5, 7, 3, true)             \
+    X(MODEL_KOMEGA, 3, 6, 6, 2, true)         \
+    X(MODEL_KOMEGA, 6, 6, 7, 3, true)         \
     X(MODEL_ADIABATIC, 3, 3, 6, 2, false)     \
     X(MODEL_ADIABATIC, 6, 3, 7, 3, false)     \
     X(MODEL_ADIABATIC, 10, 3, 16, 4, false)   \

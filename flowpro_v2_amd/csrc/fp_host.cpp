@@ -788,7 +788,12 @@ static int setup_tri(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
             vd[(q * NB + j) * 2 + 1] = basis_der(bs, j, x, y, 1);
         }
     static const double CX[3][2] = {{0, 0}, {1, 0}, {0, 1}};
-    std::vector<double> fb(3 * NQF * NB), fd(3 * NQF * NB * 2), fw(cs->face_w, cs->face_w + NQF), fx(3 * NQF * 2);
+    std::vector<double> fb(3 * NQF * NB), fd(3 * NQF * NB * 2), fw(cs->face_w, cs->face_w + NQF), fx(3 * NQF * 2),
+        fint(2 * NQF);
+    for (int p = 0; p < NQF; p++) {   // Element.interpolateVelocityAndFillElementDataObjectOnFace (Element.java:299-312)
+        fint[2 * p] = (1 - cs->face_xi[p]) / 2;
+        fint[2 * p + 1] = (1 + cs->face_xi[p]) / 2;
+    }
     for (int k = 0; k < 3; k++)
         for (int p = 0; p < NQF; p++) {
             double t = cs->face_xi[p];
@@ -820,12 +825,12 @@ static int setup_tri(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
         return off;
     };
     size_t o_vb = put(vb), o_vd = put(vd), o_vw = put(vw), o_vx = put(vx), o_fb = put(fb), o_fd = put(fd), o_fw = put(fw),
-           o_fx = put(fx), o_m = put(mref), o_mi = put(mrefinv);
+           o_fx = put(fx), o_m = put(mref), o_mi = put(mrefinv), o_fi = put(fint);
     HIPCHK(c, upload(&c->tab, tab));
     HIPCHK(c, upload(&c->isref, isref));
     c->h_isref = isref;
     c->tb = Tables{c->tab + o_vb, c->tab + o_vd, c->tab + o_vw, c->tab + o_vx, c->tab + o_fb,
-                   c->tab + o_fd, c->tab + o_fw, c->tab + o_fx, c->tab + o_m,  c->tab + o_mi};
+                   c->tab + o_fd, c->tab + o_fw, c->tab + o_fx, c->tab + o_m,  c->tab + o_mi, c->tab + o_fi};
 
     // --- geometry in the DOF order (transformation2Dtriangle, faceTransformation2Dline,
     //     Element.computeGeometry)
@@ -1694,7 +1699,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         return FP_EINVAL;
     }
     if (c->mixed) {
-        if (cs->model == MODEL_SA || cs->model == MODEL_SCALAR) {
+        if (cs->model == MODEL_SA || cs->model == MODEL_SCALAR || cs->model == MODEL_KOMEGA) {
             c->err = "mixed element types or orders: device kernels for the NavierStokes and AdiabaticEulers models";
             return FP_UNSUPPORTED;
         }
@@ -1713,7 +1718,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
             return FP_EINVAL;
         }
     }
-    if (c->quad && (cs->model == MODEL_SA || cs->model == MODEL_SCALAR)) {
+    if (c->quad && (cs->model == MODEL_SA || cs->model == MODEL_SCALAR || cs->model == MODEL_KOMEGA)) {
         c->err = "quadrilaterals: device kernels for the NavierStokes and AdiabaticEulers models";
         return FP_UNSUPPORTED;
     }
@@ -1747,26 +1752,34 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         c->NEQ = cs->model == MODEL_ADIABATIC ? 3 : 4;
     } else {
         c->NB = make_basis(cs->order).nb;
-        c->NEQ = cs->model == MODEL_SCALAR ? 1 : (cs->model == MODEL_SA ? 5 : (cs->model == MODEL_ADIABATIC ? 3 : 4));
+        c->NEQ = cs->model == MODEL_SCALAR ? 1
+                 : cs->model == MODEL_SA ? 5
+                 : cs->model == MODEL_ADIABATIC ? 3
+                 : cs->model == MODEL_KOMEGA ? 6 : 4;
     }
-    if (cs->model == MODEL_SA && !cs->wall_distance) {
-        c->err = "the Spalart-Allmaras model needs the vertex wall distance";
+    if ((cs->model == MODEL_SA || cs->model == MODEL_KOMEGA) && !cs->wall_distance) {
+        c->err = "the Spalart-Allmaras and k-omega SST models need the vertex wall distance";
+        return FP_EINVAL;
+    }
+    if (cs->model == MODEL_KOMEGA && (!(cs->cv > 0) || !(cs->p_ref > 0) || !(cs->rho_ref > 0))) {
+        c->err = "KOmegaSST needs cv, p_ref and rho_ref (the Sutherland reference temperature)";
         return FP_EINVAL;
     }
     c->B = c->NB * c->NEQ;
     c->NQV = cs->n_vol;
     c->NQF = cs->n_face;
     c->dofs = (long long)n * c->B;
-    // SpalartAllmaras.init passes isDiffusive = true to Aerodynamics.init (SpalartAllmaras.java:38)
-    // AdiabaticEulers.init passes isDiffusive = false (AdiabaticEulers.java:13-16)
+    // SpalartAllmaras.init and KOmegaSST.init pass isDiffusive = true to Aerodynamics.init
+    // (SpalartAllmaras.java:38, KOmegaSST.java:47); AdiabaticEulers.init passes false (AdiabaticEulers.java:13-16)
     const bool diffusive = cs->model == MODEL_SCALAR ? cs->scalar_k > 0
-                                                     : (cs->model == MODEL_SA || (cs->model != MODEL_ADIABATIC && cs->viscous != 0));
-    if (cs->model < 0 || cs->model > MODEL_ADIABATIC) {
+                                                     : (cs->model == MODEL_SA || cs->model == MODEL_KOMEGA ||
+                                                        (cs->model != MODEL_ADIABATIC && cs->viscous != 0));
+    if (cs->model < 0 || cs->model > MODEL_KOMEGA) {
         c->err = "unknown model";
         return FP_EINVAL;
     }
     const bool av = cs->damp_tol > 0 || cs->damp_const != 0 || cs->damp_inner_tol > 0;
-    if (av && (c->dim == 3 || c->quad || c->mixed || cs->model == MODEL_SA)) {
+    if (av && (c->dim == 3 || c->quad || c->mixed || cs->model == MODEL_SA || cs->model == MODEL_KOMEGA)) {
         c->err = "artificial viscosity: device kernels for the 2D NavierStokes and scalar models only";
         return FP_UNSUPPORTED;
     }
@@ -1819,7 +1832,23 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     P.cw2 = cs->sa_cw2;
     P.cw3 = cs->sa_cw3;
     P.cv1 = cs->sa_cv1;
-    P.Prt = cs->sa_prt;
+    P.Prt = cs->model == MODEL_KOMEGA ? cs->kw_prt : cs->sa_prt;
+    if (cs->model == MODEL_KOMEGA) {   // KOmegaSST.init (KOmegaSST.java:45-71) and sutherland (:566-570)
+        P.kIn = cs->kw_k_in;
+        P.omIn = cs->kw_om_in;
+        const double TRef = 1 / (cs->cv * (cs->kappa - 1)) * cs->p_ref / cs->rho_ref;
+        P.suthRatio = cs->kw_suth / TRef;
+        P.kwA1 = cs->kw_a1;
+        P.sk1 = cs->kw_sk1;
+        P.som1 = cs->kw_som1;
+        P.alpha1 = cs->kw_alpha1;
+        P.beta1 = cs->kw_beta1;
+        P.betast = cs->kw_betast;
+        P.sk2 = cs->kw_sk2;
+        P.som2 = cs->kw_som2;
+        P.alpha2 = cs->kw_alpha2;
+        P.beta2 = cs->kw_beta2;
+    }
     P.flux = cs->model == MODEL_NAVIER_STOKES && cs->numerical_flux == 1 ? 1 : 0;   // Aerodynamics.java:300-303
     if (cs->numerical_flux < 0 || cs->numerical_flux > 1) {
         c->err = "numerical_flux: 0 (default) or 1 (vanLeer)";
@@ -2253,7 +2282,7 @@ static int assemble_impl(fp_ctx* c, double dt, double dto, int time_order, int w
     if (eq_norms || rhs_norm) {
         const int ne = c->NEQ;
         HIPCHK(c, launch_colsum(c->s, c->nElem, 2 * ne, c->enorm, c->partial, c->scal));   // 256 x 2 NEQ <= 4096 partials
-        double h[10];
+        double h[16];   // 2 nEqs <= 12 (k-omega SST: 6 equations)
         HIPCHK(c, hipMemcpyAsync(h, c->scal, sizeof(double) * 2 * ne, hipMemcpyDeviceToHost, c->s));
         HIPCHK(c, hipStreamSynchronize(c->s));
         if (eq_norms)

@@ -1228,6 +1228,7 @@ hipError_t launch_spmv(hipStream_t s, int nrows, int B, const int* rowptr, const
         case 20: hipLaunchKernelGGL(k_bsr_spmv_b<20>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
         case 24: hipLaunchKernelGGL(k_bsr_spmv_b<24>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
         case 30: hipLaunchKernelGGL(k_bsr_spmv_b<30>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
+        case 36: hipLaunchKernelGGL(k_bsr_spmv_b<36>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
         case 40: hipLaunchKernelGGL(k_bsr_spmv_b<40>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
         case 64: hipLaunchKernelGGL(k_bsr_spmv_b<64>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
         default: hipLaunchKernelGGL(k_bsr_spmv, grd, blk, 0, s, nrows, B, rowptr, colidx, val, x, b, y, mode);
@@ -1263,7 +1264,7 @@ hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nr
 
 // (block size, row blocks held in registers): the triangles' rows have at most 4 blocks, the
 // tetrahedra's (B = 20 at P1) 5, those of meshes with squares (B = 64: order 4, or padded) 5
-#define FP_ROWREG_SIZES(X) X(20, 5) X(24, 4) X(30, 4) X(40, 4) X(64, 5)
+#define FP_ROWREG_SIZES(X) X(20, 5) X(24, 4) X(30, 4) X(36, 4) X(40, 4) X(64, 5)
 // launch shapes of the row kernel, picked per context by timing them on the case's own matrix:
 // (task waves NW, inversion waves NGW)
 #define FP_ROWREG_VARIANTS(Y, BS, MR) Y(BS, MR, 0, 4, 4) Y(BS, MR, 1, 8, 4) Y(BS, MR, 2, 8, 2) Y(BS, MR, 3, 8, 8) Y(BS, MR, 4, 4, 2)

@@ -55,7 +55,9 @@ class FpCase(C.Structure):
         ("elem_type", C.c_int),
         ("elem_types", C.POINTER(C.c_int)), ("elem_orders", C.POINTER(C.c_int)),
         ("gauss_rules_dir", C.c_char_p),
-    ]
+        ("cv", C.c_double), ("p_ref", C.c_double), ("rho_ref", C.c_double),
+    ] + [(f"kw_{n}", C.c_double) for n in ("k_in", "om_in", "suth", "a1", "sk1", "som1", "alpha1", "beta1", "betast",
+                                            "sk2", "som2", "alpha2", "beta2", "prt")]
 
 
 _lib = None

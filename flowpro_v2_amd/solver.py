@@ -141,6 +141,14 @@ class DeviceCase:
             cs.sa_vt_in = mc.vt_in
             cs.sa_sigma, cs.sa_cb1, cs.sa_cb2, cs.sa_ka = (mc.sa[k] for k in ("sigma", "cb1", "cb2", "ka"))
             cs.sa_cw2, cs.sa_cw3, cs.sa_cv1, cs.sa_prt = (mc.sa[k] for k in ("cw2", "cw3", "cv1", "Prt"))
+        cs.cv, cs.p_ref, cs.rho_ref = mc.cv, mc.p_ref, mc.rho_ref
+        if mc.kw is not None:
+            if wd is None:
+                raise UnsupportedCase("KOmegaSST needs the vertex wall distance (Case.wall_distance)")
+            cs.kw_k_in, cs.kw_om_in, cs.kw_suth = mc.kw["kIn"], mc.kw["omIn"], mc.kw["SuthConst"]
+            cs.kw_a1, cs.kw_sk1, cs.kw_som1, cs.kw_alpha1, cs.kw_beta1 = (mc.kw[k] for k in ("a1", "sk1", "som1", "alpha1", "beta1"))
+            cs.kw_betast, cs.kw_sk2, cs.kw_som2, cs.kw_alpha2 = (mc.kw[k] for k in ("betast", "sk2", "som2", "alpha2"))
+            cs.kw_beta2, cs.kw_prt = mc.kw["beta2"], mc.kw["Prt"]
         h = C.c_void_p()
         rc = L.fp_create(C.byref(cs), C.byref(h))
         if rc == native.FP_UNSUPPORTED:

@@ -58,7 +58,8 @@ typedef struct fp_case {
     const double* face_xi;        /* [n_face]; 3D: [n_face][2] reference-triangle coordinates */
     const double* face_w;         /* [n_face] */
     int model;                    /* 0 NavierStokes (Euler when viscous == 0), 1 SpalartAllmaras,
-                                     2 TestScalarEquation, 3 AdiabaticEulers (subsonic inlet) */
+                                     2 TestScalarEquation, 3 AdiabaticEulers (subsonic inlet),
+                                     4 KOmegaSST (subsonic inlet; fields at the end) */
     int viscous;
     double kappa;                 /* Aerodynamics.init derived constants */
     double p_out;                 /* outlet pressure / pRef */
@@ -106,6 +107,14 @@ typedef struct fp_case {
     const int* elem_types;        /* [n_elems]: 3 or 4, or NULL (all elem_type) */
     const int* elem_orders;       /* [n_elems]: 1..5 (1..4 on squares), or NULL (all order) */
     const char* gauss_rules_dir;
+    /* Aerodynamics.init reference values used by KOmegaSST.sutherland (KOmegaSST.java:566-570):
+     * cv and the dimensional pRef, rhoRef (TRef = pRef / (rhoRef cv (kappa - 1))) */
+    double cv, p_ref, rho_ref;
+    /* KOmegaSST (KOmegaSST.java:45-71; straight triangles of orders 2 and 3, wall_distance required,
+     * no artificial viscosity): inlet k and log(omega) (kIn, omIn), SuthConst, and the model constants
+     * a1, sk1, som1, alpha1, beta1, betast, sk2, som2, alpha2, beta2, Prt */
+    double kw_k_in, kw_om_in, kw_suth, kw_a1, kw_sk1, kw_som1, kw_alpha1, kw_beta1, kw_betast, kw_sk2, kw_som2,
+        kw_alpha2, kw_beta2, kw_prt;
 } fp_case;
 
 /* --- integer setup (bit-exact with the reference) ------------------------------ */

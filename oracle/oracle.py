@@ -24,11 +24,15 @@ _D = ["kappa", "cv", "pIn0", "rhoIn0", "pOut", "reynolds", "prandtl", "attack_an
       "sa_vtIn", "sa_sigma", "sa_cb1", "sa_cb2", "sa_ka", "sa_cw2", "sa_cw3", "sa_cv1", "sa_prt"]
 
 
+_KW = ["kw_kIn", "kw_omIn", "kw_suth", "kw_a1", "kw_sk1", "kw_som1", "kw_alpha1", "kw_beta1", "kw_betast", "kw_sk2",
+       "kw_som2", "kw_alpha2", "kw_beta2", "kw_prt"]
+
+
 class Props(C.Structure):
     _fields_ = [(n, C.c_int) for n in _I] + [(n, C.c_double) for n in _D] + [("dimension", C.c_int)] + \
         [("damp_tol", C.c_double), ("damp_const", C.c_double), ("damp_inner_tol", C.c_double),
          ("n_damp_inner_coef", C.c_int), ("damp_inner_coef", C.c_double * 5), ("numerical_flux", C.c_int),
-         ("elem_type", C.c_int), ("t_end", C.c_double)]
+         ("elem_type", C.c_int), ("t_end", C.c_double)] + [(n, C.c_double) for n in _KW]
 
 
 def build(force: bool = False) -> str:
@@ -96,7 +100,10 @@ def lib():
 
 
 _PREC = {"none": 0, "blockjacobi": 1, "blockilu0": 2, "ilu0": 3}
-_MODEL = {"NavierStokes": 0, "SpalartAllmaras": 1, "TestScalarEquation": 2, "AdiabaticEulers": 3}
+_MODEL = {"NavierStokes": 0, "SpalartAllmaras": 1, "TestScalarEquation": 2, "AdiabaticEulers": 3, "KOmegaSST": 4}
+# KOmegaSST constants of the reference's template (Aerodynamics/src/main/resources/templates/KOmegaSST.txt)
+KW_DEFAULTS = {"kIn": 23.05, "omIn": 0.84, "SuthConst": 110.0, "a1": 0.31, "sk1": 0.85, "som1": 0.5, "alpha1": 0.5532,
+               "beta1": 0.075, "betast": 0.09, "sk2": 1.0, "som2": 0.856, "alpha2": 0.4404, "beta2": 0.0828, "Prt": 0.9}
 # SpalartAllmaras constants of the reference's template (Aerodynamics/src/main/resources/templates/
 # SpalartAllmaras.txt); parameters.txt may override each
 SA_DEFAULTS = {"sigma": 0.6667, "cb1": 0.1355, "cb2": 0.622, "ka": 0.41, "cw2": 0.3, "cw3": 2.0, "cv1": 7.1,
@@ -154,6 +161,11 @@ def props_from(p: dict, threads: int = 1) -> Props:
     for key, field in (("sigma", "sa_sigma"), ("cb1", "sa_cb1"), ("cb2", "sa_cb2"), ("ka", "sa_ka"), ("cw2", "sa_cw2"),
                        ("cw3", "sa_cw3"), ("cv1", "sa_cv1"), ("Prt", "sa_prt")):
         setattr(P, field, float(p.get(key, SA_DEFAULTS[key])))
+    for key, field in (("kIn", "kw_kIn"), ("omIn", "kw_omIn"), ("SuthConst", "kw_suth"), ("a1", "kw_a1"), ("sk1", "kw_sk1"),
+                       ("som1", "kw_som1"), ("alpha1", "kw_alpha1"), ("beta1", "kw_beta1"), ("betast", "kw_betast"),
+                       ("sk2", "kw_sk2"), ("som2", "kw_som2"), ("alpha2", "kw_alpha2"), ("beta2", "kw_beta2"),
+                       ("Prt", "kw_prt")):
+        setattr(P, field, float(p.get(key, KW_DEFAULTS[key])))
     return P
 
 

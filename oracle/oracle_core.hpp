@@ -23,6 +23,9 @@
 
 namespace oracle {
 
+// largest equation count of the 2D models (KOmegaSST: rho, rho u, rho v, rho E, rho k, rho omega)
+static const int MAXEQ = 6;
+
 // ---------------------------------------------------------------- dual number
 // Forward-mode derivative carrier used for the exact Jacobian, J*v and J^T*v.
 struct Dual {
@@ -62,6 +65,8 @@ inline Dual dabs(Dual a) { return a.v < 0 ? -a : a; }
 // java.lang.Math.max: (a >= b) ? a : b for ordinary values
 inline double dmax(double a, double b) { return (a >= b) ? a : b; }
 inline Dual dmax(Dual a, Dual b) { return (a.v >= b.v) ? a : b; }
+inline double dtanh(double a) { return std::tanh(a); }
+inline Dual dtanh(Dual a) { double t = std::tanh(a.v); return Dual(t, (1 - t * t) * a.d); }
 
 // ---------------------------------------------------------------- properties
 // Raw parameters.txt values (the keys of C/Parameters.java and
@@ -105,10 +110,14 @@ struct Props {
     int numerical_flux;   // Aerodynamics.numericalFluxType: 0 "default" (Lax-Friedrichs), 1 "vanLeer"
     int elem_type;        // 2D element type of every element: 3 triangles (or 0), 4 quadrilaterals
     double t_end;         // Parameters.tEnd (Double.MAX_VALUE when absent, C/Parameters.java:143-147)
+    // KOmegaSST (AERO/KOmegaSST.java:45-71): inlet k and log(omega), the Sutherland constant and the
+    // model constants of the two blended k-omega sets
+    double kw_kIn, kw_omIn, kw_suth, kw_a1, kw_sk1, kw_som1, kw_alpha1, kw_beta1, kw_betast, kw_sk2, kw_som2,
+        kw_alpha2, kw_beta2, kw_prt;
 };
 
 struct ModelConst {
-    int kind;        // 0 NS/Euler, 1 Spalart-Allmaras, 2 scalar
+    int kind;        // 0 NS/Euler, 1 Spalart-Allmaras, 2 scalar, 3 AdiabaticEulers, 4 KOmegaSST
     int dim, nEqs;
     bool diffusive;
     bool vanLeer;    // NavierStokes interior flux: van Leer flux-vector splitting
@@ -122,6 +131,8 @@ struct ModelConst {
     double a[2], k, lambda;
     // Spalart-Allmaras constants, cw1 = cb1/ka^2 + (1 + cb2)/sigma (AERO/SpalartAllmaras.java:49-55)
     double vtIn, sigma, cb1, cb2, ka, cw1, cw2, cw3, cv1, Prt;
+    // KOmegaSST constants (AERO/KOmegaSST.java:45-71); Prt above is shared
+    double kIn, omIn, suth, a1kw, sk1, som1, alpha1, beta1, betast, sk2, som2, alpha2, beta2;
 };
 
 // C/Aerodynamics init restated: AERO/Aerodynamics.java:119-312, AERO/NavierStokes.java:18-21

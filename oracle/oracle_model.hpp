@@ -10,7 +10,7 @@ namespace oracle {
 
 static const double RHO_TOL = 1e-2;   // AERO/Aerodynamics.java:22
 static const double P_TOL = 1e-2;     // AERO/NavierStokes.java:15
-static const double P_TOL_SA = 1e-1;  // AERO/SpalartAllmaras.java:15
+static const double P_TOL_SA = 1e-1;  // AERO/SpalartAllmaras.java:15, AERO/KOmegaSST.java:15
 enum { WALL = -1, INLET = -2, OUTLET = -3, INVISCID_WALL = -4 };
 
 struct QPData {            // ElementData fields used on the path (API/ElementData.java)
@@ -33,7 +33,7 @@ template <class T> static T pressure(const ModelConst& M, T* W) {
         if (val(p) < 1e-1) p = 1e-1 * dexp((p - 1e-1) / 1e-1);
         return p;
     }
-    const double ptol = M.kind == 1 ? P_TOL_SA : P_TOL;
+    const double ptol = (M.kind == 1 || M.kind == 4) ? P_TOL_SA : P_TOL;   // AERO/KOmegaSST.java:75-88
     W[0] = limiteRho(W[0]);
     T momentum2 = 0.0;
     for (int d = 0; d < M.dim; ++d) momentum2 += W[d + 1] * W[d + 1];
@@ -59,6 +59,10 @@ template <class T> static void convectiveFlux(const ModelConst& M, T* W, const d
     if (M.kind == 3) return;   // AERO/AdiabaticEulers.java:177-193: no energy equation
     f[M.dim + 1] = (W[M.dim + 1] + p) * V;
     if (M.kind == 1) f[M.dim + 2] = W[M.dim + 2] * V;   // AERO/SpalartAllmaras.java:196
+    if (M.kind == 4) {   // AERO/KOmegaSST.java:210-211
+        f[M.dim + 2] = W[M.dim + 2] * V;
+        f[M.dim + 3] = W[M.dim + 3] * V;
+    }
 }
 // AERO/Aerodynamics.java:323-337
 template <class T> static T maxEigenvalue(const ModelConst& M, T* W, const QPData& q) {
@@ -121,7 +125,8 @@ static void numericalConvectiveFlux(const ModelConst& M, T* WL, T* WR, const dou
         case INVISCID_WALL: {
             // NavierStokes: p(WL); SpalartAllmaras: p(WR) (AERO/SpalartAllmaras.java:146)
             // AdiabaticEulers: p(WR) and no energy component (AERO/AdiabaticEulers.java:136-149)
-            T p = pressure(M, (M.kind == 1 || M.kind == 3) ? WR : WL);
+            // KOmegaSST: p(WR) (AERO/KOmegaSST.java:161-175)
+            T p = pressure(M, (M.kind == 1 || M.kind == 3 || M.kind == 4) ? WR : WL);
             f[0] = 0.0;
             double V = .0;
             for (int d = 0; d < dim; ++d) {
@@ -139,14 +144,14 @@ static void numericalConvectiveFlux(const ModelConst& M, T* WL, T* WR, const dou
         default: {
             if (M.vanLeer) {   // AERO/NavierStokes.java:174-179
                 T pL = pressure(M, WL);
-                T fp[5], fm[5];
+                T fp[MAXEQ], fm[MAXEQ];
                 vanLeerSplit(M, WL, pL, n, +1, fp);
                 T pR = pressure(M, WR);
                 vanLeerSplit(M, WR, pR, n, -1, fm);
                 for (int j = 0; j < nEqs; j++) f[j] = fp[j] + fm[j];
                 break;
             }
-            T fL[5], fR[5];
+            T fL[MAXEQ], fR[MAXEQ];
             convectiveFlux(M, WL, n, fL);
             convectiveFlux(M, WR, n, fR);
             T lamL = maxEigenvalue(M, WL, q);
@@ -203,6 +208,71 @@ template <class T> static void boundaryValueAdiabatic(const ModelConst& M, T* WL
             break;
     }
 }
+// AERO/KOmegaSST.java:218-306 (subsonic inlet): the wall keeps log(omega) and zeroes k, the inviscid
+// wall copies the inner state (no reflection), inlet k and omega from kIn and omIn
+template <class T> static void boundaryValueKW(const ModelConst& M, T* WL, const double* n, int TT, const QPData& q, T* WR) {
+    const int nEqs = M.nEqs, dim = M.dim;
+    const double kapa = M.kapa;
+    WL[0] = limiteRho(WL[0]);
+    for (int j = 0; j < nEqs; j++) WR[j] = 0.0;
+    T p = pressure(M, WL);
+    switch (TT) {
+        case WALL: {   // isDiffusive is always true for this model
+            const double* u = q.meshVelocity;
+            double absVelocity2 = .0;
+            for (int d = 0; d < dim; ++d) absVelocity2 += u[d] * u[d];
+            WR[0] = WL[0];
+            for (int d = 0; d < dim; ++d) WR[d + 1] = WR[0] * u[d];
+            WR[dim + 1] = p / (kapa - 1) + WR[0] * absVelocity2 / 2;
+            WR[dim + 2] = 0.0;
+            WR[dim + 3] = WL[dim + 3];
+            break;
+        }
+        case INVISCID_WALL:
+            for (int j = 0; j < nEqs; j++) WR[j] = WL[j];
+            break;
+        case INLET: {
+            const double pIn0 = 1, rhoIn0 = 1;
+            T mach;
+            if (val(p) > pIn0) mach = 0.0;
+            else mach = dsqrt((2 / (kapa - 1)) * (-1 + dpow(pIn0 / p, (kapa - 1) / kapa)));
+            T rhoIn = rhoIn0 * dpow((1 + ((kapa - 1) / 2) * mach * mach), 1 / (1 - kapa));
+            T inletVelocity = mach * dsqrt((kapa * p) / rhoIn);
+            T E = p / (kapa - 1) + 0.5 * rhoIn * inletVelocity * inletVelocity;
+            WR[0] = rhoIn;
+            if (!M.hasAttack) {
+                for (int d = 0; d < dim; ++d) WR[d + 1] = -rhoIn * inletVelocity * n[d];
+            } else {
+                double dir[2] = {std::cos(M.attack), std::sin(M.attack)};
+                for (int d = 0; d < dim; ++d) WR[d + 1] = rhoIn * inletVelocity * dir[d];
+            }
+            WR[dim + 1] = E;
+            WR[dim + 2] = rhoIn * M.kIn;
+            WR[dim + 3] = rhoIn * M.omIn;
+            break;
+        }
+        case OUTLET: {
+            T rhoOut = WL[0];
+            T absVelocity2 = .0;
+            for (int d = 0; d < dim; ++d) absVelocity2 += WL[d + 1] * WL[d + 1];
+            absVelocity2 /= rhoOut * rhoOut;
+            T a = dsqrt(kapa * pressure(M, WL) / WL[0]);
+            T mach = dsqrt(absVelocity2) / a;
+            if (val(mach) < 1) {
+                WR[0] = rhoOut;
+                for (int d = 0; d < dim; ++d) WR[d + 1] = rhoOut * WL[d + 1] / WL[0];
+                WR[dim + 1] = M.pOut / (kapa - 1) + rhoOut * absVelocity2 / 2;
+                WR[dim + 2] = WL[dim + 2];
+                WR[dim + 3] = WL[dim + 3];
+            } else {
+                for (int j = 0; j < nEqs; j++) WR[j] = WL[j];
+            }
+            break;
+        }
+        default:
+            break;
+    }
+}
 // AERO/NavierStokes.java:216-315 (subsonic inlet, no attack angle unless given)
 template <class T> static void boundaryValue(const ModelConst& M, T* WL, const double* n, int TT, const QPData& q, T* WR) {
     const int nEqs = M.nEqs, dim = M.dim;
@@ -212,6 +282,10 @@ template <class T> static void boundaryValue(const ModelConst& M, T* WL, const d
     }
     if (M.kind == 3) {
         boundaryValueAdiabatic(M, WL, n, TT, WR);
+        return;
+    }
+    if (M.kind == 4) {
+        boundaryValueKW(M, WL, n, TT, q, WR);
         return;
     }
     WL[0] = limiteRho(WL[0]);
@@ -340,11 +414,102 @@ template <class T> static void diffusiveFluxSA(const ModelConst& M, T* W, const 
     }
 }
 
+// KOmegaSST.max / min (AERO/KOmegaSST.java:572-585): a > b ? a : b, a < b ? a : b
+template <class T> static T kwMax(T a, T b) { return val(a) > val(b) ? a : b; }
+template <class T> static T kwMin(T a, T b) { return val(a) < val(b) ? a : b; }
+// AERO/KOmegaSST.java:566-570 sutherland: the temperature p / rho, TRef = pRef / (rhoRef cv (kappa - 1))
+template <class T> static T sutherlandKW(const ModelConst& M, T rho, T p) {
+    T Tm = p / rho;
+    const double TRef = 1 / (M.cv * (M.kapa - 1)) * M.pRef / M.rhoRef;
+    return dpow(Tm, 1.5) * (1 + M.suth / TRef) / (Tm + M.suth / TRef);
+}
+// the wall distance of the point as KOmegaSST takes it: |d|, at least 1e-5 (AERO/KOmegaSST.java:366-369)
+static double kwWallDistance(const QPData& q) {
+    double walldist = std::fabs(q.wallDistance);
+    if (walldist < 1e-5) walldist = 1e-5;
+    return walldist;
+}
+// AERO/KOmegaSST.java:309-419 diffusiveFlux (2D: the vorticity |dv_1/dx_2 - dv_2/dx_1|)
+template <class T> static void diffusiveFluxKW(const ModelConst& M, T* W, const T* dW, const double* n, const QPData& q, T* flux) {
+    const int nEqs = M.nEqs, dim = M.dim;
+    W[0] = limiteRho(W[0]);
+    T rho = W[0];
+    const double lam = -2. / 3;
+    T velocity[3];
+    T velocity2 = .0;
+    for (int d = 0; d < dim; ++d) {
+        velocity[d] = W[d + 1] / rho;
+        velocity2 += velocity[d] * velocity[d];
+    }
+    T velocityJac[9];
+    for (int d = 0; d < dim; ++d)
+        for (int f = 0; f < dim; ++f) velocityJac[dim * d + f] = (dW[f * nEqs + d + 1] - dW[f * nEqs] * velocity[d]) / rho;
+    T p = pressure(M, W);
+    T pOverRhoDer[3];
+    for (int d = 0; d < dim; ++d) {
+        T temp = .0;
+        for (int f = 0; f < dim; ++f) temp += velocity[f] * velocityJac[dim * f + d];
+        T pDer = (M.kapa - 1) * (dW[d * nEqs + dim + 1] - dW[d * nEqs] * velocity2 / 2 - rho * temp);
+        pOverRhoDer[d] = (rho * pDer - p * dW[d * nEqs]) / (rho * rho);
+    }
+    T stress[9];
+    T trace = .0;
+    for (int d = 0; d < dim; ++d) {
+        trace += velocityJac[dim * d + d];
+        for (int f = 0; f < dim; ++f) stress[dim * d + f] = velocityJac[dim * d + f] + velocityJac[dim * f + d];
+    }
+    for (int d = 0; d < dim; ++d) stress[dim * d + d] += lam * trace;
+    T k = kwMax(W[dim + 2] / rho, T(1e-10));
+    T om = W[dim + 3] / rho;
+    T etaTemp = sutherlandKW(M, rho, p);
+    T vortic = dabs(velocityJac[1] - velocityJac[2]);
+    const double walldist = kwWallDistance(q);
+    T arg2 = kwMax(2 * dsqrt(k) / (0.09 * dexp(om) * walldist) / std::sqrt(M.Re),
+                   500 * etaTemp / rho / (walldist * walldist * dexp(om)) / M.Re);
+    T F2 = dtanh(arg2 * arg2);
+    T mut = kwMax(T(0.0), rho * M.a1kw * k / kwMax(M.a1kw * dexp(om), vortic * F2));
+    T turbulentStress[9];
+    for (int i = 0; i < dim * dim; i++) turbulentStress[i] = stress[i] * mut;
+    T kMax = 2.0 / 3 * kwMax(T(0.0), rho * k);
+    for (int d = 0; d < dim; ++d) turbulentStress[dim * d + d] -= kMax;
+    T kDer[3], omDer[3];
+    for (int d = 0; d < dim; ++d) {
+        kDer[d] = (dW[d * nEqs + dim + 2] - dW[d * nEqs] * k) / rho;
+        omDer[d] = (dW[d * nEqs + dim + 3] - dW[d * nEqs] * om) / rho;
+    }
+    T constant = M.kapa / (M.kapa - 1) * (etaTemp / M.Pr + mut / M.Prt);
+    T omkDer = 0.0;
+    for (int d = 0; d < dim; ++d) omkDer += omDer[d] * kDer[d];
+    T CD = kwMax(2 * M.som2 * rho * omkDer, T(1e-10));
+    T arg11 = kwMax(dsqrt(k) / (0.09 * dexp(om) * walldist) / std::sqrt(M.Re),
+                    500 * etaTemp / rho / (dexp(om) * walldist * walldist) / M.Re);
+    T arg12 = 4 * rho * M.som2 * k / (CD * walldist * walldist);
+    T arg1 = kwMin(arg11, arg12);
+    T F1 = dtanh(dpow(arg1, 4.0));
+    T sk0 = F1 * M.sk1 + (1 - F1) * M.sk2;
+    T som0 = F1 * M.som1 + (1 - F1) * M.som2;
+    for (int j = 0; j < nEqs; j++) flux[j] = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        T tmp = .0;
+        for (int f = 0; f < dim; ++f) {
+            flux[f + 1] += (etaTemp * stress[dim * d + f] + turbulentStress[dim * d + f]) * n[d] / M.Re;
+            tmp += velocity[f] * (etaTemp * stress[dim * d + f] + turbulentStress[dim * d + f]);
+        }
+        flux[dim + 1] += (tmp + constant * pOverRhoDer[d]) * n[d] / M.Re;
+        flux[dim + 2] += (etaTemp + sk0 * mut) / M.Re * kDer[d] * n[d];
+        flux[dim + 3] += (etaTemp + som0 * mut) / M.Re * omDer[d] * n[d];
+    }
+}
+
 // AERO/NavierStokes.java:317-374
-template <class T> static void diffusiveFlux(const ModelConst& M, T* W, const T* dW, const double* n, T* flux) {
+template <class T> static void diffusiveFlux(const ModelConst& M, T* W, const T* dW, const double* n, const QPData& q, T* flux) {
     const int nEqs = M.nEqs, dim = M.dim;
     if (M.kind == 1) {
         diffusiveFluxSA(M, W, dW, n, flux);
+        return;
+    }
+    if (M.kind == 4) {
+        diffusiveFluxKW(M, W, dW, n, q, flux);
         return;
     }
     if (M.kind == 2) {   // T/user/equation/TestScalarEquation.java:108-114
@@ -395,11 +560,11 @@ template <class T> static void diffusiveFlux(const ModelConst& M, T* W, const T*
 template <class T>
 static void numericalDiffusiveFlux(const ModelConst& M, T* Wc, const T* dWc, const double* n, int TT, const QPData& q, T* flux) {
     if (M.kind == 2) {
-        diffusiveFlux(M, Wc, dWc, n, flux);
+        diffusiveFlux(M, Wc, dWc, n, q, flux);
         return;
     }
     Wc[0] = limiteRho(Wc[0]);
-    diffusiveFlux(M, Wc, dWc, n, flux);
+    diffusiveFlux(M, Wc, dWc, n, q, flux);
     if (TT < 0) {
         if (TT == WALL) {
             T work = .0;
@@ -411,7 +576,7 @@ static void numericalDiffusiveFlux(const ModelConst& M, T* Wc, const T* dWc, con
     }
 }
 static bool isIPFace(const ModelConst& M, int TT) { return M.kind == 2 ? true : TT == WALL; }
-static bool isSourcePresent(const ModelConst& M) { return M.kind == 2 || M.kind == 1; }
+static bool isSourcePresent(const ModelConst& M) { return M.kind == 2 || M.kind == 1 || M.kind == 4; }
 // AERO/SpalartAllmaras.java:367-451 sourceTerm (2D: rotationMagnitude = |U[1] - U[2]|)
 template <class T> static void sourceTermSA(const ModelConst& M, T* W, const T* dW, const QPData& q, T* source) {
     const int nEqs = M.nEqs, dim = M.dim;
@@ -447,10 +612,73 @@ template <class T> static void sourceTermSA(const ModelConst& M, T* W, const T* 
     if (val(d) < -100) d = -100.0;   // limitDestruction
     source[dim + 2] = d;
 }
+// AERO/KOmegaSST.java:447-537 sourceTerm: k and log(omega) production (limited) and destruction,
+// the cross-diffusion and the omega-gradient terms, each clipped at zero
+template <class T> static void sourceTermKW(const ModelConst& M, T* W, const T* dW, const QPData& q, T* source) {
+    const int nEqs = M.nEqs, dim = M.dim;
+    W[0] = limiteRho(W[0]);
+    T rho = W[0];
+    const double lam = -2. / 3;
+    T velocity[3];
+    for (int d = 0; d < dim; ++d) velocity[d] = W[d + 1] / rho;
+    T velocityJac[9];
+    for (int d = 0; d < dim; ++d)
+        for (int f = 0; f < dim; ++f) velocityJac[dim * d + f] = (dW[f * nEqs + d + 1] - dW[f * nEqs] * velocity[d]) / rho;
+    T p = pressure(M, W);
+    T k = kwMax(W[dim + 2] / rho, T(1e-10));
+    T om = W[dim + 3] / rho;
+    T expOm = dexp(om);
+    T etaTemp = sutherlandKW(M, rho, p);
+    T vortic = dabs(velocityJac[1] - velocityJac[2]);
+    const double walldist = kwWallDistance(q);
+    T arg2 = kwMax(2 * dsqrt(k) / (0.09 * dexp(om) * walldist) / std::sqrt(M.Re),
+                   500 * etaTemp / rho / (walldist * walldist * dexp(om)) / M.Re);
+    T F2 = dtanh(arg2 * arg2);
+    T mut = kwMax(T(0.0), rho * M.a1kw * k / kwMax(M.a1kw * dexp(om), vortic * F2));
+    T turbulentStress[9];
+    T trace = .0;
+    for (int d = 0; d < dim; ++d) {
+        trace += velocityJac[dim * d + d];
+        for (int f = 0; f < dim; ++f) turbulentStress[dim * d + f] = mut * (velocityJac[dim * d + f] + velocityJac[dim * f + d]);
+    }
+    T kMax = 2.0 / 3 * kwMax(T(0.0), rho * k);
+    for (int d = 0; d < dim; ++d) turbulentStress[dim * d + d] += mut * lam * trace - kMax;
+    T omkDer = 0.0, omDerSqr = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        T kDer = (dW[d * nEqs + dim + 2] - dW[d * nEqs] * k) / rho;
+        T omDer = (dW[d * nEqs + dim + 3] - dW[d * nEqs] * om) / rho;
+        omkDer += omDer * kDer;
+        omDerSqr += omDer * omDer;
+    }
+    T Tv = 0.0;
+    for (int d = 0; d < dim; d++)
+        for (int f = 0; f < dim; f++) Tv += turbulentStress[dim * d + f] * velocityJac[dim * d + f];
+    T CD = kwMax(2 * M.som2 * rho * omkDer, T(1e-10));
+    T arg11 = kwMax(dsqrt(k) / (0.09 * dexp(om) * walldist) / std::sqrt(M.Re),
+                    500 * etaTemp / rho / (dexp(om) * walldist * walldist) / M.Re);
+    T arg12 = 4 * rho * M.som2 * k / (CD * walldist * walldist);
+    T arg1 = kwMin(arg11, arg12);
+    T F1 = dtanh(dpow(arg1, 4.0));
+    T som0 = F1 * M.som1 + (1 - F1) * M.som2;
+    T alpha0 = F1 * M.alpha1 + (1 - F1) * M.alpha2;
+    T beta0 = F1 * M.beta1 + (1 - F1) * M.beta2;
+    if (val(Tv) > val(10 * M.betast * rho * k * expOm)) Tv = 10 * M.betast * rho * k * expOm;
+    T Tvom = alpha0 * rho / mut / expOm * Tv;
+    if (val(Tvom) > val(10 * beta0 * rho * expOm)) Tvom = 10 * beta0 * rho * expOm;
+    for (int j = 0; j < nEqs; j++) source[j] = 0.0;
+    source[dim + 2] = kwMax(Tv - M.betast * rho * k * expOm, T(0.0));
+    source[dim + 3] = kwMax(Tvom - beta0 * rho * expOm + 1 / M.Re * M.som2 * rho / expOm * omkDer * 2 * (1 - F1) +
+                                1 / M.Re * (etaTemp + som0 * mut) * omDerSqr,
+                            T(0.0));
+}
 // T/user/equation/TestScalarEquation.java:131-136
 template <class T> static void sourceTerm(const ModelConst& M, T* W, const T* dW, const QPData& q, T* s) {
     if (M.kind == 1) {
         sourceTermSA(M, W, dW, q, s);
+        return;
+    }
+    if (M.kind == 4) {
+        sourceTermKW(M, W, dW, q, s);
         return;
     }
     double phase = 2 * q.currentX[0] + q.currentX[1];

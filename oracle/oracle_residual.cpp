@@ -50,7 +50,7 @@ static void residuumWall(const Case& c, const Elem& e, int k, const T* Wown, con
             }
             elem_getX(e, cXi, q.currentX);
         }
-        T WL[5], WR[5], dWL[10], dWR[10];
+        T WL[MAXEQ], WR[MAXEQ], dWL[2 * MAXEQ], dWR[2 * MAXEQ];
         for (int m = 0; m < nEqs; m++) { WL[m] = 0.0; WR[m] = 0.0; }
         for (int m = 0; m < dim * nEqs; m++) { dWL[m] = 0.0; dWR[m] = 0.0; }
         for (int m = 0; m < nEqs; m++)
@@ -69,8 +69,8 @@ static void residuumWall(const Case& c, const Elem& e, int k, const T* Wown, con
             for (int m = 0; m < dim * nEqs; m++) dWR[m] = dWL[m];
         }
         double vn = 0;
-        T Wale[5];
-        T fn[5];
+        T Wale[MAXEQ];
+        T fn[MAXEQ];
         if (convective) {
             for (int d = 0; d < dim; d++) vn = vn + q.meshVelocity[d] * nn[d];
             if (TT > -1)
@@ -79,9 +79,9 @@ static void residuumWall(const Case& c, const Elem& e, int k, const T* Wown, con
                 for (int j = 0; j < nEqs; j++) Wale[j] = WR[j];
             numericalConvectiveFlux(M, WL, WR, nn, TT, q, fn);
         }
-        T fvn[5];
+        T fvn[MAXEQ];
         if (diffusive) {
-            T Wc[5], dWc[10];
+            T Wc[MAXEQ], dWc[2 * MAXEQ];
             for (int m = 0; m < nEqs; m++) {
                 if (TT > -1) Wc[m] = (WL[m] + WR[m]) / 2;
                 else Wc[m] = WR[m];
@@ -155,7 +155,7 @@ static void residuum(const Case& c, const Elem& e, const T* Wown, const T* const
             elem_getX(e, cXi, q.currentX);
         }
         const double u[2] = {0, 0};
-        T WInt[5], dWInt[10];
+        T WInt[MAXEQ], dWInt[2 * MAXEQ];
         for (int m = 0; m < nEqs; m++) WInt[m] = 0.0;
         for (int m = 0; m < dim * nEqs; m++) dWInt[m] = 0.0;
         for (int m = 0; m < nEqs; m++)
@@ -163,7 +163,7 @@ static void residuum(const Case& c, const Elem& e, const T* Wown, const T* const
                 WInt[m] += Wown[m * nBasis + j] * base[j];
                 for (int d = 0; d < dim; d++) dWInt[nEqs * d + m] += Wown[m * nBasis + j] * dBase[j * 2 + d];
             }
-        T f[2][5], fv[2][5], product[5];
+        T f[2][MAXEQ], fv[2][MAXEQ], product[MAXEQ];
         double nor[2] = {0, 0};
         if (volumeFluxes)
             for (int d = 0; d < dim; d++) {
@@ -174,7 +174,7 @@ static void residuum(const Case& c, const Elem& e, const T* Wown, const T* const
         if (volumeFluxes && M.diffusive)
             for (int d = 0; d < dim; d++) {
                 nor[d] = 1;
-                diffusiveFlux(M, WInt, dWInt, nor, fv[d]);
+                diffusiveFlux(M, WInt, dWInt, nor, q, fv[d]);
                 nor[d] = 0;
             }
         if (isSourcePresent(M)) sourceTerm(M, WInt, dWInt, q, product);
@@ -213,7 +213,7 @@ static void average_w(const Case& c, const Elem& e, double* Wavg) {
 // shock_senzor (:403-451) is the m = 0 entry.
 static void shock_sensors(const Case& c, const Elem& e, double kap, double* shock) {
     const int nb = e.nBasis;
-    double Ws[5];
+    double Ws[MAXEQ];
     average_w(c, e, Ws);
     for (int m = 0; m < c.Mc.nEqs; m++) {
         std::vector<double> trunc(nb, 0.0);
@@ -253,12 +253,12 @@ void limiter_all(Case& c, bool isFirstIter) {
         e.eps = 0;
         e.c_IP = 0;
         if (e.order <= 1) continue;
-        double Wavg[5];
+        double Wavg[MAXEQ];
         average_w(c, e, Wavg);
         QPData q;
         q.meshVelocity[0] = q.meshVelocity[1] = q.meshVelocity[2] = 0;
         const double lam = maxEigenvalue(c.Mc, Wavg, q);
-        double shock[5];
+        double shock[MAXEQ];
         if (c.P.damp_tol > 0) {
             shock_sensors(c, e, c.P.damp_tol, shock);
             e.eps = lam * e.elemSize / e.order * shock[0];
@@ -515,7 +515,7 @@ static double functional_elem(const Case& c, const Elem& e, int which, const dou
         for (int p = 0; p < e.nqf[k]; p++) {
             const double* baseLeft = &e.baseL[k][p * Nb];
             double Jac = e.JacF[k][p], weight = e.wF[k][p];
-            double WL[5] = {0, 0, 0, 0, 0};
+            double WL[MAXEQ] = {0, 0, 0, 0, 0};
             for (int j = 0; j < nEqs; j++)
                 for (int m = 0; m < Nb; m++) WL[j] += W[j * Nb + m] * baseLeft[m];
             double aux = 0;
@@ -565,7 +565,7 @@ void functional_grad_fd(Case& c, int which, double* dIdW) {
 void model_pointwise(const Case& c, const double* W, const double* dW, double D, double* source, double* fvx, double* fvy) {
     const ModelConst& M = c.Mc;
     const int nEqs = M.nEqs;
-    double w[5], dw[10];
+    double w[MAXEQ], dw[2 * MAXEQ];
     QPData q;
     q.currentX[0] = q.currentX[1] = 0;
     q.meshVelocity[0] = q.meshVelocity[1] = 0;
@@ -576,16 +576,16 @@ void model_pointwise(const Case& c, const double* W, const double* dW, double D,
     if (isSourcePresent(M)) sourceTerm(M, w, dw, q, source);
     const double ex[2] = {1, 0}, ey[2] = {0, 1};
     for (int m = 0; m < nEqs; m++) w[m] = W[m];
-    diffusiveFlux(M, w, dw, ex, fvx);
+    diffusiveFlux(M, w, dw, ex, q, fvx);
     for (int m = 0; m < nEqs; m++) w[m] = W[m];
-    diffusiveFlux(M, w, dw, ey, fvy);
+    diffusiveFlux(M, w, dw, ey, q, fvy);
 }
 
 // numerical flux of one face point and the physical flux of WL; on boundary faces (TT < 0)
 // the outer state is the model's boundary value of WL, as in residuumWall
 void flux_pointwise(const Case& c, const double* WL, const double* WR, const double* n, int TT, double* fnum, double* fphys) {
     const int nEqs = c.Mc.nEqs;
-    double wl[5], wr[5];
+    double wl[MAXEQ], wr[MAXEQ];
     QPData q;
     q.currentX[0] = q.currentX[1] = q.currentX[2] = 0;
     q.meshVelocity[0] = q.meshVelocity[1] = q.meshVelocity[2] = 0;

@@ -102,11 +102,12 @@ ModelConst model_init(const Props& P) {
         M.diffusive = M.k > 0;
         return M;
     }
-    M.kind = P.model == 1 ? 1 : (P.model == 3 ? 3 : 0);
+    M.kind = P.model == 1 ? 1 : (P.model == 3 ? 3 : (P.model == 4 ? 4 : 0));
     // AdiabaticEulers.init: Aerodynamics.init(props, dim, dim + 1, false) (AERO/AdiabaticEulers.java:13-16)
-    M.nEqs = M.kind == 1 ? M.dim + 3 : (M.kind == 3 ? M.dim + 1 : M.dim + 2);
+    // KOmegaSST.init: Aerodynamics.init(props, dim, dim + 4, true) (AERO/KOmegaSST.java:44-47)
+    M.nEqs = M.kind == 1 ? M.dim + 3 : (M.kind == 3 ? M.dim + 1 : (M.kind == 4 ? M.dim + 4 : M.dim + 2));
     // SpalartAllmaras.init calls Aerodynamics.init(props, dim, dim + 3, true): always diffusive
-    M.diffusive = M.kind == 1 ? true : (M.kind == 3 ? false : P.viscous != 0);
+    M.diffusive = (M.kind == 1 || M.kind == 4) ? true : (M.kind == 3 ? false : P.viscous != 0);
     M.vanLeer = M.kind == 0 && P.numerical_flux == 1;   // AERO/Aerodynamics.java:300-303
     if (M.kind == 1) {   // AERO/SpalartAllmaras.java:36-61
         M.vtIn = P.sa_vtIn;
@@ -119,6 +120,22 @@ ModelConst model_init(const Props& P) {
         M.cw3 = P.sa_cw3;
         M.cv1 = P.sa_cv1;
         M.Prt = P.sa_prt;
+    }
+    if (M.kind == 4) {   // AERO/KOmegaSST.java:45-71 (subsonic inlet: WIn is not used)
+        M.kIn = P.kw_kIn;
+        M.omIn = P.kw_omIn;
+        M.suth = P.kw_suth;
+        M.a1kw = P.kw_a1;
+        M.sk1 = P.kw_sk1;
+        M.som1 = P.kw_som1;
+        M.alpha1 = P.kw_alpha1;
+        M.beta1 = P.kw_beta1;
+        M.betast = P.kw_betast;
+        M.sk2 = P.kw_sk2;
+        M.som2 = P.kw_som2;
+        M.alpha2 = P.kw_alpha2;
+        M.beta2 = P.kw_beta2;
+        M.Prt = P.kw_prt;
     }
     // kappa + cv given (AERO/Aerodynamics.java:125-128)
     M.kapa = P.kappa;
@@ -164,6 +181,10 @@ std::vector<double> const_init_condition(const ModelConst& M) {
     if (M.kind == 3) return W;   // AERO/AdiabaticEulers.java:97-126: no energy
     W[M.dim + 1] = M.pOut / (kapa - 1) + 0.5 * rhoIn * VIn * VIn;
     if (M.kind == 1) W[M.dim + 2] = rhoIn * M.vtIn;   // AERO/SpalartAllmaras.java:135
+    if (M.kind == 4) {   // AERO/KOmegaSST.java:109-151 (subsonic branch)
+        W[M.dim + 2] = rhoIn * M.kIn;
+        W[M.dim + 3] = rhoIn * M.omIn;
+    }
     return W;
 }
 

@@ -504,7 +504,7 @@ static void l2_norms_sq(const Case& c, const Elem& e, const std::vector<double>&
 // LocalImplicitSolver.rhsNorm (C/solver/LocalImplicitSolver.java:87-97)
 double rhs_norm(const Case& c) {
     double sum = 0;
-    double tmp[5];
+    double tmp[MAXEQ];
     for (const auto& e : c.E) {
         l2_norms_sq(c, e, e.RHS, tmp);
         for (int m = 0; m < c.Mc.nEqs; m++) sum += tmp[m];
@@ -515,7 +515,7 @@ double rhs_norm(const Case& c) {
 // MasterSolver.residuumNorms (C/solver/MasterSolver.java:113-137)
 std::vector<double> residuum_norms(const Case& c) {
     std::vector<double> norms(c.Mc.nEqs, 0.0);
-    double tmp[5];
+    double tmp[MAXEQ];
     for (const auto& e : c.E) {
         l2_norms_sq(c, e, e.spatialR, tmp);
         for (int m = 0; m < c.Mc.nEqs; m++) norms[m] += tmp[m];
@@ -561,7 +561,7 @@ double time_step(Case& c) {
     double temp = 1.7976931348623157e308;
     const int nEqs = c.Mc.nEqs;
     for (auto& e : c.E) {
-        double Wavg[5] = {0, 0, 0, 0, 0};
+        double Wavg[MAXEQ] = {0, 0, 0, 0, 0};
         for (int j = 0; j < nEqs; j++)
             for (int m = 0; m < e.nBasis; m++) Wavg[j] = Wavg[j] + e.W[j * e.nBasis + m] * e.Is[m];
         double lam;
@@ -610,16 +610,16 @@ static bool positivity_limit(const Case& c, Elem& e) {
     const int Nb = e.nBasis, nEqs = c.Mc.nEqs;
     if (Nb == 1 || c.Mc.kind == 2) return false;
     g_admissible_kind = c.Mc.kind;
-    double Wavg[5] = {0, 0, 0, 0, 0};
+    double Wavg[MAXEQ] = {0, 0, 0, 0, 0};
     for (int j = 0; j < nEqs; j++)
         for (int m = 0; m < Nb; m++) Wavg[j] = Wavg[j] + e.W[j * Nb + m] * e.Is[m];
     if (!admissible(Wavg)) return false;
     auto theta_at = [&](const double* base) {
-        double Wx[5] = {0, 0, 0, 0, 0};
+        double Wx[MAXEQ] = {0, 0, 0, 0, 0};
         for (int m = 0; m < nEqs; m++)
             for (int j = 0; j < Nb; j++) Wx[m] += e.W[m * Nb + j] * base[j];
         if (admissible(Wx)) return 1.0;
-        double lo = 0, hi = 1, Wt[5];
+        double lo = 0, hi = 1, Wt[MAXEQ];
         for (int it = 0; it < 30; it++) {
             double mid = 0.5 * (lo + hi);
             for (int m = 0; m < nEqs; m++) Wt[m] = Wavg[m] + mid * (Wx[m] - Wavg[m]);
@@ -924,7 +924,7 @@ void or_l2_norms_sq(void* h, const double* R, double* out) {
     Case* c = (Case*)h;
     const int nEqs = c->Mc.nEqs;
     for (int m = 0; m < nEqs; m++) out[m] = 0;
-    double tmp[5];
+    double tmp[MAXEQ];
     for (int i = 0; i < c->nElem; i++) {
         const Elem& e = c->E[i];
         std::vector<double> Re(R + c->off[i], R + c->off[i] + (size_t)e.nBasis * nEqs);

@@ -465,10 +465,11 @@ static void tet_residuum(const TetCase& c, const TetElem& e, const T* Wown, cons
             convectiveFlux(M, WInt, nor, f[d]);
             nor[d] = 0;
         }
+        const QPData qd{};   // static mesh, no wall distance on tetrahedra
         if (M.diffusive)
             for (int d = 0; d < dim; d++) {
                 nor[d] = 1;
-                diffusiveFlux(M, WInt, dWInt, nor, fv[d]);
+                diffusiveFlux(M, WInt, dWInt, nor, qd, fv[d]);
                 nor[d] = 0;
             }
         for (int m = 0; m < nEqs; m++)
