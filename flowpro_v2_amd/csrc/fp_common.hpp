@@ -144,6 +144,21 @@ struct MixedMesh {
 };
 
 // compile-time shape of the element kernels selected at run time
+// Device-resident GMRES control block (Gmres.solve bookkeeping, Gmres.java:42-122): the Givens
+// rotations, the convergence test and the triangular solve run on the device, and every kernel of
+// an Arnoldi step returns at once when `stop` is set, so the host enqueues several steps without
+// waiting for their Hessenberg columns.
+struct GmresCtl {
+    int conv;     // the relative residual reached tol (the solve is done)
+    int stop;     // the current cycle is over (converged, or its last step done)
+    int nused;    // Arnoldi vectors of the cycle's update
+    int iters;    // Arnoldi steps performed in this solve
+    double bnrm2; // |M^-1 b| (1 if zero)
+    double error; // the last relative residual estimate
+    double nr;    // |r| at the start of the cycle
+    double tol;
+};
+
 struct ElemLaunch {
     int model, nb, neq, nqv, nqf, visc, av;
 };

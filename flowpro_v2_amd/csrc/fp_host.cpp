@@ -57,7 +57,8 @@ hipError_t launch_jacobian_tet_team(const ElemLaunch&, hipStream_t, const ModelP
 hipError_t launch_size_over_lambda_tet(const ElemLaunch&, hipStream_t, const ModelParams&, const TetMesh&, const double*,
                                        const double*, const double*, double*);
 hipError_t launch_positivity_tet(const ElemLaunch&, hipStream_t, const TetTables&, const TetMesh&, double*, const double*, int*);
-hipError_t launch_spmv(hipStream_t, int, int, const int*, const int*, const double*, const double*, const double*, double*, int);
+hipError_t launch_spmv(hipStream_t, int, int, const int*, const int*, const double*, const double*, const double*, double*, int,
+                       const int* skip = nullptr);
 template <typename TF> hipError_t launch_ilu_copy(hipStream_t, size_t, const double*, TF*);
 template <typename TF>
 hipError_t launch_ilu_factor_level(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, int*);
@@ -74,11 +75,11 @@ template <typename TF>
 hipError_t launch_ilu_fwd_level(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);
 template <typename TF>
 hipError_t launch_ilu_sweep_flow(hipStream_t, int, int, const int*, int, const TF*, IluIO*, const unsigned long long*,
-                                 unsigned long long*, int);
+                                 unsigned long long*, int, const int* skip = nullptr);
 template <typename TF>
 hipError_t launch_ilu_bwd_level(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*);
-hipError_t launch_set_io(hipStream_t, IluIO*, const double*, double*);
-hipError_t launch_block_jacobi(hipStream_t, int, int, double*, const double*, double*);
+hipError_t launch_set_io(hipStream_t, IluIO*, const double*, double*, const int* skip = nullptr);
+hipError_t launch_block_jacobi(hipStream_t, int, int, double*, const double*, double*, const int* skip = nullptr);
 hipError_t launch_gather_diag(hipStream_t, int, int, const int*, const double*, double*);
 hipError_t launch_min(hipStream_t, size_t, const double*, double*, double*);
 int min_reduction_partials();
@@ -106,13 +107,18 @@ hipError_t launch_jvp_quad(const ElemLaunch&, hipStream_t, const ModelParams&, c
 hipError_t launch_size_over_lambda_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const QuadMesh&, const double*,
                                         const double*, double*);
 hipError_t launch_positivity_quad(const ElemLaunch&, hipStream_t, const QuadTables&, const QuadMesh&, double*, int*);
-hipError_t launch_dot(hipStream_t, size_t, const double*, const double*, double*, double*, int);
-hipError_t launch_axpy_dot(hipStream_t, size_t, double*, const double*, const double*, const double*, double*, double*, int);
-hipError_t launch_scale_div(hipStream_t, size_t, double*, const double*, const double*);
-hipError_t launch_update_solution(hipStream_t, size_t, int, const double*, size_t, const double*, double*);
+hipError_t launch_dot(hipStream_t, size_t, const double*, const double*, double*, double*, int, const int* skip = nullptr);
+hipError_t launch_axpy_dot(hipStream_t, size_t, double*, const double*, const double*, const double*, double*, double*, int,
+                           const int* skip = nullptr);
+hipError_t launch_scale_div(hipStream_t, size_t, double*, const double*, const double*, const int* skip = nullptr);
+hipError_t launch_update_solution(hipStream_t, size_t, int, const double*, size_t, const double*, double*,
+                                  const int* nvp = nullptr);
+hipError_t launch_gmres_cycle_begin(hipStream_t, GmresCtl*, double*, double*, int);
+hipError_t launch_gmres_givens(hipStream_t, GmresCtl*, int, int, const double*, double*, double*, double*, double*);
+hipError_t launch_gmres_lsolve(hipStream_t, const GmresCtl*, int, const double*, const double*, double*, double*, double*);
 hipError_t launch_axpy(hipStream_t, size_t, double, const double*, double*);
 hipError_t launch_fill(hipStream_t, size_t, double, double*);
-hipError_t launch_copy(hipStream_t, size_t, const double*, double*);
+hipError_t launch_copy(hipStream_t, size_t, const double*, double*, const int* skip = nullptr);
 hipError_t launch_nonfinite(hipStream_t, size_t, const double*, int*);
 }  // namespace fp
 
@@ -174,6 +180,9 @@ struct fp_ctx {
     double *W = nullptr, *Wo = nullptr, *Wo2 = nullptr, *R = nullptr, *RHS = nullptr, *enorm = nullptr, *tmp = nullptr;
     double *x = nullptr, *V = nullptr, *w = nullptr, *r = nullptr, *aux = nullptr, *partial = nullptr, *scal = nullptr;
     double* y = nullptr;
+    double* gm = nullptr;         // GMRES on the device: H, its working copy, cs, sn, s, the copy of s
+    GmresCtl* gctl = nullptr;
+    int gmres_chunk = 4;          // Arnoldi steps enqueued between two reads of the control block
     int* istat = nullptr;
     IluIO* io = nullptr;
     bool use_graph = true;
@@ -755,7 +764,7 @@ static void free_all(fp_ctx* c) {
     if (c->io) (void)hipFree(c->io);
     void* ptrs[] = {c->m_nvd, c->m_nqvd, c->m_nbd, c->m_nqfd, c->m_vol, c->m_face, c->qv, c->qf, c->qn, c->xq, c->qmass, c->qminv, c->qis, c->av, c->avold, c->trunc, c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
                     c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->meta_tf, c->meta_tb, c->val, c->F, c->iy, c->ixs, c->ypk, c->xpk, c->fflag, c->fctl, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
-                    c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->istat};
+                    c->enorm, c->tmp, c->x, c->V, c->w, c->r, c->aux, c->partial, c->scal, c->y, c->gm, c->gctl, c->istat};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->s) (void)hipStreamDestroy(c->s);
@@ -2072,6 +2081,12 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     HIPCHK(c, dalloc(&c->partial, 4096));
     HIPCHK(c, dalloc(&c->scal, 64 + 2 * (size_t)c->m + 8));
     HIPCHK(c, dalloc(&c->y, (size_t)c->m + 1));
+    HIPCHK(c, dalloc(&c->gm, 2 * (size_t)(c->m + 1) * c->m + 4 * (size_t)c->m + 4));
+    HIPCHK(c, hipMalloc((void**)&c->gctl, sizeof(GmresCtl)));
+    {
+        const char* k = std::getenv("FLOWPRO_GMRES_CHUNK");   // Arnoldi steps per read of the control block
+        if (k && std::atoi(k) > 0) c->gmres_chunk = std::atoi(k);
+    }
     HIPCHK(c, dalloc(&c->istat, 4));
     HIPCHK(c, hipMalloc((void**)&c->io, sizeof(IluIO)));
     HIPCHK(c, hipMemsetAsync(c->io, 0, sizeof(IluIO), c->s));
@@ -2431,6 +2446,15 @@ template <typename TF> static int ilu_apply_t(fp_ctx* c, double* out, const doub
     if (c->ilu_flow) return ilu_apply_launches<TF>(c);   // two launches, no graph needed
     return run_captured<TF>(c, &c->g_apply, ilu_apply_launches<TF>);
 }
+// the dataflow application with every launch returning at once when *skip is set (GMRES steps
+// enqueued past the end of a cycle)
+template <typename TF> static int ilu_apply_skip_t(fp_ctx* c, double* out, const double* in, const int* skip) {
+    const TF* F = (const TF*)c->F;
+    HIPCHK(c, launch_set_io(c->s, c->io, in, out, skip));
+    HIPCHK(c, launch_ilu_sweep_flow<TF>(c->s, c->B, 0, c->meta_f, c->nElem, F, c->io, nullptr, c->ypk, c->flow_groups, skip));
+    HIPCHK(c, launch_ilu_sweep_flow<TF>(c->s, c->B, 1, c->meta_b, c->nElem, F, c->io, c->ypk, c->xpk, c->flow_groups, skip));
+    return FP_OK;
+}
 
 // a wait of a dataflow sweep that exceeded its spin limit (a broken dependency order) voids the
 // applications of the solve
@@ -2539,14 +2563,17 @@ static int op_prec(fp_ctx* c, double* out, const double* in) {
 }
 
 // Gmres.solve (FlowPro/src/main/java/flowpro/core/LinearSolvers/Gmres.java:42-122): left
-// preconditioned, restart m, modified Gram-Schmidt, residual relative to |M^-1 b|.
+// preconditioned, restart m, modified Gram-Schmidt, residual relative to |M^-1 b|.  The scalar
+// bookkeeping (Hessenberg column, Givens rotations, convergence test, triangular solve) runs on the
+// device (k_gmres_*), and the kernels of an Arnoldi step return at once once the cycle is over, so the
+// host enqueues gmres_chunk steps between two reads of the control block instead of waiting for every
+// Hessenberg column.  Operators or preconditioners whose kernels do not take the stop flag (the
+// matrix-free products, the per-level ILU graphs) read it after every step.
 static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_out, bool* ok, bool x_zero) {
     const size_t N = (size_t)c->dofs;
     const int m = c->m;
     int rc;
     *iters = 0;
-    std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
-    std::vector<double> cs(m), sn(m), s(m + 1);
     double bnrm2, nr, error;
     if ((rc = op_prec(c, c->r, b))) return rc;
     if ((rc = dnorm(c, c->r, &bnrm2))) return rc;
@@ -2567,61 +2594,72 @@ static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_
     }
     error = nr / bnrm2;
     if (error < c->tol) { *ok = true; *err_out = error; return FP_OK; }
-    double* hdev = c->scal + 8;   // H column on the device
-    std::vector<double> hcol(m + 2);
+    double* hdev = c->scal + 8;   // the Gram-Schmidt dots of the current step
+    double* H = c->gm;
+    double* Hw = H + (size_t)(m + 1) * m;
+    double* cs = Hw + (size_t)(m + 1) * m;
+    double* sn = cs + m;
+    double* sv = sn + m;
+    double* bw = sv + m + 1;
+    GmresCtl* ctl = c->gctl;
+    const int* stop = &ctl->stop;
+    const bool predicated = c->op == 0 && (c->precond != 2 || c->ilu_flow);
+    const int chunk = predicated ? c->gmres_chunk : 1;
+    GmresCtl h{0, 0, 0, 0, bnrm2, error, nr, c->tol};
+    HIPCHK(c, hipMemcpyAsync(ctl, &h, sizeof(GmresCtl), hipMemcpyHostToDevice, c->s));
     for (int it = 0; it < c->cycles; it++) {
-        const double norm_r = nr;   // |r| of the r computed last (before the loop or at the cycle's end)
-        HIPCHK(c, hipMemcpyAsync(c->scal + 1, &norm_r, sizeof(double), hipMemcpyHostToDevice, c->s));
-        HIPCHK(c, launch_scale_div(c->s, N, c->V, c->r, c->scal + 1));
-        std::fill(s.begin(), s.end(), 0.0);
-        s[0] = norm_r;
-        int nused = m;
-        bool done = false;
+        HIPCHK(c, launch_gmres_cycle_begin(c->s, ctl, H, sv, m));
+        HIPCHK(c, launch_scale_div(c->s, N, c->V, c->r, &ctl->nr));
         for (int i = 0; i < m; i++) {
+            if (i > 0 && i % chunk == 0) {   // read the control block: stop enqueueing once the cycle is over
+                HIPCHK(c, hipMemcpyAsync(&h, ctl, sizeof(GmresCtl), hipMemcpyDeviceToHost, c->s));
+                HIPCHK(c, hipStreamSynchronize(c->s));
+                if (h.stop) break;
+            }
             double* Vi = c->V + (size_t)i * N;
-            if ((rc = op_apply(c, Vi, c->aux))) return rc;
-            if ((rc = op_prec(c, c->w, c->aux))) return rc;
-            (*iters)++;
+            const int* sk = predicated ? stop : nullptr;
+            if (c->op == 0) {
+                HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, Vi, nullptr, c->aux, 0, sk));
+                if (c->precond == 2) {
+                    rc = c->bits == 32 ? ilu_apply_skip_t<float>(c, c->w, c->aux, sk) : ilu_apply_skip_t<double>(c, c->w, c->aux, sk);
+                    if (rc) return rc;
+                } else if (c->precond == 1) {
+                    HIPCHK(c, launch_block_jacobi(c->s, c->nElem, c->B, c->bjw, c->aux, c->w, sk));
+                } else {
+                    HIPCHK(c, launch_copy(c->s, N, c->aux, c->w, sk));
+                }
+            } else {
+                if ((rc = op_apply(c, Vi, c->aux))) return rc;
+                if ((rc = op_prec(c, c->w, c->aux))) return rc;
+            }
             // MGS: h_k = (w, V_k); w -= h_k V_k  (the next dot is fused into the update)
-            HIPCHK(c, launch_dot(c->s, N, c->w, c->V, c->partial, hdev, 0));
+            HIPCHK(c, launch_dot(c->s, N, c->w, c->V, c->partial, hdev, 0, sk));
             for (int k = 0; k <= i; k++) {
                 const double* u = k < i ? c->V + (size_t)(k + 1) * N : nullptr;
                 HIPCHK(c, launch_axpy_dot(c->s, N, c->w, c->V + (size_t)k * N, hdev + k, u, c->partial, hdev + k + 1,
-                                          k < i ? 0 : 1));
+                                          k < i ? 0 : 1, sk));
             }
-            HIPCHK(c, launch_scale_div(c->s, N, c->V + (size_t)(i + 1) * N, c->w, hdev + i + 1));
-            HIPCHK(c, hipMemcpyAsync(hcol.data(), hdev, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, c->s));
-            HIPCHK(c, hipStreamSynchronize(c->s));
-            for (int k = 0; k <= i + 1; k++) H[k][i] = hcol[k];
-            for (int k = 0; k <= i - 1; k++) {
-                double temp = cs[k] * H[k][i] + sn[k] * H[k + 1][i];
-                H[k + 1][i] = -sn[k] * H[k][i] + cs[k] * H[k + 1][i];
-                H[k][i] = temp;
-            }
-            rotmat(H[i][i], H[i + 1][i], cs[i], sn[i]);
-            double temp = cs[i] * s[i];
-            s[i + 1] = -sn[i] * s[i];
-            s[i] = temp;
-            H[i][i] = cs[i] * H[i][i] + sn[i] * H[i + 1][i];
-            H[i + 1][i] = 0;
-            error = std::fabs(s[i + 1]) / bnrm2;
-            if (error <= c->tol) {
-                nused = i + 1;
-                done = true;
-                break;
-            }
+            HIPCHK(c, launch_scale_div(c->s, N, c->V + (size_t)(i + 1) * N, c->w, hdev + i + 1, sk));
+            HIPCHK(c, launch_gmres_givens(c->s, ctl, i, m, hdev, H, cs, sn, sv));
         }
-        std::vector<double> yv = lsolve(H, s, nused);
-        HIPCHK(c, hipMemcpyAsync(c->y, yv.data(), sizeof(double) * nused, hipMemcpyHostToDevice, c->s));
-        HIPCHK(c, launch_update_solution(c->s, N, nused, c->V, N, c->y, x));
-        if (done) break;
+        HIPCHK(c, launch_gmres_lsolve(c->s, ctl, m, H, sv, Hw, bw, c->y));
+        HIPCHK(c, launch_update_solution(c->s, N, 0, c->V, N, c->y, x, &ctl->nused));
+        HIPCHK(c, hipMemcpyAsync(&h, ctl, sizeof(GmresCtl), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        error = h.error;
+        if (h.conv) break;
         if ((rc = op_residual(c, x, b, c->aux))) return rc;
         if ((rc = op_prec(c, c->r, c->aux))) return rc;
         if ((rc = dnorm(c, c->r, &nr))) return rc;
         error = nr / bnrm2;
         if (error <= c->tol) break;
+        h.nr = nr;
+        h.error = error;
+        HIPCHK(c, hipMemcpyAsync(ctl, &h, sizeof(GmresCtl), hipMemcpyHostToDevice, c->s));
     }
+    HIPCHK(c, hipMemcpyAsync(&h, ctl, sizeof(GmresCtl), hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
+    *iters = h.iters;
     *ok = error <= c->tol;
     *err_out = error;
     return FP_OK;

@@ -25,7 +25,9 @@ namespace fp {
 // (FlowPro/src/main/java/flowpro/core/LinearSolvers/SparseMatrix.java:177-204).
 __global__ void __launch_bounds__(256) k_bsr_spmv(int nrows, int B, const int* __restrict__ rowptr, const int* __restrict__ colidx,
                                                   const double* __restrict__ val, const double* __restrict__ x,
-                                                  const double* __restrict__ b, double* __restrict__ y, int mode) {
+                                                  const double* __restrict__ b, double* __restrict__ y, int mode,
+                                                  const int* __restrict__ skip) {
+    if (skip && *skip) return;
     long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int r = (int)(g / B), t = (int)(g % B);
     if (r >= nrows) return;
@@ -48,7 +50,9 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int nrows, int B, const int* _
 template <int B>
 __global__ void __launch_bounds__(256) k_bsr_spmv_b(int nrows, const int* __restrict__ rowptr, const int* __restrict__ colidx,
                                                     const double* __restrict__ val, const double* __restrict__ x,
-                                                    const double* __restrict__ b, double* __restrict__ y, int mode) {
+                                                    const double* __restrict__ b, double* __restrict__ y, int mode,
+                                                    const int* __restrict__ skip) {
+    if (skip && *skip) return;
     long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int r = (int)(g / B), t = (int)(g % B);
     if (r >= nrows) return;
@@ -877,7 +881,8 @@ __global__ void __launch_bounds__(64) k_ilu_bwd_level(const int* __restrict__ me
     io->x[(size_t)i * B + t] = (double)v;
 }
 
-__global__ void k_set_io(IluIO* io, const double* b, double* x) {
+__global__ void k_set_io(IluIO* io, const double* b, double* x, const int* skip) {
+    if (skip && *skip) return;   // the application is skipped whole: the epoch stays
     io->b = b;
     io->x = x;
     io->epoch += 1;
@@ -1019,7 +1024,8 @@ __device__ inline void flow_load(TF (&Lr)[B], const TF* blk, int t) {
 template <typename TF, int B, int WPG, int DIR>
 __global__ void __launch_bounds__(64 * WPG) k_ilu_sweep_flow(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
                                                              IluIO* __restrict__ io, const unsigned long long* __restrict__ ypk,
-                                                             unsigned long long* __restrict__ out) {
+                                                             unsigned long long* __restrict__ out, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     constexpr int NPF = FlowPrefetch<TF, B>::value;
     constexpr size_t BB = (size_t)B * B;
     constexpr bool FIRST = DIR == 0 || DIR == 2;    // reads b (else the first sweep's values)
@@ -1093,7 +1099,8 @@ __global__ void __launch_bounds__(64 * WPG) k_ilu_sweep_flow(const int* __restri
 // of one linear solve, as the reference overwrites ADiag), then back substitution.
 // One workgroup per element.  A[p][i] of the reference = work[blk*B*B + p*B + i].
 __global__ void __launch_bounds__(128) k_block_jacobi(int nElem, int B, double* __restrict__ work, const double* __restrict__ b,
-                                                      double* __restrict__ x) {
+                                                      double* __restrict__ x, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     extern __shared__ unsigned char smem_raw[];
     double* bs = reinterpret_cast<double*>(smem_raw);
     double* xs = bs + B;
@@ -1142,7 +1149,8 @@ __device__ inline double block_sum(double v) {
 }
 
 __global__ void __launch_bounds__(RED_THREADS) k_dot_partial(size_t n, const double* __restrict__ a, const double* __restrict__ b,
-                                                             double* __restrict__ partial) {
+                                                             double* __restrict__ partial, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     double s = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         s += b ? a[i] * b[i] : a[i];
@@ -1152,7 +1160,8 @@ __global__ void __launch_bounds__(RED_THREADS) k_dot_partial(size_t n, const dou
 
 // out = sum(partial) (op 0), sqrt(sum) (op 1)
 __global__ void __launch_bounds__(RED_THREADS) k_reduce_final(int np, const double* __restrict__ partial, double* __restrict__ out,
-                                                              int op) {
+                                                              int op, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     double s = 0.0;
     for (int i = threadIdx.x; i < np; i += blockDim.x) s += partial[i];
     s = block_sum(s);
@@ -1162,7 +1171,8 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce_final(int np, const doub
 // w -= h * v ; partial dot(w, u) (u may be null)
 __global__ void __launch_bounds__(RED_THREADS) k_axpy_dot(size_t n, double* __restrict__ w, const double* __restrict__ v,
                                                           const double* __restrict__ h, const double* __restrict__ u,
-                                                          double* __restrict__ partial) {
+                                                          double* __restrict__ partial, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const double hv = *h;
     double s = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -1176,15 +1186,18 @@ __global__ void __launch_bounds__(RED_THREADS) k_axpy_dot(size_t n, double* __re
 }
 
 // dst = (nrm > 0) ? src / nrm : 0   (Gmres.java:82-85, happy breakdown)
-__global__ void k_scale_div(size_t n, double* __restrict__ dst, const double* __restrict__ src, const double* __restrict__ nrm) {
+__global__ void k_scale_div(size_t n, double* __restrict__ dst, const double* __restrict__ src, const double* __restrict__ nrm,
+                            const int* __restrict__ skip) {
+    if (skip && *skip) return;
     const double h = *nrm;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         dst[i] = h > 0 ? src[i] / h : 0.0;
 }
 
-// x += sum_k V_k y_k (Gmres.updateSolution)
+// x += sum_k V_k y_k (Gmres.updateSolution); nvp (if set): the count read from the device
 __global__ void k_update_solution(size_t n, int nv, const double* __restrict__ V, size_t ldv, const double* __restrict__ y,
-                                  double* __restrict__ x) {
+                                  double* __restrict__ x, const int* __restrict__ nvp) {
+    if (nvp) nv = *nvp;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         double xi = x[i];
         for (int k = 0; k < nv; k++) xi += V[(size_t)k * ldv + i] * y[k];
@@ -1200,7 +1213,8 @@ __global__ void k_fill(size_t n, double v, double* __restrict__ y) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = v;
 }
 
-__global__ void k_copy(size_t n, const double* __restrict__ x, double* __restrict__ y) {
+__global__ void k_copy(size_t n, const double* __restrict__ x, double* __restrict__ y, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = x[i];
 }
 
@@ -1212,6 +1226,108 @@ __global__ void k_nonfinite(size_t n, const double* __restrict__ x, int* __restr
     if (bad) atomicOr(flag, 1);
 }
 
+// ------------------------------------------------------------------ GMRES bookkeeping
+// Single-thread kernels mirroring the scalar part of Gmres.solve (Gmres.java:42-122, rotmat :132-148,
+// lsolve :177-199) with the reference's operation order and no fused multiply-add, so that the
+// rotations, the residual estimate and the iteration count are those of the host restatement.
+// H is (m+1) x m row-major, the column of step i from the Gram-Schmidt dots in hcol[0..i+1].
+// the start of a cycle: H zero (the entries below its subdiagonal are never written and must read
+// as zero in lsolve), s = (|r|, 0, ...)
+__global__ void k_gmres_cycle_begin(GmresCtl* ctl, double* H, double* s, int m) {
+    const bool conv = ctl->conv != 0;
+    __syncthreads();
+    if (conv) {
+        if (threadIdx.x == 0) ctl->stop = 1;
+        return;
+    }
+    for (int k = threadIdx.x; k < (m + 1) * m; k += blockDim.x) H[k] = 0.0;
+    for (int k = threadIdx.x; k <= m; k += blockDim.x) s[k] = k == 0 ? ctl->nr : 0.0;
+    if (threadIdx.x == 0) {
+        ctl->stop = 0;
+        ctl->nused = 0;
+    }
+}
+
+__global__ void k_gmres_givens(GmresCtl* ctl, int i, int m, const double* __restrict__ hcol, double* __restrict__ H,
+                               double* __restrict__ cs, double* __restrict__ sn, double* __restrict__ s) {
+#pragma clang fp contract(off)
+    if (ctl->stop) return;
+    ctl->iters += 1;
+    for (int k = 0; k <= i + 1; k++) H[k * m + i] = hcol[k];
+    for (int k = 0; k <= i - 1; k++) {
+        const double temp = cs[k] * H[k * m + i] + sn[k] * H[(k + 1) * m + i];
+        H[(k + 1) * m + i] = -sn[k] * H[k * m + i] + cs[k] * H[(k + 1) * m + i];
+        H[k * m + i] = temp;
+    }
+    const double a = H[i * m + i], b = H[(i + 1) * m + i];
+    double c, sg;
+    if (b == 0) {
+        c = 1;
+        sg = 0;
+    } else if (fabs(b) > fabs(a)) {
+        const double t = a / b;
+        sg = 1 / sqrt(1 + t * t);
+        c = t * sg;
+    } else {
+        const double t = b / a;
+        c = 1 / sqrt(1 + t * t);
+        sg = t * c;
+    }
+    cs[i] = c;
+    sn[i] = sg;
+    const double temp = c * s[i];
+    s[i + 1] = -sg * s[i];
+    s[i] = temp;
+    H[i * m + i] = c * H[i * m + i] + sg * H[(i + 1) * m + i];
+    H[(i + 1) * m + i] = 0;
+    const double error = fabs(s[i + 1]) / ctl->bnrm2;
+    ctl->error = error;
+    if (error <= ctl->tol) {
+        ctl->nused = i + 1;
+        ctl->conv = 1;
+        ctl->stop = 1;
+    } else if (i == m - 1) {
+        ctl->nused = m;
+        ctl->stop = 1;
+    }
+}
+
+// y = lsolve(H, s, nused): elimination below the diagonal, then back substitution; A is a working
+// copy of H (the reference passes H by value)
+__global__ void k_gmres_lsolve(const GmresCtl* ctl, int m, const double* __restrict__ H, const double* __restrict__ s,
+                               double* __restrict__ A, double* __restrict__ bw, double* __restrict__ y) {
+#pragma clang fp contract(off)
+    const int N = ctl->nused;
+    for (int k = 0; k < N * m; k++) A[k] = H[k];
+    for (int k = 0; k < N; k++) bw[k] = s[k];
+    for (int p = 0; p < N; p++)
+        for (int i = p + 1; i < N; i++) {
+            const double alpha = A[i * m + p] / A[p * m + p];
+            bw[i] -= alpha * bw[p];
+            for (int j = p; j < N; j++) A[i * m + j] -= alpha * A[p * m + j];
+        }
+    for (int i = N - 1; i >= 0; i--) {
+        double sum = 0.0;
+        for (int j = i + 1; j < N; j++) sum += A[i * m + j] * y[j];
+        y[i] = (bw[i] - sum) / A[i * m + i];
+    }
+}
+
+hipError_t launch_gmres_cycle_begin(hipStream_t st, GmresCtl* ctl, double* H, double* s, int m) {
+    hipLaunchKernelGGL(k_gmres_cycle_begin, dim3(1), dim3(256), 0, st, ctl, H, s, m);
+    return hipGetLastError();
+}
+hipError_t launch_gmres_givens(hipStream_t st, GmresCtl* ctl, int i, int m, const double* hcol, double* H, double* cs, double* sn,
+                               double* s) {
+    hipLaunchKernelGGL(k_gmres_givens, dim3(1), dim3(1), 0, st, ctl, i, m, hcol, H, cs, sn, s);
+    return hipGetLastError();
+}
+hipError_t launch_gmres_lsolve(hipStream_t st, const GmresCtl* ctl, int m, const double* H, const double* s, double* A, double* bw,
+                               double* y) {
+    hipLaunchKernelGGL(k_gmres_lsolve, dim3(1), dim3(1), 0, st, ctl, m, H, s, A, bw, y);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ host launchers
 static inline unsigned grid_for(size_t n, int t) {
     size_t g = (n + t - 1) / t;
@@ -1221,17 +1337,17 @@ static inline unsigned grid_for(size_t n, int t) {
 }
 
 hipError_t launch_spmv(hipStream_t s, int nrows, int B, const int* rowptr, const int* colidx, const double* val,
-                       const double* x, const double* b, double* y, int mode) {
+                       const double* x, const double* b, double* y, int mode, const int* skip) {
     long long n = (long long)nrows * B;
     const dim3 grd((unsigned)((n + 255) / 256)), blk(256);
     switch (B) {
-        case 20: hipLaunchKernelGGL(k_bsr_spmv_b<20>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
-        case 24: hipLaunchKernelGGL(k_bsr_spmv_b<24>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
-        case 30: hipLaunchKernelGGL(k_bsr_spmv_b<30>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
-        case 36: hipLaunchKernelGGL(k_bsr_spmv_b<36>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
-        case 40: hipLaunchKernelGGL(k_bsr_spmv_b<40>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
-        case 64: hipLaunchKernelGGL(k_bsr_spmv_b<64>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode); break;
-        default: hipLaunchKernelGGL(k_bsr_spmv, grd, blk, 0, s, nrows, B, rowptr, colidx, val, x, b, y, mode);
+        case 20: hipLaunchKernelGGL(k_bsr_spmv_b<20>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode, skip); break;
+        case 24: hipLaunchKernelGGL(k_bsr_spmv_b<24>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode, skip); break;
+        case 30: hipLaunchKernelGGL(k_bsr_spmv_b<30>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode, skip); break;
+        case 36: hipLaunchKernelGGL(k_bsr_spmv_b<36>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode, skip); break;
+        case 40: hipLaunchKernelGGL(k_bsr_spmv_b<40>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode, skip); break;
+        case 64: hipLaunchKernelGGL(k_bsr_spmv_b<64>, grd, blk, 0, s, nrows, rowptr, colidx, val, x, b, y, mode, skip); break;
+        default: hipLaunchKernelGGL(k_bsr_spmv, grd, blk, 0, s, nrows, B, rowptr, colidx, val, x, b, y, mode, skip);
     }
     return hipGetLastError();
 }
@@ -1361,23 +1477,23 @@ hipError_t launch_ilu_bwd_level(hipStream_t s, int B, const int* meta, int nrows
 
 template <typename TF>
 hipError_t launch_ilu_sweep_flow(hipStream_t s, int B, int dir, const int* meta, int nrows, const TF* F, IluIO* io,
-                                 const unsigned long long* ypk, unsigned long long* out, int max_groups) {
+                                 const unsigned long long* ypk, unsigned long long* out, int max_groups, const int* skip) {
     constexpr int WPG = 4;
     const int groups = std::max(1, std::min((nrows + WPG - 1) / WPG, max_groups));
 #define X(BS)                                                                                                        \
     if (B == BS) {                                                                                                   \
         if (dir == 0)                                                                                                \
             hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 0>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
-                               io, ypk, out);                                                                        \
+                               io, ypk, out, skip);                                                                  \
         else if (dir == 1)                                                                                           \
             hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 1>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
-                               io, ypk, out);                                                                        \
+                               io, ypk, out, skip);                                                                  \
         else if (dir == 2)                                                                                           \
             hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 2>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
-                               io, ypk, out);                                                                        \
+                               io, ypk, out, skip);                                                                  \
         else                                                                                                         \
             hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 3>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
-                               io, ypk, out);                                                                        \
+                               io, ypk, out, skip);                                                                  \
         return hipGetLastError();                                                                                    \
     }
     FP_BLOCK_SIZES(X)
@@ -1385,14 +1501,14 @@ hipError_t launch_ilu_sweep_flow(hipStream_t s, int B, int dir, const int* meta,
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_set_io(hipStream_t s, IluIO* io, const double* b, double* x) {
-    hipLaunchKernelGGL(k_set_io, dim3(1), dim3(1), 0, s, io, b, x);
+hipError_t launch_set_io(hipStream_t s, IluIO* io, const double* b, double* x, const int* skip) {
+    hipLaunchKernelGGL(k_set_io, dim3(1), dim3(1), 0, s, io, b, x, skip);
     return hipGetLastError();
 }
 
 #define FP_ILU_INST(TF)                                                                                                       \
     template hipError_t launch_ilu_sweep_flow<TF>(hipStream_t, int, int, const int*, int, const TF*, IluIO*,                  \
-                                                  const unsigned long long*, unsigned long long*, int);                        \
+                                                  const unsigned long long*, unsigned long long*, int, const int*);            \
     template hipError_t launch_ilu_copy<TF>(hipStream_t, size_t, const double*, TF*);                                          \
     template hipError_t launch_ilu_factor_level<TF>(hipStream_t, int, const int*, int, const int*, const int*, const int*, TF*, \
                                                     int*);                                                                     \
@@ -1405,8 +1521,8 @@ hipError_t launch_set_io(hipStream_t s, IluIO* io, const double* b, double* x) {
 FP_ILU_INST(double)
 FP_ILU_INST(float)
 
-hipError_t launch_block_jacobi(hipStream_t s, int nElem, int B, double* work, const double* b, double* x) {
-    hipLaunchKernelGGL(k_block_jacobi, dim3(nElem), dim3(128), 2 * B * sizeof(double), s, nElem, B, work, b, x);
+hipError_t launch_block_jacobi(hipStream_t s, int nElem, int B, double* work, const double* b, double* x, const int* skip) {
+    hipLaunchKernelGGL(k_block_jacobi, dim3(nElem), dim3(128), 2 * B * sizeof(double), s, nElem, B, work, b, x, skip);
     return hipGetLastError();
 }
 
@@ -1491,26 +1607,28 @@ hipError_t launch_colsum(hipStream_t s, int nElem, int nc, const double* en, dou
     return hipGetLastError();
 }
 
-hipError_t launch_dot(hipStream_t s, size_t n, const double* a, const double* b, double* partial, double* out, int op) {
-    hipLaunchKernelGGL(k_dot_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, n, a, b, partial);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, s, RED_BLOCKS, partial, out, op);
+hipError_t launch_dot(hipStream_t s, size_t n, const double* a, const double* b, double* partial, double* out, int op,
+                      const int* skip) {
+    hipLaunchKernelGGL(k_dot_partial, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, n, a, b, partial, skip);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, s, RED_BLOCKS, partial, out, op, skip);
     return hipGetLastError();
 }
 
 hipError_t launch_axpy_dot(hipStream_t s, size_t n, double* w, const double* v, const double* h, const double* u,
-                           double* partial, double* out, int op) {
-    hipLaunchKernelGGL(k_axpy_dot, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, n, w, v, h, u, partial);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, s, RED_BLOCKS, partial, out, op);
+                           double* partial, double* out, int op, const int* skip) {
+    hipLaunchKernelGGL(k_axpy_dot, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, s, n, w, v, h, u, partial, skip);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, s, RED_BLOCKS, partial, out, op, skip);
     return hipGetLastError();
 }
 
-hipError_t launch_scale_div(hipStream_t s, size_t n, double* dst, const double* src, const double* nrm) {
-    hipLaunchKernelGGL(k_scale_div, dim3(grid_for(n, 256)), dim3(256), 0, s, n, dst, src, nrm);
+hipError_t launch_scale_div(hipStream_t s, size_t n, double* dst, const double* src, const double* nrm, const int* skip) {
+    hipLaunchKernelGGL(k_scale_div, dim3(grid_for(n, 256)), dim3(256), 0, s, n, dst, src, nrm, skip);
     return hipGetLastError();
 }
 
-hipError_t launch_update_solution(hipStream_t s, size_t n, int nv, const double* V, size_t ldv, const double* y, double* x) {
-    hipLaunchKernelGGL(k_update_solution, dim3(grid_for(n, 256)), dim3(256), 0, s, n, nv, V, ldv, y, x);
+hipError_t launch_update_solution(hipStream_t s, size_t n, int nv, const double* V, size_t ldv, const double* y, double* x,
+                                  const int* nvp) {
+    hipLaunchKernelGGL(k_update_solution, dim3(grid_for(n, 256)), dim3(256), 0, s, n, nv, V, ldv, y, x, nvp);
     return hipGetLastError();
 }
 
@@ -1524,8 +1642,8 @@ hipError_t launch_fill(hipStream_t s, size_t n, double v, double* y) {
     return hipGetLastError();
 }
 
-hipError_t launch_copy(hipStream_t s, size_t n, const double* x, double* y) {
-    hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256)), dim3(256), 0, s, n, x, y);
+hipError_t launch_copy(hipStream_t s, size_t n, const double* x, double* y, const int* skip) {
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256)), dim3(256), 0, s, n, x, y, skip);
     return hipGetLastError();
 }
 
