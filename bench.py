@@ -84,8 +84,11 @@ def autotune_summary(dev):
     v, shapes, ms = dev.autotune_report()
     if v < 0:
         return None
-    return {"kernel": "k_ilu_factor_rows", "chosen": {"task_waves": shapes[v][0], "inversion_waves": shapes[v][1]},
-            "factor_ms": {f"{nw}x{ng}": round(t, 3) for (nw, ng), t in zip(shapes, ms)}}
+    def name(nw, ng):   # ng < 0: the task waves split the block's columns in the inversion
+        return f"{nw}x{ng}" if ng > 0 else f"{nw}x{-ng}split"
+    return {"kernel": "k_ilu_factor_rows", "chosen": {"task_waves": shapes[v][0], "inversion_waves": abs(shapes[v][1]),
+                                                      "split_inversion": shapes[v][1] < 0},
+            "factor_ms": {name(nw, ng): round(t, 3) for (nw, ng), t in zip(shapes, ms)}}
 
 
 def kernel_table(dev, reps=5):
@@ -387,6 +390,22 @@ def bench_steady(args):
     print(json.dumps(result), flush=True)
 
 
+def bench_kernels(args):
+    """Each kernel family of the default workload a few times on one FP32-preconditioned context (no Newton
+    steps): the short program the counter passes of rocprofv3 run."""
+    from flowpro_v2_amd import cases, solver
+    case = make_case(args.n_theta, args.n_r)
+    dev = solver.DeviceCase(case, cases.ns_props(order=4, CFL=args.cfl), precond_bits=32)
+    w0 = dev.initial_state()
+    dev.set_state(cases.perturbed_state(w0[0][:: dev.nb], case.n_elem, dev.nb, 0.01, 3))
+    dev.assemble(1e-3, -1.0, 1, True)
+    names = {0: "residual", 1: "jacobian", 2: "spmv", 6: "ilu_factor", 4: "ilu_apply", 5: "jtv"}
+    res = {names[k]: round(dev.time_kernel(k, 3), 4) for k in (0, 1, 2, 6, 4, 5)}
+    log("kernel ms per launch: " + json.dumps(res))
+    dev.close()
+    print(json.dumps({"workload": f"kernels of NS2D DG p=3 cylinder, {case.n_elem} triangles", "kernel_ms": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,11 +430,17 @@ def main():
     ap.add_argument("--plate-nx", type=int, default=3907, help="cells along the sa_plate channel (3907 x 64 x 2 = 500k)")
     ap.add_argument("--plate-ny", type=int, default=64, help="wall-normal cells of the sa_plate channel")
     ap.add_argument("--sa-cfl", type=float, default=2.0, help="CFL of the sa_plate workload (ramped over 20 steps)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="time only the kernel families (residual, Jacobian, SpMV, factorisation, apply, J^T v) on the "
+                         "default mesh at a seeded 1%% perturbation of the freestream, for profilers")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is "
               "element partitioning with a halo layer, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.kernels_only:
+        bench_kernels(args)
+        return
     if args.workload == "batched_ns":
         bench_batched(args)
         return

@@ -496,7 +496,11 @@ __device__ __attribute__((always_inline)) inline void factor_row(
     constexpr int BB = B * B;
     constexpr int NT = 64 * NW;
     constexpr int CW = (B + NW - 1) / NW;       // columns per wave in the tasks
-    constexpr int NG = NGW < NW ? NGW : NW;     // waves of the inversion
+    // NGW > 0: NGW waves repeat the elimination of the whole block, each with its share of the
+    // identity columns; NGW < 0: all NW waves split the block's columns too and exchange the pivot
+    // column of every step through LDS
+    constexpr bool SPLIT = NGW < 0;
+    constexpr int NG = SPLIT ? NW : (NGW < NW ? NGW : NW);   // waves of the inversion
     constexpr int CG = (B + NG - 1) / NG;       // identity columns per inversion wave
     constexpr int PER = (BB + NT - 1) / NT;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, tid = threadIdx.x;
@@ -659,7 +663,68 @@ __device__ __attribute__((always_inline)) inline void factor_row(
         }
     }
     lds_barrier();
-    if (wv < NG) {
+    if constexpr (SPLIT) {
+        // wave wv keeps the columns c = wv + j NG of the block (mo) and the identity columns
+        // [wv CG, wv CG + CG) (ic); at step p the owner of column p writes it to LDS (double
+        // buffered in LL), every wave reads its row's entry, finds the same pivot with the same
+        // key as the redundant form and updates its own columns with the same fused operations,
+        // so the inverse is bit-identical to the other shapes'
+        constexpr int CM = (B + NG - 1) / NG;
+        const int g0 = wv * CG;
+        TF mo[CM], ic[CG];
+        TF rmax = 0;
+#pragma unroll
+        for (int c = 0; c < B; c++) rmax = fmax(rmax, fabs(act ? LA[c * B + r] : (TF)0));
+#pragma unroll
+        for (int j = 0; j < CM; j++) {
+            const int c = wv + j * NG;
+            mo[j] = (act && c < B) ? LA[c * B + r] : (TF)0;
+        }
+#pragma unroll
+        for (int cc = 0; cc < CG; cc++) ic[cc] = (act && g0 + cc == r) ? (TF)1 : (TF)0;
+        const TF rsc = row_scale_reciprocal(rmax);
+        bool used = !act;
+        int dest = 0;
+        bool singular = false;
+        TF myd = 0;
+#pragma unroll
+        for (int p = 0; p < B; p++) {
+            TF* colx = LL + (p & 1) * 64;
+            if (wv == p % NG) colx[lane] = mo[p / NG];
+            lds_barrier();
+            const TF colp = colx[lane];
+            const unsigned mag = __float_as_uint(fabsf((float)(colp * rsc))) & ~63u;
+            const int key = used ? 0 : (int)(mag | (unsigned)(63 - lane));
+            const int kmax = wave_max_key(key);
+            const int pr = 63 - (kmax & 63);
+            singular = singular || (kmax >> 6) == 0;
+            const TF d = pivot_reciprocal(lane_read(colp, pr));
+            const bool isp = r == pr;
+            const TF beta = isp ? (TF)0 : -colp * d;   // 0 on lanes r >= B
+#pragma unroll
+            for (int j = 0; j < CM; j++)
+                if (wv + j * NG > p && wv + j * NG < B) mo[j] = fma(beta, lane_read(mo[j], pr), mo[j]);
+#pragma unroll
+            for (int cc = 0; cc < CG; cc++) ic[cc] = fma(beta, lane_read(ic[cc], pr), ic[cc]);
+            used = used || isp;
+            dest = isp ? p : dest;
+            myd = isp ? d : myd;
+        }
+        if (act) {
+            int bad = singular ? 1 : 0;
+            TF* Dg = F + (size_t)(rs + qd) * BB;
+#pragma unroll
+            for (int cc = 0; cc < CG; cc++) {
+                const int c = g0 + cc;
+                if (c < B) {
+                    const TF v = ic[cc] * myd;
+                    if (!isfinite((double)v)) bad = 1;
+                    Dg[c * B + dest] = v;
+                }
+            }
+            if (bad) atomicOr(status, 1);
+        }
+    } else if (wv < NG) {
         const int g0 = wv * CG;
         TF mr[B], ic[CG];
 #pragma unroll
@@ -1383,14 +1448,15 @@ hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nr
 #define FP_ROWREG_SIZES(X) X(20, 5) X(24, 4) X(30, 4) X(36, 4) X(40, 4) X(64, 5)
 // launch shapes of the row kernel, picked per context by timing them on the case's own matrix:
 // (task waves NW, inversion waves NGW)
-#define FP_ROWREG_VARIANTS(Y, BS, MR) Y(BS, MR, 0, 4, 4) Y(BS, MR, 1, 8, 4) Y(BS, MR, 2, 8, 2) Y(BS, MR, 3, 8, 8) Y(BS, MR, 4, 4, 2)
-constexpr int ILU_ROW_VARIANTS = 5;
+#define FP_ROWREG_VARIANTS(Y, BS, MR) \
+    Y(BS, MR, 0, 4, 4) Y(BS, MR, 1, 8, 4) Y(BS, MR, 2, 8, 2) Y(BS, MR, 3, 8, 8) Y(BS, MR, 4, 4, 2) Y(BS, MR, 5, 4, -4) Y(BS, MR, 6, 8, -8)
+constexpr int ILU_ROW_VARIANTS = 7;
 
 int ilu_factor_rows_variants() { return ILU_ROW_VARIANTS; }
 // the shape used without tuning: 4 task waves for B = 20 (5 columns each), 8 above
 int ilu_factor_rows_default_variant(int B) { return B == 20 ? 0 : 1; }
 void ilu_factor_rows_shape(int variant, int* nw, int* ngw) {
-    static const int T[ILU_ROW_VARIANTS][2] = {{4, 4}, {8, 4}, {8, 2}, {8, 8}, {4, 2}};
+    static const int T[ILU_ROW_VARIANTS][2] = {{4, 4}, {8, 4}, {8, 2}, {8, 8}, {4, 2}, {4, -4}, {8, -8}};
     const int v = variant >= 0 && variant < ILU_ROW_VARIANTS ? variant : 0;
     *nw = T[v][0];
     *ngw = T[v][1];

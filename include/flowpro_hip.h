@@ -224,7 +224,8 @@ int fp_time_kernel(fp_ctx* ctx, int which, int reps, double* avg_ms);
 void* fp_stream(fp_ctx* ctx);
 /* the launch-shape autotuner of the block-ILU0 factor kernel (no reference counterpart: the
  * reference has one CPU loop): on the first factorisation of a context every shape
- * (shapes[v] = {task waves, inversion waves}) factors the case's own matrix, timed with HIP
+ * (shapes[v] = {task waves, inversion waves}; inversion waves < 0: all task waves split the block's
+ * columns in the inversion) factors the case's own matrix, timed with HIP
  * events, and the fastest is kept; the shapes give bit-identical factors.  variant = the chosen
  * shape (-1 before the first factorisation or when the row kernel does not serve this block
  * size), ms[v] = its time per factorisation (0 when FLOWPRO_AUTOTUNE=0).  Fills min(n, count)
