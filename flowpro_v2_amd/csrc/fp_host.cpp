@@ -183,10 +183,12 @@ struct fp_ctx {
     double* gm = nullptr;         // GMRES on the device: H, its working copy, cs, sn, s, the copy of s
     GmresCtl* gctl = nullptr;
     int gmres_chunk = 4;          // Arnoldi steps enqueued between two reads of the control block
+    bool gmres_graph = false;     // replay each chunk from a HIP graph (FLOWPRO_GMRES_GRAPH=1)
     int* istat = nullptr;
     IluIO* io = nullptr;
     bool use_graph = true;
     hipGraphExec_t g_factor = nullptr, g_apply = nullptr;
+    std::vector<hipGraphExec_t> g_arnoldi;   // the GMRES steps [k chunk, (k + 1) chunk) of a cycle, captured once
     Tables tb{};
     Mesh ms{};
     TetTables ttb{};
@@ -761,6 +763,8 @@ static bool truncation_matrix(fp_ctx* c, std::vector<double>& tr) {
 static void free_all(fp_ctx* c) {
     if (c->g_factor) (void)hipGraphExecDestroy(c->g_factor);
     if (c->g_apply) (void)hipGraphExecDestroy(c->g_apply);
+    for (hipGraphExec_t g : c->g_arnoldi)
+        if (g) (void)hipGraphExecDestroy(g);
     if (c->io) (void)hipFree(c->io);
     void* ptrs[] = {c->m_nvd, c->m_nqvd, c->m_nbd, c->m_nqfd, c->m_vol, c->m_face, c->qv, c->qf, c->qn, c->xq, c->qmass, c->qminv, c->qis, c->av, c->avold, c->trunc, c->wdist, c->geo, c->fgeo, c->tab, c->esize, c->isref, c->nbr, c->nbrf, c->blkn, c->blkd, c->rowptr, c->colidx,
                     c->diagpos, c->lvl_f, c->lvl_b, c->ftptr, c->ftask, c->meta_f, c->meta_b, c->meta_tf, c->meta_tb, c->val, c->F, c->iy, c->ixs, c->ypk, c->xpk, c->fflag, c->fctl, c->bjw, c->W, c->Wo, c->Wo2, c->R, c->RHS,
@@ -2086,6 +2090,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     {
         const char* k = std::getenv("FLOWPRO_GMRES_CHUNK");   // Arnoldi steps per read of the control block
         if (k && std::atoi(k) > 0) c->gmres_chunk = std::atoi(k);
+        const char* gg = std::getenv("FLOWPRO_GMRES_GRAPH");
+        c->gmres_graph = gg && gg[0] == '1';
     }
     HIPCHK(c, dalloc(&c->istat, 4));
     HIPCHK(c, hipMalloc((void**)&c->io, sizeof(IluIO)));
@@ -2562,6 +2568,76 @@ static int op_prec(fp_ctx* c, double* out, const double* in) {
     return prec_apply(c, out, in);
 }
 
+// Arnoldi step i of the current cycle (Gmres.java:66-116): w = M^-1 A V_i, modified Gram-Schmidt
+// against V_0..V_i with the dots on the device, V_(i+1) = w / h_(i+1,i), then the Givens update of
+// the control block.  Predicated (operator 0 with a dataflow or non-ILU preconditioner): every
+// launch returns at once when the cycle is over.
+static int arnoldi_step(fp_ctx* c, int i, bool predicated) {
+    const size_t N = (size_t)c->dofs;
+    const int m = c->m;
+    double* hdev = c->scal + 8;   // the Gram-Schmidt dots of the step
+    double* H = c->gm;
+    double* cs = H + 2 * (size_t)(m + 1) * m;
+    double* sn = cs + m;
+    double* sv = sn + m;
+    GmresCtl* ctl = c->gctl;
+    const int* sk = predicated ? &ctl->stop : nullptr;
+    double* Vi = c->V + (size_t)i * N;
+    int rc;
+    if (predicated) {
+        HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, Vi, nullptr, c->aux, 0, sk));
+        if (c->precond == 2) {
+            rc = c->bits == 32 ? ilu_apply_skip_t<float>(c, c->w, c->aux, sk) : ilu_apply_skip_t<double>(c, c->w, c->aux, sk);
+            if (rc) return rc;
+        } else if (c->precond == 1) {
+            HIPCHK(c, launch_block_jacobi(c->s, c->nElem, c->B, c->bjw, c->aux, c->w, sk));
+        } else {
+            HIPCHK(c, launch_copy(c->s, N, c->aux, c->w, sk));
+        }
+    } else {
+        if ((rc = op_apply(c, Vi, c->aux))) return rc;
+        if ((rc = op_prec(c, c->w, c->aux))) return rc;
+    }
+    // MGS: h_k = (w, V_k); w -= h_k V_k  (the next dot is fused into the update)
+    HIPCHK(c, launch_dot(c->s, N, c->w, c->V, c->partial, hdev, 0, sk));
+    for (int k = 0; k <= i; k++) {
+        const double* u = k < i ? c->V + (size_t)(k + 1) * N : nullptr;
+        HIPCHK(c, launch_axpy_dot(c->s, N, c->w, c->V + (size_t)k * N, hdev + k, u, c->partial, hdev + k + 1, k < i ? 0 : 1, sk));
+    }
+    HIPCHK(c, launch_scale_div(c->s, N, c->V + (size_t)(i + 1) * N, c->w, hdev + i + 1, sk));
+    HIPCHK(c, launch_gmres_givens(c->s, ctl, i, m, hdev, H, cs, sn, sv));
+    return FP_OK;
+}
+// the steps [i0, i1) of a cycle; with FLOWPRO_GMRES_GRAPH=1 predicated steps replay a HIP graph captured
+// at their first use (the launches of a chunk depend only on buffers fixed for the context's lifetime).
+// Off by default: on 32 concurrent NS instances of 5k-50k triangles the replays measured no faster than
+// the direct launches (156 against 162 M DOF-updates/s)
+static int arnoldi_chunk(fp_ctx* c, int i0, int i1, bool predicated) {
+    int rc;
+    if (!predicated || !c->gmres_graph) {
+        for (int i = i0; i < i1; i++)
+            if ((rc = arnoldi_step(c, i, predicated))) return rc;
+        return FP_OK;
+    }
+    const size_t slot = (size_t)(i0 / c->gmres_chunk);
+    if (c->g_arnoldi.size() <= slot) c->g_arnoldi.resize(slot + 1, nullptr);
+    hipGraphExec_t* exec = &c->g_arnoldi[slot];
+    if (!*exec) {
+        hipGraph_t g;
+        HIPCHK(c, hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        rc = FP_OK;
+        for (int i = i0; i < i1 && rc == FP_OK; i++) rc = arnoldi_step(c, i, true);
+        hipError_t e = hipStreamEndCapture(c->s, &g);
+        if (rc) return rc;
+        HIPCHK(c, e);
+        e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(c, e);
+    }
+    HIPCHK(c, hipGraphLaunch(*exec, c->s));
+    return FP_OK;
+}
+
 // Gmres.solve (FlowPro/src/main/java/flowpro/core/LinearSolvers/Gmres.java:42-122): left
 // preconditioned, restart m, modified Gram-Schmidt, residual relative to |M^-1 b|.  The scalar
 // bookkeeping (Hessenberg column, Givens rotations, convergence test, triangular solve) runs on the
@@ -2594,15 +2670,11 @@ static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_
     }
     error = nr / bnrm2;
     if (error < c->tol) { *ok = true; *err_out = error; return FP_OK; }
-    double* hdev = c->scal + 8;   // the Gram-Schmidt dots of the current step
     double* H = c->gm;
     double* Hw = H + (size_t)(m + 1) * m;
-    double* cs = Hw + (size_t)(m + 1) * m;
-    double* sn = cs + m;
-    double* sv = sn + m;
+    double* sv = Hw + (size_t)(m + 1) * m + 2 * (size_t)m;
     double* bw = sv + m + 1;
     GmresCtl* ctl = c->gctl;
-    const int* stop = &ctl->stop;
     const bool predicated = c->op == 0 && (c->precond != 2 || c->ilu_flow);
     const int chunk = predicated ? c->gmres_chunk : 1;
     GmresCtl h{0, 0, 0, 0, bnrm2, error, nr, c->tol};
@@ -2610,37 +2682,13 @@ static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_
     for (int it = 0; it < c->cycles; it++) {
         HIPCHK(c, launch_gmres_cycle_begin(c->s, ctl, H, sv, m));
         HIPCHK(c, launch_scale_div(c->s, N, c->V, c->r, &ctl->nr));
-        for (int i = 0; i < m; i++) {
-            if (i > 0 && i % chunk == 0) {   // read the control block: stop enqueueing once the cycle is over
+        for (int i0 = 0; i0 < m; i0 += chunk) {
+            if (i0 > 0) {   // read the control block: stop enqueueing once the cycle is over
                 HIPCHK(c, hipMemcpyAsync(&h, ctl, sizeof(GmresCtl), hipMemcpyDeviceToHost, c->s));
                 HIPCHK(c, hipStreamSynchronize(c->s));
                 if (h.stop) break;
             }
-            double* Vi = c->V + (size_t)i * N;
-            const int* sk = predicated ? stop : nullptr;
-            if (c->op == 0) {
-                HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, Vi, nullptr, c->aux, 0, sk));
-                if (c->precond == 2) {
-                    rc = c->bits == 32 ? ilu_apply_skip_t<float>(c, c->w, c->aux, sk) : ilu_apply_skip_t<double>(c, c->w, c->aux, sk);
-                    if (rc) return rc;
-                } else if (c->precond == 1) {
-                    HIPCHK(c, launch_block_jacobi(c->s, c->nElem, c->B, c->bjw, c->aux, c->w, sk));
-                } else {
-                    HIPCHK(c, launch_copy(c->s, N, c->aux, c->w, sk));
-                }
-            } else {
-                if ((rc = op_apply(c, Vi, c->aux))) return rc;
-                if ((rc = op_prec(c, c->w, c->aux))) return rc;
-            }
-            // MGS: h_k = (w, V_k); w -= h_k V_k  (the next dot is fused into the update)
-            HIPCHK(c, launch_dot(c->s, N, c->w, c->V, c->partial, hdev, 0, sk));
-            for (int k = 0; k <= i; k++) {
-                const double* u = k < i ? c->V + (size_t)(k + 1) * N : nullptr;
-                HIPCHK(c, launch_axpy_dot(c->s, N, c->w, c->V + (size_t)k * N, hdev + k, u, c->partial, hdev + k + 1,
-                                          k < i ? 0 : 1, sk));
-            }
-            HIPCHK(c, launch_scale_div(c->s, N, c->V + (size_t)(i + 1) * N, c->w, hdev + i + 1, sk));
-            HIPCHK(c, launch_gmres_givens(c->s, ctl, i, m, hdev, H, cs, sn, sv));
+            if ((rc = arnoldi_chunk(c, i0, std::min(m, i0 + chunk), predicated))) return rc;
         }
         HIPCHK(c, launch_gmres_lsolve(c->s, ctl, m, H, sv, Hw, bw, c->y));
         HIPCHK(c, launch_update_solution(c->s, N, 0, c->V, N, c->y, x, &ctl->nused));
