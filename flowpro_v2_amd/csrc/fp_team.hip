@@ -54,9 +54,7 @@ hipError_t launch_residual_team(const ElemLaunch& L, hipStream_t s, const ModelP
                                 double a3, double* R, double* RHS, double* enorm) {
 #define X(MK, NB, NEQ, NQV, NQF, V, A)                                                                                \
     if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V, A)) {                                                                      \
-        using LY = TeamLayout<NB, NEQ, NQV, NQF>;                                                                      \
-        constexpr int EX = NQV * volume_slots<MK>() * NEQ + LY::NQFT * NEQ + 2 * LY::B;                                \
-        constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EX>::value;                                                     \
+        constexpr int EPB = residual_team_epb<MK, NB, NEQ, NQV, NQF>();                                              \
         hipLaunchKernelGGL((k_residual_team<MK, NB, NEQ, NQV, NQF, V, A>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), 0, s, P, \
                            G, tb, ms, W, Wo, Wo2, a1, a2, a3, R, RHS, enorm);                                          \
         return hipGetLastError();                                                                                      \

@@ -19,7 +19,11 @@ namespace fp {
 
 constexpr int TEAM_THREADS = 256;   // workgroup size of the team kernels
 
-template <int NB, int NEQ, int NQV, int NQF>
+// PG: the physical gradients of the basis at the element's points are stored per element (the
+// Jacobian team contracts them once per input and point); without them the states' gradients are
+// formed from the reference gradients and the element map, and the residual team fits twice the
+// elements in its LDS.
+template <int NB, int NEQ, int NQV, int NQF, bool PG = true>
 struct TeamLayout {
     static constexpr int B = NB * NEQ;
     static constexpr int NQFT = 3 * NQF;                 // face points of an element
@@ -38,20 +42,40 @@ struct TeamLayout {
     static constexpr int TAB = T_VX + NQV * 2;
     // per element
     static constexpr int E_W = 0;                        // [B]
-    static constexpr int E_GV = E_W + B;                 // [NQV][NB][2] physical gradients (volume)
-    static constexpr int E_GF = E_GV + NQV * NB * 2;     // [3*NQF][NB][2] physical gradients (own face points)
+    static constexpr int E_GV = E_W + B;                 // [NQV][NB][2] physical gradients (volume), PG only
+    static constexpr int E_GF = E_GV + (PG ? NQV * NB * 2 : 0);   // [3*NQF][NB][2] (own face points), PG only
     static constexpr int SV = 3 * NEQ;                   // W, dW/dx, dW/dy at a volume point
     static constexpr int SF = 6 * NEQ;                   // WL, dWL/dx, dWL/dy, WR, dWR/dx, dWR/dy at a face point
-    static constexpr int E_SV = E_GF + NQFT * NB * 2;    // [NQV][SV]
+    static constexpr int E_SV = E_GF + (PG ? NQFT * NB * 2 : 0);  // [NQV][SV]
     static constexpr int E_SF = E_SV + NQV * SV;         // [3*NQF][SF]
     static constexpr int ELEM0 = E_SF + NQFT * SF;
 };
 
+// Task q of a loop over the (point, input) pairs of a team's elements: the volume points of every
+// element come first, then the face points, so that a wave runs the volume or the face code, not
+// both (element by element the two kinds of point would share every wave).  Each task still
+// writes its own LDS slot: the order changes nothing in the results.
+template <int NQV, int NQFT, int NIN>
+__device__ inline void team_task(int q, int nel, int& el, int& qp, int& c) {
+    if (q < nel * NQV * NIN) {
+        el = q / (NQV * NIN);
+        const int r = q % (NQV * NIN);
+        qp = r / NIN;
+        c = r % NIN;
+    } else {
+        const int q2 = q - nel * NQV * NIN;
+        el = q2 / (NQFT * NIN);
+        const int r = q2 % (NQFT * NIN);
+        qp = NQV + r / NIN;
+        c = r % NIN;
+    }
+}
+
 // ------------------------------------------------------------------ phase 0 / 1a helpers
-template <int NB, int NEQ, int NQV, int NQF>
+template <int NB, int NEQ, int NQV, int NQF, bool PG = true>
 __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double* __restrict__ W, int e0, int nel, double* tab,
                                   double* eb, int elemStride) {
-    using LY = TeamLayout<NB, NEQ, NQV, NQF>;
+    using LY = TeamLayout<NB, NEQ, NQV, NQF, PG>;
     constexpr int B = LY::B;
     const int tid = threadIdx.x, nt = blockDim.x;
     // one flat copy of the tables; the trip count is a compile-time constant so that all loads
@@ -81,60 +105,69 @@ __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double
         eb[el * elemStride + LY::E_W + i] = W[(size_t)(e0 + el) * B + i];
     }
     __syncthreads();
-    // physical gradients d_x = iT^T d_xi (Transformation.transformBasis)
-    constexpr int NG = (NQV + LY::NQFT) * NB;
-    for (int q = tid; q < nel * NG; q += nt) {
-        const int el = q / NG, r = q % NG;
-        const double* g = ms.geo + (size_t)(e0 + el) * GEO;
-        const double* dxi = r < NQV * NB ? &tab[LY::T_VD + 2 * r] : &tab[LY::T_FD + 2 * (r - NQV * NB)];
-        double* out = r < NQV * NB ? &eb[el * elemStride + LY::E_GV + 2 * r] : &eb[el * elemStride + LY::E_GF + 2 * (r - NQV * NB)];
-        out[0] = g[0] * dxi[0] + g[2] * dxi[1];
-        out[1] = g[1] * dxi[0] + g[3] * dxi[1];
+    if constexpr (PG) {
+        // physical gradients d_x = iT^T d_xi (Transformation.transformBasis)
+        constexpr int NG = (NQV + LY::NQFT) * NB;
+        for (int q = tid; q < nel * NG; q += nt) {
+            const int el = q / NG, r = q % NG;
+            const double* g = ms.geo + (size_t)(e0 + el) * GEO;
+            const double* dxi = r < NQV * NB ? &tab[LY::T_VD + 2 * r] : &tab[LY::T_FD + 2 * (r - NQV * NB)];
+            double* out = r < NQV * NB ? &eb[el * elemStride + LY::E_GV + 2 * r] : &eb[el * elemStride + LY::E_GF + 2 * (r - NQV * NB)];
+            out[0] = g[0] * dxi[0] + g[2] * dxi[1];
+            out[1] = g[1] * dxi[0] + g[3] * dxi[1];
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // states at the quadrature points (own side from LDS, neighbour side from HBM)
     for (int q = tid; q < nel * LY::NQ; q += nt) {
-        const int el = q / LY::NQ, qp = q % LY::NQ;
+        int el, qp, c0;
+        team_task<NQV, LY::NQFT, 1>(q, nel, el, qp, c0);
         double* E = eb + el * elemStride;
         const double* w = E + LY::E_W;
-        if (qp < NQV) {
-            const double* phi = &tab[LY::T_VB + qp * NB];
-            const double* gr = &E[LY::E_GV + qp * NB * 2];
-            double* st = &E[LY::E_SV + qp * LY::SV];
+        // own-side value and gradients at reference point r (volume points first, then face points)
+        auto own = [&](const double* phi, const double* gr, const double* dref, double* st) {
+            if constexpr (PG) {
 #pragma unroll
-            for (int m = 0; m < NEQ; m++) {
-                double s = 0, sx = 0, sy = 0;
+                for (int m = 0; m < NEQ; m++) {
+                    double s = 0, sx = 0, sy = 0;
 #pragma unroll
-                for (int j = 0; j < NB; j++) {
-                    const double c = w[m * NB + j];
-                    s += c * phi[j];
-                    sx += c * gr[2 * j];
-                    sy += c * gr[2 * j + 1];
+                    for (int j = 0; j < NB; j++) {
+                        const double c = w[m * NB + j];
+                        s += c * phi[j];
+                        sx += c * gr[2 * j];
+                        sy += c * gr[2 * j + 1];
+                    }
+                    st[m] = s;
+                    st[NEQ + m] = sx;
+                    st[2 * NEQ + m] = sy;
                 }
-                st[m] = s;
-                st[NEQ + m] = sx;
-                st[2 * NEQ + m] = sy;
+            } else {
+                // d_x W = iT^T (sum_j W_j d_xi phi_j)
+                const double* g = ms.geo + (size_t)(e0 + el) * GEO;
+                const double g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
+#pragma unroll
+                for (int m = 0; m < NEQ; m++) {
+                    double s = 0, sa = 0, sb = 0;
+#pragma unroll
+                    for (int j = 0; j < NB; j++) {
+                        const double c = w[m * NB + j];
+                        s += c * phi[j];
+                        sa += c * dref[2 * j];
+                        sb += c * dref[2 * j + 1];
+                    }
+                    st[m] = s;
+                    st[NEQ + m] = g0 * sa + g2 * sb;
+                    st[2 * NEQ + m] = g1 * sa + g3 * sb;
+                }
             }
+        };
+        if (qp < NQV) {
+            own(&tab[LY::T_VB + qp * NB], &E[LY::E_GV + qp * NB * 2], &tab[LY::T_VD + qp * NB * 2], &E[LY::E_SV + qp * LY::SV]);
         } else {
             const int f = qp - NQV, k = f / NQF, p = f % NQF;
             const int e = e0 + el;
-            const double* phi = &tab[LY::T_FB + f * NB];
-            const double* gr = &E[LY::E_GF + f * NB * 2];
             double* st = &E[LY::E_SF + f * LY::SF];
-#pragma unroll
-            for (int m = 0; m < NEQ; m++) {
-                double s = 0, sx = 0, sy = 0;
-#pragma unroll
-                for (int j = 0; j < NB; j++) {
-                    const double c = w[m * NB + j];
-                    s += c * phi[j];
-                    sx += c * gr[2 * j];
-                    sy += c * gr[2 * j + 1];
-                }
-                st[m] = s;
-                st[NEQ + m] = sx;
-                st[2 * NEQ + m] = sy;
-            }
+            own(&tab[LY::T_FB + f * NB], &E[LY::E_GF + f * NB * 2], &tab[LY::T_FD + f * NB * 2], st);
             const int TT = ms.nbr[e * 3 + k];
             if (TT > -1) {
                 const int kr = ms.nbrf[e * 3 + k];
@@ -149,14 +182,27 @@ __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double
 #pragma unroll
                 for (int m = 0; m < NEQ; m++) {
                     double s = 0, sx = 0, sy = 0;
+                    if constexpr (PG) {
 #pragma unroll
-                    for (int j = 0; j < NB; j++) {
-                        const double c = wv[m * NB + j];
-                        const double gx = gn[0] * dR[2 * j] + gn[2] * dR[2 * j + 1];
-                        const double gy = gn[1] * dR[2 * j] + gn[3] * dR[2 * j + 1];
-                        s += c * phR[j];
-                        sx += c * gx;
-                        sy += c * gy;
+                        for (int j = 0; j < NB; j++) {
+                            const double c = wv[m * NB + j];
+                            const double gx = gn[0] * dR[2 * j] + gn[2] * dR[2 * j + 1];
+                            const double gy = gn[1] * dR[2 * j] + gn[3] * dR[2 * j + 1];
+                            s += c * phR[j];
+                            sx += c * gx;
+                            sy += c * gy;
+                        }
+                    } else {
+                        double sa = 0, sb = 0;
+#pragma unroll
+                        for (int j = 0; j < NB; j++) {
+                            const double c = wv[m * NB + j];
+                            s += c * phR[j];
+                            sa += c * dR[2 * j];
+                            sb += c * dR[2 * j + 1];
+                        }
+                        sx = gn[0] * sa + gn[2] * sb;
+                        sy = gn[1] * sa + gn[3] * sb;
                     }
                     st[3 * NEQ + m] = s;
                     st[4 * NEQ + m] = sx;
@@ -251,42 +297,63 @@ __device__ inline double face_wall_distance(const Tables& tb, const Mesh& ms, in
     return tb.fint[2 * p] * w[k] + tb.fint[2 * p + 1] * w[k == 2 ? 0 : k + 1];
 }
 
-template <int NB, int NEQ, int NQV, int NQF, int EXTRA>
+template <int NB, int NEQ, int NQV, int NQF, int EXTRA, bool PG = true, int KB = 60>
 struct TeamEPB {
-    using LY = TeamLayout<NB, NEQ, NQV, NQF>;
+    using LY = TeamLayout<NB, NEQ, NQV, NQF, PG>;
     static constexpr int PER = LY::ELEM0 + EXTRA;
-    static constexpr int BUDGET = 60 * 1024 / 8;   // doubles of LDS per workgroup
+    static constexpr int BUDGET = KB * 1024 / 8;   // doubles of LDS per workgroup
     static constexpr int E0 = (BUDGET - LY::TAB) / PER;
     static constexpr int value = E0 < 1 ? 1 : (E0 > 16 ? 16 : E0);
 };
 
 // ------------------------------------------------------------------ residual
-// Launch bounds: the residual team is held to 3 waves per SIMD and the Jacobian team to 2; the
-// compiler then keeps each within 170 / 256 registers (a few spills for the largest cases), and
-// the doubled occupancy hides the LDS and HBM latency better than spill-free single-wave code
-// (measured on MI355X: NS p=3 Jacobian 12.7 -> 10.7 ms, SA p=2 23.2 -> 4.0 ms at 96k elements).
+// LDS per element of the residual team beside the staged states: the integrands (fluxes and source
+// at the volume points, the face integrand at the face points), K and the right-hand side
+template <int MK, int NB, int NEQ, int NQV, int NQF>
+constexpr int residual_team_extra() {
+    return NQV * volume_slots<MK>() * NEQ + 3 * NQF * NEQ + 2 * NB * NEQ;
+}
+// Elements per residual team: one pass of the team's threads over the quadrature points of its
+// elements (phase 1b, the flux evaluations) and at most 76 KiB of LDS, so that two teams share a CU;
+// the registers (up to 255 per lane, no spills) hold the CU to two teams anyway.  Measured on MI355X at
+// NS p=3, 200,000 triangles: 4 elements per team with the physical gradients stored and 3 teams per
+// CU (88 VGPRs spilled) 1.23 ms, 8 elements 0.77 ms, 9 elements 0.71 ms.
+constexpr int RESIDUAL_LDS_KB = 76;
+template <int MK, int NB, int NEQ, int NQV, int NQF>
+constexpr int residual_team_epb() {
+    constexpr int fit = TeamEPB<NB, NEQ, NQV, NQF, residual_team_extra<MK, NB, NEQ, NQV, NQF>(), false, RESIDUAL_LDS_KB>::value;
+    constexpr int pass = TEAM_THREADS / (NQV + 3 * NQF);
+    return fit < pass ? fit : (pass < 1 ? 1 : pass);
+}
+
+// Launch bounds: two teams per CU for the residual (255 registers, no spills) and for the Jacobian;
+// held to 3 waves per SIMD the residual spilled, and the Jacobian's doubled occupancy over spill-free
+// single-wave code hides the LDS and HBM latency better (measured on MI355X: NS p=3 Jacobian 12.7 ->
+// 10.7 ms, SA p=2 23.2 -> 4.0 ms at 96k elements).
 template <int MK, int NB, int NEQ, int NQV, int NQF, bool VISC, bool AV>
-__global__ void __launch_bounds__(256, 3) k_residual_team(ModelParams P, DGParams G, Tables tb, Mesh ms, const double* __restrict__ W,
+__global__ void __launch_bounds__(256, 2) k_residual_team(ModelParams P, DGParams G, Tables tb, Mesh ms, const double* __restrict__ W,
                                                        const double* __restrict__ Wo, const double* __restrict__ Wo2, double a1,
                                                        double a2, double a3, double* __restrict__ R, double* __restrict__ RHS,
                                                        double* __restrict__ enorm) {
-    using LY = TeamLayout<NB, NEQ, NQV, NQF>;
+    using LY = TeamLayout<NB, NEQ, NQV, NQF, false>;
     constexpr int B = LY::B;
     constexpr int NVS = volume_slots<MK>();
-    constexpr int EXTRA = NQV * NVS * NEQ + LY::NQFT * NEQ + 2 * B;   // integrands, K, RHS
-    constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EXTRA>::value;
+    constexpr int EXTRA = residual_team_extra<MK, NB, NEQ, NQV, NQF>();
+    constexpr int EPB = residual_team_epb<MK, NB, NEQ, NQV, NQF>();
     constexpr int STRIDE = LY::ELEM0 + EXTRA;
     __shared__ double tab[LY::TAB];
     __shared__ double eb[EPB * STRIDE];
     const int e0 = blockIdx.x * EPB;
     const int nel = min(EPB, ms.nElem - e0);
     if (nel <= 0) return;
-    team_stage<NB, NEQ, NQV, NQF>(tb, ms, W, e0, nel, tab, eb, STRIDE);
+    team_stage<NB, NEQ, NQV, NQF, false>(tb, ms, W, e0, nel, tab, eb, STRIDE);
     const int tid = threadIdx.x, nt = blockDim.x;
     constexpr int O_IV = LY::ELEM0, O_IF = O_IV + NQV * NVS * NEQ, O_K = O_IF + LY::NQFT * NEQ, O_H = O_K + B;
     // phase 1b: integrands at every quadrature point, weighted by J w
     for (int q = tid; q < nel * LY::NQ; q += nt) {
-        const int el = q / LY::NQ, qp = q % LY::NQ, e = e0 + el;
+        int el, qp, c0;
+        team_task<NQV, LY::NQFT, 1>(q, nel, el, qp, c0);
+        const int e = e0 + el;
         double* E = eb + el * STRIDE;
         const double* ge = ms.geo + (size_t)e * GEO;
         if (qp < NQV) {
@@ -329,9 +396,19 @@ __global__ void __launch_bounds__(256, 3) k_residual_team(ModelParams P, DGParam
         double K = 0.0;
         for (int f = 0; f < LY::NQFT; f++) K += tab[LY::T_FB + f * NB + j] * E[O_IF + f * NEQ + m];
         if constexpr (NB > 1) {
-            for (int qv = 0; qv < NQV; qv++)
-                K += E[O_IV + (qv * NVS) * NEQ + m] * E[LY::E_GV + (qv * NB + j) * 2] +
-                     E[O_IV + (qv * NVS + 1) * NEQ + m] * E[LY::E_GV + (qv * NB + j) * 2 + 1];
+            // sum_q fx d_x phi_j + fy d_y phi_j with d_x phi = iT^T d_xi phi: contracted on the reference
+            // gradients, mapped once
+            double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+            for (int qv = 0; qv < NQV; qv++) {
+                const double fx = E[O_IV + (qv * NVS) * NEQ + m], fy = E[O_IV + (qv * NVS + 1) * NEQ + m];
+                const double dx = tab[LY::T_VD + (qv * NB + j) * 2], dy = tab[LY::T_VD + (qv * NB + j) * 2 + 1];
+                a += fx * dx;
+                b += fx * dy;
+                c += fy * dx;
+                d += fy * dy;
+            }
+            const double* g = ms.geo + (size_t)e * GEO;
+            K += g[0] * a + g[2] * b + g[1] * c + g[3] * d;
         }
         if constexpr (NVS == 3) {
             for (int qv = 0; qv < NQV; qv++) K += E[O_IV + (qv * NVS + 2) * NEQ + m] * tab[LY::T_VB + qv * NB + j];
@@ -404,7 +481,9 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
     const int tid = threadIdx.x, nt = blockDim.x;
     // phase 1b: tangents of the integrands, one (point, input) pair per task
     for (int q = tid; q < nel * LY::NQ * NIN; q += nt) {
-        const int el = q / (LY::NQ * NIN), r = q % (LY::NQ * NIN), qp = r / NIN, c = r % NIN, e = e0 + el;
+        int el, qp, c;
+        team_task<NQV, LY::NQFT, NIN>(q, nel, el, qp, c);
+        const int e = e0 + el;
         const int s = c / NEQ, mp = c % NEQ;
         double* E = eb + el * STRIDE;
         const double* ge = ms.geo + (size_t)e * GEO;
