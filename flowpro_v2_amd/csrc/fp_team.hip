@@ -69,10 +69,7 @@ hipError_t launch_jacobian_team(const ElemLaunch& L, hipStream_t s, const ModelP
                                 const Mesh& ms, const double* W, double a1, double* blocks, float* fblocks) {
 #define X(MK, NB, NEQ, NQV, NQF, V, A)                                                                                \
     if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V, A)) {                                                                      \
-        using LY = TeamLayout<NB, NEQ, NQV, NQF>;                                                                      \
-        constexpr int NIN = (V || A) ? 3 * NEQ : NEQ;                                                                  \
-        constexpr int EX = NQV * NIN * volume_slots<MK>() * NEQ + LY::NQFT * NIN * NEQ;                                \
-        constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EX>::value;                                                     \
+        constexpr int EPB = jacobian_team_epb<MK, NB, NEQ, NQV, NQF, V || A>();                                     \
         hipLaunchKernelGGL((k_jacobian_team<MK, NB, NEQ, NQV, NQF, V, A, false>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), \
                            0, s, P, G, tb, ms, W, a1, blocks, fblocks, nullptr, nullptr);                              \
         return hipGetLastError();                                                                                      \
@@ -88,10 +85,7 @@ hipError_t launch_jtv_team(const ElemLaunch& L, hipStream_t s, const ModelParams
                            const Mesh& ms, const double* W, const double* v, double a1, double* out) {
 #define X(MK, NB, NEQ, NQV, NQF, V, A)                                                                                \
     if (FP_TMATCH(MK, NB, NEQ, NQV, NQF, V, A)) {                                                                      \
-        using LY = TeamLayout<NB, NEQ, NQV, NQF>;                                                                      \
-        constexpr int NIN = (V || A) ? 3 * NEQ : NEQ;                                                                  \
-        constexpr int EX = NQV * NIN * volume_slots<MK>() * NEQ + LY::NQFT * NIN * NEQ;                                \
-        constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EX>::value;                                                     \
+        constexpr int EPB = jacobian_team_epb<MK, NB, NEQ, NQV, NQF, V || A>();                                     \
         hipLaunchKernelGGL((k_jacobian_team<MK, NB, NEQ, NQV, NQF, V, A, true>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), \
                            0, s, P, G, tb, ms, W, a1, nullptr, nullptr, v, out);                                       \
         return hipGetLastError();                                                                                      \

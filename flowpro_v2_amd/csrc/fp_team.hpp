@@ -19,11 +19,9 @@ namespace fp {
 
 constexpr int TEAM_THREADS = 256;   // workgroup size of the team kernels
 
-// PG: the physical gradients of the basis at the element's points are stored per element (the
-// Jacobian team contracts them once per input and point); without them the states' gradients are
-// formed from the reference gradients and the element map, and the residual team fits twice the
-// elements in its LDS.
-template <int NB, int NEQ, int NQV, int NQF, bool PG = true>
+// The physical gradients of the basis are not staged: the teams contract on the reference gradients
+// and apply the element map once per sum (d_x = iT^T d_xi), which leaves the LDS to more elements.
+template <int NB, int NEQ, int NQV, int NQF>
 struct TeamLayout {
     static constexpr int B = NB * NEQ;
     static constexpr int NQFT = 3 * NQF;                 // face points of an element
@@ -42,11 +40,9 @@ struct TeamLayout {
     static constexpr int TAB = T_VX + NQV * 2;
     // per element
     static constexpr int E_W = 0;                        // [B]
-    static constexpr int E_GV = E_W + B;                 // [NQV][NB][2] physical gradients (volume), PG only
-    static constexpr int E_GF = E_GV + (PG ? NQV * NB * 2 : 0);   // [3*NQF][NB][2] (own face points), PG only
     static constexpr int SV = 3 * NEQ;                   // W, dW/dx, dW/dy at a volume point
     static constexpr int SF = 6 * NEQ;                   // WL, dWL/dx, dWL/dy, WR, dWR/dx, dWR/dy at a face point
-    static constexpr int E_SV = E_GF + (PG ? NQFT * NB * 2 : 0);  // [NQV][SV]
+    static constexpr int E_SV = E_W + B;                 // [NQV][SV]
     static constexpr int E_SF = E_SV + NQV * SV;         // [3*NQF][SF]
     static constexpr int ELEM0 = E_SF + NQFT * SF;
 };
@@ -72,10 +68,10 @@ __device__ inline void team_task(int q, int nel, int& el, int& qp, int& c) {
 }
 
 // ------------------------------------------------------------------ phase 0 / 1a helpers
-template <int NB, int NEQ, int NQV, int NQF, bool PG = true>
+template <int NB, int NEQ, int NQV, int NQF>
 __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double* __restrict__ W, int e0, int nel, double* tab,
                                   double* eb, int elemStride) {
-    using LY = TeamLayout<NB, NEQ, NQV, NQF, PG>;
+    using LY = TeamLayout<NB, NEQ, NQV, NQF>;
     constexpr int B = LY::B;
     const int tid = threadIdx.x, nt = blockDim.x;
     // one flat copy of the tables; the trip count is a compile-time constant so that all loads
@@ -105,69 +101,38 @@ __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double
         eb[el * elemStride + LY::E_W + i] = W[(size_t)(e0 + el) * B + i];
     }
     __syncthreads();
-    if constexpr (PG) {
-        // physical gradients d_x = iT^T d_xi (Transformation.transformBasis)
-        constexpr int NG = (NQV + LY::NQFT) * NB;
-        for (int q = tid; q < nel * NG; q += nt) {
-            const int el = q / NG, r = q % NG;
-            const double* g = ms.geo + (size_t)(e0 + el) * GEO;
-            const double* dxi = r < NQV * NB ? &tab[LY::T_VD + 2 * r] : &tab[LY::T_FD + 2 * (r - NQV * NB)];
-            double* out = r < NQV * NB ? &eb[el * elemStride + LY::E_GV + 2 * r] : &eb[el * elemStride + LY::E_GF + 2 * (r - NQV * NB)];
-            out[0] = g[0] * dxi[0] + g[2] * dxi[1];
-            out[1] = g[1] * dxi[0] + g[3] * dxi[1];
-        }
-        __syncthreads();
-    }
     // states at the quadrature points (own side from LDS, neighbour side from HBM)
     for (int q = tid; q < nel * LY::NQ; q += nt) {
         int el, qp, c0;
         team_task<NQV, LY::NQFT, 1>(q, nel, el, qp, c0);
         double* E = eb + el * elemStride;
         const double* w = E + LY::E_W;
-        // own-side value and gradients at reference point r (volume points first, then face points)
-        auto own = [&](const double* phi, const double* gr, const double* dref, double* st) {
-            if constexpr (PG) {
+        // own-side value and gradients at a point: d_x W = iT^T (sum_j W_j d_xi phi_j)
+        auto own = [&](const double* phi, const double* dref, double* st) {
+            const double* g = ms.geo + (size_t)(e0 + el) * GEO;
+            const double g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
 #pragma unroll
-                for (int m = 0; m < NEQ; m++) {
-                    double s = 0, sx = 0, sy = 0;
+            for (int m = 0; m < NEQ; m++) {
+                double s = 0, sa = 0, sb = 0;
 #pragma unroll
-                    for (int j = 0; j < NB; j++) {
-                        const double c = w[m * NB + j];
-                        s += c * phi[j];
-                        sx += c * gr[2 * j];
-                        sy += c * gr[2 * j + 1];
-                    }
-                    st[m] = s;
-                    st[NEQ + m] = sx;
-                    st[2 * NEQ + m] = sy;
+                for (int j = 0; j < NB; j++) {
+                    const double c = w[m * NB + j];
+                    s += c * phi[j];
+                    sa += c * dref[2 * j];
+                    sb += c * dref[2 * j + 1];
                 }
-            } else {
-                // d_x W = iT^T (sum_j W_j d_xi phi_j)
-                const double* g = ms.geo + (size_t)(e0 + el) * GEO;
-                const double g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
-#pragma unroll
-                for (int m = 0; m < NEQ; m++) {
-                    double s = 0, sa = 0, sb = 0;
-#pragma unroll
-                    for (int j = 0; j < NB; j++) {
-                        const double c = w[m * NB + j];
-                        s += c * phi[j];
-                        sa += c * dref[2 * j];
-                        sb += c * dref[2 * j + 1];
-                    }
-                    st[m] = s;
-                    st[NEQ + m] = g0 * sa + g2 * sb;
-                    st[2 * NEQ + m] = g1 * sa + g3 * sb;
-                }
+                st[m] = s;
+                st[NEQ + m] = g0 * sa + g2 * sb;
+                st[2 * NEQ + m] = g1 * sa + g3 * sb;
             }
         };
         if (qp < NQV) {
-            own(&tab[LY::T_VB + qp * NB], &E[LY::E_GV + qp * NB * 2], &tab[LY::T_VD + qp * NB * 2], &E[LY::E_SV + qp * LY::SV]);
+            own(&tab[LY::T_VB + qp * NB], &tab[LY::T_VD + qp * NB * 2], &E[LY::E_SV + qp * LY::SV]);
         } else {
             const int f = qp - NQV, k = f / NQF, p = f % NQF;
             const int e = e0 + el;
             double* st = &E[LY::E_SF + f * LY::SF];
-            own(&tab[LY::T_FB + f * NB], &E[LY::E_GF + f * NB * 2], &tab[LY::T_FD + f * NB * 2], st);
+            own(&tab[LY::T_FB + f * NB], &tab[LY::T_FD + f * NB * 2], st);
             const int TT = ms.nbr[e * 3 + k];
             if (TT > -1) {
                 const int kr = ms.nbrf[e * 3 + k];
@@ -181,29 +146,15 @@ __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double
                 for (int q2 = 0; q2 < B; q2++) wv[q2] = wn[q2];
 #pragma unroll
                 for (int m = 0; m < NEQ; m++) {
-                    double s = 0, sx = 0, sy = 0;
-                    if constexpr (PG) {
+                    double s = 0, sa = 0, sb = 0;
 #pragma unroll
-                        for (int j = 0; j < NB; j++) {
-                            const double c = wv[m * NB + j];
-                            const double gx = gn[0] * dR[2 * j] + gn[2] * dR[2 * j + 1];
-                            const double gy = gn[1] * dR[2 * j] + gn[3] * dR[2 * j + 1];
-                            s += c * phR[j];
-                            sx += c * gx;
-                            sy += c * gy;
-                        }
-                    } else {
-                        double sa = 0, sb = 0;
-#pragma unroll
-                        for (int j = 0; j < NB; j++) {
-                            const double c = wv[m * NB + j];
-                            s += c * phR[j];
-                            sa += c * dR[2 * j];
-                            sb += c * dR[2 * j + 1];
-                        }
-                        sx = gn[0] * sa + gn[2] * sb;
-                        sy = gn[1] * sa + gn[3] * sb;
+                    for (int j = 0; j < NB; j++) {
+                        const double c = wv[m * NB + j];
+                        s += c * phR[j];
+                        sa += c * dR[2 * j];
+                        sb += c * dR[2 * j + 1];
                     }
+                    const double sx = gn[0] * sa + gn[2] * sb, sy = gn[1] * sa + gn[3] * sb;
                     st[3 * NEQ + m] = s;
                     st[4 * NEQ + m] = sx;
                     st[5 * NEQ + m] = sy;
@@ -297,9 +248,11 @@ __device__ inline double face_wall_distance(const Tables& tb, const Mesh& ms, in
     return tb.fint[2 * p] * w[k] + tb.fint[2 * p + 1] * w[k == 2 ? 0 : k + 1];
 }
 
-template <int NB, int NEQ, int NQV, int NQF, int EXTRA, bool PG = true, int KB = 60>
+// elements per team whose staged states and EXTRA doubles each fit in KB KiB of LDS beside the
+// tables (1 to 16)
+template <int NB, int NEQ, int NQV, int NQF, int EXTRA, int KB>
 struct TeamEPB {
-    using LY = TeamLayout<NB, NEQ, NQV, NQF, PG>;
+    using LY = TeamLayout<NB, NEQ, NQV, NQF>;
     static constexpr int PER = LY::ELEM0 + EXTRA;
     static constexpr int BUDGET = KB * 1024 / 8;   // doubles of LDS per workgroup
     static constexpr int E0 = (BUDGET - LY::TAB) / PER;
@@ -316,14 +269,30 @@ constexpr int residual_team_extra() {
 // Elements per residual team: one pass of the team's threads over the quadrature points of its
 // elements (phase 1b, the flux evaluations) and at most 76 KiB of LDS, so that two teams share a CU;
 // the registers (up to 255 per lane, no spills) hold the CU to two teams anyway.  Measured on MI355X at
-// NS p=3, 200,000 triangles: 4 elements per team with the physical gradients stored and 3 teams per
-// CU (88 VGPRs spilled) 1.23 ms, 8 elements 0.77 ms, 9 elements 0.71 ms.
+// NS p=3, 200,000 triangles (tasks in element order): 4 elements per team with the physical gradients
+// staged and 3 teams per CU (88 VGPRs spilled) 1.23 ms, 8 elements 0.77 ms, 9 elements 0.71 ms; with
+// the volume tasks first 0.68 ms.
 constexpr int RESIDUAL_LDS_KB = 76;
 template <int MK, int NB, int NEQ, int NQV, int NQF>
 constexpr int residual_team_epb() {
-    constexpr int fit = TeamEPB<NB, NEQ, NQV, NQF, residual_team_extra<MK, NB, NEQ, NQV, NQF>(), false, RESIDUAL_LDS_KB>::value;
+    constexpr int fit = TeamEPB<NB, NEQ, NQV, NQF, residual_team_extra<MK, NB, NEQ, NQV, NQF>(), RESIDUAL_LDS_KB>::value;
     constexpr int pass = TEAM_THREADS / (NQV + 3 * NQF);
     return fit < pass ? fit : (pass < 1 ? 1 : pass);
+}
+
+// LDS per element of the Jacobian team beside the staged states: the derivative of every integrand
+// at every point with respect to each of the NIN inputs of the point; elements per team within
+// 76 KiB of LDS (two teams per CU, the registers allow no more).  NS p=3 at 200,000 triangles on
+// MI355X: 2 elements per team with the physical gradients staged 7.8 ms (J^T v 6.2 ms), 3 elements
+// without them 7.0 ms (5.6 ms).
+template <int MK, int NB, int NEQ, int NQV, int NQF, bool GRAD>
+constexpr int jacobian_team_extra() {
+    return (NQV * volume_slots<MK>() + 3 * NQF) * (GRAD ? 3 * NEQ : NEQ) * NEQ;
+}
+template <int MK, int NB, int NEQ, int NQV, int NQF, bool GRAD>
+constexpr int jacobian_team_epb() {
+    // the transposed product adds its per-(column, equation) partial sums
+    return TeamEPB<NB, NEQ, NQV, NQF, jacobian_team_extra<MK, NB, NEQ, NQV, NQF, GRAD>() + NB * NEQ * NEQ, 76>::value;
 }
 
 // Launch bounds: two teams per CU for the residual (255 registers, no spills) and for the Jacobian;
@@ -335,7 +304,7 @@ __global__ void __launch_bounds__(256, 2) k_residual_team(ModelParams P, DGParam
                                                        const double* __restrict__ Wo, const double* __restrict__ Wo2, double a1,
                                                        double a2, double a3, double* __restrict__ R, double* __restrict__ RHS,
                                                        double* __restrict__ enorm) {
-    using LY = TeamLayout<NB, NEQ, NQV, NQF, false>;
+    using LY = TeamLayout<NB, NEQ, NQV, NQF>;
     constexpr int B = LY::B;
     constexpr int NVS = volume_slots<MK>();
     constexpr int EXTRA = residual_team_extra<MK, NB, NEQ, NQV, NQF>();
@@ -346,7 +315,7 @@ __global__ void __launch_bounds__(256, 2) k_residual_team(ModelParams P, DGParam
     const int e0 = blockIdx.x * EPB;
     const int nel = min(EPB, ms.nElem - e0);
     if (nel <= 0) return;
-    team_stage<NB, NEQ, NQV, NQF, false>(tb, ms, W, e0, nel, tab, eb, STRIDE);
+    team_stage<NB, NEQ, NQV, NQF>(tb, ms, W, e0, nel, tab, eb, STRIDE);
     const int tid = threadIdx.x, nt = blockDim.x;
     constexpr int O_IV = LY::ELEM0, O_IF = O_IV + NQV * NVS * NEQ, O_K = O_IF + LY::NQFT * NEQ, O_H = O_K + B;
     // phase 1b: integrands at every quadrature point, weighted by J w
@@ -467,8 +436,8 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
     constexpr int B = LY::B;
     constexpr int NIN = (VISC || AV) ? 3 * NEQ : NEQ;   // the damping terms see the gradients too
     constexpr int NVS = volume_slots<MK>();
-    constexpr int EXTRA = NQV * NIN * NVS * NEQ + LY::NQFT * NIN * NEQ;
-    constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EXTRA>::value;
+    constexpr int EXTRA = jacobian_team_extra<MK, NB, NEQ, NQV, NQF, VISC || AV>();
+    constexpr int EPB = jacobian_team_epb<MK, NB, NEQ, NQV, NQF, VISC || AV>();
     constexpr int STRIDE = LY::ELEM0 + EXTRA;
     constexpr int O_DV = LY::ELEM0, O_DF = O_DV + NQV * NIN * NVS * NEQ;
     __shared__ double tab[LY::TAB];
@@ -540,6 +509,9 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
         const int el = q / (B * NEQ), r2 = q % (B * NEQ), i = r2 / NEQ, m = r2 % NEQ;
         const int mp = i / NB, jp = i % NB, e = e0 + el;
         const double* E = eb + el * STRIDE;
+        // the element map: d_x psi = g0 d_xi psi + g2 d_eta psi, d_y psi = g1 d_xi psi + g3 d_eta psi
+        const double* ge = ms.geo + (size_t)e * GEO;
+        const double g0 = ge[0], g1 = ge[1], g2 = ge[2], g3 = ge[3];
         double acc[NB];
         double part = 0.0;   // JTV
 #pragma unroll
@@ -557,7 +529,8 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
                 // t = sum_s d g_m / d u_(s, mp) * psi_s(jp)
                 double t = E[O_DF + (f * NIN + mp) * NEQ + m] * tab[LY::T_FB + f * NB + jp];
                 if constexpr (VISC || AV) {
-                    const double psx = E[LY::E_GF + (f * NB + jp) * 2], psy = E[LY::E_GF + (f * NB + jp) * 2 + 1];
+                    const double rx = tab[LY::T_FD + (f * NB + jp) * 2], ry = tab[LY::T_FD + (f * NB + jp) * 2 + 1];
+                    const double psx = g0 * rx + g2 * ry, psy = g1 * rx + g3 * ry;
                     t += E[O_DF + (f * NIN + NEQ + mp) * NEQ + m] * psx + E[O_DF + (f * NIN + 2 * NEQ + mp) * NEQ + m] * psy;
                 }
 #pragma unroll
@@ -592,17 +565,19 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
                 double tx = d0[m] * psi0, ty = d0[NEQ + m] * psi0, ts = 0.0;
                 if constexpr (NVS == 3) ts = d0[2 * NEQ + m] * psi0;
                 if constexpr (VISC || AV) {
-                    const double psx = E[LY::E_GV + (qv * NB + jp) * 2], psy = E[LY::E_GV + (qv * NB + jp) * 2 + 1];
+                    const double rx = tab[LY::T_VD + (qv * NB + jp) * 2], ry = tab[LY::T_VD + (qv * NB + jp) * 2 + 1];
+                    const double psx = g0 * rx + g2 * ry, psy = g1 * rx + g3 * ry;
                     const double* d1 = &E[O_DV + ((qv * NIN + NEQ + mp) * NVS) * NEQ];
                     const double* d2 = &E[O_DV + ((qv * NIN + 2 * NEQ + mp) * NVS) * NEQ];
                     tx += d1[m] * psx + d2[m] * psy;
                     ty += d1[NEQ + m] * psx + d2[NEQ + m] * psy;
                     if constexpr (NVS == 3) ts += d1[2 * NEQ + m] * psx + d2[2 * NEQ + m] * psy;
                 }
+                // tx d_x psi_j + ty d_y psi_j on the reference gradients
+                const double ta = g0 * tx + g1 * ty, tb2 = g2 * tx + g3 * ty;
 #pragma unroll
                 for (int j = 0; j < NB; j++) {
-                    const double gx = E[LY::E_GV + (qv * NB + j) * 2], gy = E[LY::E_GV + (qv * NB + j) * 2 + 1];
-                    acc[j] += tx * gx + ty * gy;
+                    acc[j] += ta * tab[LY::T_VD + (qv * NB + j) * 2] + tb2 * tab[LY::T_VD + (qv * NB + j) * 2 + 1];
                     if constexpr (NVS == 3) acc[j] += ts * tab[LY::T_VB + qv * NB + j];
                 }
             }

@@ -31,10 +31,7 @@ static hipError_t launch_jacobian_kw(const ElemLaunch& L, hipStream_t s, const M
                                      double* out) {
 #define X(MK, NB, NEQ, NQV, NQF, V, A)                                                                                \
     if (FP_KW_MATCH(MK, NB, NEQ, NQV, NQF, V, A)) {                                                                    \
-        using LY = TeamLayout<NB, NEQ, NQV, NQF>;                                                                      \
-        constexpr int NIN = (V || A) ? 3 * NEQ : NEQ;                                                                  \
-        constexpr int EX = NQV * NIN * volume_slots<MK>() * NEQ + LY::NQFT * NIN * NEQ;                                \
-        constexpr int EPB = TeamEPB<NB, NEQ, NQV, NQF, EX>::value;                                                     \
+        constexpr int EPB = jacobian_team_epb<MK, NB, NEQ, NQV, NQF, V || A>();                                     \
         hipLaunchKernelGGL((k_jacobian_team<MK, NB, NEQ, NQV, NQF, V, A, JTV>), dim3((ms.nElem + EPB - 1) / EPB), dim3(TEAM_THREADS), \
                            0, s, P, G, tb, ms, W, a1, blocks, fblocks, v, out);                                        \
         return hipGetLastError();                                                                                      \
