@@ -86,7 +86,8 @@ def autotune_summary(dev):
         return None
     def name(nw, ng):   # ng < 0: the task waves split the block's columns in the inversion
         return f"{nw}x{ng}" if ng > 0 else f"{nw}x{-ng}split"
-    return {"kernel": "k_ilu_factor_rows", "chosen": {"task_waves": shapes[v][0], "inversion_waves": abs(shapes[v][1]),
+    kernel = "k_ilu_factor_flow" if factor_kernel_name(dev)[1] else "k_ilu_factor_rows"
+    return {"kernel": kernel, "chosen": {"task_waves": shapes[v][0], "inversion_waves": abs(shapes[v][1]),
                                                       "split_inversion": shapes[v][1] < 0},
             "factor_ms": {name(nw, ng): round(t, 3) for (nw, ng), t in zip(shapes, ms)}}
 
