@@ -1720,8 +1720,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         for (int i = 0; i < n; i++) {
             const int o = cs->elem_orders ? cs->elem_orders[i] : cs->order;
             const int t = cs->elem_types ? cs->elem_types[i] : (cs->elem_type == 4 ? 4 : 3);
-            if (o < 1 || o > (t == 4 ? 4 : 5) || (t != 0 && t != 3 && t != 4)) {
-                c->err = "mixed meshes: element orders 1-5 on triangles, 1-4 on squares, element types 3 or 4";
+            if (o < 1 || o > 5 || (t != 0 && t != 3 && t != 4)) {
+                c->err = "mixed meshes: element orders 1-5, element types 3 or 4";
                 return FP_UNSUPPORTED;
             }
             maxo = std::max(maxo, o);
@@ -1761,7 +1761,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
             const bool sq = cs->elem_types ? cs->elem_types[i] == 4 : cs->elem_type == 4;
             nbmax = std::max(nbmax, sq ? o * o : o * (o + 1) / 2);
         }
-        c->NB = nbmax <= 4 ? 4 : nbmax <= 9 ? 9 : 16;
+        c->NB = nbmax <= 4 ? 4 : nbmax <= 9 ? 9 : nbmax <= 16 ? 16 : 25;
         c->NEQ = cs->model == MODEL_ADIABATIC ? 3 : 4;
     } else {
         c->NB = make_basis(cs->order).nb;
@@ -2100,7 +2100,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         const char* g = std::getenv("FLOWPRO_GRAPH");
         c->use_graph = !(g && g[0] == '0');
         const char* f = std::getenv("FLOWPRO_ILU_FLOW");   // "0": one sweep launch per level
-        c->ilu_flow = !(f && f[0] == '0');
+        // the dataflow sweeps keep a block row in one wavefront: blocks above 64 rows sweep per level
+        c->ilu_flow = !(f && f[0] == '0') && c->B <= 64;
         const char* fg = std::getenv("FLOWPRO_FLOW_GROUPS");
         if (fg && std::atoi(fg) > 0) c->flow_groups = std::atoi(fg);
         const char* ff = std::getenv("FLOWPRO_FACTOR_GROUPS");

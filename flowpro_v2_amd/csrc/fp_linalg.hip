@@ -453,6 +453,141 @@ __device__ inline int wave_max_key(int v) {
     return __builtin_amdgcn_readlane(v, 63);
 }
 
+// Blocks of 65 to 128 rows (order 5 on squares, B = 100): two wavefronts per block row, thread r
+// keeps row r of the row's blocks in registers (the arithmetic of k_ilu_factor_level_w: the
+// operand block staged in LDS and read as broadcasts, rows combined by row_dot).  In the
+// Gauss-Jordan inverse each wave reduces the pivot keys of its rows; the two wave maxima and their
+// lowest candidate rows meet in LDS (the lowest row among equal keys wins, as in the one-wave
+// kernel), and the pivot row, scaled, reaches the other rows through LDS: two barriers per pivot.
+template <typename TF, int B>
+__global__ void __launch_bounds__(128) k_ilu_factor_level_2w(const int* __restrict__ rows, int nrows, const int* __restrict__ rowptr,
+                                                             const int* __restrict__ colidx, const int* __restrict__ diagpos,
+                                                             TF* __restrict__ F, int* __restrict__ status) {
+    static_assert(B > 64 && B <= 128, "two wavefronts hold 65 to 128 rows");
+    constexpr int BB = B * B;
+    constexpr int PER = (BB + 127) / 128;
+    __shared__ __attribute__((aligned(16))) TF S[BB];
+    __shared__ TF prow[2 * B];
+    __shared__ float wkey[2];
+    __shared__ int wrow[2];
+    const int r = threadIdx.x, wv = r >> 6;
+    const bool act = r < B;
+    if ((int)blockIdx.x >= nrows) return;
+    const int i = rows[blockIdx.x];
+    const int r1 = rowptr[i + 1], di = diagpos[i];
+    TF a[B], l[B];
+    for (int kk = rowptr[i]; kk < di; kk++) {
+        const int k = colidx[kk];
+        TF* Aik = F + (size_t)kk * BB;
+        const TF* Dk = F + (size_t)diagpos[k] * BB;   // D_k^-1 (row k is factored)
+        if (act) {
+#pragma unroll
+            for (int m = 0; m < B; m++) a[m] = Aik[m * B + r];
+        }
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int q = r + 128 * u;
+            if (q < BB) S[q] = Dk[q];
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int c = 0; c < B; c++) l[c] = row_dot<TF, B>(a, S + c * B);
+#pragma unroll
+            for (int c = 0; c < B; c++) Aik[c * B + r] = l[c];
+        }
+        __syncthreads();
+        for (int jj = kk + 1; jj < r1; jj++) {
+            const int j = colidx[jj];
+            int kj = -1;
+            for (int s2 = rowptr[k]; s2 < rowptr[k + 1]; s2++)
+                if (colidx[s2] == j) kj = s2;
+            if (kj < 0) continue;
+            const TF* U = F + (size_t)kj * BB;
+            TF* C = F + (size_t)jj * BB;
+            if (act) {
+#pragma unroll
+                for (int c = 0; c < B; c++) a[c] = C[c * B + r];
+            }
+#pragma unroll
+            for (int u = 0; u < PER; u++) {
+                const int q = r + 128 * u;
+                if (q < BB) S[q] = U[q];
+            }
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (int c = 0; c < B; c++) a[c] -= row_dot<TF, B>(l, S + c * B);
+#pragma unroll
+                for (int c = 0; c < B; c++) C[c * B + r] = a[c];
+            }
+            __syncthreads();
+        }
+    }
+    // D_i^-1 by Gauss-Jordan with implicit partial pivoting (see k_ilu_factor_level_w)
+    TF* Dg = F + (size_t)di * BB;
+    if (act) {
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+            a[c] = Dg[c * B + r];
+            l[c] = (c == r) ? (TF)1 : (TF)0;
+        }
+    }
+    TF rmax = 0;
+    if (act) {
+#pragma unroll
+        for (int c = 0; c < B; c++) rmax = fmax(rmax, fabs(a[c]));
+    }
+    const TF rsc = row_scale_reciprocal(rmax);
+    bool used = !act;
+    int dest = 0;
+    bool singular = false;
+#pragma unroll
+    for (int p = 0; p < B; p++) {
+        const float key = used ? -1.0f : (float)(fabs(a[p]) * rsc);
+        float kmax = key;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) kmax = fmaxf(kmax, __shfl_xor(kmax, off));
+        const unsigned long long cand = __ballot(!used && key == kmax);
+        if ((r & 63) == 0) {
+            wkey[wv] = kmax;
+            wrow[wv] = cand ? wv * 64 + (int)__ffsll((long long)cand) - 1 : -1;
+        }
+        __syncthreads();
+        const float k0 = wkey[0], k1 = wkey[1];
+        const float kall = fmaxf(k0, k1);
+        const int pr = (k0 == kall && wrow[0] >= 0) ? wrow[0] : (wrow[1] >= 0 ? wrow[1] : 0);
+        if (!(kall > 0.0f)) singular = true;
+        if (r == pr) {
+            const TF d = (TF)1 / a[p];
+#pragma unroll
+            for (int c = 0; c < B; c++) {
+                a[c] = a[c] * d;
+                l[c] = l[c] * d;
+                prow[c] = a[c];
+                prow[B + c] = l[c];
+            }
+            used = true;
+            dest = p;
+        }
+        __syncthreads();
+        if (act && r != pr) {
+            const TF f = a[p];
+#pragma unroll
+            for (int c = 0; c < B; c++) { a[c] -= f * prow[c]; l[c] -= f * prow[B + c]; }
+        }
+    }
+    if (act) {
+        int bad = singular ? 1 : 0;
+#pragma unroll
+        for (int c = 0; c < B; c++) {
+            if (!isfinite((double)l[c])) bad = 1;
+            Dg[c * B + dest] = l[c];
+        }
+        if (bad) atomicOr(status, 1);
+    }
+}
+
 // Reciprocal of a pivot: v_rcp_f32 (1 ulp) for FP32 factors, IEEE division for FP64.
 __device__ inline float pivot_reciprocal(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline double pivot_reciprocal(double x) { return 1.0 / x; }
@@ -827,6 +962,8 @@ __global__ void k_factor_flow_begin(unsigned* ctl) {
 // sequence can be captured once in a graph and replayed for any vectors.
 
 template <int B> struct RowsPerWave { static constexpr int value = B <= 32 ? 64 / B : 1; };
+// threads of a level-sweep workgroup: one wavefront, two above B = 64 (one row per workgroup)
+template <int B> struct SweepThreads { static constexpr int value = B > 64 ? 128 : 64; };
 
 // Per-row records of the sweeps, one per position of the level order (16 ints): row i,
 // number n of off-diagonal blocks on the swept side (lower for the forward sweep, upper for
@@ -836,7 +973,7 @@ constexpr int ILU_MAXN = 6;
 
 // forward: y_i = b_i - sum_{k<i} L_ik y_k
 template <typename TF, int B>
-__global__ void __launch_bounds__(64) k_ilu_fwd_level(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
+__global__ void __launch_bounds__(B > 64 ? 128 : 64) k_ilu_fwd_level(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
                                                       const IluIO* __restrict__ io, TF* __restrict__ y) {
     constexpr int RPW = RowsPerWave<B>::value;
     constexpr size_t BB = (size_t)B * B;
@@ -878,7 +1015,7 @@ __global__ void __launch_bounds__(64) k_ilu_fwd_level(const int* __restrict__ me
 // (32 -> 9 us per level) and 10x at B = 64 in FP32 (68 us against 6 us for the forward level), so the
 // row is loaded after the exchange.
 template <typename TF, int B>
-__global__ void __launch_bounds__(64) k_ilu_bwd_level(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
+__global__ void __launch_bounds__(B > 64 ? 128 : 64) k_ilu_bwd_level(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
                                                       const TF* __restrict__ y, TF* __restrict__ xs,
                                                       const IluIO* __restrict__ io) {
     constexpr int RPW = RowsPerWave<B>::value;
@@ -1424,6 +1561,8 @@ hipError_t launch_ilu_copy(hipStream_t s, size_t n, const double* src, TF* dst) 
 }
 
 #define FP_BLOCK_SIZES(X) X(1) X(3) X(4) X(5) X(6) X(9) X(10) X(12) X(15) X(16) X(18) X(20) X(24) X(27) X(30) X(36) X(40) X(48) X(64)
+// block sizes above 64 (order 5 on squares, 25 functions x 4 equations): per-level factor and sweeps only
+#define FP_BIG_BLOCK_SIZES(X) X(100)
 
 template <typename TF>
 hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nrows, const int* rowptr, const int* colidx,
@@ -1439,6 +1578,14 @@ hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nr
         return hipGetLastError();                                                                                      \
     }
     FP_BLOCK_SIZES(X)
+#undef X
+#define X(BS)                                                                                                          \
+    if (B == BS) {                                                                                                     \
+        hipLaunchKernelGGL((k_ilu_factor_level_2w<TF, BS>), dim3(nrows), dim3(128), 0, s, rows, nrows, rowptr, colidx, \
+                           diagpos, F, status);                                                                        \
+        return hipGetLastError();                                                                                      \
+    }
+    FP_BIG_BLOCK_SIZES(X)
 #undef X
     return hipErrorInvalidValue;
 }
@@ -1518,10 +1665,12 @@ hipError_t launch_ilu_fwd_level(hipStream_t s, int B, const int* meta, int nrows
 #define X(BS)                                                                                                      \
     if (B == BS) {                                                                                                 \
         constexpr int R = RowsPerWave<BS>::value;                                                                  \
-        hipLaunchKernelGGL((k_ilu_fwd_level<TF, BS>), dim3((nrows + R - 1) / R), dim3(64), 0, s, meta, nrows, F, io, y); \
+        hipLaunchKernelGGL((k_ilu_fwd_level<TF, BS>), dim3((nrows + R - 1) / R), dim3(SweepThreads<BS>::value), 0, s, meta, \
+                           nrows, F, io, y);                                                                       \
         return hipGetLastError();                                                                                  \
     }
     FP_BLOCK_SIZES(X)
+    FP_BIG_BLOCK_SIZES(X)
 #undef X
     return hipErrorInvalidValue;
 }
@@ -1532,11 +1681,12 @@ hipError_t launch_ilu_bwd_level(hipStream_t s, int B, const int* meta, int nrows
 #define X(BS)                                                                                                      \
     if (B == BS) {                                                                                                 \
         constexpr int R = RowsPerWave<BS>::value;                                                                  \
-        hipLaunchKernelGGL((k_ilu_bwd_level<TF, BS>), dim3((nrows + R - 1) / R), dim3(64), 0, s, meta, nrows, F, y, xs, \
-                           io);                                                                                    \
+        hipLaunchKernelGGL((k_ilu_bwd_level<TF, BS>), dim3((nrows + R - 1) / R), dim3(SweepThreads<BS>::value), 0, s, meta, \
+                           nrows, F, y, xs, io);                                                                   \
         return hipGetLastError();                                                                                  \
     }
     FP_BLOCK_SIZES(X)
+    FP_BIG_BLOCK_SIZES(X)
 #undef X
     return hipErrorInvalidValue;
 }

@@ -227,16 +227,20 @@ __global__ void __launch_bounds__(128) k_residual_mixed(ModelParams P, DGParams 
         }
 }
 
-// face and volume points per element: 4 faces of up to 5 points (a face next to an order-5
-// triangle), 19 volume points (the triangle rule of order 5, degree 9)
-constexpr int MJ_MAXF = 20, MJ_MAXV = 19;
+// face and volume points per element: 4 faces of up to 5 points (a face next to an element of
+// order 5), 25 volume points (the 5 x 5 Gauss rule of an order-5 square; the triangle rule of
+// order 5 has 19)
+constexpr int MJ_MAXF = 20, MJ_MAXV = 25;
+// threads of the element workgroups: one wavefront, two for the padded blocks of order-5 squares
+// (B = 100 > 64 columns)
+template <int B> constexpr int mixed_team_threads() { return B > 64 ? 128 : 64; }
 
-// The residual, one 64-lane workgroup per element: lane = point evaluates the face integrand
+// The residual, one workgroup per element (64 lanes, 128 above B = 64): lane = point evaluates the face integrand
 // or the volume fluxes there (into LDS), then lane = own DOF (m, j) sums its test-function
 // contributions in the same order as k_residual_mixed (faces, then volume), adds the BDF mass
 // terms, and lanes (which, m) form the equation norms.  Same arithmetic as k_residual_mixed.
 template <int MK, int NB, bool VISC>
-__global__ void __launch_bounds__(64) k_residual_mixed_elem(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
+__global__ void __launch_bounds__(128) k_residual_mixed_elem(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
                                                             const double* __restrict__ Wo, const double* __restrict__ Wo2, double a1,
                                                             double a2, double a3, double* __restrict__ R, double* __restrict__ RHS,
                                                             double* __restrict__ enorm) {
@@ -254,7 +258,7 @@ __global__ void __launch_bounds__(64) k_residual_mixed_elem(ModelParams P, DGPar
     for (int k = 0; k < 4; k++) fstart[k + 1] = fstart[k] + (k < nv ? mm.nqf[e * 4 + k] : 0);
     const int nfp = fstart[4];
     const MPlain own{W, B};
-    for (int pt = lane; pt < nfp + nqv; pt += 64) {
+    for (int pt = lane; pt < nfp + nqv; pt += mixed_team_threads<B>()) {
         if (pt < nfp) {
             int k = 0;
             while (pt >= fstart[k + 1]) k++;
@@ -440,7 +444,7 @@ __global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G
     }
 }
 
-// Exact Jacobian, one 64-lane workgroup per element (assembly, mode 0 of k_jacobian_mixed).
+// Exact Jacobian, one workgroup per element (64 lanes, 128 above B = 64; assembly, mode 0 of k_jacobian_mixed).
 // Phase 1: the derivatives of the face integrand g(WL, dWL; WR, dWR) and of the volume fluxes
 // (fx, fy)(W, dW) at every point with respect to the 3 nEqs own inputs (W_m, dW_m/dx, dW_m/dy;
 // nEqs when inviscid), one dual evaluation per (point, input), spread over the lanes and kept
@@ -450,7 +454,7 @@ __global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G
 // dual pass per (column, point) of k_jacobian_mixed becomes one per (input, point).
 
 template <int MK, int NB, bool VISC>
-__global__ void __launch_bounds__(64) k_jacobian_mixed_elem(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
+__global__ void __launch_bounds__(128) k_jacobian_mixed_elem(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
                                                             double a1, double* __restrict__ blocks, float* __restrict__ fblocks) {
     constexpr int NE = neq_of<MK>(), B = NB * NE;
     constexpr int NIN = VISC ? 3 * NE : NE;
@@ -467,7 +471,7 @@ __global__ void __launch_bounds__(64) k_jacobian_mixed_elem(ModelParams P, DGPar
     const int nfp = fstart[4];
     const MPlain own{W, B};
     // phase 1
-    for (int it = lane; it < (nfp + nqv) * NIN; it += 64) {
+    for (int it = lane; it < (nfp + nqv) * NIN; it += mixed_team_threads<B>()) {
         const int pt = it / NIN, d = it % NIN;
         if (pt < nfp) {
             int k = 0;
@@ -745,11 +749,14 @@ __global__ void k_functional_mixed(MixedMesh mm, const double* __restrict__ W, d
 
 // ------------------------------------------------------------------ dispatch
 // (model, padded basis size, viscous): the padded size is the largest basis of the mesh rounded up
-// to 4, 9 or 16 (element orders up to 5 on triangles, 4 on squares)
+// to 4, 9, 16 or 25 (element orders up to 5 on triangles and squares)
 #define FP_MIXED_CASES(X)                                                                                            \
     X(MODEL_NAVIER_STOKES, 4, false) X(MODEL_NAVIER_STOKES, 9, false) X(MODEL_NAVIER_STOKES, 16, false)              \
+    X(MODEL_NAVIER_STOKES, 25, false)                                                                                \
     X(MODEL_NAVIER_STOKES, 4, true) X(MODEL_NAVIER_STOKES, 9, true) X(MODEL_NAVIER_STOKES, 16, true)                 \
-    X(MODEL_ADIABATIC, 4, false) X(MODEL_ADIABATIC, 9, false) X(MODEL_ADIABATIC, 16, false)
+    X(MODEL_NAVIER_STOKES, 25, true)                                                                                 \
+    X(MODEL_ADIABATIC, 4, false) X(MODEL_ADIABATIC, 9, false) X(MODEL_ADIABATIC, 16, false)                          \
+    X(MODEL_ADIABATIC, 25, false)
 #define FP_MMATCH(MK, NB, V) (L.model == MK && L.nb == NB && (L.visc != 0) == V && !L.av)
 
 bool mixed_supported(const ElemLaunch& L) {
@@ -767,7 +774,8 @@ hipError_t launch_residual_mixed(const ElemLaunch& L, hipStream_t s, const Model
     const bool elem = mm.nqfm * 4 <= MJ_MAXF && mm.nqvm <= MJ_MAXV && mm.nElem > 0;
 #define X(MK, NB, V)                                                                                                      \
     if (FP_MMATCH(MK, NB, V) && elem) {                                                                                   \
-        hipLaunchKernelGGL((k_residual_mixed_elem<MK, NB, V>), dim3(mm.nElem), dim3(64), 0, s, P, G, mm, W, Wo, Wo2, a1, a2, \
+        hipLaunchKernelGGL((k_residual_mixed_elem<MK, NB, V>), dim3(mm.nElem), dim3(mixed_team_threads<NB * neq_of<MK>()>()), 0, s, \
+                           P, G, mm, W, Wo, Wo2, a1, a2, \
                            a3, R, RHS, enorm);                                                                            \
         return hipGetLastError();                                                                                         \
     }                                                                                                                     \
@@ -791,7 +799,8 @@ hipError_t launch_jacobian_mixed(const ElemLaunch& L, hipStream_t s, const Model
 #define X(MK, NB, V)                                                                                                   \
     if (FP_MMATCH(MK, NB, V)) {                                                                                        \
         if (mode == 0)                                                                                                 \
-            hipLaunchKernelGGL((k_jacobian_mixed_elem<MK, NB, V>), dim3(mm.nElem), dim3(64), 0, s, P, G, mm, W, a1,     \
+            hipLaunchKernelGGL((k_jacobian_mixed_elem<MK, NB, V>), dim3(mm.nElem), dim3(mixed_team_threads<NB * neq_of<MK>()>()), \
+                               0, s, P, G, mm, W, a1,                                                                  \
                                blocks, fblocks);                                                                       \
         else                                                                                                           \
             hipLaunchKernelGGL((k_jacobian_mixed<MK, NB, V, double>), grd, blk, 0, s, P, G, mm, W, a1, blocks, fblocks, \
@@ -848,6 +857,7 @@ hipError_t launch_functional_mixed(const ElemLaunch& L, hipStream_t s, const Mix
     if (L.nb == 4) hipLaunchKernelGGL((k_functional_mixed<4>), grd, blk, 0, s, mm, W, kappa, which, ival, grad);
     else if (L.nb == 9) hipLaunchKernelGGL((k_functional_mixed<9>), grd, blk, 0, s, mm, W, kappa, which, ival, grad);
     else if (L.nb == 16) hipLaunchKernelGGL((k_functional_mixed<16>), grd, blk, 0, s, mm, W, kappa, which, ival, grad);
+    else if (L.nb == 25) hipLaunchKernelGGL((k_functional_mixed<25>), grd, blk, 0, s, mm, W, kappa, which, ival, grad);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -410,7 +410,7 @@ __global__ void k_positivity_quad(QuadTables tb, QuadMesh mq, double* __restrict
 }
 
 // ------------------------------------------------------------------ dispatch
-// (model, NB, NQV, NQF, viscous) for element orders 1-4: NB = NQV = k^2, NQF = k
+// (model, NB, NQV, NQF, viscous) for element orders 1-5: NB = NQV = k^2, NQF = k
 // DragCoefficient (which 0) / LiftCoefficient (which 1) contribution of every quadrilateral:
 // the pressure over its wall faces times the normal component, and the exact gradient
 template <int NB, int NQF>
@@ -461,14 +461,17 @@ __global__ void k_functional_quad(QuadTables tb, QuadMesh mq, const double* __re
     X(MODEL_NAVIER_STOKES, 4, 4, 2, false)             \
     X(MODEL_NAVIER_STOKES, 9, 9, 3, false)             \
     X(MODEL_NAVIER_STOKES, 16, 16, 4, false)           \
+    X(MODEL_NAVIER_STOKES, 25, 25, 5, false)           \
     X(MODEL_NAVIER_STOKES, 1, 1, 1, true)              \
     X(MODEL_NAVIER_STOKES, 4, 4, 2, true)              \
     X(MODEL_NAVIER_STOKES, 9, 9, 3, true)              \
     X(MODEL_NAVIER_STOKES, 16, 16, 4, true)            \
+    X(MODEL_NAVIER_STOKES, 25, 25, 5, true)            \
     X(MODEL_ADIABATIC, 1, 1, 1, false)                 \
     X(MODEL_ADIABATIC, 4, 4, 2, false)                 \
     X(MODEL_ADIABATIC, 9, 9, 3, false)                 \
-    X(MODEL_ADIABATIC, 16, 16, 4, false)
+    X(MODEL_ADIABATIC, 16, 16, 4, false)               \
+    X(MODEL_ADIABATIC, 25, 25, 5, false)
 #define FP_QMATCH(MK, NB, NQV, NQF, V) \
     (L.model == MK && L.nb == NB && L.nqv == NQV && L.nqf == NQF && (L.visc != 0) == V && !L.av)
 

@@ -28,7 +28,7 @@ def _shoelace(P):
     return 0.5 * abs((x * np.roll(y, -1) - np.roll(x, -1) * y).sum())
 
 
-@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
 def test_area_size_and_closure(order):
     case = cases.channel_bump(8, 4, quads=True, jitter=0.25)
     o = O.Oracle(case, cases.euler_props(order=order))
@@ -60,7 +60,7 @@ def _lagrange_mass_1d(k):
     return (V * w) @ V.T
 
 
-@pytest.mark.parametrize("order", [2, 3, 4])
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
 def test_mass_matrix_on_a_rectangle(order):
     a, b = 0.7, 0.3
     PXY = np.array([[0, 0], [a, 0], [a, b], [0, b]], dtype=np.float64)
@@ -70,10 +70,13 @@ def test_mass_matrix_on_a_rectangle(order):
     M1 = _lagrange_mass_1d(order)
     ref = a * b * np.kron(M1, M1)       # node r = (b_i, b_j), r = i k + j
     M = o.geometry(0)["M"]
-    assert np.abs(M - ref).max() <= 1e-12 * np.abs(ref).max()
+    # the basis coefficients come from the inverse of the monomial Vandermonde matrix of the nodes
+    # (basis2Dsquare), whose conditioning grows with the order: 1.9e-12 measured at order 5
+    tol = 1e-12 if order <= 4 else 1e-11
+    assert np.abs(M - ref).max() <= tol * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("order", [2, 3, 4])
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
 def test_free_stream_preserved_on_bilinear_quads(order):
     case = cases.channel_bump(8, 4, quads=True)
     case.PXY[:, 1] = np.repeat(np.linspace(0.0, 1.0, 5)[None, :], 9, 0).ravel()   # flat walls
@@ -82,10 +85,12 @@ def test_free_stream_preserved_on_bilinear_quads(order):
     case.PXY[inner] += 0.08 * rng.uniform(-1, 1, size=(inner.sum(), 2))
     o = O.Oracle(case, cases.euler_props(order=order))
     R = o.residual()
-    assert np.abs(R).max() < 1e-12
+    # rounding grows with the conditioning of the monomial basis: 2e-16, 2e-15, 5e-14, 3.6e-12 measured
+    # at orders 2-5
+    assert np.abs(R).max() < (1e-12 if order <= 4 else 1e-10)
 
 
-@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("order", [2, 3, 5])
 def test_conservation_through_neighbour_face_points(order):
     case = cases.square_mesh(5, 1.0, -4, quads=True)
     rng = np.random.default_rng(2)
@@ -97,9 +102,12 @@ def test_conservation_through_neighbour_face_points(order):
     o.set_w(W)
     R = o.residual()
     nb = o.nb
+    # the test functions sum to one only to the rounding of the monomial basis coefficients: 2e-16,
+    # 3e-15, 2e-13, 4e-12 measured at orders 2-5
+    tol = 1e-13 if order <= 3 else 1e-10
     for m in (0, 3):   # slip walls carry no mass and no energy flux
         s = R[:, m * nb:(m + 1) * nb].sum()
-        assert abs(s) < 1e-13 * np.abs(R[:, m * nb:(m + 1) * nb]).sum()
+        assert abs(s) < tol * np.abs(R[:, m * nb:(m + 1) * nb]).sum()
 
 
 def _scalar_props(order):
@@ -118,10 +126,10 @@ def test_convergence_rate_on_quadrilaterals(p):
     assert rate > p + 0.7
 
 
-@pytest.mark.parametrize("viscous", [False, True])
-def test_exact_jacobian_against_central_differences(viscous):
+@pytest.mark.parametrize("viscous,order", [(False, 2), (True, 3), (True, 5)])
+def test_exact_jacobian_against_central_differences(viscous, order):
     case = cases.channel_bump(4, 2, quads=True, jitter=0.2)
-    props = dict(cases.ns_props(order=3, reynolds=200.0) if viscous else cases.euler_props(order=2), jacobian="exact")
+    props = dict(cases.ns_props(order=order, reynolds=200.0) if viscous else cases.euler_props(order=order), jacobian="exact")
     o = O.Oracle(case, props)
     W = cases.perturbed_state(o.get_w()[0][:: o.nb], case.n_elem, o.nb, 0.02, 11)
     o.set_w(W)
