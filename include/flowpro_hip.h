@@ -102,10 +102,10 @@ typedef struct fp_case {
      * the larger order of their two sides, and the rules are read from the reference's gaussRules
      * folder gauss_rules_dir (gauss/<k>.txt, gaussTri/<d>.txt).  Solution vectors then use the
      * reference's layout: element after element, each [m][nb_e].  Models NavierStokes and
-     * AdiabaticEulers, orders 1-5 on triangles and 1-4 on squares, no artificial viscosity; vol_* and
+     * AdiabaticEulers, orders 1-5 on triangles and squares, no artificial viscosity; vol_* and
      * face_* are unused */
     const int* elem_types;        /* [n_elems]: 3 or 4, or NULL (all elem_type) */
-    const int* elem_orders;       /* [n_elems]: 1..5 (1..4 on squares), or NULL (all order) */
+    const int* elem_orders;       /* [n_elems]: 1..5, or NULL (all order) */
     const char* gauss_rules_dir;
     /* Aerodynamics.init reference values used by KOmegaSST.sutherland (KOmegaSST.java:566-570):
      * cv and the dimensional pRef, rhoRef (TRef = pRef / (rhoRef cv (kappa - 1))) */
