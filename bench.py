@@ -242,42 +242,67 @@ def cpu_baseline(threads, n_theta=192, n_r=64, nsteps=3):
 
 
 def bench_batched(args):
-    """Independent instances solved concurrently (flowpro_v2_amd.batch), FP64 then mixed."""
+    """Independent instances solved concurrently (flowpro_v2_amd.batch), FP64 then mixed.  --mixed-types:
+    meshes of squares and triangles; --steady: every instance runs the SER CFL law until its residuum
+    reaches --tol or --max-steps steps; then the lift, its exact gradient and its adjoint per instance
+    (a1 = 0 after steady runs, else a1 = 1/dt of the instance's last step)."""
     from flowpro_v2_amd import batch
     t0 = time.perf_counter()
+    steady = {"maxCFL": args.max_cfl, "residuum": args.tol} if args.steady else None
     inst = batch.seeded_ns_instances(n=args.instances, cfl=args.cfl, per_element_orders=args.per_element_orders,
-                                     orders=(2, 3, 4, 5) if args.per_element_orders else (2, 3, 4))
+                                     orders=(2, 3, 4, 5) if args.per_element_orders else (2, 3, 4),
+                                     mixed_types=args.mixed_types, steady=steady)
     n_elem = sum(c.n_elem for c, _ in inst)
-    log(f"{len(inst)} instances, {n_elem} triangles in total, meshes built in {time.perf_counter() - t0:.1f} s")
+    n_sq = sum(int((c.types == 4).sum()) for c, _ in inst if getattr(c, "types", None) is not None)
+    log(f"{len(inst)} instances, {n_elem} elements ({n_sq} squares) in total, meshes built in {time.perf_counter() - t0:.1f} s")
     out = {}
+    steps = args.max_steps if args.steady else args.steps
+    warm = 0 if args.steady else args.warmup   # a steady run is timed from the freestream
     for bits in ([64, 32] if args.only is None else ([64] if args.only == "fp64" else [32])):
         t0 = time.perf_counter()
         b = batch.BatchedSolver(inst, precond_bits=bits)
         dofs = b.dofs
         log(f"fp{bits}: {len(inst)} contexts ready in {time.perf_counter() - t0:.1f} s, {dofs} DOFs")
-        b.solve(args.warmup)
+        b.solve(warm)
         t0 = time.perf_counter()
-        recs = b.solve(args.steps)
+        recs = b.solve(steps)
         elapsed = time.perf_counter() - t0
+        recs = [rs[warm:] for rs in recs]   # the solver returns the whole history, warm-up steps included
         updates = sum(it.dev.dofs * sum(r.newton for r in rs) for it, rs in zip(b.instances, recs))
         t1 = time.perf_counter()
         lifts = b.functionals(1)
         t_lift = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        a1 = [0.0 if args.steady else 1.0 / rs[-1].dt for rs in recs]
+        adj = b.adjoints(1, a1)
+        t_adj = time.perf_counter() - t1
         iters = [sum(it for r in rs for it in r.gmres_iters) for rs in recs]
         out[bits] = dict(rate=updates / elapsed, elapsed=elapsed, updates=updates, dofs=dofs, iters=iters,
                          final=[rs[-1].residuum for rs in recs], lift_ms=1e3 * t_lift,
-                         lift=[v for v, _ in lifts])
-        log(f"fp{bits}: {args.steps} steps of every instance in {elapsed:.3f} s, {updates / elapsed:.4g} DOF-updates/s; "
-            f"lift and gradient of all instances in {1e3 * t_lift:.1f} ms")
+                         lift=[v for v, _ in lifts], steps=[len(rs) for rs in recs],
+                         adjoint=dict(seconds=round(t_adj, 3), a1="0 (steady)" if args.steady else "1/dt of the last step",
+                                      converged=sum(1 for rc, _, _, _ in adj if rc == 0),
+                                      gmres_iterations=[it for _, _, it, _ in adj],
+                                      max_relative_residual=max(err for _, _, _, err in adj)))
+        log(f"fp{bits}: {elapsed:.3f} s for {sum(len(rs) for rs in recs)} instance steps, {updates / elapsed:.4g} DOF-updates/s; "
+            f"lift and gradient of all instances in {1e3 * t_lift:.1f} ms; lift adjoints in {t_adj:.2f} s "
+            f"({out[bits]['adjoint']['converged']} of {len(adj)} converged)")
+        if args.steady:
+            reached = sum(1 for rs in recs if rs[-1].residuum <= args.tol)
+            log(f"fp{bits}: {reached} of {len(recs)} instances reached residuum {args.tol:g}; final residua "
+                + " ".join(f"{rs[-1].residuum:.1e}" for rs in recs))
+            out[bits]["reached"] = reached
         b.close()
     main_bits = 32 if 32 in out else 64
     r = out[main_bits]
     # the same steps with the instances one after another (one host thread): the gain of sharing the GPU
     b = batch.BatchedSolver(inst, precond_bits=main_bits, workers=1)
-    b.solve(args.warmup)
+    b.solve(warm)
     t0 = time.perf_counter()
-    recs = b.solve(args.steps)
-    serial = sum(it.dev.dofs * sum(x.newton for x in rs) for it, rs in zip(b.instances, recs)) / (time.perf_counter() - t0)
+    recs = b.solve(steps)
+    dt_serial = time.perf_counter() - t0
+    # GpuImplicitSolver.solve returns the whole history: the warm-up steps are not counted
+    serial = sum(it.dev.dofs * sum(x.newton for x in rs[warm:]) for it, rs in zip(b.instances, recs)) / dt_serial
     b.close()
     log(f"fp{main_bits}, instances one after another: {serial:.4g} DOF-updates/s")
     result = {
@@ -286,17 +311,20 @@ def bench_batched(args):
         "unit": "DOF-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * r["elapsed"] / args.steps, 3),
+        "warmup": warm,
+        "ms_per_step": round(1e3 * r["elapsed"] / max(1, max(r["steps"])), 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64 residual/Jacobian/GMRES, f32 preconditioner" if main_bits == 32 else "f64",
         "data": "synthetic (seeded cylinder meshes, freestream states, seeded Re and M per instance)",
-        "config": {"workload": f"{len(inst)} independent NS2D cylinder instances, 5k-50k triangles each, element order "
+        "config": {"workload": f"{len(inst)} independent NS2D cylinder instances, 5k-50k "
+                               f"{'squares and triangles' if args.mixed_types else 'triangles'} each, element order "
                                f"{'2-5 per element' if args.per_element_orders else '2-4 per instance'}, Re U[1e3,1e4], "
-                               f"M U[0.2,0.6], BDF1 CFL {args.cfl} ramp, blockilu0 (RCM) GMRES(30) "
-                               "tol 1e-2, one context and stream per instance",
+                               f"M U[0.2,0.6], BDF1 "
+                               + (f"SER CFL (maxCFL {args.max_cfl:g}) to residuum {args.tol:g} or {steps} steps"
+                                  if args.steady else f"CFL {args.cfl} ramp")
+                               + ", blockilu0 (RCM) GMRES(30) tol 1e-2, one context and stream per instance",
                    "instances": len(inst), "elements": n_elem, "dofs": r["dofs"], "precond_bits": main_bits},
         "fp64_value": round(out[64]["rate"], 1) if 64 in out else None,
         "mixed_over_fp64": round(out[32]["rate"] / out[64]["rate"], 3) if (32 in out and 64 in out) else None,
@@ -304,6 +332,9 @@ def bench_batched(args):
         "final_residual_max_rel_diff": (max(abs(a / b - 1) for a, b in zip(out[32]["final"], out[64]["final"]))
                                         if (32 in out and 64 in out) else None),
         "lift_and_gradient_ms": round(r["lift_ms"], 2),
+        "lift_adjoint": r["adjoint"],
+        "steps_per_instance": r["steps"],
+        "steady_reached": r.get("reached"),
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -425,6 +456,10 @@ def main():
     ap.add_argument("--mx", type=int, default=520, help="channel cells along x of the mixed_ns workload")
     ap.add_argument("--my", type=int, default=256, help="channel cells along y of the mixed_ns workload")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batched_ns workload")
+    ap.add_argument("--mixed-types", action="store_true",
+                    help="batched_ns: meshes of squares and triangles (elementType.txt per instance)")
+    ap.add_argument("--steady", action="store_true",
+                    help="batched_ns: SER CFL law to --tol within --max-steps steps per instance")
     ap.add_argument("--per-element-orders", action="store_true",
                     help="batched_ns: every element of an instance draws its order from 2-5 (order.txt)")
     ap.add_argument("--cube", type=int, default=64, help="cubes per edge of the euler3d_tet workload")

@@ -56,6 +56,20 @@ class BatchedSolver:
         with ThreadPoolExecutor(max_workers=self.workers) as pool:
             return list(pool.map(lambda it: it.dev.functional(which, gradient=True), self.instances))
 
+    def adjoints(self, which: int = 1, a1: list[float] | None = None) -> list[tuple[int, np.ndarray, int, float]]:
+        """Per instance the adjoint of LiftCoefficient (1) or DragCoefficient (0): lambda with
+        (a1 M - dR/dW)^T lambda = dI/dW at the instance's current state (GMRES on J^T v; the transposed
+        block-ILU0 where the dataflow sweeps serve the block size), a1 per instance (default 0, the steady
+        adjoint).  Returns (return code, lambda, GMRES iterations, relative residual) per instance."""
+        a1 = a1 or [0.0] * len(self.instances)
+
+        def one(k):
+            it = self.instances[k]
+            _, g = it.dev.functional(which, gradient=True)
+            return it.dev.adjoint_solve(g, a1[k])
+        with ThreadPoolExecutor(max_workers=self.workers) as pool:
+            return list(pool.map(one, range(len(self.instances))))
+
     def close(self):
         for it in self.instances:
             if it.dev is not None:
@@ -64,14 +78,17 @@ class BatchedSolver:
 
 
 def seeded_ns_instances(n: int = 64, seed: int = 11, min_elems: int = 5000, max_elems: int = 50000,
-                        orders=(2, 3, 4), cfl: float = 10.0,
-                        per_element_orders: bool = False) -> list[tuple[cases.Case, dict]]:
-    """n independent 2D NavierStokes cylinder cases: the number of triangles drawn uniformly from
+                        orders=(2, 3, 4), cfl: float = 10.0, per_element_orders: bool = False,
+                        mixed_types: bool = False, steady: dict | None = None) -> list[tuple[cases.Case, dict]]:
+    """n independent 2D NavierStokes cylinder cases: the number of elements drawn uniformly from
     [min_elems, max_elems], the element order from `orders`, Re from [1e3, 1e4] and M from
     [0.2, 0.6], all from numpy default_rng(seed).  Each mesh is the cylinder O-grid with
     n_theta ~ 1.6 n_r (n_theta a multiple of 8) and its own jitter seed.  per_element_orders:
     every element draws its order from `orders` (an order.txt per instance, seed 200 + i) and the
-    instance's order is the largest."""
+    instance's order is the largest.  mixed_types: every cell of the O-grid is a square element or two
+    triangles at random (an elementType.txt per instance, about 1.5 elements per cell).  steady: the
+    CFL is varied by the residual (CFL <= 0, the SER law up to steady["maxCFL"]) and an instance stops
+    at State.residuum <= steady["residuum"]."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
@@ -79,11 +96,20 @@ def seeded_ns_instances(n: int = 64, seed: int = 11, min_elems: int = 5000, max_
         order = int(rng.choice(orders))
         re = float(rng.uniform(1e3, 1e4))
         mach = float(rng.uniform(0.2, 0.6))
-        n_theta = max(16, 8 * int(round(math.sqrt(target / 2 * 1.6) / 8)))
-        n_r = max(4, int(round(target / (2 * n_theta))))
-        case = cases.cylinder_mesh(n_theta, n_r, seed=100 + i)
+        per_cell = 1.5 if mixed_types else 2.0
+        n_theta = max(16, 8 * int(round(math.sqrt(target / per_cell * 1.6) / 8)))
+        n_r = max(4, int(round(target / (per_cell * n_theta))))
+        if mixed_types:
+            case = cases.mixed_cylinder(n_theta, n_r, seed=100 + i)
+        else:
+            case = cases.cylinder_mesh(n_theta, n_r, seed=100 + i)
         if per_element_orders:
             case.orders = cases.seeded_orders(case.n_elem, orders, 200 + i)
             order = int(case.orders.max())
-        out.append((case, cases.ns_props(mach=mach, reynolds=re, order=order, CFL=cfl)))
+        if steady:
+            props = cases.ns_props(mach=mach, reynolds=re, order=order, CFL=-1.0, maxCFL=steady["maxCFL"],
+                                   residuum=steady["residuum"])
+        else:
+            props = cases.ns_props(mach=mach, reynolds=re, order=order, CFL=cfl)
+        out.append((case, props))
     return out

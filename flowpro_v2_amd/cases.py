@@ -166,9 +166,17 @@ def mixed_channel(nx: int = 12, ny: int = 4, quad_fraction: float = 0.5, jitter:
             rng = np.random.default_rng(seed)
             inner = (PXY[:, 0] > 1e-9) & (PXY[:, 0] < 3 - 1e-9) & (PXY[:, 1] > 1e-9) & (PXY[:, 1] < 1 - 1e-9)
             PXY[inner] += jitter * min(3.0 / nx, 1.0 / ny) * rng.uniform(-0.5, 0.5, size=(inner.sum(), 2))
-    rng = np.random.default_rng(seed + 1)
+    TP, types = _keep_some_quads(base.TP, quad_fraction, seed + 1)
+    return Case(f"mixed_{nx}x{ny}", PXY, TP, base.bnd.copy(), types=types)
+
+
+def _keep_some_quads(TPq: np.ndarray, quad_fraction: float, seed: int) -> tuple[np.ndarray, np.ndarray]:
+    """Each counter-clockwise quadrilateral kept (type 4) with probability quad_fraction (numpy
+    default_rng(seed), one draw per cell in order) or split into two counter-clockwise triangles (type 3)
+    with alternating diagonals; rows of four vertices, -1 in the fourth for a triangle."""
+    rng = np.random.default_rng(seed)
     rows, types = [], []
-    for c, q in enumerate(base.TP):
+    for c, q in enumerate(TPq):
         if rng.uniform() < quad_fraction:
             rows.append(list(q))
             types.append(4)
@@ -178,8 +186,15 @@ def mixed_channel(nx: int = 12, ny: int = 4, quad_fraction: float = 0.5, jitter:
             for t in tris:
                 rows.append(list(t) + [-1])
                 types.append(3)
-    TP = np.array(rows, dtype=np.int32)
-    return Case(f"mixed_{nx}x{ny}", PXY, TP, base.bnd.copy(), types=np.array(types, dtype=np.int32))
+    return np.array(rows, dtype=np.int32), np.array(types, dtype=np.int32)
+
+
+def mixed_cylinder(n_theta: int, n_r: int, quad_fraction: float = 0.5, seed: int = 2) -> Case:
+    """cylinder_mesh's quadrilateral O-grid (jitter seed `seed`) with each cell kept as a square element
+    or split into two triangles at random (seed + 1): an elementType.txt mesh of both types."""
+    base = cylinder_mesh(n_theta, n_r, seed=seed, quads=True)
+    TP, types = _keep_some_quads(base.TP, quad_fraction, seed + 1)
+    return Case(f"mixed_cylinder_{n_theta}x{n_r}", base.PXY, TP, base.bnd.copy(), types=types)
 
 
 def seeded_orders(n_elem: int, choices=(1, 2, 3, 4), seed: int = 3) -> np.ndarray:

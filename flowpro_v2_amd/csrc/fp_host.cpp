@@ -78,6 +78,10 @@ hipError_t launch_ilu_sweep_flow(hipStream_t, int, int, const int*, int, const T
                                  unsigned long long*, int, const int* skip = nullptr);
 template <typename TF>
 hipError_t launch_ilu_bwd_level(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*);
+template <typename TF>
+hipError_t launch_ilu_tfwd_level(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);
+template <typename TF>
+hipError_t launch_ilu_tbwd_level(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*);
 hipError_t launch_set_io(hipStream_t, IluIO*, const double*, double*, const int* skip = nullptr);
 hipError_t launch_block_jacobi(hipStream_t, int, int, double*, const double*, double*, const int* skip = nullptr);
 hipError_t launch_gather_diag(hipStream_t, int, int, const int*, const double*, double*);
@@ -2553,8 +2557,21 @@ static int op_residual(fp_ctx* c, const double* x, const double* b, double* y) {
 template <typename TF> static int ilu_apply_trans_t(fp_ctx* c, double* out, const double* in) {
     const TF* F = (const TF*)c->F;
     HIPCHK(c, launch_set_io(c->s, c->io, in, out));
-    HIPCHK(c, launch_ilu_sweep_flow<TF>(c->s, c->B, 2, c->meta_tf, c->nElem, F, c->io, nullptr, c->ypk, c->flow_groups));
-    HIPCHK(c, launch_ilu_sweep_flow<TF>(c->s, c->B, 3, c->meta_tb, c->nElem, F, c->io, c->ypk, c->xpk, c->flow_groups));
+    if (c->ilu_flow) {
+        HIPCHK(c, launch_ilu_sweep_flow<TF>(c->s, c->B, 2, c->meta_tf, c->nElem, F, c->io, nullptr, c->ypk, c->flow_groups));
+        HIPCHK(c, launch_ilu_sweep_flow<TF>(c->s, c->B, 3, c->meta_tb, c->nElem, F, c->io, c->ypk, c->xpk, c->flow_groups));
+        return FP_OK;
+    }
+    // one launch per level (blocks above 64 rows, or FLOWPRO_ILU_FLOW=0)
+    for (size_t l = 0; l + 1 < c->lvl_f_ptr.size(); l++) {
+        const int b = c->lvl_f_ptr[l], e = c->lvl_f_ptr[l + 1];
+        HIPCHK(c, launch_ilu_tfwd_level<TF>(c->s, c->B, c->meta_tf + 16 * (size_t)b, e - b, F, c->io, (TF*)c->iy));
+    }
+    for (size_t l = 0; l + 1 < c->lvl_b_ptr.size(); l++) {
+        const int b = c->lvl_b_ptr[l], e = c->lvl_b_ptr[l + 1];
+        HIPCHK(c, launch_ilu_tbwd_level<TF>(c->s, c->B, c->meta_tb + 16 * (size_t)b, e - b, F, (const TF*)c->iy, (TF*)c->ixs,
+                                            c->io));
+    }
     return FP_OK;
 }
 
@@ -2770,11 +2787,11 @@ extern "C" int fp_adjoint_solve(fp_ctx* c, double a1, const double* rhs, double*
     int rc = to_device(c, rhs, c->RHS);
     if (rc) return rc;
     // preconditioner: block-ILU0 of the assembled a1 M - dR/dW at the current state, applied
-    // transposed (dataflow sweeps); the assembled matrix and factors of the Newton step are
+    // transposed (dataflow sweeps, or one launch per level); the assembled matrix and factors of the Newton step are
     // overwritten
     c->adj_prec = false;
     c->jac_valid = false;
-    if (c->precond == 2 && c->ilu_flow) {
+    if (c->precond == 2) {
         HIPCHK(c, run_jacobian(c, a1));
         if ((rc = prec_factor(c))) return rc;
         int st = 0;

@@ -1083,6 +1083,80 @@ __global__ void __launch_bounds__(B > 64 ? 128 : 64) k_ilu_bwd_level(const int* 
     io->x[(size_t)i * B + t] = (double)v;
 }
 
+// M^-T by levels (the adjoint's preconditioner where the dataflow sweeps do not serve the block
+// size): with the records of the transposed sweeps (the blocks (k, i) of the lower neighbours' rows in
+// forward level order, the blocks (j, i) of the upper neighbours' rows in backward level order), each
+// block read transposed (row t of the transposed block = column t of the stored one, contiguous), four
+// partial sums per block as in the other sweeps.
+//   forward levels:  z_i = D_i^-T (b_i - sum_{k<i} U_ki^T z_k)   (z in y)
+//   backward levels: x_i = z_i - sum_{j>i} L_ji^T x_j            (x in xs and io->x)
+template <typename TF, int B>
+__device__ inline TF trans_block_product(const TF* blk, const TF* v, int t) {
+    const TF* row = blk + (size_t)t * B;
+    TF a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+    for (int c = 0; c + 3 < B; c += 4) {
+        a0 += row[c] * v[c];
+        a1 += row[c + 1] * v[c + 1];
+        a2 += row[c + 2] * v[c + 2];
+        a3 += row[c + 3] * v[c + 3];
+    }
+#pragma unroll
+    for (int c = B - B % 4; c < B; c++) a0 += row[c] * v[c];
+    return (a0 + a1) + (a2 + a3);
+}
+
+template <typename TF, int B>
+__global__ void __launch_bounds__(B > 64 ? 128 : 64) k_ilu_tfwd_level(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
+                                                                     const IluIO* __restrict__ io, TF* __restrict__ y) {
+    constexpr int RPW = RowsPerWave<B>::value;
+    constexpr size_t BB = (size_t)B * B;
+    __shared__ TF rhs[RPW * B];
+    const int slot = threadIdx.x / B, t = threadIdx.x % B;
+    const int li = RPW == 1 ? (int)blockIdx.x : (int)blockIdx.x * RPW + slot;
+    const bool active = slot < RPW && li < nrows;
+    int i = 0, diag = 0;
+    if (active) {
+        const int4* m = reinterpret_cast<const int4*>(meta) + (size_t)li * 4;
+        const int4 m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+        i = m0.x;
+        diag = m0.z;
+        const int cnt = m0.y;
+        const int blk[ILU_MAXN] = {m1.x, m1.y, m1.z, m1.w, m2.x, m2.y};
+        const int col[ILU_MAXN] = {m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
+        TF s = (TF)io->b[(size_t)i * B + t];
+#pragma unroll
+        for (int n = 0; n < ILU_MAXN; n++)
+            if (n < cnt) s -= trans_block_product<TF, B>(F + (size_t)blk[n] * BB, y + (size_t)col[n] * B, t);
+        rhs[slot * B + t] = s;
+    }
+    __syncthreads();
+    if (!active) return;
+    y[(size_t)i * B + t] = trans_block_product<TF, B>(F + (size_t)diag * BB, rhs + slot * B, t);
+}
+
+template <typename TF, int B>
+__global__ void __launch_bounds__(B > 64 ? 128 : 64) k_ilu_tbwd_level(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
+                                                                     const TF* __restrict__ y, TF* __restrict__ xs,
+                                                                     const IluIO* __restrict__ io) {
+    constexpr int RPW = RowsPerWave<B>::value;
+    constexpr size_t BB = (size_t)B * B;
+    const int slot = threadIdx.x / B, t = threadIdx.x % B;
+    const int li = RPW == 1 ? (int)blockIdx.x : (int)blockIdx.x * RPW + slot;
+    if (slot >= RPW || li >= nrows) return;
+    const int4* m = reinterpret_cast<const int4*>(meta) + (size_t)li * 4;
+    const int4 m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+    const int i = m0.x, cnt = m0.y;
+    const int blk[ILU_MAXN] = {m1.x, m1.y, m1.z, m1.w, m2.x, m2.y};
+    const int col[ILU_MAXN] = {m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
+    TF s = y[(size_t)i * B + t];
+#pragma unroll
+    for (int n = 0; n < ILU_MAXN; n++)
+        if (n < cnt) s -= trans_block_product<TF, B>(F + (size_t)blk[n] * BB, xs + (size_t)col[n] * B, t);
+    xs[(size_t)i * B + t] = s;
+    io->x[(size_t)i * B + t] = (double)s;
+}
+
 __global__ void k_set_io(IluIO* io, const double* b, double* x, const int* skip) {
     if (skip && *skip) return;   // the application is skipped whole: the epoch stays
     io->b = b;
@@ -1692,6 +1766,37 @@ hipError_t launch_ilu_bwd_level(hipStream_t s, int B, const int* meta, int nrows
 }
 
 template <typename TF>
+hipError_t launch_ilu_tfwd_level(hipStream_t s, int B, const int* meta, int nrows, const TF* F, const IluIO* io, TF* y) {
+#define X(BS)                                                                                                      \
+    if (B == BS) {                                                                                                 \
+        constexpr int R = RowsPerWave<BS>::value;                                                                  \
+        hipLaunchKernelGGL((k_ilu_tfwd_level<TF, BS>), dim3((nrows + R - 1) / R), dim3(SweepThreads<BS>::value), 0, s, meta, \
+                           nrows, F, io, y);                                                                       \
+        return hipGetLastError();                                                                                  \
+    }
+    FP_BLOCK_SIZES(X)
+    FP_BIG_BLOCK_SIZES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+template <typename TF>
+hipError_t launch_ilu_tbwd_level(hipStream_t s, int B, const int* meta, int nrows, const TF* F, const TF* y, TF* xs,
+                                 const IluIO* io) {
+#define X(BS)                                                                                                      \
+    if (B == BS) {                                                                                                 \
+        constexpr int R = RowsPerWave<BS>::value;                                                                  \
+        hipLaunchKernelGGL((k_ilu_tbwd_level<TF, BS>), dim3((nrows + R - 1) / R), dim3(SweepThreads<BS>::value), 0, s, meta, \
+                           nrows, F, y, xs, io);                                                                   \
+        return hipGetLastError();                                                                                  \
+    }
+    FP_BLOCK_SIZES(X)
+    FP_BIG_BLOCK_SIZES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+template <typename TF>
 hipError_t launch_ilu_sweep_flow(hipStream_t s, int B, int dir, const int* meta, int nrows, const TF* F, IluIO* io,
                                  const unsigned long long* ypk, unsigned long long* out, int max_groups, const int* skip) {
     constexpr int WPG = 4;
@@ -1733,7 +1838,9 @@ hipError_t launch_set_io(hipStream_t s, IluIO* io, const double* b, double* x, c
     template hipError_t launch_ilu_factor_rows<TF>(hipStream_t, int, int, const int*, int, const int*, const int*, const int*, \
                                                    const int*, TF*, int*);                                                     \
     template hipError_t launch_ilu_fwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);             \
-    template hipError_t launch_ilu_bwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*);
+    template hipError_t launch_ilu_bwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*); \
+    template hipError_t launch_ilu_tfwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);            \
+    template hipError_t launch_ilu_tbwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*);
 FP_ILU_INST(double)
 FP_ILU_INST(float)
 

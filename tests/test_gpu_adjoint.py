@@ -131,9 +131,12 @@ def test_jacobian_free_newton_correction():
     dev.close()
 
 
-def test_adjoint_preconditioner_first_direction():
+@pytest.mark.parametrize("flow", ["1", "0"], ids=["dataflow_sweeps", "level_sweeps"])
+def test_adjoint_preconditioner_first_direction(flow, monkeypatch):
     """GMRES(1), one cycle, on the adjoint system: the correction is parallel to M^-T b, with M
-    the block-ILU0 of a1 M - dR/dW at the current state; the oracle's M^-T (ilu_apply, trans)."""
+    the block-ILU0 of a1 M - dR/dW at the current state; the oracle's M^-T (ilu_apply, trans).  Both
+    the dataflow transposed sweeps and the per-level ones (FLOWPRO_ILU_FLOW=0)."""
+    monkeypatch.setenv("FLOWPRO_ILU_FLOW", flow)
     case = cyl()
     props = cases.ns_props(order=4, reynolds=300.0, gmresRestart=1, gmresCycles=1, iterativeSolverTol=1e-14)
     o, dev, W = _pair(case, props)

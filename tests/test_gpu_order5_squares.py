@@ -95,6 +95,26 @@ def test_order5_squares_first_krylov_direction(bits, tol):
     dev.close()
 
 
+def test_order5_squares_adjoint_preconditioner_first_direction():
+    """The per-level transposed sweeps at B = 100: GMRES(1), one cycle, on the adjoint system returns a
+    multiple of M^-T b (the oracle's transposed block-ILU0 application)."""
+    case = cases.cylinder_mesh(16, 3, quads=True)
+    props = cases.ns_props(order=5, reynolds=300.0, gmresRestart=1, gmresCycles=1, iterativeSolverTol=1e-14)
+    print("[order5] adjoint first direction", flush=True)
+    o, dev, W = _setup(case, props)
+    a1 = 20.0
+    _, g = dev.functional(0)
+    rc, lam, it, err = dev.adjoint_solve(g, a1=a1)
+    assert rc in (0, 1) and it == 1
+    o.assemble(1.0 / a1, -1.0)
+    o.linear_solve()
+    r = o.ilu_apply(o.to_dof(g), trans=True)
+    x = o.to_dof(lam)
+    x, r = x / np.linalg.norm(x), r / np.linalg.norm(r)
+    assert np.abs(x - np.sign(x @ r) * r).max() <= 1e-9
+    dev.close()
+
+
 @pytest.mark.parametrize("prec", ["blockilu0", "blockjacobi"])
 def test_order5_squares_linear_solve(prec):
     case = cases.channel_bump(6, 3, quads=True, jitter=0.2)
