@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <utility>
 
 #include "fp_common.hpp"
 
@@ -453,6 +454,20 @@ __device__ inline int wave_max_key(int v) {
     return __builtin_amdgcn_readlane(v, 63);
 }
 
+// Row r of a product with the operand block in LDS, one statement per output column (expanded from
+// an index sequence rather than an unrolled loop: at B = 100 the unroller gives up on the loop, and the
+// row arrays indexed by a loop counter would live in scratch memory).
+template <typename TF, int B, int... Cs>
+__device__ __attribute__((always_inline)) inline void row_times_block(TF (&out)[B], const TF (&x)[B], const TF* S,
+                                                                      std::integer_sequence<int, Cs...>) {
+    ((out[Cs] = row_dot<TF, B>(x, S + Cs * B)), ...);
+}
+template <typename TF, int B, int... Cs>
+__device__ __attribute__((always_inline)) inline void row_minus_block(TF (&acc)[B], const TF (&x)[B], const TF* S,
+                                                                      std::integer_sequence<int, Cs...>) {
+    ((acc[Cs] -= row_dot<TF, B>(x, S + Cs * B)), ...);
+}
+
 // Blocks of 65 to 128 rows (order 5 on squares, B = 100): two wavefronts per block row, thread r
 // keeps row r of the row's blocks in registers (the arithmetic of k_ilu_factor_level_w: the
 // operand block staged in LDS and read as broadcasts, rows combined by row_dot).  In the
@@ -491,8 +506,7 @@ __global__ void __launch_bounds__(128) k_ilu_factor_level_2w(const int* __restri
         }
         __syncthreads();
         if (act) {
-#pragma unroll
-            for (int c = 0; c < B; c++) l[c] = row_dot<TF, B>(a, S + c * B);
+            row_times_block<TF, B>(l, a, S, std::make_integer_sequence<int, B>{});
 #pragma unroll
             for (int c = 0; c < B; c++) Aik[c * B + r] = l[c];
         }
@@ -516,8 +530,7 @@ __global__ void __launch_bounds__(128) k_ilu_factor_level_2w(const int* __restri
             }
             __syncthreads();
             if (act) {
-#pragma unroll
-                for (int c = 0; c < B; c++) a[c] -= row_dot<TF, B>(l, S + c * B);
+                row_minus_block<TF, B>(a, l, S, std::make_integer_sequence<int, B>{});
 #pragma unroll
                 for (int c = 0; c < B; c++) C[c * B + r] = a[c];
             }
@@ -542,9 +555,15 @@ __global__ void __launch_bounds__(128) k_ilu_factor_level_2w(const int* __restri
     bool used = !act;
     int dest = 0;
     bool singular = false;
-#pragma unroll
+    // the pivot loop stays rolled (fully unrolled at B = 100 it exceeds the unroller's limit, and a
+    // rolled loop indexing a[p] would move the row arrays to scratch): column p of the own row is
+    // selected over the unrolled columns, so the arrays keep static indices and stay in registers
+#pragma unroll 1
     for (int p = 0; p < B; p++) {
-        const float key = used ? -1.0f : (float)(fabs(a[p]) * rsc);
+        TF ap = 0;
+#pragma unroll
+        for (int c = 0; c < B; c++) ap = (c == p) ? a[c] : ap;
+        const float key = used ? -1.0f : (float)(fabs(ap) * rsc);
         float kmax = key;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) kmax = fmaxf(kmax, __shfl_xor(kmax, off));
@@ -559,7 +578,7 @@ __global__ void __launch_bounds__(128) k_ilu_factor_level_2w(const int* __restri
         const int pr = (k0 == kall && wrow[0] >= 0) ? wrow[0] : (wrow[1] >= 0 ? wrow[1] : 0);
         if (!(kall > 0.0f)) singular = true;
         if (r == pr) {
-            const TF d = (TF)1 / a[p];
+            const TF d = (TF)1 / ap;
 #pragma unroll
             for (int c = 0; c < B; c++) {
                 a[c] = a[c] * d;
@@ -572,7 +591,7 @@ __global__ void __launch_bounds__(128) k_ilu_factor_level_2w(const int* __restri
         }
         __syncthreads();
         if (act && r != pr) {
-            const TF f = a[p];
+            const TF f = ap;
 #pragma unroll
             for (int c = 0; c < B; c++) { a[c] -= f * prow[c]; l[c] -= f * prow[B + c]; }
         }
