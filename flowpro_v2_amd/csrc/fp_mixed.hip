@@ -91,6 +91,10 @@ __device__ inline void mixed_interp(Acc c, int el, const double* phi, const doub
     }
 }
 
+// Diagonal entry of a padded unknown (the identity mass times a1): 1 at a1 = 0 (the steady adjoint's
+// -dR/dW), so that the padded rows stay regular; nothing couples to them either way.
+__device__ inline double padded_diagonal(double a1) { return a1 != 0.0 ? a1 : 1.0; }
+
 // DGpure.limiter's c_IP: the penalty on elements of order > 1 of a diffusive model
 __device__ inline double mixed_cip(const DGParams& G, const MixedMesh& mm, int e) { return mm.nbe[e] > 1 ? G.cIP : 0.0; }
 
@@ -388,12 +392,13 @@ __global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G
     if (e >= mm.nElem) return;
     if (dir % NB >= mm.nbe[e]) {
         // a padded unknown: nothing depends on it, so its column is a1 e_dir in the diagonal
-        // block (the identity mass) and zero in the neighbour blocks
+        // block (the identity mass; e_dir at a1 = 0) and zero in the neighbour blocks
+        const double pd = padded_diagonal(a1);
         if (mode == 0) {
             const size_t base = (size_t)mm.blkd[e] * B * B + (size_t)dir * B;
             for (int i = 0; i < B; i++) {
-                blocks[base + i] = (BT)(i == dir ? a1 : 0.0);
-                if (fblocks) fblocks[base + i] = (float)(i == dir ? a1 : 0.0);
+                blocks[base + i] = (BT)(i == dir ? pd : 0.0);
+                if (fblocks) fblocks[base + i] = (float)(i == dir ? pd : 0.0);
             }
             for (int k = 0; k < mm.nv[e]; k++)
                 if (mm.nbr[e * 4 + k] > -1) {
@@ -404,7 +409,7 @@ __global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G
                     }
                 }
         } else {
-            out[(size_t)e * B + dir] = a1 * v[(size_t)e * B + dir];
+            out[(size_t)e * B + dir] = pd * v[(size_t)e * B + dir];
         }
         return;
     }
@@ -530,11 +535,12 @@ __global__ void __launch_bounds__(128) k_jacobian_mixed_elem(ModelParams P, DGPa
     // phase 2
     if (lane >= B) return;
     const int dir = lane, md = dir / NB, jd = dir % NB;
-    if (jd >= nbe) {   // a padded unknown: a1 e_dir in the diagonal block, zero in the neighbour blocks
+    if (jd >= nbe) {   // a padded unknown: a1 e_dir (e_dir at a1 = 0) in the diagonal block, zero in the neighbour blocks
+        const double pd = padded_diagonal(a1);
         const size_t base = (size_t)mm.blkd[e] * B * B + (size_t)dir * B;
         for (int i = 0; i < B; i++) {
-            blocks[base + i] = i == dir ? a1 : 0.0;
-            if (fblocks) fblocks[base + i] = (float)(i == dir ? a1 : 0.0);
+            blocks[base + i] = i == dir ? pd : 0.0;
+            if (fblocks) fblocks[base + i] = (float)(i == dir ? pd : 0.0);
         }
         for (int k = 0; k < nv; k++)
             if (mm.nbr[e * 4 + k] > -1) {
