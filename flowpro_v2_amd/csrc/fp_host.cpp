@@ -80,6 +80,10 @@ template <typename TF>
 hipError_t launch_ilu_bwd_level(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*);
 template <typename TF>
 hipError_t launch_ilu_tfwd_level(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);
+bool ilu_sweep_spmv_supported(int B);
+template <typename TF>
+hipError_t launch_ilu_sweep_flow_spmv(hipStream_t, int, const int*, int, const TF*, IluIO*, unsigned long long*, int, const int*,
+                                      const int*, const int*, const double*);
 template <typename TF>
 hipError_t launch_ilu_tbwd_level(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*);
 hipError_t launch_set_io(hipStream_t, IluIO*, const double*, double*, const int* skip = nullptr);
@@ -176,6 +180,7 @@ struct fp_ctx {
     // dataflow sweeps: published rows {value bits, epoch} of y and x (FP64: two words a value)
     unsigned long long *ypk = nullptr, *xpk = nullptr;
     bool ilu_flow = true;                     // one dataflow launch per sweep (else one launch per level)
+    bool spmv_sweep = false;                  // GMRES: A v folded into the forward sweep of M^-1 A v
     bool factor_flow = false;                 // one dataflow launch for the row-kernel factorisation
     unsigned *fflag = nullptr, *fctl = nullptr;   // per-row completion epochs, {epoch, ticket}
     int factor_groups = 2048;                 // workgroups of the dataflow factorisation
@@ -2106,6 +2111,11 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         const char* f = std::getenv("FLOWPRO_ILU_FLOW");   // "0": one sweep launch per level
         // the dataflow sweeps keep a block row in one wavefront: blocks above 64 rows sweep per level
         c->ilu_flow = !(f && f[0] == '0') && c->B <= 64;
+        // "1": the Arnoldi step's product formed inside the forward sweep (bit-identical; off by default:
+        // at NS p=3, 200k triangles the rows' matrix streams did not stay ahead of the dependency front,
+        // 47.6 ms per step against 45.6 ms with the product launch)
+        const char* sv = std::getenv("FLOWPRO_SPMV_SWEEP");
+        c->spmv_sweep = sv && sv[0] == '1' && c->ilu_flow && ilu_sweep_spmv_supported(c->B);
         const char* fg = std::getenv("FLOWPRO_FLOW_GROUPS");
         if (fg && std::atoi(fg) > 0) c->flow_groups = std::atoi(fg);
         const char* ff = std::getenv("FLOWPRO_FACTOR_GROUPS");
@@ -2467,6 +2477,17 @@ template <typename TF> static int ilu_apply_skip_t(fp_ctx* c, double* out, const
     return FP_OK;
 }
 
+// M^-1 (A v) in two launches: the forward sweep forms each row's (A v)_i itself (bit-identical to the
+// product launch followed by the forward sweep), with every launch returning at once when *skip is set
+template <typename TF> static int ilu_apply_spmv_skip_t(fp_ctx* c, double* out, const double* v, const int* skip) {
+    const TF* F = (const TF*)c->F;
+    HIPCHK(c, launch_set_io(c->s, c->io, v, out, skip));
+    HIPCHK(c, launch_ilu_sweep_flow_spmv<TF>(c->s, c->B, c->meta_f, c->nElem, F, c->io, c->ypk, c->flow_groups, skip, c->rowptr,
+                                             c->colidx, c->val));
+    HIPCHK(c, launch_ilu_sweep_flow<TF>(c->s, c->B, 1, c->meta_b, c->nElem, F, c->io, c->ypk, c->xpk, c->flow_groups, skip));
+    return FP_OK;
+}
+
 // a wait of a dataflow sweep that exceeded its spin limit (a broken dependency order) voids the
 // applications of the solve
 static int check_sweep_waits(fp_ctx* c) {
@@ -2602,7 +2623,10 @@ static int arnoldi_step(fp_ctx* c, int i, bool predicated) {
     const int* sk = predicated ? &ctl->stop : nullptr;
     double* Vi = c->V + (size_t)i * N;
     int rc;
-    if (predicated) {
+    if (predicated && c->precond == 2 && c->spmv_sweep) {
+        rc = c->bits == 32 ? ilu_apply_spmv_skip_t<float>(c, c->w, Vi, sk) : ilu_apply_spmv_skip_t<double>(c, c->w, Vi, sk);
+        if (rc) return rc;
+    } else if (predicated) {
         HIPCHK(c, launch_spmv(c->s, c->nElem, c->B, c->rowptr, c->colidx, c->val, Vi, nullptr, c->aux, 0, sk));
         if (c->precond == 2) {
             rc = c->bits == 32 ? ilu_apply_skip_t<float>(c, c->w, c->aux, sk) : ilu_apply_skip_t<double>(c, c->w, c->aux, sk);

@@ -38,16 +38,18 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int nrows, int B, const int* _
         const double* a = val + (size_t)blk * BB + t;
         const double* xc = x + (size_t)colidx[blk] * B;
         if (mode) {
-            for (int c = 0; c < B; c++) s -= a[(size_t)c * B] * xc[c];
+            for (int c = 0; c < B; c++) s = fma(-a[(size_t)c * B], xc[c], s);
         } else {
-            for (int c = 0; c < B; c++) s += a[(size_t)c * B] * xc[c];
+            for (int c = 0; c < B; c++) s = fma(a[(size_t)c * B], xc[c], s);
         }
     }
     y[g] = s;
 }
 
 // the same product with the block size fixed at compile time: the column loop unrolls and its
-// loads issue ahead of the (unchanged, sequential) accumulation
+// loads issue ahead of the (unchanged, sequential) accumulation.  The multiply-adds are explicit
+// fused operations: left to contraction, the products shared by the two modes were rounded apart
+// from their sums, which the product folded into the forward sweep could not reproduce.
 template <int B>
 __global__ void __launch_bounds__(256) k_bsr_spmv_b(int nrows, const int* __restrict__ rowptr, const int* __restrict__ colidx,
                                                     const double* __restrict__ val, const double* __restrict__ x,
@@ -67,10 +69,10 @@ __global__ void __launch_bounds__(256) k_bsr_spmv_b(int nrows, const int* __rest
         for (int c = 0; c < B; c++) av[c] = a[(size_t)c * B];
         if (mode) {
 #pragma unroll
-            for (int c = 0; c < B; c++) s -= av[c] * xc[c];
+            for (int c = 0; c < B; c++) s = fma(-av[c], xc[c], s);
         } else {
 #pragma unroll
-            for (int c = 0; c < B; c++) s += av[c] * xc[c];
+            for (int c = 0; c < B; c++) s = fma(av[c], xc[c], s);
         }
     }
     y[g] = s;
@@ -1313,19 +1315,29 @@ __device__ inline void flow_load(TF (&Lr)[B], const TF* blk, int t) {
 //   DIR 1, M^-1 backward: x_i = D_i^-1 (y_i - sum_{j>i} U_ij x_j)
 //   DIR 2, M^-T forward (U^T z = b):  z_i = D_i^-T (b_i - sum_{k<i} U_ki^T z_k)
 //   DIR 3, M^-T backward (L^T x = z): x_i = z_i - sum_{j>i} L_ji^T x_j
+//   DIR 4, M^-1 A forward: DIR 0 with b_i = (A x)_i, x = io->b, the row of the FP64 BSR product
+//          formed by the row's wave before its first wait (the operations of k_bsr_spmv_b, so the
+//          result is the one of the product launch followed by DIR 0); rows far ahead of the
+//          dependency front stream their matrix rows while the front advances
 // The second sweep of each pair reads the first's published values (complete when it starts)
 // and writes io->x.  The records of DIR 2 / 3 list the blocks (k, i) / (j, i) of the
 // neighbours' rows, read transposed.
 template <typename TF, int B, int WPG, int DIR>
 __global__ void __launch_bounds__(64 * WPG) k_ilu_sweep_flow(const int* __restrict__ meta, int nrows, const TF* __restrict__ F,
                                                              IluIO* __restrict__ io, const unsigned long long* __restrict__ ypk,
-                                                             unsigned long long* __restrict__ out, const int* __restrict__ skip) {
+                                                             unsigned long long* __restrict__ out, const int* __restrict__ skip,
+                                                             const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                                             const double* __restrict__ val) {
     if (skip && *skip) return;
-    constexpr int NPF = FlowPrefetch<TF, B>::value;
+    constexpr bool SPMV = DIR == 4;                 // b = A x
+    // with the product's block columns in registers too, one block fewer prefetched (at most 3, a
+    // triangle's lower neighbours): two waves per SIMD stay resident at B = 40
+    constexpr int NPF0 = FlowPrefetch<TF, B>::value;
+    constexpr int NPF = !SPMV ? NPF0 : (NPF0 - 1 < 1 ? 1 : (NPF0 - 1 > 3 ? 3 : NPF0 - 1));
     constexpr size_t BB = (size_t)B * B;
-    constexpr bool FIRST = DIR == 0 || DIR == 2;    // reads b (else the first sweep's values)
+    constexpr bool FIRST = DIR == 0 || DIR == 2 || SPMV;   // reads b (else the first sweep's values)
     constexpr bool DIAG = DIR == 1 || DIR == 2;     // ends with the diagonal inverse
-    constexpr bool TRANS = DIR >= 2;
+    constexpr bool TRANS = DIR == 2 || DIR == 3;
     __shared__ int base_sh[2];
     const int t = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const unsigned epoch = io->epoch;
@@ -1345,14 +1357,30 @@ __global__ void __launch_bounds__(64 * WPG) k_ilu_sweep_flow(const int* __restri
             const int i = m0.x, cnt = m0.y;
             const int blk[ILU_MAXN] = {m1.x, m1.y, m1.z, m1.w, m2.x, m2.y};
             const int col[ILU_MAXN] = {m2.z, m2.w, m3.x, m3.y, m3.z, m3.w};
+            const int tc = t < B ? t : B - 1;
+            double ax = 0.0;
+            if constexpr (SPMV) {
+                const int q1 = rowptr[i + 1];
+                for (int q = rowptr[i]; q < q1; q++) {
+                    const double* a = val + (size_t)q * BB + tc;
+                    const double* xc = b + (size_t)colidx[q] * B;
+                    double av[B];
+#pragma unroll
+                    for (int c = 0; c < B; c++) av[c] = a[(size_t)c * B];
+#pragma unroll
+                    for (int c = 0; c < B; c++) ax = fma(av[c], xc[c], ax);
+                }
+                // keep the block prefetch below from being hoisted into the product: both sets of
+                // registers at once would halve the resident waves
+                __asm__ volatile("" ::: "memory");
+            }
             TF Lr[NPF][B];
 #pragma unroll
             for (int n = 0; n < NPF; n++)
                 if (n < cnt) flow_load<TF, B, TRANS>(Lr[n], F + (size_t)blk[n] * BB, t);
             TF Dr[DIAG ? B : 1];
             if constexpr (DIAG) flow_load<TF, B, TRANS>(Dr, F + (size_t)m0.z * BB, t);
-            const int tc = t < B ? t : B - 1;
-            TF s = FIRST ? (TF)b[(size_t)i * B + tc] : flow_value(ypk, (size_t)i * B + tc, (TF)0);
+            TF s = SPMV ? (TF)ax : FIRST ? (TF)b[(size_t)i * B + tc] : flow_value(ypk, (size_t)i * B + tc, (TF)0);
 #pragma unroll
             for (int n = 0; n < ILU_MAXN; n++) {
                 if (n < cnt) {
@@ -1824,19 +1852,45 @@ hipError_t launch_ilu_sweep_flow(hipStream_t s, int B, int dir, const int* meta,
     if (B == BS) {                                                                                                   \
         if (dir == 0)                                                                                                \
             hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 0>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
-                               io, ypk, out, skip);                                                                  \
+                               io, ypk, out, skip, nullptr, nullptr, nullptr);                                                                  \
         else if (dir == 1)                                                                                           \
             hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 1>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
-                               io, ypk, out, skip);                                                                  \
+                               io, ypk, out, skip, nullptr, nullptr, nullptr);                                                                  \
         else if (dir == 2)                                                                                           \
             hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 2>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
-                               io, ypk, out, skip);                                                                  \
+                               io, ypk, out, skip, nullptr, nullptr, nullptr);                                                                  \
         else                                                                                                         \
             hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 3>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, \
-                               io, ypk, out, skip);                                                                  \
+                               io, ypk, out, skip, nullptr, nullptr, nullptr);                                                                  \
         return hipGetLastError();                                                                                    \
     }
     FP_BLOCK_SIZES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+// the forward sweep of M^-1 (A x) with the product folded in (DIR 4), for the block sizes of the
+// assembled Newton operators whose dataflow sweeps are in use
+#define FP_SPMV_SWEEP_SIZES(X) X(12) X(18) X(20) X(24) X(30) X(36) X(40) X(64)
+bool ilu_sweep_spmv_supported(int B) {
+#define X(BS) if (B == BS) return true;
+    FP_SPMV_SWEEP_SIZES(X)
+#undef X
+    return false;
+}
+template <typename TF>
+hipError_t launch_ilu_sweep_flow_spmv(hipStream_t s, int B, const int* meta, int nrows, const TF* F, IluIO* io,
+                                      unsigned long long* out, int max_groups, const int* skip, const int* rowptr,
+                                      const int* colidx, const double* val) {
+    constexpr int WPG = 4;
+    const int groups = std::max(1, std::min((nrows + WPG - 1) / WPG, max_groups));
+#define X(BS)                                                                                                        \
+    if (B == BS) {                                                                                                   \
+        hipLaunchKernelGGL((k_ilu_sweep_flow<TF, BS, WPG, 4>), dim3(groups), dim3(64 * WPG), 0, s, meta, nrows, F, io, \
+                           nullptr, out, skip, rowptr, colidx, val);                                                 \
+        return hipGetLastError();                                                                                    \
+    }
+    FP_SPMV_SWEEP_SIZES(X)
 #undef X
     return hipErrorInvalidValue;
 }
@@ -1859,7 +1913,9 @@ hipError_t launch_set_io(hipStream_t s, IluIO* io, const double* b, double* x, c
     template hipError_t launch_ilu_fwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);             \
     template hipError_t launch_ilu_bwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*); \
     template hipError_t launch_ilu_tfwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const IluIO*, TF*);            \
-    template hipError_t launch_ilu_tbwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*);
+    template hipError_t launch_ilu_tbwd_level<TF>(hipStream_t, int, const int*, int, const TF*, const TF*, TF*, const IluIO*); \
+    template hipError_t launch_ilu_sweep_flow_spmv<TF>(hipStream_t, int, const int*, int, const TF*, IluIO*, unsigned long long*, \
+                                                       int, const int*, const int*, const int*, const double*);
 FP_ILU_INST(double)
 FP_ILU_INST(float)
 

@@ -54,3 +54,31 @@ def test_graph_chunks_fp32_factors(monkeypatch):
     got = _run(monkeypatch, True, 4, props, 5, bits=32)
     assert np.array_equal(got[0], ref[0]) and got[1] == ref[1]
     assert np.array_equal(got[2], ref[2])
+
+
+def _run_env(monkeypatch, env, props, steps, bits=64, case=None):
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    case = case or cases.cylinder_mesh(16, 6)
+    dev = solver.DeviceCase(case, props, precond_bits=bits)
+    dev.set_state(dev.initial_state())
+    recs = solver.GpuImplicitSolver(dev, props).solve(steps)
+    out = (np.array([r.residuum for r in recs]), [r.gmres_iters for r in recs], dev.get_state(), dev.solution())
+    dev.close()
+    return out
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+@pytest.mark.parametrize("restart", [30, 5])
+def test_spmv_folded_into_forward_sweep_bit_identical(monkeypatch, bits, restart):
+    """The Arnoldi step's A v formed row by row inside the forward dataflow sweep of M^-1
+    (FLOWPRO_SPMV_SWEEP=1) against the product launch followed by the sweep (the default): the same fused
+    multiply-adds in the same order, so every residual, iteration count, correction and state is
+    bit-identical, for FP64 and FP32 factors."""
+    props = cases.ns_props(order=3, reynolds=100.0, CFL=5.0, iterativeSolverTol=1e-6, timeOrder=2, gmresRestart=restart)
+    print(f"[gmres] folded product FP{bits} restart {restart}", flush=True)
+    ref = _run_env(monkeypatch, {"FLOWPRO_SPMV_SWEEP": "0"}, props, 5, bits)
+    got = _run_env(monkeypatch, {"FLOWPRO_SPMV_SWEEP": "1"}, props, 5, bits)
+    assert np.array_equal(got[0], ref[0]) and got[1] == ref[1]
+    assert np.array_equal(got[2], ref[2])
+    assert np.array_equal(got[3], ref[3])
