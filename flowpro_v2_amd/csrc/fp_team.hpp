@@ -67,6 +67,24 @@ __device__ inline void team_task(int q, int nel, int& el, int& qp, int& c) {
     }
 }
 
+// Task q of the Jacobian team's (point, input) loop: the value inputs (W_m) of the volume points, their
+// gradient inputs (dW_m/dx, dW_m/dy), then the same two for the face points, so that a wave holds one
+// kind of input at one kind of point: the waves of gradient inputs skip the convective fluxes.
+template <int NQV, int NQFT, int NEQ, int NIN>
+__device__ inline void jacobian_task(int q, int nel, int& el, int& qp, int& c) {
+    constexpr int NGI = NIN - NEQ;
+    const int a0 = nel * NQV * NEQ, a1 = a0 + nel * NQV * NGI, a2 = a1 + nel * NQFT * NEQ;
+    int r, np, base, nin, c0;
+    if (q < a0) { r = q; np = NQV; base = 0; nin = NEQ; c0 = 0; }
+    else if (q < a1) { r = q - a0; np = NQV; base = 0; nin = NGI; c0 = NEQ; }
+    else if (q < a2) { r = q - a1; np = NQFT; base = NQV; nin = NEQ; c0 = 0; }
+    else { r = q - a2; np = NQFT; base = NQV; nin = NGI; c0 = NEQ; }
+    el = r / (np * nin);
+    const int rr = r % (np * nin);
+    qp = base + rr / nin;
+    c = c0 + rr % nin;
+}
+
 // ------------------------------------------------------------------ phase 0 / 1a helpers
 template <int NB, int NEQ, int NQV, int NQF>
 __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double* __restrict__ W, int e0, int nel, double* tab,
@@ -166,8 +184,11 @@ __device__ inline void team_stage(const Tables& tb, const Mesh& ms, const double
 }
 
 // pointwise face integrand g (the reference's K update per face point divided by the
-// test function): -fn + fvn - cIP [W] for the given own-side state
-template <int MK, int NEQ, bool VISC, bool AV, class T>
+// test function): -fn + fvn - cIP [W] for the given own-side state.  GRADONLY (the tangent of a
+// gradient input, the states' tangents zero): the convective flux and the penalty have zero tangents
+// and are not evaluated; their only effect on the rest, the density clipping of limiteRho, is the
+// identity above RHO_TOL, and below it they are evaluated as usual.
+template <int MK, int NEQ, bool VISC, bool AV, class T, bool GRADONLY = false>
 __device__ inline void face_integrand(const ModelParams& P, const DGParams& G, const Mesh& ms, int e, int TT, const double* n, double dL,
                                       const double* X, T* WL, T* dWL, const double* st, T* g, double D) {
     T WR[NEQ], dWR[2 * NEQ];
@@ -179,10 +200,16 @@ __device__ inline void face_integrand(const ModelParams& P, const DGParams& G, c
 #pragma unroll
         for (int m = 0; m < NEQ; m++) { dWR[m] = dWL[m]; dWR[NEQ + m] = dWL[NEQ + m]; }
     }
-    T fn[NEQ];
-    numerical_convective_flux<MK>(P, WL, WR, n, TT, fn);
+    const bool conv = !GRADONLY || value(WL[0]) < RHO_TOL || value(WR[0]) < RHO_TOL;
+    if (conv) {
+        T fn[NEQ];
+        numerical_convective_flux<MK>(P, WL, WR, n, TT, fn);
 #pragma unroll
-    for (int m = 0; m < NEQ; m++) g[m] = -fn[m];
+        for (int m = 0; m < NEQ; m++) g[m] = -fn[m];
+    } else {
+#pragma unroll
+        for (int m = 0; m < NEQ; m++) g[m] = 0.0;
+    }
     if constexpr (VISC) {
         T Wc[NEQ], dWc[2 * NEQ], fvn[NEQ];
 #pragma unroll
@@ -193,7 +220,7 @@ __device__ inline void face_integrand(const ModelParams& P, const DGParams& G, c
             dWc[NEQ + m] = (dWL[NEQ + m] + dWR[NEQ + m]) / 2.0 + jump * n[1];
         }
         numerical_diffusive_flux<MK>(P, Wc, dWc, n, TT, fvn, D);
-        const bool ip = TT > -1 || is_ip_face<MK>(TT);
+        const bool ip = conv && (TT > -1 || is_ip_face<MK>(TT));
 #pragma unroll
         for (int m = 0; m < NEQ; m++) {
             g[m] += fvn[m];
@@ -212,13 +239,19 @@ template <int MK> constexpr int volume_slots() { return has_rans_source<MK>() ? 
 // pointwise volume fluxes (convective minus viscous) in x and y and the model source, called
 // in the order of DGpure.residuum (convection x, y, diffusion x, y, source): each call clips
 // the density of the shared state as the reference does.  D: wall distance at the point.
-template <int MK, int NB, int NEQ, bool VISC, bool AV, class T>
+template <int MK, int NB, int NEQ, bool VISC, bool AV, class T, bool GRADONLY = false>
 __device__ inline void volume_integrand(const ModelParams& P, const DGParams& G, const Mesh& ms, int e, T* Wq, T* dWq, double D, T* fx,
                                         T* fy, T* sv) {
     if constexpr (NB > 1) {
         const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
-        convective_flux<MK>(P, Wq, ex, fx);
-        convective_flux<MK>(P, Wq, ey, fy);
+        // GRADONLY: the convective fluxes have zero tangents (see face_integrand)
+        if (!GRADONLY || value(Wq[0]) < RHO_TOL) {
+            convective_flux<MK>(P, Wq, ex, fx);
+            convective_flux<MK>(P, Wq, ey, fy);
+        } else {
+#pragma unroll
+            for (int m = 0; m < NEQ; m++) { fx[m] = 0.0; fy[m] = 0.0; }
+        }
         if constexpr (VISC) {
             T gx[NEQ], gy[NEQ];
             diffusive_flux<MK>(P, Wq, dWq, ex, gx, D);
@@ -451,8 +484,9 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
     // phase 1b: tangents of the integrands, one (point, input) pair per task
     for (int q = tid; q < nel * LY::NQ * NIN; q += nt) {
         int el, qp, c;
-        team_task<NQV, LY::NQFT, NIN>(q, nel, el, qp, c);
+        jacobian_task<NQV, LY::NQFT, NEQ, NIN>(q, nel, el, qp, c);
         const int e = e0 + el;
+        const bool gradin = NIN > NEQ && c >= NEQ;   // uniform over a wave but at section boundaries
         const int s = c / NEQ, mp = c % NEQ;
         double* E = eb + el * STRIDE;
         const double* ge = ms.geo + (size_t)e * GEO;
@@ -466,7 +500,8 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
                 dWq[NEQ + m] = dual(st[2 * NEQ + m], (s == 2 && m == mp) ? 1.0 : 0.0);
             }
             const double D = has_rans_source<MK>() ? wall_distance_at(ms, e, &tab[LY::T_VX + 2 * qp]) : 0.0;
-            volume_integrand<MK, NB, NEQ, VISC, AV>(P, G, ms, e, Wq, dWq, D, fx, fy, sv);
+            if (gradin) volume_integrand<MK, NB, NEQ, VISC, AV, dual, true>(P, G, ms, e, Wq, dWq, D, fx, fy, sv);
+            else volume_integrand<MK, NB, NEQ, VISC, AV, dual, false>(P, G, ms, e, Wq, dWq, D, fx, fy, sv);
             const double jw = ge[4] * tab[LY::T_VW + qp];
             double* out = &E[O_DV + ((qp * NIN + c) * NVS) * NEQ];
 #pragma unroll
@@ -491,7 +526,8 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
             const double* xi = &tab[LY::T_FX + 2 * f];
             double X[2] = {ge[5] * xi[0] + ge[6] * xi[1] + ge[7], ge[8] * xi[0] + ge[9] * xi[1] + ge[10]};
             const double Df = MK == MODEL_KOMEGA ? face_wall_distance(tb, ms, e, k, p) : 0.0;
-            face_integrand<MK, NEQ, VISC, AV>(P, G, ms, e, TT, n, fg[3], X, WL, dWL, st, g, Df);
+            if (gradin) face_integrand<MK, NEQ, VISC, AV, dual, true>(P, G, ms, e, TT, n, fg[3], X, WL, dWL, st, g, Df);
+            else face_integrand<MK, NEQ, VISC, AV, dual, false>(P, G, ms, e, TT, n, fg[3], X, WL, dWL, st, g, Df);
             const double jw = fg[2] * tab[LY::T_FW + p];
             double* out = &E[O_DF + (f * NIN + c) * NEQ];
 #pragma unroll
