@@ -644,21 +644,6 @@ struct FactorFlow {
 // the row-task record: x = slot in the row | 256 for "A_ij -= L_ik U_kj" | k << 9 for "L_ik = A_ik D_k^-1"
 __device__ inline int task_lower_row(int x) { return x >> 9; }
 
-// A finished value of the row (an updated block, D_i^-1).  Dataflow mode: written at agent scope
-// (through to the device-coherent level) as it is produced, so that the row's release, which must
-// write back the XCD's L2 for the rows that wait on other XCDs, finds the row's own blocks already
-// written back instead of waiting for them; the release and the flag are unchanged.
-template <bool FLOW> __device__ inline void row_store(float* p, float v) {
-    if constexpr (FLOW) __hip_atomic_store(reinterpret_cast<unsigned*>(p), __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else *p = v;
-}
-template <bool FLOW> __device__ inline void row_store(double* p, double v) {
-    if constexpr (FLOW)
-        __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    else *p = v;
-}
-
 template <typename TF, int B, int MAXR, int NW, int NGW, bool FLOW>
 __device__ __attribute__((always_inline)) inline void factor_row(
     int i, const int* __restrict__ rowptr, const int* __restrict__ diagpos, const int* __restrict__ tptr,
@@ -811,7 +796,7 @@ __device__ __attribute__((always_inline)) inline void factor_row(
 #pragma unroll
                 for (int cc = 0; cc < CW; cc++) {
                     const int c = c0 + cc;
-                    if (c < B) row_store<FLOW>(&F[(size_t)(rs + q) * BB + c * B + r], R[q][cc]);
+                    if (c < B) F[(size_t)(rs + q) * BB + c * B + r] = R[q][cc];
                 }
             }
     }
@@ -890,7 +875,7 @@ __device__ __attribute__((always_inline)) inline void factor_row(
                 if (c < B) {
                     const TF v = ic[cc] * myd;
                     if (!isfinite((double)v)) bad = 1;
-                    row_store<FLOW>(&Dg[c * B + dest], v);
+                    Dg[c * B + dest] = v;
                 }
             }
             if (bad) atomicOr(status, 1);
@@ -939,7 +924,7 @@ __device__ __attribute__((always_inline)) inline void factor_row(
                 if (c < B) {
                     const TF v = ic[cc] * myd;
                     if (!isfinite((double)v)) bad = 1;
-                    row_store<FLOW>(&Dg[c * B + dest], v);
+                    Dg[c * B + dest] = v;
                 }
             }
             if (bad) atomicOr(status, 1);
