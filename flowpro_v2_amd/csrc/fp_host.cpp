@@ -2411,6 +2411,11 @@ template <typename TF> static int ilu_autotune(fp_ctx* c) {
     const int nv = ilu_factor_rows_variants();
     c->tune_ms.assign(nv, 0.0);
     c->ilu_variant = ilu_factor_rows_default_variant(c->B);
+    const char* fv = std::getenv("FLOWPRO_ILU_VARIANT");   // a fixed shape (tests of bit-identity)
+    if (fv && std::atoi(fv) >= 0 && std::atoi(fv) < nv) {
+        c->ilu_variant = std::atoi(fv);
+        return FP_OK;
+    }
     const char* g = std::getenv("FLOWPRO_AUTOTUNE");
     if (!c->ilu_rows || (g && g[0] == '0')) return FP_OK;
     hipEvent_t e0, e1;

@@ -1715,16 +1715,18 @@ hipError_t launch_ilu_factor_level(hipStream_t s, int B, const int* rows, int nr
 // tetrahedra's (B = 20 at P1) 5, those of meshes with squares (B = 64: order 4, or padded) 5
 #define FP_ROWREG_SIZES(X) X(20, 5) X(24, 4) X(30, 4) X(36, 4) X(40, 4) X(64, 5)
 // launch shapes of the row kernel, picked per context by timing them on the case's own matrix:
-// (task waves NW, inversion waves NGW)
+// (task waves NW, inversion waves NGW); 16 task waves for chains of few rows per level (the long
+// flat-plate channel has about 64), where a row's tasks split over more waves
 #define FP_ROWREG_VARIANTS(Y, BS, MR) \
-    Y(BS, MR, 0, 4, 4) Y(BS, MR, 1, 8, 4) Y(BS, MR, 2, 8, 2) Y(BS, MR, 3, 8, 8) Y(BS, MR, 4, 4, 2) Y(BS, MR, 5, 4, -4) Y(BS, MR, 6, 8, -8)
-constexpr int ILU_ROW_VARIANTS = 7;
+    Y(BS, MR, 0, 4, 4) Y(BS, MR, 1, 8, 4) Y(BS, MR, 2, 8, 2) Y(BS, MR, 3, 8, 8) Y(BS, MR, 4, 4, 2) Y(BS, MR, 5, 4, -4) Y(BS, MR, 6, 8, -8) \
+    Y(BS, MR, 7, 16, 4) Y(BS, MR, 8, 16, 8)
+constexpr int ILU_ROW_VARIANTS = 9;
 
 int ilu_factor_rows_variants() { return ILU_ROW_VARIANTS; }
 // the shape used without tuning: 4 task waves for B = 20 (5 columns each), 8 above
 int ilu_factor_rows_default_variant(int B) { return B == 20 ? 0 : 1; }
 void ilu_factor_rows_shape(int variant, int* nw, int* ngw) {
-    static const int T[ILU_ROW_VARIANTS][2] = {{4, 4}, {8, 4}, {8, 2}, {8, 8}, {4, 2}, {4, -4}, {8, -8}};
+    static const int T[ILU_ROW_VARIANTS][2] = {{4, 4}, {8, 4}, {8, 2}, {8, 8}, {4, 2}, {4, -4}, {8, -8}, {16, 4}, {16, 8}};
     const int v = variant >= 0 && variant < ILU_ROW_VARIANTS ? variant : 0;
     *nw = T[v][0];
     *ngw = T[v][1];

@@ -329,10 +329,12 @@ class DeviceCase:
         """(chosen shape index or -1, [(task waves, inversion waves)], [ms per factorisation]) of the
         block-ILU0 factor kernel's launch-shape autotuner."""
         v = C.c_int(-1)
-        shapes = np.zeros((8, 2), np.int32)
-        ms = np.zeros(8)
-        n = self._check(native.lib().fp_autotune_report(self.h, C.byref(v), native.iptr(shapes), native.dptr(ms), 8),
+        cap = 16
+        shapes = np.zeros((cap, 2), np.int32)
+        ms = np.zeros(cap)
+        n = self._check(native.lib().fp_autotune_report(self.h, C.byref(v), native.iptr(shapes), native.dptr(ms), cap),
                         "fp_autotune_report")
+        n = min(n, cap)
         return v.value, [tuple(int(x) for x in r) for r in shapes[:n]], ms[:n].tolist()
 
     def time_kernel(self, which: int, reps: int = 10) -> float:

@@ -48,6 +48,31 @@ def test_autotuned_factor_is_bit_identical_and_fastest(name, mk_case, mk_props, 
     rc0, it0, x0, (v0, _, ms0) = _solve(case, props, bits, False)
     assert rc1 == rc0 == 0 and it1 == it0
     assert np.array_equal(x1, x0)
-    assert len(shapes) == len(ms1) >= 4 and all(nw in (4, 8) for nw, _ in shapes)
+    assert len(shapes) == len(ms1) >= 4 and all(nw in (4, 8, 16) for nw, _ in shapes)
     assert all(t > 0 for t in ms1) and v1 == int(np.argmin(ms1))
     assert all(t == 0 for t in ms0) and v0 == (0 if x0.size // case.n_elem == 20 else 1)
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_every_launch_shape_gives_the_same_factor(monkeypatch, bits):
+    """Each launch shape of the factor kernel forced in turn (FLOWPRO_ILU_VARIANT): the GMRES correction,
+    and so the factor and both sweeps, is bit-identical to the default shape's, for every shape the
+    autotuner may pick (4, 8 and 16 task waves; redundant and split inversions)."""
+    case = cases.cylinder_mesh(32, 10)
+    props = dict(cases.ns_props(order=4, reynolds=500.0), iterativeSolverTol=1e-10)
+    monkeypatch.setenv("FLOWPRO_AUTOTUNE", "0")
+    ref = _solve(case, props, bits, False)
+    nv = len(ref[3][1])
+    assert nv >= 9
+    for v in range(nv):
+        monkeypatch.setenv("FLOWPRO_ILU_VARIANT", str(v))
+        dev = solver.DeviceCase(case, props, precond_bits=bits)
+        dev.set_state(cases.perturbed_state(dev.initial_state()[0][:: dev.nb], case.n_elem, dev.nb, 0.01, 3))
+        dev.assemble(0.05, -1.0, 1, True)
+        rc, it, _ = dev.linear_solve()
+        x = dev.solution()
+        assert dev.autotune_report()[0] == v
+        dev.close()
+        assert rc == ref[0] and it == ref[1]
+        assert np.array_equal(x, ref[2]), v
+    monkeypatch.delenv("FLOWPRO_ILU_VARIANT")
