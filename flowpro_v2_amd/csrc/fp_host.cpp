@@ -123,7 +123,7 @@ hipError_t launch_update_solution(hipStream_t, size_t, int, const double*, size_
                                   const int* nvp = nullptr);
 hipError_t launch_gmres_cycle_begin(hipStream_t, GmresCtl*, double*, double*, int);
 hipError_t launch_gmres_givens(hipStream_t, GmresCtl*, int, int, const double*, double*, double*, double*, double*);
-hipError_t launch_gmres_lsolve(hipStream_t, const GmresCtl*, int, const double*, const double*, double*, double*, double*);
+hipError_t launch_gmres_lsolve(hipStream_t, const GmresCtl*, int, const double*, const double*, double*, double*, double*, bool);
 hipError_t launch_axpy(hipStream_t, size_t, double, const double*, double*);
 hipError_t launch_fill(hipStream_t, size_t, double, double*);
 hipError_t launch_copy(hipStream_t, size_t, const double*, double*, const int* skip = nullptr);
@@ -181,6 +181,7 @@ struct fp_ctx {
     unsigned long long *ypk = nullptr, *xpk = nullptr;
     bool ilu_flow = true;                     // one dataflow launch per sweep (else one launch per level)
     bool spmv_sweep = false;                  // GMRES: A v folded into the forward sweep of M^-1 A v
+    bool lsolve_one_thread = false;           // GMRES: the triangular solve in one thread on global memory
     bool factor_flow = false;                 // one dataflow launch for the row-kernel factorisation
     unsigned *fflag = nullptr, *fctl = nullptr;   // per-row completion epochs, {epoch, ticket}
     int factor_groups = 2048;                 // workgroups of the dataflow factorisation
@@ -2116,6 +2117,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         // 47.6 ms per step against 45.6 ms with the product launch)
         const char* sv = std::getenv("FLOWPRO_SPMV_SWEEP");
         c->spmv_sweep = sv && sv[0] == '1' && c->ilu_flow && ilu_sweep_spmv_supported(c->B);
+        const char* lt = std::getenv("FLOWPRO_GMRES_LSOLVE_ONE_THREAD");   // "1": the one-thread form (tests)
+        c->lsolve_one_thread = lt && lt[0] == '1';
         const char* fg = std::getenv("FLOWPRO_FLOW_GROUPS");
         if (fg && std::atoi(fg) > 0) c->flow_groups = std::atoi(fg);
         const char* ff = std::getenv("FLOWPRO_FACTOR_GROUPS");
@@ -2737,7 +2740,7 @@ static int gmres(fp_ctx* c, const double* b, double* x, int* iters, double* err_
             }
             if ((rc = arnoldi_chunk(c, i0, std::min(m, i0 + chunk), predicated))) return rc;
         }
-        HIPCHK(c, launch_gmres_lsolve(c->s, ctl, m, H, sv, Hw, bw, c->y));
+        HIPCHK(c, launch_gmres_lsolve(c->s, ctl, m, H, sv, Hw, bw, c->y, c->lsolve_one_thread));
         HIPCHK(c, launch_update_solution(c->s, N, 0, c->V, N, c->y, x, &ctl->nused));
         HIPCHK(c, hipMemcpyAsync(&h, ctl, sizeof(GmresCtl), hipMemcpyDeviceToHost, c->s));
         HIPCHK(c, hipStreamSynchronize(c->s));

@@ -82,3 +82,18 @@ def test_spmv_folded_into_forward_sweep_bit_identical(monkeypatch, bits, restart
     assert np.array_equal(got[0], ref[0]) and got[1] == ref[1]
     assert np.array_equal(got[2], ref[2])
     assert np.array_equal(got[3], ref[3])
+
+
+@pytest.mark.parametrize("prec", ["blockilu0", "none"])
+def test_lds_triangular_solve_bit_identical(monkeypatch, prec):
+    """The GMRES triangular solve with its working copy in LDS and the rows spread over a wavefront (the
+    default) against the one-thread form on global memory (FLOWPRO_GMRES_LSOLVE_ONE_THREAD=1): the same
+    operations per entry in the same order, so every history, correction and state is bit-identical,
+    over restarted (restart 5) cycles."""
+    props = cases.ns_props(order=3, reynolds=100.0, CFL=5.0, iterativeSolverTol=1e-6, timeOrder=2, gmresRestart=5,
+                           preconditioner=prec)
+    ref = _run_env(monkeypatch, {"FLOWPRO_GMRES_LSOLVE_ONE_THREAD": "1"}, props, 5)
+    got = _run_env(monkeypatch, {"FLOWPRO_GMRES_LSOLVE_ONE_THREAD": "0"}, props, 5)
+    assert np.array_equal(got[0], ref[0]) and got[1] == ref[1]
+    assert np.array_equal(got[2], ref[2])
+    assert np.array_equal(got[3], ref[3])
