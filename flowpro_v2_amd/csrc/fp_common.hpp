@@ -114,6 +114,7 @@ struct QuadMesh {
     const int* nbrf;
     const int* blkn;
     const int* blkd;
+    const double* av;   // eps, dampInner[NEQ] per element (the artificial viscosity), or null
 };
 
 // Meshes that mix element types (triangles, quadrilaterals) and orders.  Every element is padded
@@ -195,10 +196,10 @@ struct AvParams {
 // g += epsFace (avg(grad W) . n + beta0 [W] / dL), epsFace = mean eps + dampConst + mean
 // dampInner[m]; volume: the flux loses (eps + dampConst + dampInner[m]) grad W_m.
 template <int NEQ, class T>
-__device__ inline void av_face_flux(const DGParams& G, const Mesh& ms, int e, int TT, const double* n, double dL, const T* WL,
+__device__ inline void av_face_flux(const DGParams& G, const double* av, int e, int TT, const double* n, double dL, const T* WL,
                                     const T* WR, const T* dWL, const T* dWR, T* g) {
-    const double* ae = ms.av + (size_t)e * (1 + NEQ);
-    const double* an = ms.av + (size_t)TT * (1 + NEQ);
+    const double* ae = av + (size_t)e * (1 + NEQ);
+    const double* an = av + (size_t)TT * (1 + NEQ);
 #pragma unroll
     for (int m = 0; m < NEQ; m++) {
         const double ef = 0.5 * (ae[0] + an[0]) + G.dampConst + 0.5 * (ae[1 + m] + an[1 + m]);
@@ -208,14 +209,65 @@ __device__ inline void av_face_flux(const DGParams& G, const Mesh& ms, int e, in
     }
 }
 template <int NEQ, class T>
-__device__ inline void av_volume_flux(const DGParams& G, const Mesh& ms, int e, const T* dWq, T* fx, T* fy) {
-    const double* ae = ms.av + (size_t)e * (1 + NEQ);
+__device__ inline void av_volume_flux(const DGParams& G, const double* av, int e, const T* dWq, T* fx, T* fy) {
+    const double* ae = av + (size_t)e * (1 + NEQ);
 #pragma unroll
     for (int m = 0; m < NEQ; m++) {
         const double c = ae[0] + G.dampConst + ae[1 + m];
         fx[m] -= c * dWq[m];
         fy[m] -= c * dWq[NEQ + m];
     }
+}
+template <int NEQ, class T>
+__device__ inline void av_face_flux(const DGParams& G, const Mesh& ms, int e, int TT, const double* n, double dL, const T* WL,
+                                    const T* WR, const T* dWL, const T* dWR, T* g) {
+    av_face_flux<NEQ>(G, ms.av, e, TT, n, dL, WL, WR, dWL, dWR, g);
+}
+template <int NEQ, class T>
+__device__ inline void av_volume_flux(const DGParams& G, const Mesh& ms, int e, const T* dWq, T* fx, T* fy) {
+    av_volume_flux<NEQ>(G, ms.av, e, dWq, fx, fy);
+}
+
+// DGpure.shock_senzor / shock_senzor_all_eqn (FlowPro/src/main/java/flowpro/core/element/
+// DGpure.java:403-503) for one equation: the L2 distance of the solution from its truncation
+// (trunc = TrunOrd, the projection onto linear functions in the nodal basis, at order > 2; the
+// mean otherwise) relative to its L2 norm, on a log10 scale, through a sine ramp of half-width kap
+// around S0 = log10(1 / (k-1)^4).  w: the element's NB coefficients of the equation, vb: the
+// basis at the nqv volume points, jw(q): the volume weight of point q.
+template <int NB, class JW>
+__device__ inline double shock_sensor_at(const double* vb, JW jw, int nqv, const double* w, double mean, const double* trunc,
+                                         int order, double kap) {
+    double tc[NB];
+    if (order > 2) {
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < NB; j++) s += trunc[i * NB + j] * w[j];
+            tc[i] = s;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NB; i++) tc[i] = mean;
+    }
+    double Se = 0.0, pod = 0.0;
+    for (int q = 0; q < nqv; q++) {
+        double vi = 0.0, vt = 0.0;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            vi += w[i] * vb[q * NB + i];
+            vt += tc[i] * vb[q * NB + i];
+        }
+        const double c = jw(q);
+        Se += c * (vi - vt) * (vi - vt);
+        pod += c * vi * vi;
+    }
+    Se = log10(fabs(Se / pod));
+    const double S0 = log10(1.0 / pow((double)(order - 1), 4.0));
+    double s = 0.5 * (1.0 + sin(M_PI * (Se - S0) / (2.0 * kap)));
+    if (Se < S0 - kap) s = 0.0;
+    if (Se > S0 + kap) s = 1.0;
+    return isnan(s) ? 0.0 : s;
 }
 
 // Element.positivityLimiter (FlowPro/src/main/java/flowpro/core/element/Element.java:385-476) with

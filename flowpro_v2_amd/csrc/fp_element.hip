@@ -501,40 +501,12 @@ __global__ void k_size_over_lambda(ModelParams P, Tables tb, Mesh ms, const doub
 // around S0 = log10(1 / (k-1)^4).  The Aerodynamics models give every equation the density
 // sensor (Aerodynamics.combineShockSensors); the scalar model has one equation.  trunc is the
 // projection matrix in the nodal basis, the same for every straight triangle (the map is
-// affine, so linear functions and the L2 inner product up to |det J| are preserved).
+// affine, so linear functions and the L2 inner product up to |det J| are preserved), and the
+// reference weights stand for |det J| w.
 template <int NB>
 __device__ inline double shock_sensor(const Tables& tb, int nqv, const double* w, double mean, const double* trunc, int order,
                                       double kap) {
-    double tc[NB];
-    if (order > 2) {
-#pragma unroll
-        for (int i = 0; i < NB; i++) {
-            double s = 0.0;
-#pragma unroll
-            for (int j = 0; j < NB; j++) s += trunc[i * NB + j] * w[j];
-            tc[i] = s;
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < NB; i++) tc[i] = mean;
-    }
-    double Se = 0.0, pod = 0.0;
-    for (int q = 0; q < nqv; q++) {
-        double vi = 0.0, vt = 0.0;
-#pragma unroll
-        for (int i = 0; i < NB; i++) {
-            vi += w[i] * tb.vb[q * NB + i];
-            vt += tc[i] * tb.vb[q * NB + i];
-        }
-        Se += tb.vw[q] * (vi - vt) * (vi - vt);
-        pod += tb.vw[q] * vi * vi;
-    }
-    Se = log10(fabs(Se / pod));
-    const double S0 = log10(1.0 / pow((double)(order - 1), 4.0));
-    double s = 0.5 * (1.0 + sin(M_PI * (Se - S0) / (2.0 * kap)));
-    if (Se < S0 - kap) s = 0.0;
-    if (Se > S0 + kap) s = 1.0;
-    return isnan(s) ? 0.0 : s;
+    return shock_sensor_at<NB>(tb.vb, [&](int q) { return tb.vw[q]; }, nqv, w, mean, trunc, order, kap);
 }
 
 template <int MK, int NB, int NEQ>

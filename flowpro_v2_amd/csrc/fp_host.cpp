@@ -115,6 +115,8 @@ hipError_t launch_jvp_quad(const ElemLaunch&, hipStream_t, const ModelParams&, c
 hipError_t launch_size_over_lambda_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const QuadMesh&, const double*,
                                         const double*, double*);
 hipError_t launch_positivity_quad(const ElemLaunch&, hipStream_t, const QuadTables&, const QuadMesh&, double*, int*);
+hipError_t launch_av_limiter_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const QuadTables&, const QuadMesh&,
+                                  const AvParams&, const double*, const double*, const double*, double*, double*);
 hipError_t launch_dot(hipStream_t, size_t, const double*, const double*, double*, double*, int, const int* skip = nullptr);
 hipError_t launch_axpy_dot(hipStream_t, size_t, double*, const double*, const double*, const double*, double*, double*, int,
                            const int* skip = nullptr);
@@ -1156,6 +1158,45 @@ static int setup_quad(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
         }
         esize[r] = 4 * esize[r] / perim;   // elemSize = nFaces * area / sum S (Element.java:549-554)
     }
+    if (c->dg.av && cs->order > 2) {
+        // DGpure.computeOrderTruncationMatrix (FlowPro/src/main/java/flowpro/core/element/DGpure.java:
+        // 595-626) per element: TrunOrd = M^-1 T^T Mhat^-1 T with the Taylor basis {1, x - Xs, y - Xs}
+        // at the physical volume points, weights |det J| w, M the element's mass matrix
+        std::vector<double> trunc((size_t)n * NB * NB);
+        for (int r = 0; r < n; r++) {
+            auto taylor = [&](int a, int q) {
+                return a == 0 ? 1.0 : xq[((size_t)r * NQT + q) * 2 + a - 1] - Xs[2 * (size_t)r + a - 1];
+            };
+            auto jw = [&](int q) { return qv[((size_t)r * NQV + q) * (1 + 2 * NB)]; };
+            std::vector<double> Mh(9, 0.0), T((size_t)3 * NB, 0.0), Mhi, Mi;
+            for (int a = 0; a < 3; a++) {
+                for (int b = 0; b < 3; b++)
+                    for (int q = 0; q < NQV; q++) Mh[a * 3 + b] += jw(q) * taylor(a, q) * taylor(b, q);
+                for (int b = 0; b < NB; b++)
+                    for (int q = 0; q < NQV; q++) T[a * NB + b] += jw(q) * taylor(a, q) * vb[(size_t)q * NB + b];
+            }
+            std::vector<double> M(mass.begin() + (size_t)r * NB * NB, mass.begin() + (size_t)(r + 1) * NB * NB);
+            if (!invert_scaled_pivot(Mh, 3, Mhi) || !invert_scaled_pivot(M, NB, Mi)) {
+                c->err = "singular matrix in the truncation operator";
+                return FP_EINVAL;
+            }
+            std::vector<double> X((size_t)3 * NB, 0.0), Y((size_t)NB * NB, 0.0);   // X = Mhat^-1 T, Y = T^T X
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < NB; b++)
+                    for (int k = 0; k < 3; k++) X[a * NB + b] += Mhi[a * 3 + k] * T[k * NB + b];
+            for (int a = 0; a < NB; a++)
+                for (int b = 0; b < NB; b++)
+                    for (int k = 0; k < 3; k++) Y[a * NB + b] += T[k * NB + a] * X[k * NB + b];
+            double* tr = &trunc[(size_t)r * NB * NB];
+            for (int a = 0; a < NB; a++)
+                for (int b = 0; b < NB; b++) {
+                    double v = 0;
+                    for (int k = 0; k < NB; k++) v += Mi[a * NB + k] * Y[k * NB + b];
+                    tr[a * NB + b] = v;
+                }
+        }
+        HIPCHK(c, upload(&c->trunc, trunc));
+    }
     HIPCHK(c, upload(&c->qv, qv));
     HIPCHK(c, upload(&c->qf, qf));
     HIPCHK(c, upload(&c->qn, qn));
@@ -1802,8 +1843,9 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         return FP_EINVAL;
     }
     const bool av = cs->damp_tol > 0 || cs->damp_const != 0 || cs->damp_inner_tol > 0;
-    if (av && (c->dim == 3 || c->quad || c->mixed || cs->model == MODEL_SA || cs->model == MODEL_KOMEGA)) {
-        c->err = "artificial viscosity: device kernels for the 2D NavierStokes and scalar models only";
+    if (av && (c->dim == 3 || c->mixed || cs->model == MODEL_SA || cs->model == MODEL_KOMEGA)) {
+        c->err = "artificial viscosity: device kernels for the NavierStokes, AdiabaticEulers and scalar models on triangles "
+                 "and quadrilaterals (orders 2-4)";
         return FP_UNSUPPORTED;
     }
     if (cs->n_damp_inner_coef < 0 || cs->n_damp_inner_coef > 5 ||
@@ -2049,13 +2091,16 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         HIPCHK(c, hipMemsetAsync(c->avold, 0, sizeof(double) * (size_t)n * c->NEQ, c->s));
         // DGpure.computeOrderTruncationMatrix on the reference triangle: for an affine map the
         // projection onto linear functions in the nodal basis does not depend on the element
-        std::vector<double> tr((size_t)c->NB * c->NB, 0.0);
-        if (c->order > 2 && !truncation_matrix(c, tr)) return FP_EINVAL;
-        HIPCHK(c, upload(&c->trunc, tr));
+        // (quadrilaterals hold one per element, built by setup_quad)
+        if (!c->quad) {
+            std::vector<double> tr((size_t)c->NB * c->NB, 0.0);
+            if (c->order > 2 && !truncation_matrix(c, tr)) return FP_EINVAL;
+            HIPCHK(c, upload(&c->trunc, tr));
+        }
     }
     c->ms = Mesh{n, c->geo, c->fgeo, c->nbr, c->nbrf, c->blkn, c->blkd, c->wdist, c->av};
     c->tms = TetMesh{n, c->geo, c->fgeo, c->nbr, c->blkn, c->blkd};
-    c->qms = QuadMesh{n, c->qv, c->qf, c->qn, c->xq, c->fgeo, c->qmass, c->qminv, c->qis, c->nbr, c->nbrf, c->blkn, c->blkd};
+    c->qms = QuadMesh{n, c->qv, c->qf, c->qn, c->xq, c->fgeo, c->qmass, c->qminv, c->qis, c->nbr, c->nbrf, c->blkn, c->blkd, c->av};
     if (c->mixed)
         c->mms = MixedMesh{n, c->mms.nqvm, c->mms.nqfm, c->m_nvd, c->m_nqvd, c->m_nbd, c->m_nqfd, c->m_vol, c->m_face,
                            c->fgeo, c->qmass, c->qminv, c->qis, c->nbr, c->blkn, c->blkd};
@@ -2946,7 +2991,10 @@ extern "C" int fp_limiter(fp_ctx* c, int first_step) {
     (void)first_step;   // the first-iteration branch of DGpure.limiter has an empty body
     if (!c) return FP_EINVAL;
     if (!c->dg.av || c->dim != 2) return FP_OK;
-    HIPCHK(c, launch_av_limiter(c->L, c->s, c->mp, c->tb, c->ms, c->avp, c->W, c->esize, c->isref, c->trunc, c->av, c->avold));
+    if (c->quad)
+        HIPCHK(c, launch_av_limiter_quad(c->L, c->s, c->mp, c->qtb, c->qms, c->avp, c->W, c->esize, c->trunc, c->av, c->avold));
+    else
+        HIPCHK(c, launch_av_limiter(c->L, c->s, c->mp, c->tb, c->ms, c->avp, c->W, c->esize, c->isref, c->trunc, c->av, c->avold));
     HIPCHK(c, hipStreamSynchronize(c->s));
     return FP_OK;
 }

@@ -43,11 +43,12 @@ struct QColumn {    // W + eps e_(e, dir)
     __device__ dual operator()(int el, int i) const { return dual(W[(size_t)el * B + i], i == dir ? 1.0 : 0.0); }
 };
 
-// face integrand g = -fn + fvn - cIP [W] (residuumWall's K update per point divided by the test
-// function), in the reference's call order: boundary value, convective flux, diffusive flux
-template <int MK, int NE, bool VISC, class T>
-__device__ inline void quad_face_flux(const ModelParams& P, const DGParams& G, int TT, const double* n, double dL, const double* X,
-                                      T* WL, T* dWL, T* WR, T* dWR, T* g) {
+// face integrand g = -fn + fvn - cIP [W] (+ the damping flux on interior faces: artificial
+// viscosity of element e and its neighbour) (residuumWall's K update per point divided by the test
+// function), in the reference's call order: boundary value, convective flux, diffusive flux, damping
+template <int MK, int NE, bool VISC, bool AV, class T>
+__device__ inline void quad_face_flux(const ModelParams& P, const DGParams& G, const double* av, int e, int TT, const double* n,
+                                      double dL, const double* X, T* WL, T* dWL, T* WR, T* dWR, T* g) {
     if (TT < 0) {
         boundary_value<MK>(P, WL, n, TT, X, WR);
 #pragma unroll
@@ -74,11 +75,16 @@ __device__ inline void quad_face_flux(const ModelParams& P, const DGParams& G, i
             if (ip) g[m] -= G.cIP * (WL[m] - WR[m]);
         }
     }
+    if constexpr (AV) {
+        if (TT > -1) av_face_flux<NE>(G, av, e, TT, n, dL, WL, WR, dWL, dWR, g);
+    }
 }
 
-// volume fluxes: convective in x and y, minus the viscous ones (DGpure.residuum's order)
-template <int MK, int NE, bool VISC, class T>
-__device__ inline void quad_volume_flux(const ModelParams& P, T* Wq, T* dWq, T* fx, T* fy) {
+// volume fluxes: convective in x and y, minus the viscous ones, minus the damping ones
+// (DGpure.residuum's order)
+template <int MK, int NE, bool VISC, bool AV, class T>
+__device__ inline void quad_volume_flux(const ModelParams& P, const DGParams& G, const double* av, int e, T* Wq, T* dWq, T* fx,
+                                        T* fy) {
     const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
     convective_flux<MK>(P, Wq, ex, fx);
     convective_flux<MK>(P, Wq, ey, fy);
@@ -89,6 +95,7 @@ __device__ inline void quad_volume_flux(const ModelParams& P, T* Wq, T* dWq, T* 
 #pragma unroll
         for (int m = 0; m < NE; m++) { fx[m] -= gx[m]; fy[m] -= gy[m]; }
     }
+    if constexpr (AV) av_volume_flux<NE>(G, av, e, dWq, fx, fy);
 }
 
 // state and physical gradient at a point from coefficients c(m*NB + j)
@@ -112,7 +119,7 @@ __device__ inline void quad_interp(Acc c, int el, const double* phi, const doubl
 
 // Residual of element e: sink.face(k, p, phiL, phiR, jw, g) per face point (phiR null on boundary
 // faces), sink.face_end(k), then sink.vol(jw, gradients, fx, fy) per volume point (order > 1).
-template <int MK, int NB, int NQV, int NQF, bool VISC, class T, class Own, class Nb, class Sink>
+template <int MK, int NB, int NQV, int NQF, bool VISC, bool AV, class T, class Own, class Nb, class Sink>
 __device__ void quad_residual(const ModelParams& P, const DGParams& G, const QuadTables& tb, const QuadMesh& mq, int e, Own own,
                               Nb nbw, Sink& sk) {
     constexpr int NE = neq_of<MK>();
@@ -136,7 +143,7 @@ __device__ void quad_residual(const ModelParams& P, const DGParams& G, const Qua
             }
             const double* X = mq.xq + ((size_t)e * NQT + NQV + k * NQF + p) * 2;
             T g[NE];
-            quad_face_flux<MK, NE, VISC>(P, G, TT, n, dL, X, WL, dWL, WR, dWR, g);
+            quad_face_flux<MK, NE, VISC, AV>(P, G, mq.av, e, TT, n, dL, X, WL, dWL, WR, dWR, g);
             sk.face(k, p, phiL, phiR, jf * tb.fw[p], g);
         }
         sk.face_end(k);
@@ -147,7 +154,7 @@ __device__ void quad_residual(const ModelParams& P, const DGParams& G, const Qua
             const double* qv = mq.qv + ((size_t)e * NQV + q) * (1 + 2 * NB);
             T Wq[NE], dWq[2 * NE], fx[NE], fy[NE];
             quad_interp<NB, NE, T>(own, e, tb.vb + q * NB, qv + 1, Wq, dWq);
-            quad_volume_flux<MK, NE, VISC>(P, Wq, dWq, fx, fy);
+            quad_volume_flux<MK, NE, VISC, AV>(P, G, mq.av, e, Wq, dWq, fx, fy);
             sk.vol(qv[0], qv + 1, fx, fy);
         }
     }
@@ -190,7 +197,7 @@ struct QTangentSink {
 };
 
 // R, RHS = R - M (a1 W + a2 Wo + a3 Wo2), per-element R_m^T M^-1 R_m and RHS_m^T M^-1 RHS_m
-template <int MK, int NB, int NQV, int NQF, bool VISC>
+template <int MK, int NB, int NQV, int NQF, bool VISC, bool AV>
 __global__ void __launch_bounds__(128) k_residual_quad(ModelParams P, DGParams G, QuadTables tb, QuadMesh mq, const double* __restrict__ W,
                                                        const double* __restrict__ Wo, const double* __restrict__ Wo2, double a1,
                                                        double a2, double a3, double* __restrict__ R, double* __restrict__ RHS,
@@ -201,7 +208,7 @@ __global__ void __launch_bounds__(128) k_residual_quad(ModelParams P, DGParams G
     QValueSink<NB, NE> sk;
 #pragma unroll
     for (int i = 0; i < B; i++) sk.K[i] = 0.0;
-    quad_residual<MK, NB, NQV, NQF, VISC, double>(P, G, tb, mq, e, QPlain{W, B}, QPlain{W, B}, sk);
+    quad_residual<MK, NB, NQV, NQF, VISC, AV, double>(P, G, tb, mq, e, QPlain{W, B}, QPlain{W, B}, sk);
     const double* M = mq.mass + (size_t)e * NB * NB;
     double H[B];
 #pragma unroll
@@ -284,7 +291,7 @@ struct QColumnSink : QTangentSink<NB, NE> {
     }
 };
 
-template <int MK, int NB, int NQV, int NQF, bool VISC, typename BT>
+template <int MK, int NB, int NQV, int NQF, bool VISC, bool AV, typename BT>
 __global__ void __launch_bounds__(64) k_jacobian_quad(ModelParams P, DGParams G, QuadTables tb, QuadMesh mq, const double* __restrict__ W,
                                                       double a1, BT* __restrict__ blocks, float* __restrict__ fblocks,
                                                       const double* __restrict__ v, double* __restrict__ out, int mode) {
@@ -303,7 +310,7 @@ __global__ void __launch_bounds__(64) k_jacobian_quad(ModelParams P, DGParams G,
     sk.dir = dir;
     sk.mode = mode;
     sk.dot = 0.0;
-    quad_residual<MK, NB, NQV, NQF, VISC, dual>(P, G, tb, mq, e, QColumn{W, B, dir}, QConst{W, B}, sk);
+    quad_residual<MK, NB, NQV, NQF, VISC, AV, dual>(P, G, tb, mq, e, QColumn{W, B, dir}, QConst{W, B}, sk);
     const double* M = mq.mass + (size_t)e * NB * NB;
     const int md = dir / NB, jd = dir % NB;
     if (mode == 0) {
@@ -329,7 +336,7 @@ __global__ void __launch_bounds__(64) k_jacobian_quad(ModelParams P, DGParams G,
 }
 
 // out = (a1 M - dR/dW) v by one forward tangent pass per element
-template <int MK, int NB, int NQV, int NQF, bool VISC>
+template <int MK, int NB, int NQV, int NQF, bool VISC, bool AV>
 __global__ void __launch_bounds__(64) k_jvp_quad(ModelParams P, DGParams G, QuadTables tb, QuadMesh mq, const double* __restrict__ W,
                                                  const double* __restrict__ v, double a1, double* __restrict__ out) {
     constexpr int NE = neq_of<MK>(), B = NB * NE;
@@ -338,7 +345,7 @@ __global__ void __launch_bounds__(64) k_jvp_quad(ModelParams P, DGParams G, Quad
     QTangentSink<NB, NE> sk;
 #pragma unroll
     for (int i = 0; i < B; i++) sk.K[i] = 0.0;
-    quad_residual<MK, NB, NQV, NQF, VISC, dual>(P, G, tb, mq, e, QTangent{W, v, B}, QTangent{W, v, B}, sk);
+    quad_residual<MK, NB, NQV, NQF, VISC, AV, dual>(P, G, tb, mq, e, QTangent{W, v, B}, QTangent{W, v, B}, sk);
     const double* M = mq.mass + (size_t)e * NB * NB;
 #pragma unroll
     for (int m = 0; m < NE; m++)
@@ -368,6 +375,52 @@ __global__ void k_size_over_lambda_quad(ModelParams P, QuadMesh mq, const double
     }
     const double lam = max_eigenvalue<MK>(P, wa);
     out[e] = lam > 0 ? esize[e] / lam : __builtin_huge_val();
+}
+
+// DGpure.limiter (FlowPro/src/main/java/flowpro/core/element/DGpure.java:358-400) on a
+// quadrilateral: eps = lambda(mean W) elemSize / k g_shock, the inner damping with its decaying
+// memory.  The bilinear map is not affine, so the truncation matrix is the element's own
+// (trunc: [nElem][NB][NB], DGpure.computeOrderTruncationMatrix with the physical Taylor basis)
+// and the sensor integrals take the element's |det J| w per point.  The Aerodynamics models give
+// every equation the density sensor.
+template <int MK, int NB, int NQV>
+__global__ void k_av_limiter_quad(ModelParams P, QuadTables tb, QuadMesh mq, AvParams A, const double* __restrict__ W,
+                                  const double* __restrict__ esize, const double* __restrict__ trunc, double* __restrict__ av,
+                                  double* __restrict__ old) {
+    constexpr int NE = neq_of<MK>(), B = NB * NE;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mq.nElem) return;
+    double* a = av + (size_t)e * (1 + NE);
+    a[0] = 0.0;
+    if (A.order <= 1) return;
+    const double* is = mq.is + (size_t)e * NB;
+    double w[NB], wa[NE];
+#pragma unroll
+    for (int m = 0; m < NE; m++) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) s = s + W[(size_t)e * B + m * NB + j] * is[j];
+        wa[m] = s;
+    }
+    const double mean0 = wa[0];
+#pragma unroll
+    for (int j = 0; j < NB; j++) w[j] = W[(size_t)e * B + j];
+    const double lam = max_eigenvalue<MK>(P, wa);
+    const double scale = lam * esize[e] / A.order;
+    const double* qv = mq.qv + (size_t)e * NQV * (1 + 2 * NB);
+    auto jw = [&](int q) { return qv[(size_t)q * (1 + 2 * NB)]; };
+    const double* tr = trunc ? trunc + (size_t)e * NB * NB : nullptr;
+    if (A.dampTol > 0) a[0] = scale * shock_sensor_at<NB>(tb.vb, jw, NQV, w, mean0, tr, A.order, A.dampTol);
+    if (A.dampInnerTol > 0) {
+        const double g = shock_sensor_at<NB>(tb.vb, jw, NQV, w, mean0, tr, A.order, A.dampInnerTol);
+#pragma unroll
+        for (int m = 0; m < NE; m++) {
+            double& o = old[(size_t)e * NE + m];
+            const double ind = fmax(g, o);
+            o = 0.1 * ind;
+            a[1 + m] = A.nCoef > 0 ? (A.nCoef == 1 ? A.coef[0] : A.coef[m]) * ind : scale * ind;
+        }
+    }
 }
 
 // Element.positivityLimiter (Element.java:385-476) over the volume and the four faces' points
@@ -456,28 +509,38 @@ __global__ void k_functional_quad(QuadTables tb, QuadMesh mq, const double* __re
     }
 }
 
+// (model, NB, NQV, NQF, viscous, artificial viscosity): the damping terms at orders 2-4
 #define FP_QUAD_CASES(X)                               \
-    X(MODEL_NAVIER_STOKES, 1, 1, 1, false)             \
-    X(MODEL_NAVIER_STOKES, 4, 4, 2, false)             \
-    X(MODEL_NAVIER_STOKES, 9, 9, 3, false)             \
-    X(MODEL_NAVIER_STOKES, 16, 16, 4, false)           \
-    X(MODEL_NAVIER_STOKES, 25, 25, 5, false)           \
-    X(MODEL_NAVIER_STOKES, 1, 1, 1, true)              \
-    X(MODEL_NAVIER_STOKES, 4, 4, 2, true)              \
-    X(MODEL_NAVIER_STOKES, 9, 9, 3, true)              \
-    X(MODEL_NAVIER_STOKES, 16, 16, 4, true)            \
-    X(MODEL_NAVIER_STOKES, 25, 25, 5, true)            \
-    X(MODEL_ADIABATIC, 1, 1, 1, false)                 \
-    X(MODEL_ADIABATIC, 4, 4, 2, false)                 \
-    X(MODEL_ADIABATIC, 9, 9, 3, false)                 \
-    X(MODEL_ADIABATIC, 16, 16, 4, false)               \
-    X(MODEL_ADIABATIC, 25, 25, 5, false)
-#define FP_QMATCH(MK, NB, NQV, NQF, V) \
-    (L.model == MK && L.nb == NB && L.nqv == NQV && L.nqf == NQF && (L.visc != 0) == V && !L.av)
+    X(MODEL_NAVIER_STOKES, 1, 1, 1, false, false)      \
+    X(MODEL_NAVIER_STOKES, 4, 4, 2, false, false)      \
+    X(MODEL_NAVIER_STOKES, 9, 9, 3, false, false)      \
+    X(MODEL_NAVIER_STOKES, 16, 16, 4, false, false)    \
+    X(MODEL_NAVIER_STOKES, 25, 25, 5, false, false)    \
+    X(MODEL_NAVIER_STOKES, 1, 1, 1, true, false)       \
+    X(MODEL_NAVIER_STOKES, 4, 4, 2, true, false)       \
+    X(MODEL_NAVIER_STOKES, 9, 9, 3, true, false)       \
+    X(MODEL_NAVIER_STOKES, 16, 16, 4, true, false)     \
+    X(MODEL_NAVIER_STOKES, 25, 25, 5, true, false)     \
+    X(MODEL_ADIABATIC, 1, 1, 1, false, false)          \
+    X(MODEL_ADIABATIC, 4, 4, 2, false, false)          \
+    X(MODEL_ADIABATIC, 9, 9, 3, false, false)          \
+    X(MODEL_ADIABATIC, 16, 16, 4, false, false)        \
+    X(MODEL_ADIABATIC, 25, 25, 5, false, false)        \
+    X(MODEL_NAVIER_STOKES, 4, 4, 2, false, true)       \
+    X(MODEL_NAVIER_STOKES, 9, 9, 3, false, true)       \
+    X(MODEL_NAVIER_STOKES, 16, 16, 4, false, true)     \
+    X(MODEL_NAVIER_STOKES, 4, 4, 2, true, true)        \
+    X(MODEL_NAVIER_STOKES, 9, 9, 3, true, true)        \
+    X(MODEL_NAVIER_STOKES, 16, 16, 4, true, true)      \
+    X(MODEL_ADIABATIC, 4, 4, 2, false, true)           \
+    X(MODEL_ADIABATIC, 9, 9, 3, false, true)           \
+    X(MODEL_ADIABATIC, 16, 16, 4, false, true)
+#define FP_QMATCH(MK, NB, NQV, NQF, V, A) \
+    (L.model == MK && L.nb == NB && L.nqv == NQV && L.nqf == NQF && (L.visc != 0) == V && (L.av != 0) == A)
 
 bool quad_supported(const ElemLaunch& L) {
-#define X(MK, NB, NQV, NQF, V) \
-    if (FP_QMATCH(MK, NB, NQV, NQF, V)) return true;
+#define X(MK, NB, NQV, NQF, V, A) \
+    if (FP_QMATCH(MK, NB, NQV, NQF, V, A)) return true;
     FP_QUAD_CASES(X)
 #undef X
     return false;
@@ -487,9 +550,9 @@ hipError_t launch_residual_quad(const ElemLaunch& L, hipStream_t s, const ModelP
                                 const QuadMesh& mq, const double* W, const double* Wo, const double* Wo2, double a1, double a2,
                                 double a3, double* R, double* RHS, double* enorm) {
     dim3 blk(128), grd((mq.nElem + 127) / 128);
-#define X(MK, NB, NQV, NQF, V)                                                                                       \
-    if (FP_QMATCH(MK, NB, NQV, NQF, V)) {                                                                            \
-        hipLaunchKernelGGL((k_residual_quad<MK, NB, NQV, NQF, V>), grd, blk, 0, s, P, G, tb, mq, W, Wo, Wo2, a1, a2, a3, R, \
+#define X(MK, NB, NQV, NQF, V, A)                                                                                       \
+    if (FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                                                            \
+        hipLaunchKernelGGL((k_residual_quad<MK, NB, NQV, NQF, V, A>), grd, blk, 0, s, P, G, tb, mq, W, Wo, Wo2, a1, a2, a3, R, \
                            RHS, enorm);                                                                              \
         return hipGetLastError();                                                                                    \
     }
@@ -503,9 +566,9 @@ hipError_t launch_jacobian_quad(const ElemLaunch& L, hipStream_t s, const ModelP
                                 double* out, int mode) {
     const long long threads = (long long)mq.nElem * L.nb * L.neq;
     dim3 blk(64), grd((unsigned)((threads + 63) / 64));
-#define X(MK, NB, NQV, NQF, V)                                                                                          \
-    if (FP_QMATCH(MK, NB, NQV, NQF, V)) {                                                                               \
-        hipLaunchKernelGGL((k_jacobian_quad<MK, NB, NQV, NQF, V, double>), grd, blk, 0, s, P, G, tb, mq, W, a1, blocks, fblocks, v, \
+#define X(MK, NB, NQV, NQF, V, A)                                                                                          \
+    if (FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                                                               \
+        hipLaunchKernelGGL((k_jacobian_quad<MK, NB, NQV, NQF, V, A, double>), grd, blk, 0, s, P, G, tb, mq, W, a1, blocks, fblocks, v, \
                            out, mode);                                                                                  \
         return hipGetLastError();                                                                                       \
     }
@@ -517,9 +580,9 @@ hipError_t launch_jacobian_quad(const ElemLaunch& L, hipStream_t s, const ModelP
 hipError_t launch_jvp_quad(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const QuadTables& tb,
                            const QuadMesh& mq, const double* W, const double* v, double a1, double* out) {
     dim3 blk(64), grd((mq.nElem + 63) / 64);
-#define X(MK, NB, NQV, NQF, V)                                                                                 \
-    if (FP_QMATCH(MK, NB, NQV, NQF, V)) {                                                                      \
-        hipLaunchKernelGGL((k_jvp_quad<MK, NB, NQV, NQF, V>), grd, blk, 0, s, P, G, tb, mq, W, v, a1, out);    \
+#define X(MK, NB, NQV, NQF, V, A)                                                                                 \
+    if (FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                                                      \
+        hipLaunchKernelGGL((k_jvp_quad<MK, NB, NQV, NQF, V, A>), grd, blk, 0, s, P, G, tb, mq, W, v, a1, out);    \
         return hipGetLastError();                                                                              \
     }
     FP_QUAD_CASES(X)
@@ -530,8 +593,8 @@ hipError_t launch_jvp_quad(const ElemLaunch& L, hipStream_t s, const ModelParams
 hipError_t launch_size_over_lambda_quad(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const QuadMesh& mq, const double* W,
                                         const double* esize, double* out) {
     dim3 blk(256), grd((mq.nElem + 255) / 256);
-#define X(MK, NB, NQV, NQF, V)                                                                          \
-    if (FP_QMATCH(MK, NB, NQV, NQF, V)) {                                                               \
+#define X(MK, NB, NQV, NQF, V, A)                                                                          \
+    if (FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                                               \
         hipLaunchKernelGGL((k_size_over_lambda_quad<MK, NB>), grd, blk, 0, s, P, mq, W, esize, out);    \
         return hipGetLastError();                                                                       \
     }
@@ -540,10 +603,24 @@ hipError_t launch_size_over_lambda_quad(const ElemLaunch& L, hipStream_t s, cons
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_av_limiter_quad(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const QuadTables& tb, const QuadMesh& mq,
+                                  const AvParams& Ap, const double* W, const double* esize, const double* trunc, double* av,
+                                  double* old) {
+    dim3 blk(128), grd((mq.nElem + 127) / 128);
+#define X(MK, NB, NQV, NQF, V, A)                                                                                     \
+    if (A && FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                                                     \
+        hipLaunchKernelGGL((k_av_limiter_quad<MK, NB, NQV>), grd, blk, 0, s, P, tb, mq, Ap, W, esize, trunc, av, old); \
+        return hipGetLastError();                                                                                     \
+    }
+    FP_QUAD_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_positivity_quad(const ElemLaunch& L, hipStream_t s, const QuadTables& tb, const QuadMesh& mq, double* W, int* count) {
     dim3 blk(128), grd((mq.nElem + 127) / 128);
-#define X(MK, NB, NQV, NQF, V)                                                                          \
-    if (FP_QMATCH(MK, NB, NQV, NQF, V)) {                                                               \
+#define X(MK, NB, NQV, NQF, V, A)                                                                          \
+    if (FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                                               \
         hipLaunchKernelGGL((k_positivity_quad<MK, NB, NQV, NQF>), grd, blk, 0, s, tb, mq, W, count);    \
         return hipGetLastError();                                                                       \
     }
@@ -556,8 +633,8 @@ hipError_t launch_functional_quad(const ElemLaunch& L, hipStream_t s, const Quad
                                   double kappa, int which, double* ival, double* grad) {
     if (L.model != MODEL_NAVIER_STOKES) return hipErrorInvalidValue;
     dim3 blk(128), grd((mq.nElem + 127) / 128);
-#define X(MK, NB, NQV, NQF, V)                                                                                   \
-    if (MK == MODEL_NAVIER_STOKES && FP_QMATCH(MK, NB, NQV, NQF, V)) {                                            \
+#define X(MK, NB, NQV, NQF, V, A)                                                                                   \
+    if (MK == MODEL_NAVIER_STOKES && FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                            \
         hipLaunchKernelGGL((k_functional_quad<NB, NQF>), grd, blk, 0, s, tb, mq, W, kappa, which, ival, grad);   \
         return hipGetLastError();                                                                                 \
     }

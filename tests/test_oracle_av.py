@@ -13,6 +13,13 @@ No reference test covers the damping; the checks are what the definitions fix:
   same field with the vertex values in another node order is discontinuous and fails it
   the inner damping memory: after a sensor value g, a state with a zero sensor gets the
   indicator 0.1 g (dampInner = lambda h / k * 0.1 g, or dampInnerCoef * 0.1 g) (1e-12)
+On jittered (non-affine) quadrilaterals:
+  TrunOrd is the element's own M-orthogonal projection onto span{1, x, y} in physical
+  coordinates: the properties above, and it keeps the nodal values of x and y, which are bilinear
+  on the reference square (the Taylor basis taken in reference coordinates fails this) (1e-11)
+  at order 2 (Q1, truncation to the mean) the density sensor of a bilinear density follows from
+  integrals over the bilinear map with numpy's own Gauss-Legendre rule; eps = lambda(mean state)
+  * (4 area / perimeter) / 2 * g (1e-12)
 """
 import math
 
@@ -154,3 +161,83 @@ def _is(o, e):
     are all ones, so M 1 / area holds the basis integrals."""
     g = o.geometry(e)
     return g["M"] @ np.ones(o.nb) / g["area"]
+
+
+def _quad_nodes_physical(P, k):
+    """Physical positions of basis2Dsquare's nodes (node i k + j at (xi, eta) = (b_i, b_j), b =
+    linspace(0, 1, k)) under transformation2Dsquare's bilinear map of the vertices P[0..3] at
+    (0,0), (1,0), (1,1), (0,1)."""
+    b = np.linspace(0, 1, k)
+    X = np.zeros((k * k, 2))
+    for i in range(k):
+        for j in range(k):
+            xi, eta = b[i], b[j]
+            X[i * k + j] = ((1 - xi) * (1 - eta) * P[0] + xi * (1 - eta) * P[1] + xi * eta * P[2]
+                            + (1 - xi) * eta * P[3])
+    return X
+
+
+@pytest.mark.parametrize("order", [3, 4])
+def test_quad_truncation_matrix_projects_onto_physical_linears(order):
+    case = cases.channel_bump(6, 3, quads=True, jitter=0.25)
+    o = O.Oracle(case, cases.euler_props(order=order, dampTol=1.0))
+    for e in (0, 5, 11, case.n_elem - 1):
+        P = o.trunc(e)
+        M = o.geometry(e)["M"]
+        s = np.abs(P).max()
+        assert np.abs(P @ P - P).max() <= 1e-11 * s
+        assert abs(np.trace(P) - 3) <= 1e-11
+        MP = M @ P
+        assert np.abs(MP - MP.T).max() <= 1e-11 * np.abs(MP).max()
+        X = _quad_nodes_physical(case.PXY[case.TP[e]], order)
+        for f in (np.ones(o.nb), X[:, 0], X[:, 1]):
+            assert np.abs(P @ f - f).max() <= 1e-11 * np.abs(f).max()
+        # a bilinear function of the reference coordinates that is not linear in x, y is not kept
+        g = X[:, 0] * X[:, 1]
+        assert np.abs(P @ g - g).max() > 1e-6 * np.abs(g).max()
+
+
+@pytest.mark.parametrize("kap,amp", [(3.0, 0.2), (1e-3, 0.02)])
+def test_quad_density_sensor_at_order_two(kap, amp):
+    case = cases.channel_bump(8, 3, quads=True, jitter=0.25)
+    o = O.Oracle(case, cases.euler_props(order=2, dampTol=kap))
+    assert o.nb == 4
+    w0 = o.get_w()[0][:: o.nb]
+    rho = w0[0] * (1 + amp * np.random.default_rng(6).standard_normal((case.n_elem, 4)))
+    W = np.zeros((case.n_elem, 4, 4))
+    W[:, 0] = rho
+    for m in (1, 2, 3):
+        W[:, m] = w0[m]
+    o.set_w(W.reshape(case.n_elem, 16))
+    o.limiter()
+    eps, di = o.artificial_viscosity()
+    assert np.all(di == 0)
+    x, w = np.polynomial.legendre.leggauss(6)
+    x, w = (x + 1) / 2, w / 2
+    xi, eta = np.meshgrid(x, x, indexing="ij")
+    ww = np.outer(w, w)
+    for e in range(case.n_elem):
+        P = case.PXY[case.TP[e]]
+        # Jacobian of the bilinear map and the density at the points; nodes (0,0), (0,1), (1,0), (1,1)
+        dxi = (1 - eta)[..., None] * (P[1] - P[0]) + eta[..., None] * (P[2] - P[3])
+        deta = (1 - xi)[..., None] * (P[3] - P[0]) + xi[..., None] * (P[2] - P[1])
+        J = np.abs(dxi[..., 0] * deta[..., 1] - dxi[..., 1] * deta[..., 0])
+        r = rho[e]
+        f = r[0] * (1 - xi) * (1 - eta) + r[1] * (1 - xi) * eta + r[2] * xi * (1 - eta) + r[3] * xi * eta
+        A = (J * ww).sum()
+        mean = (f * J * ww).sum() / A
+        i2 = (f * f * J * ww).sum()
+        Se = math.log10(((f - mean) ** 2 * J * ww).sum() / i2)
+        g = 0.5 * (1 + math.sin(math.pi * Se / (2 * kap)))
+        if Se < -kap:
+            g = 0.0
+        if Se > kap:
+            g = 1.0
+        perim = sum(np.linalg.norm(P[(k + 1) % 4] - P[k]) for k in range(4))
+        wbar = np.array([mean, w0[1], w0[2], w0[3]])
+        ref = _lambda(wbar, 1.4) * (4 * A / perim) / 2 * g
+        assert abs(eps[e] - ref) <= 1e-12 * max(abs(ref), 1e-300) + 1e-15
+    if kap == 3.0:
+        assert (eps > 0).mean() > 0.5
+    else:
+        assert (eps == 0).all()
