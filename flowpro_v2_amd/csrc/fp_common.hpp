@@ -142,6 +142,7 @@ struct MixedMesh {
     const int* nbr;
     const int* blkn;
     const int* blkd;
+    const double* av;   // eps, dampInner[NEQ] per element (the artificial viscosity), or null
 };
 
 // compile-time shape of the element kernels selected at run time
@@ -232,11 +233,11 @@ __device__ inline void av_volume_flux(const DGParams& G, const Mesh& ms, int e, 
 // DGpure.java:403-503) for one equation: the L2 distance of the solution from its truncation
 // (trunc = TrunOrd, the projection onto linear functions in the nodal basis, at order > 2; the
 // mean otherwise) relative to its L2 norm, on a log10 scale, through a sine ramp of half-width kap
-// around S0 = log10(1 / (k-1)^4).  w: the element's NB coefficients of the equation, vb: the
-// basis at the nqv volume points, jw(q): the volume weight of point q.
-template <int NB, class JW>
-__device__ inline double shock_sensor_at(const double* vb, JW jw, int nqv, const double* w, double mean, const double* trunc,
-                                         int order, double kap) {
+// around S0 = log10(1 / (k-1)^4).  w: the element's NB coefficients of the equation, phi(q): the
+// NB basis values at volume point q of the nqv, jw(q): the volume weight of point q.
+template <int NB, class PHI, class JW>
+__device__ inline double shock_sensor_at(PHI phi, JW jw, int nqv, const double* w, double mean, const double* trunc, int order,
+                                         double kap) {
     double tc[NB];
     if (order > 2) {
 #pragma unroll
@@ -252,11 +253,12 @@ __device__ inline double shock_sensor_at(const double* vb, JW jw, int nqv, const
     }
     double Se = 0.0, pod = 0.0;
     for (int q = 0; q < nqv; q++) {
+        const double* b = phi(q);
         double vi = 0.0, vt = 0.0;
 #pragma unroll
         for (int i = 0; i < NB; i++) {
-            vi += w[i] * vb[q * NB + i];
-            vt += tc[i] * vb[q * NB + i];
+            vi += w[i] * b[i];
+            vt += tc[i] * b[i];
         }
         const double c = jw(q);
         Se += c * (vi - vt) * (vi - vt);

@@ -506,7 +506,8 @@ __global__ void k_size_over_lambda(ModelParams P, Tables tb, Mesh ms, const doub
 template <int NB>
 __device__ inline double shock_sensor(const Tables& tb, int nqv, const double* w, double mean, const double* trunc, int order,
                                       double kap) {
-    return shock_sensor_at<NB>(tb.vb, [&](int q) { return tb.vw[q]; }, nqv, w, mean, trunc, order, kap);
+    return shock_sensor_at<NB>([&](int q) { return tb.vb + q * NB; }, [&](int q) { return tb.vw[q]; }, nqv, w, mean, trunc, order,
+                               kap);
 }
 
 template <int MK, int NB, int NEQ>

@@ -117,6 +117,8 @@ hipError_t launch_size_over_lambda_quad(const ElemLaunch&, hipStream_t, const Mo
 hipError_t launch_positivity_quad(const ElemLaunch&, hipStream_t, const QuadTables&, const QuadMesh&, double*, int*);
 hipError_t launch_av_limiter_quad(const ElemLaunch&, hipStream_t, const ModelParams&, const QuadTables&, const QuadMesh&,
                                   const AvParams&, const double*, const double*, const double*, double*, double*);
+hipError_t launch_av_limiter_mixed(const ElemLaunch&, hipStream_t, const ModelParams&, const MixedMesh&, const AvParams&,
+                                   const double*, const double*, const double*, double*, double*);
 hipError_t launch_dot(hipStream_t, size_t, const double*, const double*, double*, double*, int, const int* skip = nullptr);
 hipError_t launch_axpy_dot(hipStream_t, size_t, double*, const double*, const double*, const double*, double*, double*, int,
                            const int* skip = nullptr);
@@ -1435,6 +1437,7 @@ static int setup_mixed(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
     std::vector<double> vol((size_t)n * nqvm * VS, 0.0), face((size_t)n * 4 * nqfm * FS, 0.0), fgeo((size_t)n * 16, 0.0);
     std::vector<double> mass((size_t)n * NB * NB, 0.0), minv((size_t)n * NB * NB, 0.0), qis((size_t)n * NB, 0.0), esize(n), Xs(2 * (size_t)n);
     std::vector<int> nvd(n), nqvd(n), nbd(n), nqfd((size_t)4 * n, 0);
+    std::vector<double> trunc(c->dg.av ? (size_t)n * NB * NB : 0, 0.0);
     nbr.assign(4 * (size_t)n, -1000);
     c->h_qis.assign((size_t)n * NB, 0.0);
     for (int r = 0; r < n; r++) {
@@ -1481,6 +1484,40 @@ static int setup_mixed(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
             for (int b = 0; b < NB; b++)
                 minv[(size_t)r * NB * NB + a * NB + b] = (a < nb && b < nb) ? Mi[(size_t)a * nb + b] : (a == b ? 1.0 : 0.0);
         esize[r] = area;
+        if (c->dg.av && ord[e] > 2) {
+            // DGpure.computeOrderTruncationMatrix (FlowPro/src/main/java/flowpro/core/element/DGpure.java:
+            // 595-626): TrunOrd = M^-1 T^T Mhat^-1 T with the Taylor basis {1, x - Xs, y - Xs} at the
+            // element's physical volume points, padded with zeros to NB x NB
+            std::vector<double> Xq(2 * (size_t)vr.n);
+            for (int q = 0; q < vr.n; q++) getX(e, &vr.x[2 * q], &Xq[2 * q]);
+            auto taylor = [&](int a, int q) { return a == 0 ? 1.0 : Xq[2 * q + a - 1] - Xs[2 * (size_t)r + a - 1]; };
+            auto vq = [&](int q) { return &vol[((size_t)r * nqvm + q) * VS]; };
+            std::vector<double> Mh(9, 0.0), T((size_t)3 * nb, 0.0), Mhi, Mri;
+            for (int a = 0; a < 3; a++) {
+                for (int b = 0; b < 3; b++)
+                    for (int q = 0; q < vr.n; q++) Mh[a * 3 + b] += vq(q)[0] * taylor(a, q) * taylor(b, q);
+                for (int b = 0; b < nb; b++)
+                    for (int q = 0; q < vr.n; q++) T[(size_t)a * nb + b] += vq(q)[0] * taylor(a, q) * vq(q)[1 + b];
+            }
+            if (!invert_scaled_pivot(Mh, 3, Mhi) || !invert_scaled_pivot(Mr, nb, Mri)) {
+                c->err = "singular matrix in the truncation operator";
+                return FP_EINVAL;
+            }
+            std::vector<double> X((size_t)3 * nb, 0.0), Y((size_t)nb * nb, 0.0);   // X = Mhat^-1 T, Y = T^T X
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < nb; b++)
+                    for (int kk = 0; kk < 3; kk++) X[(size_t)a * nb + b] += Mhi[a * 3 + kk] * T[(size_t)kk * nb + b];
+            for (int a = 0; a < nb; a++)
+                for (int b = 0; b < nb; b++)
+                    for (int kk = 0; kk < 3; kk++) Y[(size_t)a * nb + b] += T[(size_t)kk * nb + a] * X[(size_t)kk * nb + b];
+            double* tr = &trunc[(size_t)r * NB * NB];
+            for (int a = 0; a < nb; a++)
+                for (int b = 0; b < nb; b++) {
+                    double v = 0;
+                    for (int kk = 0; kk < nb; kk++) v += Mri[(size_t)a * nb + kk] * Y[(size_t)kk * nb + b];
+                    tr[a * NB + b] = v;
+                }
+        }
     }
     for (int r = 0; r < n; r++) {
         const int e = c->order_[r];
@@ -1541,6 +1578,7 @@ static int setup_mixed(const fp_case* cs, fp_ctx* c, std::vector<int>& nbr) {
     }
     c->ref_dofs = g;
     HIPCHK(c, upload(&c->m_vol, vol));
+    if (c->dg.av) HIPCHK(c, upload(&c->trunc, trunc));
     HIPCHK(c, upload(&c->m_face, face));
     HIPCHK(c, upload(&c->fgeo, fgeo));
     HIPCHK(c, upload(&c->esize, esize));
@@ -1843,9 +1881,9 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         return FP_EINVAL;
     }
     const bool av = cs->damp_tol > 0 || cs->damp_const != 0 || cs->damp_inner_tol > 0;
-    if (av && (c->dim == 3 || c->mixed || cs->model == MODEL_SA || cs->model == MODEL_KOMEGA)) {
-        c->err = "artificial viscosity: device kernels for the NavierStokes, AdiabaticEulers and scalar models on triangles "
-                 "and quadrilaterals (orders 2-4)";
+    if (av && (c->dim == 3 || cs->model == MODEL_SA || cs->model == MODEL_KOMEGA)) {
+        c->err = "artificial viscosity: device kernels for the NavierStokes, AdiabaticEulers and scalar models in 2D (up to "
+                 "16 basis functions on quadrilaterals and mixed meshes)";
         return FP_UNSUPPORTED;
     }
     if (cs->n_damp_inner_coef < 0 || cs->n_damp_inner_coef > 5 ||
@@ -2092,7 +2130,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         // DGpure.computeOrderTruncationMatrix on the reference triangle: for an affine map the
         // projection onto linear functions in the nodal basis does not depend on the element
         // (quadrilaterals hold one per element, built by setup_quad)
-        if (!c->quad) {
+        if (!c->quad && !c->mixed) {
             std::vector<double> tr((size_t)c->NB * c->NB, 0.0);
             if (c->order > 2 && !truncation_matrix(c, tr)) return FP_EINVAL;
             HIPCHK(c, upload(&c->trunc, tr));
@@ -2103,7 +2141,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     c->qms = QuadMesh{n, c->qv, c->qf, c->qn, c->xq, c->fgeo, c->qmass, c->qminv, c->qis, c->nbr, c->nbrf, c->blkn, c->blkd, c->av};
     if (c->mixed)
         c->mms = MixedMesh{n, c->mms.nqvm, c->mms.nqfm, c->m_nvd, c->m_nqvd, c->m_nbd, c->m_nqfd, c->m_vol, c->m_face,
-                           c->fgeo, c->qmass, c->qminv, c->qis, c->nbr, c->blkn, c->blkd};
+                           c->fgeo, c->qmass, c->qminv, c->qis, c->nbr, c->blkn, c->blkd, c->av};
 
     // --- state, matrix, Krylov workspace
     const size_t N = (size_t)c->dofs, BB = (size_t)c->B * c->B;
@@ -2993,6 +3031,8 @@ extern "C" int fp_limiter(fp_ctx* c, int first_step) {
     if (!c->dg.av || c->dim != 2) return FP_OK;
     if (c->quad)
         HIPCHK(c, launch_av_limiter_quad(c->L, c->s, c->mp, c->qtb, c->qms, c->avp, c->W, c->esize, c->trunc, c->av, c->avold));
+    else if (c->mixed)
+        HIPCHK(c, launch_av_limiter_mixed(c->L, c->s, c->mp, c->mms, c->avp, c->W, c->esize, c->trunc, c->av, c->avold));
     else
         HIPCHK(c, launch_av_limiter(c->L, c->s, c->mp, c->tb, c->ms, c->avp, c->W, c->esize, c->isref, c->trunc, c->av, c->avold));
     HIPCHK(c, hipStreamSynchronize(c->s));

@@ -42,9 +42,11 @@ struct MColumn {
 };
 }  // namespace
 
-template <int MK, int NE, bool VISC, class T>
-__device__ inline void mixed_face_flux(const ModelParams& P, const DGParams& G, double cIP, int TT, const double* n, double dL,
-                                       const double* X, T* WL, T* dWL, T* WR, T* dWR, T* g) {
+// face integrand: boundary value, convective flux, diffusive flux with the element's c_IP, then
+// (AV) the damping flux of element e and its neighbour on interior faces
+template <int MK, int NE, bool VISC, bool AV, class T>
+__device__ inline void mixed_face_flux(const ModelParams& P, const DGParams& G, const MixedMesh& mm, int e, double cIP, int TT,
+                                       const double* n, double dL, const double* X, T* WL, T* dWL, T* WR, T* dWR, T* g) {
     if (TT < 0) {
         boundary_value<MK>(P, WL, n, TT, X, WR);
 #pragma unroll
@@ -71,6 +73,27 @@ __device__ inline void mixed_face_flux(const ModelParams& P, const DGParams& G, 
             if (ip) g[m] -= cIP * (WL[m] - WR[m]);
         }
     }
+    if constexpr (AV) {
+        if (TT > -1) av_face_flux<NE>(G, mm.av, e, TT, n, dL, WL, WR, dWL, dWR, g);
+    }
+}
+
+// volume fluxes at a point: convective in x and y, minus the viscous ones, minus (AV) the damping
+// ones (DGpure.residuum's order)
+template <int MK, int NE, bool VISC, bool AV, class T>
+__device__ inline void mixed_volume_flux(const ModelParams& P, const DGParams& G, const MixedMesh& mm, int e, T* Wq, T* dWq, T* fx,
+                                         T* fy) {
+    const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
+    convective_flux<MK>(P, Wq, ex, fx);
+    convective_flux<MK>(P, Wq, ey, fy);
+    if constexpr (VISC) {
+        T gx[NE], gy[NE];
+        diffusive_flux<MK>(P, Wq, dWq, ex, gx);
+        diffusive_flux<MK>(P, Wq, dWq, ey, gy);
+#pragma unroll
+        for (int m = 0; m < NE; m++) { fx[m] -= gx[m]; fy[m] -= gy[m]; }
+    }
+    if constexpr (AV) av_volume_flux<NE>(G, mm.av, e, dWq, fx, fy);
 }
 
 template <int NB, int NE, class T, class Acc>
@@ -98,7 +121,7 @@ __device__ inline double padded_diagonal(double a1) { return a1 != 0.0 ? a1 : 1.
 // DGpure.limiter's c_IP: the penalty on elements of order > 1 of a diffusive model
 __device__ inline double mixed_cip(const DGParams& G, const MixedMesh& mm, int e) { return mm.nbe[e] > 1 ? G.cIP : 0.0; }
 
-template <int MK, int NB, bool VISC, class T, class Own, class Nb, class Sink>
+template <int MK, int NB, bool VISC, bool AV, class T, class Own, class Nb, class Sink>
 __device__ void mixed_residual(const ModelParams& P, const DGParams& G, const MixedMesh& mm, int e, Own own, Nb nbw, Sink& sk) {
     constexpr int NE = neq_of<MK>();
     constexpr int FS = 3 + 6 * NB;
@@ -123,7 +146,7 @@ __device__ void mixed_residual(const ModelParams& P, const DGParams& G, const Mi
                 mixed_interp<NB, NE, T>(nbw, TT, phiR, phiR + NB, WR, dWR);
             }
             T g[NE];
-            mixed_face_flux<MK, NE, VISC>(P, G, cIP, TT, n, dL, fp + 1, WL, dWL, WR, dWR, g);
+            mixed_face_flux<MK, NE, VISC, AV>(P, G, mm, e, cIP, TT, n, dL, fp + 1, WL, dWL, WR, dWR, g);
             sk.face(k, p, phiL, phiR, fp[0], g);
         }
     }
@@ -134,16 +157,7 @@ __device__ void mixed_residual(const ModelParams& P, const DGParams& G, const Mi
             const double* vp = mm.vol + ((size_t)e * mm.nqvm + q) * (1 + 3 * NB);
             T Wq[NE], dWq[2 * NE], fx[NE], fy[NE];
             mixed_interp<NB, NE, T>(own, e, vp + 1, vp + 1 + NB, Wq, dWq);
-            const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
-            convective_flux<MK>(P, Wq, ex, fx);
-            convective_flux<MK>(P, Wq, ey, fy);
-            if constexpr (VISC) {
-                T gx[NE], gy[NE];
-                diffusive_flux<MK>(P, Wq, dWq, ex, gx);
-                diffusive_flux<MK>(P, Wq, dWq, ey, gy);
-#pragma unroll
-                for (int m = 0; m < NE; m++) { fx[m] -= gx[m]; fy[m] -= gy[m]; }
-            }
+            mixed_volume_flux<MK, NE, VISC, AV>(P, G, mm, e, Wq, dWq, fx, fy);
             sk.vol(vp[0], vp + 1 + NB, fx, fy);
         }
     }
@@ -182,7 +196,7 @@ struct MTangentSink {
     }
 };
 
-template <int MK, int NB, bool VISC>
+template <int MK, int NB, bool VISC, bool AV>
 __global__ void __launch_bounds__(128) k_residual_mixed(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
                                                         const double* __restrict__ Wo, const double* __restrict__ Wo2, double a1,
                                                         double a2, double a3, double* __restrict__ R, double* __restrict__ RHS,
@@ -193,7 +207,7 @@ __global__ void __launch_bounds__(128) k_residual_mixed(ModelParams P, DGParams 
     MValueSink<NB, NE> sk;
 #pragma unroll
     for (int i = 0; i < B; i++) sk.K[i] = 0.0;
-    mixed_residual<MK, NB, VISC, double>(P, G, mm, e, MPlain{W, B}, MPlain{W, B}, sk);
+    mixed_residual<MK, NB, VISC, AV, double>(P, G, mm, e, MPlain{W, B}, MPlain{W, B}, sk);
     const double* M = mm.mass + (size_t)e * NB * NB;
     double H[B];
 #pragma unroll
@@ -243,7 +257,7 @@ template <int B> constexpr int mixed_team_threads() { return B > 64 ? 128 : 64; 
 // or the volume fluxes there (into LDS), then lane = own DOF (m, j) sums its test-function
 // contributions in the same order as k_residual_mixed (faces, then volume), adds the BDF mass
 // terms, and lanes (which, m) form the equation norms.  Same arithmetic as k_residual_mixed.
-template <int MK, int NB, bool VISC>
+template <int MK, int NB, bool VISC, bool AV>
 __global__ void __launch_bounds__(128) k_residual_mixed_elem(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
                                                             const double* __restrict__ Wo, const double* __restrict__ Wo2, double a1,
                                                             double a2, double a3, double* __restrict__ R, double* __restrict__ RHS,
@@ -274,7 +288,7 @@ __global__ void __launch_bounds__(128) k_residual_mixed_elem(ModelParams P, DGPa
             double WL[NE], dWL[2 * NE], WR[NE], dWR[2 * NE], g[NE];
             mixed_interp<NB, NE, double>(own, e, fp + 3, fp + 3 + NB, WL, dWL);
             if (TT > -1) mixed_interp<NB, NE, double>(own, TT, fp + 3 + 3 * NB, fp + 3 + 4 * NB, WR, dWR);
-            mixed_face_flux<MK, NE, VISC>(P, G, cIP, TT, n, fg[3], fp + 1, WL, dWL, WR, dWR, g);
+            mixed_face_flux<MK, NE, VISC, AV>(P, G, mm, e, cIP, TT, n, fg[3], fp + 1, WL, dWL, WR, dWR, g);
 #pragma unroll
             for (int m = 0; m < NE; m++) GF[pt][m] = g[m];
         } else {
@@ -282,16 +296,7 @@ __global__ void __launch_bounds__(128) k_residual_mixed_elem(ModelParams P, DGPa
             const double* vp = mm.vol + ((size_t)e * mm.nqvm + q) * (1 + 3 * NB);
             double Wq[NE], dWq[2 * NE], fx[NE], fy[NE];
             mixed_interp<NB, NE, double>(own, e, vp + 1, vp + 1 + NB, Wq, dWq);
-            const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
-            convective_flux<MK>(P, Wq, ex, fx);
-            convective_flux<MK>(P, Wq, ey, fy);
-            if constexpr (VISC) {
-                double gx[NE], gy[NE];
-                diffusive_flux<MK>(P, Wq, dWq, ex, gx);
-                diffusive_flux<MK>(P, Wq, dWq, ey, gy);
-#pragma unroll
-                for (int m = 0; m < NE; m++) { fx[m] -= gx[m]; fy[m] -= gy[m]; }
-            }
+            mixed_volume_flux<MK, NE, VISC, AV>(P, G, mm, e, Wq, dWq, fx, fy);
 #pragma unroll
             for (int m = 0; m < NE; m++) {
                 GV[q][m] = fx[m];
@@ -382,7 +387,7 @@ struct MColumnSink : MTangentSink<NB, NE> {
 
 // Thread t takes column t % B of element t / B.  Assembly (mode 0) runs k_jacobian_mixed_elem;
 // this per-column form serves J^T v (mode 1).
-template <int MK, int NB, bool VISC, typename BT>
+template <int MK, int NB, bool VISC, bool AV, typename BT>
 __global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W, double a1,
                                                        BT* __restrict__ blocks, float* __restrict__ fblocks,
                                                        const double* __restrict__ v, double* __restrict__ out, int mode) {
@@ -424,7 +429,7 @@ __global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G
     sk.dir = dir;
     sk.mode = mode;
     sk.dot = 0.0;
-    mixed_residual<MK, NB, VISC, dual>(P, G, mm, e, MColumn{W, B, dir}, MConst{W, B}, sk);
+    mixed_residual<MK, NB, VISC, AV, dual>(P, G, mm, e, MColumn{W, B, dir}, MConst{W, B}, sk);
     const double* M = mm.mass + (size_t)e * NB * NB;
     const int md = dir / NB, jd = dir % NB;
     if (mode == 0) {
@@ -458,11 +463,12 @@ __global__ void __launch_bounds__(64) k_jacobian_mixed(ModelParams P, DGParams G
 // functions into the diagonal block and with the neighbour's into the neighbour blocks.  The
 // dual pass per (column, point) of k_jacobian_mixed becomes one per (input, point).
 
-template <int MK, int NB, bool VISC>
+template <int MK, int NB, bool VISC, bool AV>
 __global__ void __launch_bounds__(128) k_jacobian_mixed_elem(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
                                                             double a1, double* __restrict__ blocks, float* __restrict__ fblocks) {
     constexpr int NE = neq_of<MK>(), B = NB * NE;
-    constexpr int NIN = VISC ? 3 * NE : NE;
+    constexpr bool GRAD = VISC || AV;   // the diffusive and the damping terms see the gradients
+    constexpr int NIN = GRAD ? 3 * NE : NE;
     __shared__ double JF[MJ_MAXF][NIN][NE];
     __shared__ double JV[MJ_MAXV][NIN][2 * NE];
     const int e = blockIdx.x, lane = threadIdx.x;
@@ -501,7 +507,7 @@ __global__ void __launch_bounds__(128) k_jacobian_mixed_elem(ModelParams P, DGPa
 #pragma unroll
                 for (int m = 0; m < 2 * NE; m++) dWR[m] = dual(dwr[m], 0.0);
             }
-            mixed_face_flux<MK, NE, VISC>(P, G, cIP, TT, n, fg[3], fp + 1, WL, dWL, WR, dWR, g);
+            mixed_face_flux<MK, NE, VISC, AV>(P, G, mm, e, cIP, TT, n, fg[3], fp + 1, WL, dWL, WR, dWR, g);
 #pragma unroll
             for (int m = 0; m < NE; m++) JF[pt][d][m] = g[m].d;
         } else {
@@ -514,16 +520,7 @@ __global__ void __launch_bounds__(128) k_jacobian_mixed_elem(ModelParams P, DGPa
             for (int m = 0; m < NE; m++) Wq[m] = dual(wq[m], d == m ? 1.0 : 0.0);
 #pragma unroll
             for (int m = 0; m < 2 * NE; m++) dWq[m] = dual(dwq[m], d == NE + m ? 1.0 : 0.0);
-            const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
-            convective_flux<MK>(P, Wq, ex, fx);
-            convective_flux<MK>(P, Wq, ey, fy);
-            if constexpr (VISC) {
-                dual gx[NE], gy[NE];
-                diffusive_flux<MK>(P, Wq, dWq, ex, gx);
-                diffusive_flux<MK>(P, Wq, dWq, ey, gy);
-#pragma unroll
-                for (int m = 0; m < NE; m++) { fx[m] -= gx[m]; fy[m] -= gy[m]; }
-            }
+            mixed_volume_flux<MK, NE, VISC, AV>(P, G, mm, e, Wq, dWq, fx, fy);
 #pragma unroll
             for (int m = 0; m < NE; m++) {
                 JV[q][d][m] = fx[m].d;
@@ -572,7 +569,7 @@ __global__ void __launch_bounds__(128) k_jacobian_mixed_elem(ModelParams P, DGPa
 #pragma unroll
             for (int m = 0; m < NE; m++) {
                 T[m] = JF[pt][md][m] * tw;
-                if constexpr (VISC) T[m] += JF[pt][NE + md][m] * tx + JF[pt][2 * NE + md][m] * ty;
+                if constexpr (GRAD) T[m] += JF[pt][NE + md][m] * tx + JF[pt][2 * NE + md][m] * ty;
             }
 #pragma unroll
             for (int m = 0; m < NE; m++)
@@ -603,7 +600,7 @@ __global__ void __launch_bounds__(128) k_jacobian_mixed_elem(ModelParams P, DGPa
         for (int m = 0; m < NE; m++) {
             Tx[m] = JV[q][md][m] * tw;
             Ty[m] = JV[q][md][NE + m] * tw;
-            if constexpr (VISC) {
+            if constexpr (GRAD) {
                 Tx[m] += JV[q][NE + md][m] * tx + JV[q][2 * NE + md][m] * ty;
                 Ty[m] += JV[q][NE + md][NE + m] * tx + JV[q][2 * NE + md][NE + m] * ty;
             }
@@ -624,7 +621,7 @@ __global__ void __launch_bounds__(128) k_jacobian_mixed_elem(ModelParams P, DGPa
     }
 }
 
-template <int MK, int NB, bool VISC>
+template <int MK, int NB, bool VISC, bool AV>
 __global__ void __launch_bounds__(64) k_jvp_mixed(ModelParams P, DGParams G, MixedMesh mm, const double* __restrict__ W,
                                                   const double* __restrict__ v, double a1, double* __restrict__ out) {
     constexpr int NE = neq_of<MK>(), B = NB * NE;
@@ -633,7 +630,7 @@ __global__ void __launch_bounds__(64) k_jvp_mixed(ModelParams P, DGParams G, Mix
     MTangentSink<NB, NE> sk;
 #pragma unroll
     for (int i = 0; i < B; i++) sk.K[i] = 0.0;
-    mixed_residual<MK, NB, VISC, dual>(P, G, mm, e, MTangent{W, v, B}, MTangent{W, v, B}, sk);
+    mixed_residual<MK, NB, VISC, AV, dual>(P, G, mm, e, MTangent{W, v, B}, MTangent{W, v, B}, sk);
     const double* M = mm.mass + (size_t)e * NB * NB;
 #pragma unroll
     for (int m = 0; m < NE; m++)
@@ -662,6 +659,60 @@ __global__ void k_size_over_lambda_mixed(ModelParams P, MixedMesh mm, const doub
     }
     const double lam = max_eigenvalue<MK>(P, wa);
     out[e] = lam > 0 ? esize[e] / lam : __builtin_huge_val();
+}
+
+// DGpure.limiter (FlowPro/src/main/java/flowpro/core/element/DGpure.java:358-400) per element of a
+// mixed mesh, with the element's own order k (from its type and basis size): nothing at k = 1; eps
+// = lambda(mean W) elemSize / k g_shock with S0 = log10(1 / (k-1)^4); the inner damping with its
+// decaying memory.  trunc: [nElem][NB][NB], each element's TrunOrd (k > 2) padded with zeros; the
+// sensor integrals take the element's |det J| w and basis values (zero on the padded functions) per
+// point.  The Aerodynamics models give every equation the density sensor.
+template <int MK, int NB>
+__global__ void k_av_limiter_mixed(ModelParams P, MixedMesh mm, AvParams A, const double* __restrict__ W,
+                                   const double* __restrict__ esize, const double* __restrict__ trunc, double* __restrict__ av,
+                                   double* __restrict__ old) {
+    constexpr int NE = neq_of<MK>(), B = NB * NE, VS = 1 + 3 * NB;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= mm.nElem) return;
+    double* a = av + (size_t)e * (1 + NE);
+    a[0] = 0.0;
+    const int nbe = mm.nbe[e];
+    int k = 1;
+    if (mm.nv[e] == 4)
+        while (k * k < nbe) k++;
+    else
+        while (k * (k + 1) / 2 < nbe) k++;
+    if (k <= 1) return;
+    const double* is = mm.is + (size_t)e * NB;
+    double w[NB], wa[NE];
+#pragma unroll
+    for (int m = 0; m < NE; m++) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) s = s + W[(size_t)e * B + m * NB + j] * is[j];
+        wa[m] = s;
+    }
+    const double mean0 = wa[0];
+#pragma unroll
+    for (int j = 0; j < NB; j++) w[j] = W[(size_t)e * B + j];
+    const double lam = max_eigenvalue<MK>(P, wa);
+    const double scale = lam * esize[e] / k;
+    const double* vp = mm.vol + (size_t)e * mm.nqvm * VS;
+    auto jw = [&](int q) { return vp[(size_t)q * VS]; };
+    auto phi = [&](int q) { return vp + (size_t)q * VS + 1; };
+    const double* tr = trunc + (size_t)e * NB * NB;
+    const int nq = mm.nqv[e];
+    if (A.dampTol > 0) a[0] = scale * shock_sensor_at<NB>(phi, jw, nq, w, mean0, tr, k, A.dampTol);
+    if (A.dampInnerTol > 0) {
+        const double g = shock_sensor_at<NB>(phi, jw, nq, w, mean0, tr, k, A.dampInnerTol);
+#pragma unroll
+        for (int m = 0; m < NE; m++) {
+            double& o = old[(size_t)e * NE + m];
+            const double ind = fmax(g, o);
+            o = 0.1 * ind;
+            a[1 + m] = A.nCoef > 0 ? (A.nCoef == 1 ? A.coef[0] : A.coef[m]) * ind : scale * ind;
+        }
+    }
 }
 
 // Element.positivityLimiter (Element.java:385-476) over the element's volume and face points; only
@@ -756,18 +807,23 @@ __global__ void k_functional_mixed(MixedMesh mm, const double* __restrict__ W, d
 // ------------------------------------------------------------------ dispatch
 // (model, padded basis size, viscous): the padded size is the largest basis of the mesh rounded up
 // to 4, 9, 16 or 25 (element orders up to 5 on triangles and squares)
+// (model, padded basis size, viscous, artificial viscosity): damping up to the padded size 16
 #define FP_MIXED_CASES(X)                                                                                            \
-    X(MODEL_NAVIER_STOKES, 4, false) X(MODEL_NAVIER_STOKES, 9, false) X(MODEL_NAVIER_STOKES, 16, false)              \
-    X(MODEL_NAVIER_STOKES, 25, false)                                                                                \
-    X(MODEL_NAVIER_STOKES, 4, true) X(MODEL_NAVIER_STOKES, 9, true) X(MODEL_NAVIER_STOKES, 16, true)                 \
-    X(MODEL_NAVIER_STOKES, 25, true)                                                                                 \
-    X(MODEL_ADIABATIC, 4, false) X(MODEL_ADIABATIC, 9, false) X(MODEL_ADIABATIC, 16, false)                          \
-    X(MODEL_ADIABATIC, 25, false)
-#define FP_MMATCH(MK, NB, V) (L.model == MK && L.nb == NB && (L.visc != 0) == V && !L.av)
+    X(MODEL_NAVIER_STOKES, 4, false, false) X(MODEL_NAVIER_STOKES, 9, false, false)                                  \
+    X(MODEL_NAVIER_STOKES, 16, false, false) X(MODEL_NAVIER_STOKES, 25, false, false)                                \
+    X(MODEL_NAVIER_STOKES, 4, true, false) X(MODEL_NAVIER_STOKES, 9, true, false)                                    \
+    X(MODEL_NAVIER_STOKES, 16, true, false) X(MODEL_NAVIER_STOKES, 25, true, false)                                  \
+    X(MODEL_ADIABATIC, 4, false, false) X(MODEL_ADIABATIC, 9, false, false) X(MODEL_ADIABATIC, 16, false, false)     \
+    X(MODEL_ADIABATIC, 25, false, false)                                                                             \
+    X(MODEL_NAVIER_STOKES, 4, false, true) X(MODEL_NAVIER_STOKES, 9, false, true)                                    \
+    X(MODEL_NAVIER_STOKES, 16, false, true) X(MODEL_NAVIER_STOKES, 4, true, true)                                    \
+    X(MODEL_NAVIER_STOKES, 9, true, true) X(MODEL_NAVIER_STOKES, 16, true, true)                                     \
+    X(MODEL_ADIABATIC, 4, false, true) X(MODEL_ADIABATIC, 9, false, true) X(MODEL_ADIABATIC, 16, false, true)
+#define FP_MMATCH(MK, NB, V, A) (L.model == MK && L.nb == NB && (L.visc != 0) == V && (L.av != 0) == A)
 
 bool mixed_supported(const ElemLaunch& L) {
-#define X(MK, NB, V) \
-    if (FP_MMATCH(MK, NB, V)) return true;
+#define X(MK, NB, V, A) \
+    if (FP_MMATCH(MK, NB, V, A)) return true;
     FP_MIXED_CASES(X)
 #undef X
     return false;
@@ -778,15 +834,15 @@ hipError_t launch_residual_mixed(const ElemLaunch& L, hipStream_t s, const Model
                                  double* RHS, double* enorm) {
     dim3 blk(128), grd((mm.nElem + 127) / 128);
     const bool elem = mm.nqfm * 4 <= MJ_MAXF && mm.nqvm <= MJ_MAXV && mm.nElem > 0;
-#define X(MK, NB, V)                                                                                                      \
-    if (FP_MMATCH(MK, NB, V) && elem) {                                                                                   \
-        hipLaunchKernelGGL((k_residual_mixed_elem<MK, NB, V>), dim3(mm.nElem), dim3(mixed_team_threads<NB * neq_of<MK>()>()), 0, s, \
+#define X(MK, NB, V, A)                                                                                                      \
+    if (FP_MMATCH(MK, NB, V, A) && elem) {                                                                                   \
+        hipLaunchKernelGGL((k_residual_mixed_elem<MK, NB, V, A>), dim3(mm.nElem), dim3(mixed_team_threads<NB * neq_of<MK>()>()), 0, s, \
                            P, G, mm, W, Wo, Wo2, a1, a2, \
                            a3, R, RHS, enorm);                                                                            \
         return hipGetLastError();                                                                                         \
     }                                                                                                                     \
-    if (FP_MMATCH(MK, NB, V)) {                                                                                           \
-        hipLaunchKernelGGL((k_residual_mixed<MK, NB, V>), grd, blk, 0, s, P, G, mm, W, Wo, Wo2, a1, a2, a3, R, RHS, enorm); \
+    if (FP_MMATCH(MK, NB, V, A)) {                                                                                           \
+        hipLaunchKernelGGL((k_residual_mixed<MK, NB, V, A>), grd, blk, 0, s, P, G, mm, W, Wo, Wo2, a1, a2, a3, R, RHS, enorm); \
         return hipGetLastError();                                                                                         \
     }
     FP_MIXED_CASES(X)
@@ -802,14 +858,14 @@ hipError_t launch_jacobian_mixed(const ElemLaunch& L, hipStream_t s, const Model
     if (mode == 0 && (mm.nqfm * 4 > MJ_MAXF || mm.nqvm > MJ_MAXV)) return hipErrorInvalidValue;   // orders <= 5 only
     const long long threads = (long long)mm.nElem * L.nb * L.neq;
     dim3 blk(64), grd((unsigned)((threads + 63) / 64));
-#define X(MK, NB, V)                                                                                                   \
-    if (FP_MMATCH(MK, NB, V)) {                                                                                        \
+#define X(MK, NB, V, A)                                                                                                   \
+    if (FP_MMATCH(MK, NB, V, A)) {                                                                                        \
         if (mode == 0)                                                                                                 \
-            hipLaunchKernelGGL((k_jacobian_mixed_elem<MK, NB, V>), dim3(mm.nElem), dim3(mixed_team_threads<NB * neq_of<MK>()>()), \
+            hipLaunchKernelGGL((k_jacobian_mixed_elem<MK, NB, V, A>), dim3(mm.nElem), dim3(mixed_team_threads<NB * neq_of<MK>()>()), \
                                0, s, P, G, mm, W, a1,                                                                  \
                                blocks, fblocks);                                                                       \
         else                                                                                                           \
-            hipLaunchKernelGGL((k_jacobian_mixed<MK, NB, V, double>), grd, blk, 0, s, P, G, mm, W, a1, blocks, fblocks, \
+            hipLaunchKernelGGL((k_jacobian_mixed<MK, NB, V, A, double>), grd, blk, 0, s, P, G, mm, W, a1, blocks, fblocks, \
                                v, out, mode);                                                                          \
         return hipGetLastError();                                                                                      \
     }
@@ -821,9 +877,9 @@ hipError_t launch_jacobian_mixed(const ElemLaunch& L, hipStream_t s, const Model
 hipError_t launch_jvp_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
                             const double* W, const double* v, double a1, double* out) {
     dim3 blk(64), grd((mm.nElem + 63) / 64);
-#define X(MK, NB, V)                                                                              \
-    if (FP_MMATCH(MK, NB, V)) {                                                                   \
-        hipLaunchKernelGGL((k_jvp_mixed<MK, NB, V>), grd, blk, 0, s, P, G, mm, W, v, a1, out);   \
+#define X(MK, NB, V, A)                                                                              \
+    if (FP_MMATCH(MK, NB, V, A)) {                                                                   \
+        hipLaunchKernelGGL((k_jvp_mixed<MK, NB, V, A>), grd, blk, 0, s, P, G, mm, W, v, a1, out);   \
         return hipGetLastError();                                                                 \
     }
     FP_MIXED_CASES(X)
@@ -834,8 +890,8 @@ hipError_t launch_jvp_mixed(const ElemLaunch& L, hipStream_t s, const ModelParam
 hipError_t launch_size_over_lambda_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const MixedMesh& mm, const double* W,
                                          const double* esize, double* out) {
     dim3 blk(256), grd((mm.nElem + 255) / 256);
-#define X(MK, NB, V)                                                                                   \
-    if (FP_MMATCH(MK, NB, V)) {                                                                        \
+#define X(MK, NB, V, A)                                                                                   \
+    if (FP_MMATCH(MK, NB, V, A)) {                                                                        \
         hipLaunchKernelGGL((k_size_over_lambda_mixed<MK, NB>), grd, blk, 0, s, P, mm, W, esize, out);  \
         return hipGetLastError();                                                                      \
     }
@@ -844,10 +900,23 @@ hipError_t launch_size_over_lambda_mixed(const ElemLaunch& L, hipStream_t s, con
     return hipErrorInvalidValue;
 }
 
+hipError_t launch_av_limiter_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const MixedMesh& mm, const AvParams& Ap,
+                                   const double* W, const double* esize, const double* trunc, double* av, double* old) {
+    dim3 blk(128), grd((mm.nElem + 127) / 128);
+#define X(MK, NB, V, A)                                                                                          \
+    if (A && FP_MMATCH(MK, NB, V, A)) {                                                                          \
+        hipLaunchKernelGGL((k_av_limiter_mixed<MK, NB>), grd, blk, 0, s, P, mm, Ap, W, esize, trunc, av, old);   \
+        return hipGetLastError();                                                                                \
+    }
+    FP_MIXED_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_positivity_mixed(const ElemLaunch& L, hipStream_t s, const MixedMesh& mm, double* W, int* count) {
     dim3 blk(128), grd((mm.nElem + 127) / 128);
-#define X(MK, NB, V)                                                                      \
-    if (FP_MMATCH(MK, NB, V)) {                                                           \
+#define X(MK, NB, V, A)                                                                      \
+    if (FP_MMATCH(MK, NB, V, A)) {                                                           \
         hipLaunchKernelGGL((k_positivity_mixed<MK, NB>), grd, blk, 0, s, mm, W, count);   \
         return hipGetLastError();                                                         \
     }

@@ -409,10 +409,11 @@ __global__ void k_av_limiter_quad(ModelParams P, QuadTables tb, QuadMesh mq, AvP
     const double scale = lam * esize[e] / A.order;
     const double* qv = mq.qv + (size_t)e * NQV * (1 + 2 * NB);
     auto jw = [&](int q) { return qv[(size_t)q * (1 + 2 * NB)]; };
+    auto phi = [&](int q) { return tb.vb + q * NB; };
     const double* tr = trunc ? trunc + (size_t)e * NB * NB : nullptr;
-    if (A.dampTol > 0) a[0] = scale * shock_sensor_at<NB>(tb.vb, jw, NQV, w, mean0, tr, A.order, A.dampTol);
+    if (A.dampTol > 0) a[0] = scale * shock_sensor_at<NB>(phi, jw, NQV, w, mean0, tr, A.order, A.dampTol);
     if (A.dampInnerTol > 0) {
-        const double g = shock_sensor_at<NB>(tb.vb, jw, NQV, w, mean0, tr, A.order, A.dampInnerTol);
+        const double g = shock_sensor_at<NB>(phi, jw, NQV, w, mean0, tr, A.order, A.dampInnerTol);
 #pragma unroll
         for (int m = 0; m < NE; m++) {
             double& o = old[(size_t)e * NE + m];

@@ -1,8 +1,9 @@
 """Parity of the artificial viscosity on the device (fp_limiter, the damping terms in the
 residual, the exact Jacobian and the matrix-free products) with the oracle.
 
-Triangles and (non-affine, jittered) quadrilaterals; on quadrilaterals the truncation matrix is
-the element's own and the sensor integrals take |det J| w per point.
+Triangles, (non-affine, jittered) quadrilaterals and meshes mixing the two with per-element orders;
+on quadrilaterals and mixed meshes the truncation matrix is the element's own and the sensor
+integrals take |det J| w per point.
 
 Tolerances:
   eps, dampInner after one and after two limiter calls   1e-12 (relative to the largest value)
@@ -19,6 +20,7 @@ import pytest
 from flowpro_v2_amd import cases, solver
 import oracle as O
 from helpers import block_rel_err, eq_rel_err
+from test_gpu_mixed import _eq_err, _mixed, _state as _mixed_state, _tri_orders
 
 pytestmark = pytest.mark.gpu
 
@@ -134,4 +136,88 @@ def test_damped_newton_history(quads):
     assert len(hist) == len(hist_ref) == n
     assert np.abs(hist / hist_ref - 1).max() < 1e-8
     assert block_rel_err(dev.get_state(), o.get_w()) < 1e-8
+    dev.close()
+
+
+# meshes of squares and triangles with per-element orders (padded to 4, 9 or 16 functions): each
+# element takes its own order in the sensor (k = 1: no damping; S0 and lambda h / k of its order)
+# and its own truncation matrix
+MIXED_AV_CONFIGS = [
+    ("euler_mixed_1234", lambda: _mixed(6, 3, orders=(1, 2, 3, 4), seed=11),
+     lambda: cases.euler_props(order=4, dampTol=2.5, dampConst=2e-3, dampInnerTol=2.0)),
+    ("ns_mixed_234_coef", lambda: _mixed(6, 3, orders=(2, 3, 4), seed=12),
+     lambda: cases.ns_props(order=4, reynolds=300.0, dampTol=1.0, dampConst=1e-3, dampInnerTol=1.5, dampInnerCoef=[0.01])),
+    ("ae_mixed_23", lambda: _mixed(6, 3, orders=(2, 3), seed=13),
+     lambda: cases.euler_props(model="AdiabaticEulers", order=3, dampTol=2.0, dampInnerTol=2.0)),
+    ("euler_tri_orders_345", lambda: _tri_orders(6, 3, (3, 4, 5), 14),
+     lambda: cases.euler_props(order=5, dampTol=2.5, dampInnerTol=2.0)),
+]
+
+
+def _mixed_av_setup(case, props, seed=3):
+    props = dict(props, jacobian="exact")
+    o = O.Oracle(case, props)
+    dev = solver.DeviceCase(case, props)
+    assert dev.mixed and dev.dofs == o.dofs
+    W = _mixed_state(o, seed, rel=0.1)
+    o.set_w(W)
+    dev.set_state(W)
+    o.limiter()
+    dev.limiter(True)
+    return o, dev, W
+
+
+@pytest.mark.parametrize("name,mk_case,mk_props", MIXED_AV_CONFIGS, ids=[c[0] for c in MIXED_AV_CONFIGS])
+def test_mixed_limiter_residual_jacobian_products(name, mk_case, mk_props):
+    case, props = mk_case(), mk_props()
+    print(f"[av] mixed {name}", flush=True)
+    o, dev, W = _mixed_av_setup(case, props)
+    eps, di = dev.artificial_viscosity()
+    eps_ref, di_ref = o.artificial_viscosity()
+    assert (eps_ref > 0).any()
+    assert _rel(eps, eps_ref) < 1e-12
+    assert np.abs(di - di_ref).max() <= 1e-12 * max(np.abs(di_ref).max(), 1e-300)
+    o.assemble(0.05, -1.0)
+    dev.assemble(0.05, -1.0, 1, True)
+    R, _ = dev.residual()
+    assert _eq_err(R, o.residual(), o.off, o.sizes, o.neq) < 1e-12
+    d, n = dev.jacobian()
+    dref, nref = o.blocks_flat()
+    scale = max(np.abs(x).max() for x in dref)
+    assert max(np.abs(a - b).max() for a, b in zip(d, dref)) / scale < 1e-10
+    assert set(n) == set(nref)
+    assert max(np.abs(n[k] - nref[k]).max() for k in nref) / scale < 1e-10
+    v = np.random.default_rng(7).standard_normal(o.dofs)
+    for got, ref in ((dev.jv(20.0, v), o.jvp(20.0, v)), (dev.jtv(20.0, v), o.vjp(20.0, v))):
+        assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-10
+    # the second limiter call sees the memory of the first
+    W2 = W * (1 + 0.002 * np.random.default_rng(8).standard_normal(W.shape))
+    o.set_w(W2)
+    dev.set_state(W2)
+    o.limiter()
+    dev.limiter(False)
+    eps, di = dev.artificial_viscosity()
+    eps_ref, di_ref = o.artificial_viscosity()
+    assert np.abs(eps - eps_ref).max() <= 1e-12 * max(np.abs(eps_ref).max(), 1e-300)
+    assert np.abs(di - di_ref).max() <= 1e-12 * max(np.abs(di_ref).max(), 1e-300)
+    dev.close()
+
+
+def test_mixed_damped_newton_history():
+    print("[av] mixed Newton history", flush=True)
+    case = _mixed(8, 4, orders=(2, 3, 4), seed=6)
+    props = cases.euler_props(order=4, CFL=5.0, dampTol=3.0, dampInnerTol=2.0, jacobian="exact",
+                              iterativeSolverTol=1e-10, newtonIters=1)
+    o = O.Oracle(case, props)
+    W0 = _mixed_state(o, 21, rel=0.1)
+    o.set_w(W0)
+    n = 5
+    hist_ref = o.run(n)
+    dev = solver.DeviceCase(case, props)
+    dev.set_state(W0)
+    hist = np.array([h.residuum for h in solver.GpuImplicitSolver(dev, props).solve(n)])
+    assert len(hist) == len(hist_ref) == n
+    assert np.abs(hist / hist_ref - 1).max() < 1e-8
+    Wg, Wr = dev.get_state(), o.get_w()
+    assert np.abs(Wg - Wr).max() / np.abs(Wr).max() < 1e-8
     dev.close()

@@ -216,9 +216,11 @@ def test_mixed_functional_and_gradient(which):
 
 
 def test_mixed_unsupported_features():
-    case = _mixed(6, 2)
+    """Damping on mixed meshes has kernels up to 16 padded functions: a square of order 5 (25) has none."""
+    case = _mixed(6, 2, orders=(4, 5), seed=3)
+    assert ((case.types == 4) & (case.orders == 5)).any()
     with pytest.raises(solver.UnsupportedCase):
-        solver.DeviceCase(case, cases.euler_props(order=4, dampTol=1.0))
+        solver.DeviceCase(case, cases.euler_props(order=5, dampTol=1.0))
 
 
 def test_mixed_steady_adjoint_matches_dense():

@@ -20,6 +20,9 @@ On jittered (non-affine) quadrilaterals:
   at order 2 (Q1, truncation to the mean) the density sensor of a bilinear density follows from
   integrals over the bilinear map with numpy's own Gauss-Legendre rule; eps = lambda(mean state)
   * (4 area / perimeter) / 2 * g (1e-12)
+On meshes of squares and triangles with per-element orders: every element's eps and dampInner
+equal those it gets in the mesh where all elements have its order (the limiter is element-local
+and takes the element's own order) (1e-13)
 """
 import math
 
@@ -241,3 +244,35 @@ def test_quad_density_sensor_at_order_two(kap, amp):
         assert (eps > 0).mean() > 0.5
     else:
         assert (eps == 0).all()
+
+
+def test_mixed_mesh_sensor_takes_each_elements_own_order():
+    """The limiter is element-local: on a mesh of squares and triangles with per-element orders 2-4,
+    every element's eps and dampInner equal those of the same element, with the same coefficients,
+    in the mesh where every element has that element's order.  A sensor that took the mesh's largest
+    order (S0, lambda h / k) or another element's basis fails this."""
+    base = cases.mixed_channel(6, 3, jitter=0.2, seed=11)
+    orders = cases.seeded_orders(base.n_elem, (2, 3, 4), 11)
+    props = cases.euler_props(order=4, dampTol=2.5, dampInnerTol=2.0)
+    mixed = cases.Case("mixed", base.PXY, base.TP, base.bnd, types=base.types, orders=orders)
+    o = O.Oracle(mixed, props)
+    w0 = np.asarray(o.get_w(), np.float64)
+    rng = np.random.default_rng(4)
+    W = w0 * (1 + 0.1 * rng.standard_normal(w0.shape))
+    o.set_w(W)
+    o.limiter()
+    eps, di = o.artificial_viscosity()
+    assert (eps > 0).any() and (di > 0).any()
+    for k in (2, 3, 4):
+        uni = cases.Case("uniform", base.PXY, base.TP, base.bnd, types=base.types,
+                         orders=np.full(base.n_elem, k, dtype=orders.dtype))
+        ou = O.Oracle(uni, dict(props, order=k))
+        Wu = np.asarray(ou.get_w(), np.float64).copy()
+        sel = np.nonzero(orders == k)[0]
+        for e in sel:
+            Wu[ou.off[e]:ou.off[e + 1]] = W[o.off[e]:o.off[e + 1]]
+        ou.set_w(Wu)
+        ou.limiter()
+        eu, du = ou.artificial_viscosity()
+        np.testing.assert_allclose(eps[sel], eu[sel], rtol=1e-13, atol=0)
+        np.testing.assert_allclose(di[sel], du[sel], rtol=1e-13, atol=0)
