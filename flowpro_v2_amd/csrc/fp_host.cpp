@@ -1882,8 +1882,7 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
     }
     const bool av = cs->damp_tol > 0 || cs->damp_const != 0 || cs->damp_inner_tol > 0;
     if (av && (c->dim == 3 || cs->model == MODEL_SA || cs->model == MODEL_KOMEGA)) {
-        c->err = "artificial viscosity: device kernels for the NavierStokes, AdiabaticEulers and scalar models in 2D (up to "
-                 "16 basis functions on quadrilaterals and mixed meshes)";
+        c->err = "artificial viscosity: device kernels for the NavierStokes, AdiabaticEulers and scalar models in 2D";
         return FP_UNSUPPORTED;
     }
     if (cs->n_damp_inner_coef < 0 || cs->n_damp_inner_coef > 5 ||

@@ -807,7 +807,7 @@ __global__ void k_functional_mixed(MixedMesh mm, const double* __restrict__ W, d
 // ------------------------------------------------------------------ dispatch
 // (model, padded basis size, viscous): the padded size is the largest basis of the mesh rounded up
 // to 4, 9, 16 or 25 (element orders up to 5 on triangles and squares)
-// (model, padded basis size, viscous, artificial viscosity): damping up to the padded size 16
+// (model, padded basis size, viscous, artificial viscosity)
 #define FP_MIXED_CASES(X)                                                                                            \
     X(MODEL_NAVIER_STOKES, 4, false, false) X(MODEL_NAVIER_STOKES, 9, false, false)                                  \
     X(MODEL_NAVIER_STOKES, 16, false, false) X(MODEL_NAVIER_STOKES, 25, false, false)                                \
@@ -818,7 +818,9 @@ __global__ void k_functional_mixed(MixedMesh mm, const double* __restrict__ W, d
     X(MODEL_NAVIER_STOKES, 4, false, true) X(MODEL_NAVIER_STOKES, 9, false, true)                                    \
     X(MODEL_NAVIER_STOKES, 16, false, true) X(MODEL_NAVIER_STOKES, 4, true, true)                                    \
     X(MODEL_NAVIER_STOKES, 9, true, true) X(MODEL_NAVIER_STOKES, 16, true, true)                                     \
-    X(MODEL_ADIABATIC, 4, false, true) X(MODEL_ADIABATIC, 9, false, true) X(MODEL_ADIABATIC, 16, false, true)
+    X(MODEL_ADIABATIC, 4, false, true) X(MODEL_ADIABATIC, 9, false, true) X(MODEL_ADIABATIC, 16, false, true)       \
+    X(MODEL_NAVIER_STOKES, 25, false, true) X(MODEL_NAVIER_STOKES, 25, true, true)                                   \
+    X(MODEL_ADIABATIC, 25, false, true)
 #define FP_MMATCH(MK, NB, V, A) (L.model == MK && L.nb == NB && (L.visc != 0) == V && (L.av != 0) == A)
 
 bool mixed_supported(const ElemLaunch& L) {

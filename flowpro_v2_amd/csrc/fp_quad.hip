@@ -510,7 +510,7 @@ __global__ void k_functional_quad(QuadTables tb, QuadMesh mq, const double* __re
     }
 }
 
-// (model, NB, NQV, NQF, viscous, artificial viscosity): the damping terms at orders 2-4
+// (model, NB, NQV, NQF, viscous, artificial viscosity)
 #define FP_QUAD_CASES(X)                               \
     X(MODEL_NAVIER_STOKES, 1, 1, 1, false, false)      \
     X(MODEL_NAVIER_STOKES, 4, 4, 2, false, false)      \
@@ -535,7 +535,13 @@ __global__ void k_functional_quad(QuadTables tb, QuadMesh mq, const double* __re
     X(MODEL_NAVIER_STOKES, 16, 16, 4, true, true)      \
     X(MODEL_ADIABATIC, 4, 4, 2, false, true)           \
     X(MODEL_ADIABATIC, 9, 9, 3, false, true)           \
-    X(MODEL_ADIABATIC, 16, 16, 4, false, true)
+    X(MODEL_ADIABATIC, 16, 16, 4, false, true)         \
+    X(MODEL_NAVIER_STOKES, 1, 1, 1, false, true)       \
+    X(MODEL_NAVIER_STOKES, 1, 1, 1, true, true)        \
+    X(MODEL_ADIABATIC, 1, 1, 1, false, true)           \
+    X(MODEL_NAVIER_STOKES, 25, 25, 5, false, true)     \
+    X(MODEL_NAVIER_STOKES, 25, 25, 5, true, true)      \
+    X(MODEL_ADIABATIC, 25, 25, 5, false, true)
 #define FP_QMATCH(MK, NB, NQV, NQF, V, A) \
     (L.model == MK && L.nb == NB && L.nqv == NQV && L.nqf == NQF && (L.visc != 0) == V && (L.av != 0) == A)
 

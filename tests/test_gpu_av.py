@@ -40,6 +40,8 @@ AV_CONFIGS = [
                             dampInnerCoef=[0.01])),
     ("ae_q3_inner", lambda: cases.channel_bump(10, 4, quads=True, jitter=0.2),
      lambda: cases.euler_props(model="AdiabaticEulers", order=3, dampTol=2.0, dampInnerTol=2.0)),
+    ("euler_q5_inner", lambda: cases.channel_bump(6, 3, quads=True, jitter=0.2),
+     lambda: cases.euler_props(order=5, dampTol=2.5, dampConst=1e-3, dampInnerTol=2.0)),
 ]
 
 
@@ -151,6 +153,8 @@ MIXED_AV_CONFIGS = [
      lambda: cases.euler_props(model="AdiabaticEulers", order=3, dampTol=2.0, dampInnerTol=2.0)),
     ("euler_tri_orders_345", lambda: _tri_orders(6, 3, (3, 4, 5), 14),
      lambda: cases.euler_props(order=5, dampTol=2.5, dampInnerTol=2.0)),
+    ("ns_mixed_45", lambda: _mixed(5, 3, orders=(4, 5), seed=12),
+     lambda: cases.ns_props(order=5, reynolds=300.0, dampTol=1.5, dampConst=1e-3, dampInnerTol=2.0)),
 ]
 
 
@@ -220,4 +224,34 @@ def test_mixed_damped_newton_history():
     assert np.abs(hist / hist_ref - 1).max() < 1e-8
     Wg, Wr = dev.get_state(), o.get_w()
     assert np.abs(Wg - Wr).max() / np.abs(Wr).max() < 1e-8
+    dev.close()
+
+
+@pytest.mark.parametrize("viscous", [False, True], ids=["euler", "ns"])
+def test_damp_const_on_order_one_quads(viscous):
+    """Order 1 (one function per element): the limiter sets no eps (order <= 1), and the constant
+    dampConst enters the face terms only, through the jump term beta0 [W] / dL (the gradients of a
+    constant vanish); a diffusive model at global order 1 has beta0 = 0 (DGpure.java:287-289), so
+    there the damping leaves the residual unchanged."""
+    case = cases.channel_bump(12, 4, quads=True, jitter=0.2)
+    props = (cases.ns_props(order=1, reynolds=300.0, dampConst=0.05) if viscous
+             else cases.euler_props(order=1, dampConst=0.05))
+    print(f"[av] order-1 quads viscous={viscous}", flush=True)
+    o, dev, W = _setup(case, props)
+    eps, di = dev.artificial_viscosity()
+    assert np.all(eps == 0) and np.all(di == 0)
+    o.assemble(0.05, -1.0)
+    dev.assemble(0.05, -1.0, 1, True)
+    R, _ = dev.residual()
+    assert eq_rel_err(R, o.residual(), dev.nb, dev.neq) < 1e-12
+    d, n = dev.jacobian()
+    dref, nref = o.blocks()
+    assert block_rel_err(d, dref) < 1e-10
+    assert block_rel_err(n, nref) < 1e-10
+    o0 = O.Oracle(case, dict(props, dampConst=0.0, jacobian="exact"))
+    o0.set_w(W)
+    o0.limiter()
+    o0.assemble(0.05, -1.0)
+    change = np.abs(o0.residual() - o.residual()).max() / np.abs(o.residual()).max()
+    assert (change == 0.0) if viscous else (change > 1e-3)
     dev.close()

@@ -216,11 +216,11 @@ def test_mixed_functional_and_gradient(which):
 
 
 def test_mixed_unsupported_features():
-    """Damping on mixed meshes has kernels up to 16 padded functions: a square of order 5 (25) has none."""
-    case = _mixed(6, 2, orders=(4, 5), seed=3)
-    assert ((case.types == 4) & (case.orders == 5)).any()
+    """Mixed meshes have kernels for the NavierStokes and AdiabaticEulers models: the scalar test
+    equation is refused."""
+    case = _mixed(6, 2)
     with pytest.raises(solver.UnsupportedCase):
-        solver.DeviceCase(case, cases.euler_props(order=5, dampTol=1.0))
+        solver.DeviceCase(case, dict(model="TestScalarEquation", a=(1.0, 0.5), k=0.0, order=4, dt=1e8))
 
 
 def test_mixed_steady_adjoint_matches_dense():

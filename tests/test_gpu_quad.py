@@ -182,8 +182,8 @@ def test_quad_functional_and_gradient(which):
 
 
 def test_quad_unsupported_features():
-    """Damping on quadrilaterals has kernels at orders 2-4 only."""
+    """Quadrilaterals have kernels for the NavierStokes and AdiabaticEulers models: the scalar test
+    equation is refused."""
     case = cases.channel_bump(6, 2, quads=True)
-    for order in (1, 5):
-        with pytest.raises(solver.UnsupportedCase):
-            solver.DeviceCase(case, cases.euler_props(order=order, dampTol=1.0))
+    with pytest.raises(solver.UnsupportedCase):
+        solver.DeviceCase(case, dict(model="TestScalarEquation", a=(1.0, 0.5), k=0.0, order=2, dt=1e8))
