@@ -9,8 +9,9 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libflowpro_hip.so")
-SOURCES = ["fp_element.hip", "fp_team.hip", "fp_team_kw.hip", "fp_tet.hip", "fp_quad.hip", "fp_mixed.hip", "fp_linalg.hip",
-           "fp_host.cpp"]
+# fp_quad_av.hip and fp_mixed_av.hip compile the damped halves of fp_quad.hip and fp_mixed.hip
+SOURCES = ["fp_quad.hip", "fp_mixed.hip", "fp_quad_av.hip", "fp_mixed_av.hip", "fp_linalg.hip", "fp_element.hip", "fp_team.hip",
+           "fp_team_kw.hip", "fp_tet.hip", "fp_host.cpp"]
 # per-source optimisation level (default -O3): the k-omega SST team kernels are built at -O1, where they
 # agree with the CPU restatement (at -O2 and -O3 their Jacobian rows did not; see fp_team_kw.hip)
 OPT_LEVEL = {"fp_team_kw.hip": "-O1"}

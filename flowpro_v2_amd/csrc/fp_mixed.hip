@@ -807,23 +807,47 @@ __global__ void k_functional_mixed(MixedMesh mm, const double* __restrict__ W, d
 // ------------------------------------------------------------------ dispatch
 // (model, padded basis size, viscous): the padded size is the largest basis of the mesh rounded up
 // to 4, 9, 16 or 25 (element orders up to 5 on triangles and squares)
-// (model, padded basis size, viscous, artificial viscosity)
-#define FP_MIXED_CASES(X)                                                                                            \
-    X(MODEL_NAVIER_STOKES, 4, false, false) X(MODEL_NAVIER_STOKES, 9, false, false)                                  \
-    X(MODEL_NAVIER_STOKES, 16, false, false) X(MODEL_NAVIER_STOKES, 25, false, false)                                \
-    X(MODEL_NAVIER_STOKES, 4, true, false) X(MODEL_NAVIER_STOKES, 9, true, false)                                    \
-    X(MODEL_NAVIER_STOKES, 16, true, false) X(MODEL_NAVIER_STOKES, 25, true, false)                                  \
-    X(MODEL_ADIABATIC, 4, false, false) X(MODEL_ADIABATIC, 9, false, false) X(MODEL_ADIABATIC, 16, false, false)     \
-    X(MODEL_ADIABATIC, 25, false, false)                                                                             \
-    X(MODEL_NAVIER_STOKES, 4, false, true) X(MODEL_NAVIER_STOKES, 9, false, true)                                    \
-    X(MODEL_NAVIER_STOKES, 16, false, true) X(MODEL_NAVIER_STOKES, 4, true, true)                                    \
-    X(MODEL_NAVIER_STOKES, 9, true, true) X(MODEL_NAVIER_STOKES, 16, true, true)                                     \
-    X(MODEL_ADIABATIC, 4, false, true) X(MODEL_ADIABATIC, 9, false, true) X(MODEL_ADIABATIC, 16, false, true)       \
-    X(MODEL_NAVIER_STOKES, 25, false, true) X(MODEL_NAVIER_STOKES, 25, true, true)                                   \
-    X(MODEL_ADIABATIC, 25, false, true)
+// The file is compiled twice so that its halves build in parallel: as itself (FP_MIXED_PART 0: the
+// kernels without artificial viscosity, and the entry points the host calls, which hand the damped
+// cases on) and through fp_mixed_av.hip (FP_MIXED_PART 1: the kernels with it, entry points named _av).
+#ifndef FP_MIXED_PART
+#define FP_MIXED_PART 0
+#endif
+#define FP_MIXED_SHAPES(X, A)                                                                                        \
+    X(MODEL_NAVIER_STOKES, 4, false, A) X(MODEL_NAVIER_STOKES, 9, false, A) X(MODEL_NAVIER_STOKES, 16, false, A)     \
+    X(MODEL_NAVIER_STOKES, 25, false, A)                                                                             \
+    X(MODEL_NAVIER_STOKES, 4, true, A) X(MODEL_NAVIER_STOKES, 9, true, A) X(MODEL_NAVIER_STOKES, 16, true, A)        \
+    X(MODEL_NAVIER_STOKES, 25, true, A)                                                                              \
+    X(MODEL_ADIABATIC, 4, false, A) X(MODEL_ADIABATIC, 9, false, A) X(MODEL_ADIABATIC, 16, false, A)                 \
+    X(MODEL_ADIABATIC, 25, false, A)
+#if FP_MIXED_PART == 0
+#define FP_MIXED_CASES(X) FP_MIXED_SHAPES(X, false)
+#define FP_MFN(name) name
+#define FP_MAV(call) \
+    if (L.av) return call;
+bool mixed_supported_av(const ElemLaunch& L);
+hipError_t launch_residual_mixed_av(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
+                                    const double* W, const double* Wo, const double* Wo2, double a1, double a2, double a3,
+                                    double* R, double* RHS, double* enorm);
+hipError_t launch_jacobian_mixed_av(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
+                                    const double* W, double a1, double* blocks, float* fblocks, const double* v, double* out,
+                                    int mode);
+hipError_t launch_jvp_mixed_av(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
+                               const double* W, const double* v, double a1, double* out);
+hipError_t launch_av_limiter_mixed_av(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const MixedMesh& mm,
+                                      const AvParams& Ap, const double* W, const double* esize, const double* trunc, double* av,
+                                      double* old);
+#else
+#define FP_MIXED_CASES(X) FP_MIXED_SHAPES(X, true)
+#define FP_MFN(name) name##_av
+#define FP_MAV(call)
+#endif
 #define FP_MMATCH(MK, NB, V, A) (L.model == MK && L.nb == NB && (L.visc != 0) == V && (L.av != 0) == A)
+// the kernels that do not depend on the damping: every shape, damped or not
+#define FP_MSHAPE(MK, NB, V) (L.model == MK && L.nb == NB && (L.visc != 0) == V)
 
-bool mixed_supported(const ElemLaunch& L) {
+bool FP_MFN(mixed_supported)(const ElemLaunch& L) {
+    FP_MAV(mixed_supported_av(L))
 #define X(MK, NB, V, A) \
     if (FP_MMATCH(MK, NB, V, A)) return true;
     FP_MIXED_CASES(X)
@@ -831,9 +855,10 @@ bool mixed_supported(const ElemLaunch& L) {
     return false;
 }
 
-hipError_t launch_residual_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
-                                 const double* W, const double* Wo, const double* Wo2, double a1, double a2, double a3, double* R,
-                                 double* RHS, double* enorm) {
+hipError_t FP_MFN(launch_residual_mixed)(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G,
+                                         const MixedMesh& mm, const double* W, const double* Wo, const double* Wo2, double a1,
+                                         double a2, double a3, double* R, double* RHS, double* enorm) {
+    FP_MAV(launch_residual_mixed_av(L, s, P, G, mm, W, Wo, Wo2, a1, a2, a3, R, RHS, enorm))
     dim3 blk(128), grd((mm.nElem + 127) / 128);
     const bool elem = mm.nqfm * 4 <= MJ_MAXF && mm.nqvm <= MJ_MAXV && mm.nElem > 0;
 #define X(MK, NB, V, A)                                                                                                      \
@@ -854,8 +879,10 @@ hipError_t launch_residual_mixed(const ElemLaunch& L, hipStream_t s, const Model
 
 // mode 0 (assembly): one workgroup per element (k_jacobian_mixed_elem, every column including the
 // padded ones); mode 1 (J^T v): one thread per padded column
-hipError_t launch_jacobian_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
-                                 const double* W, double a1, double* blocks, float* fblocks, const double* v, double* out, int mode) {
+hipError_t FP_MFN(launch_jacobian_mixed)(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G,
+                                         const MixedMesh& mm, const double* W, double a1, double* blocks, float* fblocks,
+                                         const double* v, double* out, int mode) {
+    FP_MAV(launch_jacobian_mixed_av(L, s, P, G, mm, W, a1, blocks, fblocks, v, out, mode))
     if (mm.nElem == 0) return hipSuccess;
     if (mode == 0 && (mm.nqfm * 4 > MJ_MAXF || mm.nqvm > MJ_MAXV)) return hipErrorInvalidValue;   // orders <= 5 only
     const long long threads = (long long)mm.nElem * L.nb * L.neq;
@@ -876,8 +903,9 @@ hipError_t launch_jacobian_mixed(const ElemLaunch& L, hipStream_t s, const Model
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_jvp_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
-                            const double* W, const double* v, double a1, double* out) {
+hipError_t FP_MFN(launch_jvp_mixed)(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const MixedMesh& mm,
+                                    const double* W, const double* v, double a1, double* out) {
+    FP_MAV(launch_jvp_mixed_av(L, s, P, G, mm, W, v, a1, out))
     dim3 blk(64), grd((mm.nElem + 63) / 64);
 #define X(MK, NB, V, A)                                                                              \
     if (FP_MMATCH(MK, NB, V, A)) {                                                                   \
@@ -889,11 +917,12 @@ hipError_t launch_jvp_mixed(const ElemLaunch& L, hipStream_t s, const ModelParam
     return hipErrorInvalidValue;
 }
 
+#if FP_MIXED_PART == 0
 hipError_t launch_size_over_lambda_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const MixedMesh& mm, const double* W,
                                          const double* esize, double* out) {
     dim3 blk(256), grd((mm.nElem + 255) / 256);
 #define X(MK, NB, V, A)                                                                                   \
-    if (FP_MMATCH(MK, NB, V, A)) {                                                                        \
+    if (FP_MSHAPE(MK, NB, V)) {                                                                        \
         hipLaunchKernelGGL((k_size_over_lambda_mixed<MK, NB>), grd, blk, 0, s, P, mm, W, esize, out);  \
         return hipGetLastError();                                                                      \
     }
@@ -901,9 +930,12 @@ hipError_t launch_size_over_lambda_mixed(const ElemLaunch& L, hipStream_t s, con
 #undef X
     return hipErrorInvalidValue;
 }
+#endif
 
-hipError_t launch_av_limiter_mixed(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const MixedMesh& mm, const AvParams& Ap,
-                                   const double* W, const double* esize, const double* trunc, double* av, double* old) {
+hipError_t FP_MFN(launch_av_limiter_mixed)(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const MixedMesh& mm,
+                                           const AvParams& Ap, const double* W, const double* esize, const double* trunc, double* av,
+                                           double* old) {
+    FP_MAV(launch_av_limiter_mixed_av(L, s, P, mm, Ap, W, esize, trunc, av, old))
     dim3 blk(128), grd((mm.nElem + 127) / 128);
 #define X(MK, NB, V, A)                                                                                          \
     if (A && FP_MMATCH(MK, NB, V, A)) {                                                                          \
@@ -915,10 +947,11 @@ hipError_t launch_av_limiter_mixed(const ElemLaunch& L, hipStream_t s, const Mod
     return hipErrorInvalidValue;
 }
 
+#if FP_MIXED_PART == 0
 hipError_t launch_positivity_mixed(const ElemLaunch& L, hipStream_t s, const MixedMesh& mm, double* W, int* count) {
     dim3 blk(128), grd((mm.nElem + 127) / 128);
 #define X(MK, NB, V, A)                                                                      \
-    if (FP_MMATCH(MK, NB, V, A)) {                                                           \
+    if (FP_MSHAPE(MK, NB, V)) {                                                           \
         hipLaunchKernelGGL((k_positivity_mixed<MK, NB>), grd, blk, 0, s, mm, W, count);   \
         return hipGetLastError();                                                         \
     }
@@ -938,5 +971,6 @@ hipError_t launch_functional_mixed(const ElemLaunch& L, hipStream_t s, const Mix
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
+#endif
 
 }  // namespace fp

@@ -510,42 +510,58 @@ __global__ void k_functional_quad(QuadTables tb, QuadMesh mq, const double* __re
     }
 }
 
+// The file is compiled twice so that its halves build in parallel: as itself (FP_QUAD_PART 0: the
+// kernels without artificial viscosity, and the entry points the host calls, which hand the damped
+// cases on) and through fp_quad_av.hip (FP_QUAD_PART 1: the kernels with it, entry points named _av).
+#ifndef FP_QUAD_PART
+#define FP_QUAD_PART 0
+#endif
 // (model, NB, NQV, NQF, viscous, artificial viscosity)
-#define FP_QUAD_CASES(X)                               \
-    X(MODEL_NAVIER_STOKES, 1, 1, 1, false, false)      \
-    X(MODEL_NAVIER_STOKES, 4, 4, 2, false, false)      \
-    X(MODEL_NAVIER_STOKES, 9, 9, 3, false, false)      \
-    X(MODEL_NAVIER_STOKES, 16, 16, 4, false, false)    \
-    X(MODEL_NAVIER_STOKES, 25, 25, 5, false, false)    \
-    X(MODEL_NAVIER_STOKES, 1, 1, 1, true, false)       \
-    X(MODEL_NAVIER_STOKES, 4, 4, 2, true, false)       \
-    X(MODEL_NAVIER_STOKES, 9, 9, 3, true, false)       \
-    X(MODEL_NAVIER_STOKES, 16, 16, 4, true, false)     \
-    X(MODEL_NAVIER_STOKES, 25, 25, 5, true, false)     \
-    X(MODEL_ADIABATIC, 1, 1, 1, false, false)          \
-    X(MODEL_ADIABATIC, 4, 4, 2, false, false)          \
-    X(MODEL_ADIABATIC, 9, 9, 3, false, false)          \
-    X(MODEL_ADIABATIC, 16, 16, 4, false, false)        \
-    X(MODEL_ADIABATIC, 25, 25, 5, false, false)        \
-    X(MODEL_NAVIER_STOKES, 4, 4, 2, false, true)       \
-    X(MODEL_NAVIER_STOKES, 9, 9, 3, false, true)       \
-    X(MODEL_NAVIER_STOKES, 16, 16, 4, false, true)     \
-    X(MODEL_NAVIER_STOKES, 4, 4, 2, true, true)        \
-    X(MODEL_NAVIER_STOKES, 9, 9, 3, true, true)        \
-    X(MODEL_NAVIER_STOKES, 16, 16, 4, true, true)      \
-    X(MODEL_ADIABATIC, 4, 4, 2, false, true)           \
-    X(MODEL_ADIABATIC, 9, 9, 3, false, true)           \
-    X(MODEL_ADIABATIC, 16, 16, 4, false, true)         \
-    X(MODEL_NAVIER_STOKES, 1, 1, 1, false, true)       \
-    X(MODEL_NAVIER_STOKES, 1, 1, 1, true, true)        \
-    X(MODEL_ADIABATIC, 1, 1, 1, false, true)           \
-    X(MODEL_NAVIER_STOKES, 25, 25, 5, false, true)     \
-    X(MODEL_NAVIER_STOKES, 25, 25, 5, true, true)      \
-    X(MODEL_ADIABATIC, 25, 25, 5, false, true)
+#define FP_QUAD_SHAPES(X, A)                    \
+    X(MODEL_NAVIER_STOKES, 1, 1, 1, false, A)   \
+    X(MODEL_NAVIER_STOKES, 4, 4, 2, false, A)   \
+    X(MODEL_NAVIER_STOKES, 9, 9, 3, false, A)   \
+    X(MODEL_NAVIER_STOKES, 16, 16, 4, false, A) \
+    X(MODEL_NAVIER_STOKES, 25, 25, 5, false, A) \
+    X(MODEL_NAVIER_STOKES, 1, 1, 1, true, A)    \
+    X(MODEL_NAVIER_STOKES, 4, 4, 2, true, A)    \
+    X(MODEL_NAVIER_STOKES, 9, 9, 3, true, A)    \
+    X(MODEL_NAVIER_STOKES, 16, 16, 4, true, A)  \
+    X(MODEL_NAVIER_STOKES, 25, 25, 5, true, A)  \
+    X(MODEL_ADIABATIC, 1, 1, 1, false, A)       \
+    X(MODEL_ADIABATIC, 4, 4, 2, false, A)       \
+    X(MODEL_ADIABATIC, 9, 9, 3, false, A)       \
+    X(MODEL_ADIABATIC, 16, 16, 4, false, A)     \
+    X(MODEL_ADIABATIC, 25, 25, 5, false, A)
+#if FP_QUAD_PART == 0
+#define FP_QUAD_CASES(X) FP_QUAD_SHAPES(X, false)
+#define FP_QFN(name) name
+#define FP_QAV(call) \
+    if (L.av) return call;
+bool quad_supported_av(const ElemLaunch& L);
+hipError_t launch_residual_quad_av(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const QuadTables& tb,
+                                   const QuadMesh& mq, const double* W, const double* Wo, const double* Wo2, double a1, double a2,
+                                   double a3, double* R, double* RHS, double* enorm);
+hipError_t launch_jacobian_quad_av(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const QuadTables& tb,
+                                   const QuadMesh& mq, const double* W, double a1, double* blocks, float* fblocks, const double* v,
+                                   double* out, int mode);
+hipError_t launch_jvp_quad_av(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const QuadTables& tb,
+                              const QuadMesh& mq, const double* W, const double* v, double a1, double* out);
+hipError_t launch_av_limiter_quad_av(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const QuadTables& tb,
+                                     const QuadMesh& mq, const AvParams& Ap, const double* W, const double* esize,
+                                     const double* trunc, double* av, double* old);
+#else
+#define FP_QUAD_CASES(X) FP_QUAD_SHAPES(X, true)
+#define FP_QFN(name) name##_av
+#define FP_QAV(call)
+#endif
 #define FP_QMATCH(MK, NB, NQV, NQF, V, A) \
     (L.model == MK && L.nb == NB && L.nqv == NQV && L.nqf == NQF && (L.visc != 0) == V && (L.av != 0) == A)
+// the kernels that do not depend on the damping: every shape, damped or not
+#define FP_QSHAPE(MK, NB, NQV, NQF, V) (L.model == MK && L.nb == NB && L.nqv == NQV && L.nqf == NQF && (L.visc != 0) == V)
 
-bool quad_supported(const ElemLaunch& L) {
+bool FP_QFN(quad_supported)(const ElemLaunch& L) {
+    FP_QAV(quad_supported_av(L))
 #define X(MK, NB, NQV, NQF, V, A) \
     if (FP_QMATCH(MK, NB, NQV, NQF, V, A)) return true;
     FP_QUAD_CASES(X)
@@ -553,9 +569,10 @@ bool quad_supported(const ElemLaunch& L) {
     return false;
 }
 
-hipError_t launch_residual_quad(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const QuadTables& tb,
-                                const QuadMesh& mq, const double* W, const double* Wo, const double* Wo2, double a1, double a2,
-                                double a3, double* R, double* RHS, double* enorm) {
+hipError_t FP_QFN(launch_residual_quad)(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G,
+                                        const QuadTables& tb, const QuadMesh& mq, const double* W, const double* Wo,
+                                        const double* Wo2, double a1, double a2, double a3, double* R, double* RHS, double* enorm) {
+    FP_QAV(launch_residual_quad_av(L, s, P, G, tb, mq, W, Wo, Wo2, a1, a2, a3, R, RHS, enorm))
     dim3 blk(128), grd((mq.nElem + 127) / 128);
 #define X(MK, NB, NQV, NQF, V, A)                                                                                       \
     if (FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                                                            \
@@ -568,9 +585,10 @@ hipError_t launch_residual_quad(const ElemLaunch& L, hipStream_t s, const ModelP
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_jacobian_quad(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const QuadTables& tb,
-                                const QuadMesh& mq, const double* W, double a1, double* blocks, float* fblocks, const double* v,
-                                double* out, int mode) {
+hipError_t FP_QFN(launch_jacobian_quad)(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G,
+                                        const QuadTables& tb, const QuadMesh& mq, const double* W, double a1, double* blocks,
+                                        float* fblocks, const double* v, double* out, int mode) {
+    FP_QAV(launch_jacobian_quad_av(L, s, P, G, tb, mq, W, a1, blocks, fblocks, v, out, mode))
     const long long threads = (long long)mq.nElem * L.nb * L.neq;
     dim3 blk(64), grd((unsigned)((threads + 63) / 64));
 #define X(MK, NB, NQV, NQF, V, A)                                                                                          \
@@ -584,8 +602,9 @@ hipError_t launch_jacobian_quad(const ElemLaunch& L, hipStream_t s, const ModelP
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_jvp_quad(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const QuadTables& tb,
-                           const QuadMesh& mq, const double* W, const double* v, double a1, double* out) {
+hipError_t FP_QFN(launch_jvp_quad)(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const QuadTables& tb,
+                                   const QuadMesh& mq, const double* W, const double* v, double a1, double* out) {
+    FP_QAV(launch_jvp_quad_av(L, s, P, G, tb, mq, W, v, a1, out))
     dim3 blk(64), grd((mq.nElem + 63) / 64);
 #define X(MK, NB, NQV, NQF, V, A)                                                                                 \
     if (FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                                                      \
@@ -597,11 +616,12 @@ hipError_t launch_jvp_quad(const ElemLaunch& L, hipStream_t s, const ModelParams
     return hipErrorInvalidValue;
 }
 
+#if FP_QUAD_PART == 0
 hipError_t launch_size_over_lambda_quad(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const QuadMesh& mq, const double* W,
                                         const double* esize, double* out) {
     dim3 blk(256), grd((mq.nElem + 255) / 256);
 #define X(MK, NB, NQV, NQF, V, A)                                                                          \
-    if (FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                                               \
+    if (FP_QSHAPE(MK, NB, NQV, NQF, V)) {                                                               \
         hipLaunchKernelGGL((k_size_over_lambda_quad<MK, NB>), grd, blk, 0, s, P, mq, W, esize, out);    \
         return hipGetLastError();                                                                       \
     }
@@ -609,10 +629,12 @@ hipError_t launch_size_over_lambda_quad(const ElemLaunch& L, hipStream_t s, cons
 #undef X
     return hipErrorInvalidValue;
 }
+#endif
 
-hipError_t launch_av_limiter_quad(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const QuadTables& tb, const QuadMesh& mq,
-                                  const AvParams& Ap, const double* W, const double* esize, const double* trunc, double* av,
-                                  double* old) {
+hipError_t FP_QFN(launch_av_limiter_quad)(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const QuadTables& tb,
+                                          const QuadMesh& mq, const AvParams& Ap, const double* W, const double* esize,
+                                          const double* trunc, double* av, double* old) {
+    FP_QAV(launch_av_limiter_quad_av(L, s, P, tb, mq, Ap, W, esize, trunc, av, old))
     dim3 blk(128), grd((mq.nElem + 127) / 128);
 #define X(MK, NB, NQV, NQF, V, A)                                                                                     \
     if (A && FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                                                     \
@@ -624,10 +646,11 @@ hipError_t launch_av_limiter_quad(const ElemLaunch& L, hipStream_t s, const Mode
     return hipErrorInvalidValue;
 }
 
+#if FP_QUAD_PART == 0
 hipError_t launch_positivity_quad(const ElemLaunch& L, hipStream_t s, const QuadTables& tb, const QuadMesh& mq, double* W, int* count) {
     dim3 blk(128), grd((mq.nElem + 127) / 128);
 #define X(MK, NB, NQV, NQF, V, A)                                                                          \
-    if (FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                                               \
+    if (FP_QSHAPE(MK, NB, NQV, NQF, V)) {                                                               \
         hipLaunchKernelGGL((k_positivity_quad<MK, NB, NQV, NQF>), grd, blk, 0, s, tb, mq, W, count);    \
         return hipGetLastError();                                                                       \
     }
@@ -641,7 +664,7 @@ hipError_t launch_functional_quad(const ElemLaunch& L, hipStream_t s, const Quad
     if (L.model != MODEL_NAVIER_STOKES) return hipErrorInvalidValue;
     dim3 blk(128), grd((mq.nElem + 127) / 128);
 #define X(MK, NB, NQV, NQF, V, A)                                                                                   \
-    if (MK == MODEL_NAVIER_STOKES && FP_QMATCH(MK, NB, NQV, NQF, V, A)) {                                            \
+    if (MK == MODEL_NAVIER_STOKES && FP_QSHAPE(MK, NB, NQV, NQF, V)) {                                            \
         hipLaunchKernelGGL((k_functional_quad<NB, NQF>), grd, blk, 0, s, tb, mq, W, kappa, which, ival, grad);   \
         return hipGetLastError();                                                                                 \
     }
@@ -649,5 +672,6 @@ hipError_t launch_functional_quad(const ElemLaunch& L, hipStream_t s, const Quad
 #undef X
     return hipErrorInvalidValue;
 }
+#endif
 
 }  // namespace fp
