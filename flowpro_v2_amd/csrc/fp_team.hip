@@ -20,6 +20,8 @@ namespace fp {
     X(MODEL_NAVIER_STOKES, 10, 4, 16, 4, true, true)     \
     X(MODEL_SA, 3, 5, 6, 2, true, false)                 \
     X(MODEL_SA, 6, 5, 7, 3, true, false)                 \
+    X(MODEL_SA, 3, 5, 6, 2, true, true)                  \
+    X(MODEL_SA, 6, 5, 7, 3, true, true)                  \
     X(MODEL_ADIABATIC, 3, 3, 6, 2, false, false)         \
     X(MODEL_ADIABATIC, 6, 3, 7, 3, false, false)         \
     X(MODEL_ADIABATIC, 10, 3, 16, 4, false, false)       \

@@ -1,7 +1,8 @@
 """Parity of the artificial viscosity on the device (fp_limiter, the damping terms in the
 residual, the exact Jacobian and the matrix-free products) with the oracle.
 
-Triangles, (non-affine, jittered) quadrilaterals and meshes mixing the two with per-element orders;
+Triangles (NavierStokes, AdiabaticEulers, SpalartAllmaras), (non-affine, jittered) quadrilaterals and
+meshes mixing the two with per-element orders;
 on quadrilaterals and mixed meshes the truncation matrix is the element's own and the sensor
 integrals take |det J| w per point.
 
@@ -42,6 +43,8 @@ AV_CONFIGS = [
      lambda: cases.euler_props(model="AdiabaticEulers", order=3, dampTol=2.0, dampInnerTol=2.0)),
     ("euler_q5_inner", lambda: cases.channel_bump(6, 3, quads=True, jitter=0.2),
      lambda: cases.euler_props(order=5, dampTol=2.5, dampConst=1e-3, dampInnerTol=2.0)),
+    ("sa_p2_damp", lambda: cases.plate_channel(12, 5),
+     lambda: cases.sa_props(order=3, reynolds=1e5, dampTol=1.0, dampConst=1e-4, dampInnerTol=1.5)),
 ]
 
 
