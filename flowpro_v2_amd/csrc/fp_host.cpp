@@ -1881,9 +1881,8 @@ static int create_impl(const fp_case* cs, fp_ctx* c) {
         return FP_EINVAL;
     }
     const bool av = cs->damp_tol > 0 || cs->damp_const != 0 || cs->damp_inner_tol > 0;
-    if (av && (c->dim == 3 || cs->model == MODEL_KOMEGA || (cs->model == MODEL_SA && (c->quad || c->mixed)))) {
-        c->err = "artificial viscosity: device kernels for the NavierStokes, AdiabaticEulers and scalar models in 2D and "
-                 "SpalartAllmaras on triangles";
+    if (av && c->dim == 3) {
+        c->err = "artificial viscosity: device kernels in 2D";
         return FP_UNSUPPORTED;
     }
     if (cs->n_damp_inner_coef < 0 || cs->n_damp_inner_coef > 5 ||

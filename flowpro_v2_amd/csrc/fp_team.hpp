@@ -656,7 +656,9 @@ __global__ void __launch_bounds__(256, 2) k_jacobian_team(ModelParams P, DGParam
 // the k-omega SST cases (model, Nb, nEqs, volume points, face points, viscous, artificial viscosity)
 #define FP_TEAM_KW_CASES(X)                  \
     X(MODEL_KOMEGA, 3, 6, 6, 2, true, false) \
-    X(MODEL_KOMEGA, 6, 6, 7, 3, true, false)
+    X(MODEL_KOMEGA, 6, 6, 7, 3, true, false) \
+    X(MODEL_KOMEGA, 3, 6, 6, 2, true, true)  \
+    X(MODEL_KOMEGA, 6, 6, 7, 3, true, true)
 
 // the k-omega SST team kernels, compiled in a translation unit of their own (fp_team_kw.hip)
 hipError_t launch_residual_team_kw(const ElemLaunch& L, hipStream_t s, const ModelParams& P, const DGParams& G, const Tables& tb,

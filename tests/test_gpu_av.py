@@ -1,7 +1,7 @@
 """Parity of the artificial viscosity on the device (fp_limiter, the damping terms in the
 residual, the exact Jacobian and the matrix-free products) with the oracle.
 
-Triangles (NavierStokes, AdiabaticEulers, SpalartAllmaras), (non-affine, jittered) quadrilaterals and
+Triangles (NavierStokes, AdiabaticEulers, SpalartAllmaras, KOmegaSST), (non-affine, jittered) quadrilaterals and
 meshes mixing the two with per-element orders;
 on quadrilaterals and mixed meshes the truncation matrix is the element's own and the sensor
 integrals take |det J| w per point.
@@ -257,4 +257,32 @@ def test_damp_const_on_order_one_quads(viscous):
     o0.assemble(0.05, -1.0)
     change = np.abs(o0.residual() - o.residual()).max() / np.abs(o.residual()).max()
     assert (change == 0.0) if viscous else (change > 1e-3)
+    dev.close()
+
+
+def test_kw_damped_limiter_residual_jacobian_products():
+    """KOmegaSST (straight triangles, order 3): every equation takes the density sensor
+    (Aerodynamics.combineShockSensors); the damping terms in the residual, the exact Jacobian and the
+    products.  A 5 % perturbation with dampTol = dampInnerTol = 3 puts the sensors of most elements on
+    their ramp."""
+    case = cases.plate_channel(12, 5)
+    props = cases.kw_props(order=3, reynolds=1e5, dampTol=3.0, dampConst=1e-4, dampInnerTol=3.0)
+    print("[av] KOmegaSST", flush=True)
+    o, dev, W = _setup(case, props, rel=0.05)
+    eps, di = dev.artificial_viscosity()
+    eps_ref, di_ref = o.artificial_viscosity()
+    assert (eps_ref > 0).mean() > 0.5
+    assert _rel(eps, eps_ref) < 1e-12
+    assert _rel(di, di_ref) < 1e-12
+    o.assemble(0.05, -1.0)
+    dev.assemble(0.05, -1.0, 1, True)
+    R, _ = dev.residual()
+    assert eq_rel_err(R, o.residual(), dev.nb, dev.neq) < 1e-12
+    d, n = dev.jacobian()
+    dref, nref = o.blocks()
+    assert block_rel_err(d, dref) < 1e-10
+    assert block_rel_err(n, nref) < 1e-10
+    v = np.random.default_rng(7).standard_normal((case.n_elem, dev.B))
+    assert block_rel_err(dev.jv(20.0, v), o.jvp(20.0, v)) < 1e-10
+    assert block_rel_err(dev.jtv(20.0, v), o.vjp(20.0, v)) < 1e-10
     dev.close()
