@@ -23,6 +23,8 @@ On jittered (non-affine) quadrilaterals:
 On meshes of squares and triangles with per-element orders: every element's eps and dampInner
 equal those it gets in the mesh where all elements have its order (the limiter is element-local
 and takes the element's own order) (1e-13)
+SpalartAllmaras and KOmegaSST (Aerodynamics.combineShockSensors): dampInner equal in every equation, and
+equal to eps when dampTol = dampInnerTol on the first call (1e-13)
 """
 import math
 
@@ -276,3 +278,22 @@ def test_mixed_mesh_sensor_takes_each_elements_own_order():
         eu, du = ou.artificial_viscosity()
         np.testing.assert_allclose(eps[sel], eu[sel], rtol=1e-13, atol=0)
         np.testing.assert_allclose(di[sel], du[sel], rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("model", ["SpalartAllmaras", "KOmegaSST"])
+def test_rans_models_take_the_density_sensor(model):
+    """SpalartAllmaras and KOmegaSST extend Aerodynamics, whose combineShockSensors gives every equation
+    the density sensor: dampInner is the same in every equation, and eps and dampInner scale with the
+    same lambda h / k (eps / dampInner = g(dampTol) / g(dampInnerTol), the same ramp at the two widths)."""
+    case = cases.plate_channel(12, 5)
+    mk = cases.sa_props if model == "SpalartAllmaras" else cases.kw_props
+    props = mk(order=3, reynolds=1e5, dampTol=3.0, dampInnerTol=3.0)
+    o = O.Oracle(case, props)
+    W = cases.perturbed_state(o.get_w()[0][:: o.nb], case.n_elem, o.nb, 0.05, 3)
+    o.set_w(W)
+    o.limiter()
+    eps, di = o.artificial_viscosity()
+    assert (eps > 0).mean() > 0.5 and di.shape[1] == o.neq
+    assert np.all(di == di[:, :1])
+    on = eps > 0
+    np.testing.assert_allclose(di[on, 0], eps[on], rtol=1e-13)   # equal widths: the same g, the same scale
