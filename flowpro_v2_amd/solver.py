@@ -56,8 +56,10 @@ class DeviceCase:
         self.case = case
         self.dim = int(self.mc.dim)
         types, orders = getattr(case, "types", None), getattr(case, "orders", None)
-        # elementType.txt / order.txt: per-element types and orders (FlowProMain.java:520-560)
-        self.mixed = self.dim == 2 and (types is not None or orders is not None)
+        # elementType.txt / order.txt: per-element types and orders (FlowProMain.java:520-560).  A uniform
+        # mesh of order-5 triangles (15 functions) also runs on the mixed-mesh kernels, padded to 16.
+        tri5 = self.dim == 2 and np.asarray(case.TP).shape[1] == 3 and order == 5
+        self.mixed = self.dim == 2 and (types is not None or orders is not None or tri5)
         tp_in = np.asarray(case.TP)
         if self.mixed:
             if orders is not None:

@@ -21,6 +21,7 @@ import pytest
 
 from flowpro_v2_amd import cases, solver
 import oracle as O
+from helpers import eq_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -239,4 +240,35 @@ def test_mixed_steady_adjoint_matches_dense():
     A = o.global_matrix()
     lam_ref = np.linalg.solve(A.T, o.to_dof(g))
     assert np.abs(o.to_dof(lam) - lam_ref).max() <= 1e-8 * np.abs(lam_ref).max()
+    dev.close()
+
+
+def test_uniform_order5_triangles_without_order_file():
+    """A uniform mesh of order-5 triangles given without order.txt (the uniform-mesh layout of the
+    reference: every element 15 functions) runs on the padded mixed-mesh kernels: the residual, the
+    Newton history and the state against the oracle's uniform-mesh path."""
+    case = cases.channel_bump(6, 3, jitter=0.2)
+    props = dict(cases.euler_props(order=5, CFL=2.0, iterativeSolverTol=1e-12), jacobian="exact")
+    print("[mixed] uniform order-5 triangles", flush=True)
+    o = O.Oracle(case, props)
+    assert o.nb == 15
+    W0 = cases.perturbed_state(o.get_w()[0][:: o.nb], case.n_elem, o.nb, 0.01, 5)
+    o.set_w(W0)
+    dev = solver.DeviceCase(case, props)
+    assert dev.mixed and dev.dofs == case.n_elem * 15 * 4
+    dev.set_state(W0.ravel())
+    o.limiter()
+    o.assemble(0.05, -1.0)
+    dev.assemble(0.05, -1.0, 1, True)
+    R, _ = dev.residual()
+    Rref = o.residual()
+    assert eq_rel_err(np.asarray(R).reshape(Rref.shape), Rref, 15, 4) < 1e-12
+    o.set_w(W0)
+    dev.set_state(W0.ravel())
+    steps = 3
+    hist_ref = o.run(steps)
+    hist = [r.residuum for r in solver.GpuImplicitSolver(dev, props).solve(steps)]
+    np.testing.assert_allclose(hist, hist_ref, rtol=1e-8)
+    Wg, Wr = np.asarray(dev.get_state()).reshape(case.n_elem, -1), o.get_w()
+    assert np.abs(Wg - Wr).max() / np.abs(Wr).max() < 1e-8
     dev.close()
